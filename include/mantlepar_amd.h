@@ -1,0 +1,175 @@
+/*
+ * mantlepar_amd.h -- C ABI of the MI355X (gfx950) H/C transport library.
+ *
+ * One handle advances one rank's share of a structured mesh: ML-WENO
+ * reconstruction, smoothness-indicator weights, Lax-Friedrichs face fluxes and
+ * the SSP Runge-Kutta stage update for enthalpy H and composition C, fused with
+ * the per-cell binary-eutectic equilibrium (H,C) -> (T, phi, Cl, Cs, region).
+ *
+ * The entry points replace, for all owned cells at once, the reference's
+ * per-object calls (citations are path:line in aband/MantlePar):
+ *   StateReconstruction::Update + Enthalpy/Evaluate   src/couple/state_reconstruction.cpp:101-134
+ *   Reconstruction::Initialize/Update/Evaluate        src/mlweno/reconstruction.h:78-137
+ *   IntegrateAdvectiveFlux / ...BoundaryFlux          src/mlweno/advectiveflux.h:126-186
+ *   the preheat RHS face loop and FE/SSPRK stages     src/couple/preheat.cpp:160-203, 253-284
+ *   the SSPRK2 stage combination                      src/couple/evolve.cpp:465-492
+ *   EutecticModel::evaluate/classify/boundaries       src/phase/eutectic_rescaled.h:103-136
+ *
+ * Conventions (mirroring the reference):
+ *   - Every call returns mp_status: 0 on success, a category code otherwise,
+ *     like PetscErrorCode.  mp_last_error() returns the thread's last message.
+ *   - On error the handle's state and previously returned outputs are left
+ *     unchanged ("failed calls preserve the previous valid state",
+ *     src/mlweno/reconstruction.h:61-62).  State is double-buffered on device.
+ *   - Host arrays are owned by the caller and copied; device buffers are owned
+ *     by the handle.  A handle is single-threaded (like EutecticModel's
+ *     setParameters contract, src/phase/eutectic_rescaled.h:100-102); use one
+ *     handle per thread or external locking.
+ *   - Cell arrays are row-major H[j*nx + i] (the reference's ordering,
+ *     src/couple/initialization.cpp:786).
+ *   - Face velocities are the normal velocity at the q edge Gauss points, in
+ *     MapEdgePoint order along the canonical direction (src/core/integral.cpp:91-97,
+ *     src/couple/flow_sampling.cpp:101-115):
+ *       vnx[q][ny][nx+1]: vertical faces x_i, along +x, points ascending in y;
+ *       vny[q][ny+1][nx]: horizontal faces y_j, along +y, points ascending in x.
+ */
+#ifndef MANTLEPAR_AMD_H
+#define MANTLEPAR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t mp_status;
+
+enum {
+    MP_OK = 0,
+    MP_ERR_ARG = 1,        /* invalid argument (PETSC_ERR_ARG_*, std::invalid_argument) */
+    MP_ERR_FP = 2,         /* non-finite or unrepresentable value (PETSC_ERR_FP) */
+    MP_ERR_STATE = 3,      /* call out of order (PETSC_ERR_ARG_WRONGSTATE) */
+    MP_ERR_PHASE = 4,      /* phase fraction outside [0,1] (std::runtime_error) */
+    MP_ERR_DEVICE = 5,     /* HIP runtime failure */
+    MP_ERR_MEMORY = 6,     /* allocation or HBM budget exhausted */
+    MP_ERR_BACKFLOW = 7,   /* backflow on an outflow-only boundary (initialization.cpp:478-481) */
+    MP_ERR_COMPOSITION = 8 /* C outside the admissible interval (eutectic_rescaled.cpp:88-93) */
+};
+
+enum { MP_PREC_FP64 = 0, MP_PREC_MIXED = 1 };
+
+/* Per boundary face policy (src/couple/initialization.cpp:469-489). */
+enum { MP_BC_ZERO_FLUX = 0, MP_BC_OUTFLOW = 1, MP_BC_INFLOW_OUTFLOW = 2 };
+
+/* Time integrators: forward Euler (preheat.cpp:267), SSPRK2 (evolve.cpp:469-479),
+ * SSPRK3 (Shu-Osher; not in the reference). */
+enum { MP_SCHEME_FE = 1, MP_SCHEME_SSPRK2 = 2, MP_SCHEME_SSPRK3 = 3 };
+
+/* Uniform rectangular mesh: nx*ny cells on [x0,x1]x[y0,y1], q edge Gauss points. */
+typedef struct {
+    int32_t nx, ny;
+    double x0, x1, y0, y1;
+    int32_t q;             /* 1..8; the reference default is 5 (src/input/input.h:64) */
+} mp_mesh_desc;
+
+/* ReconstructionOptions for the production families (3x3 order-2 large
+ * candidate, four 2x2 order-1 TargetCell candidates;
+ * src/couple/state_reconstruction.cpp:64-68, src/mlweno/reconstruction.h:28-34). */
+typedef struct {
+    double epsilon;                 /* 1e-2 */
+    int32_t s;                      /* 1 */
+    double large_weight;            /* linear weight d of the large candidate, 1 */
+    double small_weight[4];         /* offsets (0,-1),(0,0),(-1,-1),(-1,0), 1 each */
+    int32_t use_constant;           /* 0 */
+    double constant_linear_weight;  /* 1e-5 */
+    int32_t lf_mode;                /* 0 local LF, 1 global LF (advectiveflux.h:18-25) */
+    double global_speed;            /* used when lf_mode == 1 */
+} mp_weno_options;
+
+/* MaterialParameters (src/phase/eutectic_rescaled.h:10-26) plus a uniform
+ * pressure P [Pa] and the hypereutectic switch: 0 rejects C > Xe like the
+ * reference, 1 evaluates the mirrored diagram for Xe < C <= 1. */
+typedef struct {
+    double Tm0, Te0, nu, L, cp, Xe, rho, rhor, mus, mul, k0, g, alpha0;
+    double pressure;
+    int32_t hypereutectic;
+} mp_phase_params;
+
+typedef struct {
+    double dt;              /* step used */
+    double max_rate;        /* max_K rate_K (src/couple/preheat.cpp:125-152) */
+    double courant;         /* dt * max_rate */
+    double total_H, total_C;          /* compensated sum |K|*u after the call */
+    double boundary_H, boundary_C;    /* sum over steps of dt * stage-weighted outward boundary flux */
+    uint32_t error_bits;
+    int32_t steps_done;
+} mp_step_report;
+
+typedef struct mp_transport mp_transport;
+typedef struct mp_batch mp_batch;
+typedef struct mp_server mp_server;
+
+/* Defaults: production options and the reference's default material. */
+void mp_default_weno_options(mp_weno_options* o);
+void mp_default_phase_params(mp_phase_params* p);
+const char* mp_last_error(void);
+const char* mp_version(void);
+
+/* --------------------------------------------------------- single mesh --- */
+/* stream: a hipStream_t, or NULL for a stream owned by the handle. */
+mp_status mp_transport_create(const mp_mesh_desc* mesh, const mp_weno_options* weno,
+                              const mp_phase_params* phase, int32_t precision, void* stream,
+                              mp_transport** out);
+mp_status mp_transport_destroy(mp_transport* t);
+mp_status mp_transport_upload_state(mp_transport* t, const double* H, const double* C);
+mp_status mp_transport_download_state(mp_transport* t, double* H, double* C);
+mp_status mp_transport_set_face_velocity(mp_transport* t, const double* vnx, const double* vny);
+/* Per boundary face policies; each array runs along +x (bottom, top) or +y
+ * (left, right).  NULL means zero flux on that side. */
+mp_status mp_transport_set_boundary(mp_transport* t, const uint8_t* bottom, const uint8_t* right,
+                                    const uint8_t* top, const uint8_t* left);
+/* Inflow states [side 0..3 = bottom,right,top,left][face][q], canonical point order. */
+mp_status mp_transport_set_inflow(mp_transport* t, const double* inflow_H, const double* inflow_C);
+/* dt = cfl / max_K rate_K for the current face velocities. */
+mp_status mp_transport_stable_dt(mp_transport* t, double cfl, double* dt);
+/* Advance nsteps at fixed dt; each step ends with the per-cell phase.  Launches
+ * are asynchronous; the report (may be NULL) synchronises. */
+mp_status mp_transport_step(mp_transport* t, int32_t nsteps, int32_t scheme, double dt,
+                            mp_step_report* report);
+/* Phase of the latest step (or of the uploaded state, computed on demand). */
+mp_status mp_transport_phase(mp_transport* t, double* T, double* phi, double* Cl, double* Cs,
+                             uint8_t* region);
+/* Diagnostics used by parity tests: L(u) of the current state, and per-cell
+ * reconstruction data for field 0 (H) or 1 (C): sigma[N][5], omega[N][5]
+ * (slots: large, small offsets 0..3; 0 for discarded candidates) and traces
+ * [N][4][q] at the faces bottom,right,top,left in canonical point order. */
+mp_status mp_transport_rhs(mp_transport* t, double* rhsH, double* rhsC, double* boundary);
+mp_status mp_transport_reconstruct(mp_transport* t, int32_t field, double* sigma,
+                                   double* omega, double* traces);
+/* Device pointers of the current state (valid until the next step/upload). */
+mp_status mp_transport_device_state(mp_transport* t, double** H, double** C);
+mp_status mp_transport_synchronize(mp_transport* t);
+/* Time nsteps of the stage kernel with HIP events on the handle's stream:
+ * ms_total = wall time of the steps; ms_stage = mean duration of one stage
+ * launch; ms_phase_stage = mean duration of the phase-fused final stage. */
+mp_status mp_transport_profile(mp_transport* t, int32_t nsteps, int32_t scheme, double dt,
+                               double* ms_total, double* ms_stage, double* ms_phase_stage);
+/* Sweep tile shape: rows per work item (>= 4). */
+mp_status mp_transport_set_tile(mp_transport* t, int32_t rows_per_item);
+
+/* ------------------------------------------------------------- phase only --- */
+/* EutecticModel::evaluate over n cells on the device (host arrays in/out).
+ * P may be NULL (uses params->pressure).  Any output may be NULL. */
+mp_status mp_phase_evaluate(const mp_phase_params* params, int64_t n, const double* H,
+                            const double* C, const double* P, double* T, double* phi,
+                            double* Cl, double* Cs, uint8_t* region, double* phi1,
+                            double* phi2, double* dTdC, double* dTdH);
+mp_status mp_phase_boundaries(const mp_phase_params* params, double C, double P,
+                              double* Tep, double* Tmp, double* H_eutectic_end,
+                              double* H_liquidus);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,129 @@
+"""ctypes binding of ``libmantlepar_amd.so`` (the C ABI in include/mantlepar_amd.h).
+
+The library is loaded from this package directory only; a missing build is an
+error, never a silent fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmantlepar_amd.so")
+
+MP_OK = 0
+STATUS_NAMES = {1: "MP_ERR_ARG", 2: "MP_ERR_FP", 3: "MP_ERR_STATE", 4: "MP_ERR_PHASE",
+                5: "MP_ERR_DEVICE", 6: "MP_ERR_MEMORY", 7: "MP_ERR_BACKFLOW",
+                8: "MP_ERR_COMPOSITION"}
+PREC_FP64, PREC_MIXED = 0, 1
+BC_ZERO_FLUX, BC_OUTFLOW, BC_INFLOW_OUTFLOW = 0, 1, 2
+SCHEME_FE, SCHEME_SSPRK2, SCHEME_SSPRK3 = 1, 2, 3
+
+
+class MantleError(RuntimeError):
+    """A nonzero mp_status, with the library's message."""
+
+    def __init__(self, code, message):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {message}")
+        self.code = code
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("x0", C.c_double), ("x1", C.c_double),
+                ("y0", C.c_double), ("y1", C.c_double), ("q", C.c_int32)]
+
+
+class WenoOptions(C.Structure):
+    _fields_ = [("epsilon", C.c_double), ("s", C.c_int32), ("large_weight", C.c_double),
+                ("small_weight", C.c_double * 4), ("use_constant", C.c_int32),
+                ("constant_linear_weight", C.c_double), ("lf_mode", C.c_int32),
+                ("global_speed", C.c_double)]
+
+
+class PhaseParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("Tm0", "Te0", "nu", "L", "cp", "Xe", "rho", "rhor", "mus", "mul", "k0", "g",
+                 "alpha0", "pressure")] + [("hypereutectic", C.c_int32)]
+
+
+class StepReport(C.Structure):
+    _fields_ = [("dt", C.c_double), ("max_rate", C.c_double), ("courant", C.c_double),
+                ("total_H", C.c_double), ("total_C", C.c_double), ("boundary_H", C.c_double),
+                ("boundary_C", C.c_double), ("error_bits", C.c_uint32),
+                ("steps_done", C.c_int32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+_vp = C.c_void_p
+
+# (name, restype, argtypes) for every entry point of include/mantlepar_amd.h
+SIGNATURES = [
+    ("mp_default_weno_options", None, [C.POINTER(WenoOptions)]),
+    ("mp_default_phase_params", None, [C.POINTER(PhaseParams)]),
+    ("mp_last_error", C.c_char_p, []),
+    ("mp_version", C.c_char_p, []),
+    ("mp_transport_create", C.c_int32, [C.POINTER(MeshDesc), C.POINTER(WenoOptions),
+                                        C.POINTER(PhaseParams), C.c_int32, _vp, C.POINTER(_vp)]),
+    ("mp_transport_destroy", C.c_int32, [_vp]),
+    ("mp_transport_upload_state", C.c_int32, [_vp, _dp, _dp]),
+    ("mp_transport_download_state", C.c_int32, [_vp, _dp, _dp]),
+    ("mp_transport_set_face_velocity", C.c_int32, [_vp, _dp, _dp]),
+    ("mp_transport_set_boundary", C.c_int32, [_vp, _u8p, _u8p, _u8p, _u8p]),
+    ("mp_transport_set_inflow", C.c_int32, [_vp, _dp, _dp]),
+    ("mp_transport_stable_dt", C.c_int32, [_vp, C.c_double, _dp]),
+    ("mp_transport_step", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double,
+                                      C.POINTER(StepReport)]),
+    ("mp_transport_phase", C.c_int32, [_vp, _dp, _dp, _dp, _dp, _u8p]),
+    ("mp_transport_rhs", C.c_int32, [_vp, _dp, _dp, _dp]),
+    ("mp_transport_reconstruct", C.c_int32, [_vp, C.c_int32, _dp, _dp, _dp]),
+    ("mp_transport_device_state", C.c_int32, [_vp, C.POINTER(_dp), C.POINTER(_dp)]),
+    ("mp_transport_synchronize", C.c_int32, [_vp]),
+    ("mp_transport_profile", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double, _dp, _dp, _dp]),
+    ("mp_transport_set_tile", C.c_int32, [_vp, C.c_int32]),
+    ("mp_phase_evaluate", C.c_int32, [C.POINTER(PhaseParams), C.c_int64, _dp, _dp, _dp, _dp,
+                                      _dp, _dp, _dp, _u8p, _dp, _dp, _dp, _dp]),
+    ("mp_phase_boundaries", C.c_int32, [C.POINTER(PhaseParams), C.c_double, C.c_double, _dp,
+                                        _dp, _dp, _dp]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load the in-tree library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m mantlepar_amd.build` "
+                              "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != MP_OK:
+        raise MantleError(code, lib().mp_last_error().decode())
+
+
+def dptr(a):
+    """double* of a C-contiguous float64 array (or NULL for None)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def u8ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.uint8 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_u8p)

@@ -1,0 +1,95 @@
+// Shared host/device layouts of the MI355X transport library.
+//
+// Everything a stage kernel reads about one mesh lives in one MeshDev record
+// in device memory; kernels address it through a wave-uniform pointer so the
+// compiler keeps the tables in scalar registers (s_load), not VGPRs.
+#pragma once
+
+#include <cstdint>
+
+namespace mp {
+
+constexpr int kMaxQ = 8;          // edge Gauss points supported on device
+constexpr int kWave = 64;         // CDNA wavefront
+constexpr int kStripCells = 62;   // owned columns per wave (2 halo lanes)
+constexpr int kPartials = 8;      // doubles per work item in the partials buffer
+
+// Error word bits (device side); the host maps them to mp_status codes.
+enum : uint32_t {
+    kErrNonFinite = 1u << 0,      // state, trace or flux not finite
+    kErrComposition = 1u << 1,    // C outside [0, Xe] (or [0,1] with the mirror)
+    kErrFraction = 1u << 2,       // unitFraction outside [0,1] beyond 64 eps
+    kErrBackflow = 1u << 3,       // inflow on an outflow-only boundary face
+    kErrInflow = 1u << 4,         // inflow without prescribed boundary data
+    kErrWeights = 1u << 5,        // no finite active log weight
+    kErrPhaseRange = 1u << 6,     // phase boundaries not resolvable
+};
+
+// Binary eutectic parameters as the device uses them.  Tep/Tmp are derived
+// on the host in long double exactly as EutecticModel::boundaries does
+// (src/phase/eutectic_rescaled.cpp:163-170) because P is uniform per mesh.
+struct PhaseDev {
+    double LD;        // L/h0
+    double Xe;        // eutectic composition
+    double Tep;       // TDe0 + P/nu/dT, rounded once
+    double Tmp;       // Tep + 1, rounded once
+    int32_t hyper;    // evaluate C > Xe on the mirrored diagram
+    int32_t pad;
+};
+
+// One uniform rectangular mesh.  All reconstruction tables are the closed
+// forms of the reference's moment / smoothness construction on rectangles,
+// in coordinates normalised by h = sqrt(dx*dy) about the target centre.
+struct MeshDev {
+    int32_t nx, ny, q, lf_mode;
+    double dx, dy, area, inv_area;
+    double aw, bw;              // dx/h, dy/h
+    double hx, hy;              // aw/2, bw/2: face offsets in normalised units
+    // 1-D quadratic through three cell averages at offsets -1,0,1:
+    //   A2 = (r + l - 2m)*k2,  A1 = (r - l)*k1,  A0 = m - A2*k0
+    double kx0, kx1, kx2, ky0, ky1, ky2;
+    // 2x2 bilinear candidates: slopes 1/aw, 1/bw, 1/(aw*bw) and the
+    // target-cell mean of the xy-term (aw^2+bw^2)/12.
+    double inv_aw, inv_bw, inv_awbw, q11w;
+    // ML-WENO weights (src/mlweno/reconstruction.cpp:212-265)
+    double epsA;                // epsilon * area
+    double pL, pS, pC;          // exponents s*r + eta(r): large, small, constant
+    double ldL, ldS[4], ldC;    // log of linear weights (-inf when zero)
+    int32_t use_constant, pad0;
+    double global_speed;
+    // edge quadrature
+    double sv[kMaxQ];           // vertical faces: point offsets hy*xi_q
+    double sh[kMaxQ];           // horizontal faces: point offsets hx*xi_q
+    double wv[kMaxQ];           // w_q * dy/2
+    double wh[kMaxQ];           // w_q * dx/2
+    PhaseDev ph;
+    // boundary data (per face policy and optional inflow [face][q])
+    const uint8_t* bc[4];       // bottom(nx) right(ny) top(nx) left(ny)
+    const double* inflowH[4];
+    const double* inflowC[4];
+    // face velocities [q][ny][nx+1] and [q][ny+1][nx]
+    const double* vnx;
+    const double* vny;
+};
+
+// Per launch, per mesh: the stage u_out = a*u_n + b*(u_k + dt*L(u_k)).
+struct StagePtrs {
+    const double* Hn;
+    const double* Cn;
+    const double* Hk;
+    const double* Ck;
+    double* Ho;
+    double* Co;
+    double a, b, dt;
+    double* T;          // phase outputs; non-null only on a phase-fused stage
+    double* phi;
+    double* Cl;
+    double* Cs;
+    uint8_t* region;
+    double* partials;   // [item][kPartials]: sumH, errH, sumC, errC, boundaryH, boundaryC
+    uint32_t* error;
+    int32_t rows;       // rows per work item
+    int32_t strips;     // work items per row band
+};
+
+}  // namespace mp

@@ -1,0 +1,76 @@
+// Host-side helpers shared by the single-mesh, batched and serving front ends.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mantlepar_amd.h"
+#include "mp_common.h"
+
+namespace mp {
+
+// Thread-local last error (mp_last_error()).
+void set_error(const char* fmt, ...);
+mp_status fail(mp_status code, const char* fmt, ...);
+mp_status status_from_bits(uint32_t bits);
+const char* describe_bits(uint32_t bits);
+
+#define MP_HIP(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return ::mp::fail(MP_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define MP_CALL(expr)                  \
+    do {                               \
+        mp_status s_ = (expr);         \
+        if (s_ != MP_OK) return s_;    \
+    } while (0)
+
+// Gauss-Legendre rule on [-1,1], ascending (Newton on P_n in long double).
+void gauss_legendre(int n, double* x, double* w);
+
+// Derived phase scales (EutecticModel::derive, src/phase/eutectic_rescaled.cpp:42-86)
+// and the uniform-pressure boundaries used by the kernels.
+struct PhaseHost {
+    PhaseDev dev;
+    double TDe0, nu, dT;
+};
+mp_status make_phase(const mp_phase_params& p, PhaseHost& out);
+
+mp_status validate_mesh(const mp_mesh_desc& m);
+mp_status validate_weno(const mp_weno_options& o);
+// Closed-form tables of the production candidates for one uniform mesh.
+mp_status make_mesh_dev(const mp_mesh_desc& mesh, const mp_weno_options& weno,
+                        const PhaseHost& phase, MeshDev& out);
+
+// Device allocation helper that frees on scope exit unless released.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept
+    {
+        if (this != &o) { reset(); ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DeviceBuffer() { reset(); }
+    mp_status alloc(size_t n);
+    void reset();
+    template <typename T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Check host arrays for finite values (the reference rejects non-finite
+// averages and velocities: src/mlweno/reconstruction.cpp:299-320,
+// src/mlweno/advectiveflux.cpp:177-179).
+bool all_finite(const double* v, size_t n);
+
+}  // namespace mp

@@ -1,0 +1,585 @@
+// Stage, diagnostic and utility kernels of the MI355X transport library.
+//
+// Stage kernel layout ("row sweep"): one wavefront owns a strip of 62 cell
+// columns plus one halo column on each side, and walks up a band of rows.
+// Each lane holds a 3x3 window of H and C cell averages in registers, loads
+// one new row per step with coalesced 512-byte wave loads, reconstructs its
+// cell, exchanges face traces with its neighbour lanes through cross-lane
+// shuffles, and finishes the row below once that row's top face is known.
+// There is no LDS tile and no halo recomputation in y; the only redundant
+// work is the two halo lanes and one extra row per band.
+//
+// The flux scatter of the reference (rhs[left] -= F/|K|, rhs[right] += F/|K|,
+// src/couple/preheat.cpp:193-196) becomes a gather: every cell sums its four
+// face fluxes, so there are no atomics on state and results do not depend on
+// launch order.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "mp_common.h"
+#include "mp_launch.h"
+#include "mp_phase.h"
+#include "mp_recon.h"
+
+namespace mp {
+
+template <int MODE>
+using RealT = typename std::conditional<MODE == 0, double, float>::type;
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Outward boundary flux of one face (src/mlweno/advectiveflux.cpp:229-268 with
+// the couple policy src/couple/initialization.cpp:469-489).  sign turns the
+// canonical (+x/+y) normal velocity into the outward one.
+template <typename R>
+__device__ __forceinline__ double boundary_flux(const MeshDev& m, const double* vn, size_t vstride,
+                                                double sign, double t, const R (&c)[3],
+                                                const double* s, const double* W, uint8_t policy,
+                                                const double* inflow, uint32_t& err)
+{
+    double F = 0.0;
+    for (int q = 0; q < m.q; ++q) {
+        const double v = sign * vn[q * vstride];
+        double f = 0.0;
+        if (policy != 0 && v != 0.0) {
+            if (v > 0.0) {
+                f = v * (t + double(line_at<R>(c, R(s[q]))));
+            } else if (policy == 2) {
+                if (inflow) f = v * inflow[q];
+                else err |= kErrInflow;
+            } else {
+                err |= kErrBackflow;
+            }
+        }
+        F += W[q] * f;
+    }
+    return F;
+}
+
+// Interior face flux (left state behind the +x/+y normal).  MODE 0 follows the
+// reference's per-point LLF form F = (F_L + F_R - alpha*(uR - uL))/2
+// (src/mlweno/advectiveflux.cpp:144-152); MODE 1 splits it into an FP64
+// first-order part and an FP32 high-order correction.
+template <int MODE>
+__device__ __forceinline__ double interior_flux(const MeshDev& m, const double* vn, size_t vstride,
+                                                double tL, const RealT<MODE> (&cL)[3], double tR,
+                                                const RealT<MODE> (&cR)[3], const double* s,
+                                                const double* W)
+{
+    using R = RealT<MODE>;
+    if constexpr (MODE == 0) {
+        double F = 0.0;
+        for (int q = 0; q < m.q; ++q) {
+            const double v = vn[q * vstride];
+            const double alpha = m.lf_mode ? m.global_speed : fabs(v);
+            const double uL = tL + line_at<double>(cL, s[q]);
+            const double uR = tR + line_at<double>(cR, s[q]);
+            F += W[q] * (0.5 * (v * uL + v * uR - alpha * (uR - uL)));
+        }
+        return F;
+    } else {
+        double Vp = 0.0, Vm = 0.0;
+        R corr = R(0);
+        for (int q = 0; q < m.q; ++q) {
+            const double v = vn[q * vstride];
+            const double alpha = m.lf_mode ? m.global_speed : fabs(v);
+            const double wl = W[q] * 0.5 * (v + alpha);
+            const double wr = W[q] * 0.5 * (v - alpha);
+            Vp += wl;
+            Vm += wr;
+            const R sq = R(s[q]);
+            corr += R(wl) * line_at<R>(cL, sq) + R(wr) * line_at<R>(cR, sq);
+        }
+        return tL * Vp + tR * Vm + double(corr);
+    }
+}
+
+__device__ __forceinline__ void two_sum_acc(double& s, double& e, double x)
+{
+    const double y = s + x;
+    e += fabs(s) >= fabs(x) ? (s - y) + x : (x - y) + s;
+    s = y;
+}
+
+template <int MODE, bool PHASE>
+__device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int strip, int j0,
+                           int j1)
+{
+    using R = RealT<MODE>;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nx = m.nx, ny = m.ny;
+    const int i = strip * kStripCells - 1 + lane;
+    const bool col_ok = i >= 0 && i < nx;
+    const bool owned = col_ok && lane >= 1 && lane <= kStripCells;
+    const int ic = clampi(i, 0, nx - 1), il = clampi(i - 1, 0, nx - 1), ir = clampi(i + 1, 0, nx - 1);
+    const size_t vxs = size_t(ny) * (nx + 1);    // q stride of vnx
+    const size_t vys = size_t(ny + 1) * nx;      // q stride of vny
+    uint32_t err = 0;
+
+    double wH[3][3], wC[3][3];
+    auto load_row = [&](const double* base, int r, double (&row)[3]) {
+        const double* p = base + size_t(clampi(r, 0, ny - 1)) * nx;
+        row[0] = p[il];
+        row[1] = p[ic];
+        row[2] = p[ir];
+    };
+    int r = j0 > 0 ? j0 - 1 : 0;
+    load_row(sp.Hk, r - 1, wH[0]); load_row(sp.Hk, r, wH[1]); load_row(sp.Hk, r + 1, wH[2]);
+    load_row(sp.Ck, r - 1, wC[0]); load_row(sp.Ck, r, wC[1]); load_row(sp.Ck, r + 1, wC[2]);
+
+    R topH[3] = {R(0), R(0), R(0)}, topC[3] = {R(0), R(0), R(0)};
+    double tTopH = 0.0, tTopC = 0.0;
+    double pFlH = 0, pFrH = 0, pFbH = 0, pFlC = 0, pFrC = 0, pFbC = 0;
+    double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
+
+    for (; r <= j1; ++r) {
+        const bool have = r < ny;
+        const bool preroll = r == j0 - 1;
+        double nH[3], nC[3];
+        const bool shift = r + 1 <= j1;
+        if (shift) {
+            load_row(sp.Hk, r + 2, nH);
+            load_row(sp.Ck, r + 2, nC);
+        }
+        FaceLines<R> LH, LC;
+        if (have) {
+            const unsigned mask = col_ok ? candidate_mask(i, r, nx, ny) : 2u;
+            uint32_t e2 = 0;
+            reconstruct<R>(m, wH, mask, LH, e2);
+            reconstruct<R>(m, wC, mask, LC, e2);
+            if (col_ok) err |= e2;
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) { LH.c[f][k] = R(0); LC.c[f][k] = R(0); }
+        }
+        const double tH = wH[1][1], tC = wC[1][1];
+
+        // Horizontal face y_r: top of row r-1, bottom of row r.
+        double FvH = 0.0, FvC = 0.0;
+        if (!preroll && col_ok) {
+            const double* vn = m.vny + size_t(r) * nx + ic;
+            if (r == 0) {
+                const double* inH = m.inflowH[0] ? m.inflowH[0] + size_t(ic) * m.q : nullptr;
+                const double* inC = m.inflowC[0] ? m.inflowC[0] + size_t(ic) * m.q : nullptr;
+                const uint8_t pol = m.bc[0] ? m.bc[0][ic] : 0;
+                const double oH = boundary_flux<R>(m, vn, vys, -1.0, tH, LH.c[0], m.sh, m.wh, pol, inH, err);
+                const double oC = boundary_flux<R>(m, vn, vys, -1.0, tC, LC.c[0], m.sh, m.wh, pol, inC, err);
+                FvH = -oH; FvC = -oC;
+                if (owned) { bH += oH; bC += oC; }
+            } else if (r == ny) {
+                const double* inH = m.inflowH[2] ? m.inflowH[2] + size_t(ic) * m.q : nullptr;
+                const double* inC = m.inflowC[2] ? m.inflowC[2] + size_t(ic) * m.q : nullptr;
+                const uint8_t pol = m.bc[2] ? m.bc[2][ic] : 0;
+                const double oH = boundary_flux<R>(m, vn, vys, 1.0, tTopH, topH, m.sh, m.wh, pol, inH, err);
+                const double oC = boundary_flux<R>(m, vn, vys, 1.0, tTopC, topC, m.sh, m.wh, pol, inC, err);
+                FvH = oH; FvC = oC;
+                if (owned) { bH += oH; bC += oC; }
+            } else {
+                FvH = interior_flux<MODE>(m, vn, vys, tTopH, topH, tH, LH.c[0], m.sh, m.wh);
+                FvC = interior_flux<MODE>(m, vn, vys, tTopC, topC, tC, LC.c[0], m.sh, m.wh);
+            }
+        }
+
+        // Finish row r-1 now that its top face is known.
+        if (!preroll && r > j0) {
+            if (owned) {
+                const size_t k = size_t(r - 1) * nx + i;
+                const double rhsH = (pFlH - pFrH + pFbH - FvH) * m.inv_area;
+                const double rhsC = (pFlC - pFrC + pFbC - FvC) * m.inv_area;
+                const double wkH = wH[0][1] + sp.dt * rhsH;
+                const double wkC = wC[0][1] + sp.dt * rhsC;
+                double oH = sp.b * wkH, oC = sp.b * wkC;
+                if (sp.a != 0.0) {
+                    oH = sp.a * sp.Hn[k] + oH;
+                    oC = sp.a * sp.Cn[k] + oC;
+                }
+                if (!isfinite(oH) || !isfinite(oC)) err |= kErrNonFinite;
+                sp.Ho[k] = oH;
+                sp.Co[k] = oC;
+                if constexpr (PHASE) {
+                    const PhaseOut po = phase_eval(m.ph, oH, oC, err);
+                    sp.T[k] = po.T;
+                    sp.phi[k] = po.phil;
+                    sp.Cl[k] = po.cl;
+                    sp.Cs[k] = po.cs;
+                    sp.region[k] = uint8_t(po.region);
+                    two_sum_acc(sH, eH, oH);
+                    two_sum_acc(sC, eC, oC);
+                }
+            }
+        }
+
+        // Vertical faces x_{i+1} of row r; neighbour lane i+1 supplies its left face.
+        if (!preroll && r < j1) {
+            const double nbtH = __shfl_down(tH, 1);
+            const double nbtC = __shfl_down(tC, 1);
+            R nbH[3], nbC[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                nbH[k] = __shfl_down(LH.c[3][k], 1);
+                nbC[k] = __shfl_down(LC.c[3][k], 1);
+            }
+            double FrH = 0.0, FrC = 0.0;
+            if (i >= -1 && i <= nx - 1 && have) {
+                const double* vn = m.vnx + size_t(r) * (nx + 1) + (i + 1);
+                if (i == -1) {
+                    const double* inH = m.inflowH[3] ? m.inflowH[3] + size_t(r) * m.q : nullptr;
+                    const double* inC = m.inflowC[3] ? m.inflowC[3] + size_t(r) * m.q : nullptr;
+                    const uint8_t pol = m.bc[3] ? m.bc[3][r] : 0;
+                    const double oH = boundary_flux<R>(m, vn, vxs, -1.0, nbtH, nbH, m.sv, m.wv, pol, inH, err);
+                    const double oC = boundary_flux<R>(m, vn, vxs, -1.0, nbtC, nbC, m.sv, m.wv, pol, inC, err);
+                    FrH = -oH; FrC = -oC;
+                    bH += oH; bC += oC;
+                } else if (i == nx - 1) {
+                    const double* inH = m.inflowH[1] ? m.inflowH[1] + size_t(r) * m.q : nullptr;
+                    const double* inC = m.inflowC[1] ? m.inflowC[1] + size_t(r) * m.q : nullptr;
+                    const uint8_t pol = m.bc[1] ? m.bc[1][r] : 0;
+                    const double oH = boundary_flux<R>(m, vn, vxs, 1.0, tH, LH.c[1], m.sv, m.wv, pol, inH, err);
+                    const double oC = boundary_flux<R>(m, vn, vxs, 1.0, tC, LC.c[1], m.sv, m.wv, pol, inC, err);
+                    FrH = oH; FrC = oC;
+                    if (owned) { bH += oH; bC += oC; }
+                } else {
+                    FrH = interior_flux<MODE>(m, vn, vxs, tH, LH.c[1], nbtH, nbH, m.sv, m.wv);
+                    FrC = interior_flux<MODE>(m, vn, vxs, tC, LC.c[1], nbtC, nbC, m.sv, m.wv);
+                }
+            }
+            pFlH = __shfl_up(FrH, 1);
+            pFlC = __shfl_up(FrC, 1);
+            pFrH = FrH; pFrC = FrC;
+            pFbH = FvH; pFbC = FvC;
+        }
+
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { topH[k] = LH.c[2][k]; topC[k] = LC.c[2][k]; }
+        tTopH = tH; tTopC = tC;
+        if (shift) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                wH[0][k] = wH[1][k]; wH[1][k] = wH[2][k]; wH[2][k] = nH[k];
+                wC[0][k] = wC[1][k]; wC[1][k] = wC[2][k]; wC[2][k] = nC[k];
+            }
+        }
+    }
+
+    // Wave reduction of the item's partial sums (fixed butterfly order).
+    double v[6] = {sH, eH, sC, eC, bH, bC};
+    for (int off = 32; off >= 1; off >>= 1) {
+        double o[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) o[k] = __shfl_xor(v[k], off);
+        double s0 = v[0], e0 = v[1] + o[1];
+        two_sum_acc(s0, e0, o[0]);
+        double s1 = v[2], e1 = v[3] + o[3];
+        two_sum_acc(s1, e1, o[2]);
+        v[0] = s0; v[1] = e0; v[2] = s1; v[3] = e1;
+        v[4] += o[4];
+        v[5] += o[5];
+    }
+    if (lane == 0 && sp.partials) {
+        double* p = sp.partials + size_t(item) * kPartials;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) p[k] = v[k];
+    }
+    if (err) atomicOr(sp.error, err);
+}
+
+template <int MODE, bool PHASE>
+__global__ __launch_bounds__(256) void stage_sweep_kernel(const MeshDev* __restrict__ mesh,
+                                                          StagePtrs sp, int nitems)
+{
+    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= nitems) return;
+    const MeshDev& m = *mesh;
+    const int strip = item % sp.strips, band = item / sp.strips;
+    const int j0 = band * sp.rows;
+    const int j1 = min(j0 + sp.rows, m.ny);
+    sweep_item<MODE, PHASE>(m, sp, item, strip, j0, j1);
+}
+
+// ------------------------------------------------------------ diagnostics
+
+template <int MODE>
+__device__ __forceinline__ void load_window(const MeshDev& m, const double* u, int i, int j,
+                                            double (&w)[3][3])
+{
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+        for (int di = -1; di <= 1; ++di)
+            w[dj + 1][di + 1] = u[size_t(clampi(j + dj, 0, m.ny - 1)) * m.nx + clampi(i + di, 0, m.nx - 1)];
+}
+
+template <int MODE>
+__global__ void recon_diag_kernel(const MeshDev* __restrict__ mesh, const double* __restrict__ u,
+                                  double* sigma, double* omega, double* traces, uint32_t* error)
+{
+    using R = RealT<MODE>;
+    const MeshDev& m = *mesh;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    const int i = c % m.nx, j = c / m.nx;
+    double w[3][3];
+    load_window<MODE>(m, u, i, j, w);
+    FaceLines<R> L;
+    uint32_t err = 0;
+    reconstruct<R>(m, w, candidate_mask(i, j, m.nx, m.ny), L, err, sigma + size_t(c) * 5,
+                   omega + size_t(c) * 5);
+    for (int f = 0; f < 4; ++f)
+        for (int q = 0; q < m.q; ++q) {
+            const double s = (f & 1) ? m.sv[q] : m.sh[q];
+            traces[(size_t(c) * 4 + f) * m.q + q] = w[1][1] + double(line_at<R>(L.c[f], R(s)));
+        }
+    if (err) atomicOr(error, err);
+}
+
+// L(u) per cell by reconstructing the cell and its four neighbours (5x the
+// work of the sweep; diagnostics only).  bnd[c*2+f] = outward boundary flux.
+template <int MODE>
+__global__ void rhs_diag_kernel(const MeshDev* __restrict__ mesh, const double* __restrict__ H,
+                                const double* __restrict__ C, double* rhsH, double* rhsC,
+                                double* bnd, uint32_t* error)
+{
+    using R = RealT<MODE>;
+    const MeshDev& m = *mesh;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    const int nx = m.nx, ny = m.ny;
+    const int i = c % nx, j = c / nx;
+    const size_t vxs = size_t(ny) * (nx + 1), vys = size_t(ny + 1) * nx;
+    uint32_t err = 0;
+    for (int field = 0; field < 2; ++field) {
+        const double* u = field ? C : H;
+        double w[3][3];
+        load_window<MODE>(m, u, i, j, w);
+        FaceLines<R> L;
+        reconstruct<R>(m, w, candidate_mask(i, j, nx, ny), L, err);
+        const double t = w[1][1];
+        double Fin = 0.0, Fout_b = 0.0;   // net inflow, outward boundary flux
+        // neighbours: 0 bottom (i,j-1), 1 right (i+1,j), 2 top (i,j+1), 3 left (i-1,j)
+        const int ni[4] = {i, i + 1, i, i - 1}, nj[4] = {j - 1, j, j + 1, j};
+        for (int f = 0; f < 4; ++f) {
+            const bool vert = f & 1;
+            const double* vn = vert ? m.vnx + size_t(j) * (nx + 1) + (f == 1 ? i + 1 : i)
+                                    : m.vny + size_t(f == 2 ? j + 1 : j) * nx + i;
+            const size_t vs = vert ? vxs : vys;
+            const double* s = vert ? m.sv : m.sh;
+            const double* W = vert ? m.wv : m.wh;
+            const bool outside = ni[f] < 0 || ni[f] >= nx || nj[f] < 0 || nj[f] >= ny;
+            const double sign = (f == 1 || f == 2) ? 1.0 : -1.0;   // outward vs canonical
+            if (outside) {
+                const int side = f;   // bottom, right, top, left
+                const int k = vert ? j : i;
+                const double* const* inflow = field ? m.inflowC : m.inflowH;
+                const double* in = inflow[side] ? inflow[side] + size_t(k) * m.q : nullptr;
+                const uint8_t pol = m.bc[side] ? m.bc[side][k] : 0;
+                const double o = boundary_flux<R>(m, vn, vs, sign, t, L.c[f], s, W, pol, in, err);
+                Fin -= o;
+                Fout_b += o;
+            } else {
+                double wn[3][3];
+                load_window<MODE>(m, u, ni[f], nj[f], wn);
+                FaceLines<R> N;
+                reconstruct<R>(m, wn, candidate_mask(ni[f], nj[f], nx, ny), N, err);
+                const int opp = (f + 2) & 3;
+                double F;
+                if (f == 1 || f == 2)
+                    F = interior_flux<MODE>(m, vn, vs, t, L.c[f], wn[1][1], N.c[opp], s, W);
+                else
+                    F = interior_flux<MODE>(m, vn, vs, wn[1][1], N.c[opp], t, L.c[f], s, W);
+                Fin += sign > 0 ? -F : F;
+            }
+        }
+        (field ? rhsC : rhsH)[c] = Fin * m.inv_area;
+        bnd[size_t(c) * 2 + field] = Fout_b;
+    }
+    if (err) atomicOr(error, err);
+}
+
+// Per-cell advective rate (src/couple/preheat.cpp:125-152); max via the
+// ordered bit pattern of non-negative doubles.
+__global__ void rate_kernel(const MeshDev* __restrict__ mesh, unsigned long long* max_bits)
+{
+    const MeshDev& m = *mesh;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double rate = 0.0;
+    if (c < m.nx * m.ny) {
+        const int nx = m.nx, ny = m.ny, i = c % nx, j = c / nx;
+        const size_t vxs = size_t(ny) * (nx + 1), vys = size_t(ny + 1) * nx;
+        double sx = 0.0, sy = 0.0;
+        for (int q = 0; q < m.q; ++q) {
+            const double a = m.wv[q] * fabs(m.vnx[q * vxs + size_t(j) * (nx + 1) + i]);
+            const double b = m.wv[q] * fabs(m.vnx[q * vxs + size_t(j) * (nx + 1) + i + 1]);
+            const double cc = m.wh[q] * fabs(m.vny[q * vys + size_t(j) * nx + i]);
+            const double d = m.wh[q] * fabs(m.vny[q * vys + size_t(j + 1) * nx + i]);
+            sx += a + b;
+            sy += cc + d;
+        }
+        rate = (sx + sy) * m.inv_area;
+    }
+    for (int off = 32; off >= 1; off >>= 1) rate = fmax(rate, __shfl_xor(rate, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(max_bits, (unsigned long long)__double_as_longlong(rate));
+}
+
+// EutecticModel::evaluate over arbitrary arrays.  P (optional) per element:
+// Tep = TDe0 + P/nu/dT formed in double-double like the long double original.
+__global__ void phase_kernel(PhaseDev ph, double TDe0, double nu, double dT, int64_t n,
+                             const double* __restrict__ H, const double* __restrict__ C,
+                             const double* __restrict__ P, double* T, double* phi, double* Cl,
+                             double* Cs, uint8_t* region, double* phi1, double* phi2,
+                             double* dTdC, double* dTdH, uint32_t* error)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t err = 0;
+    PhaseDev p = ph;
+    if (P) {
+        const double Pk = P[k];
+        if (!isfinite(Pk)) err |= kErrNonFinite;
+        DD a = dd_div(Pk, nu);
+        const double q1 = a.hi / dT;
+        const double r1 = (fma(-q1, dT, a.hi) + a.lo) / dT;
+        const DD shift = dd_two_sum(q1, r1);
+        p.Tep = dd_add({TDe0, 0.0}, shift).hi;
+        p.Tmp = p.Tep + 1.0;
+    }
+    const PhaseOut o = phase_eval(p, H[k], C[k], err);
+    if (T) T[k] = o.T;
+    if (phi) phi[k] = o.phil;
+    if (Cl) Cl[k] = o.cl;
+    if (Cs) Cs[k] = o.cs;
+    if (region) region[k] = uint8_t(o.region);
+    if (phi1) phi1[k] = o.phi1;
+    if (phi2) phi2[k] = o.phi2;
+    if (dTdC) dTdC[k] = o.dTdC;
+    if (dTdH) dTdH[k] = o.dTdH;
+    if (err) atomicOr(error, err);
+}
+
+// Phase of a state (used when no step has produced one yet).
+__global__ void state_phase_kernel(const MeshDev* __restrict__ mesh, const double* H,
+                                   const double* C, double* T, double* phi, double* Cl,
+                                   double* Cs, uint8_t* region, uint32_t* error)
+{
+    const MeshDev& m = *mesh;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    uint32_t err = 0;
+    const PhaseOut o = phase_eval(m.ph, H[c], C[c], err);
+    T[c] = o.T; phi[c] = o.phil; Cl[c] = o.cl; Cs[c] = o.cs; region[c] = uint8_t(o.region);
+    if (err) atomicOr(error, err);
+}
+
+// Step totals from the stage partials, in fixed order (one wave):
+// out[0..3] = sum H, sum C (times area), stage-weighted boundary H, C.
+__global__ void reduce_step_kernel(const double* __restrict__ partials, int nitems, int nstages,
+                                   double w0, double w1, double w2, double area, double* out)
+{
+    const int lane = threadIdx.x;
+    double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
+    const double wst[3] = {w0, w1, w2};
+    for (int st = 0; st < nstages; ++st) {
+        const double* p = partials + size_t(st) * nitems * kPartials;
+        for (int it = lane; it < nitems; it += 64) {
+            const double* q = p + size_t(it) * kPartials;
+            if (st == nstages - 1) {
+                two_sum_acc(sH, eH, q[0]); eH += q[1];
+                two_sum_acc(sC, eC, q[2]); eC += q[3];
+            }
+            bH += wst[st] * q[4];
+            bC += wst[st] * q[5];
+        }
+    }
+    double v[6] = {sH, eH, sC, eC, bH, bC};
+    for (int off = 32; off >= 1; off >>= 1) {
+        double o[6];
+        for (int k = 0; k < 6; ++k) o[k] = __shfl_xor(v[k], off);
+        double s0 = v[0], e0 = v[1] + o[1];
+        two_sum_acc(s0, e0, o[0]);
+        double s1 = v[2], e1 = v[3] + o[3];
+        two_sum_acc(s1, e1, o[2]);
+        v[0] = s0; v[1] = e0; v[2] = s1; v[3] = e1; v[4] += o[4]; v[5] += o[5];
+    }
+    if (lane == 0) {
+        out[0] = (v[0] + v[1]) * area;
+        out[1] = (v[2] + v[3]) * area;
+        out[2] = v[4];
+        out[3] = v[5];
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+hipError_t launch_stage(int mode, bool phase, const MeshDev* d_mesh, const StagePtrs& sp,
+                        int nitems, hipStream_t stream)
+{
+    const dim3 grid((nitems + 3) / 4), block(256);
+    if (mode == 0) {
+        if (phase) stage_sweep_kernel<0, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        else stage_sweep_kernel<0, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    } else {
+        if (phase) stage_sweep_kernel<1, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        else stage_sweep_kernel<1, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_recon_diag(int mode, const MeshDev* d_mesh, int ncells, const double* u,
+                             double* sigma, double* omega, double* traces, uint32_t* error,
+                             hipStream_t stream)
+{
+    const dim3 grid((ncells + 127) / 128), block(128);
+    if (mode == 0) recon_diag_kernel<0><<<grid, block, 0, stream>>>(d_mesh, u, sigma, omega, traces, error);
+    else recon_diag_kernel<1><<<grid, block, 0, stream>>>(d_mesh, u, sigma, omega, traces, error);
+    return hipGetLastError();
+}
+
+hipError_t launch_rhs_diag(int mode, const MeshDev* d_mesh, int ncells, const double* H,
+                           const double* C, double* rhsH, double* rhsC, double* bnd,
+                           uint32_t* error, hipStream_t stream)
+{
+    const dim3 grid((ncells + 127) / 128), block(128);
+    if (mode == 0) rhs_diag_kernel<0><<<grid, block, 0, stream>>>(d_mesh, H, C, rhsH, rhsC, bnd, error);
+    else rhs_diag_kernel<1><<<grid, block, 0, stream>>>(d_mesh, H, C, rhsH, rhsC, bnd, error);
+    return hipGetLastError();
+}
+
+hipError_t launch_rate(const MeshDev* d_mesh, int ncells, unsigned long long* max_bits,
+                       hipStream_t stream)
+{
+    const dim3 grid((ncells + 255) / 256), block(256);
+    rate_kernel<<<grid, block, 0, stream>>>(d_mesh, max_bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase(const PhaseDev& ph, double TDe0, double nu, double dT, int64_t n,
+                        const double* H, const double* C, const double* P, double* T,
+                        double* phi, double* Cl, double* Cs, uint8_t* region, double* phi1,
+                        double* phi2, double* dTdC, double* dTdH, uint32_t* error,
+                        hipStream_t stream)
+{
+    const dim3 grid(unsigned((n + 255) / 256)), block(256);
+    phase_kernel<<<grid, block, 0, stream>>>(ph, TDe0, nu, dT, n, H, C, P, T, phi, Cl, Cs, region,
+                                             phi1, phi2, dTdC, dTdH, error);
+    return hipGetLastError();
+}
+
+hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H, const double* C,
+                              double* T, double* phi, double* Cl, double* Cs, uint8_t* region,
+                              uint32_t* error, hipStream_t stream)
+{
+    const dim3 grid((ncells + 255) / 256), block(256);
+    state_phase_kernel<<<grid, block, 0, stream>>>(d_mesh, H, C, T, phi, Cl, Cs, region, error);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, const double* w,
+                              double area, double* out, hipStream_t stream)
+{
+    reduce_step_kernel<<<1, 64, 0, stream>>>(partials, nitems, nstages, w[0], w[1], w[2], area, out);
+    return hipGetLastError();
+}
+
+}  // namespace mp

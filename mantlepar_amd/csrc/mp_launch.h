@@ -1,0 +1,31 @@
+// Host-side launchers of the device kernels (mp_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mp_common.h"
+
+namespace mp {
+
+hipError_t launch_stage(int mode, bool phase, const MeshDev* d_mesh, const StagePtrs& sp,
+                        int nitems, hipStream_t stream);
+hipError_t launch_recon_diag(int mode, const MeshDev* d_mesh, int ncells, const double* u,
+                             double* sigma, double* omega, double* traces, uint32_t* error,
+                             hipStream_t stream);
+hipError_t launch_rhs_diag(int mode, const MeshDev* d_mesh, int ncells, const double* H,
+                           const double* C, double* rhsH, double* rhsC, double* bnd,
+                           uint32_t* error, hipStream_t stream);
+hipError_t launch_rate(const MeshDev* d_mesh, int ncells, unsigned long long* max_bits,
+                       hipStream_t stream);
+hipError_t launch_phase(const PhaseDev& ph, double TDe0, double nu, double dT, int64_t n,
+                        const double* H, const double* C, const double* P, double* T,
+                        double* phi, double* Cl, double* Cs, uint8_t* region, double* phi1,
+                        double* phi2, double* dTdC, double* dTdH, uint32_t* error,
+                        hipStream_t stream);
+hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H, const double* C,
+                              double* T, double* phi, double* Cl, double* Cs, uint8_t* region,
+                              uint32_t* error, hipStream_t stream);
+hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, const double* w,
+                              double area, double* out, hipStream_t stream);
+
+}  // namespace mp

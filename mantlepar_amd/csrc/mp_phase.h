@@ -1,0 +1,169 @@
+// Device binary-eutectic equilibrium: EutecticModel::evaluate restated for
+// FP64 on gfx950 (src/phase/eutectic_rescaled.cpp:95-282).
+//
+// The reference evaluates in 80-bit long double and rounds each public value
+// to double.  The device has no long double, so the phase boundaries, whose
+// rounding decides the region, are formed in double-double and rounded once;
+// they then agree with the reference's long double -> double results except
+// where the exact boundary lies within a few ulps of a rounding tie.  All
+// other quantities are plain FP64 and agree to a few ulps.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mp_common.h"
+
+namespace mp {
+
+struct PhaseOut {
+    double T, phil, cl, cs, phi1, phi2, dTdC, dTdH;
+    int region;     // PhaseRegion: 1 PureSolid .. 6 PureMelting
+};
+
+struct DD { double hi, lo; };
+
+__device__ __forceinline__ DD dd_two_sum(double a, double b)
+{
+    const double s = a + b;
+    const double bb = s - a;
+    const double e = (a - (s - bb)) + (b - bb);
+    return {s, e};
+}
+
+__device__ __forceinline__ DD dd_add(DD a, DD b)
+{
+    DD s = dd_two_sum(a.hi, b.hi);
+    s.lo += a.lo + b.lo;
+    return dd_two_sum(s.hi, s.lo);
+}
+
+// c / x as a double-double (quotient plus its fma residual).
+__device__ __forceinline__ DD dd_div(double c, double x)
+{
+    const double q = c / x;
+    const double r = fma(-q, x, c);
+    return {q, r / x};
+}
+
+__device__ __forceinline__ DD dd_mul_d(DD a, double b)
+{
+    const double p = a.hi * b;
+    const double e = fma(a.hi, b, -p) + a.lo * b;
+    return dd_two_sum(p, e);
+}
+
+// unitFraction (src/phase/eutectic_rescaled.cpp:108-114)
+__device__ __forceinline__ double unit_fraction(double v, uint32_t& err)
+{
+    const double tol = 64.0 * 2.220446049250313e-16;
+    if (!(v >= -tol && v <= 1.0 + tol)) err |= kErrFraction;
+    return fmin(fmax(v, 0.0), 1.0);
+}
+
+// One evaluation on the side 0 <= C <= Xe of a diagram with eutectic Xe.
+__device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, double H, double C,
+                                               uint32_t& err)
+{
+    PhaseOut s;
+    s.T = 0; s.phil = 0; s.cl = 0; s.cs = 0; s.phi1 = 0; s.phi2 = 0; s.dTdC = 0; s.dTdH = 0;
+    const double LD = p.LD, Tep = p.Tep, Tmp = p.Tmp;
+    // boundaries (:163-185)
+    const DD ratio = dd_div(C, Xe);
+    const double Heut = dd_add({Tep, 0.0}, dd_mul_d(ratio, LD)).hi;
+    double Hliq;
+    if (C == 0.0) {
+        Hliq = Tmp + LD;
+    } else {
+        const DD one_minus = dd_add({1.0, 0.0}, {-ratio.hi, -ratio.lo});
+        Hliq = dd_add(dd_add({Tep, 0.0}, one_minus), {LD, 0.0}).hi;
+    }
+    if (Tmp <= Tep || Hliq <= Tep || (C == 0.0 && Hliq <= Tmp)) err |= kErrPhaseRange;
+    // selectRegion (:95-105)
+    int region;
+    if (C == 0.0) region = H <= Tmp ? 1 : (H < Hliq ? 6 : 5);
+    else if (H <= Tep) region = 2;
+    else if (H >= Hliq) region = 5;
+    else if (H <= Heut) region = 3;
+    else region = 4;
+    s.region = region;
+    switch (region) {
+    case 1:
+        s.phi1 = 1.0; s.T = H; s.dTdH = 1.0;
+        break;
+    case 6:
+        s.phil = unit_fraction((H - Tmp) / LD, err);
+        s.phi1 = 1.0 - s.phil;
+        s.T = Tmp;
+        break;
+    case 2:
+        s.phi1 = 1.0 - C; s.phi2 = C; s.T = H; s.dTdH = 1.0;
+        break;
+    case 3: {
+        const double maxl = C / Xe;
+        const double liquid = H == Heut ? maxl : fmin(maxl, (H - Tep) / LD);
+        const double solid2 = fmax(0.0, fma(-liquid, Xe, C));
+        s.phil = unit_fraction(liquid, err);
+        s.phi2 = unit_fraction(solid2, err);
+        s.phi1 = unit_fraction(1.0 - liquid - solid2, err);
+        s.cl = Xe;
+        s.T = Tep;
+        break;
+    }
+    case 4: {
+        // LD*phi^2 - (H-Tmp)*phi - C/Xe = 0 with the cancellation-free root (:237-257)
+        const double a = H - Tmp;
+        const double r = C / Xe;
+        const double root = hypot(a, 2.0 * sqrt(LD) * sqrt(r));
+        const double liquid = a >= 0.0 ? (a + root) / (2.0 * LD) : 2.0 * r / (root - a);
+        s.phil = unit_fraction(liquid, err);
+        s.phi1 = unit_fraction(1.0 - liquid, err);
+        s.cl = unit_fraction(C / liquid, err);
+        s.T = a >= 0.0 ? Tmp - r / liquid : H - LD * liquid;
+        s.dTdH = unit_fraction(a >= 0.0 ? 2.0 * (LD / root) * (r / (root + a))
+                                        : 0.5 * (1.0 - a / root), err);
+        s.dTdC = -((LD / root) / Xe);
+        break;
+    }
+    default:
+        s.phil = 1.0; s.cl = C; s.T = H - LD; s.dTdH = 1.0;
+        break;
+    }
+    if (C == 0.0 && H > Tep && region != 5) {
+        const double gap = fabs(H - Tmp);
+        s.dTdC = gap == 0.0 ? -__builtin_inf() : -((LD / gap) / Xe);
+    }
+    const double solid = s.phi1 + s.phi2;
+    s.cs = solid > 0.0 ? s.phi2 / solid : 0.0;
+    if (!(s.phil > 0.0)) s.cl = 0.0;
+    return s;
+}
+
+// EutecticModel::evaluate with optional hypereutectic mirror (C > Xe evaluates
+// C' = 1-C on the diagram with eutectic 1-Xe and maps the phases back).
+__device__ __forceinline__ PhaseOut phase_eval(const PhaseDev& p, double H, double C,
+                                               uint32_t& err)
+{
+    if (!isfinite(H) || !isfinite(C)) {
+        err |= kErrNonFinite;
+        H = isfinite(H) ? H : 0.0;
+        C = isfinite(C) ? C : 0.0;
+    }
+    const double Xe = p.Xe;
+    const double cmax = p.hyper ? 1.0 : Xe;
+    if (C < 0.0 || C > cmax) {
+        err |= kErrComposition;
+        C = fmin(fmax(C, 0.0), cmax);
+    }
+    if (!(C > Xe)) return phase_core(p, Xe, H, C, err);
+    PhaseOut s = phase_core(p, 1.0 - Xe, H, 1.0 - C, err);
+    PhaseOut r = s;
+    r.phi1 = s.phi2;
+    r.phi2 = s.phi1;
+    r.cl = s.phil > 0.0 ? 1.0 - s.cl : 0.0;
+    const double solid = r.phi1 + r.phi2;
+    r.cs = solid > 0.0 ? r.phi2 / solid : 0.0;
+    r.dTdC = -s.dTdC;
+    return r;
+}
+
+}  // namespace mp

@@ -1,0 +1,263 @@
+// Host construction of per-mesh device records: Gauss rule, derived phase
+// scales and the closed-form reconstruction tables.
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+
+#include "mp_host.h"
+
+namespace mp {
+
+static thread_local char g_error[512];
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof g_error, fmt, ap);
+    va_end(ap);
+}
+
+mp_status fail(mp_status code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof g_error, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+const char* last_error() { return g_error; }
+
+mp_status status_from_bits(uint32_t bits)
+{
+    if (bits & kErrBackflow) return MP_ERR_BACKFLOW;
+    if (bits & kErrComposition) return MP_ERR_COMPOSITION;
+    if (bits & kErrFraction) return MP_ERR_PHASE;
+    if (bits & kErrInflow) return MP_ERR_ARG;
+    if (bits & (kErrNonFinite | kErrWeights | kErrPhaseRange)) return MP_ERR_FP;
+    return MP_OK;
+}
+
+const char* describe_bits(uint32_t bits)
+{
+    if (bits & kErrBackflow) return "Backflow encountered on an outflow-only boundary";
+    if (bits & kErrComposition) return "CD must lie in [0, Xe]";
+    if (bits & kErrFraction) return "Computed phase fraction is outside [0, 1]";
+    if (bits & kErrInflow) return "Missing inflow profile";
+    if (bits & kErrNonFinite) return "Non-finite state, trace or flux";
+    if (bits & kErrWeights) return "No finite active log weight";
+    if (bits & kErrPhaseRange) return "Phase temperature/enthalpy intervals are not resolvable";
+    return "ok";
+}
+
+void gauss_legendre(int n, double* x, double* w)
+{
+    const long double pi = 3.14159265358979323846264338327950288L;
+    for (int k = 0; k < n; ++k) {
+        long double z = cosl(pi * (k + 0.75L) / (n + 0.5L)), dp = 0;
+        for (int it = 0; it < 100; ++it) {
+            long double p0 = 1, p1 = z;
+            for (int m = 2; m <= n; ++m) {
+                const long double p2 = ((2 * m - 1) * z * p1 - (m - 1) * p0) / m;
+                p0 = p1;
+                p1 = p2;
+            }
+            if (n == 1) { p0 = 1; p1 = z; }
+            dp = n * (z * p1 - p0) / (z * z - 1);
+            const long double dz = p1 / dp;
+            z -= dz;
+            if (fabsl(dz) < 1e-19L) break;
+        }
+        long double p0 = 1, p1 = z;
+        for (int m = 2; m <= n; ++m) {
+            const long double p2 = ((2 * m - 1) * z * p1 - (m - 1) * p0) / m;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (z * p1 - p0) / (z * z - 1);
+        x[n - 1 - k] = double(z);
+        w[n - 1 - k] = double(2.0L / ((1 - z * z) * dp * dp));
+    }
+    if (n % 2) x[n / 2] = 0.0;
+    for (int k = 0; k < n / 2; ++k) {
+        const double a = 0.5 * (x[n - 1 - k] - x[k]);
+        x[k] = -a;
+        x[n - 1 - k] = a;
+        const double ww = 0.5 * (w[k] + w[n - 1 - k]);
+        w[k] = ww;
+        w[n - 1 - k] = ww;
+    }
+}
+
+static bool finite_ld(long double v) { return std::isfinite(v) && fabsl(v) <= (long double)DBL_MAX; }
+
+mp_status make_phase(const mp_phase_params& p, PhaseHost& out)
+{
+    // derive(), src/phase/eutectic_rescaled.cpp:42-86 (validation and scales
+    // the transport path uses).
+    auto pos = [](double v) { return std::isfinite(v) && v > 0.0; };
+    if (!pos(p.Tm0)) return fail(MP_ERR_ARG, "Tm0 must be positive");
+    if (!pos(p.Te0)) return fail(MP_ERR_ARG, "Te0 must be positive");
+    if (p.Tm0 <= p.Te0) return fail(MP_ERR_ARG, "Tm0 must be greater than Te0");
+    if (!std::isfinite(p.nu) || p.nu == 0.0) return fail(MP_ERR_ARG, "nu must be finite and nonzero");
+    if (!pos(p.L)) return fail(MP_ERR_ARG, "L must be positive");
+    if (!pos(p.cp)) return fail(MP_ERR_ARG, "cp must be positive");
+    if (!std::isfinite(p.Xe) || p.Xe <= 0.0 || p.Xe >= 1.0)
+        return fail(MP_ERR_ARG, "Xe must lie strictly between 0 and 1");
+    if (!pos(p.rho) || !pos(p.rhor) || !pos(p.mus) || !pos(p.mul) || !pos(p.k0) || !pos(p.g))
+        return fail(MP_ERR_ARG, "rho, rhor, mus, mul, k0 and g must be positive");
+    if (!std::isfinite(p.alpha0)) return fail(MP_ERR_ARG, "alpha0 must be finite");
+    if (!std::isfinite(p.pressure)) return fail(MP_ERR_ARG, "P must be finite");
+    const long double deltaT = (long double)p.Tm0 - p.Te0;
+    const long double h0 = (long double)p.cp * deltaT;
+    const long double LD = p.L / h0;
+    const long double TDe0 = p.Te0 / deltaT;
+    if (!finite_ld(deltaT) || !finite_ld(h0) || !finite_ld(LD) || !finite_ld(TDe0) ||
+        (double)LD <= 0.0 || (double)TDe0 <= 0.0 || (double)deltaT <= 0.0)
+        return fail(MP_ERR_FP, "phase scales are not representable");
+    PhaseHost h;
+    std::memset(&h, 0, sizeof h);
+    h.dT = (double)deltaT;
+    h.TDe0 = (double)TDe0;
+    h.nu = p.nu;
+    h.dev.LD = (double)LD;
+    h.dev.Xe = p.Xe;
+    // boundaries(): Tep and Tmp rounded once (eutectic_rescaled.cpp:143-149, 166-169)
+    const long double shift = (long double)p.pressure / p.nu / h.dT;
+    const long double tep = (long double)h.TDe0 + shift;
+    if (!finite_ld(tep)) return fail(MP_ERR_FP, "pressure-corrected temperature is not representable");
+    h.dev.Tep = (double)tep;
+    const long double tmp = (long double)h.dev.Tep + 1.0L;
+    if (!finite_ld(tmp)) return fail(MP_ERR_FP, "Tmp is not representable");
+    h.dev.Tmp = (double)tmp;
+    if (h.dev.Tmp <= h.dev.Tep) return fail(MP_ERR_FP, "Phase temperature intervals are not resolvable");
+    h.dev.hyper = p.hypereutectic ? 1 : 0;
+    out = h;
+    return MP_OK;
+}
+
+mp_status validate_mesh(const mp_mesh_desc& m)
+{
+    if (m.nx < 2 || m.ny < 2) return fail(MP_ERR_ARG, "Transport requires at least 2x2 cells");
+    if ((int64_t)m.nx * (int64_t)m.ny > (int64_t)1 << 31)
+        return fail(MP_ERR_ARG, "mesh exceeds 2^31 cells");
+    if (m.q < 1 || m.q > kMaxQ) return fail(MP_ERR_ARG, "edge Gauss points must be 1..%d", kMaxQ);
+    if (!std::isfinite(m.x0) || !std::isfinite(m.x1) || !std::isfinite(m.y0) || !std::isfinite(m.y1) ||
+        !(m.x1 > m.x0) || !(m.y1 > m.y0))
+        return fail(MP_ERR_ARG, "Mesh origin must be finite and lengths finite and positive");
+    return MP_OK;
+}
+
+mp_status validate_weno(const mp_weno_options& o)
+{
+    // src/mlweno/reconstruction.cpp:123-131, 35-38
+    if (!std::isfinite(o.epsilon) || o.epsilon <= 0 || o.s < 0)
+        return fail(MP_ERR_ARG, "h and epsilon must be finite and positive; s must be nonnegative");
+    if (!std::isfinite(o.constant_linear_weight) || o.constant_linear_weight < 0)
+        return fail(MP_ERR_ARG, "Constant linear weight must be finite and nonnegative");
+    if (!std::isfinite(o.large_weight) || o.large_weight < 0)
+        return fail(MP_ERR_ARG, "Linear weights must be finite and nonnegative");
+    for (int k = 0; k < 4; ++k)
+        if (!std::isfinite(o.small_weight[k]) || o.small_weight[k] < 0)
+            return fail(MP_ERR_ARG, "Linear weights must be finite and nonnegative");
+    if (o.lf_mode != 0 && o.lf_mode != 1) return fail(MP_ERR_ARG, "Unknown LF mode");
+    if (o.lf_mode == 1 && (!std::isfinite(o.global_speed) || o.global_speed < 0))
+        return fail(MP_ERR_ARG, "Global LF speed must be finite and nonnegative");
+    return MP_OK;
+}
+
+static double log_weight(double w)
+{
+    return w > 0 ? std::log(w) : -std::numeric_limits<double>::infinity();
+}
+
+static int eta(int r) { return r == 1 ? 1 : (r == 2 ? 3 : 4); }
+
+mp_status make_mesh_dev(const mp_mesh_desc& mesh, const mp_weno_options& weno,
+                        const PhaseHost& phase, MeshDev& out)
+{
+    MP_CALL(validate_mesh(mesh));
+    MP_CALL(validate_weno(weno));
+    MeshDev m;
+    std::memset(&m, 0, sizeof m);
+    m.nx = mesh.nx;
+    m.ny = mesh.ny;
+    m.q = mesh.q;
+    m.lf_mode = weno.lf_mode;
+    m.global_speed = weno.global_speed;
+    m.dx = (mesh.x1 - mesh.x0) / mesh.nx;
+    m.dy = (mesh.y1 - mesh.y0) / mesh.ny;
+    m.area = m.dx * m.dy;
+    if (!(m.area > 0) || !std::isfinite(m.area)) return fail(MP_ERR_FP, "Cell area underflowed to zero");
+    m.inv_area = 1.0 / m.area;
+    // h = sqrt(area) (src/couple/state_reconstruction.cpp:74)
+    const long double h = sqrtl((long double)m.area);
+    const double aw = (double)(m.dx / h), bw = (double)(m.dy / h);
+    m.aw = aw;
+    m.bw = bw;
+    m.hx = 0.5 * aw;
+    m.hy = 0.5 * bw;
+    m.kx1 = (double)(1.0L / (2.0L * aw));
+    m.kx2 = (double)(1.0L / (2.0L * aw * aw));
+    m.kx0 = (double)((long double)aw * aw / 12.0L);
+    m.ky1 = (double)(1.0L / (2.0L * bw));
+    m.ky2 = (double)(1.0L / (2.0L * bw * bw));
+    m.ky0 = (double)((long double)bw * bw / 12.0L);
+    m.inv_aw = (double)(1.0L / aw);
+    m.inv_bw = (double)(1.0L / bw);
+    m.inv_awbw = (double)(1.0L / ((long double)aw * bw));
+    m.q11w = (double)(((long double)aw * aw + (long double)bw * bw) / 12.0L);
+    m.epsA = weno.epsilon * m.area;
+    if (!(m.epsA > 0)) return fail(MP_ERR_FP, "epsilon*area underflows to zero");
+    // exponents s*r + eta(r): r = order + 1 (src/mlweno/reconstruction.cpp:43-44, 225-226)
+    m.pL = double(weno.s * 3 + eta(3));
+    m.pS = double(weno.s * 2 + eta(2));
+    m.pC = double(weno.s * 1 + eta(1));
+    m.ldL = log_weight(weno.large_weight);
+    for (int k = 0; k < 4; ++k) m.ldS[k] = log_weight(weno.small_weight[k]);
+    m.use_constant = (weno.use_constant && weno.constant_linear_weight > 0) ? 1 : 0;
+    m.ldC = log_weight(weno.constant_linear_weight);
+    double xq[kMaxQ], wq[kMaxQ];
+    gauss_legendre(mesh.q, xq, wq);
+    for (int q = 0; q < mesh.q; ++q) {
+        m.sv[q] = m.hy * xq[q];
+        m.sh[q] = m.hx * xq[q];
+        m.wv[q] = wq[q] * m.dy / 2;
+        m.wh[q] = wq[q] * m.dx / 2;
+    }
+    m.ph = phase.dev;
+    out = m;
+    return MP_OK;
+}
+
+mp_status DeviceBuffer::alloc(size_t n)
+{
+    reset();
+    if (n == 0) return MP_OK;
+    hipError_t e = hipMalloc(&ptr, n);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return fail(MP_ERR_MEMORY, "hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+    }
+    bytes = n;
+    return MP_OK;
+}
+
+void DeviceBuffer::reset()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+bool all_finite(const double* v, size_t n)
+{
+    bool ok = true;
+    for (size_t k = 0; k < n; ++k) ok &= std::isfinite(v[k]);
+    return ok;
+}
+
+}  // namespace mp

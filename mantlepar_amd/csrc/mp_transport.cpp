@@ -1,0 +1,678 @@
+// Single-mesh transport handle and the C ABI of include/mantlepar_amd.h.
+//
+// The handle owns all device buffers of one rank's share of the mesh: three
+// full H/C states (the state at the start of a call is never overwritten, so a
+// failed call leaves it intact), two stage buffers, face velocities, boundary
+// data, two phase-output sets and the reduction scratch.  A call to
+// mp_transport_step launches the stage kernels asynchronously, synchronises
+// once, and commits the new state only if the device error word is clean,
+// which mirrors the reference's collective Agree after every step
+// (src/couple/evolve.cpp:23-28, 493-499).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <vector>
+
+#include "mp_host.h"
+#include "mp_launch.h"
+#include "mp_transport.h"
+
+using namespace mp;
+
+namespace mp {
+const char* last_error();
+}
+
+// Stage tables: u_out = A*u_n + B*(u_k + dt*L(u_k)); W weights the stage
+// boundary fluxes in the step's conservation balance.
+static const double kA[3][3] = {{0, 0, 0}, {0, 0.5, 0}, {0, 0.75, 1.0 / 3.0}};
+static const double kB[3][3] = {{1, 0, 0}, {1, 0.5, 0}, {1, 0.25, 2.0 / 3.0}};
+static const double kW[3][3] = {{1, 0, 0}, {0.5, 0.5, 0}, {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0}};
+
+mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
+                             const mp_phase_params& p, int32_t prec, void* s)
+{
+    if (prec != MP_PREC_FP64 && prec != MP_PREC_MIXED) return fail(MP_ERR_ARG, "unknown precision");
+    MP_CALL(make_phase(p, phase));
+    MP_CALL(make_mesh_dev(m, w, phase, md));
+    mesh = m;
+    weno = w;
+    phase_params = p;
+    precision = prec;
+    N = size_t(m.nx) * size_t(m.ny);
+    if (s) {
+        stream = static_cast<hipStream_t>(s);
+        own_stream = false;
+    } else {
+        MP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        own_stream = true;
+    }
+    for (int k = 0; k < 5; ++k) {
+        MP_CALL(H[k].alloc(N * sizeof(double)));
+        MP_CALL(C[k].alloc(N * sizeof(double)));
+    }
+    for (int k = 0; k < 2; ++k) {
+        MP_CALL(phase_buf[k].alloc(N * (4 * sizeof(double) + 1)));
+    }
+    MP_CALL(vnx.alloc(size_t(m.q) * m.ny * (m.nx + 1) * sizeof(double)));
+    MP_CALL(vny.alloc(size_t(m.q) * (m.ny + 1) * m.nx * sizeof(double)));
+    MP_CALL(error.alloc(sizeof(uint64_t) * 2));
+    MP_CALL(d_md.alloc(sizeof(MeshDev)));
+    MP_HIP(hipMemsetAsync(H[0].ptr, 0, N * sizeof(double), stream));
+    MP_HIP(hipMemsetAsync(C[0].ptr, 0, N * sizeof(double), stream));
+    md.vnx = vnx.as<double>();
+    md.vny = vny.as<double>();
+    MP_HIP(hipMemsetAsync(vnx.ptr, 0, vnx.bytes, stream));
+    MP_HIP(hipMemsetAsync(vny.ptr, 0, vny.bytes, stream));
+    MP_CALL(sync_md());
+    set_tile(64);
+    return MP_OK;
+}
+
+mp_transport::~mp_transport()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+mp_status mp_transport::sync_md()
+{
+    MP_HIP(hipMemcpyAsync(d_md.ptr, &md, sizeof(MeshDev), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+void mp_transport::set_tile(int r)
+{
+    rows = r < 4 ? 4 : r;
+    strips = (mesh.nx + kStripCells - 1) / kStripCells;
+    bands = (mesh.ny + rows - 1) / rows;
+    nitems = strips * bands;
+}
+
+mp_status mp_transport::ensure_scratch(int nsteps)
+{
+    const size_t pbytes = size_t(3) * nitems * kPartials * sizeof(double);
+    if (partials.bytes < pbytes) MP_CALL(partials.alloc(pbytes));
+    const size_t tbytes = size_t(nsteps > 0 ? nsteps : 1) * 4 * sizeof(double);
+    if (totals.bytes < tbytes) MP_CALL(totals.alloc(tbytes));
+    return MP_OK;
+}
+
+mp_status mp_transport::upload(const double* h, const double* c)
+{
+    if (!h || !c) return fail(MP_ERR_ARG, "null state array");
+    if (!all_finite(h, N) || !all_finite(c, N)) return fail(MP_ERR_ARG, "Cell averages must be finite");
+    MP_HIP(hipMemcpyAsync(H[cur].ptr, h, N * sizeof(double), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(C[cur].ptr, c, N * sizeof(double), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    phase_valid = false;
+    return MP_OK;
+}
+
+mp_status mp_transport::download(double* h, double* c)
+{
+    if (h) MP_HIP(hipMemcpyAsync(h, H[cur].ptr, N * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (c) MP_HIP(hipMemcpyAsync(c, C[cur].ptr, N * sizeof(double), hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+mp_status mp_transport::set_velocity(const double* x, const double* y)
+{
+    if (!x || !y) return fail(MP_ERR_ARG, "null velocity array");
+    const size_t nxv = size_t(mesh.q) * mesh.ny * (mesh.nx + 1);
+    const size_t nyv = size_t(mesh.q) * (mesh.ny + 1) * mesh.nx;
+    if (!all_finite(x, nxv) || !all_finite(y, nyv)) return fail(MP_ERR_ARG, "Edge velocity must be finite");
+    if (weno.lf_mode == 1) {
+        double vmax = 0;
+        for (size_t k = 0; k < nxv; ++k) vmax = std::fmax(vmax, std::fabs(x[k]));
+        for (size_t k = 0; k < nyv; ++k) vmax = std::fmax(vmax, std::fabs(y[k]));
+        if (weno.global_speed + 1e-14 < vmax)
+            return fail(MP_ERR_ARG, "Global LF speed is smaller than the flow normal speed");
+    }
+    MP_HIP(hipMemcpyAsync(vnx.ptr, x, nxv * sizeof(double), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(vny.ptr, y, nyv * sizeof(double), hipMemcpyHostToDevice, stream));
+    MP_CALL(compute_rate());
+    have_velocity = true;
+    return MP_OK;
+}
+
+mp_status mp_transport::compute_rate()
+{
+    unsigned long long* bits = reinterpret_cast<unsigned long long*>(error.as<uint64_t>() + 1);
+    MP_HIP(hipMemsetAsync(bits, 0, sizeof(unsigned long long), stream));
+    MP_HIP(launch_rate(d_md.as<MeshDev>(), int(N), bits, stream));
+    unsigned long long host = 0;
+    MP_HIP(hipMemcpyAsync(&host, bits, sizeof host, hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    std::memcpy(&max_rate, &host, sizeof max_rate);
+    return MP_OK;
+}
+
+mp_status mp_transport::set_boundary(const uint8_t* const side[4])
+{
+    const size_t len[4] = {size_t(mesh.nx), size_t(mesh.ny), size_t(mesh.nx), size_t(mesh.ny)};
+    for (int s = 0; s < 4; ++s) {
+        if (side[s])
+            for (size_t k = 0; k < len[s]; ++k)
+                if (side[s][k] > MP_BC_INFLOW_OUTFLOW) return fail(MP_ERR_ARG, "Unknown advection boundary policy");
+    }
+    for (int s = 0; s < 4; ++s) {
+        if (!side[s]) {
+            bc[s].reset();
+            md.bc[s] = nullptr;
+            continue;
+        }
+        MP_CALL(bc[s].alloc(len[s]));
+        MP_HIP(hipMemcpyAsync(bc[s].ptr, side[s], len[s], hipMemcpyHostToDevice, stream));
+        md.bc[s] = bc[s].as<uint8_t>();
+    }
+    return sync_md();
+}
+
+mp_status mp_transport::set_inflow(const double* h, const double* c)
+{
+    const size_t total = size_t(2) * (mesh.nx + mesh.ny) * mesh.q;
+    const size_t off[4] = {0, size_t(mesh.nx) * mesh.q, size_t(mesh.nx + mesh.ny) * mesh.q,
+                           size_t(2 * mesh.nx + mesh.ny) * mesh.q};
+    for (int f = 0; f < 2; ++f) {
+        const double* src = f ? c : h;
+        DeviceBuffer& buf = f ? inflowC : inflowH;
+        const double** dst = f ? md.inflowC : md.inflowH;
+        if (!src) {
+            buf.reset();
+            for (int s = 0; s < 4; ++s) dst[s] = nullptr;
+            continue;
+        }
+        if (!all_finite(src, total)) return fail(MP_ERR_FP, "Inflow boundary callback must return a finite state");
+        MP_CALL(buf.alloc(total * sizeof(double)));
+        MP_HIP(hipMemcpyAsync(buf.ptr, src, total * sizeof(double), hipMemcpyHostToDevice, stream));
+        for (int s = 0; s < 4; ++s) dst[s] = buf.as<double>() + off[s];
+    }
+    return sync_md();
+}
+
+StagePtrs mp_transport::stage_ptrs(int src, int kin, int out, int scheme, int st, double dt,
+                                   bool phase_stage, int phase_set, int part_stage) const
+{
+    StagePtrs sp;
+    std::memset(&sp, 0, sizeof sp);
+    sp.Hn = H[src].as<double>();
+    sp.Cn = C[src].as<double>();
+    sp.Hk = H[kin].as<double>();
+    sp.Ck = C[kin].as<double>();
+    sp.Ho = H[out].as<double>();
+    sp.Co = C[out].as<double>();
+    sp.a = kA[scheme - 1][st];
+    sp.b = kB[scheme - 1][st];
+    sp.dt = dt;
+    if (phase_stage) {
+        double* base = phase_buf[phase_set].as<double>();
+        sp.T = base;
+        sp.phi = base + N;
+        sp.Cl = base + 2 * N;
+        sp.Cs = base + 3 * N;
+        sp.region = reinterpret_cast<uint8_t*>(base + 4 * N);
+    }
+    sp.partials = partials.as<double>() + size_t(part_stage) * nitems * kPartials;
+    sp.error = reinterpret_cast<uint32_t*>(error.ptr);
+    sp.rows = rows;
+    sp.strips = strips;
+    return sp;
+}
+
+// Enqueue nsteps; returns the index of the final state buffer.
+mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& final_state,
+                                      std::vector<hipEvent_t>* events)
+{
+    int others[2], n = 0;
+    for (int k = 0; k < 3; ++k)
+        if (k != cur) others[n++] = k;
+    int src = cur;
+    final_state = cur;
+    const int phase_set = 1 - phase_cur;
+    for (int s = 0; s < nsteps; ++s) {
+        const int dst = others[s % 2];
+        int kin = src;
+        for (int st = 0; st < scheme; ++st) {
+            const bool last = st == scheme - 1;
+            const int out = last ? dst : 3 + st;
+            const StagePtrs sp = stage_ptrs(src, kin, out, scheme, st, dt, last, phase_set, st);
+            if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
+            MP_HIP(launch_stage(precision, last, d_md.as<MeshDev>(), sp, nitems, stream));
+            if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
+            kin = out;
+        }
+        MP_HIP(launch_reduce_step(partials.as<double>(), nitems, scheme, kW[scheme - 1], md.area,
+                                  totals.as<double>() + size_t(s) * 4, stream));
+        src = dst;
+        final_state = dst;
+    }
+    return MP_OK;
+}
+
+mp_status mp_transport::check_step_args(int nsteps, int scheme, double dt)
+{
+    if (!have_velocity) return fail(MP_ERR_STATE, "set face velocities before stepping");
+    if (scheme < 1 || scheme > 3) return fail(MP_ERR_ARG, "scheme must be 1 (FE), 2 (SSPRK2) or 3 (SSPRK3)");
+    if (nsteps < 0) return fail(MP_ERR_ARG, "nsteps must be nonnegative");
+    if (!std::isfinite(dt) || !(dt > 0)) return fail(MP_ERR_ARG, "Time step does not advance finite time");
+    return MP_OK;
+}
+
+mp_status mp_transport::step(int nsteps, int scheme, double dt, mp_step_report* report)
+{
+    MP_CALL(check_step_args(nsteps, scheme, dt));
+    MP_CALL(ensure_scratch(nsteps));
+    MP_HIP(hipMemsetAsync(error.ptr, 0, sizeof(uint32_t), stream));
+    int final_state = cur;
+    MP_CALL(enqueue_steps(nsteps, scheme, dt, final_state, nullptr));
+    uint32_t bits = 0;
+    MP_HIP(hipMemcpyAsync(&bits, error.ptr, sizeof bits, hipMemcpyDeviceToHost, stream));
+    std::vector<double> tot;
+    if (report && nsteps > 0) {
+        tot.resize(size_t(nsteps) * 4);
+        MP_HIP(hipMemcpyAsync(tot.data(), totals.ptr, tot.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+    MP_HIP(hipStreamSynchronize(stream));
+    if (bits) return fail(status_from_bits(bits), "%s (device error bits 0x%x); state unchanged",
+                          describe_bits(bits), bits);
+    if (nsteps > 0) {
+        cur = final_state;
+        phase_cur = 1 - phase_cur;
+        phase_valid = true;
+    }
+    if (report) {
+        std::memset(report, 0, sizeof *report);
+        report->dt = dt;
+        report->max_rate = max_rate;
+        report->courant = dt * max_rate;
+        report->steps_done = nsteps;
+        long double bh = 0, bc = 0;
+        for (int s = 0; s < nsteps; ++s) {
+            bh += (long double)dt * tot[size_t(s) * 4 + 2];
+            bc += (long double)dt * tot[size_t(s) * 4 + 3];
+        }
+        report->boundary_H = double(bh);
+        report->boundary_C = double(bc);
+        if (nsteps > 0) {
+            report->total_H = tot[size_t(nsteps - 1) * 4 + 0];
+            report->total_C = tot[size_t(nsteps - 1) * 4 + 1];
+        } else {
+            MP_CALL(state_totals(report->total_H, report->total_C));
+        }
+    }
+    return MP_OK;
+}
+
+mp_status mp_transport::state_totals(double& th, double& tc)
+{
+    std::vector<double> h(N), c(N);
+    MP_CALL(download(h.data(), c.data()));
+    double s[2] = {0, 0}, e[2] = {0, 0};
+    for (int f = 0; f < 2; ++f) {
+        const std::vector<double>& u = f ? c : h;
+        for (size_t k = 0; k < N; ++k) {
+            const double y = s[f] + u[k];
+            e[f] += std::fabs(s[f]) >= std::fabs(u[k]) ? (s[f] - y) + u[k] : (u[k] - y) + s[f];
+            s[f] = y;
+        }
+    }
+    th = (s[0] + e[0]) * md.area;
+    tc = (s[1] + e[1]) * md.area;
+    return MP_OK;
+}
+
+mp_status mp_transport::profile(int nsteps, int scheme, double dt, double* ms_total,
+                                double* ms_stage, double* ms_phase_stage)
+{
+    MP_CALL(check_step_args(nsteps, scheme, dt));
+    if (nsteps < 1) return fail(MP_ERR_ARG, "profile needs at least one step");
+    MP_CALL(ensure_scratch(nsteps));
+    std::vector<hipEvent_t> ev(size_t(nsteps) * 6);
+    for (auto& e : ev) MP_HIP(hipEventCreate(&e));
+    hipEvent_t t0, t1;
+    MP_HIP(hipEventCreate(&t0));
+    MP_HIP(hipEventCreate(&t1));
+    MP_HIP(hipMemsetAsync(error.ptr, 0, sizeof(uint32_t), stream));
+    int final_state = cur;
+    MP_HIP(hipEventRecord(t0, stream));
+    mp_status st = enqueue_steps(nsteps, scheme, dt, final_state, &ev);
+    MP_HIP(hipEventRecord(t1, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    double plain = 0, fused = 0;
+    int np = 0, nf = 0;
+    for (int s = 0; s < nsteps && st == MP_OK; ++s)
+        for (int k = 0; k < scheme; ++k) {
+            float ms = 0;
+            MP_HIP(hipEventElapsedTime(&ms, ev[2 * (s * 3 + k)], ev[2 * (s * 3 + k) + 1]));
+            if (k == scheme - 1) { fused += ms; ++nf; } else { plain += ms; ++np; }
+        }
+    float total = 0;
+    MP_HIP(hipEventElapsedTime(&total, t0, t1));
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    MP_CALL(st);
+    if (ms_total) *ms_total = total;
+    if (ms_stage) *ms_stage = np ? plain / np : fused / nf;
+    if (ms_phase_stage) *ms_phase_stage = fused / nf;
+    return MP_OK;
+}
+
+mp_status mp_transport::phase_out(double* T, double* phi, double* Cl, double* Cs, uint8_t* region)
+{
+    if (!phase_valid) {
+        double* base = phase_buf[phase_cur].as<double>();
+        MP_HIP(hipMemsetAsync(error.ptr, 0, sizeof(uint32_t), stream));
+        MP_HIP(launch_state_phase(d_md.as<MeshDev>(), int(N), H[cur].as<double>(), C[cur].as<double>(),
+                                  base, base + N, base + 2 * N, base + 3 * N,
+                                  reinterpret_cast<uint8_t*>(base + 4 * N),
+                                  reinterpret_cast<uint32_t*>(error.ptr), stream));
+        uint32_t bits = 0;
+        MP_HIP(hipMemcpyAsync(&bits, error.ptr, sizeof bits, hipMemcpyDeviceToHost, stream));
+        MP_HIP(hipStreamSynchronize(stream));
+        if (bits) return fail(status_from_bits(bits), "%s (device error bits 0x%x)", describe_bits(bits), bits);
+        phase_valid = true;
+    }
+    const double* base = phase_buf[phase_cur].as<double>();
+    double* outs[4] = {T, phi, Cl, Cs};
+    for (int k = 0; k < 4; ++k)
+        if (outs[k]) MP_HIP(hipMemcpyAsync(outs[k], base + k * N, N * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (region) MP_HIP(hipMemcpyAsync(region, base + 4 * N, N, hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+mp_status mp_transport::rhs(double* rh, double* rc, double* boundary)
+{
+    if (!have_velocity) return fail(MP_ERR_STATE, "set face velocities before evaluating the RHS");
+    DeviceBuffer buf;
+    MP_CALL(buf.alloc(N * 4 * sizeof(double)));
+    double* d = buf.as<double>();
+    MP_HIP(hipMemsetAsync(error.ptr, 0, sizeof(uint32_t), stream));
+    MP_HIP(launch_rhs_diag(precision, d_md.as<MeshDev>(), int(N), H[cur].as<double>(), C[cur].as<double>(),
+                           d, d + N, d + 2 * N, reinterpret_cast<uint32_t*>(error.ptr), stream));
+    std::vector<double> host(N * 4);
+    uint32_t bits = 0;
+    MP_HIP(hipMemcpyAsync(host.data(), d, N * 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipMemcpyAsync(&bits, error.ptr, sizeof bits, hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    if (bits) return fail(status_from_bits(bits), "%s (device error bits 0x%x)", describe_bits(bits), bits);
+    if (rh) std::memcpy(rh, host.data(), N * sizeof(double));
+    if (rc) std::memcpy(rc, host.data() + N, N * sizeof(double));
+    if (boundary) {
+        long double b[2] = {0, 0};
+        for (size_t k = 0; k < N; ++k) {
+            b[0] += host[2 * N + 2 * k];
+            b[1] += host[2 * N + 2 * k + 1];
+        }
+        boundary[0] = double(b[0]);
+        boundary[1] = double(b[1]);
+    }
+    return MP_OK;
+}
+
+mp_status mp_transport::reconstruct(int field, double* sigma, double* omega, double* traces)
+{
+    if (field != 0 && field != 1) return fail(MP_ERR_ARG, "field must be 0 (H) or 1 (C)");
+    const size_t nt = N * 4 * size_t(mesh.q);
+    DeviceBuffer buf;
+    MP_CALL(buf.alloc((N * 10 + nt) * sizeof(double)));
+    double* d = buf.as<double>();
+    MP_HIP(hipMemsetAsync(error.ptr, 0, sizeof(uint32_t), stream));
+    MP_HIP(launch_recon_diag(precision, d_md.as<MeshDev>(), int(N),
+                             field ? C[cur].as<double>() : H[cur].as<double>(), d, d + 5 * N,
+                             d + 10 * N, reinterpret_cast<uint32_t*>(error.ptr), stream));
+    std::vector<double> host(N * 10 + nt);
+    uint32_t bits = 0;
+    MP_HIP(hipMemcpyAsync(host.data(), d, host.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipMemcpyAsync(&bits, error.ptr, sizeof bits, hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    if (bits) return fail(status_from_bits(bits), "%s (device error bits 0x%x)", describe_bits(bits), bits);
+    if (sigma) std::memcpy(sigma, host.data(), 5 * N * sizeof(double));
+    if (omega) std::memcpy(omega, host.data() + 5 * N, 5 * N * sizeof(double));
+    if (traces) std::memcpy(traces, host.data() + 10 * N, nt * sizeof(double));
+    return MP_OK;
+}
+
+// ===================================================================== C ABI
+
+#define MP_GUARD(...)                                                      \
+    try {                                                                  \
+        __VA_ARGS__                                                        \
+    } catch (const std::bad_alloc&) {                                      \
+        return fail(MP_ERR_MEMORY, "host allocation failed");              \
+    } catch (const std::exception& e) {                                    \
+        return fail(MP_ERR_DEVICE, "%s", e.what());                        \
+    }
+
+extern "C" {
+
+void mp_default_weno_options(mp_weno_options* o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->epsilon = 1e-2;
+    o->s = 1;
+    o->large_weight = 1.0;
+    for (int k = 0; k < 4; ++k) o->small_weight[k] = 1.0;
+    o->use_constant = 0;
+    o->constant_linear_weight = 1e-5;
+    o->lf_mode = 0;
+    o->global_speed = 0.0;
+}
+
+void mp_default_phase_params(mp_phase_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->Tm0 = 2000.0; p->Te0 = 1480.0; p->nu = 6.5e6; p->L = 5.0e5; p->cp = 1200.0; p->Xe = 0.25;
+    p->rho = 3000.0; p->rhor = 500.0; p->mus = 1.0e19; p->mul = 1.0; p->k0 = 1.0e-8; p->g = 10.0;
+    p->alpha0 = 3.0e-5;
+    p->pressure = 0.0;
+    p->hypereutectic = 0;
+}
+
+const char* mp_last_error(void) { return mp::last_error(); }
+const char* mp_version(void) { return "mantlepar_amd 0.1 (gfx950)"; }
+
+mp_status mp_transport_create(const mp_mesh_desc* mesh, const mp_weno_options* weno,
+                              const mp_phase_params* phase, int32_t precision, void* stream,
+                              mp_transport** out)
+{
+    MP_GUARD({
+        if (!mesh || !out) return fail(MP_ERR_ARG, "null argument");
+        mp_weno_options w;
+        mp_phase_params p;
+        mp_default_weno_options(&w);
+        mp_default_phase_params(&p);
+        if (weno) w = *weno;
+        if (phase) p = *phase;
+        mp_transport* t = new mp_transport();
+        mp_status s = t->init(*mesh, w, p, precision, stream);
+        if (s) { delete t; return s; }
+        *out = t;
+        return MP_OK;
+    })
+}
+
+mp_status mp_transport_destroy(mp_transport* t)
+{
+    delete t;
+    return MP_OK;
+}
+
+mp_status mp_transport_upload_state(mp_transport* t, const double* H, const double* C)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->upload(H, C); })
+}
+
+mp_status mp_transport_download_state(mp_transport* t, double* H, double* C)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->download(H, C); })
+}
+
+mp_status mp_transport_set_face_velocity(mp_transport* t, const double* vnx, const double* vny)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->set_velocity(vnx, vny); })
+}
+
+mp_status mp_transport_set_boundary(mp_transport* t, const uint8_t* bottom, const uint8_t* right,
+                                    const uint8_t* top, const uint8_t* left)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    const uint8_t* sides[4] = {bottom, right, top, left};
+    MP_GUARD({ return t->set_boundary(sides); })
+}
+
+mp_status mp_transport_set_inflow(mp_transport* t, const double* inflow_H, const double* inflow_C)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->set_inflow(inflow_H, inflow_C); })
+}
+
+mp_status mp_transport_stable_dt(mp_transport* t, double cfl, double* dt)
+{
+    if (!t || !dt) return fail(MP_ERR_ARG, "null argument");
+    if (!t->have_velocity) return fail(MP_ERR_STATE, "set face velocities first");
+    if (!std::isfinite(cfl) || !(cfl > 0)) return fail(MP_ERR_ARG, "CFL must be finite and positive");
+    if (!(t->max_rate > 0)) return fail(MP_ERR_STATE, "zero advective rate: the step is unbounded");
+    *dt = cfl / t->max_rate;
+    return MP_OK;
+}
+
+mp_status mp_transport_step(mp_transport* t, int32_t nsteps, int32_t scheme, double dt,
+                            mp_step_report* report)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->step(nsteps, scheme, dt, report); })
+}
+
+mp_status mp_transport_phase(mp_transport* t, double* T, double* phi, double* Cl, double* Cs,
+                             uint8_t* region)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->phase_out(T, phi, Cl, Cs, region); })
+}
+
+mp_status mp_transport_rhs(mp_transport* t, double* rhsH, double* rhsC, double* boundary)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->rhs(rhsH, rhsC, boundary); })
+}
+
+mp_status mp_transport_reconstruct(mp_transport* t, int32_t field, double* sigma, double* omega,
+                                   double* traces)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->reconstruct(field, sigma, omega, traces); })
+}
+
+mp_status mp_transport_device_state(mp_transport* t, double** H, double** C)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    if (H) *H = t->H[t->cur].as<double>();
+    if (C) *C = t->C[t->cur].as<double>();
+    return MP_OK;
+}
+
+mp_status mp_transport_synchronize(mp_transport* t)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_HIP(hipStreamSynchronize(t->stream));
+    return MP_OK;
+}
+
+mp_status mp_transport_profile(mp_transport* t, int32_t nsteps, int32_t scheme, double dt,
+                               double* ms_total, double* ms_stage, double* ms_phase_stage)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->profile(nsteps, scheme, dt, ms_total, ms_stage, ms_phase_stage); })
+}
+
+mp_status mp_transport_set_tile(mp_transport* t, int32_t rows_per_item)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    if (rows_per_item < 4 || rows_per_item > 1 << 20) return fail(MP_ERR_ARG, "rows per item must be >= 4");
+    t->set_tile(rows_per_item);
+    return MP_OK;
+}
+
+mp_status mp_phase_evaluate(const mp_phase_params* params, int64_t n, const double* H,
+                            const double* C, const double* P, double* T, double* phi, double* Cl,
+                            double* Cs, uint8_t* region, double* phi1, double* phi2, double* dTdC,
+                            double* dTdH)
+{
+    MP_GUARD({
+        if (!params || n < 0 || (n > 0 && (!H || !C))) return fail(MP_ERR_ARG, "null argument");
+        PhaseHost ph;
+        MP_CALL(make_phase(*params, ph));
+        if (n == 0) return MP_OK;
+        hipStream_t s;
+        MP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } guard{s};
+        const size_t nb = size_t(n) * sizeof(double);
+        DeviceBuffer in, out, err;
+        MP_CALL(in.alloc(nb * 3));
+        MP_CALL(out.alloc(nb * 8 + size_t(n)));
+        MP_CALL(err.alloc(sizeof(uint32_t)));
+        double* dH = in.as<double>();
+        double* dC = dH + n;
+        double* dP = P ? dC + n : nullptr;
+        MP_HIP(hipMemcpyAsync(dH, H, nb, hipMemcpyHostToDevice, s));
+        MP_HIP(hipMemcpyAsync(dC, C, nb, hipMemcpyHostToDevice, s));
+        if (P) MP_HIP(hipMemcpyAsync(dP, P, nb, hipMemcpyHostToDevice, s));
+        MP_HIP(hipMemsetAsync(err.ptr, 0, sizeof(uint32_t), s));
+        double* o = out.as<double>();
+        uint8_t* dreg = reinterpret_cast<uint8_t*>(o + 8 * n);
+        MP_HIP(launch_phase(ph.dev, ph.TDe0, ph.nu, ph.dT, n, dH, dC, dP, o, o + n, o + 2 * n,
+                            o + 3 * n, dreg, o + 4 * n, o + 5 * n, o + 6 * n, o + 7 * n,
+                            err.as<uint32_t>(), s));
+        uint32_t bits = 0;
+        MP_HIP(hipMemcpyAsync(&bits, err.ptr, sizeof bits, hipMemcpyDeviceToHost, s));
+        MP_HIP(hipStreamSynchronize(s));
+        if (bits) return fail(status_from_bits(bits), "%s (device error bits 0x%x)", describe_bits(bits), bits);
+        double* outs[8] = {T, phi, Cl, Cs, phi1, phi2, dTdC, dTdH};
+        for (int k = 0; k < 8; ++k)
+            if (outs[k]) MP_HIP(hipMemcpyAsync(outs[k], o + k * n, nb, hipMemcpyDeviceToHost, s));
+        if (region) MP_HIP(hipMemcpyAsync(region, dreg, size_t(n), hipMemcpyDeviceToHost, s));
+        MP_HIP(hipStreamSynchronize(s));
+        return MP_OK;
+    })
+}
+
+mp_status mp_phase_boundaries(const mp_phase_params* params, double C, double P, double* Tep,
+                              double* Tmp, double* H_eutectic_end, double* H_liquidus)
+{
+    // EutecticModel::boundaries, src/phase/eutectic_rescaled.cpp:163-185 (host, long double)
+    if (!params) return fail(MP_ERR_ARG, "null argument");
+    mp_phase_params p = *params;
+    p.pressure = P;
+    PhaseHost ph;
+    MP_CALL(make_phase(p, ph));
+    if (!std::isfinite(C) || C < 0.0 || C > p.Xe) return fail(MP_ERR_COMPOSITION, "CD must lie in [0, Xe]");
+    const long double ratio = (long double)C / p.Xe;
+    const long double heut = (long double)ph.dev.Tep + ph.dev.LD * ratio;
+    long double hliq = (long double)ph.dev.Tep + (1.0L - ratio) + ph.dev.LD;
+    if (C == 0.0) hliq = (long double)ph.dev.Tmp + ph.dev.LD;
+    if (!std::isfinite(heut) || !std::isfinite(hliq)) return fail(MP_ERR_FP, "boundary is not representable");
+    const double he = double(heut), hl = double(hliq);
+    if (ph.dev.Tmp <= ph.dev.Tep || hl <= ph.dev.Tep || (C == 0.0 && hl <= ph.dev.Tmp))
+        return fail(MP_ERR_FP, "Phase temperature/enthalpy intervals are not resolvable");
+    if (Tep) *Tep = ph.dev.Tep;
+    if (Tmp) *Tmp = ph.dev.Tmp;
+    if (H_eutectic_end) *H_eutectic_end = he;
+    if (H_liquidus) *H_liquidus = hl;
+    return MP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// The single-mesh handle behind mp_transport_* (defined at global scope to
+// match the opaque C type).
+#pragma once
+
+#include <vector>
+
+#include "mp_host.h"
+
+struct mp_transport {
+    mp_mesh_desc mesh{};
+    mp_weno_options weno{};
+    mp_phase_params phase_params{};
+    int32_t precision = MP_PREC_FP64;
+    mp::PhaseHost phase{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    mp::MeshDev md{};
+    mp::DeviceBuffer d_md;
+    size_t N = 0;
+    // 0..2: full states (the call's starting state is never written), 3..4: stage buffers
+    mp::DeviceBuffer H[5], C[5];
+    int cur = 0;
+    mp::DeviceBuffer vnx, vny;
+    bool have_velocity = false;
+    double max_rate = 0;
+    mp::DeviceBuffer bc[4], inflowH, inflowC;
+    // phase outputs: T, phi, Cl, Cs (N doubles each) then region (N bytes)
+    mp::DeviceBuffer phase_buf[2];
+    int phase_cur = 0;
+    bool phase_valid = false;
+    mp::DeviceBuffer partials, totals, error;
+    int rows = 64, strips = 1, bands = 1, nitems = 1;
+
+    mp_transport() = default;
+    mp_transport(const mp_transport&) = delete;
+    mp_transport& operator=(const mp_transport&) = delete;
+    ~mp_transport();
+
+    mp_status init(const mp_mesh_desc& m, const mp_weno_options& w, const mp_phase_params& p,
+                   int32_t prec, void* stream);
+    mp_status sync_md();
+    void set_tile(int rows_per_item);
+    mp_status ensure_scratch(int nsteps);
+    mp_status upload(const double* h, const double* c);
+    mp_status download(double* h, double* c);
+    mp_status set_velocity(const double* x, const double* y);
+    mp_status compute_rate();
+    mp_status set_boundary(const uint8_t* const side[4]);
+    mp_status set_inflow(const double* h, const double* c);
+    mp::StagePtrs stage_ptrs(int src, int kin, int out, int scheme, int st, double dt,
+                             bool phase_stage, int phase_set, int part_stage) const;
+    mp_status enqueue_steps(int nsteps, int scheme, double dt, int& final_state,
+                            std::vector<hipEvent_t>* events);
+    mp_status check_step_args(int nsteps, int scheme, double dt);
+    mp_status step(int nsteps, int scheme, double dt, mp_step_report* report);
+    mp_status state_totals(double& th, double& tc);
+    mp_status profile(int nsteps, int scheme, double dt, double* ms_total, double* ms_stage,
+                      double* ms_phase_stage);
+    mp_status phase_out(double* T, double* phi, double* Cl, double* Cs, uint8_t* region);
+    mp_status rhs(double* rh, double* rc, double* boundary);
+    mp_status reconstruct(int field, double* sigma, double* omega, double* traces);
+};

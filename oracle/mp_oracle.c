@@ -1,0 +1,1614 @@
+/*
+ * mp_oracle.c -- CPU restatement of MantlePar's H/C transport step.
+ *
+ * TEST INFRASTRUCTURE ONLY: correctness checker and timed CPU baseline for the
+ * HIP library.  Loaded only by tests/, the smoke check and bench.py's
+ * cpu_baseline leg.  Each block cites the reference file:line it restates.
+ *
+ * Parity pins (see tests/test_oracle_*.py):
+ *   phase                golden vectors written by the reference's own
+ *                        src/phase compiled unmodified (oracle/_ref), plus the
+ *                        closed forms of src/phase/test/test_eutectic.cpp.
+ *   gauss, geometry      polynomial exactness, closed forms.
+ *   stencil, weno        Q1/Q2 reproduction, analytic indicators, the
+ *                        direct-formula weights, extreme weights, boundary
+ *                        candidate counts (src/mlweno/test/test_reconstruction.cpp).
+ *   flux, operator       LLF closed form, constant-state preservation, global
+ *                        balance, brute force on tiny meshes.
+ *   SSP-RK3              stability polynomial and third-order convergence
+ *                        (not in the reference: it uses SSPRK2, evolve.cpp:469-479).
+ * The mlweno sources need PETSc and LAPACKE, which this image lacks, so they are
+ * unbuildable here; nothing of theirs is compiled.
+ */
+#include "mp_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static _Thread_local char g_error[256];
+
+const char* mpo_last_error(void) { return g_error; }
+
+static int fail(int code, const char* message)
+{
+    snprintf(g_error, sizeof g_error, "%s", message);
+    return code;
+}
+
+#define CHECK(cond, code, msg) do { if (!(cond)) return fail((code), (msg)); } while (0)
+#define CALL(expr) do { int e_ = (expr); if (e_) return e_; } while (0)
+
+static int finite_d(double v) { return isfinite(v); }
+
+/* =============================================================== geometry */
+
+/* Gauss-Legendre points on [-1,1], ascending.  The reference obtains the rule
+ * from PetscDTGaussQuadrature (src/core/integral.cpp:7-20); this is Newton's
+ * method on P_n, exact to rounding. */
+int mpo_gauss_rule(int n, double* points, double* weights)
+{
+    CHECK(n >= 1 && n <= 64, MPO_ERR_ARG, "Gauss quadrature requires 1..64 points");
+    for (int k = 0; k < n; ++k) {
+        long double x = cosl(3.14159265358979323846264338327950288L * (k + 0.75L) / (n + 0.5L));
+        long double dp = 0;
+        for (int it = 0; it < 100; ++it) {
+            long double p0 = 1, p1 = x;
+            for (int m = 2; m <= n; ++m) {
+                long double p2 = ((2 * m - 1) * x * p1 - (m - 1) * p0) / m;
+                p0 = p1; p1 = p2;
+            }
+            if (n == 1) { p1 = x; p0 = 1; }
+            dp = n * (x * p1 - p0) / (x * x - 1);
+            long double dx = p1 / dp;
+            x -= dx;
+            if (fabsl(dx) < 1e-19L) break;
+        }
+        {
+            long double p0 = 1, p1 = x;
+            for (int m = 2; m <= n; ++m) {
+                long double p2 = ((2 * m - 1) * x * p1 - (m - 1) * p0) / m;
+                p0 = p1; p1 = p2;
+            }
+            dp = n * (x * p1 - p0) / (x * x - 1);
+        }
+        /* descending x for ascending k; store ascending */
+        points[n - 1 - k] = (double)x;
+        weights[n - 1 - k] = (double)(2.0L / ((1 - x * x) * dp * dp));
+    }
+    if (n % 2 == 1) points[n / 2] = 0.0;
+    for (int k = 0; k < n / 2; ++k) {   /* exact symmetry */
+        double a = 0.5 * (points[n - 1 - k] - points[k]);
+        points[k] = -a; points[n - 1 - k] = a;
+        double w = 0.5 * (weights[k] + weights[n - 1 - k]);
+        weights[k] = w; weights[n - 1 - k] = w;
+    }
+    return MPO_OK;
+}
+
+/* src/core/integral.cpp:91-97 */
+mpo_point mpo_map_edge_point(double ref, const mpo_point edge[2])
+{
+    const double a = 0.5 * (1.0 - ref), b = 0.5 * (1.0 + ref);
+    mpo_point r = {{a * edge[0].p[0] + b * edge[1].p[0], a * edge[0].p[1] + b * edge[1].p[1]}};
+    return r;
+}
+
+/* src/core/integral.cpp:99-106 */
+mpo_point mpo_map_cell_point(mpo_point ref, const mpo_point c[4])
+{
+    const mpo_point lower[2] = {c[0], c[1]}, upper[2] = {c[3], c[2]};
+    const mpo_point vertical[2] = {mpo_map_edge_point(ref.p[0], lower),
+                                   mpo_map_edge_point(ref.p[0], upper)};
+    return mpo_map_edge_point(ref.p[1], vertical);
+}
+
+/* src/core/integral.cpp:108-121 */
+double mpo_cell_jacobian(mpo_point ref, const mpo_point c[4])
+{
+    const double s = 0.5 * (1.0 + ref.p[0]), t = 0.5 * (1.0 + ref.p[1]);
+    double dxi[2], deta[2];
+    for (int d = 0; d < 2; ++d) {
+        dxi[d] = 0.5 * ((1.0 - t) * (c[1].p[d] - c[0].p[d]) + t * (c[2].p[d] - c[3].p[d]));
+        deta[d] = 0.5 * ((1.0 - s) * (c[3].p[d] - c[0].p[d]) + s * (c[2].p[d] - c[1].p[d]));
+    }
+    return dxi[0] * deta[1] - dxi[1] * deta[0];
+}
+
+/* src/core/integral.cpp:62-89 */
+int mpo_edge_geometry(const mpo_point e[2], double* length, mpo_point* normal)
+{
+    for (int k = 0; k < 2; ++k)
+        CHECK(finite_d(e[k].p[0]) && finite_d(e[k].p[1]), MPO_ERR_ARG, "Edge coordinates must be finite");
+    const double dx = e[1].p[0] - e[0].p[0], dy = e[1].p[1] - e[0].p[1];
+    CHECK(finite_d(dx) && finite_d(dy), MPO_ERR_ARG, "Edge displacement must be finite");
+    const double scale = fmax(fabs(dx), fabs(dy));
+    CHECK(scale > 0.0, MPO_ERR_ARG, "Edge endpoints must be distinct");
+    const double ux = dx / scale, uy = dy / scale;
+    const double norm = sqrt(ux * ux + uy * uy);
+    const double len = scale * norm;
+    CHECK(finite_d(len) && len > 0.0, MPO_ERR_ARG, "Edge length must be finite and positive");
+    *length = len;
+    normal->p[0] = uy / norm;
+    normal->p[1] = -ux / norm;
+    return MPO_OK;
+}
+
+/* src/core/quad_validation.h:15-46 */
+int mpo_validate_quad(const mpo_point c[4])
+{
+    double xmin = c[0].p[0], xmax = xmin, ymin = c[0].p[1], ymax = ymin;
+    for (int k = 0; k < 4; ++k) {
+        CHECK(finite_d(c[k].p[0]) && finite_d(c[k].p[1]), MPO_ERR_ARG,
+              "Quadrilateral coordinates must be finite");
+        xmin = fmin(xmin, c[k].p[0]); xmax = fmax(xmax, c[k].p[0]);
+        ymin = fmin(ymin, c[k].p[1]); ymax = fmax(ymax, c[k].p[1]);
+    }
+    const double sx = xmax - xmin, sy = ymax - ymin;
+    CHECK(finite_d(sx) && finite_d(sy) && sx > 0.0 && sy > 0.0, MPO_ERR_ARG,
+          "Quadrilateral extents must be finite and positive");
+    const double tol = 64.0 * DBL_EPSILON;
+    for (int k = 0; k < 4; ++k) {
+        const mpo_point* p = &c[k]; const mpo_point* nx = &c[(k + 1) % 4];
+        const mpo_point* pv = &c[(k + 3) % 4];
+        const double ux = (nx->p[0] - p->p[0]) / sx, uy = (nx->p[1] - p->p[1]) / sy;
+        const double vx = (pv->p[0] - p->p[0]) / sx, vy = (pv->p[1] - p->p[1]) / sy;
+        CHECK(ux * vy - uy * vx > tol, MPO_ERR_ARG,
+              "Quadrilateral must be counterclockwise, convex and nonsingular");
+    }
+    return MPO_OK;
+}
+
+typedef struct { int n; double x[16], w[16]; } rule1d;
+
+static int make_rule(int n, rule1d* r)
+{
+    r->n = n;
+    return mpo_gauss_rule(n, r->x, r->w);
+}
+
+typedef double (*cell_fn)(mpo_point p, const void* ctx);
+
+/* IntegrateCell, src/core/integral.h:49-78 (double accumulation, j outer). */
+static int integrate_cell(const mpo_point c[4], const rule1d* r, cell_fn f, const void* ctx,
+                          double* out)
+{
+    CALL(mpo_validate_quad(c));
+    double sum = 0.0;
+    for (int j = 0; j < r->n; ++j)
+        for (int i = 0; i < r->n; ++i) {
+            mpo_point ref = {{r->x[i], r->x[j]}};
+            mpo_point p = mpo_map_cell_point(ref, c);
+            double J = mpo_cell_jacobian(ref, c);
+            CHECK(finite_d(p.p[0]) && finite_d(p.p[1]) && finite_d(J) && J > 0.0, MPO_ERR_FP,
+                  "Cell mapping produced an invalid point or Jacobian");
+            double v = f(p, ctx);
+            CHECK(finite_d(v), MPO_ERR_FP, "Cell integrand returned a non-finite value");
+            sum += r->w[i] * r->w[j] * J * v;
+        }
+    CHECK(finite_d(sum), MPO_ERR_FP, "Cell integral is not finite");
+    *out = sum;
+    return MPO_OK;
+}
+
+static double fn_one(mpo_point p, const void* ctx) { (void)p; (void)ctx; return 1.0; }
+
+typedef struct { int i, j; } mono_ctx;
+
+static double power_int(double x, int d)
+{
+    double r = 1;
+    for (int k = 0; k < d; ++k) r *= x;
+    return r;
+}
+
+static double fn_monomial(mpo_point p, const void* ctx)
+{
+    const mono_ctx* m = (const mono_ctx*)ctx;
+    return power_int(p.p[0], m->i) * power_int(p.p[1], m->j);
+}
+
+/* ====================================================== TensorStencilPoly */
+
+/* src/mlweno/tensorstencilpoly.cpp:51-57 */
+static double falling(int degree, int derivative)
+{
+    double r = 1;
+    for (int k = 0; k < derivative; ++k) r *= (double)(degree - k);
+    return r;
+}
+
+/* Horner in normalized coordinates, src/mlweno/tensorstencilpoly.cpp:61-74 */
+static double normalized_derivative(const double* c, int si, int sj, double x, double y,
+                                    int dx, int dy)
+{
+    if (dx >= si || dy >= sj) return 0;
+    double result = 0;
+    for (int j = sj; j-- > dy;) {
+        double row = 0;
+        for (int i = si; i-- > dx;) row = row * x + c[j * si + i] * falling(i, dx);
+        result = result * y + row * falling(j, dy);
+    }
+    return result;
+}
+
+static int normalize_quad(const mpo_point c[4], mpo_point center, double h, mpo_point out[4])
+{
+    mpo_point w[4];
+    for (int k = 0; k < 4; ++k)
+        for (int d = 0; d < 2; ++d) w[k].p[d] = (c[k].p[d] - center.p[d]) / h;
+    CALL(mpo_validate_quad(w));
+    memcpy(out, w, sizeof w);
+    return MPO_OK;
+}
+
+/* Dense LU with partial pivoting and solve, the dgesv contract used at
+ * src/mlweno/tensorstencilpoly.cpp:209-215 (row-major, B overwritten by X). */
+static int dense_solve(int n, double* A, double* B)
+{
+    int piv[MPO_MAX_STENCIL];
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = fabs(A[k * n + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(A[i * n + k]) > best) { best = fabs(A[i * n + k]); p = i; }
+        piv[k] = p;
+        if (A[p * n + k] == 0.0) return fail(MPO_ERR_LIB, "Stencil moment solve failed (singular)");
+        if (p != k)
+            for (int j = 0; j < n; ++j) {
+                double t = A[k * n + j]; A[k * n + j] = A[p * n + j]; A[p * n + j] = t;
+            }
+        const double inv = 1.0 / A[k * n + k];
+        for (int i = k + 1; i < n; ++i) {
+            A[i * n + k] *= inv;
+            const double l = A[i * n + k];
+            for (int j = k + 1; j < n; ++j) A[i * n + j] -= l * A[k * n + j];
+        }
+    }
+    for (int k = 0; k < n; ++k)       /* row interchanges of B */
+        if (piv[k] != k)
+            for (int j = 0; j < n; ++j) {
+                double t = B[k * n + j]; B[k * n + j] = B[piv[k] * n + j]; B[piv[k] * n + j] = t;
+            }
+    for (int c = 0; c < n; ++c) {
+        for (int i = 0; i < n; ++i) {           /* L y = b (unit lower) */
+            double s = B[i * n + c];
+            for (int k = 0; k < i; ++k) s -= A[i * n + k] * B[k * n + c];
+            B[i * n + c] = s;
+        }
+        for (int i = n - 1; i >= 0; --i) {      /* U x = y */
+            double s = B[i * n + c];
+            for (int k = i + 1; k < n; ++k) s -= A[i * n + k] * B[k * n + c];
+            B[i * n + c] = s / A[i * n + i];
+        }
+    }
+    return MPO_OK;
+}
+
+/* BuildSmoothness, src/mlweno/tensorstencilpoly.cpp:318-368 */
+static int build_smoothness(const mpo_stencil* s, const mpo_point nc[4], int gp, double* matrix)
+{
+    rule1d r;
+    CALL(make_rule(gp, &r));
+    double area = 0;
+    CALL(integrate_cell(nc, &r, fn_one, NULL, &area));
+    CHECK(area > 0, MPO_ERR_FP, "Smoothness integration area must be positive");
+    const int n = s->count;
+    double work[MPO_MAX_STENCIL * MPO_MAX_STENCIL];
+    double der[MPO_MAX_STENCIL];
+    memset(work, 0, sizeof(double) * n * n);
+    for (int qy = 0; qy < r.n; ++qy)
+        for (int qx = 0; qx < r.n; ++qx) {
+            mpo_point ref = {{r.x[qx], r.x[qy]}};
+            mpo_point p = mpo_map_cell_point(ref, nc);
+            const double weight = r.w[qx] * r.w[qy] * (mpo_cell_jacobian(ref, nc) / area);
+            CHECK(finite_d(weight) && weight > 0, MPO_ERR_FP, "Invalid smoothness quadrature weight");
+            for (int dy = 0; dy < s->size_j; ++dy)
+                for (int dx = 0; dx < s->size_i; ++dx) {
+                    if (dx + dy == 0 || dx + dy > s->order) continue;
+                    for (int c = 0; c < n; ++c) {
+                        der[c] = normalized_derivative(s->coef + c * n, s->size_i, s->size_j,
+                                                       p.p[0], p.p[1], dx, dy);
+                        CHECK(finite_d(der[c]), MPO_ERR_FP, "Smoothness basis derivative is not finite");
+                    }
+                    for (int row = 0; row < n; ++row)
+                        for (int col = 0; col <= row; ++col)
+                            work[row * n + col] += weight * der[row] * der[col];
+                }
+        }
+    for (int row = 0; row < n; ++row)
+        for (int col = 0; col <= row; ++col) {
+            CHECK(finite_d(work[row * n + col]), MPO_ERR_FP, "Smoothness matrix is not finite");
+            work[col * n + row] = work[row * n + col];
+        }
+    memcpy(matrix, work, sizeof(double) * n * n);
+    return MPO_OK;
+}
+
+/* TensorStencilPoly::Initialize(corners, ...), src/mlweno/tensorstencilpoly.cpp:146-231 */
+int mpo_stencil_init(mpo_stencil* out, const mpo_point (*corners)[4], int start_i, int start_j,
+                     int size_i, int size_j, int order, double h)
+{
+    CHECK(size_i > 0 && size_j > 0 && size_i * size_j <= MPO_MAX_STENCIL, MPO_ERR_ARG,
+          "Stencil dimensions must be positive and fit the oracle's stencil limit");
+    CHECK(start_i >= 0 && start_j >= 0, MPO_ERR_ARG, "Stencil origin or last cell index is out of range");
+    CHECK(order >= 0 && finite_d(h) && h > 0, MPO_ERR_ARG,
+          "Smoothness order must be nonnegative and h finite and positive");
+    const int n = size_i * size_j;
+    for (int c = 0; c < n; ++c) CALL(mpo_validate_quad(corners[c]));
+    mpo_stencil* w = (mpo_stencil*)malloc(sizeof(mpo_stencil));
+    if (!w) return fail(MPO_ERR_MEM, "out of memory");
+    memset(w, 0, sizeof *w);
+    w->start_i = start_i; w->start_j = start_j; w->size_i = size_i; w->size_j = size_j;
+    w->count = n; w->order = order; w->h = h; w->target_index = -1;
+    memcpy(w->corners, corners, sizeof(mpo_point) * 4 * n);
+
+    /* centre: mean of the four outer vertices, as differences from the anchor (:163-176) */
+    const mpo_point anchor = corners[0][0];
+    const mpo_point outer[4] = {anchor, corners[size_i - 1][1], corners[n - 1][2],
+                                corners[(size_j - 1) * size_i][3]};
+    for (int d = 0; d < 2; ++d) {
+        double offset = 0;
+        for (int k = 0; k < 4; ++k) offset += (outer[k].p[d] - anchor.p[d]) / 4;
+        w->center.p[d] = anchor.p[d] + offset;
+        if (!finite_d(w->center.p[d])) { free(w); return fail(MPO_ERR_FP, "Stencil center is not finite"); }
+    }
+
+    const int gp = size_i > size_j ? (size_i > 3 ? size_i : 3) : (size_j > 3 ? size_j : 3);
+    rule1d r;
+    int e = make_rule(gp, &r);
+    double moments[MPO_MAX_STENCIL * MPO_MAX_STENCIL], inverse[MPO_MAX_STENCIL * MPO_MAX_STENCIL];
+    memset(inverse, 0, sizeof inverse);
+    for (int c = 0; c < n && !e; ++c) {
+        mpo_point nc[4];
+        double area = 0;
+        e = normalize_quad(corners[c], w->center, h, nc);
+        if (!e) e = integrate_cell(nc, &r, fn_one, NULL, &area);
+        if (!e && !(area > 0)) e = fail(MPO_ERR_FP, "Normalized cell area must be positive");
+        for (int j = 0; j < size_j && !e; ++j)
+            for (int i = 0; i < size_i && !e; ++i) {
+                mono_ctx m = {i, j};
+                double moment = 0;
+                e = integrate_cell(nc, &r, fn_monomial, &m, &moment);
+                moments[c * n + j * size_i + i] = moment / area;
+                if (!e && !finite_d(moments[c * n + j * size_i + i]))
+                    e = fail(MPO_ERR_FP, "Cell moment is not finite");
+            }
+        inverse[c * n + c] = 1;
+    }
+    if (!e) e = dense_solve(n, moments, inverse);
+    for (int c = 0; c < n && !e; ++c)
+        for (int k = 0; k < n; ++k) {
+            const double v = inverse[k * n + c];
+            if (!finite_d(v)) { e = fail(MPO_ERR_FP, "Stencil solve produced a non-finite coefficient"); break; }
+            w->coef[c * n + k] = v;
+        }
+    if (!e) {
+        const mpo_point ref[4] = {{{-0.5, -0.5}}, {{0.5, -0.5}}, {{0.5, 0.5}}, {{-0.5, 0.5}}};
+        e = build_smoothness(w, ref, gp, w->sigma);
+    }
+    if (!e) memcpy(out, w, sizeof *w);
+    free(w);
+    return e;
+}
+
+/* SetTargetSmoothness, src/mlweno/tensorstencilpoly.cpp:383-398 */
+int mpo_stencil_set_target(mpo_stencil* s, int li, int lj)
+{
+    CHECK(s->count > 0, MPO_ERR_STATE, "Initialize the stencil before selecting a target cell");
+    CHECK(li >= 0 && li < s->size_i && lj >= 0 && lj < s->size_j, MPO_ERR_ARG,
+          "Target must be a local cell offset within the stencil");
+    const int index = lj * s->size_i + li;
+    mpo_point nc[4];
+    CALL(normalize_quad(s->corners[index], s->center, s->h, nc));
+    const int degree = (s->size_i - 1) + (s->size_j - 1);
+    double matrix[MPO_MAX_STENCIL * MPO_MAX_STENCIL];
+    CALL(build_smoothness(s, nc, degree > 3 ? degree : 3, matrix));
+    memcpy(s->target_sigma, matrix, sizeof(double) * s->count * s->count);
+    s->target_index = index;
+    return MPO_OK;
+}
+
+static int check_averages(const mpo_stencil* s, const double* a)
+{
+    CHECK(s->count > 0, MPO_ERR_STATE, "Initialize the stencil before evaluation");
+    for (int k = 0; k < s->count; ++k)
+        CHECK(finite_d(a[k]), MPO_ERR_ARG, "Cell averages must be finite");
+    return MPO_OK;
+}
+
+/* ApplySmoothness, src/mlweno/tensorstencilpoly.cpp:400-459 (value only) */
+static int apply_smoothness(const mpo_stencil* s, const double* matrix, const double* a,
+                            double* value)
+{
+    CALL(check_averages(s, a));
+    const int n = s->count;
+    long double d[MPO_MAX_STENCIL];
+    for (int i = 0; i < n; ++i) d[i] = (long double)a[i] - a[0];
+    long double sum = 0, magnitude = 0;
+    for (int row = 0; row < n; ++row)
+        for (int col = 0; col < n; ++col) {
+            const long double term = d[row] * matrix[row * n + col] * d[col];
+            sum += term;
+            magnitude += fabsl(term);
+        }
+    CHECK(isfinite(sum) && isfinite(magnitude), MPO_ERR_FP, "Smoothness evaluation overflowed");
+    const long double tol = 64 * (long double)DBL_EPSILON * magnitude;
+    CHECK(sum >= -tol, MPO_ERR_FP, "Smoothness quadratic form is negative beyond roundoff");
+    const double result = (double)(sum > 0 ? sum : 0.0L);
+    CHECK(finite_d(result), MPO_ERR_FP, "Smoothness indicator is not representable");
+    *value = result;
+    return MPO_OK;
+}
+
+int mpo_stencil_smoothness(const mpo_stencil* s, const double* a, int use_target, double* value)
+{
+    if (use_target) {
+        CHECK(s->target_index >= 0, MPO_ERR_STATE,
+              "Call SetTargetSmoothness for this target cell before querying it");
+        return apply_smoothness(s, s->target_sigma, a, value);
+    }
+    return apply_smoothness(s, s->sigma, a, value);
+}
+
+static int stencil_eval_at(const mpo_stencil* s, const double* a, mpo_point point,
+                           mpo_point center, double h, int dx, int dy, double* value);
+
+/* EvaluateDerivative with PolynomialValue, src/mlweno/tensorstencilpoly.cpp:247-316 */
+int mpo_stencil_eval(const mpo_stencil* s, const double* a, mpo_point point, int dx, int dy,
+                     double* value)
+{
+    return stencil_eval_at(s, a, point, s->center, s->h, dx, dy, value);
+}
+
+static int stencil_eval_at(const mpo_stencil* s, const double* a, mpo_point point,
+                           mpo_point center, double h, int dx, int dy, double* value)
+{
+    CALL(check_averages(s, a));
+    const int n = s->count;
+    double poly[MPO_MAX_STENCIL];
+    memset(poly, 0, sizeof(double) * n);
+    poly[0] = a[0];
+    for (int c = 0; c < n; ++c) {
+        const double diff = a[c] - a[0];
+        CHECK(finite_d(diff), MPO_ERR_FP, "Cell-average difference is not representable");
+        for (int k = 0; k < n; ++k) poly[k] += diff * s->coef[c * n + k];
+    }
+    for (int k = 0; k < n; ++k)
+        CHECK(finite_d(poly[k]), MPO_ERR_FP, "Reconstructed coefficient is not finite");
+    CHECK(dx >= 0 && dy >= 0 && finite_d(point.p[0]) && finite_d(point.p[1]), MPO_ERR_ARG,
+          "Derivative orders must be nonnegative and the point finite");
+    const double x = (point.p[0] - center.p[0]) / h;
+    const double y = (point.p[1] - center.p[1]) / h;
+    CHECK(finite_d(x) && finite_d(y), MPO_ERR_FP, "Normalized evaluation point is not finite");
+    double result = normalized_derivative(poly, s->size_i, s->size_j, x, y, dx, dy);
+    if (dx < s->size_i && dy < s->size_j) {
+        for (int k = 0; k < dx; ++k) result /= h;
+        for (int k = 0; k < dy; ++k) result /= h;
+    }
+    CHECK(finite_d(result), MPO_ERR_FP, "Polynomial value or derivative is not finite");
+    *value = result;
+    return MPO_OK;
+}
+
+/* ========================================================= Reconstruction */
+
+void mpo_production_options(mpo_weno_options* o)
+{
+    /* src/couple/state_reconstruction.cpp:64-68; defaults src/mlweno/reconstruction.h:28-34 */
+    memset(o, 0, sizeof *o);
+    o->large.size_i = 3; o->large.size_j = 3; o->large.order = 2;
+    o->large.noffsets = 1; o->large.off_i[0] = -1; o->large.off_j[0] = -1;
+    o->small.size_i = 2; o->small.size_j = 2; o->small.order = 1;
+    o->small.noffsets = 4;
+    const int si[4] = {0, 0, -1, -1}, sj[4] = {-1, 0, -1, 0};
+    for (int k = 0; k < 4; ++k) { o->small.off_i[k] = si[k]; o->small.off_j[k] = sj[k]; }
+    o->small.target_smoothness = 1;
+    o->use_constant = 0;
+    o->constant_linear_weight = 1e-5;
+    o->epsilon = 1e-2;
+    o->s = 1;
+}
+
+void mpo_cell_corners(const mpo_mesh* m, int i, int j, mpo_point c[4])
+{
+    const int ci[4] = {i, i + 1, i + 1, i}, cj[4] = {j, j, j + 1, j + 1};
+    for (int k = 0; k < 4; ++k) {
+        const double* v = m->coords + ((size_t)cj[k] * (size_t)(m->nx + 1) + (size_t)ci[k]) * 2;
+        c[k].p[0] = v[0]; c[k].p[1] = v[1];
+    }
+}
+
+/* MeshInfo::CacheAreas: one-point Gauss rule, src/core/mesh_info.cpp:287-300 */
+double mpo_cell_area(const mpo_mesh* m, int i, int j)
+{
+    mpo_point c[4];
+    mpo_cell_corners(m, i, j, c);
+    const mpo_point ref = {{0.0, 0.0}};
+    return 2.0 * 2.0 * mpo_cell_jacobian(ref, c);
+}
+
+/* src/mlweno/reconstruction.cpp:10-41 */
+static int check_family(const mpo_family* f)
+{
+    CHECK(f->size_i > 0 && f->size_j > 0, MPO_ERR_ARG,
+          "Stencil dimensions must be positive with a representable product");
+    CHECK(f->order >= 0, MPO_ERR_ARG, "Smoothness order must be nonnegative");
+    CHECK(f->noffsets >= 0 && f->noffsets <= MPO_MAX_CAND, MPO_ERR_ARG, "Too many offsets");
+    for (int k = 0; k < f->noffsets; ++k) {
+        CHECK(f->off_i[k] <= 0 && f->off_i[k] >= 1 - f->size_i && f->off_j[k] <= 0 &&
+              f->off_j[k] >= 1 - f->size_j, MPO_ERR_ARG,
+              "Every prescribed stencil must contain the target cell");
+        const double w = f->has_weights ? f->linear_weight[k] : 1.0;
+        CHECK(finite_d(w) && w >= 0, MPO_ERR_ARG, "Linear weights must be finite and nonnegative");
+    }
+    return MPO_OK;
+}
+
+int mpo_recon_storage_count(const mpo_weno_options* opt)
+{
+    return opt->large.noffsets + opt->small.noffsets;
+}
+
+/* Reconstruction::Initialize, src/mlweno/reconstruction.cpp:109-210 */
+int mpo_recon_init(mpo_recon* out, const mpo_mesh* mesh, int ti, int tj,
+                   const mpo_weno_options* opt, double h, mpo_stencil* storage,
+                   const mpo_stencil* const* shared)
+{
+    CHECK(ti >= 0 && ti < mesh->nx && tj >= 0 && tj < mesh->ny, MPO_ERR_ARG,
+          "Reconstruction target must be a physical cell");
+    CHECK(finite_d(h) && h > 0 && finite_d(opt->epsilon) && opt->epsilon > 0 && opt->s >= 0,
+          MPO_ERR_ARG, "h and epsilon must be finite and positive; s must be nonnegative");
+    CHECK(finite_d(opt->constant_linear_weight) && opt->constant_linear_weight >= 0, MPO_ERR_ARG,
+          "Constant linear weight must be finite and nonnegative");
+    CALL(check_family(&opt->large));
+    CALL(check_family(&opt->small));
+    mpo_recon w;
+    memset(&w, 0, sizeof w);
+    w.nx = mesh->nx; w.ny = mesh->ny; w.ti = ti; w.tj = tj;
+    w.req_bi = ti; w.req_bj = tj; w.req_ei = ti + 1; w.req_ej = tj + 1;
+    w.opt = *opt;
+    for (int f = 0; f < 2; ++f) {
+        const mpo_family* fam = f == 0 ? &opt->large : &opt->small;
+        for (int k = 0; k < fam->noffsets; ++k) {
+            const double lw = fam->has_weights ? fam->linear_weight[k] : 1.0;
+            if (lw == 0) continue;
+            const int si = ti + fam->off_i[k], sj = tj + fam->off_j[k];
+            if (si < 0 || sj < 0 || fam->size_i > mesh->nx || fam->size_j > mesh->ny ||
+                si > mesh->nx - fam->size_i || sj > mesh->ny - fam->size_j)
+                continue;   /* discarded, not shifted (:150-153) */
+            const int slot = f == 0 ? k : opt->large.noffsets + k;
+            mpo_candidate* c = &w.cand[w.ncand];
+            c->family = f; c->configured_index = k; c->linear_index = k; c->linear_weight = lw;
+            c->start_i = si; c->start_j = sj; c->h = h;
+            {
+                /* stencil centre: mean of the four outer vertices (tensorstencilpoly.cpp:163-176) */
+                mpo_point q0[4], q1[4], q2[4], q3[4];
+                mpo_cell_corners(mesh, si, sj, q0);
+                mpo_cell_corners(mesh, si + fam->size_i - 1, sj, q1);
+                mpo_cell_corners(mesh, si + fam->size_i - 1, sj + fam->size_j - 1, q2);
+                mpo_cell_corners(mesh, si, sj + fam->size_j - 1, q3);
+                const mpo_point anchor = q0[0];
+                const mpo_point outer[4] = {anchor, q1[1], q2[2], q3[3]};
+                for (int d = 0; d < 2; ++d) {
+                    double offset = 0;
+                    for (int m = 0; m < 4; ++m) offset += (outer[m].p[d] - anchor.p[d]) / 4;
+                    c->center.p[d] = anchor.p[d] + offset;
+                }
+            }
+            if (shared) {
+                CHECK(shared[slot] != NULL, MPO_ERR_STATE, "Missing shared stencil table");
+                c->stencil = shared[slot];
+            } else {
+                mpo_point corners[MPO_MAX_STENCIL][4];
+                for (int jj = 0; jj < fam->size_j; ++jj)
+                    for (int ii = 0; ii < fam->size_i; ++ii)
+                        mpo_cell_corners(mesh, si + ii, sj + jj, corners[jj * fam->size_i + ii]);
+                CALL(mpo_stencil_init(&storage[slot], (const mpo_point (*)[4])corners, si, sj,
+                                      fam->size_i, fam->size_j, fam->order, h));
+                if (fam->target_smoothness)
+                    CALL(mpo_stencil_set_target(&storage[slot], -fam->off_i[k], -fam->off_j[k]));
+                c->stencil = &storage[slot];
+            }
+            if (si < w.req_bi) w.req_bi = si;
+            if (sj < w.req_bj) w.req_bj = sj;
+            if (si + fam->size_i > w.req_ei) w.req_ei = si + fam->size_i;
+            if (sj + fam->size_j > w.req_ej) w.req_ej = sj + fam->size_j;
+            ++w.ncand;
+        }
+    }
+    CHECK(w.ncand > 0 || (opt->use_constant && opt->constant_linear_weight > 0), MPO_ERR_ARG,
+          "No positive-weight candidate remains at this target cell");
+    *out = w;
+    return MPO_OK;
+}
+
+static int eta(int r) { return r == 1 ? 1 : (r == 2 ? 3 : 4); }
+
+/* LogDenominator, src/mlweno/reconstruction.cpp:46-54 */
+static long double log_denominator(double sigma, long double log_reg)
+{
+    if (sigma == 0) return log_reg;
+    const long double ls = logl((long double)sigma);
+    const long double hi = ls > log_reg ? ls : log_reg, lo = ls > log_reg ? log_reg : ls;
+    return hi + log1pl(expl(lo - hi));
+}
+
+/* NormalizeWeights, src/mlweno/reconstruction.cpp:212-265 */
+static int normalize_weights(const mpo_recon* r, const double* sigma, double area,
+                             double* weights /* ncand+1 */)
+{
+    long double la[MPO_MAX_CAND + 1];
+    for (int k = 0; k <= r->ncand; ++k) la[k] = -INFINITY;
+    const long double lreg = logl((long double)r->opt.epsilon) + logl((long double)area);
+    for (int k = 0; k < r->ncand; ++k) {
+        const mpo_family* fam = r->cand[k].family == 0 ? &r->opt.large : &r->opt.small;
+        const int rr = fam->order + 1;
+        const long double ex = (long double)r->opt.s * rr + eta(rr);
+        la[k] = logl((long double)r->cand[k].linear_weight) - ex * log_denominator(sigma[k], lreg);
+        CHECK(isfinite(la[k]), MPO_ERR_FP, "Polynomial log weight is not finite");
+    }
+    if (r->opt.use_constant && r->opt.constant_linear_weight > 0) {
+        const long double ex = (long double)r->opt.s + eta(1);
+        la[r->ncand] = logl((long double)r->opt.constant_linear_weight) - ex * lreg;
+        CHECK(isfinite(la[r->ncand]), MPO_ERR_FP, "Constant log weight is not finite");
+    }
+    long double mx = la[0];
+    for (int k = 1; k <= r->ncand; ++k) if (la[k] > mx) mx = la[k];
+    CHECK(isfinite(mx), MPO_ERR_FP, "No finite active log weight");
+    long double sum = 0;
+    for (int k = 0; k <= r->ncand; ++k) { la[k] = expl(la[k] - mx); sum += la[k]; }
+    CHECK(isfinite(sum) && sum > 0, MPO_ERR_FP, "Invalid ML-WENO weight normalization");
+    for (int k = 0; k <= r->ncand; ++k) weights[k] = (double)(la[k] / sum);
+    return MPO_OK;
+}
+
+/* Reconstruction::UpdateImpl (value path), src/mlweno/reconstruction.cpp:275-355 */
+int mpo_recon_update(mpo_recon* r, const double* values, int bi, int bj, int ei, int ej,
+                     double area)
+{
+    CHECK(finite_d(area) && area > 0, MPO_ERR_ARG, "Weight area scale must be finite and positive");
+    CHECK(bi >= 0 && bj >= 0 && ei > bi && ej > bj && ei <= r->nx && ej <= r->ny, MPO_ERR_ARG,
+          "Cell-average range must be a nonempty rectangle of physical cells");
+    CHECK(bi <= r->req_bi && bj <= r->req_bj && ei >= r->req_ei && ej >= r->req_ej, MPO_ERR_ARG,
+          "Cell averages are unavailable locally; update/increase the solution halo");
+    const size_t pitch = (size_t)(ei - bi);
+#define PATCH(ii, jj) values[(size_t)((jj) - bj) * pitch + (size_t)((ii) - bi)]
+    const double ta = PATCH(r->ti, r->tj);
+    CHECK(finite_d(ta), MPO_ERR_ARG, "Target cell average must be finite");
+    double centered[MPO_MAX_CAND][MPO_MAX_STENCIL];
+    double sigma[MPO_MAX_CAND], weights[MPO_MAX_CAND + 1];
+    for (int k = 0; k < r->ncand; ++k) {
+        const mpo_stencil* s = r->cand[k].stencil;
+        for (int j = 0; j < s->size_j; ++j)
+            for (int i = 0; i < s->size_i; ++i) {
+                const double avg = PATCH(r->cand[k].start_i + i, r->cand[k].start_j + j);
+                CHECK(finite_d(avg), MPO_ERR_ARG, "Referenced cell averages must be finite");
+                const double diff = avg - ta;
+                CHECK(finite_d(diff), MPO_ERR_FP, "Centered cell average is not representable");
+                centered[k][j * s->size_i + i] = diff;
+            }
+        const mpo_family* fam = r->cand[k].family == 0 ? &r->opt.large : &r->opt.small;
+        CALL(mpo_stencil_smoothness(s, centered[k], fam->target_smoothness, &sigma[k]));
+    }
+#undef PATCH
+    CALL(normalize_weights(r, sigma, area, weights));
+    for (int k = 0; k < r->ncand; ++k) {
+        memcpy(r->cand[k].centered, centered[k], sizeof(double) * r->cand[k].stencil->count);
+        r->cand[k].smoothness = sigma[k];
+        r->cand[k].weight = weights[k];
+    }
+    r->target_average = ta;
+    r->constant_weight = weights[r->ncand];
+    r->area = area;
+    r->has_weights = 1;
+    return MPO_OK;
+}
+
+/* Reconstruction::EvaluateDerivative, src/mlweno/reconstruction.cpp:477-502 */
+int mpo_recon_eval(const mpo_recon* r, mpo_point x, int dx, int dy, double* value)
+{
+    CHECK(r->has_weights, MPO_ERR_STATE, "Update reconstruction before evaluation");
+    CHECK(dx >= 0 && dy >= 0 && finite_d(x.p[0]) && finite_d(x.p[1]), MPO_ERR_ARG,
+          "Derivative orders must be nonnegative and the point finite");
+    long double result = (dx == 0 && dy == 0) ? r->target_average : 0;
+    for (int k = 0; k < r->ncand; ++k) {
+        if (r->cand[k].weight == 0) continue;
+        double residual = 0;
+        CALL(stencil_eval_at(r->cand[k].stencil, r->cand[k].centered, x, r->cand[k].center,
+                             r->cand[k].h, dx, dy, &residual));
+        result += (long double)r->cand[k].weight * residual;
+    }
+    CHECK(isfinite(result) && fabsl(result) <= DBL_MAX, MPO_ERR_FP,
+          "Reconstructed value is not representable");
+    *value = (double)result;
+    return MPO_OK;
+}
+
+/* ========================================================= advective flux */
+
+static int real_value(long double v, double* out)
+{
+    CHECK(isfinite(v) && fabsl(v) <= DBL_MAX, MPO_ERR_FP, "Advective flux or derivative is not representable");
+    *out = (double)v;
+    return MPO_OK;
+}
+
+/* CheckPoint, src/mlweno/advectiveflux.cpp:20-37 (position/time omitted: the
+ * linear law does not read them) */
+static int check_point(mpo_point v, mpo_point n)
+{
+    for (int d = 0; d < 2; ++d)
+        CHECK(finite_d(v.p[d]) && finite_d(n.p[d]), MPO_ERR_ARG,
+              "Flux position, velocity and normal must be finite");
+    const double norm = hypot(n.p[0], n.p[1]);
+    CHECK(finite_d(norm) && fabs(norm - 1) <= 128 * DBL_EPSILON, MPO_ERR_ARG,
+          "Advective flux requires a unit normal");
+    return MPO_OK;
+}
+
+/* LinearAdvectiveFlux, src/mlweno/advectiveflux.cpp:272-285 */
+static int linear_flux(double u, mpo_point v, mpo_point n, double* value, double* derivative)
+{
+    CALL(check_point(v, n));
+    CHECK(finite_d(u), MPO_ERR_ARG, "Linear-advection state must be finite");
+    double vn;
+    CALL(real_value((long double)v.p[0] * n.p[0] + (long double)v.p[1] * n.p[1], &vn));
+    CALL(real_value((long double)vn * u, value));
+    *derivative = vn;
+    return MPO_OK;
+}
+
+static int bounds_ok(double bound, double required)
+{
+    return bound >= required || required - bound <= 64 * DBL_EPSILON * fmax(bound, required);
+}
+
+/* FluxImpl (value path), src/mlweno/advectiveflux.cpp:129-164 */
+int mpo_llf_point(double left, double right, mpo_point v, mpo_point n, int mode,
+                  double global_speed, double* flux, double* local_speed)
+{
+    CALL(check_point(v, n));
+    CHECK(mode == 0 || mode == 1, MPO_ERR_ARG, "Unknown LF mode");
+    if (mode == 1)
+        CHECK(finite_d(global_speed) && global_speed >= 0, MPO_ERR_ARG,
+              "Global LF speed must be finite and nonnegative");
+    double fl, dl, fr, dr;
+    CHECK(finite_d(left), MPO_ERR_ARG, "Advective trace must be finite");
+    CALL(linear_flux(left, v, n, &fl, &dl));
+    CHECK(finite_d(right), MPO_ERR_ARG, "Advective trace must be finite");
+    CALL(linear_flux(right, v, n, &fr, &dr));
+    const double a = fabs(dl), b = fabs(dr);
+    const double speed = a > b ? a : b;   /* LocalSpeed endpoint estimate (:91-127) */
+    const double alpha = mode == 0 ? speed : global_speed;
+    CHECK(bounds_ok(alpha, speed), MPO_ERR_ARG,
+          "Supplied global LF speed is smaller than the current local bound");
+    const long double jump = (long double)right - left;
+    CALL(real_value(((long double)fl + fr - (long double)alpha * jump) / 2, flux));
+    if (local_speed) *local_speed = speed;
+    return MPO_OK;
+}
+
+/* EdgeImpl (value path), src/mlweno/advectiveflux.cpp:192-227 */
+int mpo_integrate_edge_flux(const mpo_point edge[2], int nq, const double* qp, const double* qw,
+                            const mpo_point* velocity, const double* left, const double* right,
+                            int mode, double global_speed, double* flux)
+{
+    (void)qp;
+    double length; mpo_point normal;
+    CALL(mpo_edge_geometry(edge, &length, &normal));
+    long double sum = 0;
+    for (int q = 0; q < nq; ++q) {
+        CHECK(finite_d(velocity[q].p[0]) && finite_d(velocity[q].p[1]), MPO_ERR_ARG,
+              "Edge velocity must be finite");
+        double f;
+        CALL(mpo_llf_point(left[q], right[q], velocity[q], normal, mode, global_speed, &f, NULL));
+        const long double weight = (long double)qw[q] * length / 2;
+        sum += weight * f;
+    }
+    return real_value(sum, flux);
+}
+
+/* BuildLinearAdvectionBoundaryData, src/mlweno/advectiveflux.cpp:394-424 */
+int mpo_boundary_data(const mpo_point edge[2], int nq, const double* qp, const double* qw,
+                      const mpo_point* velocity, const double* inflow_values, int* types,
+                      double* values)
+{
+    (void)qp; (void)qw;
+    double length; mpo_point normal;
+    CALL(mpo_edge_geometry(edge, &length, &normal));
+    for (int q = 0; q < nq; ++q) {
+        CALL(check_point(velocity[q], normal));
+        double vn;
+        CALL(real_value((long double)velocity[q].p[0] * normal.p[0] +
+                        (long double)velocity[q].p[1] * normal.p[1], &vn));
+        if (vn < 0) {
+            CHECK(inflow_values != NULL, MPO_ERR_ARG, "An inflow point requires prescribed boundary data");
+            CHECK(finite_d(inflow_values[q]), MPO_ERR_FP, "Inflow boundary callback must return a finite state");
+            types[q] = 0; values[q] = inflow_values[q];
+        } else if (vn > 0) {
+            types[q] = 1; values[q] = 0;
+        } else {
+            types[q] = 2; values[q] = 0;
+        }
+    }
+    return MPO_OK;
+}
+
+/* BoundaryImpl (value path), src/mlweno/advectiveflux.cpp:229-268 */
+int mpo_integrate_boundary_flux(const mpo_point edge[2], int nq, const double* qp,
+                                const double* qw, const mpo_point* velocity,
+                                const double* interior, const int* types, const double* values,
+                                double* flux)
+{
+    (void)qp;
+    double length; mpo_point normal;
+    CALL(mpo_edge_geometry(edge, &length, &normal));
+    long double sum = 0;
+    for (int q = 0; q < nq; ++q) {
+        CHECK(types[q] >= 0 && types[q] <= 2, MPO_ERR_ARG, "Unknown advection boundary policy");
+        CHECK(finite_d(interior[q]), MPO_ERR_ARG, "Interior boundary trace must be finite");
+        CALL(check_point(velocity[q], normal));
+        if (types[q] == 2) continue;
+        double f, d;
+        CALL(linear_flux(types[q] == 1 ? interior[q] : values[q], velocity[q], normal, &f, &d));
+        const long double weight = (long double)qw[q] * length / 2;
+        sum += weight * f;
+    }
+    return real_value(sum, flux);
+}
+
+/* =================================================================== phase */
+
+void mpo_material_defaults(mpo_material* p)
+{
+    /* src/phase/eutectic_rescaled.h:10-26 */
+    p->Tm0 = 2000.0; p->Te0 = 1480.0; p->nu = 6.5e6; p->L = 5.0e5; p->cp = 1200.0; p->Xe = 0.25;
+    p->rho = 3000.0; p->rhor = 500.0; p->mus = 1.0e19; p->mul = 1.0; p->k0 = 1.0e-8;
+    p->g = 10.0; p->alpha0 = 3.0e-5;
+}
+
+static int checked_double(long double v, double* out)
+{
+    CHECK(isfinite(v) && fabsl(v) <= (long double)DBL_MAX, MPO_ERR_FP, "value is not representable");
+    *out = (double)v;
+    return MPO_OK;
+}
+
+static int positive_scale(long double v, double* out)
+{
+    CALL(checked_double(v, out));
+    CHECK(*out > 0.0, MPO_ERR_FP, "value is not a positive scale");
+    return MPO_OK;
+}
+
+#define REQ_FINITE(v) CHECK(finite_d(v), MPO_ERR_ARG, #v " must be finite")
+#define REQ_POSITIVE(v) do { REQ_FINITE(v); CHECK((v) > 0.0, MPO_ERR_ARG, #v " must be positive"); } while (0)
+
+/* derive, src/phase/eutectic_rescaled.cpp:42-86 */
+int mpo_phase_set_parameters(mpo_phase_model* m, const mpo_material* p)
+{
+    REQ_POSITIVE(p->Tm0);
+    REQ_POSITIVE(p->Te0);
+    CHECK(p->Tm0 > p->Te0, MPO_ERR_ARG, "Tm0 must be greater than Te0");
+    REQ_FINITE(p->nu);
+    CHECK(p->nu != 0.0, MPO_ERR_ARG, "nu must be nonzero");
+    REQ_POSITIVE(p->L);
+    REQ_POSITIVE(p->cp);
+    REQ_FINITE(p->Xe);
+    CHECK(p->Xe > 0.0 && p->Xe < 1.0, MPO_ERR_ARG, "Xe must lie strictly between 0 and 1");
+    REQ_POSITIVE(p->rho);
+    REQ_POSITIVE(p->rhor);
+    REQ_POSITIVE(p->mus);
+    REQ_POSITIVE(p->mul);
+    REQ_POSITIVE(p->k0);
+    REQ_POSITIVE(p->g);
+    REQ_FINITE(p->alpha0);
+    mpo_derived d;
+    const long double deltaT = (long double)p->Tm0 - p->Te0;
+    const long double h0 = (long double)p->cp * deltaT;
+    CALL(positive_scale(deltaT, &d.dT));
+    CALL(positive_scale(h0, &d.h0));
+    CALL(checked_double(1.0L / p->nu, &d.gamma));
+    CHECK(d.gamma != 0.0, MPO_ERR_FP, "gamma underflows to zero");
+    CALL(positive_scale(p->L / h0, &d.LD));
+    CALL(positive_scale(p->Tm0 / deltaT, &d.TDm0));
+    CALL(positive_scale(p->Te0 / deltaT, &d.TDe0));
+    CALL(positive_scale(1.0L / p->k0, &d.invk0));
+    const long double l0 = sqrtl((long double)p->mus * p->k0 / p->mul);
+    const long double u0 = (long double)p->k0 * p->rhor * p->g / p->mul;
+    CALL(positive_scale(l0, &d.l0));
+    CALL(positive_scale(u0, &d.u0));
+    CALL(positive_scale((long double)p->rhor * p->g * l0, &d.p0));
+    CALL(positive_scale(l0 / u0, &d.t0));
+    m->p = *p;
+    m->d = d;
+    return MPO_OK;
+}
+
+/* boundaries, src/phase/eutectic_rescaled.cpp:143-149, 163-185 (Xe passed in so
+ * the hypereutectic mirror can reuse it) */
+static int phase_bounds_xe(const mpo_phase_model* m, double Xe, double C, double P,
+                           mpo_phase_bounds* b)
+{
+    CHECK(finite_d(C), MPO_ERR_ARG, "CD must be finite");
+    CHECK(C >= 0.0 && C <= Xe, MPO_ERR_ARG, "CD must lie in [0, Xe]");
+    CHECK(finite_d(m->d.TDe0) && finite_d(P), MPO_ERR_ARG, "TD and P must be finite");
+    const long double shift = (long double)P / m->p.nu / m->d.dT;
+    mpo_phase_bounds w;
+    CALL(checked_double((long double)m->d.TDe0 + shift, &w.Tep));
+    CALL(checked_double((long double)w.Tep + 1.0L, &w.Tmp));
+    const long double ratio = (long double)C / Xe;
+    CALL(checked_double((long double)w.Tep + m->d.LD * ratio, &w.H_eutectic_end));
+    CALL(checked_double((long double)w.Tep + (1.0L - ratio) + m->d.LD, &w.H_liquidus));
+    if (C == 0.0) CALL(checked_double((long double)w.Tmp + m->d.LD, &w.H_liquidus));
+    CHECK(!(w.Tmp <= w.Tep || w.H_liquidus <= w.Tep || (C == 0.0 && w.H_liquidus <= w.Tmp)),
+          MPO_ERR_FP, "Phase temperature/enthalpy intervals are not resolvable");
+    *b = w;
+    return MPO_OK;
+}
+
+int mpo_phase_boundaries(const mpo_phase_model* m, double C, double P, mpo_phase_bounds* b)
+{
+    return phase_bounds_xe(m, m->p.Xe, C, P, b);
+}
+
+/* selectRegion, src/phase/eutectic_rescaled.cpp:95-105 */
+static int select_region(double H, double C, const mpo_phase_bounds* b)
+{
+    if (C == 0.0) {
+        if (H <= b->Tmp) return 1;
+        if (H < b->H_liquidus) return 6;
+        return 5;
+    }
+    if (H <= b->Tep) return 2;
+    if (H >= b->H_liquidus) return 5;
+    if (H <= b->H_eutectic_end) return 3;
+    return 4;
+}
+
+/* unitFraction, src/phase/eutectic_rescaled.cpp:108-114 */
+static int unit_fraction(long double v, double* out)
+{
+    const long double tol = 64.0L * (long double)DBL_EPSILON;
+    CHECK(isfinite(v) && v >= -tol && v <= 1.0L + tol, MPO_ERR_RUNTIME,
+          "Computed phase fraction is outside [0, 1]");
+    *out = (double)(v < 0 ? 0.0L : (v > 1 ? 1.0L : v));
+    return MPO_OK;
+}
+
+static double negative_derivative(long double mag)
+{
+    if (mag > (long double)DBL_MAX) return -INFINITY;
+    return -(double)mag;
+}
+
+/* EutecticModel::evaluate, src/phase/eutectic_rescaled.cpp:192-282 */
+static int phase_eval_xe(const mpo_phase_model* m, double Xe_d, double H, double C, double P,
+                         mpo_phase_state* out)
+{
+    CHECK(finite_d(H), MPO_ERR_ARG, "HD must be finite");
+    mpo_phase_bounds b;
+    CALL(phase_bounds_xe(m, Xe_d, C, P, &b));
+    const long double latent = m->d.LD;
+    const long double Xe = Xe_d;
+    mpo_phase_state s;
+    memset(&s, 0, sizeof s);
+    s.finite_temperature_derivatives = 1;
+    s.region = select_region(H, C, &b);
+    s.Tep = b.Tep;
+    switch (s.region) {
+    case 1:
+        s.phi1 = 1.0; s.TDp = H; s.dTD_dHD = 1.0;
+        break;
+    case 6:
+        CALL(unit_fraction(((long double)H - b.Tmp) / latent, &s.phil));
+        s.phi1 = 1.0 - s.phil;
+        s.TDp = b.Tmp;
+        break;
+    case 2:
+        s.phi1 = 1.0 - C; s.phi2 = C; s.TDp = H; s.dTD_dHD = 1.0;
+        break;
+    case 3: {
+        const long double maxl = (long double)C / Xe;
+        long double liquid;
+        if (H == b.H_eutectic_end) liquid = maxl;
+        else {
+            const long double e = ((long double)H - b.Tep) / latent;
+            liquid = maxl < e ? maxl : e;
+        }
+        long double solid2 = (long double)C - liquid * Xe;
+        if (solid2 < 0) solid2 = 0;
+        CALL(unit_fraction(liquid, &s.phil));
+        CALL(unit_fraction(solid2, &s.phi2));
+        CALL(unit_fraction(1.0L - liquid - solid2, &s.phi1));
+        s.cl = Xe_d;
+        s.TDp = b.Tep;
+        break;
+    }
+    case 4: {
+        const long double a = (long double)H - b.Tmp;
+        const long double ratio = (long double)C / Xe;
+        const long double root = hypotl(a, 2.0L * sqrtl(latent) * sqrtl(ratio));
+        const long double liquid = a >= 0.0L ? (a + root) / (2.0L * latent) : 2.0L * ratio / (root - a);
+        CALL(unit_fraction(liquid, &s.phil));
+        CALL(unit_fraction(1.0L - liquid, &s.phi1));
+        CALL(unit_fraction((long double)C / liquid, &s.cl));
+        CALL(checked_double(a >= 0.0L ? (long double)b.Tmp - ratio / liquid
+                                      : (long double)H - latent * liquid, &s.TDp));
+        CALL(unit_fraction(a >= 0.0L ? 2.0L * (latent / root) * (ratio / (root + a))
+                                     : 0.5L * (1.0L - a / root), &s.dTD_dHD));
+        s.dTD_dCD = negative_derivative((latent / root) / Xe);
+        break;
+    }
+    case 5:
+        s.phil = 1.0; s.cl = C;
+        CALL(checked_double((long double)H - latent, &s.TDp));
+        s.dTD_dHD = 1.0;
+        break;
+    }
+    if (C == 0.0 && H > b.Tep && s.region != 5) {
+        const long double gap = fabsl((long double)H - b.Tmp);
+        s.dTD_dCD = gap == 0.0L ? -INFINITY : negative_derivative((latent / gap) / Xe);
+    }
+    const double solid = s.phi1 + s.phi2;
+    s.has_solid = solid > 0.0;
+    s.has_liquid = s.phil > 0.0;
+    s.cs = s.has_solid ? s.phi2 / solid : 0.0;
+    if (!s.has_liquid) s.cl = 0.0;
+    s.finite_temperature_derivatives = isfinite(s.dTD_dCD) && isfinite(s.dTD_dHD);
+    *out = s;
+    return MPO_OK;
+}
+
+int mpo_phase_evaluate(const mpo_phase_model* m, double H, double C, double P, mpo_phase_state* s)
+{
+    return phase_eval_xe(m, m->p.Xe, H, C, P, s);
+}
+
+int mpo_phase_classify(const mpo_phase_model* m, double H, double C, double P, int* region)
+{
+    CHECK(finite_d(H), MPO_ERR_ARG, "HD must be finite");
+    mpo_phase_bounds b;
+    CALL(mpo_phase_boundaries(m, C, P, &b));
+    *region = select_region(H, C, &b);
+    return MPO_OK;
+}
+
+/* Hypereutectic mirror (extension; the reference rejects C > Xe,
+ * src/phase/eutectic_rescaled.cpp:88-93).  Component 2 plays the role of
+ * component 1 on the mirrored axis C' = 1 - C with eutectic Xe' = 1 - Xe. */
+int mpo_phase_evaluate_ext(const mpo_phase_model* m, double H, double C, double P,
+                           mpo_phase_state* out)
+{
+    if (!(C > m->p.Xe)) return phase_eval_xe(m, m->p.Xe, H, C, P, out);
+    CHECK(finite_d(C) && C <= 1.0, MPO_ERR_ARG, "CD must lie in [0, 1]");
+    mpo_phase_state s;
+    CALL(phase_eval_xe(m, 1.0 - m->p.Xe, H, 1.0 - C, P, &s));
+    mpo_phase_state r = s;
+    r.phi1 = s.phi2;
+    r.phi2 = s.phi1;
+    r.cl = s.has_liquid ? 1.0 - s.cl : 0.0;
+    const double solid = r.phi1 + r.phi2;
+    r.cs = s.has_solid ? r.phi2 / solid : 0.0;
+    r.dTD_dCD = -s.dTD_dCD;
+    *out = r;
+    return MPO_OK;
+}
+
+int mpo_phase_batch(const mpo_phase_model* m, int64_t n, const double* H, const double* C,
+                    double P, int hyper, double* T, double* phi, double* Cl, double* Cs,
+                    uint8_t* region)
+{
+    int err = 0;
+    char msg[256] = {0};
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) {
+        mpo_phase_state s;
+        int e = hyper ? mpo_phase_evaluate_ext(m, H[k], C[k], P, &s)
+                      : mpo_phase_evaluate(m, H[k], C[k], P, &s);
+        if (e) {
+#pragma omp critical
+            { if (!err) { err = e; snprintf(msg, sizeof msg, "%s", g_error); } }
+            continue;
+        }
+        if (T) T[k] = s.TDp;
+        if (phi) phi[k] = s.phil;
+        if (Cl) Cl[k] = s.cl;
+        if (Cs) Cs[k] = s.cs;
+        if (region) region[k] = (uint8_t)s.region;
+    }
+    if (err) return fail(err, msg);
+    return MPO_OK;
+}
+
+/* ====================================================== transport operator */
+
+struct mpo_transport {
+    mpo_mesh mesh;
+    double* coords;
+    int nx, ny, q;
+    double qp[16], qw[16];
+    mpo_weno_options opt;
+    int table_mode, threads, slots;
+    mpo_stencil* stencils;            /* mode 0: [cell][slot]; mode 1: [slot] */
+    double* area;                     /* [cell] */
+    double* vnx;                      /* [q][ny][nx+1] */
+    double* vny;                      /* [q][ny+1][nx] */
+    uint8_t* bc[4];                   /* bottom, right, top, left */
+    double* inflow[2][4];             /* field, side -> [k][q] canonical order */
+    double* traces;                   /* [cell][4][q] */
+    double* fv;                       /* vertical face flux  [ny][nx+1] (+x) */
+    double* fh;                       /* horizontal face flux [ny+1][nx] (+y) */
+};
+
+static size_t cidx(const mpo_transport* t, int i, int j) { return (size_t)j * t->nx + i; }
+
+void mpo_transport_destroy(mpo_transport* t)
+{
+    if (!t) return;
+    free(t->coords); free(t->stencils); free(t->area); free(t->vnx); free(t->vny);
+    for (int s = 0; s < 4; ++s) { free(t->bc[s]); free(t->inflow[0][s]); free(t->inflow[1][s]); }
+    free(t->traces); free(t->fv); free(t->fh);
+    free(t);
+}
+
+static void set_threads(const mpo_transport* t)
+{
+#ifdef _OPENMP
+    if (t->threads > 0) omp_set_num_threads(t->threads);
+#else
+    (void)t;
+#endif
+}
+
+static int side_len(const mpo_transport* t, int side) { return (side == 0 || side == 2) ? t->nx : t->ny; }
+
+int mpo_transport_create(mpo_transport** out, const mpo_mesh* mesh, int q,
+                         const mpo_weno_options* opt, const double* vnx, const double* vny,
+                         const uint8_t* bc_bottom, const uint8_t* bc_right,
+                         const uint8_t* bc_top, const uint8_t* bc_left,
+                         const double* inflow_H, const double* inflow_C, int table_mode,
+                         int threads)
+{
+    CHECK(mesh->nx >= 2 && mesh->ny >= 2, MPO_ERR_ARG, "Transport requires at least 2x2 cells");
+    CHECK(q >= 1 && q <= 16, MPO_ERR_ARG, "Edge quadrature needs 1..16 points");
+    CHECK(table_mode == 0 || table_mode == 1, MPO_ERR_ARG, "Unknown table mode");
+    mpo_transport* t = (mpo_transport*)calloc(1, sizeof *t);
+    if (!t) return fail(MPO_ERR_MEM, "out of memory");
+    t->nx = mesh->nx; t->ny = mesh->ny; t->q = q; t->opt = *opt;
+    t->table_mode = table_mode; t->threads = threads;
+    t->slots = mpo_recon_storage_count(opt);
+    const size_t N = (size_t)t->nx * t->ny;
+    const size_t nv = (size_t)(t->nx + 1) * (t->ny + 1) * 2;
+    t->coords = (double*)malloc(nv * sizeof(double));
+    t->area = (double*)malloc(N * sizeof(double));
+    t->vnx = (double*)malloc((size_t)q * t->ny * (t->nx + 1) * sizeof(double));
+    t->vny = (double*)malloc((size_t)q * (t->ny + 1) * t->nx * sizeof(double));
+    t->traces = (double*)malloc(N * 4 * q * sizeof(double));
+    t->fv = (double*)malloc((size_t)t->ny * (t->nx + 1) * sizeof(double));
+    t->fh = (double*)malloc((size_t)(t->ny + 1) * t->nx * sizeof(double));
+    if (!t->coords || !t->area || !t->vnx || !t->vny || !t->traces || !t->fv || !t->fh) {
+        mpo_transport_destroy(t); return fail(MPO_ERR_MEM, "out of memory");
+    }
+    memcpy(t->coords, mesh->coords, nv * sizeof(double));
+    t->mesh.nx = t->nx; t->mesh.ny = t->ny; t->mesh.coords = t->coords;
+    memcpy(t->vnx, vnx, (size_t)q * t->ny * (t->nx + 1) * sizeof(double));
+    memcpy(t->vny, vny, (size_t)q * (t->ny + 1) * t->nx * sizeof(double));
+    const uint8_t* bcs[4] = {bc_bottom, bc_right, bc_top, bc_left};
+    size_t inflow_offset = 0;
+    for (int s = 0; s < 4; ++s) {
+        const int n = side_len(t, s);
+        t->bc[s] = (uint8_t*)malloc((size_t)n);
+        for (int k = 0; k < n; ++k) t->bc[s][k] = bcs[s] ? bcs[s][k] : MPO_BC_ZERO;
+        for (int f = 0; f < 2; ++f) {
+            const double* src = f == 0 ? inflow_H : inflow_C;
+            if (src) {
+                t->inflow[f][s] = (double*)malloc((size_t)n * q * sizeof(double));
+                memcpy(t->inflow[f][s], src + inflow_offset, (size_t)n * q * sizeof(double));
+            }
+        }
+        inflow_offset += (size_t)n * q;
+    }
+    int e = mpo_gauss_rule(q, t->qp, t->qw);
+    for (size_t c = 0; c < N && !e; ++c) {
+        t->area[c] = mpo_cell_area(&t->mesh, (int)(c % t->nx), (int)(c / t->nx));
+        if (!(t->area[c] > 0) || !finite_d(t->area[c])) e = fail(MPO_ERR_FP, "Cell area underflowed to zero");
+    }
+    if (!e && table_mode == 0) {
+        t->stencils = (mpo_stencil*)malloc(N * t->slots * sizeof(mpo_stencil));
+        if (!t->stencils) e = fail(MPO_ERR_MEM, "out of memory for per-cell stencil tables");
+    } else if (!e) {
+        t->stencils = (mpo_stencil*)calloc((size_t)t->slots, sizeof(mpo_stencil));
+        /* one table per candidate type, from the first cell where it exists */
+        int have[2 * MPO_MAX_CAND] = {0};
+        int found = 0;
+        for (size_t c = 0; c < N && !e && found < t->slots; ++c) {
+            mpo_recon r;
+            mpo_stencil tmp[2 * MPO_MAX_CAND];
+            const int i = (int)(c % t->nx), j = (int)(c / t->nx);
+            e = mpo_recon_init(&r, &t->mesh, i, j, opt, sqrt(t->area[c]), tmp, NULL);
+            for (int k = 0; k < r.ncand && !e; ++k) {
+                const int slot = r.cand[k].family == 0 ? r.cand[k].configured_index
+                                                       : opt->large.noffsets + r.cand[k].configured_index;
+                if (!have[slot]) { t->stencils[slot] = tmp[slot]; have[slot] = 1; ++found; }
+            }
+        }
+    }
+    if (!e && table_mode == 0) {
+        int err = 0;
+        char msg[256] = {0};
+        set_threads(t);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (size_t c = 0; c < N; ++c) {
+            mpo_recon r;
+            const int i = (int)(c % t->nx), j = (int)(c / t->nx);
+            int ee = mpo_recon_init(&r, &t->mesh, i, j, opt, sqrt(t->area[c]),
+                                    t->stencils + c * t->slots, NULL);
+            if (ee) {
+#pragma omp critical
+                { if (!err) { err = ee; snprintf(msg, sizeof msg, "%s", g_error); } }
+            }
+        }
+        if (err) e = fail(err, msg);
+    }
+    if (e) { mpo_transport_destroy(t); return e; }
+    *out = t;
+    return MPO_OK;
+}
+
+/* Build one cell's reconstruction over its production candidates. */
+static int cell_recon(const mpo_transport* t, int i, int j, mpo_recon* r)
+{
+    const mpo_stencil* ptr[2 * MPO_MAX_CAND];
+    const size_t c = cidx(t, i, j);
+    for (int s = 0; s < t->slots; ++s)
+        ptr[s] = t->table_mode == 0 ? t->stencils + c * t->slots + s : t->stencils + s;
+    return mpo_recon_init(r, &t->mesh, i, j, &t->opt, sqrt(t->area[c]), NULL, ptr);
+}
+
+/* Face f of cell (i,j) in canonical orientation: 0 bottom (+x), 1 right (+y),
+ * 2 top (+x), 3 left (+y). */
+static void canonical_face(const mpo_transport* t, int i, int j, int f, mpo_point e[2])
+{
+    mpo_point c[4];
+    mpo_cell_corners(&t->mesh, i, j, c);
+    switch (f) {
+    case 0: e[0] = c[0]; e[1] = c[1]; break;
+    case 1: e[0] = c[1]; e[1] = c[2]; break;
+    case 2: e[0] = c[3]; e[1] = c[2]; break;
+    default: e[0] = c[0]; e[1] = c[3]; break;
+    }
+}
+
+/* StateReconstruction::Update + trace evaluation, src/couple/state_reconstruction.cpp:101-126 */
+static int field_traces(mpo_transport* t, const double* u, double* sigma, double* omega)
+{
+    const size_t N = (size_t)t->nx * t->ny;
+    int err = 0;
+    char msg[256] = {0};
+    set_threads(t);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (size_t c = 0; c < N; ++c) {
+        if (err) continue;
+        const int i = (int)(c % t->nx), j = (int)(c / t->nx);
+        mpo_recon r;
+        int e = cell_recon(t, i, j, &r);
+        if (!e) e = mpo_recon_update(&r, u, 0, 0, t->nx, t->ny, t->area[c]);
+        for (int f = 0; f < 4 && !e; ++f) {
+            mpo_point edge[2];
+            canonical_face(t, i, j, f, edge);
+            for (int q = 0; q < t->q && !e; ++q)
+                e = mpo_recon_eval(&r, mpo_map_edge_point(t->qp[q], edge), 0, 0,
+                                   &t->traces[(c * 4 + f) * t->q + q]);
+        }
+        if (!e && sigma) {
+            for (int k = 0; k < 5; ++k) { sigma[c * 5 + k] = 0; omega[c * 5 + k] = 0; }
+            for (int k = 0; k < r.ncand; ++k) {
+                const int slot = r.cand[k].family == 0 ? r.cand[k].configured_index
+                                                       : t->opt.large.noffsets + r.cand[k].configured_index;
+                if (slot < 5) { sigma[c * 5 + slot] = r.cand[k].smoothness; omega[c * 5 + slot] = r.cand[k].weight; }
+            }
+        }
+        if (e) {
+#pragma omp critical
+            { if (!err) { err = e; snprintf(msg, sizeof msg, "%s", g_error); } }
+        }
+    }
+    if (err) return fail(err, msg);
+    return MPO_OK;
+}
+
+int mpo_transport_reconstruct(mpo_transport* t, const double* u, double* sigma, double* omega,
+                              double* traces)
+{
+    CALL(field_traces(t, u, sigma, omega));
+    if (traces) memcpy(traces, t->traces, (size_t)t->nx * t->ny * 4 * t->q * sizeof(double));
+    return MPO_OK;
+}
+
+static mpo_point axis_velocity(double vn, int vertical)
+{
+    mpo_point v = {{vertical ? vn : 0.0, vertical ? 0.0 : vn}};
+    return v;
+}
+
+/* Interior and boundary face fluxes for one field from the stored traces.
+ * Face loop and LLF integration: src/mlweno/test/transport_test_support.h:278-306,
+ * src/mlweno/advectiveflux.cpp:192-227; boundary policy
+ * src/couple/initialization.cpp:469-489 with src/mlweno/advectiveflux.cpp:229-268. */
+static int face_fluxes(mpo_transport* t, int field)
+{
+    const int nx = t->nx, ny = t->ny, nq = t->q;
+    int err = 0;
+    char msg[256] = {0};
+    set_threads(t);
+    /* vertical faces: x_i, i = 0..nx; the edge is cell (i-1,j)'s right side (CCW) */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i <= nx && !err; ++i) {
+            mpo_point vel[16];
+            double left[16], right[16], flux = 0;
+            int e = 0;
+            if (i > 0 && i < nx) {
+                mpo_point edge[2];
+                canonical_face(t, i - 1, j, 1, edge);
+                for (int q = 0; q < nq; ++q) {
+                    vel[q] = axis_velocity(t->vnx[((size_t)q * ny + j) * (nx + 1) + i], 1);
+                    left[q] = t->traces[(cidx(t, i - 1, j) * 4 + 1) * nq + q];
+                    right[q] = t->traces[(cidx(t, i, j) * 4 + 3) * nq + q];
+                }
+                e = mpo_integrate_edge_flux(edge, nq, t->qp, t->qw, vel, left, right, 0, 0, &flux);
+            } else {
+                /* boundary: interior cell's CCW edge, outward normal */
+                const int side = i == 0 ? 3 : 1;
+                const int ci = i == 0 ? 0 : nx - 1;
+                mpo_point c4[4], edge[2];
+                mpo_cell_corners(&t->mesh, ci, j, c4);
+                if (side == 1) { edge[0] = c4[1]; edge[1] = c4[2]; }
+                else { edge[0] = c4[3]; edge[1] = c4[0]; }
+                const int rev = side == 3;     /* left edge runs along -y */
+                int types[16]; double vals[16], inflow[16];
+                const double* src = t->inflow[field][side];
+                for (int q = 0; q < nq; ++q) {
+                    const int qc = rev ? nq - 1 - q : q;
+                    vel[q] = axis_velocity(t->vnx[((size_t)qc * ny + j) * (nx + 1) + i], 1);
+                    left[q] = t->traces[(cidx(t, ci, j) * 4 + side) * nq + qc];
+                    inflow[q] = src ? src[(size_t)j * nq + qc] : NAN;
+                }
+                double length; mpo_point normal;
+                e = mpo_edge_geometry(edge, &length, &normal);
+                const int policy = t->bc[side][j];
+                for (int q = 0; q < nq && !e; ++q) {
+                    const double speed = vel[q].p[0] * normal.p[0] + vel[q].p[1] * normal.p[1];
+                    if (policy == MPO_BC_ZERO || speed == 0) { types[q] = 2; vals[q] = 0; }
+                    else if (speed < 0) {
+                        if (policy != MPO_BC_INFLOW_OUTFLOW) e = fail(MPO_ERR_BACKFLOW, "Backflow encountered on an outflow-only boundary.");
+                        else if (!src) e = fail(MPO_ERR_ARG, "Missing inflow profile.");
+                        else { types[q] = 0; vals[q] = inflow[q]; }
+                    } else { types[q] = 1; vals[q] = 0; }
+                }
+                if (!e) e = mpo_integrate_boundary_flux(edge, nq, t->qp, t->qw, vel, left, types, vals, &flux);
+                if (!e && side == 3) flux = -flux;   /* store along +x */
+            }
+            if (e) {
+#pragma omp critical
+                { if (!err) { err = e; snprintf(msg, sizeof msg, "%s", g_error); } }
+            }
+            t->fv[(size_t)j * (nx + 1) + i] = flux;
+        }
+    }
+    /* horizontal faces: y_j, j = 0..ny; the edge is cell (i,j-1)'s top side (CCW, -x) */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j <= ny; ++j) {
+        for (int i = 0; i < nx && !err; ++i) {
+            mpo_point vel[16];
+            double left[16], right[16], flux = 0;
+            int e = 0;
+            if (j > 0 && j < ny) {
+                mpo_point c4[4], edge[2];
+                mpo_cell_corners(&t->mesh, i, j - 1, c4);
+                edge[0] = c4[2]; edge[1] = c4[3];
+                for (int q = 0; q < nq; ++q) {
+                    const int qc = nq - 1 - q;
+                    vel[q] = axis_velocity(t->vny[((size_t)qc * (ny + 1) + j) * nx + i], 0);
+                    left[q] = t->traces[(cidx(t, i, j - 1) * 4 + 2) * nq + qc];
+                    right[q] = t->traces[(cidx(t, i, j) * 4 + 0) * nq + qc];
+                }
+                e = mpo_integrate_edge_flux(edge, nq, t->qp, t->qw, vel, left, right, 0, 0, &flux);
+            } else {
+                const int side = j == 0 ? 0 : 2;
+                const int cj = j == 0 ? 0 : ny - 1;
+                mpo_point c4[4], edge[2];
+                mpo_cell_corners(&t->mesh, i, cj, c4);
+                if (side == 0) { edge[0] = c4[0]; edge[1] = c4[1]; }
+                else { edge[0] = c4[2]; edge[1] = c4[3]; }
+                const int rev = side == 2;     /* top edge runs along -x */
+                int types[16]; double vals[16], inflow[16];
+                const double* src = t->inflow[field][side];
+                for (int q = 0; q < nq; ++q) {
+                    const int qc = rev ? nq - 1 - q : q;
+                    vel[q] = axis_velocity(t->vny[((size_t)qc * (ny + 1) + j) * nx + i], 0);
+                    left[q] = t->traces[(cidx(t, i, cj) * 4 + side) * nq + qc];
+                    inflow[q] = src ? src[(size_t)i * nq + qc] : NAN;
+                }
+                double length; mpo_point normal;
+                e = mpo_edge_geometry(edge, &length, &normal);
+                const int policy = t->bc[side][i];
+                for (int q = 0; q < nq && !e; ++q) {
+                    const double speed = vel[q].p[0] * normal.p[0] + vel[q].p[1] * normal.p[1];
+                    if (policy == MPO_BC_ZERO || speed == 0) { types[q] = 2; vals[q] = 0; }
+                    else if (speed < 0) {
+                        if (policy != MPO_BC_INFLOW_OUTFLOW) e = fail(MPO_ERR_BACKFLOW, "Backflow encountered on an outflow-only boundary.");
+                        else if (!src) e = fail(MPO_ERR_ARG, "Missing inflow profile.");
+                        else { types[q] = 0; vals[q] = inflow[q]; }
+                    } else { types[q] = 1; vals[q] = 0; }
+                }
+                if (!e) e = mpo_integrate_boundary_flux(edge, nq, t->qp, t->qw, vel, left, types, vals, &flux);
+                if (!e && side == 0) flux = -flux;   /* store along +y */
+            }
+            if (e) {
+#pragma omp critical
+                { if (!err) { err = e; snprintf(msg, sizeof msg, "%s", g_error); } }
+            }
+            t->fh[(size_t)j * nx + i] = flux;
+        }
+    }
+    if (err) return fail(err, msg);
+    return MPO_OK;
+}
+
+/* Gather form of the +-flux/area scatter, src/couple/preheat.cpp:193-196 */
+static void gather_rhs(const mpo_transport* t, double* rhs, double* bflux)
+{
+    const int nx = t->nx, ny = t->ny;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const double in = t->fv[(size_t)j * (nx + 1) + i] - t->fv[(size_t)j * (nx + 1) + i + 1]
+                            + t->fh[(size_t)j * nx + i] - t->fh[(size_t)(j + 1) * nx + i];
+            rhs[cidx(t, i, j)] = in / t->area[cidx(t, i, j)];
+        }
+    /* outward boundary total, fixed order */
+    long double b = 0;
+    for (int j = 0; j < ny; ++j) b += t->fv[(size_t)j * (nx + 1) + nx] - t->fv[(size_t)j * (nx + 1)];
+    for (int i = 0; i < nx; ++i) b += t->fh[(size_t)ny * nx + i] - t->fh[i];
+    *bflux = (double)b;
+}
+
+int mpo_transport_rhs(mpo_transport* t, const double* H, const double* C, double* rhsH,
+                      double* rhsC, double* bfluxH, double* bfluxC)
+{
+    double bh = 0, bc = 0;
+    CALL(field_traces(t, H, NULL, NULL));
+    CALL(face_fluxes(t, 0));
+    gather_rhs(t, rhsH, &bh);
+    CALL(field_traces(t, C, NULL, NULL));
+    CALL(face_fluxes(t, 1));
+    gather_rhs(t, rhsC, &bc);
+    if (bfluxH) *bfluxH = bh;
+    if (bfluxC) *bfluxC = bc;
+    return MPO_OK;
+}
+
+/* Per-cell rate, src/couple/preheat.cpp:125-152 (advection part) */
+int mpo_transport_rates(mpo_transport* t, double* rate, double* max_rate)
+{
+    const int nx = t->nx, ny = t->ny, nq = t->q;
+    const size_t N = (size_t)nx * ny;
+    double* r = (double*)calloc(N, sizeof(double));
+    if (!r) return fail(MPO_ERR_MEM, "out of memory");
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i <= nx; ++i) {
+            mpo_point e[2];
+            const int ci = i < nx ? i : nx - 1;
+            canonical_face(t, ci, j, i < nx ? 3 : 1, e);
+            double length; mpo_point n;
+            CALL(mpo_edge_geometry(e, &length, &n));
+            double speed = 0;
+            for (int q = 0; q < nq; ++q)
+                speed += .5 * length * t->qw[q] * fabs(t->vnx[((size_t)q * ny + j) * (nx + 1) + i]);
+            if (i > 0) r[cidx(t, i - 1, j)] += speed / t->area[cidx(t, i - 1, j)];
+            if (i < nx) r[cidx(t, i, j)] += speed / t->area[cidx(t, i, j)];
+        }
+    for (int j = 0; j <= ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            mpo_point e[2];
+            const int cj = j < ny ? j : ny - 1;
+            canonical_face(t, i, cj, j < ny ? 0 : 2, e);
+            double length; mpo_point n;
+            CALL(mpo_edge_geometry(e, &length, &n));
+            double speed = 0;
+            for (int q = 0; q < nq; ++q)
+                speed += .5 * length * t->qw[q] * fabs(t->vny[((size_t)q * (ny + 1) + j) * nx + i]);
+            if (j > 0) r[cidx(t, i, j - 1)] += speed / t->area[cidx(t, i, j - 1)];
+            if (j < ny) r[cidx(t, i, j)] += speed / t->area[cidx(t, i, j)];
+        }
+    double mx = 0;
+    for (size_t c = 0; c < N; ++c) if (r[c] > mx) mx = r[c];
+    if (rate) memcpy(rate, r, N * sizeof(double));
+    free(r);
+    if (max_rate) *max_rate = mx;
+    return MPO_OK;
+}
+
+/* Neumaier compensated sum of area*u, fixed order. */
+double mpo_transport_integral(const mpo_transport* t, const double* u)
+{
+    const size_t N = (size_t)t->nx * t->ny;
+    double s = 0, c = 0;
+    for (size_t k = 0; k < N; ++k) {
+        const double x = t->area[k] * u[k];
+        const double y = s + x;
+        c += fabs(s) >= fabs(x) ? (s - y) + x : (x - y) + s;
+        s = y;
+    }
+    return s + c;
+}
+
+/* Stage combination u_out = a*u_n + b*(u_k + dt*L).  Forward Euler and SSPRK2
+ * follow src/couple/preheat.cpp:267 and src/couple/evolve.cpp:469-479; SSPRK3
+ * (Shu-Osher) is this build's addition. */
+static void combine(size_t N, double a, const double* un, double b, const double* uk,
+                    double dt, const double* L, double* out)
+{
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < N; ++k) {
+        const double w = uk[k] + dt * L[k];
+        out[k] = a == 0.0 ? b * w : a * un[k] + b * w;
+    }
+}
+
+int mpo_transport_advance(mpo_transport* t, double* H, double* C, int nsteps, double dt,
+                          int scheme, double* totals)
+{
+    CHECK(scheme >= 1 && scheme <= 3, MPO_ERR_ARG, "scheme must be 1, 2 or 3");
+    CHECK(finite_d(dt) && dt > 0, MPO_ERR_ARG, "dt must be finite and positive");
+    const size_t N = (size_t)t->nx * t->ny;
+    double* buf = (double*)malloc(N * 8 * sizeof(double));
+    if (!buf) return fail(MPO_ERR_MEM, "out of memory");
+    double *H1 = buf, *C1 = buf + N, *H2 = buf + 2 * N, *C2 = buf + 3 * N;
+    double *LH = buf + 4 * N, *LC = buf + 5 * N, *Hn = buf + 6 * N, *Cn = buf + 7 * N;
+    static const double A[3][3] = {{0, 0, 0}, {0, 0.5, 0}, {0, 0.75, 1.0 / 3.0}};
+    static const double B[3][3] = {{1, 0, 0}, {1, 0.5, 0}, {1, 0.25, 2.0 / 3.0}};
+    static const double W[3][3] = {{1, 0, 0}, {0.5, 0.5, 0}, {1.0 / 6, 1.0 / 6, 2.0 / 3}};
+    int e = 0;
+    set_threads(t);
+    for (int s = 0; s < nsteps && !e; ++s) {
+        memcpy(Hn, H, N * sizeof(double)); memcpy(Cn, C, N * sizeof(double));
+        const double* hk = Hn; const double* ck = Cn;
+        double bh_total = 0, bc_total = 0;
+        for (int st = 0; st < scheme && !e; ++st) {
+            double bh, bc;
+            e = mpo_transport_rhs(t, hk, ck, LH, LC, &bh, &bc);
+            if (e) break;
+            double* ho = st == scheme - 1 ? H : (st % 2 == 0 ? H1 : H2);
+            double* co = st == scheme - 1 ? C : (st % 2 == 0 ? C1 : C2);
+            combine(N, A[scheme - 1][st], Hn, B[scheme - 1][st], hk, dt, LH, ho);
+            combine(N, A[scheme - 1][st], Cn, B[scheme - 1][st], ck, dt, LC, co);
+            bh_total += W[scheme - 1][st] * bh; bc_total += W[scheme - 1][st] * bc;
+            hk = ho; ck = co;
+        }
+        if (!e && totals) {
+            totals[4 * s + 0] = mpo_transport_integral(t, H);
+            totals[4 * s + 1] = mpo_transport_integral(t, C);
+            totals[4 * s + 2] = bh_total;
+            totals[4 * s + 3] = bc_total;
+        }
+    }
+    free(buf);
+    return e;
+}
+
+uint64_t mpo_mix64(uint64_t v)
+{
+    v += UINT64_C(0x9e3779b97f4a7c15);
+    v = (v ^ (v >> 30)) * UINT64_C(0xbf58476d1ce4e5b9);
+    v = (v ^ (v >> 27)) * UINT64_C(0x94d049bb133111eb);
+    return v ^ (v >> 31);
+}

@@ -1,0 +1,247 @@
+/*
+ * mp_oracle.h -- CPU restatement of MantlePar's H/C transport step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness checker and the
+ * timed CPU baseline for the HIP transport library.  Only tests/, the smoke
+ * check in __graft_entry__.py and the cpu_baseline leg of bench.py may load
+ * it.  The product path (mantlepar_amd) never links or calls it.
+ *
+ * Every function restates the reference algorithm from its behaviour (C11,
+ * long double where the reference accumulates in long double).  Citations are
+ * path:line in aband/MantlePar.
+ *
+ *   geometry      src/core/integral.cpp:7-121, src/core/quad_validation.h:15-46
+ *   stencil       src/mlweno/tensorstencilpoly.cpp:61-459
+ *   weno          src/mlweno/reconstruction.cpp:43-355, 474-502
+ *   face flux     src/mlweno/advectiveflux.cpp:20-164, 192-268, 394-437
+ *   operator      src/mlweno/test/transport_test_support.h:253-345 (face loop),
+ *                 src/couple/preheat.cpp:125-203 (rates, RHS scatter),
+ *                 src/couple/state_reconstruction.cpp:64-75 (production options)
+ *   phase         src/phase/eutectic_rescaled.cpp:42-282
+ *
+ * Status codes: 0 = success; nonzero = the error category below.  Outputs are
+ * written only on success, as in the reference ("checked output arguments are
+ * assigned only on success", src/mlweno/reconstruction.h:61-62).
+ */
+#ifndef MP_ORACLE_H
+#define MP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MPO_OK = 0,
+    MPO_ERR_ARG = 1,        /* invalid argument (PETSC_ERR_ARG_*, std::invalid_argument) */
+    MPO_ERR_FP = 2,         /* non-finite / unrepresentable (PETSC_ERR_FP, overflow_error) */
+    MPO_ERR_STATE = 3,      /* wrong object state */
+    MPO_ERR_RUNTIME = 4,    /* runtime_error (phase fraction outside [0,1]) */
+    MPO_ERR_LIB = 5,        /* singular moment solve (PETSC_ERR_LIB) */
+    MPO_ERR_MEM = 6,
+    MPO_ERR_BACKFLOW = 7    /* backflow on an outflow-only boundary */
+};
+
+const char* mpo_last_error(void);
+
+/* ---------------------------------------------------------------- geometry */
+typedef struct { double p[2]; } mpo_point;
+
+int mpo_gauss_rule(int n, double* points, double* weights);
+mpo_point mpo_map_edge_point(double ref, const mpo_point edge[2]);
+mpo_point mpo_map_cell_point(mpo_point ref, const mpo_point corners[4]);
+double mpo_cell_jacobian(mpo_point ref, const mpo_point corners[4]);
+int mpo_edge_geometry(const mpo_point edge[2], double* length, mpo_point* normal);
+int mpo_validate_quad(const mpo_point corners[4]);
+
+/* ------------------------------------------------------- TensorStencilPoly */
+#define MPO_MAX_STENCIL 12
+
+typedef struct {
+    int start_i, start_j, size_i, size_j, count, order;
+    mpo_point center;
+    double h;
+    double coef[MPO_MAX_STENCIL * MPO_MAX_STENCIL];   /* basis-major: coef[cell*n+k] */
+    double sigma[MPO_MAX_STENCIL * MPO_MAX_STENCIL];  /* reference-square indicator */
+    int target_index;                                 /* -1: no TargetCell matrix */
+    double target_sigma[MPO_MAX_STENCIL * MPO_MAX_STENCIL];
+    mpo_point corners[MPO_MAX_STENCIL][4];
+} mpo_stencil;
+
+int mpo_stencil_init(mpo_stencil* s, const mpo_point (*corners)[4], int start_i, int start_j,
+                     int size_i, int size_j, int order, double h);
+int mpo_stencil_set_target(mpo_stencil* s, int local_i, int local_j);
+int mpo_stencil_smoothness(const mpo_stencil* s, const double* averages, int use_target,
+                           double* value);
+int mpo_stencil_eval(const mpo_stencil* s, const double* averages, mpo_point x, int dx,
+                     int dy, double* value);
+
+/* ----------------------------------------------------------- Reconstruction */
+#define MPO_MAX_CAND 8
+
+typedef struct {
+    int size_i, size_j, order;
+    int noffsets;
+    int off_i[MPO_MAX_CAND], off_j[MPO_MAX_CAND];
+    double linear_weight[MPO_MAX_CAND];   /* used when has_weights */
+    int has_weights;
+    int target_smoothness;                /* 0 ReferenceSquare, 1 TargetCell */
+} mpo_family;
+
+typedef struct {
+    mpo_family large, small;
+    int use_constant;
+    double constant_linear_weight;
+    double epsilon;
+    int s;
+} mpo_weno_options;
+
+/* Production options: src/couple/state_reconstruction.cpp:64-68. */
+void mpo_production_options(mpo_weno_options* o);
+
+typedef struct {
+    int family;             /* 0 large, 1 small */
+    int configured_index;
+    int linear_index;       /* index into the family's offsets */
+    double linear_weight;
+    const mpo_stencil* stencil;   /* geometry tables (owned elsewhere) */
+    int start_i, start_j;         /* candidate stencil origin (ReconstructionCandidateInfo) */
+    mpo_point center;             /* centre of this candidate's stencil */
+    double h;
+    double centered[MPO_MAX_STENCIL];
+    double smoothness, weight;
+} mpo_candidate;
+
+typedef struct {
+    int nx, ny;             /* mesh cell dimensions */
+    int ti, tj;             /* target */
+    int req_bi, req_bj, req_ei, req_ej;   /* RequiredCells (half open) */
+    mpo_weno_options opt;
+    int ncand;
+    mpo_candidate cand[MPO_MAX_CAND];
+    int has_weights;
+    double target_average, constant_weight, area;
+} mpo_recon;
+
+/* Mesh: (ny+1)*(nx+1) vertices, coords[(j*(nx+1)+i)*2 + d]. */
+typedef struct {
+    int nx, ny;
+    const double* coords;
+} mpo_mesh;
+
+void mpo_cell_corners(const mpo_mesh* m, int i, int j, mpo_point c[4]);
+double mpo_cell_area(const mpo_mesh* m, int i, int j);
+
+/* Initialize: stencil tables are built into `storage` (caller-provided, at
+ * least mpo_recon_storage_count(opt) entries) unless `shared` is non-null, in
+ * which case it uses shared[slot].  slot = k for large offset k and
+ * large.noffsets + k for small offset k. */
+int mpo_recon_storage_count(const mpo_weno_options* opt);
+int mpo_recon_init(mpo_recon* r, const mpo_mesh* mesh, int ti, int tj,
+                   const mpo_weno_options* opt, double h, mpo_stencil* storage,
+                   const mpo_stencil* const* shared);
+int mpo_recon_update(mpo_recon* r, const double* values, int bi, int bj, int ei, int ej,
+                     double area);
+int mpo_recon_eval(const mpo_recon* r, mpo_point x, int dx, int dy, double* value);
+
+/* ------------------------------------------------------------ advective flux */
+/* Local LF (mode 0) or global LF (mode 1) for the linear law F_n(u)=(v.n)u. */
+int mpo_llf_point(double left, double right, mpo_point velocity, mpo_point normal, int mode,
+                  double global_speed, double* flux, double* local_speed);
+int mpo_integrate_edge_flux(const mpo_point edge[2], int nq, const double* qp,
+                            const double* qw, const mpo_point* velocity, const double* left,
+                            const double* right, int mode, double global_speed,
+                            double* flux);
+/* Boundary types: 0 prescribed, 1 extrapolated, 2 zero flux. */
+int mpo_boundary_data(const mpo_point edge[2], int nq, const double* qp, const double* qw,
+                      const mpo_point* velocity, const double* inflow_values, int* types,
+                      double* values);
+int mpo_integrate_boundary_flux(const mpo_point edge[2], int nq, const double* qp,
+                                const double* qw, const mpo_point* velocity,
+                                const double* interior, const int* types,
+                                const double* values, double* flux);
+
+/* -------------------------------------------------------------------- phase */
+typedef struct {
+    double Tm0, Te0, nu, L, cp, Xe, rho, rhor, mus, mul, k0, g, alpha0;
+} mpo_material;
+
+typedef struct {
+    double dT, h0, gamma, LD, TDm0, TDe0, invk0, l0, u0, p0, t0;
+} mpo_derived;
+
+typedef struct {
+    mpo_material p;
+    mpo_derived d;
+} mpo_phase_model;
+
+typedef struct { double Tep, Tmp, H_eutectic_end, H_liquidus; } mpo_phase_bounds;
+
+typedef struct {
+    double phi1, phi2, phil, cl, cs;
+    int region;           /* 1 PureSolid .. 6 PureMelting (PhaseRegion values) */
+    double TDp, Tep, dTD_dCD, dTD_dHD;
+    int has_solid, has_liquid, finite_temperature_derivatives;
+} mpo_phase_state;
+
+void mpo_material_defaults(mpo_material* p);
+int mpo_phase_set_parameters(mpo_phase_model* m, const mpo_material* p);
+int mpo_phase_boundaries(const mpo_phase_model* m, double C, double P, mpo_phase_bounds* b);
+int mpo_phase_evaluate(const mpo_phase_model* m, double H, double C, double P,
+                       mpo_phase_state* s);
+int mpo_phase_classify(const mpo_phase_model* m, double H, double C, double P, int* region);
+/* Hypereutectic extension (no reference behaviour): for C > Xe evaluates the
+ * mirrored diagram C -> 1-C, Xe -> 1-Xe and maps phases back. */
+int mpo_phase_evaluate_ext(const mpo_phase_model* m, double H, double C, double P,
+                           mpo_phase_state* s);
+/* Batched form used by the transport operator and by tests. */
+int mpo_phase_batch(const mpo_phase_model* m, int64_t n, const double* H, const double* C,
+                    double P, int hyper, double* T, double* phi, double* Cl, double* Cs,
+                    uint8_t* region);
+
+/* -------------------------------------------------------- transport operator */
+enum { MPO_BC_ZERO = 0, MPO_BC_OUTFLOW = 1, MPO_BC_INFLOW_OUTFLOW = 2 };
+
+typedef struct mpo_transport mpo_transport;
+
+/* vnx: [q][ny][nx+1] normal velocity on vertical faces along +x;
+ * vny: [q][ny+1][nx] normal velocity on horizontal faces along +y.
+ * Samples follow the ascending Gauss points along +y resp. +x.
+ * bc_policy[side*n + k]: per boundary face (side 0 bottom, 1 right, 2 top,
+ * 3 left; k runs along +x or +y).  inflow_{H,C}: [side][k][q] or NULL.
+ * table_mode 0: per-cell stencil tables; 1: one table per candidate type
+ * (valid for uniform rectangular meshes). */
+int mpo_transport_create(mpo_transport** out, const mpo_mesh* mesh, int q,
+                         const mpo_weno_options* opt, const double* vnx, const double* vny,
+                         const uint8_t* bc_bottom, const uint8_t* bc_right,
+                         const uint8_t* bc_top, const uint8_t* bc_left,
+                         const double* inflow_H, const double* inflow_C, int table_mode,
+                         int threads);
+void mpo_transport_destroy(mpo_transport* t);
+/* Per-cell CFL rate (src/couple/preheat.cpp:125-152) and dt = cfl/max rate. */
+int mpo_transport_rates(mpo_transport* t, double* rate, double* max_rate);
+/* One spatial operator evaluation L(u) for both fields, and boundary flux totals. */
+int mpo_transport_rhs(mpo_transport* t, const double* H, const double* C, double* rhsH,
+                      double* rhsC, double* bfluxH, double* bfluxC);
+/* Reconstruction diagnostics per cell: sigma[5], omega[5] (candidate slots:
+ * large, small offsets 0..3; 0 for discarded) and traces at the 4 faces' Gauss
+ * points [N][4][q] in face order bottom,right,top,left, Gauss order along +x/+y. */
+int mpo_transport_reconstruct(mpo_transport* t, const double* u, double* sigma,
+                              double* omega, double* traces);
+/* Advance nsteps with the given scheme (1 forward Euler, 2 SSPRK2, 3 SSPRK3)
+ * at fixed dt.  totals[4*s+{0,1,2,3}] = sum A*H, sum A*C, step boundary flux H,
+ * C (stage-weighted), for each step s (may be NULL). */
+int mpo_transport_advance(mpo_transport* t, double* H, double* C, int nsteps, double dt,
+                          int scheme, double* totals);
+/* Compensated (Neumaier) sum of area*u in fixed order. */
+double mpo_transport_integral(const mpo_transport* t, const double* u);
+
+/* ------------------------------------------------------------ workloads */
+/* SplitMix64 finaliser of src/core/mesh.cpp:12-18. */
+uint64_t mpo_mix64(uint64_t v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,507 @@
+"""ctypes bindings for the C oracle (test infrastructure only)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_mp.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref_phase.so")
+
+__all__ = [
+    "lib", "OracleError", "build", "Material", "PhaseModel", "Transport", "gauss_rule",
+    "ref_phase_available", "ref_phase_evaluate", "ref_phase_derived", "WenoOptions",
+    "StencilPoly", "Recon", "llf_point", "integrate_edge_flux", "REGION",
+]
+
+REGION = {"PureSolid": 1, "TwoSolids": 2, "Eutectic": 3, "SolidLiquid": 4, "Liquid": 5,
+          "PureMelting": 6}
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"oracle error {code}: {message}")
+        self.code = code
+
+
+def build(ref=False):
+    """Compile the oracle (and the reference phase library when ``ref``)."""
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+    if ref:
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class Point(C.Structure):
+    _fields_ = [("p", C.c_double * 2)]
+
+
+class Material(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("Tm0", "Te0", "nu", "L", "cp", "Xe", "rho", "rhor", "mus", "mul", "k0", "g",
+                 "alpha0")]
+
+    @classmethod
+    def defaults(cls, **kw):
+        m = cls()
+        lib().mpo_material_defaults(C.byref(m))
+        for k, v in kw.items():
+            setattr(m, k, v)
+        return m
+
+    def as_array(self):
+        return np.array([getattr(self, n) for n, _ in self._fields_], dtype=np.float64)
+
+
+class Derived(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("dT", "h0", "gamma", "LD", "TDm0", "TDe0", "invk0", "l0", "u0", "p0", "t0")]
+
+
+class _PhaseModel(C.Structure):
+    _fields_ = [("p", Material), ("d", Derived)]
+
+
+class Bounds(C.Structure):
+    _fields_ = [("Tep", C.c_double), ("Tmp", C.c_double), ("H_eutectic_end", C.c_double),
+                ("H_liquidus", C.c_double)]
+
+
+class PhaseState(C.Structure):
+    _fields_ = [("phi1", C.c_double), ("phi2", C.c_double), ("phil", C.c_double),
+                ("cl", C.c_double), ("cs", C.c_double), ("region", C.c_int),
+                ("TDp", C.c_double), ("Tep", C.c_double), ("dTD_dCD", C.c_double),
+                ("dTD_dHD", C.c_double), ("has_solid", C.c_int), ("has_liquid", C.c_int),
+                ("finite_temperature_derivatives", C.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+MAX_STENCIL = 12
+MAX_CAND = 8
+
+
+class Stencil(C.Structure):
+    _fields_ = [("start_i", C.c_int), ("start_j", C.c_int), ("size_i", C.c_int),
+                ("size_j", C.c_int), ("count", C.c_int), ("order", C.c_int),
+                ("center", Point), ("h", C.c_double),
+                ("coef", C.c_double * (MAX_STENCIL * MAX_STENCIL)),
+                ("sigma", C.c_double * (MAX_STENCIL * MAX_STENCIL)),
+                ("target_index", C.c_int),
+                ("target_sigma", C.c_double * (MAX_STENCIL * MAX_STENCIL)),
+                ("corners", (Point * 4) * MAX_STENCIL)]
+
+
+class Family(C.Structure):
+    _fields_ = [("size_i", C.c_int), ("size_j", C.c_int), ("order", C.c_int),
+                ("noffsets", C.c_int), ("off_i", C.c_int * MAX_CAND),
+                ("off_j", C.c_int * MAX_CAND), ("linear_weight", C.c_double * MAX_CAND),
+                ("has_weights", C.c_int), ("target_smoothness", C.c_int)]
+
+
+class WenoOptions(C.Structure):
+    _fields_ = [("large", Family), ("small", Family), ("use_constant", C.c_int),
+                ("constant_linear_weight", C.c_double), ("epsilon", C.c_double),
+                ("s", C.c_int)]
+
+    @classmethod
+    def production(cls):
+        o = cls()
+        lib().mpo_production_options(C.byref(o))
+        return o
+
+
+class Candidate(C.Structure):
+    _fields_ = [("family", C.c_int), ("configured_index", C.c_int), ("linear_index", C.c_int),
+                ("linear_weight", C.c_double), ("stencil", C.POINTER(Stencil)),
+                ("start_i", C.c_int), ("start_j", C.c_int), ("center", Point), ("h", C.c_double),
+                ("centered", C.c_double * MAX_STENCIL), ("smoothness", C.c_double),
+                ("weight", C.c_double)]
+
+
+class _Recon(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("ti", C.c_int), ("tj", C.c_int),
+                ("req_bi", C.c_int), ("req_bj", C.c_int), ("req_ei", C.c_int),
+                ("req_ej", C.c_int), ("opt", WenoOptions), ("ncand", C.c_int),
+                ("cand", Candidate * MAX_CAND), ("has_weights", C.c_int),
+                ("target_average", C.c_double), ("constant_weight", C.c_double),
+                ("area", C.c_double)]
+
+
+class _Mesh(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("coords", C.POINTER(C.c_double))]
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def _declare(L):
+    L.mpo_last_error.restype = C.c_char_p
+    L.mpo_gauss_rule.argtypes = [C.c_int, _dp, _dp]
+    L.mpo_material_defaults.argtypes = [C.POINTER(Material)]
+    L.mpo_phase_set_parameters.argtypes = [C.POINTER(_PhaseModel), C.POINTER(Material)]
+    L.mpo_phase_boundaries.argtypes = [C.POINTER(_PhaseModel), C.c_double, C.c_double,
+                                       C.POINTER(Bounds)]
+    L.mpo_phase_evaluate.argtypes = [C.POINTER(_PhaseModel), C.c_double, C.c_double,
+                                     C.c_double, C.POINTER(PhaseState)]
+    L.mpo_phase_evaluate_ext.argtypes = L.mpo_phase_evaluate.argtypes
+    L.mpo_phase_classify.argtypes = [C.POINTER(_PhaseModel), C.c_double, C.c_double,
+                                     C.c_double, C.POINTER(C.c_int)]
+    L.mpo_phase_batch.argtypes = [C.POINTER(_PhaseModel), C.c_int64, _dp, _dp, C.c_double,
+                                  C.c_int, _dp, _dp, _dp, _dp, _u8p]
+    L.mpo_production_options.argtypes = [C.POINTER(WenoOptions)]
+    L.mpo_stencil_init.argtypes = [C.POINTER(Stencil), C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, C.c_int, C.c_double]
+    L.mpo_stencil_set_target.argtypes = [C.POINTER(Stencil), C.c_int, C.c_int]
+    L.mpo_stencil_smoothness.argtypes = [C.POINTER(Stencil), _dp, C.c_int,
+                                         C.POINTER(C.c_double)]
+    L.mpo_stencil_eval.argtypes = [C.POINTER(Stencil), _dp, Point, C.c_int, C.c_int,
+                                   C.POINTER(C.c_double)]
+    L.mpo_recon_storage_count.argtypes = [C.POINTER(WenoOptions)]
+    L.mpo_recon_init.argtypes = [C.POINTER(_Recon), C.POINTER(_Mesh), C.c_int, C.c_int,
+                                 C.POINTER(WenoOptions), C.c_double, C.c_void_p, C.c_void_p]
+    L.mpo_recon_update.argtypes = [C.POINTER(_Recon), _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_double]
+    L.mpo_recon_eval.argtypes = [C.POINTER(_Recon), Point, C.c_int, C.c_int,
+                                 C.POINTER(C.c_double)]
+    L.mpo_llf_point.argtypes = [C.c_double, C.c_double, Point, Point, C.c_int, C.c_double,
+                                C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.mpo_integrate_edge_flux.argtypes = [C.POINTER(Point), C.c_int, _dp, _dp,
+                                          C.POINTER(Point), _dp, _dp, C.c_int, C.c_double,
+                                          C.POINTER(C.c_double)]
+    L.mpo_transport_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(_Mesh), C.c_int,
+                                       C.POINTER(WenoOptions), _dp, _dp, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_int, C.c_int]
+    L.mpo_transport_destroy.argtypes = [C.c_void_p]
+    L.mpo_transport_rates.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_double)]
+    L.mpo_transport_rhs.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
+    L.mpo_transport_reconstruct.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    L.mpo_transport_advance.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int,
+                                        C.c_void_p]
+    L.mpo_transport_integral.argtypes = [C.c_void_p, _dp]
+    L.mpo_transport_integral.restype = C.c_double
+    L.mpo_mix64.argtypes = [C.c_uint64]
+    L.mpo_mix64.restype = C.c_uint64
+
+
+def _check(code):
+    if code:
+        raise OracleError(code, lib().mpo_last_error().decode())
+
+
+def gauss_rule(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    _check(lib().mpo_gauss_rule(n, x, w))
+    return x, w
+
+
+class PhaseModel:
+    """EutecticModel restatement (src/phase/eutectic_rescaled.h:103-136)."""
+
+    def __init__(self, material=None, **kw):
+        self.material = material if material is not None else Material.defaults(**kw)
+        self._m = _PhaseModel()
+        _check(lib().mpo_phase_set_parameters(C.byref(self._m), C.byref(self.material)))
+
+    @property
+    def derived(self):
+        return {n: getattr(self._m.d, n) for n, _ in Derived._fields_}
+
+    def boundaries(self, Cd, P=0.0):
+        b = Bounds()
+        _check(lib().mpo_phase_boundaries(C.byref(self._m), Cd, P, C.byref(b)))
+        return {n: getattr(b, n) for n, _ in Bounds._fields_}
+
+    def evaluate(self, H, Cd, P=0.0, hyper=False):
+        s = PhaseState()
+        f = lib().mpo_phase_evaluate_ext if hyper else lib().mpo_phase_evaluate
+        _check(f(C.byref(self._m), H, Cd, P, C.byref(s)))
+        return s.as_dict()
+
+    def classify(self, H, Cd, P=0.0):
+        r = C.c_int()
+        _check(lib().mpo_phase_classify(C.byref(self._m), H, Cd, P, C.byref(r)))
+        return r.value
+
+    def batch(self, H, Cd, P=0.0, hyper=True):
+        H = np.ascontiguousarray(H, dtype=np.float64).ravel()
+        Cd = np.ascontiguousarray(Cd, dtype=np.float64).ravel()
+        n = H.size
+        T, phi, cl, cs = (np.zeros(n) for _ in range(4))
+        reg = np.zeros(n, dtype=np.uint8)
+        _check(lib().mpo_phase_batch(C.byref(self._m), n, H, Cd, P, int(hyper), T, phi, cl, cs,
+                                     reg))
+        return T, phi, cl, cs, reg
+
+
+_ref = None
+
+
+def ref_phase_available():
+    return os.path.exists(REF_PATH)
+
+
+def _reflib():
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(REF_PATH)
+        _ref.ref_phase_evaluate.argtypes = [_dp, C.c_long, _dp, _dp, _dp, _dp,
+                                            np.ctypeslib.ndpointer(dtype=np.int32)]
+        _ref.ref_phase_derived.argtypes = [_dp, _dp]
+    return _ref
+
+
+def ref_phase_evaluate(material, H, Cd, P):
+    """The reference's own EutecticModel::evaluate on arrays (oracle/_ref)."""
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    Cd = np.ascontiguousarray(Cd, dtype=np.float64)
+    P = np.ascontiguousarray(np.broadcast_to(P, H.shape), dtype=np.float64)
+    out = np.zeros((H.size, 11))
+    status = np.zeros(H.size, dtype=np.int32)
+    rc = _reflib().ref_phase_evaluate(material.as_array(), H.size, H, Cd, P, out, status)
+    if rc:
+        raise OracleError(rc, "reference model construction failed")
+    return out, status
+
+
+def ref_phase_derived(material):
+    d = np.zeros(11)
+    rc = _reflib().ref_phase_derived(material.as_array(), d)
+    return rc, d
+
+
+
+
+def _corners_array(corners):
+    arr = (Point * 4 * MAX_STENCIL)()
+    for c, quad in enumerate(corners):
+        for k in range(4):
+            arr[c][k].p[0] = quad[k][0]
+            arr[c][k].p[1] = quad[k][1]
+    return arr
+
+
+class StencilPoly:
+    """TensorStencilPoly restatement (geometry-only Initialize)."""
+
+    def __init__(self, corners, start, size, order, h):
+        self._s = Stencil()
+        self._corners = _corners_array(corners)
+        _check(lib().mpo_stencil_init(C.byref(self._s), C.addressof(self._corners), start[0],
+                                      start[1], size[0], size[1], order, h))
+
+    def set_target(self, li, lj):
+        _check(lib().mpo_stencil_set_target(C.byref(self._s), li, lj))
+
+    def smoothness(self, averages, target=False):
+        v = C.c_double()
+        _check(lib().mpo_stencil_smoothness(C.byref(self._s),
+                                            np.ascontiguousarray(averages, dtype=np.float64),
+                                            int(target), C.byref(v)))
+        return v.value
+
+    def evaluate(self, averages, point, dx=0, dy=0):
+        v = C.c_double()
+        pt = Point()
+        pt.p[0], pt.p[1] = point
+        _check(lib().mpo_stencil_eval(C.byref(self._s),
+                                      np.ascontiguousarray(averages, dtype=np.float64), pt, dx,
+                                      dy, C.byref(v)))
+        return v.value
+
+    @property
+    def center(self):
+        return (self._s.center.p[0], self._s.center.p[1])
+
+    @property
+    def coefficients(self):
+        n = self._s.count
+        return np.array(self._s.coef[:n * n]).reshape(n, n)
+
+
+class Recon:
+    """Reconstruction restatement: Initialize/Update/Evaluate on a vertex mesh."""
+
+    def __init__(self, coords, nx, ny, target, options=None, h=None):
+        self._coords = np.ascontiguousarray(coords, dtype=np.float64)
+        self._mesh = _Mesh(nx, ny, self._coords.ctypes.data_as(C.POINTER(C.c_double)))
+        self.options = options if options is not None else WenoOptions.production()
+        n = lib().mpo_recon_storage_count(C.byref(self.options))
+        self._storage = (Stencil * max(n, 1))()
+        self._r = _Recon()
+        if h is None:
+            h = 1.0
+        _check(lib().mpo_recon_init(C.byref(self._r), C.byref(self._mesh), target[0], target[1],
+                                    C.byref(self.options), h, C.addressof(self._storage), None))
+
+    @property
+    def candidate_count(self):
+        return self._r.ncand
+
+    def candidate(self, k):
+        c = self._r.cand[k]
+        s = c.stencil.contents
+        return {"family": c.family, "configured_index": c.configured_index,
+                "linear_weight": c.linear_weight, "smoothness": c.smoothness,
+                "weight": c.weight, "start": (c.start_i, c.start_j),
+                "size": (s.size_i, s.size_j), "order": s.order}
+
+    @property
+    def required(self):
+        r = self._r
+        return (r.req_bi, r.req_bj, r.req_ei, r.req_ej)
+
+    @property
+    def constant_weight(self):
+        return self._r.constant_weight
+
+    def update(self, values, rng, area):
+        _check(lib().mpo_recon_update(C.byref(self._r),
+                                      np.ascontiguousarray(values, dtype=np.float64).ravel(),
+                                      rng[0], rng[1], rng[2], rng[3], area))
+
+    def evaluate(self, point, dx=0, dy=0):
+        v = C.c_double()
+        pt = Point()
+        pt.p[0], pt.p[1] = point
+        _check(lib().mpo_recon_eval(C.byref(self._r), pt, dx, dy, C.byref(v)))
+        return v.value
+
+
+def llf_point(left, right, velocity, normal, mode=0, global_speed=0.0):
+    v = Point(); n = Point()
+    v.p[0], v.p[1] = velocity
+    n.p[0], n.p[1] = normal
+    f = C.c_double()
+    s = C.c_double()
+    _check(lib().mpo_llf_point(left, right, v, n, mode, global_speed, C.byref(f), C.byref(s)))
+    return f.value
+
+
+def integrate_edge_flux(edge, velocity, left, right, nq=5, mode=0, global_speed=0.0):
+    x, w = gauss_rule(nq)
+    e = (Point * 2)()
+    for k in range(2):
+        e[k].p[0], e[k].p[1] = edge[k]
+    vel = (Point * nq)()
+    for q in range(nq):
+        vel[q].p[0], vel[q].p[1] = velocity[q]
+    f = C.c_double()
+    _check(lib().mpo_integrate_edge_flux(e, nq, x, w, vel,
+                                         np.ascontiguousarray(left, dtype=np.float64),
+                                         np.ascontiguousarray(right, dtype=np.float64), mode,
+                                         global_speed, C.byref(f)))
+    return f.value
+
+
+def _u8_or_none(a):
+    if a is None:
+        return None, None
+    arr = np.ascontiguousarray(a, dtype=np.uint8)
+    return arr, arr.ctypes.data_as(C.c_void_p)
+
+
+class Transport:
+    """Reference-faithful H/C transport operator on a structured quad mesh.
+
+    vnx: [q][ny][nx+1] normal velocity along +x at vertical-face Gauss points,
+    vny: [q][ny+1][nx] normal velocity along +y at horizontal-face Gauss points,
+    both in ascending Gauss order along +y resp. +x.  bc: dict side->uint8 array
+    of per-face policies (0 zero flux, 1 outflow only, 2 inflow/outflow).
+    """
+
+    def __init__(self, coords, nx, ny, vnx, vny, q=5, options=None, bc=None, inflow_H=None,
+                 inflow_C=None, table_mode=0, threads=0):
+        self.nx, self.ny, self.q = nx, ny, q
+        self._coords = np.ascontiguousarray(coords, dtype=np.float64).ravel()
+        self._mesh = _Mesh(nx, ny, self._coords.ctypes.data_as(C.POINTER(C.c_double)))
+        self.options = options if options is not None else WenoOptions.production()
+        bc = bc or {}
+        keep = []
+        ptrs = []
+        for side in ("bottom", "right", "top", "left"):
+            arr, ptr = _u8_or_none(bc.get(side))
+            keep.append(arr)
+            ptrs.append(ptr)
+        infl = []
+        for a in (inflow_H, inflow_C):
+            if a is None:
+                infl.append(None)
+            else:
+                arr = np.ascontiguousarray(a, dtype=np.float64).ravel()
+                keep.append(arr)
+                infl.append(arr.ctypes.data_as(C.c_void_p))
+        self._keep = keep
+        h = C.c_void_p()
+        _check(lib().mpo_transport_create(
+            C.byref(h), C.byref(self._mesh), q, C.byref(self.options),
+            np.ascontiguousarray(vnx, dtype=np.float64).ravel(),
+            np.ascontiguousarray(vny, dtype=np.float64).ravel(), *ptrs, infl[0], infl[1],
+            table_mode, threads))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().mpo_transport_destroy(h)
+            self._h = None
+
+    @property
+    def N(self):
+        return self.nx * self.ny
+
+    def rates(self):
+        r = np.zeros(self.N)
+        m = C.c_double()
+        _check(lib().mpo_transport_rates(self._h, r, C.byref(m)))
+        return r.reshape(self.ny, self.nx), m.value
+
+    def rhs(self, H, Cc):
+        H = np.ascontiguousarray(H, dtype=np.float64).ravel()
+        Cc = np.ascontiguousarray(Cc, dtype=np.float64).ravel()
+        LH = np.zeros(self.N)
+        LC = np.zeros(self.N)
+        bh = C.c_double()
+        bc = C.c_double()
+        _check(lib().mpo_transport_rhs(self._h, H, Cc, LH, LC, C.byref(bh), C.byref(bc)))
+        return LH.reshape(self.ny, self.nx), LC.reshape(self.ny, self.nx), bh.value, bc.value
+
+    def reconstruct(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).ravel()
+        sig = np.zeros(self.N * 5)
+        om = np.zeros(self.N * 5)
+        tr = np.zeros(self.N * 4 * self.q)
+        _check(lib().mpo_transport_reconstruct(self._h, u, sig, om, tr))
+        return (sig.reshape(self.ny, self.nx, 5), om.reshape(self.ny, self.nx, 5),
+                tr.reshape(self.ny, self.nx, 4, self.q))
+
+    def advance(self, H, Cc, nsteps, dt, scheme=3):
+        H = np.array(H, dtype=np.float64).ravel()
+        Cc = np.array(Cc, dtype=np.float64).ravel()
+        totals = np.zeros(4 * max(nsteps, 1))
+        _check(lib().mpo_transport_advance(self._h, H, Cc, nsteps, dt, scheme,
+                                           totals.ctypes.data_as(C.c_void_p)))
+        return (H.reshape(self.ny, self.nx), Cc.reshape(self.ny, self.nx),
+                totals.reshape(-1, 4)[:nsteps])
+
+    def integral(self, u):
+        return lib().mpo_transport_integral(self._h,
+                                            np.ascontiguousarray(u, dtype=np.float64).ravel())
