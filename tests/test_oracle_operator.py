@@ -1,0 +1,200 @@
+"""The oracle's transport operator and time integrators.
+
+Pins: constant-state preservation and global mass/boundary balance
+(src/mlweno/test/test_advection_diffusion.cpp:125-148,
+src/mlweno/test/transport_test_support.h:342-344); the LLF closed form; a brute
+force face loop on a tiny mesh built from the per-object API; the per-step
+conservation balance (src/couple/evolve.cpp:486-490); the SSP stability
+polynomials 1+z, 1+z+z^2/2, 1+z+z^2/2+z^3/6 on a linear operator; third-order
+temporal convergence of SSP-RK3; backflow rejection
+(src/couple/initialization.cpp:478-481).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from mantlepar_amd import workloads as W
+
+
+def uniform_flow(spec, ux, uy):
+    vnx = np.full((spec.q, spec.ny, spec.nx + 1), ux)
+    vny = np.full((spec.q, spec.ny + 1, spec.nx), uy)
+    return vnx, vny
+
+
+def make(spec, vnx=None, vny=None, bc=None, inflow=None, eps=None, table_mode=0):
+    if vnx is None:
+        vnx, vny = W.face_velocity(spec)
+    opt = O.WenoOptions.production()
+    if eps is not None:
+        opt.epsilon = eps
+    inH = inC = None
+    if inflow is not None:
+        inH, inC = inflow
+    return O.Transport(W.vertex_coords(spec), spec.nx, spec.ny, vnx, vny, q=spec.q, options=opt,
+                       bc=bc, inflow_H=inH, inflow_C=inC, table_mode=table_mode)
+
+
+def open_boundary(spec, ux=0.3, uy=-0.2, seed=0):
+    vnx, vny = uniform_flow(spec, ux, uy)
+    n = spec.q
+    rng = np.random.default_rng(seed)
+    inH = rng.random(2 * (spec.nx + spec.ny) * n)
+    inC = rng.random(2 * (spec.nx + spec.ny) * n)
+    bc = {s: np.full(spec.nx if s in ("bottom", "top") else spec.ny, 2, dtype=np.uint8)
+          for s in ("bottom", "right", "top", "left")}
+    return vnx, vny, bc, (inH, inC)
+
+
+def test_constant_state_is_preserved():
+    spec = W.MeshSpec(16, 12)
+    t = make(spec)
+    H = np.full((spec.ny, spec.nx), 0.7)
+    C = np.full((spec.ny, spec.nx), 0.3)
+    LH, LC, bh, bc = t.rhs(H, C)
+    assert np.abs(LH).max() < 2e-10 and np.abs(LC).max() < 2e-10
+
+
+def test_global_balance_with_open_boundaries():
+    spec = W.MeshSpec(14, 11, seed=3)
+    vnx, vny, bc, inflow = open_boundary(spec)
+    t = make(spec, vnx, vny, bc, inflow)
+    H, C = W.enthalpy(spec), W.composition(spec)
+    LH, LC, bh, bcf = t.rhs(H, C)
+    area = spec.dx * spec.dy
+    for L, b in ((LH, bh), (LC, bcf)):
+        mass = area * L.sum() + b
+        scale = 1 + abs(b) + area * np.abs(L).sum()
+        assert abs(mass) < 2e-11 * scale
+        assert abs(b) > 1e-3          # the boundaries are really open
+
+
+def test_llf_closed_form():
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        uL, uR = rng.normal(size=2)
+        v = rng.normal(size=2)
+        n = np.array([1.0, 0.0]) if rng.random() < 0.5 else np.array([0.0, -1.0])
+        vn = v @ n
+        f = O.llf_point(uL, uR, tuple(v), tuple(n))
+        assert abs(f - vn * (uL if vn > 0 else uR)) <= 4e-16 * (abs(vn) * (abs(uL) + abs(uR)) + 1e-300)
+        a = abs(vn) + 0.5
+        g = O.llf_point(uL, uR, tuple(v), tuple(n), mode=1, global_speed=a)
+        assert abs(g - 0.5 * (vn * uL + vn * uR - a * (uR - uL))) < 1e-15 * (1 + abs(g))
+    with pytest.raises(O.OracleError):
+        O.llf_point(1.0, 2.0, (2.0, 0.0), (1.0, 0.0), mode=1, global_speed=1.0)
+
+
+def test_brute_force_face_loop_on_tiny_mesh():
+    """Per-object API (Recon + integrate_edge_flux) assembled face by face in the
+    order of src/mlweno/test/transport_test_support.h:253-269, against rhs()."""
+    spec = W.MeshSpec(4, 3, seed=6)
+    vnx, vny = W.face_velocity(spec)
+    coords = W.vertex_coords(spec)
+    H = W.enthalpy(spec)
+    t = make(spec, vnx, vny)
+    LH, _, _, _ = t.rhs(H, W.composition(spec))
+    nx, ny, nq = spec.nx, spec.ny, spec.q
+    recon = {}
+    for j in range(ny):
+        for i in range(nx):
+            r = O.Recon(coords.reshape(-1), nx, ny, (i, j), h=np.sqrt(spec.dx * spec.dy))
+            r.update(H.ravel(), (0, 0, nx, ny), spec.dx * spec.dy)
+            recon[(i, j)] = r
+    xq, _ = O.gauss_rule(nq)
+    rhs = np.zeros((ny, nx))
+
+    def corners(i, j):
+        return [coords[j, i], coords[j, i + 1], coords[j + 1, i + 1], coords[j + 1, i]]
+
+    for j in range(ny):
+        for i in range(nx):
+            c = corners(i, j)
+            for side, nb in ((1, (i + 1, j)), (2, (i, j + 1))):
+                if nb[0] >= nx or nb[1] >= ny:
+                    continue
+                edge = (tuple(c[side]), tuple(c[(side + 1) % 4]))
+                pts = [0.5 * (1 - x) * np.array(edge[0]) + 0.5 * (1 + x) * np.array(edge[1])
+                       for x in xq]
+                if side == 1:
+                    vel = [(vnx[q, j, i + 1], 0.0) for q in range(nq)]
+                else:
+                    vel = [(0.0, vny[nq - 1 - q, j + 1, i]) for q in range(nq)]
+                left = [recon[(i, j)].evaluate(tuple(p)) for p in pts]
+                right = [recon[nb].evaluate(tuple(p)) for p in pts]
+                F = O.integrate_edge_flux(edge, vel, left, right, nq)
+                rhs[j, i] -= F / (spec.dx * spec.dy)
+                rhs[nb[1], nb[0]] += F / (spec.dx * spec.dy)
+    assert np.abs(rhs - LH).max() <= 1e-13 * max(1.0, np.abs(LH).max())
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 3])
+def test_conservation_balance_per_step(scheme):
+    spec = W.MeshSpec(12, 10, seed=4)
+    vnx, vny, bc, inflow = open_boundary(spec, 0.25, 0.15, seed=scheme)
+    t = make(spec, vnx, vny, bc, inflow)
+    H, C = W.enthalpy(spec), W.composition(spec)
+    _, mr = t.rates()
+    dt = 0.4 / mr
+    before = (t.integral(H), t.integral(C))
+    H1, C1, tot = t.advance(H, C, 1, dt, scheme)
+    for k, (b, f) in enumerate(((before[0], tot[0, 2]), (before[1], tot[0, 3]))):
+        after = tot[0, k]
+        assert abs(after - b + dt * f) < 1e-12 * max(1.0, abs(b))
+
+
+def test_closed_domain_conserves_totals_exactly():
+    spec = W.MeshSpec(20, 20, seed=1)
+    t = make(spec)
+    H, C = W.enthalpy(spec), W.composition(spec)
+    _, mr = t.rates()
+    _, _, tot = t.advance(H, C, 10, 0.4 / mr, 3)
+    assert abs(tot[-1, 0] - t.integral(H)) < 1e-14
+    assert abs(tot[-1, 1] - t.integral(C)) < 1e-14
+
+
+@pytest.mark.parametrize("scheme,coef", [(1, (1, 1)), (2, (1, 1, 0.5)), (3, (1, 1, 0.5, 1 / 6))])
+def test_stability_polynomial_on_linear_operator(scheme, coef):
+    """epsilon = 1e30 freezes the weights at the linear weights, so L is linear and
+    one step must equal sum_k coef_k (dt L)^k u."""
+    spec = W.MeshSpec(10, 9, seed=8)
+    t = make(spec, eps=1e30)
+    H, C = W.enthalpy(spec), W.composition(spec)
+    _, mr = t.rates()
+    dt = 0.4 / mr
+    H1, C1, _ = t.advance(H, C, 1, dt, scheme)
+    powers_H, powers_C = [H], [C]
+    for _ in range(len(coef) - 1):
+        LH, LC, _, _ = t.rhs(powers_H[-1], powers_C[-1])
+        powers_H.append(dt * LH)
+        powers_C.append(dt * LC)
+    expH = sum(c * p for c, p in zip(coef, powers_H))
+    expC = sum(c * p for c, p in zip(coef, powers_C))
+    assert np.abs(H1 - expH).max() < 1e-13
+    assert np.abs(C1 - expC).max() < 1e-13
+
+
+def test_ssprk3_is_third_order_in_time():
+    spec = W.MeshSpec(24, 24, seed=3)
+    t = make(spec, table_mode=1)
+    H = W.enthalpy(spec)
+    C = np.full((spec.ny, spec.nx), 0.3)
+    _, mr = t.rates()
+    dt0 = 0.4 / mr
+    sols = []
+    for k in (1, 2, 4):
+        Hk, _, _ = t.advance(H, C, 4 * k, dt0 / k, 3)
+        sols.append(Hk)
+    d1 = np.abs(sols[0] - sols[1]).max()
+    d2 = np.abs(sols[1] - sols[2]).max()
+    assert 6.5 < d1 / d2 < 9.5, (d1, d2)
+
+
+def test_backflow_on_outflow_only_boundary_is_rejected():
+    spec = W.MeshSpec(8, 6)
+    vnx, vny = uniform_flow(spec, 0.3, 0.0)       # inflow through the left side
+    bc = {"left": np.full(spec.ny, 1, dtype=np.uint8)}
+    t = make(spec, vnx, vny, bc)
+    with pytest.raises(O.OracleError) as e:
+        t.rhs(W.enthalpy(spec), W.composition(spec))
+    assert e.value.code == 7
