@@ -70,6 +70,15 @@ struct MeshDev {
     // face velocities [q][ny][nx+1] and [q][ny+1][nx]
     const double* vnx;
     const double* vny;
+    // Per-face flux moments (mixed path), built once per velocity refresh:
+    //   V+ = sum_q W_q (v_q + alpha_q)/2, V- = sum_q W_q (v_q - alpha_q)/2   (FP64)
+    //   M+_m = sum_q W_q (v_q + alpha_q)/2 s_q^m, M-_m likewise, m = 1, 2   (FP32)
+    // so an interior face flux is tL*V+ + tR*V- + sum_m (cL_m M+_m + cR_m M-_m)
+    // with c the face-line coefficients of the centred reconstructions.
+    const double* fxv;          // [2][ny][nx+1]: V+, V-
+    const float* fxm;           // [4][ny][nx+1]: M+_1, M+_2, M-_1, M-_2
+    const double* fyv;          // [2][ny+1][nx]
+    const float* fym;           // [4][ny+1][nx]
 };
 
 // Per launch, per mesh: the stage u_out = a*u_n + b*(u_k + dt*L(u_k)).

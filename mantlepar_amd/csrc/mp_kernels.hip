@@ -57,10 +57,9 @@ __device__ __forceinline__ double boundary_flux(const MeshDev& m, const double* 
     return F;
 }
 
-// Interior face flux (left state behind the +x/+y normal).  MODE 0 follows the
-// reference's per-point LLF form F = (F_L + F_R - alpha*(uR - uL))/2
-// (src/mlweno/advectiveflux.cpp:144-152); MODE 1 splits it into an FP64
-// first-order part and an FP32 high-order correction.
+// Interior face flux (left state behind the +x/+y normal) for one field, per
+// point, in the reference's LLF form F = (F_L + F_R - alpha*(uR - uL))/2
+// (src/mlweno/advectiveflux.cpp:144-152).  Used by the diagnostics kernels.
 template <int MODE>
 __device__ __forceinline__ double interior_flux(const MeshDev& m, const double* vn, size_t vstride,
                                                 double tL, const RealT<MODE> (&cL)[3], double tR,
@@ -68,41 +67,127 @@ __device__ __forceinline__ double interior_flux(const MeshDev& m, const double* 
                                                 const double* W)
 {
     using R = RealT<MODE>;
-    if constexpr (MODE == 0) {
-        double F = 0.0;
-        for (int q = 0; q < m.q; ++q) {
-            const double v = vn[q * vstride];
-            const double alpha = m.lf_mode ? m.global_speed : fabs(v);
-            const double uL = tL + line_at<double>(cL, s[q]);
-            const double uR = tR + line_at<double>(cR, s[q]);
-            F += W[q] * (0.5 * (v * uL + v * uR - alpha * (uR - uL)));
-        }
-        return F;
-    } else {
-        double Vp = 0.0, Vm = 0.0;
-        R corr = R(0);
-        for (int q = 0; q < m.q; ++q) {
-            const double v = vn[q * vstride];
-            const double alpha = m.lf_mode ? m.global_speed : fabs(v);
-            const double wl = W[q] * 0.5 * (v + alpha);
-            const double wr = W[q] * 0.5 * (v - alpha);
-            Vp += wl;
-            Vm += wr;
-            const R sq = R(s[q]);
-            corr += R(wl) * line_at<R>(cL, sq) + R(wr) * line_at<R>(cR, sq);
-        }
-        return tL * Vp + tR * Vm + double(corr);
+    double F = 0.0;
+    for (int q = 0; q < m.q; ++q) {
+        const double v = vn[q * vstride];
+        const double alpha = m.lf_mode ? m.global_speed : fabs(v);
+        const double uL = tL + double(line_at<R>(cL, R(s[q])));
+        const double uR = tR + double(line_at<R>(cR, R(s[q])));
+        F += W[q] * (0.5 * (v * uL + v * uR - alpha * (uR - uL)));
     }
+    return F;
+}
+
+// Both fields' LLF fluxes through one interior face, per Gauss point, in FP64
+// (the plain drop-in form).  NQ > 0 fixes the point count at compile time.
+template <int NQ>
+__device__ __forceinline__ void llf_pair(const MeshDev& m, const double* vn, size_t vstride,
+                                         const double* s, const double* W,
+                                         double tLH, const double (&cLH)[3], double tRH,
+                                         const double (&cRH)[3], double tLC,
+                                         const double (&cLC)[3], double tRC,
+                                         const double (&cRC)[3], double& FH, double& FC)
+{
+    double fh = 0.0, fc = 0.0;
+    const int nq = NQ > 0 ? NQ : m.q;
+#pragma unroll
+    for (int q = 0; q < (NQ > 0 ? NQ : kMaxQ); ++q) {
+        if (NQ == 0 && q >= nq) break;
+        const double v = vn[q * vstride];
+        const double alpha = m.lf_mode ? m.global_speed : fabs(v);
+        const double sq = s[q], w = W[q];
+        const double uLH = tLH + line_at<double>(cLH, sq), uRH = tRH + line_at<double>(cRH, sq);
+        const double uLC = tLC + line_at<double>(cLC, sq), uRC = tRC + line_at<double>(cRC, sq);
+        fh += w * (0.5 * (v * uLH + v * uRH - alpha * (uRH - uLH)));
+        fc += w * (0.5 * (v * uLC + v * uRC - alpha * (uRC - uLC)));
+    }
+    FH = fh;
+    FC = fc;
+}
+
+// Both fields through one interior face from the precomputed flux moments:
+// the first-order part t*V in FP64, the high-order correction in FP32.
+__device__ __forceinline__ void moment_pair(double Vp, double Vm, float Mp1, float Mp2, float Mm1,
+                                            float Mm2, double tLH, const float (&cLH)[3],
+                                            double tRH, const float (&cRH)[3], double tLC,
+                                            const float (&cLC)[3], double tRC,
+                                            const float (&cRC)[3], double& FH, double& FC)
+{
+    const float vp = float(Vp), vm = float(Vm);
+    const float ch = cLH[0] * vp + cLH[1] * Mp1 + cLH[2] * Mp2 + cRH[0] * vm + cRH[1] * Mm1 + cRH[2] * Mm2;
+    const float cc = cLC[0] * vp + cLC[1] * Mp1 + cLC[2] * Mp2 + cRC[0] * vm + cRC[1] * Mm1 + cRC[2] * Mm2;
+    FH = fma(tLH, Vp, tRH * Vm) + double(ch);
+    FC = fma(tLC, Vp, tRC * Vm) + double(cc);
+}
+
+// Flux moments of every face (one thread per face), once per velocity refresh.
+__global__ void face_prep_kernel(const MeshDev* __restrict__ mesh, double* fxv, float* fxm,
+                                 double* fyv, float* fym)
+{
+    const MeshDev& m = *mesh;
+    const int64_t nxf = int64_t(m.ny) * (m.nx + 1), nyf = int64_t(m.ny + 1) * m.nx;
+    int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (f >= nxf + nyf) return;
+    const bool vert = f < nxf;
+    if (!vert) f -= nxf;
+    const int64_t nf = vert ? nxf : nyf;
+    const double* vn = (vert ? m.vnx : m.vny) + f;
+    const double* s = vert ? m.sv : m.sh;
+    const double* W = vert ? m.wv : m.wh;
+    double vp = 0, vm = 0, p1 = 0, p2 = 0, m1 = 0, m2 = 0;
+    for (int q = 0; q < m.q; ++q) {
+        const double v = vn[q * nf];
+        const double alpha = m.lf_mode ? m.global_speed : fabs(v);
+        const double wl = W[q] * 0.5 * (v + alpha), wr = W[q] * 0.5 * (v - alpha);
+        vp += wl;
+        vm += wr;
+        p1 += wl * s[q];
+        p2 += wl * s[q] * s[q];
+        m1 += wr * s[q];
+        m2 += wr * s[q] * s[q];
+    }
+    double* dv = vert ? fxv : fyv;
+    float* dm = vert ? fxm : fym;
+    dv[f] = vp;
+    dv[nf + f] = vm;
+    dm[f] = float(p1);
+    dm[nf + f] = float(p2);
+    dm[2 * nf + f] = float(m1);
+    dm[3 * nf + f] = float(m2);
 }
 
 __device__ __forceinline__ void two_sum_acc(double& s, double& e, double x)
 {
+#pragma clang fp contract(off)
     const double y = s + x;
     e += fabs(s) >= fabs(x) ? (s - y) + x : (x - y) + s;
     s = y;
 }
 
-template <int MODE, bool PHASE>
+// Flux of both fields through one interior face.
+template <int MODE, int NQ>
+__device__ __forceinline__ void interior_pair(const MeshDev& m, bool vertical, size_t face,
+                                              double tLH, const RealT<MODE> (&cLH)[3], double tRH,
+                                              const RealT<MODE> (&cRH)[3], double tLC,
+                                              const RealT<MODE> (&cLC)[3], double tRC,
+                                              const RealT<MODE> (&cRC)[3], double& FH,
+                                              double& FC)
+{
+    if constexpr (MODE == 0) {
+        const double* vn = vertical ? m.vnx + face : m.vny + face;
+        const size_t vs = vertical ? size_t(m.ny) * (m.nx + 1) : size_t(m.ny + 1) * m.nx;
+        llf_pair<NQ>(m, vn, vs, vertical ? m.sv : m.sh, vertical ? m.wv : m.wh, tLH, cLH, tRH, cRH,
+                     tLC, cLC, tRC, cRC, FH, FC);
+    } else {
+        const size_t nf = vertical ? size_t(m.ny) * (m.nx + 1) : size_t(m.ny + 1) * m.nx;
+        const double* V = vertical ? m.fxv : m.fyv;
+        const float* Mm = vertical ? m.fxm : m.fym;
+        moment_pair(V[face], V[nf + face], Mm[face], Mm[nf + face], Mm[2 * nf + face],
+                    Mm[3 * nf + face], tLH, cLH, tRH, cRH, tLC, cLC, tRC, cRC, FH, FC);
+    }
+}
+
+template <int MODE, bool PHASE, int NQ>
 __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int strip, int j0,
                            int j1)
 {
@@ -178,8 +263,8 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
                 FvH = oH; FvC = oC;
                 if (owned) { bH += oH; bC += oC; }
             } else {
-                FvH = interior_flux<MODE>(m, vn, vys, tTopH, topH, tH, LH.c[0], m.sh, m.wh);
-                FvC = interior_flux<MODE>(m, vn, vys, tTopC, topC, tC, LC.c[0], m.sh, m.wh);
+                interior_pair<MODE, NQ>(m, false, size_t(r) * nx + ic, tTopH, topH, tH, LH.c[0],
+                                        tTopC, topC, tC, LC.c[0], FvH, FvC);
             }
         }
 
@@ -224,7 +309,8 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
             }
             double FrH = 0.0, FrC = 0.0;
             if (i >= -1 && i <= nx - 1 && have) {
-                const double* vn = m.vnx + size_t(r) * (nx + 1) + (i + 1);
+                const size_t face = size_t(r) * (nx + 1) + (i + 1);
+                const double* vn = m.vnx + face;
                 if (i == -1) {
                     const double* inH = m.inflowH[3] ? m.inflowH[3] + size_t(r) * m.q : nullptr;
                     const double* inC = m.inflowC[3] ? m.inflowC[3] + size_t(r) * m.q : nullptr;
@@ -242,8 +328,8 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
                     FrH = oH; FrC = oC;
                     if (owned) { bH += oH; bC += oC; }
                 } else {
-                    FrH = interior_flux<MODE>(m, vn, vxs, tH, LH.c[1], nbtH, nbH, m.sv, m.wv);
-                    FrC = interior_flux<MODE>(m, vn, vxs, tC, LC.c[1], nbtC, nbC, m.sv, m.wv);
+                    interior_pair<MODE, NQ>(m, true, face, tH, LH.c[1], nbtH, nbH, tC, LC.c[1],
+                                            nbtC, nbC, FrH, FrC);
                 }
             }
             pFlH = __shfl_up(FrH, 1);
@@ -286,8 +372,9 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
     if (err) atomicOr(sp.error, err);
 }
 
-template <int MODE, bool PHASE>
-__global__ __launch_bounds__(256) void stage_sweep_kernel(const MeshDev* __restrict__ mesh,
+// Register budget: two waves per SIMD for FP64 (256 registers), three for mixed.
+template <int MODE, bool PHASE, int NQ>
+__global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void stage_sweep_kernel(const MeshDev* __restrict__ mesh,
                                                           StagePtrs sp, int nitems)
 {
     const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -296,8 +383,9 @@ __global__ __launch_bounds__(256) void stage_sweep_kernel(const MeshDev* __restr
     const int strip = item % sp.strips, band = item / sp.strips;
     const int j0 = band * sp.rows;
     const int j1 = min(j0 + sp.rows, m.ny);
-    sweep_item<MODE, PHASE>(m, sp, item, strip, j0, j1);
+    sweep_item<MODE, PHASE, NQ>(m, sp, item, strip, j0, j1);
 }
+
 
 // ------------------------------------------------------------ diagnostics
 
@@ -512,17 +600,29 @@ __global__ void reduce_step_kernel(const double* __restrict__ partials, int nite
 
 // ---------------------------------------------------------------- launchers
 
-hipError_t launch_stage(int mode, bool phase, const MeshDev* d_mesh, const StagePtrs& sp,
+hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
                         int nitems, hipStream_t stream)
 {
     const dim3 grid((nitems + 3) / 4), block(256);
-    if (mode == 0) {
-        if (phase) stage_sweep_kernel<0, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-        else stage_sweep_kernel<0, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    if (mode == 0 && q == 5) {
+        if (phase) stage_sweep_kernel<0, true, 5><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        else stage_sweep_kernel<0, false, 5><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    } else if (mode == 0) {
+        if (phase) stage_sweep_kernel<0, true, 0><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        else stage_sweep_kernel<0, false, 0><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
     } else {
-        if (phase) stage_sweep_kernel<1, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-        else stage_sweep_kernel<1, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        if (phase) stage_sweep_kernel<1, true, 0><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        else stage_sweep_kernel<1, false, 0><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_face_prep(const MeshDev* d_mesh, int nx, int ny, double* fxv, float* fxm,
+                            double* fyv, float* fym, hipStream_t stream)
+{
+    const int64_t n = int64_t(ny) * (nx + 1) + int64_t(ny + 1) * nx;
+    const dim3 grid(unsigned((n + 255) / 256)), block(256);
+    face_prep_kernel<<<grid, block, 0, stream>>>(d_mesh, fxv, fxm, fyv, fym);
     return hipGetLastError();
 }
 

@@ -7,8 +7,10 @@
 
 namespace mp {
 
-hipError_t launch_stage(int mode, bool phase, const MeshDev* d_mesh, const StagePtrs& sp,
+hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
                         int nitems, hipStream_t stream);
+hipError_t launch_face_prep(const MeshDev* d_mesh, int nx, int ny, double* fxv, float* fxm,
+                            double* fyv, float* fym, hipStream_t stream);
 hipError_t launch_recon_diag(int mode, const MeshDev* d_mesh, int ncells, const double* u,
                              double* sigma, double* omega, double* traces, uint32_t* error,
                              hipStream_t stream);

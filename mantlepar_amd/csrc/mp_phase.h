@@ -5,8 +5,10 @@
 // to double.  The device has no long double, so the phase boundaries, whose
 // rounding decides the region, are formed in double-double and rounded once;
 // they then agree with the reference's long double -> double results except
-// where the exact boundary lies within a few ulps of a rounding tie.  All
-// other quantities are plain FP64 and agree to a few ulps.
+// where the exact boundary lies within a few ulps of a rounding tie.  The
+// eutectic-branch fractions (tiny differences of O(1) terms near the end of
+// eutectic melting) are also formed in double-double.  All other quantities
+// are plain FP64 and agree to a few ulps.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,8 +24,13 @@ struct PhaseOut {
 
 struct DD { double hi, lo; };
 
+// Error-free transformations must not be contracted: hipcc fuses a product
+// feeding a sum into an FMA by default, which silently drops the rounding
+// error these helpers exist to capture.  Each helper turns contraction off.
+
 __device__ __forceinline__ DD dd_two_sum(double a, double b)
 {
+#pragma clang fp contract(off)
     const double s = a + b;
     const double bb = s - a;
     const double e = (a - (s - bb)) + (b - bb);
@@ -32,6 +39,7 @@ __device__ __forceinline__ DD dd_two_sum(double a, double b)
 
 __device__ __forceinline__ DD dd_add(DD a, DD b)
 {
+#pragma clang fp contract(off)
     DD s = dd_two_sum(a.hi, b.hi);
     s.lo += a.lo + b.lo;
     return dd_two_sum(s.hi, s.lo);
@@ -40,13 +48,31 @@ __device__ __forceinline__ DD dd_add(DD a, DD b)
 // c / x as a double-double (quotient plus its fma residual).
 __device__ __forceinline__ DD dd_div(double c, double x)
 {
+#pragma clang fp contract(off)
     const double q = c / x;
     const double r = fma(-q, x, c);
     return {q, r / x};
 }
 
+// (a.hi + a.lo) / b as a double-double.
+__device__ __forceinline__ DD dd_div_d(DD a, double b)
+{
+#pragma clang fp contract(off)
+    const double q1 = a.hi / b;
+    const double r = fma(-q1, b, a.hi) + a.lo;
+    return dd_two_sum(q1, r / b);
+}
+
+__device__ __forceinline__ DD dd_neg(DD a) { return {-a.hi, -a.lo}; }
+
+__device__ __forceinline__ bool dd_less(DD a, DD b)
+{
+    return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo);
+}
+
 __device__ __forceinline__ DD dd_mul_d(DD a, double b)
 {
+#pragma clang fp contract(off)
     const double p = a.hi * b;
     const double e = fma(a.hi, b, -p) + a.lo * b;
     return dd_two_sum(p, e);
@@ -99,12 +125,20 @@ __device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, dou
         s.phi1 = 1.0 - C; s.phi2 = C; s.T = H; s.dTdH = 1.0;
         break;
     case 3: {
-        const double maxl = C / Xe;
-        const double liquid = H == Heut ? maxl : fmin(maxl, (H - Tep) / LD);
-        const double solid2 = fmax(0.0, fma(-liquid, Xe, C));
-        s.phil = unit_fraction(liquid, err);
-        s.phi2 = unit_fraction(solid2, err);
-        s.phi1 = unit_fraction(1.0 - liquid - solid2, err);
+        // Fractions in double-double: near the end of eutectic melting both
+        // solid fractions are tiny differences of O(1) terms.
+        const DD maxl = ratio;
+        DD liquid = maxl;
+        if (H != Heut) {
+            const DD e = dd_div_d(dd_two_sum(H, -Tep), LD);
+            if (dd_less(e, maxl)) liquid = e;
+        }
+        DD solid2 = dd_add({C, 0.0}, dd_mul_d(liquid, -Xe));
+        if (solid2.hi < 0.0) solid2 = {0.0, 0.0};
+        const DD solid1 = dd_add(dd_add({1.0, 0.0}, dd_neg(liquid)), dd_neg(solid2));
+        s.phil = unit_fraction(liquid.hi + liquid.lo, err);
+        s.phi2 = unit_fraction(solid2.hi + solid2.lo, err);
+        s.phi1 = unit_fraction(solid1.hi + solid1.lo, err);
         s.cl = Xe;
         s.T = Tep;
         break;

@@ -60,6 +60,15 @@ mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
     }
     MP_CALL(vnx.alloc(size_t(m.q) * m.ny * (m.nx + 1) * sizeof(double)));
     MP_CALL(vny.alloc(size_t(m.q) * (m.ny + 1) * m.nx * sizeof(double)));
+    const size_t nxf = size_t(m.ny) * (m.nx + 1), nyf = size_t(m.ny + 1) * m.nx;
+    MP_CALL(fxv.alloc(2 * nxf * sizeof(double)));
+    MP_CALL(fxm.alloc(4 * nxf * sizeof(float)));
+    MP_CALL(fyv.alloc(2 * nyf * sizeof(double)));
+    MP_CALL(fym.alloc(4 * nyf * sizeof(float)));
+    md.fxv = fxv.as<double>();
+    md.fxm = fxm.as<float>();
+    md.fyv = fyv.as<double>();
+    md.fym = fym.as<float>();
     MP_CALL(error.alloc(sizeof(uint64_t) * 2));
     MP_CALL(d_md.alloc(sizeof(MeshDev)));
     MP_HIP(hipMemsetAsync(H[0].ptr, 0, N * sizeof(double), stream));
@@ -137,6 +146,8 @@ mp_status mp_transport::set_velocity(const double* x, const double* y)
     }
     MP_HIP(hipMemcpyAsync(vnx.ptr, x, nxv * sizeof(double), hipMemcpyHostToDevice, stream));
     MP_HIP(hipMemcpyAsync(vny.ptr, y, nyv * sizeof(double), hipMemcpyHostToDevice, stream));
+    MP_HIP(launch_face_prep(d_md.as<MeshDev>(), mesh.nx, mesh.ny, fxv.as<double>(), fxm.as<float>(),
+                            fyv.as<double>(), fym.as<float>(), stream));
     MP_CALL(compute_rate());
     have_velocity = true;
     return MP_OK;
@@ -244,7 +255,7 @@ mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& fi
             const int out = last ? dst : 3 + st;
             const StagePtrs sp = stage_ptrs(src, kin, out, scheme, st, dt, last, phase_set, st);
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
-            MP_HIP(launch_stage(precision, last, d_md.as<MeshDev>(), sp, nitems, stream));
+            MP_HIP(launch_stage(precision, last, mesh.q, d_md.as<MeshDev>(), sp, nitems, stream));
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
             kin = out;
         }
