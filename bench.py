@@ -33,7 +33,21 @@ def parse():
     p.add_argument("--rows", type=int, default=0, help="rows per work item (0: autotune)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="run only a plain 1 GiB device copy (read 1 GiB, write 1 GiB) so that "
+                        "FETCH_SIZE/WRITE_SIZE counters can be scaled")
     return p.parse_args()
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 27                      # 2^27 doubles = 1 GiB
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 3}))
 
 
 def algorithmic_bytes(q, state_b=8, vel_b=8, phase_b=8):
@@ -70,6 +84,9 @@ def cpu_baseline(args):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path shards by mesh strips; see the design notes).", file=sys.stderr)
@@ -90,6 +107,7 @@ def main():
         dt = t.stable_dt(0.4)
         if args.rows:
             t.set_tile(args.rows)
+            rows_used = args.rows
         else:
             best = None
             for rows in (16, 32, 64, 128):
@@ -99,6 +117,7 @@ def main():
                 if best is None or ms < best[0]:
                     best = (ms, rows)
             t.set_tile(best[1])
+            rows_used = best[1]
         t.upload(H, Cc)
         if args.warmup:
             t.step(args.warmup, dt)
@@ -111,7 +130,8 @@ def main():
         # kernel durations with HIP events on the handle's stream, over the same work
         t.upload(H, Cc)
         prof = t.profile(min(args.steps, 20), dt)
-        res[name] = {"seconds": el, "report": rep, "H": Hf, "C": Cf, "profile": prof}
+        res[name] = {"seconds": el, "report": rep, "H": Hf, "C": Cf, "profile": prof,
+                     "rows": rows_used}
         t.close()
 
     N = spec.cells
@@ -150,7 +170,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "stage_sweep_kernel<1,false> (stages 1-2)",
                      "bytes_per_cell_launch": 0.5 * (b["stage1"] + b["stage2"]),
-                     "ms_per_launch": prof["ms_stage"]},
+                     "ms_per_launch": prof["ms_stage"], "rows_per_item": mixed["rows"]},
         "conservation": {"total_H": mixed["report"]["total_H"], "total_C": mixed["report"]["total_C"]},
     }
     if not args.no_cpu:

@@ -560,41 +560,55 @@ __global__ void state_phase_kernel(const MeshDev* __restrict__ mesh, const doubl
     if (err) atomicOr(error, err);
 }
 
-// Step totals from the stage partials, in fixed order (one wave):
-// out[0..3] = sum H, sum C (times area), stage-weighted boundary H, C.
-__global__ void reduce_step_kernel(const double* __restrict__ partials, int nitems, int nstages,
-                                   double w0, double w1, double w2, double area, double* out)
+// Step totals from the stage partials in a fixed order: 1024 threads take
+// strided slices, then a fixed LDS tree combines them (compensated for the
+// state sums).  out[0..3] = sum H, sum C (times area), stage-weighted
+// boundary H, C.
+constexpr int kReduceThreads = 1024;
+
+__global__ __launch_bounds__(kReduceThreads) void reduce_step_kernel(
+    const double* __restrict__ partials, int nitems, int nstages, double w0, double w1,
+    double w2, double area, double* out)
 {
-    const int lane = threadIdx.x;
+    __shared__ double sh[6][kReduceThreads];
+    const int tid = threadIdx.x;
     double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
     const double wst[3] = {w0, w1, w2};
     for (int st = 0; st < nstages; ++st) {
         const double* p = partials + size_t(st) * nitems * kPartials;
-        for (int it = lane; it < nitems; it += 64) {
+        const double w = wst[st];
+        const bool last = st == nstages - 1;
+        for (int it = tid; it < nitems; it += kReduceThreads) {
             const double* q = p + size_t(it) * kPartials;
-            if (st == nstages - 1) {
-                two_sum_acc(sH, eH, q[0]); eH += q[1];
-                two_sum_acc(sC, eC, q[2]); eC += q[3];
+            if (last) {
+                two_sum_acc(sH, eH, q[0]);
+                eH += q[1];
+                two_sum_acc(sC, eC, q[2]);
+                eC += q[3];
             }
-            bH += wst[st] * q[4];
-            bC += wst[st] * q[5];
+            bH += w * q[4];
+            bC += w * q[5];
         }
     }
-    double v[6] = {sH, eH, sC, eC, bH, bC};
-    for (int off = 32; off >= 1; off >>= 1) {
-        double o[6];
-        for (int k = 0; k < 6; ++k) o[k] = __shfl_xor(v[k], off);
-        double s0 = v[0], e0 = v[1] + o[1];
-        two_sum_acc(s0, e0, o[0]);
-        double s1 = v[2], e1 = v[3] + o[3];
-        two_sum_acc(s1, e1, o[2]);
-        v[0] = s0; v[1] = e0; v[2] = s1; v[3] = e1; v[4] += o[4]; v[5] += o[5];
+    sh[0][tid] = sH; sh[1][tid] = eH; sh[2][tid] = sC; sh[3][tid] = eC; sh[4][tid] = bH; sh[5][tid] = bC;
+    __syncthreads();
+    for (int half = kReduceThreads / 2; half >= 1; half >>= 1) {
+        if (tid < half) {
+            double s0 = sh[0][tid], e0 = sh[1][tid] + sh[1][tid + half];
+            two_sum_acc(s0, e0, sh[0][tid + half]);
+            double s1 = sh[2][tid], e1 = sh[3][tid] + sh[3][tid + half];
+            two_sum_acc(s1, e1, sh[2][tid + half]);
+            sh[0][tid] = s0; sh[1][tid] = e0; sh[2][tid] = s1; sh[3][tid] = e1;
+            sh[4][tid] += sh[4][tid + half];
+            sh[5][tid] += sh[5][tid + half];
+        }
+        __syncthreads();
     }
-    if (lane == 0) {
-        out[0] = (v[0] + v[1]) * area;
-        out[1] = (v[2] + v[3]) * area;
-        out[2] = v[4];
-        out[3] = v[5];
+    if (tid == 0) {
+        out[0] = (sh[0][0] + sh[1][0]) * area;
+        out[1] = (sh[2][0] + sh[3][0]) * area;
+        out[2] = sh[4][0];
+        out[3] = sh[5][0];
     }
 }
 
@@ -678,7 +692,7 @@ hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H
 hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, const double* w,
                               double area, double* out, hipStream_t stream)
 {
-    reduce_step_kernel<<<1, 64, 0, stream>>>(partials, nitems, nstages, w[0], w[1], w[2], area, out);
+    reduce_step_kernel<<<1, kReduceThreads, 0, stream>>>(partials, nitems, nstages, w[0], w[1], w[2], area, out);
     return hipGetLastError();
 }
 
