@@ -33,6 +33,12 @@ def parse():
     p.add_argument("--rows", type=int, default=0, help="rows per work item (0: autotune)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--batch-steps", type=int, default=50)
+    p.add_argument("--serve-requests", type=int, default=400)
+    p.add_argument("--serve-rate", type=float, default=300.0, help="Poisson arrivals per second")
+    p.add_argument("--no-batch", action="store_true")
+    p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--tune-cache", default=os.path.join(HERE, "gpurun_out", "batch_tune.json"))
     p.add_argument("--copy-calibration", action="store_true",
                    help="run only a plain 1 GiB device copy (read 1 GiB, write 1 GiB) so that "
                         "FETCH_SIZE/WRITE_SIZE counters can be scaled")
@@ -80,6 +86,155 @@ def cpu_baseline(args):
             "sample": f"1 SSP-RK3 step (FP64, per-point polynomial rebuild, long-double "
                       f"accumulation) on a {spec.nx}x{spec.ny} band (y in [0,0.25]) with the "
                       f"same cell size as the {args.nx}x{args.ny} mesh; {el:.2f} s"}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def batch_inputs(count=256, seed=3):
+    from mantlepar_amd import workloads as W
+    specs = W.batch_specs(count, seed=seed)
+    return [(s, W.enthalpy(s), W.composition(s), W.face_velocity(s)) for s in specs]
+
+
+def bench_batch(args, meshes):
+    """The 256-mesh batch: one persistent launch per stage over all meshes,
+    autotuned schedule, hipGraph replay; mixed and FP64.  The plain FP64 GPU
+    path runs the same meshes one handle at a time."""
+    import mantlepar_amd as M
+    from mantlepar_amd import workloads as W
+    K = args.batch_steps
+    cells = sum(s.cells for s, *_ in meshes)
+    res = {}
+    rng = np.random.default_rng(7)
+    for prec, name in ((M.PREC_MIXED, "mixed"), (M.PREC_FP64, "fp64"), (M.PREC_FP64, "fp64p")):
+        b = M.Batch(len(meshes), phase=W.config_phase(), precision=prec)
+        ids = []
+        inputs = []
+        for s, H, Cc, (vx, vy) in meshes:
+            if name == "fp64p":    # FP64 from an FP32-sized perturbation of H and C
+                H = H * (1.0 + 2.0 ** -24 * rng.standard_normal(H.shape))
+                Cc = Cc * (1.0 + 2.0 ** -24 * rng.standard_normal(Cc.shape))
+            inputs.append((H, Cc))
+            k = b.add_mesh(s.nx, s.ny, s.x0, s.x1, s.y0, s.y1, q=s.q)
+            b.upload(k, H, Cc)
+            b.set_face_velocity(k, vx, vy)
+            b.set_dt(k, b.stable_dt(k, 0.4))
+            ids.append(k)
+        os.makedirs(os.path.dirname(args.tune_cache), exist_ok=True)
+        tune = b.autotune(2, cache_path=args.tune_cache)
+        b.step(2, graph=True)                        # warm-up and graph capture of 2 steps
+        for (H, Cc), k in zip(inputs, ids):
+            b.upload(k, H, Cc)
+        b.step(K, graph=True)                        # capture the timed shape once
+        for (H, Cc), k in zip(inputs, ids):
+            b.upload(k, H, Cc)
+        t0 = time.perf_counter()
+        b.step(K, graph=True)
+        el = time.perf_counter() - t0
+        prof = b.profile(min(K, 10))
+        res[name] = {"seconds": el, "tune": tune, "profile": prof,
+                     "states": [b.download(k) for k in ids], "info": b.info()}
+        b.close()
+        log(f"batch {name}: {cells * K / el:.3e} cell-steps/s, schedule {tune}")
+    # plain FP64 drop-in path: one mp_transport handle per mesh, run in turn
+    handles = []
+    for s, H, Cc, (vx, vy) in meshes:
+        t = M.Transport(s.nx, s.ny, s.x0, s.x1, s.y0, s.y1, q=s.q, phase=W.config_phase(),
+                        precision=M.PREC_FP64)
+        t.upload(H, Cc)
+        t.set_face_velocity(vx, vy)
+        handles.append((t, t.stable_dt(0.4)))
+    for t, dt in handles:
+        t.step(1, dt)
+    for (s, H, Cc, _), (t, dt) in zip(meshes, handles):
+        t.upload(H, Cc)
+    t0 = time.perf_counter()
+    for t, dt in handles:
+        t.step(K, dt, M.SCHEME_SSPRK3)
+    el_plain = time.perf_counter() - t0
+    for t, _ in handles:
+        t.close()
+    log(f"batch plain fp64 handles: {cells * K / el_plain:.3e} cell-steps/s")
+    def rel(a, b):
+        return float(np.abs(a - b).max() / np.abs(b).max())
+
+    # Accuracy against FP64, split by the FP64 method's own conditioning: a
+    # mesh is "stable" when FP64 runs from (H, C) and from (H, C)(1 + 2^-24 xi)
+    # (max relative difference near 3e-7 at the start) stay within 1e-6 of
+    # each other after the same steps.
+    err = {"H": 0.0, "C": 0.0, "H_stable": 0.0, "C_stable": 0.0, "unstable_meshes": 0,
+           "fp64_self_spread_max": 0.0}
+    for (hm, cm), (h6, c6), (hp, cp) in zip(res["mixed"]["states"], res["fp64"]["states"],
+                                            res["fp64p"]["states"]):
+        eh, ec = rel(hm, h6), rel(cm, c6)
+        spread = max(rel(hp, h6), rel(cp, c6))
+        err["H"], err["C"] = max(err["H"], eh), max(err["C"], ec)
+        err["fp64_self_spread_max"] = max(err["fp64_self_spread_max"], spread)
+        if spread <= 1e-6:
+            err["H_stable"], err["C_stable"] = max(err["H_stable"], eh), max(err["C_stable"], ec)
+        else:
+            err["unstable_meshes"] += 1
+    mixed_v = cells * K / res["mixed"]["seconds"]
+    fp64_v = cells * K / res["fp64"]["seconds"]
+    plain_v = cells * K / el_plain
+    return {
+        "workload": f"{len(meshes)} meshes, nx in {{40..400}} step 20, ny log-uniform [5,800], "
+                    f"{K} SSP-RK3 steps, hipGraph replay",
+        "meshes": len(meshes), "cells": cells, "steps": K,
+        "mixed": {"value": mixed_v, "ms_per_step": res["mixed"]["seconds"] / K * 1e3,
+                  "schedule": res["mixed"]["tune"], "profile": res["mixed"]["profile"]},
+        "fp64": {"value": fp64_v, "ms_per_step": res["fp64"]["seconds"] / K * 1e3,
+                 "schedule": res["fp64"]["tune"]},
+        "plain_fp64_handles": {"value": plain_v, "ms_per_step": el_plain / K * 1e3},
+        "mixed_over_plain_fp64": mixed_v / plain_v,
+        "mixed_over_batched_fp64": mixed_v / fp64_v,
+        "error_vs_fp64": err,
+        "unit": "cell-steps/s",
+    }
+
+
+def bench_serve(args, meshes):
+    """Poisson arrivals of transport runs (meshes of the batch distribution,
+    20-200 seeded SSP-RK3 steps), continuous batching under a 200 GB HBM
+    budget; p50/p99 submit-to-done latency and aggregate cell-steps/s."""
+    import mantlepar_amd as M
+    from mantlepar_amd import workloads as W
+    rng = np.random.default_rng(17)
+    n = args.serve_requests
+    picks = rng.integers(0, len(meshes), n)
+    steps = rng.integers(20, 201, n)
+    gaps = rng.exponential(1.0 / args.serve_rate, n)
+    out = {"requests": n, "rate_per_s": args.serve_rate, "hbm_budget_bytes": 200 * 10**9,
+           "workload": "Poisson arrivals; mesh drawn from the 256-mesh batch pool; "
+                       "20-200 SSP-RK3 steps; CFL 0.4"}
+    for prec, name in ((M.PREC_MIXED, "mixed"), (M.PREC_FP64, "fp64")):
+        srv = M.Server(max_slots=512, phase=W.config_phase(), precision=prec,
+                       hbm_budget_bytes=200 * 10**9, max_epoch_steps=8)
+        t_next = time.perf_counter()
+        ids = []
+        for i in range(n):
+            t_next += gaps[i]
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            s, H, Cc, (vx, vy) = meshes[picks[i]]
+            ids.append(srv.submit(s.nx, s.ny, H, Cc, vx, vy, int(steps[i]), 0.4,
+                                  s.x0, s.x1, s.y0, s.y1, q=s.q, outputs=("H", "C")))
+        srv.drain()
+        st = srv.stats()
+        for rid in ids:
+            srv.wait(rid)
+        srv.close()
+        out[name] = {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+                     "max_ms": st["max_ms"], "cell_steps_per_s": st["cell_steps"] / st["elapsed_s"],
+                     "completed": st["completed"], "failed": st["failed"],
+                     "max_resident": st["max_resident"], "epochs": st["epochs"],
+                     "hbm_peak_bytes": st["hbm_peak"]}
+        log(f"serve {name}: p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms, "
+            f"{out[name]['cell_steps_per_s']:.3e} cell-steps/s")
+    return out
 
 
 def main():
@@ -133,6 +288,7 @@ def main():
         res[name] = {"seconds": el, "report": rep, "H": Hf, "C": Cf, "profile": prof,
                      "rows": rows_used}
         t.close()
+        log(f"single {name}: {spec.cells * args.steps / el:.3e} cell-steps/s (rows {rows_used})")
 
     N = spec.cells
     K = args.steps
@@ -173,6 +329,12 @@ def main():
                      "ms_per_launch": prof["ms_stage"], "rows_per_item": mixed["rows"]},
         "conservation": {"total_H": mixed["report"]["total_H"], "total_C": mixed["report"]["total_C"]},
     }
+    if not (args.no_batch and args.no_serve):
+        meshes = batch_inputs()
+        if not args.no_batch:
+            out["batch"] = bench_batch(args, meshes)
+        if not args.no_serve:
+            out["serve"] = bench_serve(args, meshes)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(args)

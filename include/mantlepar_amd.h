@@ -54,7 +54,8 @@ enum {
     MP_ERR_DEVICE = 5,     /* HIP runtime failure */
     MP_ERR_MEMORY = 6,     /* allocation or HBM budget exhausted */
     MP_ERR_BACKFLOW = 7,   /* backflow on an outflow-only boundary (initialization.cpp:478-481) */
-    MP_ERR_COMPOSITION = 8 /* C outside the admissible interval (eutectic_rescaled.cpp:88-93) */
+    MP_ERR_COMPOSITION = 8, /* C outside the admissible interval (eutectic_rescaled.cpp:88-93) */
+    MP_ERR_PENDING = 9      /* request not finished within the wait timeout */
 };
 
 enum { MP_PREC_FP64 = 0, MP_PREC_MIXED = 1 };
@@ -157,6 +158,129 @@ mp_status mp_transport_profile(mp_transport* t, int32_t nsteps, int32_t scheme, 
                                double* ms_total, double* ms_stage, double* ms_phase_stage);
 /* Sweep tile shape: rows per work item (>= 4). */
 mp_status mp_transport_set_tile(mp_transport* t, int32_t rows_per_item);
+
+/* --------------------------------------------------------------- batch --- */
+/* Many independent meshes advanced together.  The reference runs one mesh per
+ * rank, each with its own StateReconstruction and stepping loop
+ * (src/couple/state_reconstruction.cpp:35-120, src/couple/preheat.cpp:234-287);
+ * a batch holds many such shares on one device and runs every stage as ONE
+ * persistent launch over all of them.  Each mesh keeps its own dt, boundary
+ * data, error word and report, and its results are bitwise those of an
+ * mp_transport handle on the same inputs.  No reference counterpart for the
+ * batching itself.  Mesh ids are slot numbers in [0, capacity). */
+enum { MP_BATCH_EAGER = 0, MP_BATCH_GRAPH = 1 };
+
+/* hbm_budget_bytes bounds the device memory of all resident meshes (0: no bound). */
+mp_status mp_batch_create(int32_t capacity, const mp_weno_options* weno,
+                          const mp_phase_params* phase, int32_t precision,
+                          uint64_t hbm_budget_bytes, void* stream, mp_batch** out);
+mp_status mp_batch_destroy(mp_batch* b);
+mp_status mp_batch_add_mesh(mp_batch* b, const mp_mesh_desc* mesh, int32_t* id);
+mp_status mp_batch_remove_mesh(mp_batch* b, int32_t id);
+mp_status mp_batch_upload_state(mp_batch* b, int32_t id, const double* H, const double* C);
+mp_status mp_batch_download_state(mp_batch* b, int32_t id, double* H, double* C);
+mp_status mp_batch_set_face_velocity(mp_batch* b, int32_t id, const double* vnx,
+                                     const double* vny);
+mp_status mp_batch_set_boundary(mp_batch* b, int32_t id, const uint8_t* bottom,
+                                const uint8_t* right, const uint8_t* top, const uint8_t* left);
+mp_status mp_batch_set_inflow(mp_batch* b, int32_t id, const double* inflow_H,
+                              const double* inflow_C);
+/* dt = cfl / max rate of mesh id (returned, not applied). */
+mp_status mp_batch_stable_dt(mp_batch* b, int32_t id, double cfl, double* dt);
+/* Fixed step of mesh id for the following mp_batch_step calls. */
+mp_status mp_batch_set_dt(mp_batch* b, int32_t id, double dt);
+/* Advance every resident mesh nsteps with its own dt.  flags: MP_BATCH_GRAPH
+ * replays a hipGraph of the whole call (captured on first use, re-captured
+ * only when membership, schedule or nsteps change).  reports[id] for each
+ * resident id (array of capacity entries, may be NULL).  A mesh whose step
+ * fails keeps its state and reports its error bits; the others advance.  The
+ * status is that of the first failed mesh. */
+mp_status mp_batch_step(mp_batch* b, int32_t nsteps, int32_t scheme, int32_t flags,
+                        mp_step_report* reports);
+mp_status mp_batch_phase(mp_batch* b, int32_t id, double* T, double* phi, double* Cl,
+                         double* Cs, uint8_t* region);
+/* Schedule: rows per work item (>= 8) and persistent grid size in blocks of
+ * four waves (0: one wave per work item). */
+mp_status mp_batch_set_schedule(mp_batch* b, int32_t rows_per_item, int32_t blocks);
+mp_status mp_batch_get_schedule(mp_batch* b, int32_t* rows_per_item, int32_t* blocks);
+/* Time candidate schedules on the resident meshes (state is left unchanged),
+ * apply the fastest and return it.  cache_path (may be NULL) is a JSON file
+ * keyed by device, precision, scheme and batch shape: a hit skips the search,
+ * a miss appends the result. */
+mp_status mp_batch_autotune(mp_batch* b, int32_t nsteps, int32_t scheme, const char* cache_path,
+                            int32_t* rows_per_item, int32_t* blocks, double* ms_per_step);
+/* Time nsteps (state unchanged): ms_total, mean stage launch, mean
+ * phase-fused stage launch (HIP events; eager launches). */
+mp_status mp_batch_profile(mp_batch* b, int32_t nsteps, int32_t scheme, double* ms_total,
+                           double* ms_stage, double* ms_phase_stage);
+/* Resident meshes, their cells, work items and arena bytes in use. */
+mp_status mp_batch_info(mp_batch* b, int32_t* meshes, int64_t* cells, int64_t* items,
+                        uint64_t* hbm_in_use);
+
+/* ------------------------------------------------------------- serving --- */
+/* Continuous batching of transport runs.  Requests arrive at any time from any
+ * thread; a server thread admits them into slots of one device engine while
+ * slots and the HBM budget allow, advances all resident runs together in
+ * epochs of at most max_epoch_steps steps (one persistent launch per stage),
+ * and retires each run after its last step: outputs are written to the
+ * caller's arrays and the slot and memory are released.  A run's results do
+ * not depend on which other runs shared its launches.  No reference
+ * counterpart (the reference is a batch MPI program). */
+typedef struct {
+    mp_mesh_desc mesh;
+    const double* H;          /* initial cell averages, j*nx+i */
+    const double* C;
+    const double* vnx;        /* face velocities as mp_transport_set_face_velocity */
+    const double* vny;
+    const uint8_t* bc[4];     /* bottom, right, top, left policies (NULL: zero flux) */
+    double cfl;               /* dt = cfl / max rate */
+    int32_t nsteps;
+    /* outputs (any may be NULL); written before the request completes */
+    double* H_out;
+    double* C_out;
+    double* T;
+    double* phi;
+    double* Cl;
+    double* Cs;
+    uint8_t* region;
+} mp_request;
+
+typedef struct {
+    int64_t id;
+    int32_t status;               /* mp_status of the run */
+    uint32_t error_bits;
+    int32_t steps_done;
+    double dt;
+    double total_H, total_C, boundary_H, boundary_C;
+    double t_submit, t_admit, t_done;   /* seconds on the server clock */
+} mp_request_result;
+
+typedef struct {
+    int64_t submitted, completed, failed, queued, resident;
+    double cell_steps;            /* sum over completed runs of cells * steps */
+    double elapsed_s;             /* first submit to last completion */
+    double p50_ms, p99_ms, mean_ms, max_ms;   /* submit -> done latency */
+    uint64_t hbm_in_use, hbm_peak, hbm_budget;
+    int64_t epochs, max_resident;
+} mp_server_stats;
+
+/* Input and output arrays of a request must stay valid until it completes. */
+mp_status mp_server_create(int32_t max_slots, const mp_weno_options* weno,
+                           const mp_phase_params* phase, int32_t precision, int32_t scheme,
+                           uint64_t hbm_budget_bytes, int32_t max_epoch_steps, mp_server** out);
+mp_status mp_server_submit(mp_server* s, const mp_request* req, int64_t* id);
+/* Wait up to timeout_s (< 0: forever) for request id; MP_ERR_PENDING on timeout. */
+mp_status mp_server_wait(mp_server* s, int64_t id, double timeout_s, mp_request_result* out);
+/* Wait until every submitted request has completed. */
+mp_status mp_server_drain(mp_server* s);
+mp_status mp_server_stats_get(mp_server* s, mp_server_stats* out);
+/* Rows per work item for runs admitted afterwards (>= 8; default 8).  A run's
+ * reported boundary sums depend on it at rounding level; states and phase
+ * outputs do not. */
+mp_status mp_server_set_tile(mp_server* s, int32_t rows_per_item);
+mp_status mp_server_destroy(mp_server* s);
+/* Seconds on the server clock (the clock of t_submit / t_done). */
+double mp_server_now(mp_server* s);
 
 /* ------------------------------------------------------------- phase only --- */
 /* EutecticModel::evaluate over n cells on the device (host arrays in/out).

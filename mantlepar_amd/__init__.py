@@ -5,3 +5,4 @@ from ._lib import LIB_PATH, MantleError, lib  # noqa: F401
 from .transport import (BC_INFLOW_OUTFLOW, BC_OUTFLOW, BC_ZERO_FLUX, PREC_FP64,  # noqa: F401
                         PREC_MIXED, SCHEME_FE, SCHEME_SSPRK2, SCHEME_SSPRK3, EutecticModel,
                         Transport, default_phase, default_weno)
+from .batch import BATCH_EAGER, BATCH_GRAPH, Batch, Server  # noqa: F401,E402

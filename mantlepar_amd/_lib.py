@@ -14,7 +14,9 @@ LIB_PATH = os.path.join(HERE, "libmantlepar_amd.so")
 MP_OK = 0
 STATUS_NAMES = {1: "MP_ERR_ARG", 2: "MP_ERR_FP", 3: "MP_ERR_STATE", 4: "MP_ERR_PHASE",
                 5: "MP_ERR_DEVICE", 6: "MP_ERR_MEMORY", 7: "MP_ERR_BACKFLOW",
-                8: "MP_ERR_COMPOSITION"}
+                8: "MP_ERR_COMPOSITION", 9: "MP_ERR_PENDING"}
+MP_ERR_PENDING = 9
+BATCH_EAGER, BATCH_GRAPH = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 BC_ZERO_FLUX, BC_OUTFLOW, BC_INFLOW_OUTFLOW = 0, 1, 2
 SCHEME_FE, SCHEME_SSPRK2, SCHEME_SSPRK3 = 1, 2, 3
@@ -56,6 +58,36 @@ class StepReport(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class Request(C.Structure):
+    _fields_ = [("mesh", MeshDesc), ("H", C.c_void_p), ("C", C.c_void_p), ("vnx", C.c_void_p),
+                ("vny", C.c_void_p), ("bc", C.c_void_p * 4), ("cfl", C.c_double),
+                ("nsteps", C.c_int32), ("H_out", C.c_void_p), ("C_out", C.c_void_p),
+                ("T", C.c_void_p), ("phi", C.c_void_p), ("Cl", C.c_void_p), ("Cs", C.c_void_p),
+                ("region", C.c_void_p)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("error_bits", C.c_uint32),
+                ("steps_done", C.c_int32), ("dt", C.c_double), ("total_H", C.c_double),
+                ("total_C", C.c_double), ("boundary_H", C.c_double), ("boundary_C", C.c_double),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("queued", C.c_int64), ("resident", C.c_int64), ("cell_steps", C.c_double),
+                ("elapsed_s", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_ms", C.c_double), ("max_ms", C.c_double), ("hbm_in_use", C.c_uint64),
+                ("hbm_peak", C.c_uint64), ("hbm_budget", C.c_uint64), ("epochs", C.c_int64),
+                ("max_resident", C.c_int64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 _dp = C.POINTER(C.c_double)
 _u8p = C.POINTER(C.c_uint8)
 _vp = C.c_void_p
@@ -84,6 +116,37 @@ SIGNATURES = [
     ("mp_transport_synchronize", C.c_int32, [_vp]),
     ("mp_transport_profile", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double, _dp, _dp, _dp]),
     ("mp_transport_set_tile", C.c_int32, [_vp, C.c_int32]),
+    ("mp_batch_create", C.c_int32, [C.c_int32, C.POINTER(WenoOptions), C.POINTER(PhaseParams),
+                                    C.c_int32, C.c_uint64, _vp, C.POINTER(_vp)]),
+    ("mp_batch_destroy", C.c_int32, [_vp]),
+    ("mp_batch_add_mesh", C.c_int32, [_vp, C.POINTER(MeshDesc), C.POINTER(C.c_int32)]),
+    ("mp_batch_remove_mesh", C.c_int32, [_vp, C.c_int32]),
+    ("mp_batch_upload_state", C.c_int32, [_vp, C.c_int32, _dp, _dp]),
+    ("mp_batch_download_state", C.c_int32, [_vp, C.c_int32, _dp, _dp]),
+    ("mp_batch_set_face_velocity", C.c_int32, [_vp, C.c_int32, _dp, _dp]),
+    ("mp_batch_set_boundary", C.c_int32, [_vp, C.c_int32, _u8p, _u8p, _u8p, _u8p]),
+    ("mp_batch_set_inflow", C.c_int32, [_vp, C.c_int32, _dp, _dp]),
+    ("mp_batch_stable_dt", C.c_int32, [_vp, C.c_int32, C.c_double, _dp]),
+    ("mp_batch_set_dt", C.c_int32, [_vp, C.c_int32, C.c_double]),
+    ("mp_batch_step", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.POINTER(StepReport)]),
+    ("mp_batch_phase", C.c_int32, [_vp, C.c_int32, _dp, _dp, _dp, _dp, _u8p]),
+    ("mp_batch_set_schedule", C.c_int32, [_vp, C.c_int32, C.c_int32]),
+    ("mp_batch_get_schedule", C.c_int32, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("mp_batch_autotune", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_char_p,
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32), _dp]),
+    ("mp_batch_profile", C.c_int32, [_vp, C.c_int32, C.c_int32, _dp, _dp, _dp]),
+    ("mp_batch_info", C.c_int32, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
+    ("mp_server_create", C.c_int32, [C.c_int32, C.POINTER(WenoOptions), C.POINTER(PhaseParams),
+                                     C.c_int32, C.c_int32, C.c_uint64, C.c_int32,
+                                     C.POINTER(_vp)]),
+    ("mp_server_submit", C.c_int32, [_vp, C.POINTER(Request), C.POINTER(C.c_int64)]),
+    ("mp_server_wait", C.c_int32, [_vp, C.c_int64, C.c_double, C.POINTER(RequestResult)]),
+    ("mp_server_drain", C.c_int32, [_vp]),
+    ("mp_server_stats_get", C.c_int32, [_vp, C.POINTER(ServerStats)]),
+    ("mp_server_set_tile", C.c_int32, [_vp, C.c_int32]),
+    ("mp_server_destroy", C.c_int32, [_vp]),
+    ("mp_server_now", C.c_double, [_vp]),
     ("mp_phase_evaluate", C.c_int32, [C.POINTER(PhaseParams), C.c_int64, _dp, _dp, _dp, _dp,
                                       _dp, _dp, _dp, _u8p, _dp, _dp, _dp, _dp]),
     ("mp_phase_boundaries", C.c_int32, [C.POINTER(PhaseParams), C.c_double, C.c_double, _dp,

@@ -1,0 +1,557 @@
+// Multi-mesh engine: arena, slots, work lists and stage sequencing.
+#include "mp_engine.h"
+
+#include <cmath>
+#include <algorithm>
+#include <climits>
+#include <cstring>
+
+#include "mp_launch.h"
+
+namespace mp {
+
+// Stage tables as in the single-mesh handle.
+static const double kA[3][3] = {{0, 0, 0}, {0, 0.5, 0}, {0, 0.75, 1.0 / 3.0}};
+static const double kB[3][3] = {{1, 0, 0}, {1, 0.5, 0}, {1, 0.25, 2.0 / 3.0}};
+static const double kW[3][3] = {{1, 0, 0}, {0.5, 0.5, 0}, {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0}};
+
+// ------------------------------------------------------------------ Arena
+
+Arena::~Arena()
+{
+    for (auto& c : chunks_) (void)hipFree(c.base);
+}
+
+void Arena::configure(size_t budget_bytes, size_t chunk_bytes)
+{
+    budget_ = budget_bytes;
+    chunk_ = chunk_bytes;
+}
+
+mp_status Arena::alloc(size_t bytes, void** out)
+{
+    const size_t need = round(bytes);
+    if (in_use_ + need > budget_) return fail(MP_ERR_MEMORY, "HBM budget exhausted (%zu + %zu > %zu)", in_use_, need, budget_);
+    for (size_t ci = 0; ci < chunks_.size(); ++ci) {
+        Chunk& c = chunks_[ci];
+        for (auto it = c.free.begin(); it != c.free.end(); ++it) {
+            if (it->second < need) continue;
+            const size_t off = it->first, sz = it->second;
+            c.free.erase(it);
+            if (sz > need) c.free[off + need] = sz - need;
+            void* p = c.base + off;
+            live_[p] = {ci, need};
+            in_use_ += need;
+            *out = p;
+            return MP_OK;
+        }
+    }
+    Chunk c;
+    const size_t chunk = chunk_ < budget_ ? chunk_ : budget_;
+    c.size = need > chunk ? need : chunk;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&c.base), c.size);
+    if (e != hipSuccess) return fail(MP_ERR_MEMORY, "hipMalloc(%zu) failed: %s", c.size, hipGetErrorString(e));
+    if (c.size > need) c.free[need] = c.size - need;
+    chunks_.push_back(c);
+    void* p = chunks_.back().base;
+    live_[p] = {chunks_.size() - 1, need};
+    in_use_ += need;
+    *out = p;
+    return MP_OK;
+}
+
+void Arena::release(void* p)
+{
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    Chunk& c = chunks_[it->second.first];
+    size_t off = size_t(static_cast<char*>(p) - c.base), sz = it->second.second;
+    in_use_ -= sz;
+    live_.erase(it);
+    // coalesce with neighbours
+    auto next = c.free.lower_bound(off);
+    if (next != c.free.end() && next->first == off + sz) {
+        sz += next->second;
+        next = c.free.erase(next);
+    }
+    if (next != c.free.begin()) {
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == off) {
+            prev->second += sz;
+            return;
+        }
+    }
+    c.free[off] = sz;
+}
+
+// ----------------------------------------------------------------- Engine
+
+Engine::~Engine()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+    if (h_stages) (void)hipHostFree(h_stages);
+}
+
+mp_status Engine::init(int cap, int32_t prec, const mp_weno_options& w, const mp_phase_params& p,
+                       void* s, size_t budget_bytes)
+{
+    if (cap < 1) return fail(MP_ERR_ARG, "capacity must be positive");
+    if (prec != MP_PREC_FP64 && prec != MP_PREC_MIXED) return fail(MP_ERR_ARG, "unknown precision");
+    MP_CALL(validate_weno(w));
+    MP_CALL(make_phase(p, phase));
+    precision = prec;
+    weno = w;
+    phase_params = p;
+    capacity = cap;
+    if (s) {
+        stream = static_cast<hipStream_t>(s);
+    } else {
+        MP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        own_stream = true;
+    }
+    arena.configure(budget_bytes, size_t(1) << 30);
+    slots.assign(size_t(cap), Slot());
+    MP_CALL(d_meshes.alloc(sizeof(MeshDev) * cap));
+    MP_CALL(d_stages.alloc(sizeof(StagePtrs) * 9 * cap));
+    MP_CALL(d_nitems.alloc(sizeof(int) * cap));
+    MP_CALL(d_active.alloc(sizeof(int) * cap));
+    MP_CALL(d_errors.alloc(sizeof(uint32_t) * cap));
+    MP_CALL(d_maxbits.alloc(sizeof(unsigned long long) * cap));
+    MP_CALL(reserve_steps(1));
+    MP_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_stages), sizeof(StagePtrs) * 9 * cap,
+                         hipHostMallocDefault));
+    std::memset(h_stages, 0, sizeof(StagePtrs) * 9 * cap);
+    h_nitems.assign(size_t(cap), 0);
+    return MP_OK;
+}
+
+static size_t faces_x(const mp_mesh_desc& d) { return size_t(d.ny) * (d.nx + 1); }
+static size_t faces_y(const mp_mesh_desc& d) { return size_t(d.ny + 1) * d.nx; }
+
+static int items_for(const mp_mesh_desc& d, int rows)
+{
+    const int strips = (d.nx + kStripCells - 1) / kStripCells;
+    return strips * ((d.ny + rows - 1) / rows);
+}
+
+size_t Engine::slot_bytes(const mp_mesh_desc& d, int rows_min)
+{
+    const size_t N = size_t(d.nx) * d.ny;
+    size_t b = 0;
+    b += Arena::round(10 * N * sizeof(double));
+    b += Arena::round(size_t(d.q) * faces_x(d) * sizeof(double));
+    b += Arena::round(size_t(d.q) * faces_y(d) * sizeof(double));
+    b += Arena::round(2 * faces_x(d) * sizeof(double)) + Arena::round(4 * faces_x(d) * sizeof(float));
+    b += Arena::round(2 * faces_y(d) * sizeof(double)) + Arena::round(4 * faces_y(d) * sizeof(float));
+    b += 2 * Arena::round(N * (4 * sizeof(double) + 1));
+    b += Arena::round(size_t(3) * items_for(d, rows_min) * kPartials * sizeof(double));
+    b += Arena::round(size_t(2) * (d.nx + d.ny));
+    b += Arena::round(size_t(4) * (d.nx + d.ny) * d.q * sizeof(double));
+    return b;
+}
+
+mp_status Engine::add_mesh(const mp_mesh_desc& d, int* out)
+{
+    int k = -1;
+    for (int i = 0; i < capacity; ++i)
+        if (!slots[i].live) { k = i; break; }
+    if (k < 0) return fail(MP_ERR_MEMORY, "no free mesh slot (capacity %d)", capacity);
+    Slot sl;
+    MP_CALL(make_mesh_dev(d, weno, phase, sl.md));
+    sl.desc = d;
+    sl.N = size_t(d.nx) * d.ny;
+    MP_CALL(arena.alloc(slot_bytes(d), &sl.block));
+    char* p = static_cast<char*>(sl.block);
+    auto take = [&](size_t bytes) { char* r = p; p += Arena::round(bytes); return r; };
+    const size_t N = sl.N;
+    double* states = reinterpret_cast<double*>(take(10 * N * sizeof(double)));
+    for (int i = 0; i < 5; ++i) {
+        sl.H[i] = states + size_t(2 * i) * N;
+        sl.C[i] = states + size_t(2 * i + 1) * N;
+    }
+    sl.md.vnx = reinterpret_cast<double*>(take(size_t(d.q) * faces_x(d) * sizeof(double)));
+    sl.md.vny = reinterpret_cast<double*>(take(size_t(d.q) * faces_y(d) * sizeof(double)));
+    sl.md.fxv = reinterpret_cast<double*>(take(2 * faces_x(d) * sizeof(double)));
+    sl.md.fxm = reinterpret_cast<float*>(take(4 * faces_x(d) * sizeof(float)));
+    sl.md.fyv = reinterpret_cast<double*>(take(2 * faces_y(d) * sizeof(double)));
+    sl.md.fym = reinterpret_cast<float*>(take(4 * faces_y(d) * sizeof(float)));
+    sl.phase[0] = reinterpret_cast<double*>(take(N * (4 * sizeof(double) + 1)));
+    sl.phase[1] = reinterpret_cast<double*>(take(N * (4 * sizeof(double) + 1)));
+    sl.partials = reinterpret_cast<double*>(take(size_t(3) * items_for(d, 8) * kPartials * sizeof(double)));
+    sl.bc = reinterpret_cast<uint8_t*>(take(size_t(2) * (d.nx + d.ny)));
+    sl.inflow = reinterpret_cast<double*>(take(size_t(4) * (d.nx + d.ny) * d.q * sizeof(double)));
+    sl.live = true;
+    slots[k] = sl;
+    set_rows(k, 64);
+    mp_status s = MP_OK;
+    // zero state until the first upload (as a fresh single-mesh handle)
+    hipError_t e = hipMemsetAsync(sl.H[0], 0, 10 * N * sizeof(double), stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_meshes.as<MeshDev>() + k, &slots[k].md, sizeof(MeshDev),
+                                  hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) s = fail(MP_ERR_DEVICE, "mesh upload: %s", hipGetErrorString(e));
+    if (s) {
+        remove_mesh(k);
+        return s;
+    }
+    *out = k;
+    return MP_OK;
+}
+
+void Engine::remove_mesh(int k)
+{
+    if (k < 0 || k >= capacity || !slots[k].live) return;
+    arena.release(slots[k].block);
+    slots[k] = Slot();
+}
+
+void Engine::set_rows(int k, int rows)
+{
+    Slot& s = slots[k];
+    s.rows = rows < 8 ? 8 : (rows > s.desc.ny ? s.desc.ny : rows);
+    if (s.rows < 8) s.rows = 8;
+    s.strips = (s.desc.nx + kStripCells - 1) / kStripCells;
+    s.bands = (s.desc.ny + s.rows - 1) / s.rows;
+    s.nitems = s.strips * s.bands;
+}
+
+mp_status Engine::upload_async(int k, const double* H, const double* C)
+{
+    Slot& s = slots[k];
+    if (!H || !C) return fail(MP_ERR_ARG, "null state array");
+    if (!all_finite(H, s.N) || !all_finite(C, s.N)) return fail(MP_ERR_ARG, "Cell averages must be finite");
+    MP_HIP(hipMemcpyAsync(s.H[s.cur], H, s.N * sizeof(double), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(s.C[s.cur], C, s.N * sizeof(double), hipMemcpyHostToDevice, stream));
+    s.phase_valid = false;
+    return MP_OK;
+}
+
+mp_status Engine::upload(int k, const double* H, const double* C)
+{
+    MP_CALL(upload_async(k, H, C));
+    MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+mp_status Engine::download_async(int k, double* H, double* C)
+{
+    Slot& s = slots[k];
+    if (H) MP_HIP(hipMemcpyAsync(H, s.H[s.cur], s.N * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (C) MP_HIP(hipMemcpyAsync(C, s.C[s.cur], s.N * sizeof(double), hipMemcpyDeviceToHost, stream));
+    return MP_OK;
+}
+
+mp_status Engine::download(int k, double* H, double* C)
+{
+    MP_CALL(download_async(k, H, C));
+    MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+mp_status Engine::set_velocity(int k, const double* vx, const double* vy, bool wait)
+{
+    Slot& s = slots[k];
+    const mp_mesh_desc& d = s.desc;
+    const size_t nxv = size_t(d.q) * faces_x(d), nyv = size_t(d.q) * faces_y(d);
+    if (!vx || !vy) return fail(MP_ERR_ARG, "null velocity array");
+    if (!all_finite(vx, nxv) || !all_finite(vy, nyv)) return fail(MP_ERR_ARG, "Edge velocity must be finite");
+    if (weno.lf_mode == 1) {
+        double vmax = 0;
+        for (size_t i = 0; i < nxv; ++i) vmax = std::fmax(vmax, std::fabs(vx[i]));
+        for (size_t i = 0; i < nyv; ++i) vmax = std::fmax(vmax, std::fabs(vy[i]));
+        if (weno.global_speed + 1e-14 < vmax)
+            return fail(MP_ERR_ARG, "Global LF speed is smaller than the flow normal speed");
+    }
+    MP_HIP(hipMemcpyAsync(const_cast<double*>(s.md.vnx), vx, nxv * sizeof(double), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(const_cast<double*>(s.md.vny), vy, nyv * sizeof(double), hipMemcpyHostToDevice, stream));
+    MeshDev* dm = d_meshes.as<MeshDev>() + k;
+    MP_HIP(launch_face_prep(dm, d.nx, d.ny, const_cast<double*>(s.md.fxv), const_cast<float*>(s.md.fxm),
+                            const_cast<double*>(s.md.fyv), const_cast<float*>(s.md.fym), stream));
+    unsigned long long* bits = d_maxbits.as<unsigned long long>() + k;
+    MP_CALL(launch_clear(reinterpret_cast<uint32_t*>(bits), 2, stream) == hipSuccess
+                ? MP_OK : fail(MP_ERR_DEVICE, "clear failed"));
+    MP_HIP(launch_rate(dm, int(s.N), bits, stream));
+    if (wait) {
+        unsigned long long host = 0;
+        MP_HIP(hipMemcpyAsync(&host, bits, sizeof host, hipMemcpyDeviceToHost, stream));
+        MP_HIP(hipStreamSynchronize(stream));
+        std::memcpy(&s.max_rate, &host, sizeof s.max_rate);
+    }
+    s.have_velocity = true;
+    return MP_OK;
+}
+
+mp_status Engine::set_boundary(int k, const uint8_t* const side[4])
+{
+    Slot& s = slots[k];
+    const mp_mesh_desc& d = s.desc;
+    const size_t len[4] = {size_t(d.nx), size_t(d.ny), size_t(d.nx), size_t(d.ny)};
+    const size_t off[4] = {0, len[0], len[0] + len[1], len[0] + len[1] + len[2]};
+    for (int f = 0; f < 4; ++f)
+        if (side[f])
+            for (size_t i = 0; i < len[f]; ++i)
+                if (side[f][i] > MP_BC_INFLOW_OUTFLOW) return fail(MP_ERR_ARG, "Unknown advection boundary policy");
+    for (int f = 0; f < 4; ++f) {
+        if (!side[f]) {
+            s.md.bc[f] = nullptr;
+            continue;
+        }
+        MP_HIP(hipMemcpyAsync(s.bc + off[f], side[f], len[f], hipMemcpyHostToDevice, stream));
+        s.md.bc[f] = s.bc + off[f];
+    }
+    MP_HIP(hipMemcpyAsync(d_meshes.as<MeshDev>() + k, &s.md, sizeof(MeshDev), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+mp_status Engine::set_inflow(int k, const double* h, const double* c)
+{
+    Slot& s = slots[k];
+    const mp_mesh_desc& d = s.desc;
+    const size_t total = size_t(2) * (d.nx + d.ny) * d.q;
+    const size_t off[4] = {0, size_t(d.nx) * d.q, size_t(d.nx + d.ny) * d.q, size_t(2 * d.nx + d.ny) * d.q};
+    for (int f = 0; f < 2; ++f) {
+        const double* src = f ? c : h;
+        if (src && !all_finite(src, total))
+            return fail(MP_ERR_FP, "Inflow boundary callback must return a finite state");
+    }
+    for (int f = 0; f < 2; ++f) {
+        const double* src = f ? c : h;
+        const double** dst = f ? s.md.inflowC : s.md.inflowH;
+        double* buf = s.inflow + size_t(f) * total;
+        if (!src) {
+            for (int e = 0; e < 4; ++e) dst[e] = nullptr;
+            continue;
+        }
+        MP_HIP(hipMemcpyAsync(buf, src, total * sizeof(double), hipMemcpyHostToDevice, stream));
+        for (int e = 0; e < 4; ++e) dst[e] = buf + off[e];
+    }
+    MP_HIP(hipMemcpyAsync(d_meshes.as<MeshDev>() + k, &s.md, sizeof(MeshDev), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+mp_status Engine::phase_out(int k, double* T, double* phi, double* Cl, double* Cs, uint8_t* region,
+                            bool async)
+{
+    Slot& s = slots[k];
+    if (!s.phase_valid) {
+        double* base = s.phase[s.phase_cur];
+        MP_HIP(launch_clear(d_errors.as<uint32_t>() + k, 1, stream));
+        MP_HIP(launch_state_phase(d_meshes.as<MeshDev>() + k, int(s.N), s.H[s.cur], s.C[s.cur], base,
+                                  base + s.N, base + 2 * s.N, base + 3 * s.N,
+                                  reinterpret_cast<uint8_t*>(base + 4 * s.N),
+                                  d_errors.as<uint32_t>() + k, stream));
+        uint32_t bits = 0;
+        MP_HIP(hipMemcpyAsync(&bits, d_errors.as<uint32_t>() + k, sizeof bits, hipMemcpyDeviceToHost, stream));
+        MP_HIP(hipStreamSynchronize(stream));
+        if (bits) return fail(status_from_bits(bits), "%s (device error bits 0x%x)", describe_bits(bits), bits);
+        s.phase_valid = true;
+    }
+    const double* base = s.phase[s.phase_cur];
+    double* outs[4] = {T, phi, Cl, Cs};
+    for (int i = 0; i < 4; ++i)
+        if (outs[i]) MP_HIP(hipMemcpyAsync(outs[i], base + i * s.N, s.N * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (region) MP_HIP(hipMemcpyAsync(region, base + 4 * s.N, s.N, hipMemcpyDeviceToHost, stream));
+    if (!async) MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+mp_status Engine::set_active(const std::vector<int>& list, const std::vector<int>& steps)
+{
+    if (!steps.empty() && steps.size() != list.size()) return fail(MP_ERR_ARG, "steps and slots differ in length");
+    std::vector<int> order(list.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = int(i);
+    if (!steps.empty())
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return steps[a] > steps[b]; });
+    active.clear();
+    active_steps.clear();
+    slot_steps.assign(size_t(capacity), 0);
+    q_common = -1;
+    for (int i : order) {
+        const int k = list[i];
+        const Slot& s = slots[k];
+        if (k < 0 || k >= capacity || !s.live) return fail(MP_ERR_STATE, "inactive slot %d in the work list", k);
+        active.push_back(k);
+        active_steps.push_back(steps.empty() ? INT32_MAX : steps[i]);
+        slot_steps[k] = active_steps.back();
+        h_nitems[k] = s.nitems;
+        q_common = q_common < 0 ? s.desc.q : (q_common == s.desc.q ? q_common : 0);
+    }
+    if (q_common < 0) q_common = 0;
+    // One interleaved item list per distinct prefix of the ordered slots (a
+    // prefix is the set still stepping at some step).  Interleaving meshes
+    // band by band lets a persistent wave stride spread each mesh over the
+    // whole chip.
+    std::vector<int> prefixes;
+    for (size_t i = 0; i < active.size(); ++i)
+        if (i + 1 == active.size() || active_steps[i + 1] != active_steps[i]) prefixes.push_back(int(i + 1));
+    std::vector<int2> items;
+    lists.clear();
+    for (int p : prefixes) {
+        const int off = int(items.size());
+        int maxn = 0;
+        for (int i = 0; i < p; ++i) maxn = std::max(maxn, slots[active[i]].nitems);
+        for (int it = 0; it < maxn; ++it)
+            for (int i = 0; i < p; ++i)
+                if (it < slots[active[i]].nitems) items.push_back(make_int2(active[i], it));
+        lists[p] = {off, int(items.size()) - off};
+    }
+    nitems_total = int(items.size());
+    ++generation;
+    if (d_items.bytes < items.size() * sizeof(int2))
+        MP_CALL(d_items.alloc(items.size() * sizeof(int2) + 1024));
+    if (!items.empty())
+        MP_HIP(hipMemcpyAsync(d_items.ptr, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(d_nitems.ptr, h_nitems.data(), sizeof(int) * capacity, hipMemcpyHostToDevice, stream));
+    if (!active.empty())
+        MP_HIP(hipMemcpyAsync(d_active.ptr, active.data(), sizeof(int) * active.size(), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+int Engine::steps_of(int k) const
+{
+    return k >= 0 && k < int(slot_steps.size()) ? slot_steps[k] : 0;
+}
+
+// Three pointer sets per slot: step 0 (cur -> o1), odd steps (o1 -> o2),
+// even steps >= 2 (o2 -> o1); each with the scheme's stages.
+mp_status Engine::write_tables(int scheme)
+{
+    for (int k : active) {
+        Slot& s = slots[k];
+        int o[2], n = 0;
+        for (int i = 0; i < 3; ++i)
+            if (i != s.cur) o[n++] = i;
+        const int src_of[3] = {s.cur, o[0], o[1]};
+        const int dst_of[3] = {o[0], o[1], o[0]};
+        for (int set = 0; set < 3; ++set)
+            for (int st = 0; st < 3; ++st) {
+                StagePtrs sp;
+                std::memset(&sp, 0, sizeof sp);
+                const int src = src_of[set], dst = dst_of[set];
+                const bool last = st == scheme - 1;
+                const int kin = st == 0 ? src : 3 + (st - 1);
+                const int out = last ? dst : 3 + st;
+                sp.Hn = s.H[src]; sp.Cn = s.C[src];
+                sp.Hk = s.H[kin]; sp.Ck = s.C[kin];
+                sp.Ho = s.H[out]; sp.Co = s.C[out];
+                sp.a = st < scheme ? kA[scheme - 1][st] : 0.0;
+                sp.b = st < scheme ? kB[scheme - 1][st] : 0.0;
+                sp.dt = s.dt;
+                if (last) {
+                    double* base = s.phase[1 - s.phase_cur];
+                    sp.T = base; sp.phi = base + s.N; sp.Cl = base + 2 * s.N; sp.Cs = base + 3 * s.N;
+                    sp.region = reinterpret_cast<uint8_t*>(base + 4 * s.N);
+                }
+                sp.partials = s.partials + size_t(st) * s.nitems * kPartials;
+                sp.error = d_errors.as<uint32_t>() + k;
+                sp.rows = s.rows;
+                sp.strips = s.strips;
+                h_stages[size_t(set * 3 + st) * capacity + k] = sp;
+            }
+    }
+    MP_HIP(hipMemcpyAsync(d_stages.ptr, h_stages, sizeof(StagePtrs) * 9 * capacity,
+                          hipMemcpyHostToDevice, stream));
+    return MP_OK;
+}
+
+mp_status Engine::reserve_steps(int nsteps)
+{
+    if (nsteps <= steps_capacity) return MP_OK;
+    MP_CALL(d_totals.alloc(sizeof(double) * 4 * size_t(capacity) * nsteps));
+    steps_capacity = nsteps;
+    ++generation;
+    return MP_OK;
+}
+
+mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* events)
+{
+    if (nsteps > steps_capacity) return fail(MP_ERR_STATE, "per-step totals not reserved");
+    MP_HIP(launch_clear(d_errors.as<uint32_t>(), capacity, stream));
+    const StagePtrs* base = d_stages.as<StagePtrs>();
+    for (int s = 0; s < nsteps; ++s) {
+        const int set = s == 0 ? 0 : (s % 2 ? 1 : 2);
+        int p = 0;      // slots still stepping: a prefix of the ordered list
+        while (p < int(active.size()) && active_steps[p] > s) ++p;
+        if (p == 0) {
+            if (events)
+                for (int st = 0; st < scheme; ++st) {
+                    MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
+                    MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
+                }
+            continue;
+        }
+        const auto& L = lists.at(p);
+        for (int st = 0; st < scheme; ++st) {
+            if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
+            MP_HIP(launch_batch_stage(precision, st == scheme - 1, q_common, d_meshes.as<MeshDev>(),
+                                      base + size_t(set * 3 + st) * capacity,
+                                      d_items.as<int2>() + L.first, L.second, blocks, stream));
+            if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
+        }
+        MP_HIP(launch_batch_reduce(base + size_t(set * 3) * capacity, d_nitems.as<int>(),
+                                   d_meshes.as<MeshDev>(), d_active.as<int>(), p, scheme,
+                                   kW[scheme - 1], d_totals.as<double>() + size_t(s) * 4 * capacity,
+                                   stream));
+    }
+    return MP_OK;
+}
+
+mp_status Engine::fetch_status(int nsteps, std::vector<uint32_t>& errors, std::vector<double>* totals)
+{
+    errors.assign(size_t(capacity), 0u);
+    MP_HIP(hipMemcpyAsync(errors.data(), d_errors.ptr, sizeof(uint32_t) * capacity, hipMemcpyDeviceToHost, stream));
+    if (totals && nsteps > 0) {
+        totals->assign(size_t(4) * capacity * nsteps, 0.0);
+        MP_HIP(hipMemcpyAsync(totals->data(), d_totals.ptr, sizeof(double) * totals->size(),
+                              hipMemcpyDeviceToHost, stream));
+    }
+    MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+void Engine::make_report(int k, int nsteps, const std::vector<double>& tot, uint32_t bits,
+                         mp_step_report* r) const
+{
+    const Slot& s = slots[k];
+    std::memset(r, 0, sizeof *r);
+    r->dt = s.dt;
+    r->max_rate = s.max_rate;
+    r->courant = s.dt * s.max_rate;
+    r->error_bits = bits;
+    r->steps_done = bits ? 0 : nsteps;
+    if (bits || nsteps <= 0) return;
+    long double bh = 0, bc = 0;
+    for (int st = 0; st < nsteps; ++st) {
+        const double* t = tot.data() + (size_t(st) * capacity + k) * 4;
+        bh += (long double)s.dt * t[2];
+        bc += (long double)s.dt * t[3];
+    }
+    r->boundary_H = double(bh);
+    r->boundary_C = double(bc);
+    const double* t = tot.data() + (size_t(nsteps - 1) * capacity + k) * 4;
+    r->total_H = t[0];
+    r->total_C = t[1];
+}
+
+void Engine::commit(int nsteps, const std::vector<uint32_t>& errors)
+{
+    if (nsteps <= 0) return;
+    for (size_t i = 0; i < active.size(); ++i) {
+        const int k = active[i];
+        const int done = std::min(nsteps, active_steps[i]);
+        Slot& s = slots[k];
+        if (errors[k] || done <= 0) continue;
+        int o[2], n = 0;
+        for (int c = 0; c < 3; ++c)
+            if (c != s.cur) o[n++] = c;
+        s.cur = (done - 1) % 2 ? o[1] : o[0];
+        s.phase_cur = 1 - s.phase_cur;
+        s.phase_valid = true;
+    }
+}
+
+}  // namespace mp

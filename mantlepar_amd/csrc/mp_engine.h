@@ -1,0 +1,134 @@
+// Multi-mesh engine shared by the batched (mp_batch_*) and serving
+// (mp_server_*) front ends.
+//
+// Every resident mesh occupies one slot: one contiguous block of an HBM
+// arena holding its states, face data, phase outputs and partials, plus a
+// MeshDev record in a device array.  A step over any set of live slots is one
+// persistent launch per stage (batch_sweep_kernel) over a work list of
+// (slot, item) pairs, followed by one per-slot reduction; the stage pointer
+// records live in device memory at fixed addresses, so a captured hipGraph
+// stays valid while the host rewrites them between calls.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <vector>
+
+#include "mp_host.h"
+
+namespace mp {
+
+// First-fit sub-allocator over large device chunks; never frees chunks, so
+// releasing a block never synchronises the device.
+class Arena {
+public:
+    Arena() = default;
+    Arena(const Arena&) = delete;
+    Arena& operator=(const Arena&) = delete;
+    ~Arena();
+    void configure(size_t budget_bytes, size_t chunk_bytes);
+    mp_status alloc(size_t bytes, void** out);
+    void release(void* p);
+    size_t in_use() const { return in_use_; }
+    size_t budget() const { return budget_; }
+    bool fits(size_t bytes) const { return in_use_ + round(bytes) <= budget_; }
+    static size_t round(size_t b) { return (b + 255) & ~size_t(255); }
+
+private:
+    struct Chunk {
+        char* base = nullptr;
+        size_t size = 0;
+        std::map<size_t, size_t> free;   // offset -> size
+    };
+    std::vector<Chunk> chunks_;
+    std::map<void*, std::pair<size_t, size_t>> live_;   // ptr -> (chunk, size)
+    size_t budget_ = 0, chunk_ = size_t(1) << 30, in_use_ = 0;
+};
+
+struct Slot {
+    bool live = false;
+    mp_mesh_desc desc{};
+    MeshDev md{};
+    size_t N = 0;
+    void* block = nullptr;
+    double* H[5] = {};
+    double* C[5] = {};
+    double* phase[2] = {};     // T, phi, Cl, Cs (N each) then region bytes
+    double* partials = nullptr;
+    uint8_t* bc = nullptr;     // bottom(nx) right(ny) top(nx) left(ny) policies
+    double* inflow = nullptr;  // H then C, each [face][q] in side order
+    int cur = 0, phase_cur = 0;
+    bool phase_valid = false, have_velocity = false;
+    double max_rate = 0, dt = 0;
+    int rows = 64, strips = 1, bands = 1, nitems = 1;
+};
+
+class Engine {
+public:
+    Engine() = default;
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+    ~Engine();
+
+    mp_status init(int capacity, int32_t precision, const mp_weno_options& weno,
+                   const mp_phase_params& phase, void* stream, size_t budget_bytes);
+    static size_t slot_bytes(const mp_mesh_desc& d, int rows_min = 8);
+    mp_status add_mesh(const mp_mesh_desc& d, int* slot);
+    void remove_mesh(int slot);
+    mp_status upload(int slot, const double* H, const double* C);
+    mp_status upload_async(int slot, const double* H, const double* C);
+    mp_status download(int slot, double* H, double* C);
+    mp_status download_async(int slot, double* H, double* C);
+    mp_status set_velocity(int slot, const double* vnx, const double* vny, bool wait = true);
+    mp_status set_boundary(int slot, const uint8_t* const side[4]);
+    mp_status set_inflow(int slot, const double* H, const double* C);
+    mp_status phase_out(int slot, double* T, double* phi, double* Cl, double* Cs, uint8_t* region,
+                        bool async = false);
+    void set_rows(int slot, int rows);
+    // Work lists over the given slots (called when membership, tiles or the
+    // step pattern change).  steps[i] (optional) is how many of the next
+    // enqueued steps slot list[i] takes: at step s only slots with steps > s
+    // run, so runs of different lengths share an epoch without padding.
+    mp_status set_active(const std::vector<int>& slots, const std::vector<int>& steps = {});
+    int steps_of(int slot) const;
+    // Enqueue nsteps of `scheme` over the active slots, each with its own dt.
+    // write_tables uploads the per-slot stage pointers (not capturable);
+    // enqueue then issues kernels only, reading device tables, so it may be
+    // captured into a graph.  totals receive [step][slot][4] per-step sums.
+    mp_status write_tables(int scheme);
+    mp_status reserve_steps(int nsteps);
+    mp_status enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* events = nullptr);
+    // After the enqueued work completes: commit each slot whose error word is 0.
+    void commit(int nsteps, const std::vector<uint32_t>& errors);
+    mp_status fetch_status(int nsteps, std::vector<uint32_t>& errors, std::vector<double>* totals);
+    // Per-slot report from the per-step totals (same arithmetic as the
+    // single-mesh handle).
+    void make_report(int slot, int nsteps, const std::vector<double>& totals, uint32_t bits,
+                     mp_step_report* r) const;
+    // Work list generation: bumped whenever device tables or buffers move.
+    uint64_t generation = 0;
+
+    int32_t precision = MP_PREC_FP64;
+    mp_weno_options weno{};
+    mp_phase_params phase_params{};
+    PhaseHost phase{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    Arena arena;
+    std::vector<Slot> slots;
+    std::vector<int> active;         // slots, ordered by steps (descending)
+    std::vector<int> active_steps;   // steps per entry of active (INT32_MAX: all)
+    std::vector<int> slot_steps;     // per slot: steps in the current pattern (0: idle)
+    std::map<int, std::pair<int, int>> lists;   // prefix length -> (item offset, count)
+    int capacity = 0;
+    int blocks = 0;            // persistent grid (0: one block per 4 items)
+    int nitems_total = 0;
+    int steps_capacity = 0;    // per-step totals reserved
+    int q_common = 0;          // q shared by all active slots, or 0
+    DeviceBuffer d_meshes, d_stages, d_items, d_nitems, d_active, d_errors, d_totals, d_maxbits;
+    StagePtrs* h_stages = nullptr;   // pinned mirror of d_stages [9][capacity]
+    std::vector<int> h_nitems;
+};
+
+}  // namespace mp

@@ -5,6 +5,8 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -17,6 +19,7 @@ namespace mp {
 void set_error(const char* fmt, ...);
 mp_status fail(mp_status code, const char* fmt, ...);
 mp_status status_from_bits(uint32_t bits);
+const char* last_error();
 const char* describe_bits(uint32_t bits);
 
 #define MP_HIP(expr)                                                                  \
@@ -31,6 +34,16 @@ const char* describe_bits(uint32_t bits);
         mp_status s_ = (expr);         \
         if (s_ != MP_OK) return s_;    \
     } while (0)
+
+// C ABI bodies: host exceptions become status codes.
+#define MP_GUARD(...)                                                      \
+    try {                                                                  \
+        __VA_ARGS__                                                        \
+    } catch (const std::bad_alloc&) {                                      \
+        return ::mp::fail(MP_ERR_MEMORY, "host allocation failed");        \
+    } catch (const std::exception& e) {                                    \
+        return ::mp::fail(MP_ERR_DEVICE, "%s", e.what());                  \
+    }
 
 // Gauss-Legendre rule on [-1,1], ascending (Newton on P_n in long double).
 void gauss_legendre(int n, double* x, double* w);

@@ -387,6 +387,151 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void stage_sweep_kernel(con
 }
 
 
+// Step totals from the stage partials in a fixed order: 1024 threads take
+// strided slices, then a fixed LDS tree combines them (compensated for the
+// state sums).  out[0..3] = sum H, sum C (times area), stage-weighted
+// boundary H, C.
+constexpr int kReduceThreads = 1024;
+
+__global__ __launch_bounds__(kReduceThreads) void reduce_step_kernel(
+    const double* __restrict__ partials, int nitems, int nstages, double w0, double w1,
+    double w2, double area, double* out)
+{
+    __shared__ double sh[6][kReduceThreads];
+    const int tid = threadIdx.x;
+    double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
+    const double wst[3] = {w0, w1, w2};
+    for (int st = 0; st < nstages; ++st) {
+        const double* p = partials + size_t(st) * nitems * kPartials;
+        const double w = wst[st];
+        const bool last = st == nstages - 1;
+        for (int it = tid; it < nitems; it += kReduceThreads) {
+            const double* q = p + size_t(it) * kPartials;
+            if (last) {
+                two_sum_acc(sH, eH, q[0]);
+                eH += q[1];
+                two_sum_acc(sC, eC, q[2]);
+                eC += q[3];
+            }
+            bH += w * q[4];
+            bC += w * q[5];
+        }
+    }
+    sh[0][tid] = sH; sh[1][tid] = eH; sh[2][tid] = sC; sh[3][tid] = eC; sh[4][tid] = bH; sh[5][tid] = bC;
+    __syncthreads();
+    for (int half = kReduceThreads / 2; half >= 1; half >>= 1) {
+        if (tid < half) {
+            double s0 = sh[0][tid], e0 = sh[1][tid] + sh[1][tid + half];
+            two_sum_acc(s0, e0, sh[0][tid + half]);
+            double s1 = sh[2][tid], e1 = sh[3][tid] + sh[3][tid + half];
+            two_sum_acc(s1, e1, sh[2][tid + half]);
+            sh[0][tid] = s0; sh[1][tid] = e0; sh[2][tid] = s1; sh[3][tid] = e1;
+            sh[4][tid] += sh[4][tid + half];
+            sh[5][tid] += sh[5][tid + half];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        out[0] = (sh[0][0] + sh[1][0]) * area;
+        out[1] = (sh[2][0] + sh[3][0]) * area;
+        out[2] = sh[4][0];
+        out[3] = sh[5][0];
+    }
+}
+
+// ----------------------------------------------------------- batched sweep
+//
+// Many independent meshes in one persistent launch: each wave strides over a
+// work list of (mesh, item) pairs, so meshes of any size share the chip and
+// a small mesh never occupies a launch of its own.  Every mesh keeps its own
+// MeshDev record, stage pointers, partials and error word; its results do not
+// depend on which other meshes share the launch.
+template <int MODE, bool PHASE, int NQ>
+__global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
+    const MeshDev* __restrict__ meshes, const StagePtrs* __restrict__ stages,
+    const int2* __restrict__ items, int nitems)
+{
+    // The work index is wave-uniform; saying so keeps every per-mesh table
+    // load scalar (SGPRs) instead of replicating it across lanes.
+    const int nwaves = gridDim.x * 4;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    for (int it = blockIdx.x * 4 + wave; it < nitems; it += nwaves) {
+        const int2 w = items[it];
+        const int slot = __builtin_amdgcn_readfirstlane(w.x);
+        const int local = __builtin_amdgcn_readfirstlane(w.y);
+        const MeshDev& m = meshes[slot];
+        const StagePtrs& sp = stages[slot];
+        const int strip = local % sp.strips, band = local / sp.strips;
+        const int j0 = band * sp.rows;
+        if (j0 >= m.ny) continue;      // never taken with a current work list
+        const int j1 = min(j0 + sp.rows, m.ny);
+        sweep_item<MODE, PHASE, NQ>(m, sp, local, strip, j0, j1);
+    }
+}
+
+// Per-mesh step totals: one block per mesh, the same fixed order as the
+// single-mesh reduction, so each mesh's totals are bitwise those of the
+// single-mesh path.  Partials of mesh k start at stages[k].partials with
+// nitems[k] items per stage; out[k*4 + 0..3] as in reduce_step_kernel.
+__global__ __launch_bounds__(kReduceThreads) void batch_reduce_kernel(
+    const StagePtrs* __restrict__ stages, const int* __restrict__ nitems,
+    const MeshDev* __restrict__ meshes, const int* __restrict__ slots, int nstages, double w0,
+    double w1, double w2, double* out)
+{
+    __shared__ double sh[6][kReduceThreads];
+    const int k = slots[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int n = nitems[k];
+    const double* base = stages[k].partials;
+    double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
+    const double wst[3] = {w0, w1, w2};
+    for (int st = 0; st < nstages; ++st) {
+        const double* p = base + size_t(st) * n * kPartials;
+        const double w = wst[st];
+        const bool last = st == nstages - 1;
+        for (int it = tid; it < n; it += kReduceThreads) {
+            const double* q = p + size_t(it) * kPartials;
+            if (last) {
+                two_sum_acc(sH, eH, q[0]);
+                eH += q[1];
+                two_sum_acc(sC, eC, q[2]);
+                eC += q[3];
+            }
+            bH += w * q[4];
+            bC += w * q[5];
+        }
+    }
+    sh[0][tid] = sH; sh[1][tid] = eH; sh[2][tid] = sC; sh[3][tid] = eC; sh[4][tid] = bH; sh[5][tid] = bC;
+    __syncthreads();
+    for (int half = kReduceThreads / 2; half >= 1; half >>= 1) {
+        if (tid < half) {
+            double s0 = sh[0][tid], e0 = sh[1][tid] + sh[1][tid + half];
+            two_sum_acc(s0, e0, sh[0][tid + half]);
+            double s1 = sh[2][tid], e1 = sh[3][tid] + sh[3][tid + half];
+            two_sum_acc(s1, e1, sh[2][tid + half]);
+            sh[0][tid] = s0; sh[1][tid] = e0; sh[2][tid] = s1; sh[3][tid] = e1;
+            sh[4][tid] += sh[4][tid + half];
+            sh[5][tid] += sh[5][tid + half];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double area = meshes[k].area;
+        double* o = out + size_t(k) * 4;
+        o[0] = (sh[0][0] + sh[1][0]) * area;
+        o[1] = (sh[2][0] + sh[3][0]) * area;
+        o[2] = sh[4][0];
+        o[3] = sh[5][0];
+    }
+}
+
+// Zero a small device array with a kernel (graph-safe: no memset node).
+__global__ void clear_kernel(uint32_t* p, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0u;
+}
+
 // ------------------------------------------------------------ diagnostics
 
 template <int MODE>
@@ -560,58 +705,6 @@ __global__ void state_phase_kernel(const MeshDev* __restrict__ mesh, const doubl
     if (err) atomicOr(error, err);
 }
 
-// Step totals from the stage partials in a fixed order: 1024 threads take
-// strided slices, then a fixed LDS tree combines them (compensated for the
-// state sums).  out[0..3] = sum H, sum C (times area), stage-weighted
-// boundary H, C.
-constexpr int kReduceThreads = 1024;
-
-__global__ __launch_bounds__(kReduceThreads) void reduce_step_kernel(
-    const double* __restrict__ partials, int nitems, int nstages, double w0, double w1,
-    double w2, double area, double* out)
-{
-    __shared__ double sh[6][kReduceThreads];
-    const int tid = threadIdx.x;
-    double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
-    const double wst[3] = {w0, w1, w2};
-    for (int st = 0; st < nstages; ++st) {
-        const double* p = partials + size_t(st) * nitems * kPartials;
-        const double w = wst[st];
-        const bool last = st == nstages - 1;
-        for (int it = tid; it < nitems; it += kReduceThreads) {
-            const double* q = p + size_t(it) * kPartials;
-            if (last) {
-                two_sum_acc(sH, eH, q[0]);
-                eH += q[1];
-                two_sum_acc(sC, eC, q[2]);
-                eC += q[3];
-            }
-            bH += w * q[4];
-            bC += w * q[5];
-        }
-    }
-    sh[0][tid] = sH; sh[1][tid] = eH; sh[2][tid] = sC; sh[3][tid] = eC; sh[4][tid] = bH; sh[5][tid] = bC;
-    __syncthreads();
-    for (int half = kReduceThreads / 2; half >= 1; half >>= 1) {
-        if (tid < half) {
-            double s0 = sh[0][tid], e0 = sh[1][tid] + sh[1][tid + half];
-            two_sum_acc(s0, e0, sh[0][tid + half]);
-            double s1 = sh[2][tid], e1 = sh[3][tid] + sh[3][tid + half];
-            two_sum_acc(s1, e1, sh[2][tid + half]);
-            sh[0][tid] = s0; sh[1][tid] = e0; sh[2][tid] = s1; sh[3][tid] = e1;
-            sh[4][tid] += sh[4][tid + half];
-            sh[5][tid] += sh[5][tid + half];
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        out[0] = (sh[0][0] + sh[1][0]) * area;
-        out[1] = (sh[2][0] + sh[3][0]) * area;
-        out[2] = sh[4][0];
-        out[3] = sh[5][0];
-    }
-}
-
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
@@ -693,6 +786,42 @@ hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, c
                               double area, double* out, hipStream_t stream)
 {
     reduce_step_kernel<<<1, kReduceThreads, 0, stream>>>(partials, nitems, nstages, w[0], w[1], w[2], area, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes,
+                              const StagePtrs* stages, const int2* items, int nitems, int blocks,
+                              hipStream_t stream)
+{
+    if (nitems <= 0) return hipSuccess;
+    const dim3 grid(blocks > 0 ? blocks : (nitems + 3) / 4), block(256);
+    if (mode == 0 && q == 5) {
+        if (phase) batch_sweep_kernel<0, true, 5><<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
+        else batch_sweep_kernel<0, false, 5><<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
+    } else if (mode == 0) {
+        if (phase) batch_sweep_kernel<0, true, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
+        else batch_sweep_kernel<0, false, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
+    } else {
+        if (phase) batch_sweep_kernel<1, true, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
+        else batch_sweep_kernel<1, false, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reduce(const StagePtrs* stages, const int* nitems, const MeshDev* meshes,
+                               const int* slots, int nslots, int nstages, const double* w,
+                               double* out, hipStream_t stream)
+{
+    if (nslots <= 0) return hipSuccess;
+    batch_reduce_kernel<<<nslots, kReduceThreads, 0, stream>>>(stages, nitems, meshes, slots,
+                                                               nstages, w[0], w[1], w[2], out);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear(uint32_t* p, int n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    clear_kernel<<<(n + 255) / 256, 256, 0, stream>>>(p, n);
     return hipGetLastError();
 }
 

@@ -27,6 +27,13 @@ hipError_t launch_phase(const PhaseDev& ph, double TDe0, double nu, double dT, i
 hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H, const double* C,
                               double* T, double* phi, double* Cl, double* Cs, uint8_t* region,
                               uint32_t* error, hipStream_t stream);
+hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes,
+                              const StagePtrs* stages, const int2* items, int nitems, int blocks,
+                              hipStream_t stream);
+hipError_t launch_batch_reduce(const StagePtrs* stages, const int* nitems, const MeshDev* meshes,
+                               const int* slots, int nslots, int nstages, const double* w,
+                               double* out, hipStream_t stream);
+hipError_t launch_clear(uint32_t* p, int n, hipStream_t stream);
 hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, const double* w,
                               double area, double* out, hipStream_t stream);
 

@@ -78,8 +78,24 @@ mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
     MP_HIP(hipMemsetAsync(vnx.ptr, 0, vnx.bytes, stream));
     MP_HIP(hipMemsetAsync(vny.ptr, 0, vny.bytes, stream));
     MP_CALL(sync_md());
-    set_tile(64);
+    set_tile(default_rows());
     return MP_OK;
+}
+
+// Rows per work item when the caller sets none: the largest of 64, 32, 16
+// that still gives every wave slot of the chip two items, else 8.  A small
+// mesh then runs as many short items (its stage latency is one item's rows),
+// a large one as fewer, longer items (less halo and epilogue work).
+int mp_transport::default_rows() const
+{
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const long long slots = (long long)cus * 4 * (precision == MP_PREC_MIXED ? 3 : 2);
+    const long long strips = (mesh.nx + kStripCells - 1) / kStripCells;
+    for (int r : {64, 32, 16})
+        if (strips * ((mesh.ny + r - 1) / r) >= 2 * slots) return r;
+    return 8;
 }
 
 mp_transport::~mp_transport()
@@ -453,15 +469,6 @@ mp_status mp_transport::reconstruct(int field, double* sigma, double* omega, dou
 }
 
 // ===================================================================== C ABI
-
-#define MP_GUARD(...)                                                      \
-    try {                                                                  \
-        __VA_ARGS__                                                        \
-    } catch (const std::bad_alloc&) {                                      \
-        return fail(MP_ERR_MEMORY, "host allocation failed");              \
-    } catch (const std::exception& e) {                                    \
-        return fail(MP_ERR_DEVICE, "%s", e.what());                        \
-    }
 
 extern "C" {
 

@@ -41,6 +41,7 @@ struct mp_transport {
                    int32_t prec, void* stream);
     mp_status sync_md();
     void set_tile(int rows_per_item);
+    int default_rows() const;
     mp_status ensure_scratch(int nsteps);
     mp_status upload(const double* h, const double* c);
     mp_status download(double* h, double* c);
