@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "mp_engine.h"
+#include "mp_launch.h"
 
 using namespace mp;
 
@@ -263,8 +264,10 @@ mp_status mp_batch::autotune(int nsteps, int scheme, const char* cache, int* row
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) == hipSuccess)
             (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        const int row_set[] = {8, 16, 32, 64, 128};
-        const int grid_mult[] = {0, 1, 2, 3, 4, 6};
+        // rows per item x grid: one wave per item (0) or a persistent grid of
+        // 1..3 resident blocks per CU taking items from a queue
+        const int row_set[] = {8, 12, 16, 24, 32, 64};
+        const int grid_mult[] = {0, 1, 2, 3};
         MP_CALL(prepare(nsteps, scheme));   // warm the code objects and tables
         MP_CALL(eng.enqueue(1, scheme, nullptr));
         MP_HIP(hipStreamSynchronize(eng.stream));
@@ -316,6 +319,13 @@ mp_status mp_batch_create(int32_t capacity, const mp_weno_options* weno, const m
         const size_t budget = hbm_budget_bytes ? size_t(hbm_budget_bytes) : ~size_t(0) / 2;
         mp_status s = b->eng.init(capacity, precision, w, p, stream, budget);
         if (s) { delete b; return s; }
+        // Defaults before any tuning: short items on a persistent grid that
+        // fills every resident wave slot.
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess)
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        b->rows = precision == MP_PREC_MIXED ? 8 : 16;
+        b->eng.blocks = std::max(1, cus) * std::max(1, batch_blocks_per_cu(precision));
         *out = b;
         return MP_OK;
     })

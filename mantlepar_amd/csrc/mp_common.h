@@ -28,6 +28,8 @@ enum : uint32_t {
 // Binary eutectic parameters as the device uses them.  Tep/Tmp are derived
 // on the host in long double exactly as EutecticModel::boundaries does
 // (src/phase/eutectic_rescaled.cpp:163-170) because P is uniform per mesh.
+// Reciprocals are double-doubles (hi + lo) formed on the host, so the device
+// multiplies where the reference divides, at double-double accuracy.
 struct PhaseDev {
     double LD;        // L/h0
     double Xe;        // eutectic composition
@@ -35,6 +37,22 @@ struct PhaseDev {
     double Tmp;       // Tep + 1, rounded once
     int32_t hyper;    // evaluate C > Xe on the mirrored diagram
     int32_t pad;
+    double ixh, ixl;  // 1/Xe
+    double imh, iml;  // 1/(1 - Xe), of the double 1 - Xe the mirror uses
+    double ilh, ill;  // 1/LD
+};
+
+// Per-face flux moments of the mixed path, built once per velocity refresh.
+// With W_q the Gauss weights times half the face length, alpha_q the LF speed
+// and s_q the tangential point offsets:
+//   vp = sum_q W_q (v_q + alpha_q)/2,  vm = sum_q W_q (v_q - alpha_q)/2      (FP64)
+//   p_m = sum_q W_q (v_q + alpha_q)/2 s_q^m,  m_m likewise with -alpha  (FP32)
+// so an interior face flux is tL*vp + tR*vm + sum_m (cL_m p_m + cR_m m_m),
+// c being the face-line coefficients of the centred reconstructions.  One
+// 32-byte record per face: two 16-byte loads per face in the sweep.
+struct __attribute__((aligned(16))) FaceMom {
+    double vp, vm;
+    float p1, p2, m1, m2;
 };
 
 // One uniform rectangular mesh.  All reconstruction tables are the closed
@@ -70,15 +88,9 @@ struct MeshDev {
     // face velocities [q][ny][nx+1] and [q][ny+1][nx]
     const double* vnx;
     const double* vny;
-    // Per-face flux moments (mixed path), built once per velocity refresh:
-    //   V+ = sum_q W_q (v_q + alpha_q)/2, V- = sum_q W_q (v_q - alpha_q)/2   (FP64)
-    //   M+_m = sum_q W_q (v_q + alpha_q)/2 s_q^m, M-_m likewise, m = 1, 2   (FP32)
-    // so an interior face flux is tL*V+ + tR*V- + sum_m (cL_m M+_m + cR_m M-_m)
-    // with c the face-line coefficients of the centred reconstructions.
-    const double* fxv;          // [2][ny][nx+1]: V+, V-
-    const float* fxm;           // [4][ny][nx+1]: M+_1, M+_2, M-_1, M-_2
-    const double* fyv;          // [2][ny+1][nx]
-    const float* fym;           // [4][ny+1][nx]
+    // Per-face flux moments (mixed path): [ny][nx+1] and [ny+1][nx].
+    const FaceMom* fx;
+    const FaceMom* fy;
 };
 
 // Per launch, per mesh: the stage u_out = a*u_n + b*(u_k + dt*L(u_k)).

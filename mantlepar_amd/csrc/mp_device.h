@@ -1,0 +1,79 @@
+// Device helpers shared by the stage sweeps (FP64 and mixed) and the
+// diagnostics kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mp_common.h"
+#include "mp_recon.h"
+
+namespace mp {
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Compensated accumulation s + e += x (Neumaier); contraction would break it.
+__device__ __forceinline__ void two_sum_acc(double& s, double& e, double x)
+{
+#pragma clang fp contract(off)
+    const double y = s + x;
+    e += fabs(s) >= fabs(x) ? (s - y) + x : (x - y) + s;
+    s = y;
+}
+
+// Outward boundary flux of one face (src/mlweno/advectiveflux.cpp:229-268 with
+// the couple policy src/couple/initialization.cpp:469-489).  sign turns the
+// canonical (+x/+y) normal velocity into the outward one.
+template <typename R>
+__device__ __forceinline__ double boundary_flux(const MeshDev& m, const double* vn, size_t vstride,
+                                                double sign, double t, const R (&c)[3],
+                                                const double* s, const double* W, uint8_t policy,
+                                                const double* inflow, uint32_t& err)
+{
+    double F = 0.0;
+    for (int q = 0; q < m.q; ++q) {
+        const double v = sign * vn[q * vstride];
+        double f = 0.0;
+        if (policy != 0 && v != 0.0) {
+            if (v > 0.0) {
+                f = v * (t + double(line_at<R>(c, R(s[q]))));
+            } else if (policy == 2) {
+                if (inflow) f = v * inflow[q];
+                else err |= kErrInflow;
+            } else {
+                err |= kErrBackflow;
+            }
+        }
+        F += W[q] * f;
+    }
+    return F;
+}
+
+// Wave reduction of one work item's partial sums in a fixed butterfly order
+// (compensated for the state sums), stored by lane 0; error bits are or-ed
+// into the launch's error word.
+__device__ __forceinline__ void item_partials(const StagePtrs& sp, int item, double sH, double eH,
+                                              double sC, double eC, double bH, double bC,
+                                              uint32_t err)
+{
+    double v[6] = {sH, eH, sC, eC, bH, bC};
+    for (int off = 32; off >= 1; off >>= 1) {
+        double o[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) o[k] = __shfl_xor(v[k], off);
+        double s0 = v[0], e0 = v[1] + o[1];
+        two_sum_acc(s0, e0, o[0]);
+        double s1 = v[2], e1 = v[3] + o[3];
+        two_sum_acc(s1, e1, o[2]);
+        v[0] = s0; v[1] = e0; v[2] = s1; v[3] = e1;
+        v[4] += o[4];
+        v[5] += o[5];
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && sp.partials) {
+        double* p = sp.partials + size_t(item) * kPartials;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) p[k] = v[k];
+    }
+    if (err) atomicOr(sp.error, err);
+}
+
+}  // namespace mp

@@ -142,8 +142,7 @@ size_t Engine::slot_bytes(const mp_mesh_desc& d, int rows_min)
     b += Arena::round(10 * N * sizeof(double));
     b += Arena::round(size_t(d.q) * faces_x(d) * sizeof(double));
     b += Arena::round(size_t(d.q) * faces_y(d) * sizeof(double));
-    b += Arena::round(2 * faces_x(d) * sizeof(double)) + Arena::round(4 * faces_x(d) * sizeof(float));
-    b += Arena::round(2 * faces_y(d) * sizeof(double)) + Arena::round(4 * faces_y(d) * sizeof(float));
+    b += Arena::round(faces_x(d) * sizeof(FaceMom)) + Arena::round(faces_y(d) * sizeof(FaceMom));
     b += 2 * Arena::round(N * (4 * sizeof(double) + 1));
     b += Arena::round(size_t(3) * items_for(d, rows_min) * kPartials * sizeof(double));
     b += Arena::round(size_t(2) * (d.nx + d.ny));
@@ -172,10 +171,8 @@ mp_status Engine::add_mesh(const mp_mesh_desc& d, int* out)
     }
     sl.md.vnx = reinterpret_cast<double*>(take(size_t(d.q) * faces_x(d) * sizeof(double)));
     sl.md.vny = reinterpret_cast<double*>(take(size_t(d.q) * faces_y(d) * sizeof(double)));
-    sl.md.fxv = reinterpret_cast<double*>(take(2 * faces_x(d) * sizeof(double)));
-    sl.md.fxm = reinterpret_cast<float*>(take(4 * faces_x(d) * sizeof(float)));
-    sl.md.fyv = reinterpret_cast<double*>(take(2 * faces_y(d) * sizeof(double)));
-    sl.md.fym = reinterpret_cast<float*>(take(4 * faces_y(d) * sizeof(float)));
+    sl.md.fx = reinterpret_cast<FaceMom*>(take(faces_x(d) * sizeof(FaceMom)));
+    sl.md.fy = reinterpret_cast<FaceMom*>(take(faces_y(d) * sizeof(FaceMom)));
     sl.phase[0] = reinterpret_cast<double*>(take(N * (4 * sizeof(double) + 1)));
     sl.phase[1] = reinterpret_cast<double*>(take(N * (4 * sizeof(double) + 1)));
     sl.partials = reinterpret_cast<double*>(take(size_t(3) * items_for(d, 8) * kPartials * sizeof(double)));
@@ -266,8 +263,8 @@ mp_status Engine::set_velocity(int k, const double* vx, const double* vy, bool w
     MP_HIP(hipMemcpyAsync(const_cast<double*>(s.md.vnx), vx, nxv * sizeof(double), hipMemcpyHostToDevice, stream));
     MP_HIP(hipMemcpyAsync(const_cast<double*>(s.md.vny), vy, nyv * sizeof(double), hipMemcpyHostToDevice, stream));
     MeshDev* dm = d_meshes.as<MeshDev>() + k;
-    MP_HIP(launch_face_prep(dm, d.nx, d.ny, const_cast<double*>(s.md.fxv), const_cast<float*>(s.md.fxm),
-                            const_cast<double*>(s.md.fyv), const_cast<float*>(s.md.fym), stream));
+    MP_HIP(launch_face_prep(dm, d.nx, d.ny, const_cast<FaceMom*>(s.md.fx), const_cast<FaceMom*>(s.md.fy),
+                            stream));
     unsigned long long* bits = d_maxbits.as<unsigned long long>() + k;
     MP_CALL(launch_clear(reinterpret_cast<uint32_t*>(bits), 2, stream) == hipSuccess
                 ? MP_OK : fail(MP_ERR_DEVICE, "clear failed"));
@@ -462,6 +459,7 @@ mp_status Engine::reserve_steps(int nsteps)
 {
     if (nsteps <= steps_capacity) return MP_OK;
     MP_CALL(d_totals.alloc(sizeof(double) * 4 * size_t(capacity) * nsteps));
+    MP_CALL(d_queues.alloc(sizeof(int) * 3 * size_t(nsteps)));
     steps_capacity = nsteps;
     ++generation;
     return MP_OK;
@@ -471,6 +469,7 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
 {
     if (nsteps > steps_capacity) return fail(MP_ERR_STATE, "per-step totals not reserved");
     MP_HIP(launch_clear(d_errors.as<uint32_t>(), capacity, stream));
+    if (blocks > 0) MP_HIP(launch_clear(d_queues.as<uint32_t>(), 3 * nsteps, stream));
     const StagePtrs* base = d_stages.as<StagePtrs>();
     for (int s = 0; s < nsteps; ++s) {
         const int set = s == 0 ? 0 : (s % 2 ? 1 : 2);
@@ -489,7 +488,8 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
             MP_HIP(launch_batch_stage(precision, st == scheme - 1, q_common, d_meshes.as<MeshDev>(),
                                       base + size_t(set * 3 + st) * capacity,
-                                      d_items.as<int2>() + L.first, L.second, blocks, stream));
+                                      d_items.as<int2>() + L.first, L.second, blocks,
+                                      d_queues.as<int>() + 3 * s + st, stream));
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
         }
         MP_HIP(launch_batch_reduce(base + size_t(set * 3) * capacity, d_nitems.as<int>(),

@@ -122,11 +122,12 @@ public:
     std::vector<int> slot_steps;     // per slot: steps in the current pattern (0: idle)
     std::map<int, std::pair<int, int>> lists;   // prefix length -> (item offset, count)
     int capacity = 0;
-    int blocks = 0;            // persistent grid (0: one block per 4 items)
+    int blocks = 0;            // persistent grid in blocks of 4 waves (0: one wave per item)
     int nitems_total = 0;
     int steps_capacity = 0;    // per-step totals reserved
     int q_common = 0;          // q shared by all active slots, or 0
     DeviceBuffer d_meshes, d_stages, d_items, d_nitems, d_active, d_errors, d_totals, d_maxbits;
+    DeviceBuffer d_queues;     // per step and stage: work counter of the persistent grid
     StagePtrs* h_stages = nullptr;   // pinned mirror of d_stages [9][capacity]
     std::vector<int> h_nitems;
 };

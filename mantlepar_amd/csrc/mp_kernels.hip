@@ -18,6 +18,7 @@
 #include <type_traits>
 
 #include "mp_common.h"
+#include "mp_device.h"
 #include "mp_launch.h"
 #include "mp_phase.h"
 #include "mp_recon.h"
@@ -26,36 +27,6 @@ namespace mp {
 
 template <int MODE>
 using RealT = typename std::conditional<MODE == 0, double, float>::type;
-
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-// Outward boundary flux of one face (src/mlweno/advectiveflux.cpp:229-268 with
-// the couple policy src/couple/initialization.cpp:469-489).  sign turns the
-// canonical (+x/+y) normal velocity into the outward one.
-template <typename R>
-__device__ __forceinline__ double boundary_flux(const MeshDev& m, const double* vn, size_t vstride,
-                                                double sign, double t, const R (&c)[3],
-                                                const double* s, const double* W, uint8_t policy,
-                                                const double* inflow, uint32_t& err)
-{
-    double F = 0.0;
-    for (int q = 0; q < m.q; ++q) {
-        const double v = sign * vn[q * vstride];
-        double f = 0.0;
-        if (policy != 0 && v != 0.0) {
-            if (v > 0.0) {
-                f = v * (t + double(line_at<R>(c, R(s[q]))));
-            } else if (policy == 2) {
-                if (inflow) f = v * inflow[q];
-                else err |= kErrInflow;
-            } else {
-                err |= kErrBackflow;
-            }
-        }
-        F += W[q] * f;
-    }
-    return F;
-}
 
 // Interior face flux (left state behind the +x/+y normal) for one field, per
 // point, in the reference's LLF form F = (F_L + F_R - alpha*(uR - uL))/2
@@ -121,8 +92,7 @@ __device__ __forceinline__ void moment_pair(double Vp, double Vm, float Mp1, flo
 }
 
 // Flux moments of every face (one thread per face), once per velocity refresh.
-__global__ void face_prep_kernel(const MeshDev* __restrict__ mesh, double* fxv, float* fxm,
-                                 double* fyv, float* fym)
+__global__ void face_prep_kernel(const MeshDev* __restrict__ mesh, FaceMom* fx, FaceMom* fy)
 {
     const MeshDev& m = *mesh;
     const int64_t nxf = int64_t(m.ny) * (m.nx + 1), nyf = int64_t(m.ny + 1) * m.nx;
@@ -146,22 +116,14 @@ __global__ void face_prep_kernel(const MeshDev* __restrict__ mesh, double* fxv, 
         m1 += wr * s[q];
         m2 += wr * s[q] * s[q];
     }
-    double* dv = vert ? fxv : fyv;
-    float* dm = vert ? fxm : fym;
-    dv[f] = vp;
-    dv[nf + f] = vm;
-    dm[f] = float(p1);
-    dm[nf + f] = float(p2);
-    dm[2 * nf + f] = float(m1);
-    dm[3 * nf + f] = float(m2);
-}
-
-__device__ __forceinline__ void two_sum_acc(double& s, double& e, double x)
-{
-#pragma clang fp contract(off)
-    const double y = s + x;
-    e += fabs(s) >= fabs(x) ? (s - y) + x : (x - y) + s;
-    s = y;
+    FaceMom o;
+    o.vp = vp;
+    o.vm = vm;
+    o.p1 = float(p1);
+    o.p2 = float(p2);
+    o.m1 = float(m1);
+    o.m2 = float(m2);
+    (vert ? fx : fy)[f] = o;
 }
 
 // Flux of both fields through one interior face.
@@ -179,11 +141,9 @@ __device__ __forceinline__ void interior_pair(const MeshDev& m, bool vertical, s
         llf_pair<NQ>(m, vn, vs, vertical ? m.sv : m.sh, vertical ? m.wv : m.wh, tLH, cLH, tRH, cRH,
                      tLC, cLC, tRC, cRC, FH, FC);
     } else {
-        const size_t nf = vertical ? size_t(m.ny) * (m.nx + 1) : size_t(m.ny + 1) * m.nx;
-        const double* V = vertical ? m.fxv : m.fyv;
-        const float* Mm = vertical ? m.fxm : m.fym;
-        moment_pair(V[face], V[nf + face], Mm[face], Mm[nf + face], Mm[2 * nf + face],
-                    Mm[3 * nf + face], tLH, cLH, tRH, cRH, tLC, cLC, tRC, cRC, FH, FC);
+        const FaceMom f = (vertical ? m.fx : m.fy)[face];
+        moment_pair(f.vp, f.vm, f.p1, f.p2, f.m1, f.m2, tLH, cLH, tRH, cRH, tLC, cLC, tRC, cRC,
+                    FH, FC);
     }
 }
 
@@ -350,29 +310,11 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
         }
     }
 
-    // Wave reduction of the item's partial sums (fixed butterfly order).
-    double v[6] = {sH, eH, sC, eC, bH, bC};
-    for (int off = 32; off >= 1; off >>= 1) {
-        double o[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) o[k] = __shfl_xor(v[k], off);
-        double s0 = v[0], e0 = v[1] + o[1];
-        two_sum_acc(s0, e0, o[0]);
-        double s1 = v[2], e1 = v[3] + o[3];
-        two_sum_acc(s1, e1, o[2]);
-        v[0] = s0; v[1] = e0; v[2] = s1; v[3] = e1;
-        v[4] += o[4];
-        v[5] += o[5];
-    }
-    if (lane == 0 && sp.partials) {
-        double* p = sp.partials + size_t(item) * kPartials;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) p[k] = v[k];
-    }
-    if (err) atomicOr(sp.error, err);
+    item_partials(sp, item, sH, eH, sC, eC, bH, bC, err);
 }
 
-// Register budget: two waves per SIMD for FP64 (256 registers), three for mixed.
+// Register budget: two waves per SIMD (256 registers).  The mixed path has its
+// own sweep (mp_sweep32.hip).
 template <int MODE, bool PHASE, int NQ>
 __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void stage_sweep_kernel(const MeshDev* __restrict__ mesh,
                                                           StagePtrs sp, int nitems)
@@ -446,16 +388,25 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_step_kernel(
 // a small mesh never occupies a launch of its own.  Every mesh keeps its own
 // MeshDev record, stage pointers, partials and error word; its results do not
 // depend on which other meshes share the launch.
+//
+// Persistent form (queue != nullptr): every wave takes the next work item from
+// a counter until the list is exhausted; otherwise one wave per item.
 template <int MODE, bool PHASE, int NQ>
 __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
     const MeshDev* __restrict__ meshes, const StagePtrs* __restrict__ stages,
-    const int2* __restrict__ items, int nitems)
+    const int2* __restrict__ items, int nitems, int* queue)
 {
     // The work index is wave-uniform; saying so keeps every per-mesh table
     // load scalar (SGPRs) instead of replicating it across lanes.
-    const int nwaves = gridDim.x * 4;
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    for (int it = blockIdx.x * 4 + wave; it < nitems; it += nwaves) {
+    int it = blockIdx.x * 4 + wave;
+    for (;;) {
+        if (queue) {
+            int t = 0;
+            if ((threadIdx.x & (kWave - 1)) == 0) t = atomicAdd(queue, 1);
+            it = __builtin_amdgcn_readfirstlane(t);
+        }
+        if (it >= nitems) break;
         const int2 w = items[it];
         const int slot = __builtin_amdgcn_readfirstlane(w.x);
         const int local = __builtin_amdgcn_readfirstlane(w.y);
@@ -463,9 +414,9 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
         const StagePtrs& sp = stages[slot];
         const int strip = local % sp.strips, band = local / sp.strips;
         const int j0 = band * sp.rows;
-        if (j0 >= m.ny) continue;      // never taken with a current work list
-        const int j1 = min(j0 + sp.rows, m.ny);
-        sweep_item<MODE, PHASE, NQ>(m, sp, local, strip, j0, j1);
+        if (j0 < m.ny)                 // always true with a current work list
+            sweep_item<MODE, PHASE, NQ>(m, sp, local, strip, j0, min(j0 + sp.rows, m.ny));
+        if (!queue) break;
     }
 }
 
@@ -718,18 +669,17 @@ hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, cons
         if (phase) stage_sweep_kernel<0, true, 0><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
         else stage_sweep_kernel<0, false, 0><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
     } else {
-        if (phase) stage_sweep_kernel<1, true, 0><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-        else stage_sweep_kernel<1, false, 0><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        return launch_stage32(phase, d_mesh, sp, nitems, stream);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_face_prep(const MeshDev* d_mesh, int nx, int ny, double* fxv, float* fxm,
-                            double* fyv, float* fym, hipStream_t stream)
+hipError_t launch_face_prep(const MeshDev* d_mesh, int nx, int ny, FaceMom* fx, FaceMom* fy,
+                            hipStream_t stream)
 {
     const int64_t n = int64_t(ny) * (nx + 1) + int64_t(ny + 1) * nx;
     const dim3 grid(unsigned((n + 255) / 256)), block(256);
-    face_prep_kernel<<<grid, block, 0, stream>>>(d_mesh, fxv, fxm, fyv, fym);
+    face_prep_kernel<<<grid, block, 0, stream>>>(d_mesh, fx, fy);
     return hipGetLastError();
 }
 
@@ -791,21 +741,30 @@ hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, c
 
 hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes,
                               const StagePtrs* stages, const int2* items, int nitems, int blocks,
-                              hipStream_t stream)
+                              int* queue, hipStream_t stream)
 {
     if (nitems <= 0) return hipSuccess;
+    int* qp = blocks > 0 ? queue : nullptr;
     const dim3 grid(blocks > 0 ? blocks : (nitems + 3) / 4), block(256);
     if (mode == 0 && q == 5) {
-        if (phase) batch_sweep_kernel<0, true, 5><<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
-        else batch_sweep_kernel<0, false, 5><<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
+        if (phase) batch_sweep_kernel<0, true, 5><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp);
+        else batch_sweep_kernel<0, false, 5><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp);
     } else if (mode == 0) {
-        if (phase) batch_sweep_kernel<0, true, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
-        else batch_sweep_kernel<0, false, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
+        if (phase) batch_sweep_kernel<0, true, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp);
+        else batch_sweep_kernel<0, false, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp);
     } else {
-        if (phase) batch_sweep_kernel<1, true, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
-        else batch_sweep_kernel<1, false, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
+        return launch_batch_stage32(phase, meshes, stages, items, nitems, blocks, queue, stream);
     }
     return hipGetLastError();
+}
+
+int batch_blocks_per_cu(int mode)
+{
+    if (mode != 0) return batch_blocks_per_cu32();
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_sweep_kernel<0, false, 0>, 256, 0) != hipSuccess)
+        return 0;
+    return n;
 }
 
 hipError_t launch_batch_reduce(const StagePtrs* stages, const int* nitems, const MeshDev* meshes,

@@ -9,8 +9,8 @@ namespace mp {
 
 hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
                         int nitems, hipStream_t stream);
-hipError_t launch_face_prep(const MeshDev* d_mesh, int nx, int ny, double* fxv, float* fxm,
-                            double* fyv, float* fym, hipStream_t stream);
+hipError_t launch_face_prep(const MeshDev* d_mesh, int nx, int ny, FaceMom* fx, FaceMom* fy,
+                            hipStream_t stream);
 hipError_t launch_recon_diag(int mode, const MeshDev* d_mesh, int ncells, const double* u,
                              double* sigma, double* omega, double* traces, uint32_t* error,
                              hipStream_t stream);
@@ -27,13 +27,24 @@ hipError_t launch_phase(const PhaseDev& ph, double TDe0, double nu, double dT, i
 hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H, const double* C,
                               double* T, double* phi, double* Cl, double* Cs, uint8_t* region,
                               uint32_t* error, hipStream_t stream);
+// blocks > 0: persistent grid of that many blocks taking items from *queue
+// (an int that must be zero at launch); blocks == 0: one wave per item.
 hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes,
                               const StagePtrs* stages, const int2* items, int nitems, int blocks,
-                              hipStream_t stream);
+                              int* queue, hipStream_t stream);
 hipError_t launch_batch_reduce(const StagePtrs* stages, const int* nitems, const MeshDev* meshes,
                                const int* slots, int nslots, int nstages, const double* w,
                                double* out, hipStream_t stream);
 hipError_t launch_clear(uint32_t* p, int n, hipStream_t stream);
+// Resident blocks per CU of the batched stage kernel (0 if the query fails).
+int batch_blocks_per_cu(int mode);
+// Mixed-precision sweeps (mp_sweep32.hip).
+int batch_blocks_per_cu32();
+hipError_t launch_stage32(bool phase, const MeshDev* d_mesh, const StagePtrs& sp, int nitems,
+                          hipStream_t stream);
+hipError_t launch_batch_stage32(bool phase, const MeshDev* meshes, const StagePtrs* stages,
+                                const int2* items, int nitems, int blocks, int* queue,
+                                hipStream_t stream);
 hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, const double* w,
                               double area, double* out, hipStream_t stream);
 

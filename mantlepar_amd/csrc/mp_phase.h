@@ -3,7 +3,9 @@
 //
 // The reference evaluates in 80-bit long double and rounds each public value
 // to double.  The device has no long double, so the phase boundaries, whose
-// rounding decides the region, are formed in double-double and rounded once;
+// rounding decides the region, are formed in double-double (divisions by Xe,
+// 1 - Xe and LD become products with host-formed double-double reciprocals)
+// and rounded once;
 // they then agree with the reference's long double -> double results except
 // where the exact boundary lies within a few ulps of a rounding tie.  The
 // eutectic-branch fractions (tiny differences of O(1) terms near the end of
@@ -78,6 +80,14 @@ __device__ __forceinline__ DD dd_mul_d(DD a, double b)
     return dd_two_sum(p, e);
 }
 
+__device__ __forceinline__ DD dd_mul(DD a, DD b)
+{
+#pragma clang fp contract(off)
+    const double p = a.hi * b.hi;
+    const double e = fma(a.hi, b.hi, -p) + (a.hi * b.lo + a.lo * b.hi);
+    return dd_two_sum(p, e);
+}
+
 // unitFraction (src/phase/eutectic_rescaled.cpp:108-114)
 __device__ __forceinline__ double unit_fraction(double v, uint32_t& err)
 {
@@ -86,15 +96,16 @@ __device__ __forceinline__ double unit_fraction(double v, uint32_t& err)
     return fmin(fmax(v, 0.0), 1.0);
 }
 
-// One evaluation on the side 0 <= C <= Xe of a diagram with eutectic Xe.
-__device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, double H, double C,
-                                               uint32_t& err)
+// One evaluation on the side 0 <= C <= Xe of a diagram with eutectic Xe;
+// (ixh, ixl) is 1/Xe as a double-double.
+__device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, double ixh, double ixl,
+                                               double H, double C, uint32_t& err)
 {
     PhaseOut s;
     s.T = 0; s.phil = 0; s.cl = 0; s.cs = 0; s.phi1 = 0; s.phi2 = 0; s.dTdC = 0; s.dTdH = 0;
     const double LD = p.LD, Tep = p.Tep, Tmp = p.Tmp;
-    // boundaries (:163-185)
-    const DD ratio = dd_div(C, Xe);
+    // boundaries (:163-185); C/Xe as C times the double-double 1/Xe
+    const DD ratio = dd_mul_d({ixh, ixl}, C);
     const double Heut = dd_add({Tep, 0.0}, dd_mul_d(ratio, LD)).hi;
     double Hliq;
     if (C == 0.0) {
@@ -117,7 +128,7 @@ __device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, dou
         s.phi1 = 1.0; s.T = H; s.dTdH = 1.0;
         break;
     case 6:
-        s.phil = unit_fraction((H - Tmp) / LD, err);
+        s.phil = unit_fraction((H - Tmp) * p.ilh, err);
         s.phi1 = 1.0 - s.phil;
         s.T = Tmp;
         break;
@@ -130,7 +141,7 @@ __device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, dou
         const DD maxl = ratio;
         DD liquid = maxl;
         if (H != Heut) {
-            const DD e = dd_div_d(dd_two_sum(H, -Tep), LD);
+            const DD e = dd_mul(dd_two_sum(H, -Tep), {p.ilh, p.ill});
             if (dd_less(e, maxl)) liquid = e;
         }
         DD solid2 = dd_add({C, 0.0}, dd_mul_d(liquid, -Xe));
@@ -144,15 +155,17 @@ __device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, dou
         break;
     }
     case 4: {
-        // LD*phi^2 - (H-Tmp)*phi - C/Xe = 0 with the cancellation-free root (:237-257)
+        // LD*phi^2 - (H-Tmp)*phi - C/Xe = 0 with the cancellation-free root
+        // (:237-257); root = hypot(a, 2 sqrt(LD r)) with one square root.
         const double a = H - Tmp;
-        const double r = C / Xe;
-        const double root = hypot(a, 2.0 * sqrt(LD) * sqrt(r));
-        const double liquid = a >= 0.0 ? (a + root) / (2.0 * LD) : 2.0 * r / (root - a);
+        const double r = ratio.hi;
+        const double root = sqrt(fma(a, a, 4.0 * LD * r));
+        const double liquid = a >= 0.0 ? (a + root) * (0.5 * p.ilh) : 2.0 * r / (root - a);
         s.phil = unit_fraction(liquid, err);
         s.phi1 = unit_fraction(1.0 - liquid, err);
-        s.cl = unit_fraction(C / liquid, err);
-        s.T = a >= 0.0 ? Tmp - r / liquid : H - LD * liquid;
+        const double cl = C / liquid;
+        s.cl = unit_fraction(cl, err);
+        s.T = a >= 0.0 ? Tmp - cl * ixh : H - LD * liquid;    // r/liquid = cl/Xe
         s.dTdH = unit_fraction(a >= 0.0 ? 2.0 * (LD / root) * (r / (root + a))
                                         : 0.5 * (1.0 - a / root), err);
         s.dTdC = -((LD / root) / Xe);
@@ -167,7 +180,7 @@ __device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, dou
         s.dTdC = gap == 0.0 ? -__builtin_inf() : -((LD / gap) / Xe);
     }
     const double solid = s.phi1 + s.phi2;
-    s.cs = solid > 0.0 ? s.phi2 / solid : 0.0;
+    s.cs = s.phi2 > 0.0 && solid > 0.0 ? s.phi2 / solid : 0.0;
     if (!(s.phil > 0.0)) s.cl = 0.0;
     return s;
 }
@@ -188,14 +201,14 @@ __device__ __forceinline__ PhaseOut phase_eval(const PhaseDev& p, double H, doub
         err |= kErrComposition;
         C = fmin(fmax(C, 0.0), cmax);
     }
-    if (!(C > Xe)) return phase_core(p, Xe, H, C, err);
-    PhaseOut s = phase_core(p, 1.0 - Xe, H, 1.0 - C, err);
+    if (!(C > Xe)) return phase_core(p, Xe, p.ixh, p.ixl, H, C, err);
+    PhaseOut s = phase_core(p, 1.0 - Xe, p.imh, p.iml, H, 1.0 - C, err);
     PhaseOut r = s;
     r.phi1 = s.phi2;
     r.phi2 = s.phi1;
     r.cl = s.phil > 0.0 ? 1.0 - s.cl : 0.0;
     const double solid = r.phi1 + r.phi2;
-    r.cs = solid > 0.0 ? r.phi2 / solid : 0.0;
+    r.cs = r.phi2 > 0.0 && solid > 0.0 ? r.phi2 / solid : 0.0;
     r.dTdC = -s.dTdC;
     return r;
 }

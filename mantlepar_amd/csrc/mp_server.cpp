@@ -317,6 +317,9 @@ mp_status mp_server_create(int32_t max_slots, const mp_weno_options* weno, const
         if (st) { delete s; return st; }
         s->scheme = scheme;
         s->max_epoch = max_epoch_steps;
+        // One wave per work item: with few resident runs this measured faster
+        // than a persistent grid.
+        s->eng.blocks = 0;
         s->worker = std::thread([s] { s->loop(); });
         *out = s;
         return MP_OK;

@@ -1,0 +1,463 @@
+// Mixed-precision stage sweep (FP32 reconstruction and flux corrections, FP64
+// state, first-order flux part, update, phase and sums), written for gfx950.
+//
+// Layout as the FP64 sweep: one wavefront owns a strip of 62 columns plus a
+// halo lane on each side and walks up a band of rows.  What differs is how
+// little work each cell repeats:
+//  * every lane loads only its own column of each new row; the left and right
+//    neighbours arrive through DPP wavefront shifts (full-rate VALU moves, no
+//    LDS), and only the two halo lanes load one extra column;
+//  * the FP64 differences a reconstruction needs are formed once per row, not
+//    once per cell: each row keeps u, the neighbour differences dl, dr and its
+//    x-direction quadratic (a1, a2), and consecutive rows share the vertical
+//    difference e.  The three cells that use a row (rows r-1, r, r+1 around
+//    it) read these instead of recomputing eight differences each;
+//  * face flux moments arrive as one 32-byte record per face.
+//
+// The arithmetic is the production ML-WENO of the reference
+// (src/mlweno/reconstruction.cpp:212-265, 489-496) in the closed forms of
+// mp_recon.h, regrouped around the per-row data: the small candidates' mixed
+// term c_xy - c_x - c_y is the difference of two rows' neighbour
+// differences, and their x and y slopes take two values each.
+#include <hip/hip_runtime.h>
+
+#include "mp_common.h"
+#include "mp_device.h"
+#include "mp_launch.h"
+#include "mp_phase.h"
+#include "mp_recon.h"
+
+namespace mp {
+namespace {
+
+// Wavefront shifts (DPP wave_shr:1 and wave_shl:1).  Lane L receives lane
+// L-1 (shr) or L+1 (shl); the end lane keeps its own value.  All lanes must
+// be active, so callers use them only in wave-uniform control flow.
+__device__ __forceinline__ int shr1_b32(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int shl1_b32(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+
+__device__ __forceinline__ double shr1(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = unsigned(shr1_b32(int(unsigned(b)))), hi = unsigned(shr1_b32(int(unsigned(b >> 32))));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+__device__ __forceinline__ double shl1(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = unsigned(shl1_b32(int(unsigned(b)))), hi = unsigned(shl1_b32(int(unsigned(b >> 32))));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+__device__ __forceinline__ float shl1(float v) { return __builtin_bit_cast(float, shl1_b32(__builtin_bit_cast(int, v))); }
+
+// Per-mesh constants in FP32, converted once per work item.
+struct K32 {
+    float kx0, kx1, kx2, ky0, ky1, ky2;
+    float iaw, ibw, iawbw, q11w, hx, hy;
+    float epsA, pL, pS, ldL, ldS[4], laC;
+    bool use_constant;
+};
+
+__device__ __forceinline__ K32 make_k32(const MeshDev& m)
+{
+    K32 k;
+    k.kx0 = float(m.kx0); k.kx1 = float(m.kx1); k.kx2 = float(m.kx2);
+    k.ky0 = float(m.ky0); k.ky1 = float(m.ky1); k.ky2 = float(m.ky2);
+    k.iaw = float(m.inv_aw); k.ibw = float(m.inv_bw); k.iawbw = float(m.inv_awbw);
+    k.q11w = float(m.q11w); k.hx = float(m.hx); k.hy = float(m.hy);
+    const float sc = wscale<float>();
+    k.epsA = float(m.epsA);
+    k.pL = float(m.pL);
+    k.pS = float(m.pS);
+    k.ldL = float(m.ldL) * sc;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) k.ldS[s] = float(m.ldS[s]) * sc;
+    k.use_constant = m.use_constant != 0;
+    k.laC = k.use_constant ? float(m.ldC) * sc - float(m.pC) * wlog<float>(k.epsA) : -INFINITY;
+    return k;
+}
+
+// One row of one field at this lane's column.
+struct Row32 {
+    double u;        // cell average
+    float dl, dr;    // left / right neighbour minus u (FP64 difference, rounded once)
+    float a1, a2;    // x quadratic through the three averages: slope and curvature terms
+    float e;         // u minus the average of the row below
+};
+
+__device__ __forceinline__ Row32 make_row(double u, double outer, double below, bool first, bool last,
+                                          const K32& k)
+{
+    const double l = shr1(u), r = shl1(u);
+    Row32 R;
+    R.u = u;
+    R.e = float(u - below);
+    R.dl = float((first ? outer : l) - u);
+    R.dr = float((last ? outer : r) - u);
+    R.a1 = (R.dr - R.dl) * k.kx1;
+    R.a2 = (R.dr + R.dl) * k.kx2;
+    return R;
+}
+
+// Reconstruction of the cell at row r from the rows below (R0), at (R1) and
+// above (R2) it.
+__device__ __forceinline__ void recon32(const K32& k, const Row32& R0, const Row32& R1,
+                                        const Row32& R2, unsigned mask, FaceLines<float>& out,
+                                        uint32_t& err)
+{
+    const float elo = R1.e, ehi = R2.e;   // u(r) - u(r-1), u(r+1) - u(r)
+    // Large candidate: the rows' x quadratics, centred on the target, then y.
+    const float g00 = -elo - R0.a2 * k.kx0, g10 = -R1.a2 * k.kx0, g20 = ehi - R2.a2 * k.kx0;
+    float P[3][3];   // P[b][a]: coefficient of x^a y^b
+    {
+        const float l[3] = {g00, R0.a1, R0.a2}, mm[3] = {g10, R1.a1, R1.a2}, r[3] = {g20, R2.a1, R2.a2};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float b2 = (r[a] + l[a] - 2.0f * mm[a]) * k.ky2;
+            P[2][a] = b2;
+            P[1][a] = (r[a] - l[a]) * k.ky1;
+            P[0][a] = mm[a] - b2 * k.ky0;
+        }
+    }
+    float sig[5];
+    {
+        const float p10 = P[0][1], p01 = P[1][0], p20 = P[0][2], p02 = P[2][0], p11 = P[1][1];
+        const float p21 = P[1][2], p12 = P[2][1], p22 = P[2][2];
+        const float e1 = p10 + p12 * (1.0f / 12.0f);
+        const float e2 = p01 + p21 * (1.0f / 12.0f);
+        const float x20 = 2.0f * (p20 + p22 * (1.0f / 12.0f));
+        const float y02 = 2.0f * (p02 + p22 * (1.0f / 12.0f));
+        sig[0] = e1 * e1 + e2 * e2 + (13.0f / 12.0f) * (x20 * x20 + y02 * y02) +
+                 (7.0f / 6.0f) * p11 * p11 + (7.0f / 10.0f) * (p12 * p12 + p21 * p21) +
+                 (43.0f / 270.0f) * p22 * p22;
+    }
+    // Small candidates, offsets (0,-1),(0,0),(-1,-1),(-1,0): (sx, sy) =
+    // (+,-), (+,+), (-,-), (-,+).  Slopes take two values per direction; the
+    // mixed term is a difference of neighbour differences of two rows.
+    const float q10p = R1.dr * k.iaw, q10m = -R1.dl * k.iaw;
+    const float q01p = ehi * k.ibw, q01m = elo * k.ibw;
+    float q11[4];
+    q11[0] = (R1.dr - R0.dr) * k.iawbw;
+    q11[1] = (R2.dr - R1.dr) * k.iawbw;
+    q11[2] = (R0.dl - R1.dl) * k.iawbw;
+    q11[3] = (R1.dl - R2.dl) * k.iawbw;
+    {
+        const float xp = q10p * q10p, xm = q10m * q10m, yp = q01p * q01p, ym = q01m * q01m;
+        sig[1] = xp + ym + q11[0] * q11[0] * k.q11w;
+        sig[2] = xp + yp + q11[1] * q11[1] * k.q11w;
+        sig[3] = xm + ym + q11[2] * q11[2] * k.q11w;
+        sig[4] = xm + yp + q11[3] * q11[3] * k.q11w;
+    }
+    // Normalised weights in log space, base 2 (src/mlweno/reconstruction.cpp:212-265).
+    float la[6];
+    la[0] = (mask & 1u) ? k.ldL - k.pL * wlog<float>(sig[0] + k.epsA) : -INFINITY;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        la[s + 1] = (mask & (2u << s)) ? k.ldS[s] - k.pS * wlog<float>(sig[s + 1] + k.epsA) : -INFINITY;
+    la[5] = k.laC;
+    float mx = la[0];
+#pragma unroll
+    for (int s = 1; s < 6; ++s) mx = la[s] > mx ? la[s] : mx;
+    if (!(mx > -INFINITY) || !isfinite(mx)) {
+        err |= kErrWeights;
+        mx = 0.0f;
+    }
+    float om[6], sum = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        om[s] = la[s] > -INFINITY ? wexp<float>(la[s] - mx) : 0.0f;
+        sum += om[s];
+    }
+    const float inv = 1.0f / sum;
+#pragma unroll
+    for (int s = 0; s < 5; ++s) om[s] *= inv;
+    // Blend (Q1 candidates embed in the Q2 monomials; the constant candidate's
+    // centred polynomial is zero).
+    float B[3][3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) B[b][a] = om[0] * P[b][a];
+    B[0][1] += (om[1] + om[2]) * q10p + (om[3] + om[4]) * q10m;
+    B[1][0] += (om[2] + om[4]) * q01p + (om[1] + om[3]) * q01m;
+    B[1][1] += om[1] * q11[0] + om[2] * q11[1] + om[3] * q11[2] + om[4] * q11[3];
+    const float hx = k.hx, hy = k.hy;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        out.c[1][b] = B[b][0] + hx * (B[b][1] + hx * B[b][2]);   // right, x = +hx
+        out.c[3][b] = B[b][0] - hx * (B[b][1] - hx * B[b][2]);   // left,  x = -hx
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        out.c[2][a] = B[0][a] + hy * (B[1][a] + hy * B[2][a]);   // top,    y = +hy
+        out.c[0][a] = B[0][a] - hy * (B[1][a] - hy * B[2][a]);   // bottom, y = -hy
+    }
+}
+
+// Both fields through one interior face from its moment record: first-order
+// part in FP64, high-order correction in FP32.
+__device__ __forceinline__ void face_flux(const FaceMom& f, double tLH, const float (&cLH)[3],
+                                          double tRH, const float (&cRH)[3], double tLC,
+                                          const float (&cLC)[3], double tRC,
+                                          const float (&cRC)[3], double& FH, double& FC)
+{
+    const float vp = float(f.vp), vm = float(f.vm);
+    const float ch = cLH[0] * vp + cLH[1] * f.p1 + cLH[2] * f.p2 + cRH[0] * vm + cRH[1] * f.m1 + cRH[2] * f.m2;
+    const float cc = cLC[0] * vp + cLC[1] * f.p1 + cLC[2] * f.p2 + cRC[0] * vm + cRC[1] * f.m1 + cRC[2] * f.m2;
+    FH = fma(tLH, f.vp, tRH * f.vm) + double(ch);
+    FC = fma(tLC, f.vp, tRC * f.vm) + double(cc);
+}
+
+template <bool PHASE>
+__device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, int strip, int j0, int j1)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nx = m.nx, ny = m.ny;
+    const int i = strip * kStripCells - 1 + lane;
+    const bool col_ok = i >= 0 && i < nx;
+    const bool owned = col_ok && lane >= 1 && lane <= kStripCells;
+    const bool first = lane == 0, last = lane == kWave - 1;
+    const int ic = clampi(i, 0, nx - 1);
+    const int io = clampi(first ? i - 1 : i + 1, 0, nx - 1);   // halo lanes' outer column
+    const size_t vxs = size_t(ny) * (nx + 1);
+    const size_t vys = size_t(ny + 1) * nx;
+    const K32 k = make_k32(m);
+    uint32_t err = 0;
+
+    auto load = [&](const double* base, int row, double& u, double& o) {
+        const double* p = base + size_t(clampi(row, 0, ny - 1)) * nx;
+        u = p[ic];
+        o = (first || last) ? p[io] : 0.0;
+    };
+
+    int r = j0 > 0 ? j0 - 1 : 0;
+    Row32 H0, H1, H2, C0, C1, C2;
+    {
+        double u[3], o[3], v[3], w[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            load(sp.Hk, r - 1 + d, u[d], o[d]);
+            load(sp.Ck, r - 1 + d, v[d], w[d]);
+        }
+        H0 = make_row(u[0], o[0], u[0], first, last, k);
+        H1 = make_row(u[1], o[1], u[0], first, last, k);
+        H2 = make_row(u[2], o[2], u[1], first, last, k);
+        C0 = make_row(v[0], w[0], v[0], first, last, k);
+        C1 = make_row(v[1], w[1], v[0], first, last, k);
+        C2 = make_row(v[2], w[2], v[1], first, last, k);
+    }
+
+    // Face records and u_n are fetched one row ahead, like the averages, so
+    // no load is consumed in the iteration that issues it.
+    const int ix = clampi(i + 1, 0, nx);           // vertical face x_{i+1}
+    auto load_fy = [&](int row) { return m.fy[size_t(row) * nx + ic]; };
+    auto load_fx = [&](int row) { return m.fx[size_t(clampi(row, 0, ny - 1)) * (nx + 1) + ix]; };
+    const bool need_n = sp.a != 0.0;
+    FaceMom fyc = load_fy(r), fxc = load_fx(r);
+    double hn = 0.0, cn = 0.0;                     // u_n of the row being finished
+
+    float topH[3] = {0.f, 0.f, 0.f}, topC[3] = {0.f, 0.f, 0.f};
+    // (left - right + bottom) face fluxes of the row waiting for its top face,
+    // summed in that order as soon as they are known.
+    double accH = 0, accC = 0;
+    double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
+
+    for (; r <= j1; ++r) {
+        const bool have = r < ny;
+        const bool preroll = r == j0 - 1;
+        const bool shift = r + 1 <= j1;
+        double nH = 0, oH = 0, nC = 0, oC = 0;
+        if (shift) {
+            load(sp.Hk, r + 2, nH, oH);
+            load(sp.Ck, r + 2, nC, oC);
+        }
+        FaceLines<float> LH, LC;
+        if (have) {
+            const unsigned mask = col_ok ? candidate_mask(i, r, nx, ny) : 2u;
+            uint32_t e2 = 0;
+            recon32(k, H0, H1, H2, mask, LH, e2);
+            recon32(k, C0, C1, C2, mask, LC, e2);
+            if (col_ok) err |= e2;
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) { LH.c[f][c] = 0.f; LC.c[f][c] = 0.f; }
+        }
+        const double tH = H1.u, tC = C1.u;
+
+        // Horizontal face y_r: top of row r-1 (centre H0.u), bottom of row r.
+        double FvH = 0.0, FvC = 0.0;
+        if (!preroll && col_ok) {
+            if (r == 0 || r == ny) {
+                const int side = r == 0 ? 0 : 2;
+                const double sign = r == 0 ? -1.0 : 1.0;
+                const double* vn = m.vny + size_t(r) * nx + ic;
+                const double* inH = m.inflowH[side] ? m.inflowH[side] + size_t(ic) * m.q : nullptr;
+                const double* inC = m.inflowC[side] ? m.inflowC[side] + size_t(ic) * m.q : nullptr;
+                const uint8_t pol = m.bc[side] ? m.bc[side][ic] : 0;
+                const double oHf = r == 0 ? boundary_flux<float>(m, vn, vys, sign, tH, LH.c[0], m.sh, m.wh, pol, inH, err)
+                                          : boundary_flux<float>(m, vn, vys, sign, H0.u, topH, m.sh, m.wh, pol, inH, err);
+                const double oCf = r == 0 ? boundary_flux<float>(m, vn, vys, sign, tC, LC.c[0], m.sh, m.wh, pol, inC, err)
+                                          : boundary_flux<float>(m, vn, vys, sign, C0.u, topC, m.sh, m.wh, pol, inC, err);
+                FvH = sign * oHf;
+                FvC = sign * oCf;
+                if (owned) { bH += oHf; bC += oCf; }
+            } else {
+                face_flux(fyc, H0.u, topH, tH, LH.c[0], C0.u, topC, tC, LC.c[0], FvH, FvC);
+            }
+        }
+        if (shift) fyc = load_fy(r + 1);   // used next iteration
+
+        // Finish row r-1 now that its top face is known.
+        if (!preroll && r > j0 && owned) {
+            const size_t kk = size_t(r - 1) * nx + i;
+            const double rhsH = (accH - FvH) * m.inv_area;
+            const double rhsC = (accC - FvC) * m.inv_area;
+            double outH = sp.b * (H0.u + sp.dt * rhsH), outC = sp.b * (C0.u + sp.dt * rhsC);
+            if (need_n) {
+                outH = sp.a * hn + outH;
+                outC = sp.a * cn + outC;
+            }
+            if (!isfinite(outH) || !isfinite(outC)) err |= kErrNonFinite;
+            sp.Ho[kk] = outH;
+            sp.Co[kk] = outC;
+            if constexpr (PHASE) {
+                const PhaseOut po = phase_eval(m.ph, outH, outC, err);
+                sp.T[kk] = po.T;
+                sp.phi[kk] = po.phil;
+                sp.Cl[kk] = po.cl;
+                sp.Cs[kk] = po.cs;
+                sp.region[kk] = uint8_t(po.region);
+                two_sum_acc(sH, eH, outH);
+                two_sum_acc(sC, eC, outC);
+            }
+        }
+
+        if (need_n && have) {               // u_n of row r, finished next iteration
+            const size_t kn = size_t(r) * nx + ic;
+            hn = sp.Hn[kn];
+            cn = sp.Cn[kn];
+        }
+
+        // Vertical faces x_{i+1} of row r; lane i+1 supplies its centre and left line.
+        if (!preroll && r < j1) {
+            const double nbtH = shl1(tH), nbtC = shl1(tC);
+            float nbH[3], nbC[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                nbH[c] = shl1(LH.c[3][c]);
+                nbC[c] = shl1(LC.c[3][c]);
+            }
+            double FrH = 0.0, FrC = 0.0;
+            if (i >= -1 && i <= nx - 1 && have) {
+                if (i == -1 || i == nx - 1) {
+                    const bool left = i == -1;
+                    const int side = left ? 3 : 1;
+                    const double sign = left ? -1.0 : 1.0;
+                    const double* vn = m.vnx + size_t(r) * (nx + 1) + (i + 1);
+                    const double* inH = m.inflowH[side] ? m.inflowH[side] + size_t(r) * m.q : nullptr;
+                    const double* inC = m.inflowC[side] ? m.inflowC[side] + size_t(r) * m.q : nullptr;
+                    const uint8_t pol = m.bc[side] ? m.bc[side][r] : 0;
+                    const double oHf = left ? boundary_flux<float>(m, vn, vxs, sign, nbtH, nbH, m.sv, m.wv, pol, inH, err)
+                                            : boundary_flux<float>(m, vn, vxs, sign, tH, LH.c[1], m.sv, m.wv, pol, inH, err);
+                    const double oCf = left ? boundary_flux<float>(m, vn, vxs, sign, nbtC, nbC, m.sv, m.wv, pol, inC, err)
+                                            : boundary_flux<float>(m, vn, vxs, sign, tC, LC.c[1], m.sv, m.wv, pol, inC, err);
+                    FrH = sign * oHf;
+                    FrC = sign * oCf;
+                    if (left || owned) { bH += oHf; bC += oCf; }
+                } else {
+                    face_flux(fxc, tH, LH.c[1], nbtH, nbH, tC, LC.c[1], nbtC, nbC, FrH, FrC);
+                }
+            }
+            accH = shr1(FrH) - FrH + FvH;
+            accC = shr1(FrC) - FrC + FvC;
+        }
+
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { topH[c] = LH.c[2][c]; topC[c] = LC.c[2][c]; }
+        if (shift) fxc = load_fx(r + 1);   // used next iteration
+        if (shift) {
+            H0 = H1; H1 = H2;
+            C0 = C1; C1 = C2;
+            H2 = make_row(nH, oH, H1.u, first, last, k);
+            C2 = make_row(nC, oC, C1.u, first, last, k);
+        }
+    }
+    item_partials(sp, item, sH, eH, sC, eC, bH, bC, err);
+}
+
+template <bool PHASE>
+__global__ __launch_bounds__(256, 3) void stage32_kernel(const MeshDev* __restrict__ mesh, StagePtrs sp, int nitems)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int item = blockIdx.x * 4 + wave;
+    if (item >= nitems) return;
+    const MeshDev& m = *mesh;
+    const int strip = item % sp.strips, band = item / sp.strips;
+    const int j0 = band * sp.rows;
+    const int j1 = min(j0 + sp.rows, m.ny);
+    sweep32_item<PHASE>(m, sp, item, strip, j0, j1);
+}
+
+// Persistent form (queue != nullptr): every wave takes the next work item
+// from a counter until the list is exhausted; otherwise one wave per item.
+template <bool PHASE>
+__global__ __launch_bounds__(256, 3) void batch32_kernel(const MeshDev* __restrict__ meshes,
+                                                         const StagePtrs* __restrict__ stages,
+                                                         const int2* __restrict__ items, int nitems,
+                                                         int* queue)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    int it = blockIdx.x * 4 + wave;
+    for (;;) {
+        if (queue) {
+            int t = 0;
+            if ((threadIdx.x & (kWave - 1)) == 0) t = atomicAdd(queue, 1);
+            it = __builtin_amdgcn_readfirstlane(t);
+        }
+        if (it >= nitems) break;
+        const int2 w = items[it];
+        const int slot = __builtin_amdgcn_readfirstlane(w.x);
+        const int local = __builtin_amdgcn_readfirstlane(w.y);
+        const MeshDev& m = meshes[slot];
+        const StagePtrs& sp = stages[slot];
+        const int strip = local % sp.strips, band = local / sp.strips;
+        const int j0 = band * sp.rows;
+        if (j0 < m.ny)                 // always true with a current work list
+            sweep32_item<PHASE>(m, sp, local, strip, j0, min(j0 + sp.rows, m.ny));
+        if (!queue) break;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_stage32(bool phase, const MeshDev* d_mesh, const StagePtrs& sp, int nitems,
+                          hipStream_t stream)
+{
+    const dim3 grid((nitems + 3) / 4), block(256);
+    if (phase) stage32_kernel<true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    else stage32_kernel<false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    return hipGetLastError();
+}
+
+int batch_blocks_per_cu32()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch32_kernel<false>, 256, 0) != hipSuccess) return 0;
+    return n;
+}
+
+hipError_t launch_batch_stage32(bool phase, const MeshDev* meshes, const StagePtrs* stages,
+                                const int2* items, int nitems, int blocks, int* queue,
+                                hipStream_t stream)
+{
+    if (nitems <= 0) return hipSuccess;
+    int* q = blocks > 0 ? queue : nullptr;
+    const dim3 grid(blocks > 0 ? blocks : (nitems + 3) / 4), block(256);
+    if (phase) batch32_kernel<true><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q);
+    else batch32_kernel<false><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q);
+    return hipGetLastError();
+}
+
+}  // namespace mp

@@ -135,6 +135,14 @@ mp_status make_phase(const mp_phase_params& p, PhaseHost& out)
     h.dev.Tmp = (double)tmp;
     if (h.dev.Tmp <= h.dev.Tep) return fail(MP_ERR_FP, "Phase temperature intervals are not resolvable");
     h.dev.hyper = p.hypereutectic ? 1 : 0;
+    // 1/x as hi + lo with lo = (1 - hi*x)/x: exact to a few units of 2^-100
+    auto recip = [](double x, double& hi, double& lo) {
+        hi = 1.0 / x;
+        lo = std::fma(-hi, x, 1.0) / x;
+    };
+    recip(h.dev.Xe, h.dev.ixh, h.dev.ixl);
+    recip(1.0 - h.dev.Xe, h.dev.imh, h.dev.iml);
+    recip(h.dev.LD, h.dev.ilh, h.dev.ill);
     out = h;
     return MP_OK;
 }

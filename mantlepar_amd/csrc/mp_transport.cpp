@@ -61,14 +61,10 @@ mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
     MP_CALL(vnx.alloc(size_t(m.q) * m.ny * (m.nx + 1) * sizeof(double)));
     MP_CALL(vny.alloc(size_t(m.q) * (m.ny + 1) * m.nx * sizeof(double)));
     const size_t nxf = size_t(m.ny) * (m.nx + 1), nyf = size_t(m.ny + 1) * m.nx;
-    MP_CALL(fxv.alloc(2 * nxf * sizeof(double)));
-    MP_CALL(fxm.alloc(4 * nxf * sizeof(float)));
-    MP_CALL(fyv.alloc(2 * nyf * sizeof(double)));
-    MP_CALL(fym.alloc(4 * nyf * sizeof(float)));
-    md.fxv = fxv.as<double>();
-    md.fxm = fxm.as<float>();
-    md.fyv = fyv.as<double>();
-    md.fym = fym.as<float>();
+    MP_CALL(fx.alloc(nxf * sizeof(FaceMom)));
+    MP_CALL(fy.alloc(nyf * sizeof(FaceMom)));
+    md.fx = fx.as<FaceMom>();
+    md.fy = fy.as<FaceMom>();
     MP_CALL(error.alloc(sizeof(uint64_t) * 2));
     MP_CALL(d_md.alloc(sizeof(MeshDev)));
     MP_HIP(hipMemsetAsync(H[0].ptr, 0, N * sizeof(double), stream));
@@ -82,20 +78,21 @@ mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
     return MP_OK;
 }
 
-// Rows per work item when the caller sets none: the largest of 64, 32, 16
-// that still gives every wave slot of the chip two items, else 8.  A small
-// mesh then runs as many short items (its stage latency is one item's rows),
-// a large one as fewer, longer items (less halo and epilogue work).
+// Rows per work item when the caller sets none: 8, or 16 for the FP64 sweep
+// on a mesh large enough to give every wave slot of the chip two 16-row items.
+// These were the fastest measured on MI355X (2048x4096: 8 rows for the mixed
+// sweep, 16 for FP64; longer items, even ones sized to fill the chip in a
+// single round, ran slower), and a small mesh's stage time is the time of one
+// item, so short items also serve small meshes best.
 int mp_transport::default_rows() const
 {
+    if (precision == MP_PREC_MIXED) return 8;
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const long long slots = (long long)cus * 4 * (precision == MP_PREC_MIXED ? 3 : 2);
+    const long long slots = (long long)cus * 4 * 2;
     const long long strips = (mesh.nx + kStripCells - 1) / kStripCells;
-    for (int r : {64, 32, 16})
-        if (strips * ((mesh.ny + r - 1) / r) >= 2 * slots) return r;
-    return 8;
+    return strips * ((mesh.ny + 15) / 16) >= 2 * slots ? 16 : 8;
 }
 
 mp_transport::~mp_transport()
@@ -162,8 +159,8 @@ mp_status mp_transport::set_velocity(const double* x, const double* y)
     }
     MP_HIP(hipMemcpyAsync(vnx.ptr, x, nxv * sizeof(double), hipMemcpyHostToDevice, stream));
     MP_HIP(hipMemcpyAsync(vny.ptr, y, nyv * sizeof(double), hipMemcpyHostToDevice, stream));
-    MP_HIP(launch_face_prep(d_md.as<MeshDev>(), mesh.nx, mesh.ny, fxv.as<double>(), fxm.as<float>(),
-                            fyv.as<double>(), fym.as<float>(), stream));
+    MP_HIP(launch_face_prep(d_md.as<MeshDev>(), mesh.nx, mesh.ny, fx.as<FaceMom>(), fy.as<FaceMom>(),
+                            stream));
     MP_CALL(compute_rate());
     have_velocity = true;
     return MP_OK;

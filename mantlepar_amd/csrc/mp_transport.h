@@ -21,7 +21,7 @@ struct mp_transport {
     mp::DeviceBuffer H[5], C[5];
     int cur = 0;
     mp::DeviceBuffer vnx, vny;
-    mp::DeviceBuffer fxv, fxm, fyv, fym;   // per-face flux moments
+    mp::DeviceBuffer fx, fy;   // per-face flux moments
     bool have_velocity = false;
     double max_rate = 0;
     mp::DeviceBuffer bc[4], inflowH, inflowC;
