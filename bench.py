@@ -36,6 +36,8 @@ def parse():
     p.add_argument("--batch-steps", type=int, default=50)
     p.add_argument("--serve-requests", type=int, default=400)
     p.add_argument("--serve-rate", type=float, default=300.0, help="Poisson arrivals per second")
+    p.add_argument("--full-steps", type=int, default=1000,
+                   help="steps of the 8192x8192 full-chip run (0: skip it)")
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--tune-cache", default=os.path.join(HERE, "gpurun_out", "batch_tune.json"))
@@ -237,6 +239,133 @@ def bench_serve(args, meshes):
     return out
 
 
+def _threaded(fn, n, threads=16):
+    """Run fn(lo, hi) over [0, n) in row blocks on a thread pool (numpy
+    releases the GIL inside its loops)."""
+    from concurrent.futures import ThreadPoolExecutor
+    step = max(1, (n + threads - 1) // threads)
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda lo: fn(lo, min(n, lo + step)), range(0, n, step)))
+
+
+def bench_full_chip(args):
+    """8192x8192 cells, SSP-RK3 at CFL 0.4, face velocities refreshed every 50
+    steps (psi amplitude drift +-10%) plus a through-flow leaving through the
+    outflow-only top segment x in [0, 0.025]; mixed and FP64.  Reports the
+    stepping throughput (refresh uploads timed apart), the conservation
+    balance of each run and the mixed-vs-FP64 drift of the global totals."""
+    import mantlepar_amd as M
+    from mantlepar_amd import workloads as W
+    spec, K, cfl, every = W.config5()
+    K = args.full_steps
+    nref = max(1, K // every)
+    H = W.enthalpy(spec)
+    Cc = W.composition(spec)
+    bx, by = W.face_velocity(spec)
+    tx, ty = W.throughflow_velocity(spec)
+    # the through-flow is zero outside a corner block: keep only that block
+    jx = np.nonzero(np.abs(tx).max(axis=(0, 2)))[0]
+    ix = np.nonzero(np.abs(tx).max(axis=(0, 1)))[0]
+    jy = np.nonzero(np.abs(ty).max(axis=(0, 2)))[0]
+    iy = np.nonzero(np.abs(ty).max(axis=(0, 1)))[0]
+    bx_blk = (slice(None), slice(jx[0], jx[-1] + 1), slice(ix[0], ix[-1] + 1))
+    by_blk = (slice(None), slice(jy[0], jy[-1] + 1), slice(iy[0], iy[-1] + 1))
+    tx_blk, ty_blk = tx[bx_blk].copy(), ty[by_blk].copy()
+    del tx, ty
+    bc, inflow = W.throughflow_boundary(spec)
+    amps = W.refresh_amplitudes(nref, seed=spec.seed)
+    vx, vy = np.empty_like(bx), np.empty_like(by)
+
+    def refresh(a):
+        _threaded(lambda lo, hi: np.multiply(bx[:, lo:hi], a, out=vx[:, lo:hi]), bx.shape[1])
+        _threaded(lambda lo, hi: np.multiply(by[:, lo:hi], a, out=vy[:, lo:hi]), by.shape[1])
+        vx[bx_blk] += tx_blk
+        vy[by_blk] += ty_blk
+
+    area = spec.dx * spec.dy
+    res = {}
+    rng = np.random.default_rng(11)
+    for prec, name in ((M.PREC_MIXED, "mixed"), (M.PREC_FP64, "fp64"), (M.PREC_FP64, "fp64p")):
+        t = M.Transport(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q,
+                        phase=W.config_phase(), precision=prec)
+        if name == "fp64p":
+            # FP64 from (H, C)(1 + 2^-24 xi): the method's own sensitivity to
+            # differences of FP32 size, against which pointwise errors are read
+            t.upload(H * (1.0 + 2.0 ** -24 * rng.standard_normal(H.shape)),
+                     Cc * (1.0 + 2.0 ** -24 * rng.standard_normal(Cc.shape)))
+        else:
+            t.upload(H, Cc)
+        t.set_boundary(bc["bottom"], bc["right"], bc["top"], bc["left"])
+        t.set_inflow(*inflow)
+        tot0 = {"total_H": float(np.sum(H)) * area, "total_C": float(np.sum(Cc)) * area}
+        b_h = b_c = 0.0
+        t_step = t_refresh = 0.0
+        rep = tot0
+        for k in range(nref):
+            t0 = time.perf_counter()
+            refresh(amps[k])
+            t.set_face_velocity(vx, vy)
+            dt = t.stable_dt(cfl)
+            t.synchronize()
+            t1 = time.perf_counter()
+            rep = t.step(min(every, K - k * every), dt)
+            t2 = time.perf_counter()
+            t_refresh += t1 - t0
+            t_step += t2 - t1
+            b_h += rep["boundary_H"]
+            b_c += rep["boundary_C"]
+        Hf, Cf = t.download()
+        ph = t.phase_state()
+        prof = t.profile(4, dt)
+        t.close()
+        res[name] = {"t_step": t_step, "t_refresh": t_refresh, "H": Hf, "C": Cf, "T": ph["T"],
+                     "phi": ph["phi"], "tot0": (tot0["total_H"], tot0["total_C"]),
+                     "tot1": (rep["total_H"], rep["total_C"]), "b": (b_h, b_c), "profile": prof}
+        log(f"full-chip {name}: {spec.cells * K / t_step:.3e} cell-steps/s "
+            f"(refresh {t_refresh:.1f} s)")
+
+    def rel(a, b):
+        return float(np.abs(a - b).max() / np.abs(b).max())
+
+    mx, fp, fpp = res["mixed"], res["fp64"], res.pop("fp64p")
+    out = {"workload": f"{spec.nx}x{spec.ny} cells, {K} SSP-RK3 steps, CFL 0.4, velocities "
+                       f"refreshed every {every} steps (amplitude drift +-10%), through-flow "
+                       "out of the top segment x in [0, 0.025] (outflow-only policy)",
+           "cells": spec.cells, "steps": K, "unit": "cell-steps/s"}
+    for name, r in res.items():
+        out[name] = {
+            "value": spec.cells * K / r["t_step"],
+            "ms_per_step": r["t_step"] / K * 1e3,
+            "refresh_s_total": r["t_refresh"],
+            "ms_stage": r["profile"]["ms_stage"],
+            "ms_phase_stage": r["profile"]["ms_phase_stage"],
+            # total(end) - total(start) + outward boundary flux: zero up to rounding
+            "conservation_defect_rel": {
+                "H": (r["tot1"][0] - r["tot0"][0] + r["b"][0]) / r["tot0"][0],
+                "C": (r["tot1"][1] - r["tot0"][1] + r["b"][1]) / r["tot0"][1]},
+            "outflow": {"H": r["b"][0], "C": r["b"][1]},
+        }
+    out["drift_vs_fp64"] = {
+        "total_H_rel": (mx["tot1"][0] - fp["tot1"][0]) / fp["tot1"][0],
+        "total_C_rel": (mx["tot1"][1] - fp["tot1"][1]) / fp["tot1"][1],
+        "state_max_rel": {"H": rel(mx["H"], fp["H"]), "C": rel(mx["C"], fp["C"]),
+                          "T": rel(mx["T"], fp["T"]), "phi": float(np.abs(mx["phi"] - fp["phi"]).max())},
+        "cells_H_diff_above_1e-6": int(np.count_nonzero(np.abs(mx["H"] - fp["H"]) > 1e-6)),
+        # the same measures between FP64 runs whose initial states differ by 2^-24
+        "fp64_self_spread": {"H": rel(fpp["H"], fp["H"]), "C": rel(fpp["C"], fp["C"]),
+                             "T": rel(fpp["T"], fp["T"]),
+                             "phi": float(np.abs(fpp["phi"] - fp["phi"]).max()),
+                             "cells_H_diff_above_1e-6":
+                                 int(np.count_nonzero(np.abs(fpp["H"] - fp["H"]) > 1e-6))},
+    }
+    b = algorithmic_bytes(spec.q)
+    out["roofline_stage"] = {"ms_per_launch": mx["profile"]["ms_stage"],
+                             "achieved": spec.cells * 0.5 * (b["stage1"] + b["stage2"]) /
+                             (mx["profile"]["ms_stage"] * 1e-3) / 1e9,
+                             "unit": "GB/s"}
+    return out
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -282,10 +411,12 @@ def main():
         rep = t.step(args.steps, dt)
         el = time.perf_counter() - t0
         Hf, Cf = t.download()
+        ph = t.phase_state()
         # kernel durations with HIP events on the handle's stream, over the same work
         t.upload(H, Cc)
         prof = t.profile(min(args.steps, 20), dt)
-        res[name] = {"seconds": el, "report": rep, "H": Hf, "C": Cf, "profile": prof,
+        res[name] = {"seconds": el, "report": rep, "H": Hf, "C": Cf, "T": ph["T"],
+                     "phi": ph["phi"], "region": ph["region"], "profile": prof,
                      "rows": rows_used}
         t.close()
         log(f"single {name}: {spec.cells * args.steps / el:.3e} cell-steps/s (rows {rows_used})")
@@ -297,6 +428,9 @@ def main():
     fp64_value = N * K / fp64["seconds"]
     errH = float(np.max(np.abs(mixed["H"] - fp64["H"])) / np.max(np.abs(fp64["H"])))
     errC = float(np.max(np.abs(mixed["C"] - fp64["C"])) / np.max(np.abs(fp64["C"])))
+    errT = float(np.max(np.abs(mixed["T"] - fp64["T"])) / np.max(np.abs(fp64["T"])))
+    errphi = float(np.max(np.abs(mixed["phi"] - fp64["phi"])))
+    region_diff = int(np.count_nonzero(mixed["region"] != fp64["region"]))
     b = algorithmic_bytes(spec.q)
     prof = mixed["profile"]
     stage_bytes = N * 0.5 * (b["stage1"] + b["stage2"])
@@ -321,7 +455,8 @@ def main():
                    "corrections, FP64 state, first-order flux, phase and sums)"},
         "fp64": {"value": fp64_value, "ms_per_step": fp64["seconds"] / K * 1e3,
                  "mixed_over_fp64": value / fp64_value},
-        "error_vs_fp64": {"H": errH, "C": errC, "steps": K},
+        "error_vs_fp64": {"H": errH, "C": errC, "T": errT, "phi_abs": errphi,
+                          "region_cells_differing": region_diff, "steps": K},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "stage_sweep_kernel<1,false> (stages 1-2)",
@@ -329,6 +464,8 @@ def main():
                      "ms_per_launch": prof["ms_stage"], "rows_per_item": mixed["rows"]},
         "conservation": {"total_H": mixed["report"]["total_H"], "total_C": mixed["report"]["total_C"]},
     }
+    if args.full_steps > 0:
+        out["full_chip"] = bench_full_chip(args)
     if not (args.no_batch and args.no_serve):
         meshes = batch_inputs()
         if not args.no_batch:

@@ -179,14 +179,14 @@ __device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, dou
         const double gap = fabs(H - Tmp);
         s.dTdC = gap == 0.0 ? -__builtin_inf() : -((LD / gap) / Xe);
     }
-    const double solid = s.phi1 + s.phi2;
-    s.cs = s.phi2 > 0.0 && solid > 0.0 ? s.phi2 / solid : 0.0;
     if (!(s.phil > 0.0)) s.cl = 0.0;
-    return s;
+    return s;   // cs is formed by the caller, after any mirroring
 }
 
 // EutecticModel::evaluate with optional hypereutectic mirror (C > Xe evaluates
-// C' = 1-C on the diagram with eutectic 1-Xe and maps the phases back).
+// C' = 1-C on the diagram with eutectic 1-Xe and maps the phases back).  The
+// mirror only selects the core's inputs, so a wave whose lanes lie on both
+// sides of Xe runs the core once.
 __device__ __forceinline__ PhaseOut phase_eval(const PhaseDev& p, double H, double C,
                                                uint32_t& err)
 {
@@ -201,16 +201,21 @@ __device__ __forceinline__ PhaseOut phase_eval(const PhaseDev& p, double H, doub
         err |= kErrComposition;
         C = fmin(fmax(C, 0.0), cmax);
     }
-    if (!(C > Xe)) return phase_core(p, Xe, p.ixh, p.ixl, H, C, err);
-    PhaseOut s = phase_core(p, 1.0 - Xe, p.imh, p.iml, H, 1.0 - C, err);
-    PhaseOut r = s;
-    r.phi1 = s.phi2;
-    r.phi2 = s.phi1;
-    r.cl = s.phil > 0.0 ? 1.0 - s.cl : 0.0;
-    const double solid = r.phi1 + r.phi2;
-    r.cs = r.phi2 > 0.0 && solid > 0.0 ? r.phi2 / solid : 0.0;
-    r.dTdC = -s.dTdC;
-    return r;
+    const bool mirror = C > Xe;
+    const double xs = mirror ? 1.0 - Xe : Xe;
+    const double ih = mirror ? p.imh : p.ixh, il = mirror ? p.iml : p.ixl;
+    const double cs = mirror ? 1.0 - C : C;
+    PhaseOut s = phase_core(p, xs, ih, il, H, cs, err);
+    if (mirror) {
+        const double a = s.phi1;
+        s.phi1 = s.phi2;
+        s.phi2 = a;
+        s.cl = s.phil > 0.0 ? 1.0 - s.cl : 0.0;
+        s.dTdC = -s.dTdC;
+    }
+    const double solid = s.phi1 + s.phi2;
+    s.cs = s.phi2 > 0.0 && solid > 0.0 ? s.phi2 / solid : 0.0;
+    return s;
 }
 
 }  // namespace mp

@@ -155,6 +155,77 @@ def face_velocity(spec):
     return np.ascontiguousarray(vnx), np.ascontiguousarray(vny)
 
 
+def _smoothstep(t):
+    t = np.clip(t, 0.0, 1.0)
+    return t * t * (3.0 - 2.0 * t)
+
+
+def _dsmoothstep(t):
+    return np.where((t > 0.0) & (t < 1.0), 6.0 * t * (1.0 - t), 0.0)
+
+
+def throughflow_velocity(spec, Q=0.02, y_in=(0.9, 1.0), x_out=(0.0, 0.025)):
+    """Face velocities of psi2 = Q*rho(y)*sigma(x), rho rising 0 -> 1 over y_in
+    and sigma falling 1 -> 0 over x_out (smoothstep ramps).  Flow enters
+    through the left side for y in y_in and leaves through the top for x in
+    x_out; psi2 vanishes on the bottom and right sides.  Sampled at the edge
+    Gauss points with the same per-face correction as face_velocity, so every
+    cell stays discretely divergence-free.  Returns (vnx, vny) like
+    face_velocity; the arrays are zero outside the corner block.
+
+    The ramp ends are snapped to face edges: inside a ramp the smoothstep
+    derivative is a cubic, which the edge Gauss rule integrates exactly, so
+    the per-face corrections vanish to rounding and no Gauss point of an
+    outflow face turns inward."""
+    g, w = gauss_legendre(spec.q)
+    ya, yb, xa, xb = _snapped_ramps(spec, y_in, x_out)
+    xs = spec.x0 + np.arange(spec.nx + 1) * spec.dx          # vertical face abscissae
+    ys = spec.y0 + np.arange(spec.ny + 1) * spec.dy          # horizontal face ordinates
+    rho = _smoothstep((ys - ya) / (yb - ya))
+    sig = 1.0 - _smoothstep((xs - xa) / (xb - xa))
+    # vertical faces: u = dpsi2/dy = Q sigma(x) rho'(y)
+    yq = ys[:-1][None, :] + 0.5 * (1.0 + g[:, None]) * spec.dy              # (q, ny)
+    drho = _dsmoothstep((yq - ya) / (yb - ya)) / (yb - ya)
+    S = (0.5 * spec.dy) * (w @ drho)
+    fy = drho + ((rho[1:] - rho[:-1]) - S)[None, :] / spec.dy                # (q, ny)
+    vnx = Q * fy[:, :, None] * sig[None, None, :]
+    # horizontal faces: v = -dpsi2/dx = -Q sigma'(x) rho(y)
+    xq = xs[:-1][None, :] + 0.5 * (1.0 + g[:, None]) * spec.dx              # (q, nx)
+    dsig = -_dsmoothstep((xq - xa) / (xb - xa)) / (xb - xa)
+    Sx = (0.5 * spec.dx) * (w @ dsig)
+    fx = dsig + ((sig[1:] - sig[:-1]) - Sx)[None, :] / spec.dx               # (q, nx)
+    vny = -Q * rho[None, :, None] * fx[:, None, :]
+    return np.ascontiguousarray(vnx), np.ascontiguousarray(vny)
+
+
+def _snapped_ramps(spec, y_in, x_out):
+    def snap(v, lo, h):
+        return lo + np.round((v - lo) / h) * h
+    return (snap(y_in[0], spec.y0, spec.dy), snap(y_in[1], spec.y0, spec.dy),
+            snap(x_out[0], spec.x0, spec.dx), snap(x_out[1], spec.x0, spec.dx))
+
+
+def throughflow_boundary(spec, y_in=(0.9, 1.0), x_out=(0.0, 0.025), H_in=0.9, C_in=0.3):
+    """Policies and inflow states for throughflow_velocity: inflow/outflow on
+    the left faces that overlap y_in, outflow-only on the top faces that
+    overlap x_out, zero flux elsewhere.  Returns (bc dict, (inflow_H,
+    inflow_C)) in the layouts of Transport.set_boundary / set_inflow."""
+    ya, _, _, xb = _snapped_ramps(spec, y_in, x_out)
+    ys = spec.y0 + np.arange(spec.ny + 1) * spec.dy
+    xs = spec.x0 + np.arange(spec.nx + 1) * spec.dx
+    left = np.where(ys[1:] > ya + 0.5 * spec.dy, 2, 0).astype(np.uint8)
+    top = np.where(xs[:-1] < xb - 0.5 * spec.dx, 1, 0).astype(np.uint8)
+    bc = {"bottom": np.zeros(spec.nx, np.uint8), "right": np.zeros(spec.ny, np.uint8),
+          "top": top, "left": left}
+    n = 2 * (spec.nx + spec.ny) * spec.q
+    return bc, (np.full(n, H_in), np.full(n, C_in))
+
+
+def refresh_amplitudes(count, seed=5, drift=0.1):
+    """psi amplitude factor of each velocity refresh: 1 + drift*(2u - 1)."""
+    return 1.0 + drift * (2.0 * unit_uniform(seed + 31337, np.arange(count)) - 1.0)
+
+
 def config_phase(**kw):
     """Phase parameters of the synthetic configs: latent heat LD = 1,
     eutectic at Xe = 0.4, Tep = 0.6 (Tm0 = 1600 K, Te0 = 600 K), P = 0, and
@@ -172,6 +243,13 @@ def config1():
 def config2():
     """2048x4096 cells, 100 SSP-RK3 steps at CFL 0.4."""
     return MeshSpec(2048, 4096, seed=2), 100, 0.4
+
+
+def config5():
+    """Full-chip run: 8192x8192 cells, 1000 SSP-RK3 steps at CFL 0.4, face
+    velocities refreshed every 50 steps (psi amplitude drift +-10%), outflow
+    through the top segment x in [0, 0.025]."""
+    return MeshSpec(8192, 8192, seed=5), 1000, 0.4, 50
 
 
 def batch_specs(count=256, seed=3):
