@@ -245,7 +245,7 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
                 sp.Ho[k] = oH;
                 sp.Co[k] = oC;
                 if constexpr (PHASE) {
-                    const PhaseOut po = phase_eval(m.ph, oH, oC, err);
+                    const PhaseOut po = phase_eval<false>(m.ph, oH, oC, err);
                     sp.T[k] = po.T;
                     sp.phi[k] = po.phil;
                     sp.Cl[k] = po.cl;

@@ -97,22 +97,33 @@ __device__ __forceinline__ double unit_fraction(double v, uint32_t& err)
 }
 
 // One evaluation on the side 0 <= C <= Xe of a diagram with eutectic Xe;
-// (ixh, ixl) is 1/Xe as a double-double.
+// (ixh, ixl) is 1/Xe as a double-double.  DERIV = false skips dT/dC and dT/dH
+// (the stage kernels do not store them; analytically dT/dH lies in [0, 1]:
+// root >= |a| and root^2 >= 4 LD r, so its unitFraction check cannot fire).
+template <bool DERIV = true>
 __device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, double ixh, double ixl,
                                                double H, double C, uint32_t& err)
 {
     PhaseOut s;
     s.T = 0; s.phil = 0; s.cl = 0; s.cs = 0; s.phi1 = 0; s.phi2 = 0; s.dTdC = 0; s.dTdH = 0;
     const double LD = p.LD, Tep = p.Tep, Tmp = p.Tmp;
-    // boundaries (:163-185); C/Xe as C times the double-double 1/Xe
-    const DD ratio = dd_mul_d({ixh, ixl}, C);
-    const double Heut = dd_add({Tep, 0.0}, dd_mul_d(ratio, LD)).hi;
-    double Hliq;
-    if (C == 0.0) {
-        Hliq = Tmp + LD;
-    } else {
-        const DD one_minus = dd_add({1.0, 0.0}, {-ratio.hi, -ratio.lo});
-        Hliq = dd_add(dd_add({Tep, 0.0}, one_minus), {LD, 0.0}).hi;
+    // Boundaries (:163-185) in plain FP64 first: C/Xe as C times the rounded 1/Xe,
+    // each boundary within 3 eps (|Tep| + 1 + LD) of its correctly rounded value.
+    // Only when H lies within a 16 eps margin of one of them are they redone in
+    // double-double, which reproduces the reference's long double -> double
+    // rounding, so the classification below is the reference's in every case.
+    const double rf = C * ixh;
+    double Heut = fma(rf, LD, Tep);
+    double Hliq = C == 0.0 ? Tmp + LD : (Tep + (1.0 - rf)) + LD;
+    const double tol = 16.0 * 2.220446049250313e-16 * (fabs(Tep) + 1.0 + LD);
+    const bool near = fabs(H - Heut) <= tol || fabs(H - Hliq) <= tol;
+    if (near) {
+        const DD ratio = dd_mul_d({ixh, ixl}, C);
+        Heut = dd_add({Tep, 0.0}, dd_mul_d(ratio, LD)).hi;
+        if (C != 0.0) {
+            const DD one_minus = dd_add({1.0, 0.0}, {-ratio.hi, -ratio.lo});
+            Hliq = dd_add(dd_add({Tep, 0.0}, one_minus), {LD, 0.0}).hi;
+        }
     }
     if (Tmp <= Tep || Hliq <= Tep || (C == 0.0 && Hliq <= Tmp)) err |= kErrPhaseRange;
     // selectRegion (:95-105)
@@ -136,20 +147,31 @@ __device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, dou
         s.phi1 = 1.0 - C; s.phi2 = C; s.T = H; s.dTdH = 1.0;
         break;
     case 3: {
-        // Fractions in double-double: near the end of eutectic melting both
-        // solid fractions are tiny differences of O(1) terms.
-        const DD maxl = ratio;
-        DD liquid = maxl;
-        if (H != Heut) {
-            const DD e = dd_mul(dd_two_sum(H, -Tep), {p.ilh, p.ill});
-            if (dd_less(e, maxl)) liquid = e;
+        // Fractions in FP64 while the solid fraction is at least 1/16 (then
+        // Cs = phi2/(phi1+phi2) carries at most ~5e-15); near the boundaries and
+        // near the end of eutectic melting, where both solid fractions are tiny
+        // differences of O(1) terms, in double-double.
+        const double e = (H - Tep) * p.ilh;
+        double liquid = e < rf ? e : rf;
+        double solid2 = fmax(fma(-Xe, liquid, C), 0.0);
+        double solid1 = (1.0 - liquid) - solid2;
+        if (near || !(solid1 + solid2 >= 0.0625)) {
+            const DD maxl = dd_mul_d({ixh, ixl}, C);
+            DD liq = maxl;
+            if (H != Heut) {
+                const DD ed = dd_mul(dd_two_sum(H, -Tep), {p.ilh, p.ill});
+                if (dd_less(ed, maxl)) liq = ed;
+            }
+            DD s2 = dd_add({C, 0.0}, dd_mul_d(liq, -Xe));
+            if (s2.hi < 0.0) s2 = {0.0, 0.0};
+            const DD s1 = dd_add(dd_add({1.0, 0.0}, dd_neg(liq)), dd_neg(s2));
+            liquid = liq.hi + liq.lo;
+            solid2 = s2.hi + s2.lo;
+            solid1 = s1.hi + s1.lo;
         }
-        DD solid2 = dd_add({C, 0.0}, dd_mul_d(liquid, -Xe));
-        if (solid2.hi < 0.0) solid2 = {0.0, 0.0};
-        const DD solid1 = dd_add(dd_add({1.0, 0.0}, dd_neg(liquid)), dd_neg(solid2));
-        s.phil = unit_fraction(liquid.hi + liquid.lo, err);
-        s.phi2 = unit_fraction(solid2.hi + solid2.lo, err);
-        s.phi1 = unit_fraction(solid1.hi + solid1.lo, err);
+        s.phil = unit_fraction(liquid, err);
+        s.phi2 = unit_fraction(solid2, err);
+        s.phi1 = unit_fraction(solid1, err);
         s.cl = Xe;
         s.T = Tep;
         break;
@@ -158,7 +180,7 @@ __device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, dou
         // LD*phi^2 - (H-Tmp)*phi - C/Xe = 0 with the cancellation-free root
         // (:237-257); root = hypot(a, 2 sqrt(LD r)) with one square root.
         const double a = H - Tmp;
-        const double r = ratio.hi;
+        const double r = rf;
         const double root = sqrt(fma(a, a, 4.0 * LD * r));
         const double liquid = a >= 0.0 ? (a + root) * (0.5 * p.ilh) : 2.0 * r / (root - a);
         s.phil = unit_fraction(liquid, err);
@@ -166,16 +188,18 @@ __device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, dou
         const double cl = C / liquid;
         s.cl = unit_fraction(cl, err);
         s.T = a >= 0.0 ? Tmp - cl * ixh : H - LD * liquid;    // r/liquid = cl/Xe
-        s.dTdH = unit_fraction(a >= 0.0 ? 2.0 * (LD / root) * (r / (root + a))
-                                        : 0.5 * (1.0 - a / root), err);
-        s.dTdC = -((LD / root) / Xe);
+        if constexpr (DERIV) {
+            s.dTdH = unit_fraction(a >= 0.0 ? 2.0 * (LD / root) * (r / (root + a))
+                                            : 0.5 * (1.0 - a / root), err);
+            s.dTdC = -((LD / root) / Xe);
+        }
         break;
     }
     default:
         s.phil = 1.0; s.cl = C; s.T = H - LD; s.dTdH = 1.0;
         break;
     }
-    if (C == 0.0 && H > Tep && region != 5) {
+    if (DERIV && C == 0.0 && H > Tep && region != 5) {
         const double gap = fabs(H - Tmp);
         s.dTdC = gap == 0.0 ? -__builtin_inf() : -((LD / gap) / Xe);
     }
@@ -187,6 +211,7 @@ __device__ __forceinline__ PhaseOut phase_core(const PhaseDev& p, double Xe, dou
 // C' = 1-C on the diagram with eutectic 1-Xe and maps the phases back).  The
 // mirror only selects the core's inputs, so a wave whose lanes lie on both
 // sides of Xe runs the core once.
+template <bool DERIV = true>
 __device__ __forceinline__ PhaseOut phase_eval(const PhaseDev& p, double H, double C,
                                                uint32_t& err)
 {
@@ -205,7 +230,7 @@ __device__ __forceinline__ PhaseOut phase_eval(const PhaseDev& p, double H, doub
     const double xs = mirror ? 1.0 - Xe : Xe;
     const double ih = mirror ? p.imh : p.ixh, il = mirror ? p.iml : p.ixl;
     const double cs = mirror ? 1.0 - C : C;
-    PhaseOut s = phase_core(p, xs, ih, il, H, cs, err);
+    PhaseOut s = phase_core<DERIV>(p, xs, ih, il, H, cs, err);
     if (mirror) {
         const double a = s.phi1;
         s.phi1 = s.phi2;
