@@ -100,7 +100,9 @@ __device__ __forceinline__ Row32 make_row(double u, double outer, double below, 
 }
 
 // Reconstruction of the cell at row r from the rows below (R0), at (R1) and
-// above (R2) it.
+// above (R2) it.  EDGE = false is the interior form: every candidate lies in
+// the mesh and there is no constant candidate, so no weight needs masking.
+template <bool EDGE>
 __device__ __forceinline__ void recon32(const K32& k, const Row32& R0, const Row32& R1,
                                         const Row32& R2, unsigned mask, FaceLines<float>& out,
                                         uint32_t& err)
@@ -149,24 +151,40 @@ __device__ __forceinline__ void recon32(const K32& k, const Row32& R0, const Row
         sig[4] = xm + yp + q11[3] * q11[3] * k.q11w;
     }
     // Normalised weights in log space, base 2 (src/mlweno/reconstruction.cpp:212-265).
-    float la[6];
-    la[0] = (mask & 1u) ? k.ldL - k.pL * wlog<float>(sig[0] + k.epsA) : -INFINITY;
+    float la[6], om[6], sum = 0.0f;
+    if constexpr (EDGE) {
+        la[0] = (mask & 1u) ? k.ldL - k.pL * wlog<float>(sig[0] + k.epsA) : -INFINITY;
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
-        la[s + 1] = (mask & (2u << s)) ? k.ldS[s] - k.pS * wlog<float>(sig[s + 1] + k.epsA) : -INFINITY;
-    la[5] = k.laC;
-    float mx = la[0];
+        for (int s = 0; s < 4; ++s)
+            la[s + 1] = (mask & (2u << s)) ? k.ldS[s] - k.pS * wlog<float>(sig[s + 1] + k.epsA) : -INFINITY;
+        la[5] = k.laC;
+        float mx = la[0];
 #pragma unroll
-    for (int s = 1; s < 6; ++s) mx = la[s] > mx ? la[s] : mx;
-    if (!(mx > -INFINITY) || !isfinite(mx)) {
-        err |= kErrWeights;
-        mx = 0.0f;
-    }
-    float om[6], sum = 0.0f;
+        for (int s = 1; s < 6; ++s) mx = la[s] > mx ? la[s] : mx;
+        if (!(mx > -INFINITY) || !isfinite(mx)) {
+            err |= kErrWeights;
+            mx = 0.0f;
+        }
 #pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        om[s] = la[s] > -INFINITY ? wexp<float>(la[s] - mx) : 0.0f;
-        sum += om[s];
+        for (int s = 0; s < 6; ++s) {
+            om[s] = la[s] > -INFINITY ? wexp<float>(la[s] - mx) : 0.0f;
+            sum += om[s];
+        }
+    } else {
+        la[0] = k.ldL - k.pL * wlog<float>(sig[0] + k.epsA);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) la[s + 1] = k.ldS[s] - k.pS * wlog<float>(sig[s + 1] + k.epsA);
+        float mx = fmaxf(fmaxf(fmaxf(la[0], la[1]), fmaxf(la[2], la[3])), la[4]);
+        if (!isfinite(mx)) {
+            err |= kErrWeights;
+            mx = 0.0f;
+        }
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            om[s] = wexp<float>(la[s] - mx);
+            sum += om[s];
+        }
+        om[5] = 0.0f;
     }
     const float inv = 1.0f / sum;
 #pragma unroll
@@ -223,6 +241,11 @@ __device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, in
     const size_t vys = size_t(ny + 1) * nx;
     const K32 k = make_k32(m);
     uint32_t err = 0;
+    // every lane's 3x3 neighbourhood inside the mesh (wave-uniform).  The
+    // phase-fused kernel keeps the single general path: the second copy of the
+    // reconstruction pushed it past its register budget.
+    const bool inner_strip = !PHASE && strip >= 1 && strip * kStripCells + kWave <= nx &&
+                             !k.use_constant;
 
     auto load = [&](const double* base, int row, double& u, double& o) {
         const double* p = base + size_t(clampi(row, 0, ny - 1)) * nx;
@@ -272,11 +295,14 @@ __device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, in
             load(sp.Ck, r + 2, nC, oC);
         }
         FaceLines<float> LH, LC;
-        if (have) {
+        if (!PHASE && have && inner_strip && r >= 1 && r <= ny - 2) {
+            recon32<false>(k, H0, H1, H2, 31u, LH, err);
+            recon32<false>(k, C0, C1, C2, 31u, LC, err);
+        } else if (have) {
             const unsigned mask = col_ok ? candidate_mask(i, r, nx, ny) : 2u;
             uint32_t e2 = 0;
-            recon32(k, H0, H1, H2, mask, LH, e2);
-            recon32(k, C0, C1, C2, mask, LC, e2);
+            recon32<true>(k, H0, H1, H2, mask, LH, e2);
+            recon32<true>(k, C0, C1, C2, mask, LC, e2);
             if (col_ok) err |= e2;
         } else {
 #pragma unroll
