@@ -282,6 +282,51 @@ mp_status mp_server_destroy(mp_server* s);
 /* Seconds on the server clock (the clock of t_submit / t_done). */
 double mp_server_now(mp_server* s);
 
+/* ------------------------------------------------------------- formats --- */
+/* The H/C handoff between a preheat and an evolution: cellH1.dat and
+ * cellC1.dat (text matrices, one row per j, 17 significant digits) and the
+ * schema-1 transport_state.json manifest, as written by
+ * src/couple/initialization.cpp:642-686 (matrices), :823-841 (manifest) and
+ * checked by LoadPreheatState, src/couple/evolve.cpp:54-125.  Host only. */
+typedef struct {
+    char family[64];          /* "rectangular", "perturbed_quadrilateral", ... */
+    int32_t nx, ny;
+    double domain[4];         /* x0, x1, y0, y1 */
+    double perturbation;
+    uint64_t seed;
+    double enthalpy_Jkg, length_m, time_scale_s;   /* scales h0, l0, t0 */
+    double time;              /* nondimensional */
+    int64_t accepted_steps;   /* -1 when the manifest has none */
+} mp_handoff_meta;
+
+/* Writes the three files into an existing directory, each through a temporary
+ * file and a rename. */
+mp_status mp_handoff_write(const char* directory, const mp_handoff_meta* meta, const double* H,
+                           const double* C);
+/* Reads and validates against expect (family, nx, ny, domain and scales to
+ * 1e-12, perturbation and seed unless rectangular); cell averages must be
+ * finite and >= 0, C <= c_max.  found (may be NULL) receives the manifest's
+ * time, accepted steps, perturbation and seed.  H, C are assigned on success. */
+mp_status mp_handoff_read(const char* directory, const mp_handoff_meta* expect, double c_max,
+                          double* H, double* C, mp_handoff_meta* found);
+
+/* One edge Gauss-point sample of the flow solve (FlowPointSample,
+ * src/couple/visualization.h:8-12): edge id (horizontal edges first,
+ * j*nx+i; then vertical, nx*(ny+1) + j*(nx+1) + i; src/core/mesh_info.h:79),
+ * point index q in the edge's canonical (increasing i or j) order, and the
+ * Stokes velocity. */
+typedef struct {
+    int64_t edge_id;
+    int32_t q;
+    double vx, vy;
+} mp_edge_sample;
+
+/* Face velocities for mp_transport_set_face_velocity from edge samples, as
+ * the preheat builds its faces (src/couple/preheat.cpp:83-90): the last
+ * sample of an (edge, q) wins.  Every face point must be sampled. */
+mp_status mp_face_velocity_from_samples(int32_t nx, int32_t ny, int32_t q, int64_t nsamples,
+                                        const mp_edge_sample* samples, double* vnx, double* vny);
+
 /* ------------------------------------------------------------- phase only --- */
 /* EutecticModel::evaluate over n cells on the device (host arrays in/out).
  * P may be NULL (uses params->pressure).  Any output may be NULL. */

@@ -88,6 +88,17 @@ class ServerStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class HandoffMeta(C.Structure):
+    _fields_ = [("family", C.c_char * 64), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("domain", C.c_double * 4), ("perturbation", C.c_double), ("seed", C.c_uint64),
+                ("enthalpy_Jkg", C.c_double), ("length_m", C.c_double),
+                ("time_scale_s", C.c_double), ("time", C.c_double), ("accepted_steps", C.c_int64)]
+
+
+class EdgeSample(C.Structure):
+    _fields_ = [("edge_id", C.c_int64), ("q", C.c_int32), ("vx", C.c_double), ("vy", C.c_double)]
+
+
 _dp = C.POINTER(C.c_double)
 _u8p = C.POINTER(C.c_uint8)
 _vp = C.c_void_p
@@ -147,6 +158,11 @@ SIGNATURES = [
     ("mp_server_set_tile", C.c_int32, [_vp, C.c_int32]),
     ("mp_server_destroy", C.c_int32, [_vp]),
     ("mp_server_now", C.c_double, [_vp]),
+    ("mp_handoff_write", C.c_int32, [C.c_char_p, C.POINTER(HandoffMeta), _dp, _dp]),
+    ("mp_handoff_read", C.c_int32, [C.c_char_p, C.POINTER(HandoffMeta), C.c_double, _dp, _dp,
+                                    C.POINTER(HandoffMeta)]),
+    ("mp_face_velocity_from_samples", C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int64,
+                                                  C.c_void_p, _dp, _dp]),
     ("mp_phase_evaluate", C.c_int32, [C.POINTER(PhaseParams), C.c_int64, _dp, _dp, _dp, _dp,
                                       _dp, _dp, _dp, _u8p, _dp, _dp, _dp, _dp]),
     ("mp_phase_boundaries", C.c_int32, [C.POINTER(PhaseParams), C.c_double, C.c_double, _dp,

@@ -62,7 +62,9 @@ def test_phase_boundaries_entry_point_matches_oracle():
 
 
 def test_missing_library_fails_loudly(tmp_path, monkeypatch):
-    import importlib
+    # monkeypatch restores the path and the loaded handle afterwards; the
+    # module is not reloaded, so its exception class stays the one other tests
+    # imported
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "absent.so"))
     monkeypatch.setattr(_lib, "_lib", None)
     try:
@@ -71,4 +73,3 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
         assert "no CPU fallback" in str(e)
     else:
         raise AssertionError("expected ImportError")
-    importlib.reload(_lib)
