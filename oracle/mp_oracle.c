@@ -863,6 +863,215 @@ int mpo_integrate_boundary_flux(const mpo_point edge[2], int nq, const double* q
     return real_value(sum, flux);
 }
 
+/* ============================================================ diffusive flux */
+
+/* CreateLagrangeDerivativeWeights, src/mlweno/diffusiveflux.cpp:203-254:
+ * first-derivative weights of the Lagrange interpolant through npts equally
+ * spaced nodes 0..n (unit spacing), at the middle (n/2, even npts only) or at
+ * an end node. */
+int mpo_lagrange_weights(int npts, int location, double* w)
+{
+    CHECK(npts >= 2, MPO_ERR_ARG, "A first-derivative rule requires at least two points");
+    CHECK(location >= 0 && location <= 2, MPO_ERR_ARG, "Unknown Lagrange derivative location");
+    CHECK(location != MPO_LAGRANGE_MIDDLE || npts % 2 == 0, MPO_ERR_ARG,
+          "Legacy midpoint derivative requires an even number of points");
+    CHECK(npts <= MPO_MAX_SAMPLES + 1, MPO_ERR_ARG, "Too many Lagrange nodes");
+    const int n = npts - 1;
+    double work[MPO_MAX_SAMPLES + 1];
+    if (location == MPO_LAGRANGE_MIDDLE) {
+        /* d/dx of the interpolant at x = n/2: w_i = f_i C(n,i) / (n - 2i)^2,
+         * antisymmetric (w_{n-i} = -w_i), |f_i| = 2^(1-n) prod_{odd k<=n} k/(k-1)
+         * with the sign alternating from the outermost node inward. */
+        long double factor = exp2l(1.0L - n);
+        for (int k = 3; k <= n; k += 2) factor *= (long double)k / (k - 1);
+        if ((npts / 2) % 2) factor = -factor;
+        CHECK(factor != 0 && isfinite(factor), MPO_ERR_FP,
+              "Lagrange order exceeds representable coefficient range");
+        long double choose = 1;
+        for (int i = 0; i < npts / 2; ++i) {
+            const long double d = (long double)n - 2.0L * i;
+            CALL(real_value(factor * choose / (d * d), &work[i]));
+            work[n - i] = -work[i];
+            choose *= (long double)(n - i) / (i + 1);
+            factor = -factor;
+        }
+    } else {
+        /* end node n: w_n = H_n (harmonic number), w_i = (-1)^(n-i) C(n,i)/(n-i) */
+        long double harmonic = 0;
+        for (int j = 1; j <= n; ++j) harmonic += 1.0L / j;
+        CALL(real_value(harmonic, &work[n]));
+        long double choose = 1, sign = n % 2 ? -1 : 1;
+        for (int i = 0; i < n; ++i) {
+            CALL(real_value(sign * choose / (n - i), &work[i]));
+            choose *= (long double)(n - i) / (i + 1);
+            sign = -sign;
+        }
+        if (location == MPO_LAGRANGE_END_LO) {
+            for (int i = 0; i < npts / 2; ++i) {
+                const double tmp = work[i]; work[i] = work[n - i]; work[n - i] = tmp;
+            }
+            for (int i = 0; i < npts; ++i) work[i] = -work[i];
+        }
+    }
+    memcpy(w, work, (size_t)npts * sizeof(double));
+    return MPO_OK;
+}
+
+static int same_point(mpo_point a, mpo_point b) { return a.p[0] == b.p[0] && a.p[1] == b.p[1]; }
+
+/* FindFace, src/mlweno/diffusiveflux.cpp:24-40 */
+static int find_face(const mpo_point cell[4], const mpo_point edge[2], int reverse, int* face)
+{
+    CALL(mpo_validate_quad(cell));
+    for (int f = 0; f < 4; ++f)
+        if (same_point(cell[f], edge[reverse ? 1 : 0]) && same_point(cell[(f + 1) % 4], edge[reverse ? 0 : 1])) {
+            *face = f;
+            return MPO_OK;
+        }
+    return fail(MPO_ERR_ARG, "Edge must match the cell vertices with the required orientation");
+}
+
+/* RayDistance, src/mlweno/diffusiveflux.cpp:42-74: distance along a unit ray
+ * from a point of face `face` to the first of the cell's other faces. */
+static int ray_distance(const mpo_point cell[4], int face, mpo_point o, mpo_point dir, double* out)
+{
+    long double exit_ = INFINITY;
+    for (int f = 0; f < 4; ++f) {
+        if (f == face) continue;
+        const mpo_point a = cell[f], b = cell[(f + 1) % 4];
+        const long double tx = (long double)b.p[0] - a.p[0], ty = (long double)b.p[1] - a.p[1];
+        const long double rx = (long double)o.p[0] - a.p[0], ry = (long double)o.p[1] - a.p[1];
+        const long double inside = tx * ry - ty * rx;
+        CHECK(isfinite(inside) && inside > 0, MPO_ERR_ARG,
+              "Edge quadrature point must lie strictly between the other cell faces");
+        const long double rate = tx * dir.p[1] - ty * dir.p[0];
+        CHECK(isfinite(rate), MPO_ERR_FP, "Normal-ray intersection overflowed");
+        if (rate < 0) {
+            const long double d = inside / (-rate);
+            if (d < exit_) exit_ = d;
+        }
+    }
+    double work;
+    CALL(real_value(exit_, &work));
+    CHECK(work > 0, MPO_ERR_ARG, "Normal sampling requires positive available distance");
+    *out = work;
+    return MPO_OK;
+}
+
+/* ShiftPoint, src/mlweno/diffusiveflux.cpp:76-86 */
+static int shift_point(mpo_point p, mpo_point n, long double d, mpo_point* out)
+{
+    mpo_point w;
+    for (int k = 0; k < 2; ++k) CALL(real_value((long double)p.p[k] + d * n.p[k], &w.p[k]));
+    *out = w;
+    return MPO_OK;
+}
+
+/* DiffusiveSampling::Build, src/mlweno/diffusiveflux.cpp:256-338.  right ==
+ * NULL builds the boundary form (n samples inside plus the face node). */
+int mpo_diffusive_sampling(const mpo_point edge[2], const mpo_point left[4], const mpo_point* right,
+                           int nq, const double* qp, const double* qw, int n, double extent,
+                           mpo_sampling* out)
+{
+    CHECK(n >= 2 && n % 2 == 0 && n <= MPO_MAX_SAMPLES, MPO_ERR_ARG,
+          "Diffusion requires an even sample count >= 2 with room for a boundary node");
+    CHECK(finite_d(extent) && extent > 0 && extent <= 1, MPO_ERR_ARG,
+          "Normal sampling extentFraction must lie in (0,1]");
+    CHECK(nq >= 1 && nq <= 16, MPO_ERR_ARG, "Edge quadrature needs 1..16 points");
+    mpo_sampling s;
+    memset(&s, 0, sizeof s);
+    CALL(mpo_edge_geometry(edge, &s.length, &s.normal));
+    int lf = 0, rf = 0;
+    CALL(find_face(left, edge, 0, &lf));
+    if (right) CALL(find_face(right, edge, 1, &rf));
+    s.boundary = right == NULL;
+    s.nq = nq;
+    s.count = s.boundary ? n + 1 : n;
+    CALL(mpo_lagrange_weights(s.count, s.boundary ? MPO_LAGRANGE_END_HI : MPO_LAGRANGE_MIDDLE,
+                              s.coefficients));
+    const mpo_point inward = {{-s.normal.p[0], -s.normal.p[1]}};
+    const int m = s.count;
+    for (int q = 0; q < nq; ++q) {
+        CHECK(qp[q] > -1 && qp[q] < 1, MPO_ERR_ARG, "Normal sampling needs open-edge quadrature points");
+        s.points[q] = mpo_map_edge_point(qp[q], edge);
+        double available, other;
+        CALL(ray_distance(left, lf, s.points[q], inward, &available));
+        if (right) {
+            CALL(ray_distance(right, rf, s.points[q], s.normal, &other));
+            if (other < available) available = other;
+        }
+        const long double spacing = (long double)extent * available *
+                                    (s.boundary ? 1.0L / n : 2.0L / (n - 1));
+        CALL(real_value(spacing, &s.spacing[q]));
+        CHECK(s.spacing[q] > 0, MPO_ERR_FP, "Normal sample spacing underflowed");
+        CALL(real_value((long double)s.length * qw[q] / 2, &s.weights[q]));
+        CHECK(s.weights[q] > 0, MPO_ERR_FP, "Physical edge quadrature weight underflowed");
+        for (int k = 0; k < m; ++k) {
+            const long double offset = s.boundary ? (long double)k - n : (long double)k - (n - 1) / 2.0L;
+            const int a = q * m + k;
+            CALL(shift_point(s.points[q], s.normal, offset * s.spacing[q], &s.samples[a]));
+            s.sides[a] = s.boundary ? (k + 1 == m ? MPO_SIDE_BOUNDARY : MPO_SIDE_LEFT)
+                                    : (k < m / 2 ? MPO_SIDE_LEFT : MPO_SIDE_RIGHT);
+            long double signed_distance = 0;
+            for (int d = 0; d < 2; ++d)
+                signed_distance += ((long double)s.samples[a].p[d] - s.points[q].p[d]) * s.normal.p[d];
+            CHECK((offset < 0 && signed_distance < 0) || (offset > 0 && signed_distance > 0) ||
+                  (offset == 0 && signed_distance == 0), MPO_ERR_FP,
+                  "Coordinates cannot resolve the requested normal sample offset");
+            if (k > 0) {
+                long double step = 0;
+                for (int d = 0; d < 2; ++d)
+                    step += ((long double)s.samples[a].p[d] - s.samples[a - 1].p[d]) * s.normal.p[d];
+                CHECK(step > 0, MPO_ERR_FP, "Coordinates cannot resolve distinct normal samples");
+            }
+        }
+    }
+    *out = s;
+    return MPO_OK;
+}
+
+/* IntegrateImpl, value path with the identity diffused quantity,
+ * src/mlweno/diffusiveflux.cpp:102-200 and :364-372.  bc_type/bc_value are
+ * per quadrature point for a boundary sampling (NULL for interior);
+ * samples[q*count+k] ignores the boundary slot. */
+int mpo_diffusive_flux(const mpo_sampling* s, const double* samples, const double* kappa,
+                       const int* bc_type, const double* bc_value, double* flux)
+{
+    CHECK(s->boundary == (bc_type != NULL), MPO_ERR_ARG,
+          "Use the matching interior or boundary flux function");
+    const int m = s->count;
+    double values[MPO_MAX_SAMPLES + 1];
+    long double integral = 0;
+    for (int q = 0; q < s->nq; ++q) {
+        const double weight = s->weights[q];
+        const int type = bc_type ? bc_type[q] : MPO_DIFF_STATE;
+        if (bc_type) {
+            CHECK(type >= MPO_DIFF_STATE && type <= MPO_DIFF_ZERO_FLUX, MPO_ERR_ARG,
+                  "Unknown diffusion boundary condition");
+            if (type == MPO_DIFF_ZERO_FLUX) continue;
+            CHECK(finite_d(bc_value[q]), MPO_ERR_ARG, "Boundary data must be finite");
+            if (type == MPO_DIFF_FLUX) {
+                integral += (long double)weight * bc_value[q];
+                continue;
+            }
+        }
+        CHECK(finite_d(kappa[q]) && kappa[q] >= 0, MPO_ERR_ARG,
+              "Diffusivity must be finite and nonnegative");
+        for (int k = 0; k < m; ++k) {
+            const double u = (bc_type && k + 1 == m) ? bc_value[q] : samples[q * m + k];
+            CHECK(finite_d(u), MPO_ERR_ARG, "Diffusion sample state must be finite");
+            values[k] = u;
+        }
+        long double gradient = 0;
+        for (int k = 0; k + 1 < m; ++k)
+            gradient += (long double)s->coefficients[k] * ((long double)values[k] - values[m - 1]);
+        gradient /= s->spacing[q];
+        CHECK(isfinite(gradient), MPO_ERR_FP, "Sampled normal derivative overflowed");
+        integral -= (long double)weight * kappa[q] * gradient;
+    }
+    return real_value(integral, flux);
+}
+
 /* =================================================================== phase */
 
 void mpo_material_defaults(mpo_material* p)
@@ -1145,6 +1354,10 @@ struct mpo_transport {
     double* traces;                   /* [cell][4][q] */
     double* fv;                       /* vertical face flux  [ny][nx+1] (+x) */
     double* fh;                       /* horizontal face flux [ny+1][nx] (+y) */
+    /* bound limiter (off unless set): cell rule, diffusion sampling, C bound */
+    int limit, lim_cell_points, lim_samples;
+    double lim_extent, lim_cupper;
+    double* theta[2];                 /* last limiter factors per field [cell] */
 };
 
 static size_t cidx(const mpo_transport* t, int i, int j) { return (size_t)j * t->nx + i; }
@@ -1154,7 +1367,7 @@ void mpo_transport_destroy(mpo_transport* t)
     if (!t) return;
     free(t->coords); free(t->stencils); free(t->area); free(t->vnx); free(t->vny);
     for (int s = 0; s < 4; ++s) { free(t->bc[s]); free(t->inflow[0][s]); free(t->inflow[1][s]); }
-    free(t->traces); free(t->fv); free(t->fh);
+    free(t->traces); free(t->fv); free(t->fh); free(t->theta[0]); free(t->theta[1]);
     free(t);
 }
 
@@ -1193,7 +1406,10 @@ int mpo_transport_create(mpo_transport** out, const mpo_mesh* mesh, int q,
     t->traces = (double*)malloc(N * 4 * q * sizeof(double));
     t->fv = (double*)malloc((size_t)t->ny * (t->nx + 1) * sizeof(double));
     t->fh = (double*)malloc((size_t)(t->ny + 1) * t->nx * sizeof(double));
-    if (!t->coords || !t->area || !t->vnx || !t->vny || !t->traces || !t->fv || !t->fh) {
+    t->theta[0] = (double*)malloc(N * sizeof(double));
+    t->theta[1] = (double*)malloc(N * sizeof(double));
+    if (!t->coords || !t->area || !t->vnx || !t->vny || !t->traces || !t->fv || !t->fh ||
+        !t->theta[0] || !t->theta[1]) {
         mpo_transport_destroy(t); return fail(MPO_ERR_MEM, "out of memory");
     }
     memcpy(t->coords, mesh->coords, nv * sizeof(double));
@@ -1286,8 +1502,84 @@ static void canonical_face(const mpo_transport* t, int i, int j, int f, mpo_poin
     }
 }
 
-/* StateReconstruction::Update + trace evaluation, src/couple/state_reconstruction.cpp:101-126 */
-static int field_traces(mpo_transport* t, const double* u, double* sigma, double* omega)
+int mpo_transport_set_limiter(mpo_transport* t, int enable, int cell_points, int samples,
+                              double extent, double c_upper)
+{
+    if (!enable) { t->limit = 0; return MPO_OK; }
+    CHECK(cell_points >= 1 && cell_points <= 16, MPO_ERR_ARG, "Cell quadrature needs 1..16 points per axis");
+    CHECK(samples >= 2 && samples % 2 == 0 && samples <= MPO_MAX_SAMPLES, MPO_ERR_ARG,
+          "Diffusion requires an even sample count >= 2 with room for a boundary node");
+    CHECK(finite_d(extent) && extent > 0 && extent <= 1, MPO_ERR_ARG,
+          "Normal sampling extentFraction must lie in (0,1]");
+    CHECK(finite_d(c_upper) && c_upper > 0, MPO_ERR_ARG, "Composition bound must be positive");
+    t->limit = 1; t->lim_cell_points = cell_points; t->lim_samples = samples;
+    t->lim_extent = extent; t->lim_cupper = c_upper;
+    return MPO_OK;
+}
+
+/* StateReconstruction::Limit, src/couple/state_reconstruction.cpp:136-192, for
+ * one field of one cell (upper = +inf for H, Xe for C): the largest factor
+ * theta <= 1 such that mean + theta*(R(p) - mean) stays within the 3x3
+ * neighbourhood's bounds, clamped to [0, upper], at the cell centre, the cell
+ * Gauss points, the edge Gauss points and the in-cell normal-line diffusion
+ * samples of every edge; rounded inward by 1e-13. */
+static int cell_limit(const mpo_transport* t, const mpo_recon* r, const double* u, int i, int j,
+                      double upper, double* theta_out)
+{
+    const int nx = t->nx, ny = t->ny;
+    const double mean = u[cidx(t, i, j)];
+    CHECK(finite_d(mean) && mean >= 0 && mean <= upper, MPO_ERR_ARG,
+          "Inadmissible cell average in coupled reconstruction");
+    double lo = mean, hi = mean;
+    for (int y = j > 0 ? j - 1 : 0; y < (j + 2 < ny ? j + 2 : ny); ++y)
+        for (int x = i > 0 ? i - 1 : 0; x < (i + 2 < nx ? i + 2 : nx); ++x) {
+            const double v = u[cidx(t, x, y)];
+            lo = fmin(lo, v); hi = fmax(hi, v);
+        }
+    lo = fmax(0.0, lo);
+    hi = fmin(upper, hi);
+    rule1d cell;
+    CALL(make_rule(t->lim_cell_points, &cell));
+    mpo_point c[4];
+    mpo_cell_corners(&t->mesh, i, j, c);
+    double theta = 1.0;
+#define LIMIT_AT(P) do { \
+        double v_; CALL(mpo_recon_eval(r, (P), 0, 0, &v_)); \
+        CHECK(finite_d(v_), MPO_ERR_FP, "Nonfinite reconstructed state"); \
+        const double d_ = v_ - mean; \
+        if (v_ > hi) theta = fmin(theta, (hi - mean) / d_); \
+        if (v_ < lo) theta = fmin(theta, (lo - mean) / d_); \
+    } while (0)
+    const mpo_point centre = {{0.0, 0.0}};
+    LIMIT_AT(mpo_map_cell_point(centre, c));
+    for (int a = 0; a < cell.n; ++a)
+        for (int b = 0; b < cell.n; ++b) {
+            const mpo_point ref = {{cell.x[a], cell.x[b]}};
+            LIMIT_AT(mpo_map_cell_point(ref, c));
+        }
+    /* neighbour across edge e (e = 0 bottom, 1 right, 2 top, 3 left) */
+    const int ni[4] = {i, i + 1, i, i - 1}, nj[4] = {j - 1, j, j + 1, j};
+    for (int e = 0; e < 4; ++e) {
+        const mpo_point edge[2] = {c[e], c[(e + 1) % 4]};
+        for (int q = 0; q < t->q; ++q) LIMIT_AT(mpo_map_edge_point(t->qp[q], edge));
+        const int inside = ni[e] >= 0 && ni[e] < nx && nj[e] >= 0 && nj[e] < ny;
+        mpo_point other[4];
+        if (inside) mpo_cell_corners(&t->mesh, ni[e], nj[e], other);
+        mpo_sampling s;
+        CALL(mpo_diffusive_sampling(edge, c, inside ? other : NULL, t->q, t->qp, t->qw,
+                                    t->lim_samples, t->lim_extent, &s));
+        for (int a = 0; a < s.nq * s.count; ++a)
+            if (s.sides[a] == MPO_SIDE_LEFT) LIMIT_AT(s.samples[a]);
+    }
+#undef LIMIT_AT
+    *theta_out = theta < 1 ? fmax(0.0, theta * (1 - 1e-13)) : 1.0;
+    return MPO_OK;
+}
+
+/* StateReconstruction::Update + trace evaluation,
+ * src/couple/state_reconstruction.cpp:101-134: a trace is
+ * mean + theta*(R(p) - mean), theta = 1 unless the limiter is on. */
+static int field_traces(mpo_transport* t, int field, const double* u, double* sigma, double* omega)
 {
     const size_t N = (size_t)t->nx * t->ny;
     int err = 0;
@@ -1300,12 +1592,18 @@ static int field_traces(mpo_transport* t, const double* u, double* sigma, double
         mpo_recon r;
         int e = cell_recon(t, i, j, &r);
         if (!e) e = mpo_recon_update(&r, u, 0, 0, t->nx, t->ny, t->area[c]);
+        double theta = 1.0;
+        if (!e && t->limit)
+            e = cell_limit(t, &r, u, i, j, field == 0 ? INFINITY : t->lim_cupper, &theta);
+        t->theta[field][c] = theta;
         for (int f = 0; f < 4 && !e; ++f) {
             mpo_point edge[2];
             canonical_face(t, i, j, f, edge);
-            for (int q = 0; q < t->q && !e; ++q)
-                e = mpo_recon_eval(&r, mpo_map_edge_point(t->qp[q], edge), 0, 0,
-                                   &t->traces[(c * 4 + f) * t->q + q]);
+            for (int q = 0; q < t->q && !e; ++q) {
+                double v;
+                e = mpo_recon_eval(&r, mpo_map_edge_point(t->qp[q], edge), 0, 0, &v);
+                t->traces[(c * 4 + f) * t->q + q] = u[c] + theta * (v - u[c]);
+            }
         }
         if (!e && sigma) {
             for (int k = 0; k < 5; ++k) { sigma[c * 5 + k] = 0; omega[c * 5 + k] = 0; }
@@ -1324,10 +1622,29 @@ static int field_traces(mpo_transport* t, const double* u, double* sigma, double
     return MPO_OK;
 }
 
+int mpo_transport_limits(mpo_transport* t, const double* H, const double* C, double* thetaH,
+                         double* thetaC)
+{
+    const size_t N = (size_t)t->nx * t->ny;
+    CALL(field_traces(t, 0, H, NULL, NULL));
+    memcpy(thetaH, t->theta[0], N * sizeof(double));
+    CALL(field_traces(t, 1, C, NULL, NULL));
+    memcpy(thetaC, t->theta[1], N * sizeof(double));
+    return MPO_OK;
+}
+
+int mpo_transport_traces(mpo_transport* t, int field, const double* u, double* traces)
+{
+    CHECK(field == 0 || field == 1, MPO_ERR_ARG, "field must be 0 (H) or 1 (C)");
+    CALL(field_traces(t, field, u, NULL, NULL));
+    memcpy(traces, t->traces, (size_t)t->nx * t->ny * 4 * t->q * sizeof(double));
+    return MPO_OK;
+}
+
 int mpo_transport_reconstruct(mpo_transport* t, const double* u, double* sigma, double* omega,
                               double* traces)
 {
-    CALL(field_traces(t, u, sigma, omega));
+    CALL(field_traces(t, 0, u, sigma, omega));
     if (traces) memcpy(traces, t->traces, (size_t)t->nx * t->ny * 4 * t->q * sizeof(double));
     return MPO_OK;
 }
@@ -1485,10 +1802,10 @@ int mpo_transport_rhs(mpo_transport* t, const double* H, const double* C, double
                       double* rhsC, double* bfluxH, double* bfluxC)
 {
     double bh = 0, bc = 0;
-    CALL(field_traces(t, H, NULL, NULL));
+    CALL(field_traces(t, 0, H, NULL, NULL));
     CALL(face_fluxes(t, 0));
     gather_rhs(t, rhsH, &bh);
-    CALL(field_traces(t, C, NULL, NULL));
+    CALL(field_traces(t, 1, C, NULL, NULL));
     CALL(face_fluxes(t, 1));
     gather_rhs(t, rhsC, &bc);
     if (bfluxH) *bfluxH = bh;

@@ -200,6 +200,34 @@ int mpo_phase_batch(const mpo_phase_model* m, int64_t n, const double* H, const 
                     double P, int hyper, double* T, double* phi, double* Cl, double* Cs,
                     uint8_t* region);
 
+/* ---------------------------------------------------------- diffusive flux */
+#define MPO_MAX_SAMPLES 8
+enum { MPO_LAGRANGE_MIDDLE = 0, MPO_LAGRANGE_END_LO = 1, MPO_LAGRANGE_END_HI = 2 };
+enum { MPO_SIDE_LEFT = 0, MPO_SIDE_RIGHT = 1, MPO_SIDE_BOUNDARY = 2 };
+/* DiffusiveBoundaryType order of src/mlweno/diffusiveflux.h */
+enum { MPO_DIFF_STATE = 0, MPO_DIFF_QUANTITY = 1, MPO_DIFF_FLUX = 2, MPO_DIFF_ZERO_FLUX = 3 };
+
+typedef struct {
+    int nq, count, boundary;
+    double length;
+    mpo_point normal;
+    mpo_point points[16];                       /* edge quadrature points */
+    double spacing[16], weights[16];            /* physical ds, W_q = w_q*length/2 */
+    double coefficients[MPO_MAX_SAMPLES + 1];   /* dimensionless derivative weights */
+    mpo_point samples[16 * (MPO_MAX_SAMPLES + 1)];   /* [q][k] */
+    int sides[16 * (MPO_MAX_SAMPLES + 1)];
+} mpo_sampling;
+
+/* src/mlweno/diffusiveflux.cpp:203-254 */
+int mpo_lagrange_weights(int npts, int location, double* w);
+/* src/mlweno/diffusiveflux.cpp:256-358 (right == NULL: boundary sampling) */
+int mpo_diffusive_sampling(const mpo_point edge[2], const mpo_point left[4], const mpo_point* right,
+                           int nq, const double* qp, const double* qw, int n, double extent,
+                           mpo_sampling* out);
+/* src/mlweno/diffusiveflux.cpp:102-200, identity diffused quantity */
+int mpo_diffusive_flux(const mpo_sampling* s, const double* samples, const double* kappa,
+                       const int* bc_type, const double* bc_value, double* flux);
+
 /* -------------------------------------------------------- transport operator */
 enum { MPO_BC_ZERO = 0, MPO_BC_OUTFLOW = 1, MPO_BC_INFLOW_OUTFLOW = 2 };
 
@@ -229,6 +257,17 @@ int mpo_transport_rhs(mpo_transport* t, const double* H, const double* C, double
  * points [N][4][q] in face order bottom,right,top,left, Gauss order along +x/+y. */
 int mpo_transport_reconstruct(mpo_transport* t, const double* u, double* sigma,
                               double* omega, double* traces);
+/* Bound limiter of the coupled reconstruction (src/couple/state_reconstruction.cpp:136-192):
+ * cell_points Gauss points per axis, `samples` normal-line diffusion samples
+ * per edge point with extent fraction `extent`, composition bounded by c_upper
+ * (Xe).  enable = 0 switches it off (the default). */
+int mpo_transport_set_limiter(mpo_transport* t, int enable, int cell_points, int samples,
+                              double extent, double c_upper);
+/* Traces of one field (0 H, 1 C: the limiter bounds C by c_upper) [N][4][q]. */
+int mpo_transport_traces(mpo_transport* t, int field, const double* u, double* traces);
+/* Limiter factors theta of both fields for one state [N] (1 with the limiter off). */
+int mpo_transport_limits(mpo_transport* t, const double* H, const double* C, double* thetaH,
+                         double* thetaC);
 /* Advance nsteps with the given scheme (1 forward Euler, 2 SSPRK2, 3 SSPRK3)
  * at fixed dt.  totals[4*s+{0,1,2,3}] = sum A*H, sum A*C, step boundary flux H,
  * C (stage-weighted), for each step s (may be NULL). */

@@ -13,7 +13,14 @@ __all__ = [
     "lib", "OracleError", "build", "Material", "PhaseModel", "Transport", "gauss_rule",
     "ref_phase_available", "ref_phase_evaluate", "ref_phase_derived", "WenoOptions",
     "StencilPoly", "Recon", "llf_point", "integrate_edge_flux", "REGION",
+    "lagrange_weights", "Sampling", "diffusive_sampling", "diffusive_flux", "LAGRANGE",
+    "SIDE", "DIFF_BC", "MAX_SAMPLES",
 ]
+
+LAGRANGE = {"Middle": 0, "EndLo": 1, "EndHi": 2}
+SIDE = {"Left": 0, "Right": 1, "Boundary": 2}
+DIFF_BC = {"PrescribedState": 0, "PrescribedQuantity": 1, "PrescribedFlux": 2, "ZeroFlux": 3}
+MAX_SAMPLES = 8
 
 REGION = {"PureSolid": 1, "TwoSolids": 2, "Eutectic": 3, "SolidLiquid": 4, "Liquid": 5,
           "PureMelting": 6}
@@ -150,6 +157,26 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
+class Sampling(C.Structure):
+    """Normal-line diffusion sampling of one edge (mpo_sampling)."""
+    _fields_ = [("nq", C.c_int), ("count", C.c_int), ("boundary", C.c_int),
+                ("length", C.c_double), ("normal", Point), ("points", Point * 16),
+                ("spacing", C.c_double * 16), ("weights", C.c_double * 16),
+                ("coefficients", C.c_double * (MAX_SAMPLES + 1)),
+                ("samples", Point * (16 * (MAX_SAMPLES + 1))),
+                ("sides", C.c_int * (16 * (MAX_SAMPLES + 1)))]
+
+    def sample_points(self):
+        n = self.nq * self.count
+        return np.array([[self.samples[a].p[0], self.samples[a].p[1]] for a in range(n)])
+
+    def sample_sides(self):
+        return np.array([self.sides[a] for a in range(self.nq * self.count)])
+
+    def quadrature_points(self):
+        return np.array([[self.points[q].p[0], self.points[q].p[1]] for q in range(self.nq)])
+
+
 def _declare(L):
     L.mpo_last_error.restype = C.c_char_p
     L.mpo_gauss_rule.argtypes = [C.c_int, _dp, _dp]
@@ -197,6 +224,15 @@ def _declare(L):
                                         C.c_void_p]
     L.mpo_transport_integral.argtypes = [C.c_void_p, _dp]
     L.mpo_transport_integral.restype = C.c_double
+    L.mpo_lagrange_weights.argtypes = [C.c_int, C.c_int, _dp]
+    L.mpo_diffusive_sampling.argtypes = [C.POINTER(Point), C.POINTER(Point), C.c_void_p, C.c_int,
+                                         _dp, _dp, C.c_int, C.c_double, C.POINTER(Sampling)]
+    L.mpo_diffusive_flux.argtypes = [C.POINTER(Sampling), _dp, _dp, C.c_void_p, C.c_void_p,
+                                     C.POINTER(C.c_double)]
+    L.mpo_transport_set_limiter.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
+                                            C.c_double]
+    L.mpo_transport_limits.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    L.mpo_transport_traces.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.mpo_mix64.argtypes = [C.c_uint64]
     L.mpo_mix64.restype = C.c_uint64
 
@@ -412,6 +448,50 @@ def integrate_edge_flux(edge, velocity, left, right, nq=5, mode=0, global_speed=
     return f.value
 
 
+def lagrange_weights(npts, location="Middle"):
+    """First-derivative weights on nodes 0..npts-1 (src/mlweno/diffusiveflux.cpp:203-254)."""
+    w = np.zeros(npts)
+    _check(lib().mpo_lagrange_weights(npts, LAGRANGE[location], w))
+    return w
+
+
+def _points(pts):
+    arr = (Point * len(pts))()
+    for k, p in enumerate(pts):
+        arr[k].p[0], arr[k].p[1] = float(p[0]), float(p[1])
+    return arr
+
+
+def diffusive_sampling(edge, left, right=None, rule=5, n=4, extent=0.9):
+    """DiffusiveSampling::Build (src/mlweno/diffusiveflux.cpp:256-358); ``rule`` is a
+    Gauss point count or a (points, weights) pair; right=None builds the boundary form."""
+    qp, qw = gauss_rule(rule) if np.isscalar(rule) else (np.asarray(rule[0], float),
+                                                          np.asarray(rule[1], float))
+    s = Sampling()
+    rarr = _points(right) if right is not None else None
+    _check(lib().mpo_diffusive_sampling(_points(edge), _points(left),
+                                        C.cast(rarr, C.c_void_p) if rarr is not None else None,
+                                        len(qp), np.ascontiguousarray(qp),
+                                        np.ascontiguousarray(qw), n, extent, C.byref(s)))
+    return s
+
+
+def diffusive_flux(s, samples, kappa, bc_type=None, bc_value=None):
+    """IntegrateDiffusiveFlux / IntegrateDiffusiveBoundaryFlux with the identity quantity."""
+    out = C.c_double()
+    samples = np.ascontiguousarray(samples, dtype=np.float64)
+    kappa = np.ascontiguousarray(kappa, dtype=np.float64)
+    bt = bv = None
+    if bc_type is not None:
+        bt = np.ascontiguousarray(bc_type, dtype=np.int32)
+        bv = np.ascontiguousarray(bc_value, dtype=np.float64)
+    _check(lib().mpo_diffusive_flux(C.byref(s), samples, kappa,
+                                    bt.ctypes.data_as(C.c_void_p) if bt is not None else None,
+                                    bv.ctypes.data_as(C.c_void_p) if bv is not None else None,
+                                    C.byref(out)))
+    return out.value
+
+
 def _u8_or_none(a):
     if a is None:
         return None, None
@@ -501,6 +581,25 @@ class Transport:
                                            totals.ctypes.data_as(C.c_void_p)))
         return (H.reshape(self.ny, self.nx), Cc.reshape(self.ny, self.nx),
                 totals.reshape(-1, 4)[:nsteps])
+
+    def set_limiter(self, enable=True, cell_points=5, samples=4, extent=0.9, c_upper=0.4):
+        """Bound limiter of src/couple/state_reconstruction.cpp:136-192 on every trace."""
+        _check(lib().mpo_transport_set_limiter(self._h, int(enable), cell_points, samples,
+                                               extent, c_upper))
+
+    def traces(self, u, field=0):
+        tr = np.zeros(self.N * 4 * self.q)
+        _check(lib().mpo_transport_traces(self._h, field,
+                                          np.ascontiguousarray(u, dtype=np.float64).ravel(), tr))
+        return tr.reshape(self.ny, self.nx, 4, self.q)
+
+    def limits(self, H, Cc):
+        tH = np.zeros(self.N)
+        tC = np.zeros(self.N)
+        _check(lib().mpo_transport_limits(self._h, np.ascontiguousarray(H, dtype=np.float64).ravel(),
+                                          np.ascontiguousarray(Cc, dtype=np.float64).ravel(),
+                                          tH, tC))
+        return tH.reshape(self.ny, self.nx), tC.reshape(self.ny, self.nx)
 
     def integral(self, u):
         return lib().mpo_transport_integral(self._h,

@@ -1,0 +1,191 @@
+"""Oracle pins for normal-line diffusion (CPU only).
+
+The cases restate the reference's own checks of src/mlweno/diffusiveflux.cpp
+in src/mlweno/test/test_diffusiveflux.cpp: Lagrange derivative weights
+(two- and four-point values, polynomial moments, :72-101), interior and
+boundary sampling geometry and fluxes on rectangles and skewed quads
+(:103-167), physics and argument failures (:230-315) and the normal-sampling
+convergence order (:317-345).  Tolerances are the reference's own
+(Near: |a-b| <= tol*(1+|a|+|b|)).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+ONE = ([0.0], [2.0])
+THREE = ([-math.sqrt(3 / 5), 0.0, math.sqrt(3 / 5)], [5 / 9, 8 / 9, 5 / 9])
+FIVE = O.gauss_rule(5)
+RECT_L = [[-1, -.5], [0, -.5], [0, .5], [-1, .5]]
+RECT_R = [[0, -.5], [1.5, -.5], [1.5, .5], [0, .5]]
+QUAD_L = [[-1.3, -.8], [.1, -.6], [.3, 1], [-1.1, .7]]
+QUAD_R = [[.1, -.6], [1.2, -.9], [1.4, .8], [.3, 1]]
+
+
+def near(a, b, tol=3e-10):
+    return math.isfinite(a) and math.isfinite(b) and abs(a - b) <= tol * (1 + abs(a) + abs(b))
+
+
+def poly(p):
+    x, y = p
+    return 2 + .7 * x - .3 * y + .2 * x * x + .4 * x * y + .5 * y * y
+
+
+def poly_grad(p):
+    return np.array([.7 + .4 * p[0] + .4 * p[1], -.3 + .4 * p[0] + p[1]])
+
+
+def linear(p):
+    return 2 + .2 * p[0] - .3 * p[1]
+
+
+def exact(s, grad):
+    n = np.array([s.normal.p[0], s.normal.p[1]])
+    return -sum(s.weights[q] * grad(s.quadrature_points()[q]) @ n for q in range(s.nq))
+
+
+def inside(cell, p):
+    for j in range(4):
+        a, b = cell[j], cell[(j + 1) % 4]
+        x, y = b[0] - a[0], b[1] - a[1]
+        xx, yy = p[0] - a[0], p[1] - a[1]
+        assert x * yy - y * xx >= -2e-14 * (abs(x * yy) + abs(y * xx)), "sample stays in its cell"
+
+
+def test_lagrange_weights():
+    assert O.lagrange_weights(2, "Middle").tolist() == [-1.0, 1.0]
+    w4 = O.lagrange_weights(4, "Middle")
+    assert near(w4[0], 1 / 24) and near(w4[1], -9 / 8)
+    for npts in range(2, O.MAX_SAMPLES + 2):
+        for loc in ("Middle", "EndLo", "EndHi"):
+            if loc == "Middle" and npts % 2:
+                continue
+            w = O.lagrange_weights(npts, loc)
+            n = npts - 1
+            x = n / 2 if loc == "Middle" else (0 if loc == "EndLo" else n)
+            for p in range(n + 1):
+                terms = [w[i] * ((i - x) / n) ** p for i in range(npts)]
+                target = 1 / n if p == 1 else 0
+                assert abs(sum(terms) - target) < 3e-13 * (1 + sum(abs(t) for t in terms))
+    for bad in ((3, "Middle"), (1, "EndHi")):
+        with pytest.raises(O.OracleError):
+            O.lagrange_weights(*bad)
+
+
+@pytest.mark.parametrize("mesh", range(4))
+def test_interior_geometry_and_fluxes(mesh):
+    left = [list(p) for p in (QUAD_L if mesh % 2 else RECT_L)]
+    right = [list(p) for p in (QUAD_R if mesh % 2 else RECT_R)]
+    if mesh >= 2:
+        for cell in (left, right):
+            for p in cell:
+                p[0] *= 1e-6
+    edge = [left[1], left[2]]
+    tol = 3e-11 if mesh < 2 else 2e-8
+    for n in (2, 4, 6, 8):
+        for rule in (ONE, THREE, FIVE):
+            s = O.diffusive_sampling(edge, left, right, rule, n, .9)
+            r = O.diffusive_sampling([edge[1], edge[0]], right, left, rule, n, .9)
+            assert s.count == n and not s.boundary
+            pts, sides = s.sample_points(), s.sample_sides()
+            for a in range(len(pts)):
+                is_left = a % n < n // 2
+                assert sides[a] == (O.SIDE["Left"] if is_left else O.SIDE["Right"])
+                inside(left if is_left else right, pts[a])
+            kappa = np.ones(s.nq)
+            u = np.array([poly(p) for p in pts])
+            ru = np.array([poly(p) for p in r.sample_points()])
+            f, rf = O.diffusive_flux(s, u, kappa), O.diffusive_flux(r, ru, kappa)
+            assert near(f, exact(s, poly_grad), tol), "quadratic physical flux"
+            assert near(f, -rf, tol), "orientation reversal conservation"
+            assert O.diffusive_flux(s, np.full(len(u), 1e7), kappa) == 0.0
+            jump = np.where(sides == O.SIDE["Left"], 1.0, 3.0)
+            assert O.diffusive_flux(s, jump, kappa) < 0
+
+
+def test_boundary_strips():
+    for scale in ((1, 1), (1e-5, 2), (2, 1e-5)):
+        cell = [[p[0] * scale[0], p[1] * scale[1]] for p in QUAD_L]
+        for side in range(4):
+            for n in (2, 4, 6):
+                s = O.diffusive_sampling([cell[side], cell[(side + 1) % 4]], cell, None, THREE, n, .9)
+                assert s.count == n + 1 and s.boundary
+                pts, sides = s.sample_points(), s.sample_sides()
+                u = np.array([poly(p) for p in pts])
+                qp = s.quadrature_points()
+                for q in range(3):
+                    assert sides[q * (n + 1) + n] == O.SIDE["Boundary"]
+                    u[q * (n + 1) + n] = np.nan          # deliberately ignored
+                for a in range(len(u)):
+                    if sides[a] != O.SIDE["Boundary"]:
+                        inside(cell, pts[a])
+                f = O.diffusive_flux(s, u, np.ones(3), [0, 0, 0], [poly(p) for p in qp])
+                assert near(f, exact(s, poly_grad), 2e-8), "Dirichlet quadratic flux on all faces"
+
+
+def test_physics_and_failures():
+    edge = [QUAD_L[1], QUAD_L[2]]
+    s = O.diffusive_sampling(edge, QUAD_L, None, THREE)
+    u = np.array([linear(p) for p in s.sample_points()])
+    qp = s.quadrature_points()
+    state = O.diffusive_flux(s, u, np.ones(3), [0] * 3, [linear(p) for p in qp])
+    quantity = O.diffusive_flux(s, u, np.ones(3), [1] * 3, [linear(p) for p in qp])
+    assert near(state, quantity), "state and quantity Dirichlet agree"
+    n = np.array([s.normal.p[0], s.normal.p[1]])
+    assert near(state, -s.length * (np.array([.2, -.3]) @ n))
+    f = O.diffusive_flux(s, np.full(len(u), np.nan), np.full(3, np.nan), [2] * 3, [-.7] * 3)
+    assert near(f, -.7 * s.length), "Neumann data are physical outward flux"
+    assert O.diffusive_flux(s, u, np.ones(3), [3] * 3, [np.nan] * 3) == 0.0
+    interior = O.diffusive_sampling(edge, QUAD_L, QUAD_R, THREE)
+    ui = np.array([linear(p) for p in interior.sample_points()])
+    assert O.diffusive_flux(interior, ui, np.zeros(3)) == 0.0
+    bad = [
+        lambda: O.diffusive_sampling(edge, QUAD_L, QUAD_R, THREE, 3, .9),
+        lambda: O.diffusive_sampling(edge, QUAD_R, QUAD_L, THREE),
+        lambda: O.diffusive_sampling([edge[1], edge[0]], QUAD_L, None, THREE),
+        lambda: O.diffusive_sampling(edge, QUAD_L, QUAD_R, THREE, 4, 0.0),
+        lambda: O.diffusive_sampling(edge, QUAD_L, QUAD_R, THREE, 4, 1.1),
+        lambda: O.diffusive_sampling(edge, QUAD_L, QUAD_R, ([-1, 1], [1, 1])),
+        lambda: O.diffusive_flux(interior, ui, [1, -1, 1]),
+        lambda: O.diffusive_flux(interior, np.r_[np.nan, ui[1:]], np.ones(3)),
+        lambda: O.diffusive_flux(s, u, np.ones(3)),
+        lambda: O.diffusive_flux(interior, ui, np.ones(3), [0] * 3, [0.0] * 3),
+    ]
+    for call in bad:
+        with pytest.raises(O.OracleError):
+            call()
+
+
+def sine(p):
+    return math.sin(1.2 * p[0] + .7 * p[1])
+
+
+def sine_integral(edge, s):
+    mid = .6 * (edge[0][0] + edge[1][0]) + .35 * (edge[0][1] + edge[1][1])
+    half = .6 * (edge[1][0] - edge[0][0]) + .35 * (edge[1][1] - edge[0][1])
+    n = (s.normal.p[0], s.normal.p[1])
+    return -s.length * (1.2 * n[0] + .7 * n[1]) * math.cos(mid) * (1 if half == 0 else math.sin(half) / half)
+
+
+@pytest.mark.parametrize("quad", [False, True])
+@pytest.mark.parametrize("boundary", [False, True])
+def test_convergence_order(quad, boundary):
+    for n in (2, 4, 6):
+        previous, minimum = 0.0, 100.0
+        for h in (.8, .4, .2, .1):
+            left = [[.31 + h * p[0], .27 + h * p[1]] for p in (QUAD_L if quad else RECT_L)]
+            right = [[.31 + h * p[0], .27 + h * p[1]] for p in (QUAD_R if quad else RECT_R)]
+            edge = [left[1], left[2]]
+            s = O.diffusive_sampling(edge, left, None if boundary else right, FIVE, n, .9)
+            u = np.array([sine(p) for p in s.sample_points()])
+            if boundary:
+                f = O.diffusive_flux(s, u, np.ones(5), [0] * 5, [sine(p) for p in s.quadrature_points()])
+            else:
+                f = O.diffusive_flux(s, u, np.ones(5))
+            err = abs(f - sine_integral(edge, s)) / s.length
+            if previous > 0:
+                minimum = min(minimum, math.log2(previous / err))
+            previous = err
+        assert minimum > n - .45, (n, minimum)
