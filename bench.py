@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--serve-rate", type=float, default=300.0, help="Poisson arrivals per second")
     p.add_argument("--full-steps", type=int, default=1000,
                    help="steps of the 8192x8192 full-chip run (0: skip it)")
+    p.add_argument("--limiter-steps", type=int, default=20,
+                   help="steps of the FP64 bound-limiter leg (0: skip)")
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--tune-cache", default=os.path.join(HERE, "gpurun_out", "batch_tune.json"))
@@ -366,6 +368,42 @@ def bench_full_chip(args):
     return out
 
 
+def bench_limiter(args, spec, H, Cc, vx, vy, phase):
+    """FP64 sweep with the bound limiter of the coupled reconstruction on every
+    stage, beside the same sweep without it, on the single-mesh workload."""
+    import mantlepar_amd as M
+    t = M.Transport(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q,
+                    phase=phase, precision=M.PREC_FP64)
+    t.set_face_velocity(vx, vy)
+    dt = t.stable_dt(0.4)
+    t.set_tile(16)
+    out = {}
+    for on in (False, True):
+        t.set_limiter(on)
+        t.upload(H, Cc)
+        t.step(2, dt)
+        t.upload(H, Cc)
+        t.synchronize()
+        t0 = time.perf_counter()
+        t.step(args.limiter_steps, dt)
+        el = time.perf_counter() - t0
+        t.upload(H, Cc)
+        prof = t.profile(min(args.limiter_steps, 10), dt)
+        out["limited" if on else "unlimited"] = {
+            "cell_steps_per_s": spec.cells * args.limiter_steps / el,
+            "ms_stage": prof["ms_stage"], "ms_phase_stage": prof["ms_phase_stage"]}
+    t.upload(H, Cc)
+    tH, tC = t.limits()
+    t.close()
+    out["cells_limited_H"] = int(np.count_nonzero(tH < 1))
+    out["cells_limited_C"] = int(np.count_nonzero(tC < 1))
+    out["stage_overhead"] = out["limited"]["ms_stage"] / out["unlimited"]["ms_stage"]
+    out["steps"] = args.limiter_steps
+    out["points_per_cell_field"] = 86
+    log(f"limiter: {out['limited']['cell_steps_per_s']:.3e} cell-steps/s, stage x{out['stage_overhead']:.2f}")
+    return out
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -464,6 +502,8 @@ def main():
                      "ms_per_launch": prof["ms_stage"], "rows_per_item": mixed["rows"]},
         "conservation": {"total_H": mixed["report"]["total_H"], "total_C": mixed["report"]["total_C"]},
     }
+    if args.limiter_steps > 0:
+        out["limiter_fp64"] = bench_limiter(args, spec, H, Cc, vx, vy, phase)
     if args.full_steps > 0:
         out["full_chip"] = bench_full_chip(args)
     if not (args.no_batch and args.no_serve):

@@ -158,6 +158,21 @@ mp_status mp_transport_profile(mp_transport* t, int32_t nsteps, int32_t scheme, 
                                double* ms_total, double* ms_stage, double* ms_phase_stage);
 /* Sweep tile shape: rows per work item (>= 4). */
 mp_status mp_transport_set_tile(mp_transport* t, int32_t rows_per_item);
+/* Bound limiter of every reconstruction the stage kernels build: replaces
+ * StateReconstruction::Limit (src/couple/state_reconstruction.cpp:136-192;
+ * the reference calls it per coupled RHS, src/couple/evolve.cpp:281), with
+ * config.input.quadrature.cellPointsPerAxis, config.sampling.numberOfSamples
+ * and extentFraction (src/mlweno/diffusiveflux.h:30-38; defaults 5, 4, 0.9)
+ * as arguments.  Each cell's factor theta makes mean + theta*(R(p) - mean)
+ * stay within its 3x3 neighbourhood's bounds, clamped to H >= 0 and
+ * 0 <= C <= Xe (C <= 1 with the hypereutectic mirror).  FP64 precision mode
+ * only; an average outside those ranges fails the step with MP_ERR_ARG
+ * ("Inadmissible cell average in coupled reconstruction").  enable = 0
+ * switches it off (the default). */
+mp_status mp_transport_set_limiter(mp_transport* t, int32_t enable, int32_t cell_points_per_axis,
+                                   int32_t diffusion_samples, double extent_fraction);
+/* Limiter factors theta[N] of H and C for the current state (diagnostics). */
+mp_status mp_transport_limits(mp_transport* t, double* thetaH, double* thetaC);
 
 /* --------------------------------------------------------------- batch --- */
 /* Many independent meshes advanced together.  The reference runs one mesh per

@@ -13,6 +13,7 @@ constexpr int kMaxQ = 8;          // edge Gauss points supported on device
 constexpr int kWave = 64;         // CDNA wavefront
 constexpr int kStripCells = 62;   // owned columns per wave (2 halo lanes)
 constexpr int kPartials = 8;      // doubles per work item in the partials buffer
+constexpr int kMaxLimS = 8;       // limiter: normal-line samples per edge point
 
 // Error word bits (device side); the host maps them to mp_status codes.
 enum : uint32_t {
@@ -23,6 +24,7 @@ enum : uint32_t {
     kErrInflow = 1u << 4,         // inflow without prescribed boundary data
     kErrWeights = 1u << 5,        // no finite active log weight
     kErrPhaseRange = 1u << 6,     // phase boundaries not resolvable
+    kErrInadmissible = 1u << 7,   // limiter: average outside H >= 0, 0 <= C <= Xe
 };
 
 // Binary eutectic parameters as the device uses them.  Tep/Tmp are derived
@@ -81,6 +83,16 @@ struct MeshDev {
     double wv[kMaxQ];           // w_q * dy/2
     double wh[kMaxQ];           // w_q * dx/2
     PhaseDev ph;
+    // Bound limiter of the reconstruction (limit = 0: off).  Point offsets in
+    // the same normalised units as hx, hy: cell Gauss points lim_cx x lim_cy;
+    // in-cell normal-line samples of the right face at x = lim_x*[k] and of the
+    // top face at y = lim_y*[k] (left and bottom faces mirror them), for an
+    // interior edge (lim_ni = n/2 samples) and a boundary edge (lim_nb = n).
+    int32_t limit, lim_ncp, lim_ni, lim_nb;
+    double lim_cupper;          // upper bound of C: Xe (1 with the mirror)
+    double lim_cx[kMaxQ], lim_cy[kMaxQ];
+    double lim_xi[kMaxLimS], lim_xb[kMaxLimS];
+    double lim_yi[kMaxLimS], lim_yb[kMaxLimS];
     // boundary data (per face policy and optional inflow [face][q])
     const uint8_t* bc[4];       // bottom(nx) right(ny) top(nx) left(ny)
     const double* inflowH[4];

@@ -59,6 +59,10 @@ mp_status make_phase(const mp_phase_params& p, PhaseHost& out);
 mp_status validate_mesh(const mp_mesh_desc& m);
 mp_status validate_weno(const mp_weno_options& o);
 // Closed-form tables of the production candidates for one uniform mesh.
+// Bound limiter point tables (src/couple/state_reconstruction.cpp:136-192 with
+// the sampling of src/mlweno/diffusiveflux.cpp:256-338) for a uniform mesh.
+mp_status set_limiter_tables(MeshDev& m, int cell_points, int samples, double extent,
+                             double c_upper);
 mp_status make_mesh_dev(const mp_mesh_desc& mesh, const mp_weno_options& weno,
                         const PhaseHost& phase, MeshDev& out);
 

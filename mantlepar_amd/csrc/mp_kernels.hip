@@ -20,6 +20,7 @@
 #include "mp_common.h"
 #include "mp_device.h"
 #include "mp_launch.h"
+#include "mp_limit.h"
 #include "mp_phase.h"
 #include "mp_recon.h"
 
@@ -147,11 +148,13 @@ __device__ __forceinline__ void interior_pair(const MeshDev& m, bool vertical, s
     }
 }
 
-template <int MODE, bool PHASE, int NQ>
+// LIM: apply the bound limiter to both fields' reconstructions (FP64 only).
+template <int MODE, bool PHASE, int NQ, bool LIM = false>
 __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int strip, int j0,
                            int j1)
 {
     using R = RealT<MODE>;
+    static_assert(!LIM || MODE == 0, "the limiter runs in the FP64 sweep");
     const int lane = threadIdx.x & (kWave - 1);
     const int nx = m.nx, ny = m.ny;
     const int i = strip * kStripCells - 1 + lane;
@@ -191,8 +194,27 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
         if (have) {
             const unsigned mask = col_ok ? candidate_mask(i, r, nx, ny) : 2u;
             uint32_t e2 = 0;
-            reconstruct<R>(m, wH, mask, LH, e2);
-            reconstruct<R>(m, wC, mask, LC, e2);
+            if constexpr (LIM) {
+                double BH[3][3], BC[3][3];
+                blend<double>(m, wH, mask, BH, e2);
+                blend<double>(m, wC, mask, BC, e2);
+                const unsigned sides = (r == 0 ? 1u : 0u) | (i == nx - 1 ? 2u : 0u) |
+                                       (r == ny - 1 ? 4u : 0u) | (i == 0 ? 8u : 0u);
+                const double thH = limit_factor<NQ>(m, wH, BH, sides, INFINITY, e2);
+                const double thC = limit_factor<NQ>(m, wC, BC, sides, m.lim_cupper, e2);
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        BH[b][a] *= thH;
+                        BC[b][a] *= thC;
+                    }
+                face_lines<double>(m, BH, LH);
+                face_lines<double>(m, BC, LC);
+            } else {
+                reconstruct<R>(m, wH, mask, LH, e2);
+                reconstruct<R>(m, wC, mask, LC, e2);
+            }
             if (col_ok) err |= e2;
         } else {
 #pragma unroll
@@ -315,7 +337,7 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
 
 // Register budget: two waves per SIMD (256 registers).  The mixed path has its
 // own sweep (mp_sweep32.hip).
-template <int MODE, bool PHASE, int NQ>
+template <int MODE, bool PHASE, int NQ, bool LIM = false>
 __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void stage_sweep_kernel(const MeshDev* __restrict__ mesh,
                                                           StagePtrs sp, int nitems)
 {
@@ -325,7 +347,7 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void stage_sweep_kernel(con
     const int strip = item % sp.strips, band = item / sp.strips;
     const int j0 = band * sp.rows;
     const int j1 = min(j0 + sp.rows, m.ny);
-    sweep_item<MODE, PHASE, NQ>(m, sp, item, strip, j0, j1);
+    sweep_item<MODE, PHASE, NQ, LIM>(m, sp, item, strip, j0, j1);
 }
 
 
@@ -496,9 +518,11 @@ __device__ __forceinline__ void load_window(const MeshDev& m, const double* u, i
             w[dj + 1][di + 1] = u[size_t(clampi(j + dj, 0, m.ny - 1)) * m.nx + clampi(i + di, 0, m.nx - 1)];
 }
 
+// Traces of one field (field 1 = C bounds the limiter by lim_cupper).
 template <int MODE>
 __global__ void recon_diag_kernel(const MeshDev* __restrict__ mesh, const double* __restrict__ u,
-                                  double* sigma, double* omega, double* traces, uint32_t* error)
+                                  int field, double* sigma, double* omega, double* traces,
+                                  uint32_t* error)
 {
     using R = RealT<MODE>;
     const MeshDev& m = *mesh;
@@ -509,13 +533,47 @@ __global__ void recon_diag_kernel(const MeshDev* __restrict__ mesh, const double
     load_window<MODE>(m, u, i, j, w);
     FaceLines<R> L;
     uint32_t err = 0;
-    reconstruct<R>(m, w, candidate_mask(i, j, m.nx, m.ny), L, err, sigma + size_t(c) * 5,
-                   omega + size_t(c) * 5);
+    R B[3][3];
+    blend<R>(m, w, candidate_mask(i, j, m.nx, m.ny), B, err, sigma + size_t(c) * 5,
+             omega + size_t(c) * 5);
+    if constexpr (MODE == 0) {
+        if (m.limit) {
+            const unsigned sides = (j == 0 ? 1u : 0u) | (i == m.nx - 1 ? 2u : 0u) |
+                                   (j == m.ny - 1 ? 4u : 0u) | (i == 0 ? 8u : 0u);
+            const double th = limit_factor<0>(m, w, B, sides, field ? m.lim_cupper : INFINITY, err);
+            for (int b = 0; b < 3; ++b)
+                for (int a = 0; a < 3; ++a) B[b][a] *= th;
+        }
+    }
+    face_lines<R>(m, B, L);
     for (int f = 0; f < 4; ++f)
         for (int q = 0; q < m.q; ++q) {
             const double s = (f & 1) ? m.sv[q] : m.sh[q];
             traces[(size_t(c) * 4 + f) * m.q + q] = w[1][1] + double(line_at<R>(L.c[f], R(s)));
         }
+    if (err) atomicOr(error, err);
+}
+
+// Limiter factors of both fields per cell (FP64 blend), for diagnostics.
+__global__ void limit_diag_kernel(const MeshDev* __restrict__ mesh, const double* __restrict__ H,
+                                  const double* __restrict__ C, double* thetaH, double* thetaC,
+                                  uint32_t* error)
+{
+    const MeshDev& m = *mesh;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    const int i = c % m.nx, j = c / m.nx;
+    const unsigned mask = candidate_mask(i, j, m.nx, m.ny);
+    const unsigned sides = (j == 0 ? 1u : 0u) | (i == m.nx - 1 ? 2u : 0u) |
+                           (j == m.ny - 1 ? 4u : 0u) | (i == 0 ? 8u : 0u);
+    uint32_t err = 0;
+    for (int field = 0; field < 2; ++field) {
+        double w[3][3], B[3][3];
+        load_window<0>(m, field ? C : H, i, j, w);
+        blend<double>(m, w, mask, B, err);
+        (field ? thetaC : thetaH)[c] =
+            limit_factor<0>(m, w, B, sides, field ? m.lim_cupper : INFINITY, err);
+    }
     if (err) atomicOr(error, err);
 }
 
@@ -659,10 +717,19 @@ __global__ void state_phase_kernel(const MeshDev* __restrict__ mesh, const doubl
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
-                        int nitems, hipStream_t stream)
+                        int nitems, hipStream_t stream, bool limit)
 {
     const dim3 grid((nitems + 3) / 4), block(256);
-    if (mode == 0 && q == 5) {
+    if (limit) {
+        if (mode != 0) return hipErrorInvalidValue;
+        if (q == 5) {
+            if (phase) stage_sweep_kernel<0, true, 5, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+            else stage_sweep_kernel<0, false, 5, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        } else {
+            if (phase) stage_sweep_kernel<0, true, 0, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+            else stage_sweep_kernel<0, false, 0, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        }
+    } else if (mode == 0 && q == 5) {
         if (phase) stage_sweep_kernel<0, true, 5><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
         else stage_sweep_kernel<0, false, 5><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
     } else if (mode == 0) {
@@ -671,6 +738,14 @@ hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, cons
     } else {
         return launch_stage32(phase, d_mesh, sp, nitems, stream);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_limit_diag(const MeshDev* d_mesh, int ncells, const double* H, const double* C,
+                             double* thetaH, double* thetaC, uint32_t* error, hipStream_t stream)
+{
+    const dim3 grid((ncells + 127) / 128), block(128);
+    limit_diag_kernel<<<grid, block, 0, stream>>>(d_mesh, H, C, thetaH, thetaC, error);
     return hipGetLastError();
 }
 
@@ -684,12 +759,12 @@ hipError_t launch_face_prep(const MeshDev* d_mesh, int nx, int ny, FaceMom* fx, 
 }
 
 hipError_t launch_recon_diag(int mode, const MeshDev* d_mesh, int ncells, const double* u,
-                             double* sigma, double* omega, double* traces, uint32_t* error,
-                             hipStream_t stream)
+                             int field, double* sigma, double* omega, double* traces,
+                             uint32_t* error, hipStream_t stream)
 {
     const dim3 grid((ncells + 127) / 128), block(128);
-    if (mode == 0) recon_diag_kernel<0><<<grid, block, 0, stream>>>(d_mesh, u, sigma, omega, traces, error);
-    else recon_diag_kernel<1><<<grid, block, 0, stream>>>(d_mesh, u, sigma, omega, traces, error);
+    if (mode == 0) recon_diag_kernel<0><<<grid, block, 0, stream>>>(d_mesh, u, field, sigma, omega, traces, error);
+    else recon_diag_kernel<1><<<grid, block, 0, stream>>>(d_mesh, u, field, sigma, omega, traces, error);
     return hipGetLastError();
 }
 

@@ -7,13 +7,16 @@
 
 namespace mp {
 
+// limit: apply the bound limiter (FP64 sweep only).
 hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
-                        int nitems, hipStream_t stream);
+                        int nitems, hipStream_t stream, bool limit = false);
+hipError_t launch_limit_diag(const MeshDev* d_mesh, int ncells, const double* H, const double* C,
+                             double* thetaH, double* thetaC, uint32_t* error, hipStream_t stream);
 hipError_t launch_face_prep(const MeshDev* d_mesh, int nx, int ny, FaceMom* fx, FaceMom* fy,
                             hipStream_t stream);
 hipError_t launch_recon_diag(int mode, const MeshDev* d_mesh, int ncells, const double* u,
-                             double* sigma, double* omega, double* traces, uint32_t* error,
-                             hipStream_t stream);
+                             int field, double* sigma, double* omega, double* traces,
+                             uint32_t* error, hipStream_t stream);
 hipError_t launch_rhs_diag(int mode, const MeshDev* d_mesh, int ncells, const double* H,
                            const double* C, double* rhsH, double* rhsC, double* bnd,
                            uint32_t* error, hipStream_t stream);

@@ -55,15 +55,14 @@ template <typename R> __device__ __forceinline__ R wscale();   // log(2)-base fa
 template <> __device__ __forceinline__ double wscale<double>() { return 1.0; }
 template <> __device__ __forceinline__ float wscale<float>() { return 1.4426950408889634f; }
 
-// Reconstruct one target.  w[dj+1][di+1] holds the cell averages around the
-// target; entries of discarded candidates may be any finite value.
-// sigma_out/omega_out (optional) receive the five candidates' indicators and
-// weights for diagnostics.
+// Centred blend of one target: B[b][a] is the coefficient of x^a y^b.
+// w[dj+1][di+1] holds the cell averages around the target; entries of
+// discarded candidates may be any finite value.  sigma_out/omega_out
+// (optional) receive the five candidates' indicators and weights.
 template <typename R>
-__device__ __forceinline__ void reconstruct(const MeshDev& m, const double (&w)[3][3],
-                                            unsigned mask, FaceLines<R>& out, uint32_t& err,
-                                            double* sigma_out = nullptr,
-                                            double* omega_out = nullptr)
+__device__ __forceinline__ void blend(const MeshDev& m, const double (&w)[3][3], unsigned mask,
+                                      R (&B)[3][3], uint32_t& err, double* sigma_out = nullptr,
+                                      double* omega_out = nullptr)
 {
     const double t = w[1][1];
     R c[3][3];
@@ -160,7 +159,6 @@ __device__ __forceinline__ void reconstruct(const MeshDev& m, const double (&w)[
     }
     // Blend (Q1 candidates embed in the Q2 monomials; the constant candidate's
     // centred polynomial is zero).
-    R B[3][3];
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
@@ -171,7 +169,12 @@ __device__ __forceinline__ void reconstruct(const MeshDev& m, const double (&w)[
         B[1][0] += om[k + 1] * q01[k];
         B[1][1] += om[k + 1] * q11[k];
     }
-    // Face lines.
+}
+
+// The blend on the four face lines.
+template <typename R>
+__device__ __forceinline__ void face_lines(const MeshDev& m, const R (&B)[3][3], FaceLines<R>& out)
+{
     const R hx = R(m.hx), hy = R(m.hy);
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
@@ -183,6 +186,18 @@ __device__ __forceinline__ void reconstruct(const MeshDev& m, const double (&w)[
         out.c[2][a] = B[0][a] + hy * (B[1][a] + hy * B[2][a]);   // top,    y = +hy
         out.c[0][a] = B[0][a] - hy * (B[1][a] - hy * B[2][a]);   // bottom, y = -hy
     }
+}
+
+// Reconstruct one target onto its face lines (see blend).
+template <typename R>
+__device__ __forceinline__ void reconstruct(const MeshDev& m, const double (&w)[3][3],
+                                            unsigned mask, FaceLines<R>& out, uint32_t& err,
+                                            double* sigma_out = nullptr,
+                                            double* omega_out = nullptr)
+{
+    R B[3][3];
+    blend<R>(m, w, mask, B, err, sigma_out, omega_out);
+    face_lines<R>(m, B, out);
 }
 
 template <typename R>

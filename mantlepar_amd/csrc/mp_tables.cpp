@@ -34,6 +34,8 @@ const char* last_error() { return g_error; }
 mp_status status_from_bits(uint32_t bits)
 {
     if (bits & kErrBackflow) return MP_ERR_BACKFLOW;
+    // the limiter runs before any phase evaluation of the stage
+    if (bits & kErrInadmissible) return MP_ERR_ARG;
     if (bits & kErrComposition) return MP_ERR_COMPOSITION;
     if (bits & kErrFraction) return MP_ERR_PHASE;
     if (bits & kErrInflow) return MP_ERR_ARG;
@@ -44,6 +46,7 @@ mp_status status_from_bits(uint32_t bits)
 const char* describe_bits(uint32_t bits)
 {
     if (bits & kErrBackflow) return "Backflow encountered on an outflow-only boundary";
+    if (bits & kErrInadmissible) return "Inadmissible cell average in coupled reconstruction";
     if (bits & kErrComposition) return "CD must lie in [0, Xe]";
     if (bits & kErrFraction) return "Computed phase fraction is outside [0, 1]";
     if (bits & kErrInflow) return "Missing inflow profile";
@@ -238,6 +241,46 @@ mp_status make_mesh_dev(const mp_mesh_desc& mesh, const mp_weno_options& weno,
     }
     m.ph = phase.dev;
     out = m;
+    return MP_OK;
+}
+
+mp_status set_limiter_tables(MeshDev& m, int cell_points, int samples, double extent,
+                             double c_upper)
+{
+    if (cell_points < 1 || cell_points > kMaxQ)
+        return fail(MP_ERR_ARG, "cell_points_per_axis must be 1..%d", kMaxQ);
+    if (samples < 2 || samples % 2 != 0 || samples > kMaxLimS)
+        return fail(MP_ERR_ARG, "Diffusion requires an even sample count >= 2 (at most %d here)", kMaxLimS);
+    if (!(std::isfinite(extent) && extent > 0 && extent <= 1))
+        return fail(MP_ERR_ARG, "Normal sampling extentFraction must lie in (0,1]");
+    double xc[kMaxQ], wc[kMaxQ];
+    gauss_legendre(cell_points, xc, wc);
+    m.lim_ncp = cell_points;
+    for (int k = 0; k < cell_points; ++k) {
+        m.lim_cx[k] = m.hx * xc[k];
+        m.lim_cy[k] = m.hy * xc[k];
+    }
+    // A face's in-cell samples sit at depth d_k inward from the face along its
+    // normal.  The available normal-ray distance of a uniform rectangle is its
+    // width (the same on both sides of an interior face), so with n samples and
+    // extent f: interior spacing f*w*2/(n-1), offsets k - (n-1)/2 for the n/2
+    // in-cell ones; boundary spacing f*w/n, offsets k - n for k < n.
+    const long double f = extent;
+    const int n = samples;
+    m.lim_ni = n / 2;
+    m.lim_nb = n;
+    for (int k = 0; k < n / 2; ++k) {
+        const long double depth = -((long double)k - (n - 1) / 2.0L) * f * 2.0L / (n - 1);
+        m.lim_xi[k] = (double)((long double)m.hx - depth * m.aw);
+        m.lim_yi[k] = (double)((long double)m.hy - depth * m.bw);
+    }
+    for (int k = 0; k < n; ++k) {
+        const long double depth = -((long double)k - n) * f / n;
+        m.lim_xb[k] = (double)((long double)m.hx - depth * m.aw);
+        m.lim_yb[k] = (double)((long double)m.hy - depth * m.bw);
+    }
+    m.lim_cupper = c_upper;
+    m.limit = 1;
     return MP_OK;
 }
 

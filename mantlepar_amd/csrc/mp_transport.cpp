@@ -268,7 +268,8 @@ mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& fi
             const int out = last ? dst : 3 + st;
             const StagePtrs sp = stage_ptrs(src, kin, out, scheme, st, dt, last, phase_set, st);
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
-            MP_HIP(launch_stage(precision, last, mesh.q, d_md.as<MeshDev>(), sp, nitems, stream));
+            MP_HIP(launch_stage(precision, last, mesh.q, d_md.as<MeshDev>(), sp, nitems, stream,
+                                md.limit != 0));
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
             kin = out;
         }
@@ -451,8 +452,8 @@ mp_status mp_transport::reconstruct(int field, double* sigma, double* omega, dou
     double* d = buf.as<double>();
     MP_HIP(hipMemsetAsync(error.ptr, 0, sizeof(uint32_t), stream));
     MP_HIP(launch_recon_diag(precision, d_md.as<MeshDev>(), int(N),
-                             field ? C[cur].as<double>() : H[cur].as<double>(), d, d + 5 * N,
-                             d + 10 * N, reinterpret_cast<uint32_t*>(error.ptr), stream));
+                             field ? C[cur].as<double>() : H[cur].as<double>(), field, d,
+                             d + 5 * N, d + 10 * N, reinterpret_cast<uint32_t*>(error.ptr), stream));
     std::vector<double> host(N * 10 + nt);
     uint32_t bits = 0;
     MP_HIP(hipMemcpyAsync(host.data(), d, host.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -462,6 +463,38 @@ mp_status mp_transport::reconstruct(int field, double* sigma, double* omega, dou
     if (sigma) std::memcpy(sigma, host.data(), 5 * N * sizeof(double));
     if (omega) std::memcpy(omega, host.data() + 5 * N, 5 * N * sizeof(double));
     if (traces) std::memcpy(traces, host.data() + 10 * N, nt * sizeof(double));
+    return MP_OK;
+}
+
+mp_status mp_transport::set_limiter(int enable, int cell_points, int samples, double extent)
+{
+    if (!enable) {
+        md.limit = 0;
+        return sync_md();
+    }
+    if (precision != MP_PREC_FP64)
+        return fail(MP_ERR_ARG, "the bound limiter runs in the FP64 precision mode");
+    MP_CALL(set_limiter_tables(md, cell_points, samples, extent,
+                               phase_params.hypereutectic ? 1.0 : phase_params.Xe));
+    return sync_md();
+}
+
+mp_status mp_transport::limits(double* thetaH, double* thetaC)
+{
+    if (!md.limit) return fail(MP_ERR_STATE, "the bound limiter is off");
+    DeviceBuffer buf;
+    MP_CALL(buf.alloc(2 * N * sizeof(double)));
+    double* d = buf.as<double>();
+    MP_HIP(hipMemsetAsync(error.ptr, 0, sizeof(uint32_t), stream));
+    MP_HIP(launch_limit_diag(d_md.as<MeshDev>(), int(N), H[cur].as<double>(), C[cur].as<double>(),
+                             d, d + N, reinterpret_cast<uint32_t*>(error.ptr), stream));
+    uint32_t bits = 0;
+    MP_HIP(hipMemcpyAsync(&bits, error.ptr, sizeof bits, hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    if (bits) return fail(status_from_bits(bits), "%s (device error bits 0x%x)", describe_bits(bits), bits);
+    if (thetaH) MP_HIP(hipMemcpyAsync(thetaH, d, N * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (thetaC) MP_HIP(hipMemcpyAsync(thetaC, d + N, N * sizeof(double), hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipStreamSynchronize(stream));
     return MP_OK;
 }
 
@@ -612,6 +645,19 @@ mp_status mp_transport_profile(mp_transport* t, int32_t nsteps, int32_t scheme, 
 {
     if (!t) return fail(MP_ERR_ARG, "null handle");
     MP_GUARD({ return t->profile(nsteps, scheme, dt, ms_total, ms_stage, ms_phase_stage); })
+}
+
+mp_status mp_transport_set_limiter(mp_transport* t, int32_t enable, int32_t cell_points_per_axis,
+                                   int32_t diffusion_samples, double extent_fraction)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->set_limiter(enable, cell_points_per_axis, diffusion_samples, extent_fraction); })
+}
+
+mp_status mp_transport_limits(mp_transport* t, double* thetaH, double* thetaC)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->limits(thetaH, thetaC); })
 }
 
 mp_status mp_transport_set_tile(mp_transport* t, int32_t rows_per_item)

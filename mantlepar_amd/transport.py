@@ -175,6 +175,20 @@ class Transport:
     def set_tile(self, rows):
         check(lib().mp_transport_set_tile(self._h, rows))
 
+    def set_limiter(self, enable=True, cell_points_per_axis=5, diffusion_samples=4,
+                    extent_fraction=0.9):
+        """StateReconstruction::Limit (src/couple/state_reconstruction.cpp:136-192) on
+        every reconstruction of the stage kernels (FP64 precision mode)."""
+        check(lib().mp_transport_set_limiter(self._h, int(enable), cell_points_per_axis,
+                                             diffusion_samples, extent_fraction))
+
+    def limits(self):
+        """Limiter factors (theta_H, theta_C) of the current state."""
+        tH = np.empty(self.N)
+        tC = np.empty(self.N)
+        check(lib().mp_transport_limits(self._h, dptr(tH), dptr(tC)))
+        return tH.reshape(self.ny, self.nx), tC.reshape(self.ny, self.nx)
+
     def profile(self, nsteps, dt, scheme=SCHEME_SSPRK3):
         vals = [C.c_double() for _ in range(3)]
         check(lib().mp_transport_profile(self._h, nsteps, scheme, dt,
