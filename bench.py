@@ -404,6 +404,30 @@ def bench_limiter(args, spec, H, Cc, vx, vy, phase):
     return out
 
 
+def bench_diffusion(args, spec, H, Cc, vx, vy, phase):
+    """Mixed and FP64 sweeps with the legacy preheat thermal diffusion of H
+    folded into the face fluxes, beside the same sweeps without it."""
+    import mantlepar_amd as M
+    out = {}
+    kappa = 1e-6
+    for prec, name in ((M.PREC_MIXED, "mixed"), (M.PREC_FP64, "fp64")):
+        t = M.Transport(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q,
+                        phase=phase, precision=prec)
+        t.set_face_velocity(vx, vy)
+        dt = t.stable_dt(0.4)
+        res = {}
+        for k in (0.0, kappa):
+            t.set_diffusion(k, 4, 0.9)
+            t.upload(H, Cc)
+            res["diffusion" if k else "advection_only"] = t.profile(10, dt)["ms_stage"]
+        res["stage_overhead"] = res["diffusion"] / res["advection_only"]
+        out[name] = res
+        t.close()
+    out["kappa"] = kappa
+    log(f"diffusion: mixed stage x{out['mixed']['stage_overhead']:.3f}, fp64 x{out['fp64']['stage_overhead']:.3f}")
+    return out
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -504,6 +528,7 @@ def main():
     }
     if args.limiter_steps > 0:
         out["limiter_fp64"] = bench_limiter(args, spec, H, Cc, vx, vy, phase)
+        out["diffusion"] = bench_diffusion(args, spec, H, Cc, vx, vy, phase)
     if args.full_steps > 0:
         out["full_chip"] = bench_full_chip(args)
     if not (args.no_batch and args.no_serve):

@@ -171,6 +171,26 @@ mp_status mp_transport_set_tile(mp_transport* t, int32_t rows_per_item);
  * switches it off (the default). */
 mp_status mp_transport_set_limiter(mp_transport* t, int32_t enable, int32_t cell_points_per_axis,
                                    int32_t diffusion_samples, double extent_fraction);
+/* Thermal diffusion of H in the legacy preheat form, added to every stage's
+ * face fluxes: replaces the diffusion half of the preheat RHS
+ * (src/couple/preheat.cpp:176-190: IntegrateDiffusiveFlux /
+ * IntegrateDiffusiveBoundaryFlux of src/mlweno/diffusiveflux.cpp:102-200 on
+ * normal-line samples that take each side's cell average of H) with
+ * kappa = config thermalDiffusivity (src/couple/initialization.cpp:284) and
+ * config.sampling (numberOfSamples, extentFraction; defaults 4, 0.9).  The
+ * stable-step rate gains the diffusion terms of src/couple/preheat.cpp:137-148.
+ * C is not diffused.  kappa = 0 switches it off (the default). */
+mp_status mp_transport_set_diffusion(mp_transport* t, double kappa, int32_t diffusion_samples,
+                                     double extent_fraction);
+/* Thermal boundary data per boundary Gauss point, replacing ThermalData /
+ * EvaluateThermalBoundary (src/couple/preheat.cpp:52-62,
+ * src/couple/initialization.cpp:490-506): types[side][face][q] 0 zero flux,
+ * 1 prescribed temperature (the boundary node's value), 2 prescribed outward
+ * flux per length; values likewise; sides bottom, right, top, left, faces
+ * along +x/+y, points in canonical order.  types = NULL: zero flux everywhere
+ * (the default). */
+mp_status mp_transport_set_thermal_boundary(mp_transport* t, const uint8_t* types,
+                                            const double* values);
 /* Limiter factors theta[N] of H and C for the current state (diagnostics). */
 mp_status mp_transport_limits(mp_transport* t, double* thetaH, double* thetaC);
 

@@ -129,6 +129,8 @@ SIGNATURES = [
     ("mp_transport_set_tile", C.c_int32, [_vp, C.c_int32]),
     ("mp_transport_set_limiter", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_double]),
     ("mp_transport_limits", C.c_int32, [_vp, _dp, _dp]),
+    ("mp_transport_set_diffusion", C.c_int32, [_vp, C.c_double, C.c_int32, C.c_double]),
+    ("mp_transport_set_thermal_boundary", C.c_int32, [_vp, _u8p, _dp]),
     ("mp_batch_create", C.c_int32, [C.c_int32, C.POINTER(WenoOptions), C.POINTER(PhaseParams),
                                     C.c_int32, C.c_uint64, _vp, C.POINTER(_vp)]),
     ("mp_batch_destroy", C.c_int32, [_vp]),

@@ -93,6 +93,14 @@ struct MeshDev {
     double lim_cx[kMaxQ], lim_cy[kMaxQ];
     double lim_xi[kMaxLimS], lim_xb[kMaxLimS];
     double lim_yi[kMaxLimS], lim_yb[kMaxLimS];
+    // Thermal diffusion of H in the legacy preheat form (samples are the cell
+    // averages of each side): interior face flux dif_x*(H_L - H_R) along +x on
+    // vertical faces, dif_y likewise on horizontal ones (0: off); a boundary
+    // point with prescribed temperature b adds W_q*dif_b*(H_in - b) outward,
+    // one with prescribed outward flux g adds W_q*g.
+    double dif_x, dif_y, dif_bx, dif_by;
+    const uint8_t* tbc[4];      // thermal policy [face][q]: 0 zero flux, 1 temperature, 2 flux
+    const double* tval[4];      // thermal data [face][q]
     // boundary data (per face policy and optional inflow [face][q])
     const uint8_t* bc[4];       // bottom(nx) right(ny) top(nx) left(ny)
     const double* inflowH[4];

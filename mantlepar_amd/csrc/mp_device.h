@@ -48,6 +48,24 @@ __device__ __forceinline__ double boundary_flux(const MeshDev& m, const double* 
     return F;
 }
 
+// Outward thermal flux of boundary face k of `side` (legacy preheat
+// diffusion, src/couple/preheat.cpp:176-190): t is the boundary cell's
+// average of H, W the face's Gauss weights, coef dif_bx or dif_by.
+__device__ __forceinline__ double thermal_flux(const MeshDev& m, int side, int k, double t,
+                                               const double* W, double coef)
+{
+    const uint8_t* pol = m.tbc[side];
+    if (!pol) return 0.0;
+    pol += size_t(k) * m.q;
+    const double* v = m.tval[side] + size_t(k) * m.q;
+    double F = 0.0;
+    for (int q = 0; q < m.q; ++q) {
+        if (pol[q] == 1) F += W[q] * (coef * (t - v[q]));
+        else if (pol[q] == 2) F += W[q] * v[q];
+    }
+    return F;
+}
+
 // Wave reduction of one work item's partial sums in a fixed butterfly order
 // (compensated for the state sums), stored by lane 0; error bits are or-ed
 // into the launch's error word.

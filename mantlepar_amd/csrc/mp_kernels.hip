@@ -146,6 +146,7 @@ __device__ __forceinline__ void interior_pair(const MeshDev& m, bool vertical, s
         moment_pair(f.vp, f.vm, f.p1, f.p2, f.m1, f.m2, tLH, cLH, tRH, cRH, tLC, cLC, tRC, cRC,
                     FH, FC);
     }
+    FH = fma(vertical ? m.dif_x : m.dif_y, tLH - tRH, FH);   // thermal diffusion of H
 }
 
 // LIM: apply the bound limiter to both fields' reconstructions (FP64 only).
@@ -232,7 +233,8 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
                 const double* inH = m.inflowH[0] ? m.inflowH[0] + size_t(ic) * m.q : nullptr;
                 const double* inC = m.inflowC[0] ? m.inflowC[0] + size_t(ic) * m.q : nullptr;
                 const uint8_t pol = m.bc[0] ? m.bc[0][ic] : 0;
-                const double oH = boundary_flux<R>(m, vn, vys, -1.0, tH, LH.c[0], m.sh, m.wh, pol, inH, err);
+                const double oH = boundary_flux<R>(m, vn, vys, -1.0, tH, LH.c[0], m.sh, m.wh, pol, inH, err) +
+                                  thermal_flux(m, 0, ic, tH, m.wh, m.dif_by);
                 const double oC = boundary_flux<R>(m, vn, vys, -1.0, tC, LC.c[0], m.sh, m.wh, pol, inC, err);
                 FvH = -oH; FvC = -oC;
                 if (owned) { bH += oH; bC += oC; }
@@ -240,7 +242,8 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
                 const double* inH = m.inflowH[2] ? m.inflowH[2] + size_t(ic) * m.q : nullptr;
                 const double* inC = m.inflowC[2] ? m.inflowC[2] + size_t(ic) * m.q : nullptr;
                 const uint8_t pol = m.bc[2] ? m.bc[2][ic] : 0;
-                const double oH = boundary_flux<R>(m, vn, vys, 1.0, tTopH, topH, m.sh, m.wh, pol, inH, err);
+                const double oH = boundary_flux<R>(m, vn, vys, 1.0, tTopH, topH, m.sh, m.wh, pol, inH, err) +
+                                  thermal_flux(m, 2, ic, tTopH, m.wh, m.dif_by);
                 const double oC = boundary_flux<R>(m, vn, vys, 1.0, tTopC, topC, m.sh, m.wh, pol, inC, err);
                 FvH = oH; FvC = oC;
                 if (owned) { bH += oH; bC += oC; }
@@ -297,7 +300,8 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
                     const double* inH = m.inflowH[3] ? m.inflowH[3] + size_t(r) * m.q : nullptr;
                     const double* inC = m.inflowC[3] ? m.inflowC[3] + size_t(r) * m.q : nullptr;
                     const uint8_t pol = m.bc[3] ? m.bc[3][r] : 0;
-                    const double oH = boundary_flux<R>(m, vn, vxs, -1.0, nbtH, nbH, m.sv, m.wv, pol, inH, err);
+                    const double oH = boundary_flux<R>(m, vn, vxs, -1.0, nbtH, nbH, m.sv, m.wv, pol, inH, err) +
+                                      thermal_flux(m, 3, r, nbtH, m.wv, m.dif_bx);
                     const double oC = boundary_flux<R>(m, vn, vxs, -1.0, nbtC, nbC, m.sv, m.wv, pol, inC, err);
                     FrH = -oH; FrC = -oC;
                     bH += oH; bC += oC;
@@ -305,7 +309,8 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
                     const double* inH = m.inflowH[1] ? m.inflowH[1] + size_t(r) * m.q : nullptr;
                     const double* inC = m.inflowC[1] ? m.inflowC[1] + size_t(r) * m.q : nullptr;
                     const uint8_t pol = m.bc[1] ? m.bc[1][r] : 0;
-                    const double oH = boundary_flux<R>(m, vn, vxs, 1.0, tH, LH.c[1], m.sv, m.wv, pol, inH, err);
+                    const double oH = boundary_flux<R>(m, vn, vxs, 1.0, tH, LH.c[1], m.sv, m.wv, pol, inH, err) +
+                                      thermal_flux(m, 1, r, tH, m.wv, m.dif_bx);
                     const double oC = boundary_flux<R>(m, vn, vxs, 1.0, tC, LC.c[1], m.sv, m.wv, pol, inC, err);
                     FrH = oH; FrC = oC;
                     if (owned) { bH += oH; bC += oC; }
@@ -617,7 +622,8 @@ __global__ void rhs_diag_kernel(const MeshDev* __restrict__ mesh, const double* 
                 const double* const* inflow = field ? m.inflowC : m.inflowH;
                 const double* in = inflow[side] ? inflow[side] + size_t(k) * m.q : nullptr;
                 const uint8_t pol = m.bc[side] ? m.bc[side][k] : 0;
-                const double o = boundary_flux<R>(m, vn, vs, sign, t, L.c[f], s, W, pol, in, err);
+                double o = boundary_flux<R>(m, vn, vs, sign, t, L.c[f], s, W, pol, in, err);
+                if (field == 0) o += thermal_flux(m, side, k, t, W, vert ? m.dif_bx : m.dif_by);
                 Fin -= o;
                 Fout_b += o;
             } else {
@@ -627,10 +633,11 @@ __global__ void rhs_diag_kernel(const MeshDev* __restrict__ mesh, const double* 
                 reconstruct<R>(m, wn, candidate_mask(ni[f], nj[f], nx, ny), N, err);
                 const int opp = (f + 2) & 3;
                 double F;
+                const double D = field == 0 ? (vert ? m.dif_x : m.dif_y) : 0.0;
                 if (f == 1 || f == 2)
-                    F = interior_flux<MODE>(m, vn, vs, t, L.c[f], wn[1][1], N.c[opp], s, W);
+                    F = fma(D, t - wn[1][1], interior_flux<MODE>(m, vn, vs, t, L.c[f], wn[1][1], N.c[opp], s, W));
                 else
-                    F = interior_flux<MODE>(m, vn, vs, wn[1][1], N.c[opp], t, L.c[f], s, W);
+                    F = fma(D, wn[1][1] - t, interior_flux<MODE>(m, vn, vs, wn[1][1], N.c[opp], t, L.c[f], s, W));
                 Fin += sign > 0 ? -F : F;
             }
         }
@@ -659,7 +666,20 @@ __global__ void rate_kernel(const MeshDev* __restrict__ mesh, unsigned long long
             sx += a + b;
             sy += cc + d;
         }
-        rate = (sx + sy) * m.inv_area;
+        // thermal diffusion: |sum of sample partials| per face side
+        // (src/couple/preheat.cpp:137-148)
+        double dif = (i > 0 ? m.dif_x : 0.0) + (i < nx - 1 ? m.dif_x : 0.0) +
+                     (j > 0 ? m.dif_y : 0.0) + (j < ny - 1 ? m.dif_y : 0.0);
+        const int bside[4] = {j == 0 ? 0 : -1, i == nx - 1 ? 1 : -1, j == ny - 1 ? 2 : -1, i == 0 ? 3 : -1};
+        for (int f = 0; f < 4; ++f) {
+            if (bside[f] < 0 || !m.tbc[f]) continue;
+            const int k = (f & 1) ? j : i;
+            const double* W = (f & 1) ? m.wv : m.wh;
+            const double coef = (f & 1) ? m.dif_bx : m.dif_by;
+            for (int q = 0; q < m.q; ++q)
+                if (m.tbc[f][size_t(k) * m.q + q] == 1) dif += W[q] * coef;
+        }
+        rate = (sx + sy + dif) * m.inv_area;
     }
     for (int off = 32; off >= 1; off >>= 1) rate = fmax(rate, __shfl_xor(rate, off));
     if ((threadIdx.x & 63) == 0) atomicMax(max_bits, (unsigned long long)__double_as_longlong(rate));

@@ -214,15 +214,17 @@ __device__ __forceinline__ void recon32(const K32& k, const Row32& R0, const Row
 
 // Both fields through one interior face from its moment record: first-order
 // part in FP64, high-order correction in FP32.
-__device__ __forceinline__ void face_flux(const FaceMom& f, double tLH, const float (&cLH)[3],
-                                          double tRH, const float (&cRH)[3], double tLC,
+// D: thermal diffusion coefficient of H on this face (0: off).
+__device__ __forceinline__ void face_flux(const FaceMom& f, double D, double tLH,
+                                          const float (&cLH)[3], double tRH,
+                                          const float (&cRH)[3], double tLC,
                                           const float (&cLC)[3], double tRC,
                                           const float (&cRC)[3], double& FH, double& FC)
 {
     const float vp = float(f.vp), vm = float(f.vm);
     const float ch = cLH[0] * vp + cLH[1] * f.p1 + cLH[2] * f.p2 + cRH[0] * vm + cRH[1] * f.m1 + cRH[2] * f.m2;
     const float cc = cLC[0] * vp + cLC[1] * f.p1 + cLC[2] * f.p2 + cRC[0] * vm + cRC[1] * f.m1 + cRC[2] * f.m2;
-    FH = fma(tLH, f.vp, tRH * f.vm) + double(ch);
+    FH = fma(D, tLH - tRH, fma(tLH, f.vp, tRH * f.vm) + double(ch));
     FC = fma(tLC, f.vp, tRC * f.vm) + double(cc);
 }
 
@@ -322,15 +324,16 @@ __device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, in
                 const double* inH = m.inflowH[side] ? m.inflowH[side] + size_t(ic) * m.q : nullptr;
                 const double* inC = m.inflowC[side] ? m.inflowC[side] + size_t(ic) * m.q : nullptr;
                 const uint8_t pol = m.bc[side] ? m.bc[side][ic] : 0;
-                const double oHf = r == 0 ? boundary_flux<float>(m, vn, vys, sign, tH, LH.c[0], m.sh, m.wh, pol, inH, err)
-                                          : boundary_flux<float>(m, vn, vys, sign, H0.u, topH, m.sh, m.wh, pol, inH, err);
+                const double oHf = (r == 0 ? boundary_flux<float>(m, vn, vys, sign, tH, LH.c[0], m.sh, m.wh, pol, inH, err)
+                                           : boundary_flux<float>(m, vn, vys, sign, H0.u, topH, m.sh, m.wh, pol, inH, err)) +
+                                   thermal_flux(m, side, ic, r == 0 ? tH : H0.u, m.wh, m.dif_by);
                 const double oCf = r == 0 ? boundary_flux<float>(m, vn, vys, sign, tC, LC.c[0], m.sh, m.wh, pol, inC, err)
                                           : boundary_flux<float>(m, vn, vys, sign, C0.u, topC, m.sh, m.wh, pol, inC, err);
                 FvH = sign * oHf;
                 FvC = sign * oCf;
                 if (owned) { bH += oHf; bC += oCf; }
             } else {
-                face_flux(fyc, H0.u, topH, tH, LH.c[0], C0.u, topC, tC, LC.c[0], FvH, FvC);
+                face_flux(fyc, m.dif_y, H0.u, topH, tH, LH.c[0], C0.u, topC, tC, LC.c[0], FvH, FvC);
             }
         }
         if (shift) fyc = load_fy(r + 1);   // used next iteration
@@ -385,15 +388,16 @@ __device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, in
                     const double* inH = m.inflowH[side] ? m.inflowH[side] + size_t(r) * m.q : nullptr;
                     const double* inC = m.inflowC[side] ? m.inflowC[side] + size_t(r) * m.q : nullptr;
                     const uint8_t pol = m.bc[side] ? m.bc[side][r] : 0;
-                    const double oHf = left ? boundary_flux<float>(m, vn, vxs, sign, nbtH, nbH, m.sv, m.wv, pol, inH, err)
-                                            : boundary_flux<float>(m, vn, vxs, sign, tH, LH.c[1], m.sv, m.wv, pol, inH, err);
+                    const double oHf = (left ? boundary_flux<float>(m, vn, vxs, sign, nbtH, nbH, m.sv, m.wv, pol, inH, err)
+                                             : boundary_flux<float>(m, vn, vxs, sign, tH, LH.c[1], m.sv, m.wv, pol, inH, err)) +
+                                       thermal_flux(m, side, r, left ? nbtH : tH, m.wv, m.dif_bx);
                     const double oCf = left ? boundary_flux<float>(m, vn, vxs, sign, nbtC, nbC, m.sv, m.wv, pol, inC, err)
                                             : boundary_flux<float>(m, vn, vxs, sign, tC, LC.c[1], m.sv, m.wv, pol, inC, err);
                     FrH = sign * oHf;
                     FrC = sign * oCf;
                     if (left || owned) { bH += oHf; bC += oCf; }
                 } else {
-                    face_flux(fxc, tH, LH.c[1], nbtH, nbH, tC, LC.c[1], nbtC, nbC, FrH, FrC);
+                    face_flux(fxc, m.dif_x, tH, LH.c[1], nbtH, nbH, tC, LC.c[1], nbtC, nbC, FrH, FrC);
                 }
             }
             accH = shr1(FrH) - FrH + FvH;

@@ -284,6 +284,56 @@ mp_status set_limiter_tables(MeshDev& m, int cell_points, int samples, double ex
     return MP_OK;
 }
 
+// First-derivative weights at x of the Lagrange interpolant through the
+// nodes 0..npts-1 (unit spacing), each rounded to double as the reference
+// stores them (src/mlweno/diffusiveflux.cpp:203-254): w_i = L_i'(x).
+static void lagrange_derivative(int npts, long double x, double* w)
+{
+    for (int i = 0; i < npts; ++i) {
+        long double d = 0;
+        for (int k = 0; k < npts; ++k) {
+            if (k == i) continue;
+            long double p = 1.0L / (i - k);
+            for (int l = 0; l < npts; ++l)
+                if (l != i && l != k) p *= (x - l) / (long double)(i - l);
+            d += p;
+        }
+        w[i] = (double)d;
+    }
+}
+
+mp_status set_diffusion_tables(MeshDev& m, double kappa, int samples, double extent)
+{
+    if (!(std::isfinite(kappa) && kappa >= 0)) return fail(MP_ERR_ARG, "diffusivity must be nonnegative");
+    if (samples < 2 || samples % 2 != 0 || samples > kMaxLimS)
+        return fail(MP_ERR_ARG, "Diffusion requires an even sample count >= 2 (at most %d here)", kMaxLimS);
+    if (!(std::isfinite(extent) && extent > 0 && extent <= 1))
+        return fail(MP_ERR_ARG, "Normal sampling extentFraction must lie in (0,1]");
+    const int n = samples;
+    // Interior face: n samples at (k - (n-1)/2)*ds along the normal, ds =
+    // 2 f w/(n-1) with w the width across the face; with the cell averages as
+    // samples only the in-cell half differs from the last sample, so
+    // F = -kappa * length * S / ds * (H_L - H_R), S = sum_{k<n/2} w_k (< 0).
+    double wm[kMaxLimS + 1], we[kMaxLimS + 1];
+    lagrange_derivative(n, (n - 1) / 2.0L, wm);
+    long double S = 0;
+    for (int k = 0; k < n / 2; ++k) S += wm[k];
+    // Boundary face: n in-cell samples and the face node at the end, ds = f w/n;
+    // F = kappa * Hn / ds * (H_in - b) per unit length, Hn = -sum_{k<n} w_k.
+    lagrange_derivative(n + 1, (long double)n, we);
+    long double Hn = 0;
+    for (int k = 0; k < n; ++k) Hn -= we[k];
+    const double dsx = (double)((long double)extent * m.dx * 2.0L / (n - 1));
+    const double dsy = (double)((long double)extent * m.dy * 2.0L / (n - 1));
+    const double dbx = (double)((long double)extent * m.dx / n);
+    const double dby = (double)((long double)extent * m.dy / n);
+    m.dif_x = (double)(-(long double)kappa * m.dy * S / dsx);
+    m.dif_y = (double)(-(long double)kappa * m.dx * S / dsy);
+    m.dif_bx = (double)((long double)kappa * Hn / dbx);
+    m.dif_by = (double)((long double)kappa * Hn / dby);
+    return MP_OK;
+}
+
 mp_status DeviceBuffer::alloc(size_t n)
 {
     reset();

@@ -498,6 +498,52 @@ mp_status mp_transport::limits(double* thetaH, double* thetaC)
     return MP_OK;
 }
 
+mp_status mp_transport::set_diffusion(double kappa, int samples, double extent)
+{
+    MeshDev m = md;
+    MP_CALL(set_diffusion_tables(m, kappa, samples, extent));
+    md = m;
+    MP_CALL(sync_md());
+    if (have_velocity) MP_CALL(compute_rate());
+    return MP_OK;
+}
+
+mp_status mp_transport::set_thermal_boundary(const uint8_t* types, const double* values)
+{
+    const size_t faces = size_t(2) * (mesh.nx + mesh.ny), total = faces * mesh.q;
+    const size_t off[4] = {0, size_t(mesh.nx) * mesh.q, size_t(mesh.nx + mesh.ny) * mesh.q,
+                           size_t(2 * mesh.nx + mesh.ny) * mesh.q};
+    if (!types) {
+        tbc.reset();
+        tval.reset();
+        for (int s = 0; s < 4; ++s) { md.tbc[s] = nullptr; md.tval[s] = nullptr; }
+    } else {
+        for (size_t k = 0; k < total; ++k)
+            if (types[k] > 2) return fail(MP_ERR_ARG, "Unknown thermal boundary policy");
+        std::vector<double> v(total, 0.0);
+        if (values) {
+            for (size_t k = 0; k < total; ++k)
+                if (types[k] != 0 && !std::isfinite(values[k]))
+                    return fail(MP_ERR_ARG, "Thermal boundary data must be finite");
+            std::memcpy(v.data(), values, total * sizeof(double));
+        } else {
+            for (size_t k = 0; k < total; ++k)
+                if (types[k] != 0) return fail(MP_ERR_ARG, "Missing thermal boundary profile.");
+        }
+        MP_CALL(tbc.alloc(total));
+        MP_CALL(tval.alloc(total * sizeof(double)));
+        MP_HIP(hipMemcpyAsync(tbc.ptr, types, total, hipMemcpyHostToDevice, stream));
+        MP_HIP(hipMemcpyAsync(tval.ptr, v.data(), total * sizeof(double), hipMemcpyHostToDevice, stream));
+        for (int s = 0; s < 4; ++s) {
+            md.tbc[s] = static_cast<const uint8_t*>(tbc.ptr) + off[s];
+            md.tval[s] = tval.as<double>() + off[s];
+        }
+    }
+    MP_CALL(sync_md());
+    if (have_velocity) MP_CALL(compute_rate());
+    return MP_OK;
+}
+
 // ===================================================================== C ABI
 
 extern "C" {
@@ -652,6 +698,20 @@ mp_status mp_transport_set_limiter(mp_transport* t, int32_t enable, int32_t cell
 {
     if (!t) return fail(MP_ERR_ARG, "null handle");
     MP_GUARD({ return t->set_limiter(enable, cell_points_per_axis, diffusion_samples, extent_fraction); })
+}
+
+mp_status mp_transport_set_diffusion(mp_transport* t, double kappa, int32_t diffusion_samples,
+                                     double extent_fraction)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->set_diffusion(kappa, diffusion_samples, extent_fraction); })
+}
+
+mp_status mp_transport_set_thermal_boundary(mp_transport* t, const uint8_t* types,
+                                            const double* values)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->set_thermal_boundary(types, values); })
 }
 
 mp_status mp_transport_limits(mp_transport* t, double* thetaH, double* thetaC)

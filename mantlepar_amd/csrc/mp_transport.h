@@ -25,6 +25,7 @@ struct mp_transport {
     bool have_velocity = false;
     double max_rate = 0;
     mp::DeviceBuffer bc[4], inflowH, inflowC;
+    mp::DeviceBuffer tbc, tval;   // thermal boundary policies and data
     // phase outputs: T, phi, Cl, Cs (N doubles each) then region (N bytes)
     mp::DeviceBuffer phase_buf[2];
     int phase_cur = 0;
@@ -63,4 +64,6 @@ struct mp_transport {
     mp_status reconstruct(int field, double* sigma, double* omega, double* traces);
     mp_status set_limiter(int enable, int cell_points, int samples, double extent);
     mp_status limits(double* thetaH, double* thetaC);
+    mp_status set_diffusion(double kappa, int samples, double extent);
+    mp_status set_thermal_boundary(const uint8_t* types, const double* values);
 };

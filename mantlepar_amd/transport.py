@@ -182,6 +182,18 @@ class Transport:
         check(lib().mp_transport_set_limiter(self._h, int(enable), cell_points_per_axis,
                                              diffusion_samples, extent_fraction))
 
+    def set_diffusion(self, kappa, diffusion_samples=4, extent_fraction=0.9):
+        """Legacy preheat thermal diffusion of H (src/couple/preheat.cpp:176-190)."""
+        check(lib().mp_transport_set_diffusion(self._h, kappa, diffusion_samples,
+                                               extent_fraction))
+
+    def set_thermal_boundary(self, types=None, values=None):
+        """types/values: arrays [2*(nx+ny)*q] (sides bottom, right, top, left, each
+        [face][q]); types 0 zero flux, 1 prescribed temperature, 2 outward flux."""
+        t = None if types is None else np.ascontiguousarray(types, dtype=np.uint8).ravel()
+        v = None if values is None else _f64(values).ravel()
+        check(lib().mp_transport_set_thermal_boundary(self._h, u8ptr(t), dptr(v)))
+
     def limits(self):
         """Limiter factors (theta_H, theta_C) of the current state."""
         tH = np.empty(self.N)

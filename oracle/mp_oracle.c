@@ -1358,6 +1358,11 @@ struct mpo_transport {
     int limit, lim_cell_points, lim_samples;
     double lim_extent, lim_cupper;
     double* theta[2];                 /* last limiter factors per field [cell] */
+    /* thermal diffusion of H (off unless set): kappa, sampling, boundary data */
+    int diffuse, dif_samples;
+    double kappa, dif_extent;
+    uint8_t* tbc[4];                  /* [face][q]: 0 zero flux, 1 temperature, 2 flux */
+    double* tval[4];                  /* [face][q] canonical order */
 };
 
 static size_t cidx(const mpo_transport* t, int i, int j) { return (size_t)j * t->nx + i; }
@@ -1368,6 +1373,7 @@ void mpo_transport_destroy(mpo_transport* t)
     free(t->coords); free(t->stencils); free(t->area); free(t->vnx); free(t->vny);
     for (int s = 0; s < 4; ++s) { free(t->bc[s]); free(t->inflow[0][s]); free(t->inflow[1][s]); }
     free(t->traces); free(t->fv); free(t->fh); free(t->theta[0]); free(t->theta[1]);
+    for (int s = 0; s < 4; ++s) { free(t->tbc[s]); free(t->tval[s]); }
     free(t);
 }
 
@@ -1655,12 +1661,95 @@ static mpo_point axis_velocity(double vn, int vertical)
     return v;
 }
 
+int mpo_transport_set_diffusion(mpo_transport* t, double kappa, int samples, double extent,
+                                const uint8_t* types, const double* values)
+{
+    CHECK(finite_d(kappa) && kappa >= 0, MPO_ERR_ARG, "diffusivity must be nonnegative");
+    CHECK(samples >= 2 && samples % 2 == 0 && samples <= MPO_MAX_SAMPLES, MPO_ERR_ARG,
+          "Diffusion requires an even sample count >= 2 with room for a boundary node");
+    CHECK(finite_d(extent) && extent > 0 && extent <= 1, MPO_ERR_ARG,
+          "Normal sampling extentFraction must lie in (0,1]");
+    size_t to = 0, vo = 0;
+    for (int s = 0; s < 4; ++s) {
+        const int n = side_len(t, s);
+        free(t->tbc[s]); free(t->tval[s]);
+        t->tbc[s] = (uint8_t*)calloc((size_t)n * t->q, 1);
+        t->tval[s] = (double*)calloc((size_t)n * t->q, sizeof(double));
+        if (!t->tbc[s] || !t->tval[s]) return fail(MPO_ERR_MEM, "out of memory");
+        for (int k = 0; k < n * t->q; ++k) {
+            t->tbc[s][k] = types ? types[to + k] : 0;
+            CHECK(t->tbc[s][k] <= 2, MPO_ERR_ARG, "Unknown thermal boundary policy");
+        }
+        if (values) memcpy(t->tval[s], values + vo, (size_t)n * t->q * sizeof(double));
+        to += (size_t)n * t->q;
+        vo += (size_t)n * t->q;
+    }
+    t->diffuse = 1; t->kappa = kappa; t->dif_samples = samples; t->dif_extent = extent;
+    return MPO_OK;
+}
+
+/* Legacy preheat diffusion of one face (src/couple/preheat.cpp:176-190): the
+ * normal-line samples take the constant cell averages of H of their side;
+ * a boundary face takes its thermal data (EvaluateThermalBoundary,
+ * src/couple/initialization.cpp:490-506: temperature -> PrescribedQuantity,
+ * flux -> PrescribedFlux).  edge is CCW for cell L; R < 0 marks a boundary
+ * face of `side` (face index k, Gauss order reversed when rev).  Also returns
+ * the rate terms |sum of sample partials| per side
+ * (src/couple/preheat.cpp:137-148). */
+static int face_diffusion(const mpo_transport* t, const mpo_point edge[2], int li, int lj, int ri,
+                          int rj, const double* u, int side, int k, int rev, double* flux,
+                          double* rate_left, double* rate_right)
+{
+    mpo_point lc[4], rc[4];
+    mpo_cell_corners(&t->mesh, li, lj, lc);
+    const int boundary = ri < 0;
+    if (!boundary) mpo_cell_corners(&t->mesh, ri, rj, rc);
+    mpo_sampling s;
+    CALL(mpo_diffusive_sampling(edge, lc, boundary ? NULL : rc, t->q, t->qp, t->qw, t->dif_samples,
+                                t->dif_extent, &s));
+    double samples[16 * (MPO_MAX_SAMPLES + 1)], kappa[16], bv[16];
+    int bt[16];
+    const double uL = u[cidx(t, li, lj)], uR = boundary ? 0.0 : u[cidx(t, ri, rj)];
+    for (int a = 0; a < s.nq * s.count; ++a)
+        samples[a] = s.sides[a] == MPO_SIDE_LEFT ? uL : (s.sides[a] == MPO_SIDE_RIGHT ? uR : 0.0);
+    for (int q = 0; q < s.nq; ++q) {
+        kappa[q] = t->kappa;
+        if (boundary) {
+            const int qc = rev ? s.nq - 1 - q : q;
+            const int pol = t->tbc[side] ? t->tbc[side][(size_t)k * s.nq + qc] : 0;
+            bt[q] = pol == 1 ? MPO_DIFF_QUANTITY : (pol == 2 ? MPO_DIFF_FLUX : MPO_DIFF_ZERO_FLUX);
+            bv[q] = t->tval[side] ? t->tval[side][(size_t)k * s.nq + qc] : 0.0;
+        }
+    }
+    CALL(mpo_diffusive_flux(&s, samples, kappa, boundary ? bt : NULL, boundary ? bv : NULL, flux));
+    /* sample partials of IntegrateImpl's derivative path (identity law):
+     * -W_q kappa / ds_q times the coefficient, the last slot taking minus the
+     * sum of the others; skipped points (zero or prescribed flux) give none */
+    long double dl = 0, dr = 0;
+    long double last = 0;
+    for (int kk = 0; kk + 1 < s.count; ++kk) last -= s.coefficients[kk];
+    for (int q = 0; q < s.nq; ++q) {
+        if (boundary && bt[q] != MPO_DIFF_QUANTITY && bt[q] != MPO_DIFF_STATE) continue;
+        const long double scale = -(long double)s.weights[q] * kappa[q] / s.spacing[q];
+        for (int kk = 0; kk < s.count; ++kk) {
+            const int a = q * s.count + kk;
+            const long double coeff = kk + 1 == s.count ? last : s.coefficients[kk];
+            if (s.sides[a] == MPO_SIDE_LEFT) dl += scale * coeff;
+            else if (s.sides[a] == MPO_SIDE_RIGHT) dr += scale * coeff;
+        }
+    }
+    if (rate_left) *rate_left = fabs((double)dl);
+    if (rate_right) *rate_right = fabs((double)dr);
+    return MPO_OK;
+}
+
 /* Interior and boundary face fluxes for one field from the stored traces.
  * Face loop and LLF integration: src/mlweno/test/transport_test_support.h:278-306,
  * src/mlweno/advectiveflux.cpp:192-227; boundary policy
  * src/couple/initialization.cpp:469-489 with src/mlweno/advectiveflux.cpp:229-268. */
-static int face_fluxes(mpo_transport* t, int field)
+static int face_fluxes(mpo_transport* t, int field, const double* u)
 {
+    const int diffuse = field == 0 && t->diffuse;
     const int nx = t->nx, ny = t->ny, nq = t->q;
     int err = 0;
     char msg[256] = {0};
@@ -1681,6 +1770,11 @@ static int face_fluxes(mpo_transport* t, int field)
                     right[q] = t->traces[(cidx(t, i, j) * 4 + 3) * nq + q];
                 }
                 e = mpo_integrate_edge_flux(edge, nq, t->qp, t->qw, vel, left, right, 0, 0, &flux);
+                if (!e && diffuse) {
+                    double d;
+                    e = face_diffusion(t, edge, i - 1, j, i, j, u, 0, 0, 0, &d, NULL, NULL);
+                    flux += d;
+                }
             } else {
                 /* boundary: interior cell's CCW edge, outward normal */
                 const int side = i == 0 ? 3 : 1;
@@ -1711,6 +1805,11 @@ static int face_fluxes(mpo_transport* t, int field)
                     } else { types[q] = 1; vals[q] = 0; }
                 }
                 if (!e) e = mpo_integrate_boundary_flux(edge, nq, t->qp, t->qw, vel, left, types, vals, &flux);
+                if (!e && diffuse) {
+                    double d;
+                    e = face_diffusion(t, edge, ci, j, -1, -1, u, side, j, rev, &d, NULL, NULL);
+                    flux += d;
+                }
                 if (!e && side == 3) flux = -flux;   /* store along +x */
             }
             if (e) {
@@ -1738,6 +1837,11 @@ static int face_fluxes(mpo_transport* t, int field)
                     right[q] = t->traces[(cidx(t, i, j) * 4 + 0) * nq + qc];
                 }
                 e = mpo_integrate_edge_flux(edge, nq, t->qp, t->qw, vel, left, right, 0, 0, &flux);
+                if (!e && diffuse) {
+                    double d;
+                    e = face_diffusion(t, edge, i, j - 1, i, j, u, 0, 0, 0, &d, NULL, NULL);
+                    flux += d;
+                }
             } else {
                 const int side = j == 0 ? 0 : 2;
                 const int cj = j == 0 ? 0 : ny - 1;
@@ -1767,6 +1871,11 @@ static int face_fluxes(mpo_transport* t, int field)
                     } else { types[q] = 1; vals[q] = 0; }
                 }
                 if (!e) e = mpo_integrate_boundary_flux(edge, nq, t->qp, t->qw, vel, left, types, vals, &flux);
+                if (!e && diffuse) {
+                    double d;
+                    e = face_diffusion(t, edge, i, cj, -1, -1, u, side, i, rev, &d, NULL, NULL);
+                    flux += d;
+                }
                 if (!e && side == 0) flux = -flux;   /* store along +y */
             }
             if (e) {
@@ -1803,10 +1912,10 @@ int mpo_transport_rhs(mpo_transport* t, const double* H, const double* C, double
 {
     double bh = 0, bc = 0;
     CALL(field_traces(t, 0, H, NULL, NULL));
-    CALL(face_fluxes(t, 0));
+    CALL(face_fluxes(t, 0, H));
     gather_rhs(t, rhsH, &bh);
     CALL(field_traces(t, 1, C, NULL, NULL));
-    CALL(face_fluxes(t, 1));
+    CALL(face_fluxes(t, 1, C));
     gather_rhs(t, rhsC, &bc);
     if (bfluxH) *bfluxH = bh;
     if (bfluxC) *bfluxC = bc;
@@ -1819,7 +1928,8 @@ int mpo_transport_rates(mpo_transport* t, double* rate, double* max_rate)
     const int nx = t->nx, ny = t->ny, nq = t->q;
     const size_t N = (size_t)nx * ny;
     double* r = (double*)calloc(N, sizeof(double));
-    if (!r) return fail(MPO_ERR_MEM, "out of memory");
+    double* dummy_u = (double*)calloc(N, sizeof(double));   /* rates do not depend on H */
+    if (!r || !dummy_u) { free(r); free(dummy_u); return fail(MPO_ERR_MEM, "out of memory"); }
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i <= nx; ++i) {
             mpo_point e[2];
@@ -1830,8 +1940,23 @@ int mpo_transport_rates(mpo_transport* t, double* rate, double* max_rate)
             double speed = 0;
             for (int q = 0; q < nq; ++q)
                 speed += .5 * length * t->qw[q] * fabs(t->vnx[((size_t)q * ny + j) * (nx + 1) + i]);
-            if (i > 0) r[cidx(t, i - 1, j)] += speed / t->area[cidx(t, i - 1, j)];
-            if (i < nx) r[cidx(t, i, j)] += speed / t->area[cidx(t, i, j)];
+            double dl = 0, dr = 0;
+            if (t->diffuse) {
+                double f;
+                mpo_point ce[2];
+                if (i > 0 && i < nx) {
+                    canonical_face(t, i - 1, j, 1, ce);
+                    CALL(face_diffusion(t, ce, i - 1, j, i, j, dummy_u, 0, 0, 0, &f, &dl, &dr));
+                } else {
+                    mpo_point c4[4];
+                    mpo_cell_corners(&t->mesh, ci, j, c4);
+                    if (i == nx) { ce[0] = c4[1]; ce[1] = c4[2]; } else { ce[0] = c4[3]; ce[1] = c4[0]; }
+                    CALL(face_diffusion(t, ce, ci, j, -1, -1, dummy_u, i == nx ? 1 : 3, j, i == 0,
+                                        &f, i == 0 ? &dr : &dl, NULL));
+                }
+            }
+            if (i > 0) r[cidx(t, i - 1, j)] += (speed + dl) / t->area[cidx(t, i - 1, j)];
+            if (i < nx) r[cidx(t, i, j)] += (speed + dr) / t->area[cidx(t, i, j)];
         }
     for (int j = 0; j <= ny; ++j)
         for (int i = 0; i < nx; ++i) {
@@ -1843,13 +1968,31 @@ int mpo_transport_rates(mpo_transport* t, double* rate, double* max_rate)
             double speed = 0;
             for (int q = 0; q < nq; ++q)
                 speed += .5 * length * t->qw[q] * fabs(t->vny[((size_t)q * (ny + 1) + j) * nx + i]);
-            if (j > 0) r[cidx(t, i, j - 1)] += speed / t->area[cidx(t, i, j - 1)];
-            if (j < ny) r[cidx(t, i, j)] += speed / t->area[cidx(t, i, j)];
+            double dl = 0, dr = 0;
+            if (t->diffuse) {
+                double f;
+                mpo_point ce[2];
+                if (j > 0 && j < ny) {
+                    mpo_point c4[4];
+                    mpo_cell_corners(&t->mesh, i, j - 1, c4);
+                    ce[0] = c4[2]; ce[1] = c4[3];
+                    CALL(face_diffusion(t, ce, i, j - 1, i, j, dummy_u, 0, 0, 0, &f, &dl, &dr));
+                } else {
+                    mpo_point c4[4];
+                    mpo_cell_corners(&t->mesh, i, cj, c4);
+                    if (j == ny) { ce[0] = c4[2]; ce[1] = c4[3]; } else { ce[0] = c4[0]; ce[1] = c4[1]; }
+                    CALL(face_diffusion(t, ce, i, cj, -1, -1, dummy_u, j == ny ? 2 : 0, i, j == ny,
+                                        &f, j == 0 ? &dr : &dl, NULL));
+                }
+            }
+            if (j > 0) r[cidx(t, i, j - 1)] += (speed + dl) / t->area[cidx(t, i, j - 1)];
+            if (j < ny) r[cidx(t, i, j)] += (speed + dr) / t->area[cidx(t, i, j)];
         }
     double mx = 0;
     for (size_t c = 0; c < N; ++c) if (r[c] > mx) mx = r[c];
     if (rate) memcpy(rate, r, N * sizeof(double));
     free(r);
+    free(dummy_u);
     if (max_rate) *max_rate = mx;
     return MPO_OK;
 }

@@ -263,6 +263,14 @@ int mpo_transport_reconstruct(mpo_transport* t, const double* u, double* sigma,
  * (Xe).  enable = 0 switches it off (the default). */
 int mpo_transport_set_limiter(mpo_transport* t, int enable, int cell_points, int samples,
                               double extent, double c_upper);
+/* Thermal diffusion of H as the legacy preheat applies it (src/couple/preheat.cpp:176-190):
+ * normal-line samples of the cell averages, diffusivity kappa, `samples` per
+ * interior edge point with extent fraction `extent`.  types[side][face][q]
+ * (NULL: zero flux everywhere) 0 zero flux, 1 prescribed temperature, 2
+ * prescribed outward flux; values likewise, canonical point order.  The rates
+ * gain the diffusion terms of src/couple/preheat.cpp:137-148. */
+int mpo_transport_set_diffusion(mpo_transport* t, double kappa, int samples, double extent,
+                                const uint8_t* types, const double* values);
 /* Traces of one field (0 H, 1 C: the limiter bounds C by c_upper) [N][4][q]. */
 int mpo_transport_traces(mpo_transport* t, int field, const double* u, double* traces);
 /* Limiter factors theta of both fields for one state [N] (1 with the limiter off). */

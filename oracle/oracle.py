@@ -233,6 +233,8 @@ def _declare(L):
                                             C.c_double]
     L.mpo_transport_limits.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     L.mpo_transport_traces.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    L.mpo_transport_set_diffusion.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double,
+                                              C.c_void_p, C.c_void_p]
     L.mpo_mix64.argtypes = [C.c_uint64]
     L.mpo_mix64.restype = C.c_uint64
 
@@ -586,6 +588,27 @@ class Transport:
         """Bound limiter of src/couple/state_reconstruction.cpp:136-192 on every trace."""
         _check(lib().mpo_transport_set_limiter(self._h, int(enable), cell_points, samples,
                                                extent, c_upper))
+
+    def set_diffusion(self, kappa, samples=4, extent=0.9, types=None, values=None):
+        """Legacy preheat thermal diffusion of H (src/couple/preheat.cpp:176-190).
+        types: dict side -> uint8 per boundary face, or per face and point [n][q]
+        (0 zero flux, 1 temperature, 2 outward flux); values: [sum of side
+        faces][q] in side order bottom, right, top, left."""
+        t = v = None
+        if types is not None:
+            parts = []
+            for sd, n in (("bottom", self.nx), ("right", self.ny), ("top", self.nx),
+                          ("left", self.ny)):
+                a = np.asarray(types.get(sd, np.zeros(n)), dtype=np.uint8)
+                parts.append(np.broadcast_to(a.reshape(n, -1), (n, self.q)).ravel())
+            t = np.ascontiguousarray(np.concatenate(parts))
+        if values is not None:
+            v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        self._dif_keep = (t, v)
+        _check(lib().mpo_transport_set_diffusion(
+            self._h, kappa, samples, extent,
+            t.ctypes.data_as(C.c_void_p) if t is not None else None,
+            v.ctypes.data_as(C.c_void_p) if v is not None else None))
 
     def traces(self, u, field=0):
         tr = np.zeros(self.N * 4 * self.q)

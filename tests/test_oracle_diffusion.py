@@ -6,7 +6,8 @@ in src/mlweno/test/test_diffusiveflux.cpp: Lagrange derivative weights
 boundary sampling geometry and fluxes on rectangles and skewed quads
 (:103-167), physics and argument failures (:230-315) and the normal-sampling
 convergence order (:317-345).  Tolerances are the reference's own
-(Near: |a-b| <= tol*(1+|a|+|b|)).
+(Near: |a-b| <= tol*(1+|a|+|b|)).  The transport operator's preheat
+diffusion is pinned by closed forms on uniform rectangles (below).
 """
 import math
 
@@ -14,6 +15,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from mantlepar_amd import workloads as W
 
 ONE = ([0.0], [2.0])
 THREE = ([-math.sqrt(3 / 5), 0.0, math.sqrt(3 / 5)], [5 / 9, 8 / 9, 5 / 9])
@@ -189,3 +191,75 @@ def test_convergence_order(quad, boundary):
                 minimum = min(minimum, math.log2(previous / err))
             previous = err
         assert minimum > n - .45, (n, minimum)
+
+
+# ---------------------------------------------------------------- preheat operator
+# The legacy preheat diffusion (src/couple/preheat.cpp:176-190) samples each
+# side's cell average.  On a uniform rectangle with n = 4 samples and extent
+# 0.9 the reference's four-point weights (1/24, -9/8, 9/8, -1/24; checked
+# above) give an interior face flux kappa*len*(13/12)/(0.6 w)*(H_L - H_R), and
+# the end weights give a Dirichlet face kappa*len*(25/12)/(0.225 w)*(H_in - b)
+# (Hn = 1 + 1/2 + 1/3 + 1/4, ds = 0.9 w / 4): closed forms independent of the
+# oracle's geometric construction.
+
+
+
+def preheat(spec, kappa, types=None, values=None):
+    vx = np.zeros((spec.q, spec.ny, spec.nx + 1))
+    vy = np.zeros((spec.q, spec.ny + 1, spec.nx))
+    t = O.Transport(W.vertex_coords(spec), spec.nx, spec.ny, vx, vy, q=spec.q)
+    t.set_diffusion(kappa, 4, 0.9, types, values)
+    return t
+
+
+def closed_form_rhs(spec, H, kappa, bvals=None):
+    dx, dy = spec.dx, spec.dy
+    Dx = kappa * dy * (13 / 12) / (0.6 * dx)
+    Dy = kappa * dx * (13 / 12) / (0.6 * dy)
+    r = np.zeros_like(H)
+    r[:, 1:] += Dx * (H[:, :-1] - H[:, 1:])
+    r[:, :-1] -= Dx * (H[:, :-1] - H[:, 1:])
+    r[1:, :] += Dy * (H[:-1, :] - H[1:, :])
+    r[:-1, :] -= Dy * (H[:-1, :] - H[1:, :])
+    if bvals is not None:                      # prescribed temperature on the left side
+        Bx = kappa * dy * (25 / 12) / (0.225 * dx)
+        r[:, 0] -= Bx * (H[:, 0] - bvals)
+    return r / (dx * dy)
+
+
+def test_preheat_diffusion_closed_forms():
+    spec = W.MeshSpec(9, 7, seed=1)
+    rng = np.random.default_rng(4)
+    H = 0.5 + rng.random((spec.ny, spec.nx))
+    Cc = np.full_like(H, 0.2)
+    kappa = 3e-3
+    t = preheat(spec, kappa)
+    LH, LC, bh, _ = t.rhs(H, Cc)
+    want = closed_form_rhs(spec, H, kappa)
+    assert np.abs(LH - want).max() <= 1e-12 * np.abs(want).max()
+    assert np.all(LC == 0) and bh == 0.0
+    area = spec.dx * spec.dy
+    assert abs((LH * area).sum()) <= 1e-14 * np.abs(LH * area).sum()
+    # prescribed temperature on the left side (same value at every point),
+    # prescribed outward flux on the top side, zero flux elsewhere
+    q, nx, ny = spec.q, spec.nx, spec.ny
+    types = {"left": np.ones(ny), "top": np.full(nx, 2)}
+    b = 0.25 + 0.1 * np.arange(ny)
+    g = 0.07
+    values = np.zeros((2 * (nx + ny), q))
+    values[2 * nx + ny:] = b[:, None]
+    values[nx + ny:2 * nx + ny] = g
+    t2 = preheat(spec, kappa, types, values)
+    LH2, _, bh2, _ = t2.rhs(H, Cc)
+    want2 = closed_form_rhs(spec, H, kappa, bvals=b)
+    want2[-1, :] -= g * spec.dx / area
+    assert np.abs(LH2 - want2).max() <= 1e-12 * np.abs(want2).max()
+    Bx = kappa * spec.dy * (25 / 12) / (0.225 * spec.dx)
+    assert abs(bh2 - (Bx * (H[:, 0] - b).sum() + g * (spec.x1 - spec.x0))) <= 1e-12 * abs(bh2)
+    # rates: |sample partials| per side, D per interior face and B on the Dirichlet side
+    r, _ = t2.rates()
+    Dx = kappa * spec.dy * (13 / 12) / (0.6 * spec.dx)
+    Dy = kappa * spec.dx * (13 / 12) / (0.6 * spec.dy)
+    assert abs(r[3, 4] - (2 * Dx + 2 * Dy) / area) <= 1e-12 * r[3, 4]
+    assert abs(r[3, 0] - (Dx + 2 * Dy + Bx) / area) <= 1e-12 * r[3, 0]
+    assert abs(r[ny - 1, 4] - (2 * Dx + Dy) / area) <= 1e-12 * r[ny - 1, 4]
