@@ -221,6 +221,78 @@ def throughflow_boundary(spec, y_in=(0.9, 1.0), x_out=(0.0, 0.025), H_in=0.9, C_
     return bc, (np.full(n, H_in), np.full(n, C_in))
 
 
+def _flow_fields(spec, x, y, stokes, darcy, phi_max):
+    """Analytic flow at points: Stokes velocity of psi = stokes*sin(pi X)sin(pi Y)
+    (X, Y the unit coordinates), an upward Darcy flux whose x-component
+    vanishes on the side walls, and a flow porosity in [0, phi_max]."""
+    Lx, Ly = spec.x1 - spec.x0, spec.y1 - spec.y0
+    X, Y = (x - spec.x0) / Lx, (y - spec.y0) / Ly
+    # the walls carry exactly no Stokes flow (sin(pi) is not 0 in floating point)
+    sX = np.where(np.isclose(X, 0.0, atol=1e-12) | np.isclose(X, 1.0, atol=1e-12), 0.0,
+                  np.sin(np.pi * X))
+    sY = np.where(np.isclose(Y, 0.0, atol=1e-12) | np.isclose(Y, 1.0, atol=1e-12), 0.0,
+                  np.sin(np.pi * Y))
+    su = stokes * np.pi / Ly * sX * np.cos(np.pi * Y)
+    sv = -stokes * np.pi / Lx * np.cos(np.pi * X) * sY
+    du = 0.2 * darcy * np.sin(2 * np.pi * X) * np.cos(np.pi * Y)
+    dv = darcy * (1.0 + 0.5 * np.cos(np.pi * X))
+    phi = phi_max * (0.5 + 0.5 * np.sin(2 * np.pi * X) * np.cos(1.5 * np.pi * Y)) ** 2
+    return np.stack([su, sv], -1), np.stack([du, dv], -1), phi
+
+
+def coupled_flow(spec, stokes=0.05, darcy=0.05, phi_max=0.2, cell_points=5):
+    """Flow samples for the coupled law in the reference's FlowPointSample layout:
+    edges by id (horizontal first, j*nx+i; then vertical nx*(ny+1)+j*(nx+1)+i)
+    and canonical q (increasing x or y), cells by id and b*n+a."""
+    g, _ = gauss_legendre(spec.q)
+    t = (1.0 + g) / 2.0
+    nx, ny = spec.nx, spec.ny
+    jh, ih = np.meshgrid(np.arange(ny + 1), np.arange(nx), indexing="ij")
+    xh = spec.x0 + (ih[..., None] + t) * spec.dx
+    yh = np.broadcast_to((spec.y0 + jh * spec.dy)[..., None], xh.shape)
+    jv, iv = np.meshgrid(np.arange(ny), np.arange(nx + 1), indexing="ij")
+    yv = spec.y0 + (jv[..., None] + t) * spec.dy
+    xv = np.broadcast_to((spec.x0 + iv * spec.dx)[..., None], yv.shape)
+    x = np.concatenate([xh.reshape(-1, spec.q), xv.reshape(-1, spec.q)])
+    y = np.concatenate([yh.reshape(-1, spec.q), yv.reshape(-1, spec.q)])
+    es, ed, ep = _flow_fields(spec, x, y, stokes, darcy, phi_max)
+    gc, _ = gauss_legendre(cell_points)
+    tc = (1.0 + gc) / 2.0
+    jc, ic = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    xc = spec.x0 + (ic.reshape(-1, 1, 1) + tc[None, None, :]) * spec.dx
+    yc = spec.y0 + (jc.reshape(-1, 1, 1) + tc[None, :, None]) * spec.dy
+    xc, yc = np.broadcast_arrays(xc, yc)
+    cs, cd, cp = _flow_fields(spec, xc.reshape(nx * ny, -1), yc.reshape(nx * ny, -1), stokes,
+                              darcy, phi_max)
+    return dict(edge_stokes=es, edge_darcy=ed, edge_phi=ep, cell_stokes=cs, cell_darcy=cd,
+                cell_phi=cp)
+
+
+def coupled_boundary(spec, H_in=0.55, C_in=0.2, T_bottom=0.62):
+    """Boundary data of the coupled workload: the Darcy flux enters through the
+    bottom (inflow/outflow policy, prescribed H and C) and leaves through the
+    top (outflow only); the side walls carry no normal flow (zero flux).  The
+    bottom holds a prescribed temperature, the other sides are insulated.
+    Returns (policies, (inflow_H, inflow_C), thermal types, thermal values) in
+    the [side][face][q] layouts."""
+    nx, ny, q = spec.nx, spec.ny, spec.q
+    bc = {"bottom": np.full(nx, 2, np.uint8), "right": np.zeros(ny, np.uint8),
+          "top": np.ones(nx, np.uint8), "left": np.zeros(ny, np.uint8)}
+    n = 2 * (nx + ny) * q
+    ttypes = np.zeros(n, np.uint8)
+    tvals = np.zeros(n)
+    ttypes[:nx * q] = 1
+    tvals[:nx * q] = T_bottom
+    return bc, (np.full(n, H_in), np.full(n, C_in)), ttypes, tvals
+
+
+def coupled_state(spec, c_lo=0.05, c_hi=0.35):
+    """H of the synthetic configs (partly above the eutectic temperature 0.6)
+    and C seeded uniform in [c_lo, c_hi], inside (0, Xe) for Xe = 0.4."""
+    u = unit_uniform(spec.seed + 77, np.arange(spec.cells, dtype=np.uint64))
+    return enthalpy(spec), (c_lo + (c_hi - c_lo) * u).reshape(spec.ny, spec.nx)
+
+
 def refresh_amplitudes(count, seed=5, drift=0.1):
     """psi amplitude factor of each velocity refresh: 1 + drift*(2u - 1)."""
     return 1.0 + drift * (2.0 * unit_uniform(seed + 31337, np.arange(count)) - 1.0)

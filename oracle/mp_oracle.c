@@ -1363,6 +1363,7 @@ struct mpo_transport {
     double kappa, dif_extent;
     uint8_t* tbc[4];                  /* [face][q]: 0 zero flux, 1 temperature, 2 flux */
     double* tval[4];                  /* [face][q] canonical order */
+    uint8_t* bcC[4];                  /* advection policies of C (NULL: those of H) */
 };
 
 static size_t cidx(const mpo_transport* t, int i, int j) { return (size_t)j * t->nx + i; }
@@ -1373,7 +1374,7 @@ void mpo_transport_destroy(mpo_transport* t)
     free(t->coords); free(t->stencils); free(t->area); free(t->vnx); free(t->vny);
     for (int s = 0; s < 4; ++s) { free(t->bc[s]); free(t->inflow[0][s]); free(t->inflow[1][s]); }
     free(t->traces); free(t->fv); free(t->fh); free(t->theta[0]); free(t->theta[1]);
-    for (int s = 0; s < 4; ++s) { free(t->tbc[s]); free(t->tval[s]); }
+    for (int s = 0; s < 4; ++s) { free(t->tbc[s]); free(t->tval[s]); free(t->bcC[s]); }
     free(t);
 }
 
@@ -2059,6 +2060,477 @@ int mpo_transport_advance(mpo_transport* t, double* H, double* C, int nsteps, do
             totals[4 * s + 1] = mpo_transport_integral(t, C);
             totals[4 * s + 2] = bh_total;
             totals[4 * s + 3] = bc_total;
+        }
+    }
+    free(buf);
+    return e;
+}
+
+/* ================================================== coupled transport law */
+
+/* Separate advection policies for C (c.boundary.composition); NULL side: the H
+ * policy of that side. */
+int mpo_transport_set_composition_boundary(mpo_transport* t, const uint8_t* bottom,
+                                           const uint8_t* right, const uint8_t* top,
+                                           const uint8_t* left)
+{
+    const uint8_t* src[4] = {bottom, right, top, left};
+    for (int s = 0; s < 4; ++s) {
+        free(t->bcC[s]);
+        t->bcC[s] = NULL;
+        if (!src[s]) continue;
+        const int n = side_len(t, s);
+        t->bcC[s] = (uint8_t*)malloc((size_t)n);
+        if (!t->bcC[s]) return fail(MPO_ERR_MEM, "out of memory");
+        memcpy(t->bcC[s], src[s], (size_t)n);
+    }
+    return MPO_OK;
+}
+
+/* PhaseTransportVelocities, src/couple/evolve.cpp:36-52 */
+static void transport_velocities(mpo_point stokes, mpo_point darcy, double phi, double theta,
+                                 const mpo_phase_state* a, const mpo_phase_state* b,
+                                 mpo_point* mixture, mpo_point* effective);
+
+void mpo_phase_transport_velocities(const double solid[2], const double relative[2], double phi,
+                                    double theta, double cl_a, double cs_a, double cl_b,
+                                    double cs_b, double mixture[2], double effective[2])
+{
+    mpo_phase_state a, b;
+    memset(&a, 0, sizeof a);
+    memset(&b, 0, sizeof b);
+    a.cl = cl_a; a.cs = cs_a; b.cl = cl_b; b.cs = cs_b;
+    const mpo_point st = {{solid[0], solid[1]}}, dr = {{relative[0], relative[1]}};
+    mpo_point m, e;
+    transport_velocities(st, dr, phi, theta, &a, &b, &m, &e);
+    mixture[0] = m.p[0]; mixture[1] = m.p[1];
+    effective[0] = e.p[0]; effective[1] = e.p[1];
+}
+
+static void transport_velocities(mpo_point stokes, mpo_point darcy, double phi, double theta,
+                                 const mpo_phase_state* a, const mpo_phase_state* b,
+                                 mpo_point* mixture, mpo_point* effective)
+{
+    const double cl = .5 * (a->cl + b->cl), cs = .5 * (a->cs + b->cs);
+    const double bulk = phi * cl + (1 - phi) * cs;
+    const double factor = phi == 0 ? 0 : pow(phi, 1 + theta);
+    for (int k = 0; k < 2; ++k) {
+        const double flux = factor * darcy.p[k];
+        mixture->p[k] = stokes.p[k] + flux;
+        effective->p[k] = stokes.p[k] + (bulk > 0 ? cl / bulk * flux : 0);
+    }
+}
+
+typedef struct {
+    const mpo_transport* t;
+    const mpo_phase_model* pm;
+    const mpo_coupled_params* cp;
+    mpo_recon* rh;                /* [cell] limited reconstructions of H and C */
+    mpo_recon* rc;
+    const double *H, *C;
+    double* thH;                  /* [cell] limiter factors */
+    double* thC;
+} coupled_ctx;
+
+static int coupled_phase_at(const coupled_ctx* x, double H, double C, mpo_phase_state* s)
+{
+    return x->cp->hypereutectic ? mpo_phase_evaluate_ext(x->pm, H, C, x->cp->pressure, s)
+                                : mpo_phase_evaluate(x->pm, H, C, x->cp->pressure, s);
+}
+
+/* Phase of the limited reconstruction at p, src/couple/evolve.cpp:270-277 with
+ * StateReconstruction::Evaluate, src/couple/state_reconstruction.cpp:127-134 */
+static int coupled_phase(const coupled_ctx* x, int i, int j, mpo_point p, mpo_phase_state* s,
+                         double* Hp, double* Cp)
+{
+    const size_t c = cidx(x->t, i, j);
+    double h, cc;
+    CALL(mpo_recon_eval(&x->rh[c], p, 0, 0, &h));
+    CALL(mpo_recon_eval(&x->rc[c], p, 0, 0, &cc));
+    h = x->H[c] + x->thH[c] * (h - x->H[c]);
+    cc = x->C[c] + x->thC[c] * (cc - x->C[c]);
+    CALL(coupled_phase_at(x, h, cc, s));
+    CHECK(s->TDp >= 0 && s->phil < 1, MPO_ERR_ARG,
+          "Coupled Darcy-Stokes requires nonnegative temperature and porosity < 1");
+    if (Hp) *Hp = h;
+    if (Cp) *Cp = cc;
+    return MPO_OK;
+}
+
+static size_t edge_of(const mpo_transport* t, int vertical, int i, int j)
+{
+    return vertical ? (size_t)t->nx * (t->ny + 1) + (size_t)j * (t->nx + 1) + i
+                    : (size_t)j * t->nx + i;
+}
+
+/* One face of the coupled RHS, src/couple/evolve.cpp:291-354.  edge runs CCW
+ * for cell (li, lj); (ri, rj) = (-1, -1) on the boundary (side, face k, Gauss
+ * order reversed against the canonical one when rev).  eid: reference edge id. */
+static int coupled_face(const coupled_ctx* x, const mpo_flow* flow, const mpo_point edge[2],
+                        int li, int lj, int ri, int rj, int side, int k, int rev, size_t eid,
+                        double out[4])
+{
+    const mpo_transport* t = x->t;
+    const mpo_coupled_params* cp = x->cp;
+    const int nq = t->q, boundary = ri < 0;
+    double length; mpo_point normal;
+    CALL(mpo_edge_geometry(edge, &length, &normal));
+    double fluxH = 0, fluxC = 0, fluxMix = 0, rate = 0;
+    for (int q = 0; q < nq; ++q) {
+        const int qc = rev ? nq - 1 - q : q;
+        const size_t a_ = eid * nq + qc;
+        const mpo_point stokes = {{flow->edge_stokes[2 * a_], flow->edge_stokes[2 * a_ + 1]}};
+        const mpo_point darcy = {{flow->edge_darcy[2 * a_], flow->edge_darcy[2 * a_ + 1]}};
+        const double ephi = flow->edge_phi[a_];
+        const mpo_point p = mpo_map_edge_point(t->qp[q], edge);
+        mpo_phase_state a, b;
+        double hL, cL, hR, cR;
+        CALL(coupled_phase(x, li, lj, p, &a, &hL, &cL));
+        if (boundary) { b = a; hR = hL; cR = cL; }
+        else CALL(coupled_phase(x, ri, rj, p, &b, &hR, &cR));
+        (void)hR;
+        mpo_point mix, eff;
+        transport_velocities(stokes, darcy, ephi, cp->theta, &a, &b, &mix, &eff);
+        const double vn = mix.p[0] * normal.p[0] + mix.p[1] * normal.p[1];
+        const double ve = eff.p[0] * normal.p[0] + eff.p[1] * normal.p[1];
+        const double vs = stokes.p[0] * normal.p[0] + stokes.p[1] * normal.p[1];
+        const double alphaH = cp->lf_mode ? cp->global_speed : fabs(vn);
+        const double alphaC = cp->lf_mode ? cp->global_speed : fabs(ve);
+        CHECK(alphaH + 1e-14 >= fabs(vn) && alphaC + 1e-14 >= fabs(ve), MPO_ERR_ARG,
+              "Global LF speed is below the stage flow speed");
+        double fh = .5 * (vn * (a.TDp + b.TDp) - alphaH * (b.TDp - a.TDp)) - cp->LD * (1 - ephi) * vs;
+        double fc = .5 * (ve * (cL + cR) - alphaC * (cR - cL));
+        if (boundary) {
+            /* EvaluateAdvectionBoundary, src/couple/initialization.cpp:469-489 */
+            const int polH = t->bc[side][k];
+            const int polC = t->bcC[side] ? t->bcC[side][k] : polH;
+            int typeH, typeC;       /* 0 prescribed, 1 extrapolated, 2 zero flux */
+            double valH = 0, valC = 0;
+            for (int f = 0; f < 2; ++f) {
+                const int pol = f ? polC : polH;
+                const double speed = f ? ve : vn;
+                int* type = f ? &typeC : &typeH;
+                double* val = f ? &valC : &valH;
+                CHECK(isfinite(speed), MPO_ERR_ARG, "normal speed must be finite");
+                if (pol == MPO_BC_ZERO || speed == 0) *type = 2;
+                else if (speed < 0) {
+                    CHECK(pol == MPO_BC_INFLOW_OUTFLOW, MPO_ERR_BACKFLOW,
+                          "Backflow encountered on an outflow-only boundary.");
+                    const double* src = t->inflow[f][side];
+                    CHECK(src != NULL, MPO_ERR_ARG, "Missing inflow profile.");
+                    *type = 0;
+                    *val = src[(size_t)k * nq + qc];
+                } else *type = 1;
+            }
+            const double inflowC = typeC == 0 ? valC : cL;
+            double T = a.TDp;
+            if (typeH == 0) {
+                mpo_phase_state sb;
+                CALL(coupled_phase_at(x, valH, inflowC, &sb));
+                T = sb.TDp;
+            }
+            fh = typeH == 2 ? 0 : vn * T - cp->LD * (1 - ephi) * vs;
+            fc = typeC == 2 ? 0 : ve * (typeC == 0 ? valC : cL);
+        }
+        const double weight = .5 * length * t->qw[q];
+        fluxH += weight * fh;
+        fluxC += weight * fc;
+        fluxMix += weight * vn;
+        const double sensitivity = 1 + fmax(fabs(a.dTD_dCD), fabs(b.dTD_dCD));
+        CHECK(isfinite(sensitivity), MPO_ERR_FP, "Singular phase derivative in CFL estimate");
+        rate += weight * (alphaC + alphaH + 2 * fabs(vs)) * sensitivity;
+    }
+    /* normal-line diffusion of T, src/couple/evolve.cpp:334-347 */
+    mpo_point lc[4], rc[4];
+    mpo_cell_corners(&t->mesh, li, lj, lc);
+    if (!boundary) mpo_cell_corners(&t->mesh, ri, rj, rc);
+    mpo_sampling smp;
+    CALL(mpo_diffusive_sampling(edge, lc, boundary ? NULL : rc, nq, t->qp, t->qw, cp->samples,
+                                cp->extent, &smp));
+    double samples[16 * (MPO_MAX_SAMPLES + 1)], kappa[16], bv[16];
+    int bt[16];
+    for (int a = 0; a < smp.nq * smp.count; ++a) {
+        samples[a] = 0;
+        if (smp.sides[a] == MPO_SIDE_BOUNDARY) continue;
+        mpo_phase_state ps;
+        if (smp.sides[a] == MPO_SIDE_LEFT) CALL(coupled_phase(x, li, lj, smp.samples[a], &ps, NULL, NULL));
+        else CALL(coupled_phase(x, ri, rj, smp.samples[a], &ps, NULL, NULL));
+        samples[a] = ps.TDp;
+    }
+    for (int q = 0; q < nq; ++q) {
+        kappa[q] = cp->kappa;
+        if (boundary) {
+            const int qc = rev ? nq - 1 - q : q;
+            const int pol = t->tbc[side] ? t->tbc[side][(size_t)k * nq + qc] : 0;
+            bt[q] = pol == 1 ? MPO_DIFF_QUANTITY : (pol == 2 ? MPO_DIFF_FLUX : MPO_DIFF_ZERO_FLUX);
+            bv[q] = t->tval[side] ? t->tval[side][(size_t)k * nq + qc] : 0.0;
+        }
+    }
+    double dflux;
+    CALL(mpo_diffusive_flux(&smp, samples, kappa, boundary ? bt : NULL, boundary ? bv : NULL, &dflux));
+    fluxH += dflux;
+    /* |sample partials| (identity quantity), src/couple/evolve.cpp:348-350 */
+    long double last = 0;
+    for (int kk = 0; kk + 1 < smp.count; ++kk) last -= smp.coefficients[kk];
+    for (int q = 0; q < nq; ++q) {
+        if (boundary && bt[q] != MPO_DIFF_QUANTITY) continue;
+        const long double scale = -(long double)smp.weights[q] * kappa[q] / smp.spacing[q];
+        for (int kk = 0; kk < smp.count; ++kk) {
+            const int a = q * smp.count + kk;
+            if (smp.sides[a] == MPO_SIDE_BOUNDARY) continue;
+            const long double coeff = kk + 1 == smp.count ? last : smp.coefficients[kk];
+            double d;
+            CALL(real_value(scale * coeff, &d));
+            rate += fabs(d);
+        }
+    }
+    out[0] = fluxH; out[1] = fluxC; out[2] = fluxMix; out[3] = rate;
+    return MPO_OK;
+}
+
+int mpo_coupled_rhs(mpo_transport* t, const mpo_phase_model* pm, const mpo_coupled_params* cp,
+                    const mpo_flow* flow, const double* H, const double* C, double* rhsH,
+                    double* rhsC, double* rate, double* div, double* T, double* phi,
+                    mpo_coupled_rates* totals)
+{
+    CHECK(t->limit, MPO_ERR_STATE, "the coupled law needs the bound limiter (set_limiter)");
+    const int nx = t->nx, ny = t->ny;
+    const size_t N = (size_t)nx * ny;
+    coupled_ctx x = {t, pm, cp, NULL, NULL, H, C, NULL, NULL};
+    x.rh = (mpo_recon*)malloc(N * sizeof(mpo_recon));
+    x.rc = (mpo_recon*)malloc(N * sizeof(mpo_recon));
+    x.thH = (double*)malloc(N * sizeof(double));
+    x.thC = (double*)malloc(N * sizeof(double));
+    double* acc = (double*)calloc(N * 6, sizeof(double));  /* rhsH rhsC rate div T phi */
+    int e = 0;
+    if (!x.rh || !x.rc || !x.thH || !x.thC || !acc) e = fail(MPO_ERR_MEM, "out of memory");
+    /* Update + Limit, src/couple/state_reconstruction.cpp:101-192 */
+    for (size_t c = 0; c < N && !e; ++c) {
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        e = cell_recon(t, i, j, &x.rh[c]);
+        if (!e) e = mpo_recon_update(&x.rh[c], H, 0, 0, nx, ny, t->area[c]);
+        if (!e) e = cell_recon(t, i, j, &x.rc[c]);
+        if (!e) e = mpo_recon_update(&x.rc[c], C, 0, 0, nx, ny, t->area[c]);
+        if (!e) e = cell_limit(t, &x.rh[c], H, i, j, INFINITY, &x.thH[c]);
+        if (!e) e = cell_limit(t, &x.rc[c], C, i, j, t->lim_cupper, &x.thC[c]);
+    }
+    double bH = 0, bC = 0, sH = 0;
+    /* faces: vertical x_i (left cell (i-1, j)), then horizontal y_j (left (i, j-1)) */
+    for (int j = 0; j < ny && !e; ++j)
+        for (int i = 0; i <= nx && !e; ++i) {
+            double o[4];
+            mpo_point c4[4], edge[2];
+            const size_t eid = edge_of(t, 1, i, j);
+            if (i > 0 && i < nx) {
+                mpo_cell_corners(&t->mesh, i - 1, j, c4);
+                edge[0] = c4[1]; edge[1] = c4[2];
+                e = coupled_face(&x, flow, edge, i - 1, j, i, j, 0, 0, 0, eid, o);
+                if (e) break;
+                const size_t L = cidx(t, i - 1, j), R = cidx(t, i, j);
+                acc[L * 6] -= o[0] / t->area[L]; acc[R * 6] += o[0] / t->area[R];
+                acc[L * 6 + 1] -= o[1] / t->area[L]; acc[R * 6 + 1] += o[1] / t->area[R];
+                acc[L * 6 + 2] += o[3] / t->area[L]; acc[R * 6 + 2] += o[3] / t->area[R];
+                acc[L * 6 + 3] += o[2] / t->area[L]; acc[R * 6 + 3] -= o[2] / t->area[R];
+            } else {
+                const int ci = i == 0 ? 0 : nx - 1, side = i == 0 ? 3 : 1;
+                mpo_cell_corners(&t->mesh, ci, j, c4);
+                if (side == 1) { edge[0] = c4[1]; edge[1] = c4[2]; }
+                else { edge[0] = c4[3]; edge[1] = c4[0]; }
+                e = coupled_face(&x, flow, edge, ci, j, -1, -1, side, j, side == 3, eid, o);
+                if (e) break;
+                const size_t L = cidx(t, ci, j);
+                acc[L * 6] -= o[0] / t->area[L]; acc[L * 6 + 1] -= o[1] / t->area[L];
+                acc[L * 6 + 2] += o[3] / t->area[L]; acc[L * 6 + 3] += o[2] / t->area[L];
+                bH += o[0]; bC += o[1];
+            }
+        }
+    for (int j = 0; j <= ny && !e; ++j)
+        for (int i = 0; i < nx && !e; ++i) {
+            double o[4];
+            mpo_point c4[4], edge[2];
+            const size_t eid = edge_of(t, 0, i, j);
+            if (j > 0 && j < ny) {
+                /* the reference's edge runs from (i, j) to (i+1, j) with the
+                 * lower cell on its right; orient it CCW for the lower cell */
+                mpo_cell_corners(&t->mesh, i, j - 1, c4);
+                edge[0] = c4[2]; edge[1] = c4[3];
+                e = coupled_face(&x, flow, edge, i, j - 1, i, j, 0, 0, 1, eid, o);
+                if (e) break;
+                const size_t L = cidx(t, i, j - 1), R = cidx(t, i, j);
+                acc[L * 6] -= o[0] / t->area[L]; acc[R * 6] += o[0] / t->area[R];
+                acc[L * 6 + 1] -= o[1] / t->area[L]; acc[R * 6 + 1] += o[1] / t->area[R];
+                acc[L * 6 + 2] += o[3] / t->area[L]; acc[R * 6 + 2] += o[3] / t->area[R];
+                acc[L * 6 + 3] += o[2] / t->area[L]; acc[R * 6 + 3] -= o[2] / t->area[R];
+            } else {
+                const int cj = j == 0 ? 0 : ny - 1, side = j == 0 ? 0 : 2;
+                mpo_cell_corners(&t->mesh, i, cj, c4);
+                if (side == 0) { edge[0] = c4[0]; edge[1] = c4[1]; }
+                else { edge[0] = c4[2]; edge[1] = c4[3]; }
+                e = coupled_face(&x, flow, edge, i, cj, -1, -1, side, i, side == 2, eid, o);
+                if (e) break;
+                const size_t L = cidx(t, i, cj);
+                acc[L * 6] -= o[0] / t->area[L]; acc[L * 6 + 1] -= o[1] / t->area[L];
+                acc[L * 6 + 2] += o[3] / t->area[L]; acc[L * 6 + 3] += o[2] / t->area[L];
+                bH += o[0]; bC += o[1];
+            }
+        }
+    /* cell quadrature: cooling source and phase diagnostics, src/couple/evolve.cpp:357-370 */
+    rule1d cell;
+    if (!e) e = make_rule(cp->cell_points, &cell);
+    const int np = cell.n * cell.n;
+    for (size_t c = 0; c < N && !e; ++c) {
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        mpo_point c4[4];
+        mpo_cell_corners(&t->mesh, i, j, c4);
+        for (int b = 0; b < cell.n && !e; ++b)
+            for (int a = 0; a < cell.n && !e; ++a) {
+                const size_t k = c * np + (size_t)b * cell.n + a;
+                const mpo_point ref = {{cell.x[a], cell.x[b]}};
+                const mpo_point p = mpo_map_cell_point(ref, c4);
+                const double weight = cell.w[a] * cell.w[b] * mpo_cell_jacobian(ref, c4);
+                mpo_phase_state ps;
+                e = coupled_phase(&x, i, j, p, &ps, NULL, NULL);
+                if (e) break;
+                const mpo_point stokes = {{flow->cell_stokes[2 * k], flow->cell_stokes[2 * k + 1]}};
+                const mpo_point darcy = {{flow->cell_darcy[2 * k], flow->cell_darcy[2 * k + 1]}};
+                mpo_point mix, eff;
+                transport_velocities(stokes, darcy, flow->cell_phi[k], cp->theta, &ps, &ps, &mix, &eff);
+                const double source = -cp->cooling * ps.TDp * mix.p[1];
+                const double wa = weight / t->area[c];
+                acc[c * 6] += wa * source;
+                acc[c * 6 + 4] += wa * ps.TDp;
+                acc[c * 6 + 5] += wa * ps.phil;
+                acc[c * 6 + 2] += wa * cp->cooling * fabs(mix.p[1]);
+                sH += weight * source;
+            }
+    }
+    double mr = 0, md = 0;
+    for (size_t c = 0; c < N && !e; ++c) {
+        mr = fmax(mr, acc[c * 6 + 2]);
+        md = fmax(md, fabs(acc[c * 6 + 3]));
+    }
+    if (!e && !isfinite(mr)) e = fail(MPO_ERR_FP, "Nonfinite coupled stage rate");
+    if (!e) {
+        for (size_t c = 0; c < N; ++c) {
+            if (rhsH) rhsH[c] = acc[c * 6];
+            if (rhsC) rhsC[c] = acc[c * 6 + 1];
+            if (rate) rate[c] = acc[c * 6 + 2];
+            if (div) div[c] = acc[c * 6 + 3];
+            if (T) T[c] = acc[c * 6 + 4];
+            if (phi) phi[c] = acc[c * 6 + 5];
+        }
+        totals->boundaryH = bH; totals->boundaryC = bC; totals->sourceH = sH;
+        totals->maximumRate = mr; totals->divergence = md;
+    }
+    free(x.rh); free(x.rc); free(x.thH); free(x.thC); free(acc);
+    return e;
+}
+
+/* Admissible, src/couple/evolve.cpp:386-393 */
+static int coupled_admissible(size_t N, const double* H, const double* C, double Xe)
+{
+    double hmin = INFINITY, hmax = -INFINITY, cmin = INFINITY, cmax = -INFINITY;
+    for (size_t k = 0; k < N; ++k) {
+        hmin = fmin(hmin, H[k]); hmax = fmax(hmax, H[k]);
+        cmin = fmin(cmin, C[k]); cmax = fmax(cmax, C[k]);
+        if (!isfinite(H[k]) || !isfinite(C[k])) return 0;
+    }
+    return isfinite(hmin) && isfinite(hmax) && isfinite(cmin) && isfinite(cmax) && hmin >= 0 &&
+           cmin >= 0 && cmax <= Xe;
+}
+
+static double coupled_integral(const mpo_transport* t, const double* u)
+{
+    long double s = 0;
+    for (size_t k = 0; k < (size_t)t->nx * t->ny; ++k) s += (long double)u[k] * t->area[k];
+    return (double)s;
+}
+
+/* Run, src/couple/evolve.cpp:411-507: SSPRK2 steps with admissibility and CFL
+ * retries (dt halved, at most 40) and the conservation balance check; the flow
+ * samples stay fixed (the flow solve of every RHS is outside this path).
+ * H, C advance in place over nsteps accepted steps from time *time. */
+int mpo_coupled_run(mpo_transport* t, const mpo_phase_model* pm, const mpo_coupled_params* cp,
+                    const mpo_flow* flow, const mpo_coupled_time* tc, double* H, double* C,
+                    double* time, int nsteps, mpo_coupled_step* steps)
+{
+    const size_t N = (size_t)t->nx * t->ny;
+    CHECK(tc->cfl > 0 && tc->cfl <= 1, MPO_ERR_ARG, "Coupled CFL must be in (0,1]");
+    double* buf = (double*)malloc(N * 8 * sizeof(double));
+    if (!buf) return fail(MPO_ERR_MEM, "out of memory");
+    double *oldH = buf, *oldC = buf + N, *rH = buf + 2 * N, *rC = buf + 3 * N;
+    double *fH = buf + 4 * N, *fC = buf + 5 * N;
+    mpo_coupled_rates current, second;
+    int e = mpo_coupled_rhs(t, pm, cp, flow, H, C, rH, rC, NULL, NULL, NULL, NULL, &current);
+    memcpy(oldH, H, N * sizeof(double)); memcpy(oldC, C, N * sizeof(double));
+    const double Xe = cp->admissible_c_max;
+    int accepted = 0;
+    for (int s = 0; s < nsteps && !e; ++s) {
+        const double remaining = tc->end - *time;
+        if (remaining <= 32 * DBL_EPSILON * fmax(1.0, fabs(tc->end))) { *time = tc->end; break; }
+        const double oldHi = coupled_integral(t, H), oldCi = coupled_integral(t, C);
+        const double start = *time;
+        double dt = fmin(remaining, tc->fixed ? tc->initial : (accepted == 0 ? tc->initial : tc->maximum));
+        const double limit = current.maximumRate > 0 ? tc->cfl / current.maximumRate : tc->maximum;
+        if (tc->fixed && !(dt <= limit * (1 + 1e-12))) {
+            e = fail(MPO_ERR_ARG, "Fixed time step exceeds the phase-coupled CFL bound; use control: cfl");
+            break;
+        }
+        if (!tc->fixed) dt = fmin(dt, limit);
+        memcpy(fH, rH, N * sizeof(double)); memcpy(fC, rC, N * sizeof(double));
+        const mpo_coupled_rates first = current;
+        int retries = 0;
+        double courant = 0;
+        for (;;) {
+            if (!(dt >= fmin(remaining, tc->minimum) && retries < 40)) {
+                e = fail(MPO_ERR_RUNTIME, "Coupled step cannot satisfy admissibility/CFL above minimum step");
+                break;
+            }
+            for (size_t k = 0; k < N; ++k) { H[k] = oldH[k] + dt * fH[k]; C[k] = oldC[k] + dt * fC[k]; }
+            *time = start + dt;
+            int valid = coupled_admissible(N, H, C, Xe);
+            if (valid) {
+                e = mpo_coupled_rhs(t, pm, cp, flow, H, C, rH, rC, NULL, NULL, NULL, NULL, &second);
+                if (e) break;
+                courant = dt * fmax(current.maximumRate, second.maximumRate);
+                valid = courant <= tc->cfl * (1 + 1e-12);
+            }
+            if (valid) {
+                for (size_t k = 0; k < N; ++k) {
+                    H[k] = .5 * (H[k] + dt * rH[k]) + .5 * oldH[k];
+                    C[k] = .5 * (C[k] + dt * rC[k]) + .5 * oldC[k];
+                }
+                valid = coupled_admissible(N, H, C, Xe);
+            }
+            if (valid) break;
+            if (tc->fixed) { e = fail(MPO_ERR_ARG, "Fixed coupled step failed stage admissibility/CFL"); break; }
+            dt *= .5;
+            ++retries;
+        }
+        double balH = 0, balC = 0;
+        if (!e) {
+            const double Hi = coupled_integral(t, H), Ci = coupled_integral(t, C);
+            balH = Hi - oldHi + .5 * dt * (current.boundaryH + second.boundaryH - current.sourceH - second.sourceH);
+            balC = Ci - oldCi + .5 * dt * (current.boundaryC + second.boundaryC);
+            if (!(fabs(balH) < 1e-10 * fmax(1.0, fabs(oldHi)) && fabs(balC) < 1e-10))
+                e = fail(MPO_ERR_RUNTIME, "Coupled finite-volume conservation balance failed");
+        }
+        if (!e) e = mpo_coupled_rhs(t, pm, cp, flow, H, C, rH, rC, NULL, NULL, NULL, NULL, &current);
+        if (e) {
+            *time = start;
+            memcpy(H, oldH, N * sizeof(double)); memcpy(C, oldC, N * sizeof(double));
+            break;
+        }
+        ++accepted;
+        memcpy(oldH, H, N * sizeof(double)); memcpy(oldC, C, N * sizeof(double));
+        if (steps) {
+            mpo_coupled_step* r = &steps[s];
+            r->dt = dt; r->courant = courant; r->retries = retries;
+            r->balanceH = balH; r->balanceC = balC;
+            r->boundaryH = .5 * (first.boundaryH + second.boundaryH);
+            r->boundaryC = .5 * (first.boundaryC + second.boundaryC);
+            r->sourceH = .5 * (first.sourceH + second.sourceH);
+            r->maximumRate = current.maximumRate;
         }
     }
     free(buf);

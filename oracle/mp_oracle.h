@@ -284,6 +284,66 @@ int mpo_transport_advance(mpo_transport* t, double* H, double* C, int nsteps, do
 /* Compensated (Neumaier) sum of area*u in fixed order. */
 double mpo_transport_integral(const mpo_transport* t, const double* u);
 
+/* --------------------------------------------------- coupled transport law */
+/* Parameters of the coupled RHS (src/couple/evolve.cpp:279-384). */
+typedef struct {
+    double LD;               /* c.phase.derived().LD */
+    double theta;            /* c.flow.material.theta (Darcy flux phi^(1+theta)) */
+    double cooling;          /* alpha0*l0*g/cp */
+    double kappa;            /* c.thermalDiffusivity */
+    int lf_mode;             /* 0 local, 1 global Lax-Friedrichs */
+    double global_speed;
+    int cell_points;         /* c.input.quadrature.cellPointsPerAxis */
+    int samples;             /* c.sampling.numberOfSamples */
+    double extent;           /* c.sampling.extentFraction */
+    double pressure;         /* constant phase pressure [Pa] */
+    int hypereutectic;       /* evaluate C > Xe on the mirrored diagram */
+    double admissible_c_max; /* Admissible: C <= this (Xe) */
+} mpo_coupled_params;
+
+/* Flow samples (FlowPointSample, src/couple/visualization.h:8-12), fixed over
+ * the call.  Edge arrays by reference edge id (horizontal first, j*nx+i; then
+ * vertical nx*(ny+1) + j*(nx+1) + i) and canonical q: stokes/darcy [e][q][2],
+ * phi [e][q].  Cell arrays [cell][b*n+a][2] / [cell][b*n+a] for the cell
+ * Gauss points (a along x, b along y, n = cell_points). */
+typedef struct {
+    const double *edge_stokes, *edge_darcy, *edge_phi;
+    const double *cell_stokes, *cell_darcy, *cell_phi;
+} mpo_flow;
+
+typedef struct { double boundaryH, boundaryC, sourceH, maximumRate, divergence; } mpo_coupled_rates;
+
+/* time.* controls of the coupled loop (src/input/input.h) */
+typedef struct { double cfl, initial, maximum, minimum, end; int fixed; } mpo_coupled_time;
+
+typedef struct {
+    double dt, courant, balanceH, balanceC, boundaryH, boundaryC, sourceH, maximumRate;
+    int retries;
+} mpo_coupled_step;
+
+/* PhaseTransportVelocities (src/couple/evolve.cpp:36-52) on two phase states
+ * given by their cl, cs. */
+void mpo_phase_transport_velocities(const double solid[2], const double relative[2], double phi,
+                                    double theta, double cl_a, double cs_a, double cl_b,
+                                    double cs_b, double mixture[2], double effective[2]);
+/* Advection policies of C when they differ from those of H. */
+int mpo_transport_set_composition_boundary(mpo_transport* t, const uint8_t* bottom,
+                                           const uint8_t* right, const uint8_t* top,
+                                           const uint8_t* left);
+/* One coupled RHS (src/couple/evolve.cpp:279-384) on the limited
+ * reconstruction (the transport's limiter must be on; C bound = c_upper):
+ * rhsH, rhsC, per-cell rate and mixture divergence, cell-averaged T and phi
+ * (any may be NULL) and the totals.  Thermal boundary data: set_diffusion's
+ * types/values; inflow: the transport's inflow arrays. */
+int mpo_coupled_rhs(mpo_transport* t, const mpo_phase_model* pm, const mpo_coupled_params* cp,
+                    const mpo_flow* flow, const double* H, const double* C, double* rhsH,
+                    double* rhsC, double* rate, double* div, double* T, double* phi,
+                    mpo_coupled_rates* totals);
+/* SSPRK2 steps of src/couple/evolve.cpp:411-507 with the flow held fixed. */
+int mpo_coupled_run(mpo_transport* t, const mpo_phase_model* pm, const mpo_coupled_params* cp,
+                    const mpo_flow* flow, const mpo_coupled_time* tc, double* H, double* C,
+                    double* time, int nsteps, mpo_coupled_step* steps);
+
 /* ------------------------------------------------------------ workloads */
 /* SplitMix64 finaliser of src/core/mesh.cpp:12-18. */
 uint64_t mpo_mix64(uint64_t v);

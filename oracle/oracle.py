@@ -14,7 +14,7 @@ __all__ = [
     "ref_phase_available", "ref_phase_evaluate", "ref_phase_derived", "WenoOptions",
     "StencilPoly", "Recon", "llf_point", "integrate_edge_flux", "REGION",
     "lagrange_weights", "Sampling", "diffusive_sampling", "diffusive_flux", "LAGRANGE",
-    "SIDE", "DIFF_BC", "MAX_SAMPLES",
+    "SIDE", "DIFF_BC", "MAX_SAMPLES", "CoupledParams", "transport_velocities", "edge_count",
 ]
 
 LAGRANGE = {"Middle": 0, "EndLo": 1, "EndHi": 2}
@@ -177,6 +177,66 @@ class Sampling(C.Structure):
         return np.array([[self.points[q].p[0], self.points[q].p[1]] for q in range(self.nq)])
 
 
+class CoupledParams(C.Structure):
+    """Parameters of the coupled RHS (mpo_coupled_params)."""
+    _fields_ = [("LD", C.c_double), ("theta", C.c_double), ("cooling", C.c_double),
+                ("kappa", C.c_double), ("lf_mode", C.c_int), ("global_speed", C.c_double),
+                ("cell_points", C.c_int), ("samples", C.c_int), ("extent", C.c_double),
+                ("pressure", C.c_double), ("hypereutectic", C.c_int),
+                ("admissible_c_max", C.c_double)]
+
+    @classmethod
+    def make(cls, **kw):
+        d = dict(LD=1.0, theta=1.0, cooling=0.0, kappa=0.0, lf_mode=0, global_speed=0.0,
+                 cell_points=5, samples=4, extent=0.9, pressure=0.0, hypereutectic=0,
+                 admissible_c_max=0.4)
+        d.update(kw)
+        return cls(**d)
+
+
+class _Flow(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("edge_stokes", "edge_darcy", "edge_phi", "cell_stokes",
+                                          "cell_darcy", "cell_phi")]
+
+
+class _CoupledRates(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("boundaryH", "boundaryC", "sourceH", "maximumRate",
+                                          "divergence")]
+
+
+class _CoupledTime(C.Structure):
+    _fields_ = [("cfl", C.c_double), ("initial", C.c_double), ("maximum", C.c_double),
+                ("minimum", C.c_double), ("end", C.c_double), ("fixed", C.c_int)]
+
+
+class _CoupledStep(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("dt", "courant", "balanceH", "balanceC", "boundaryH",
+                                          "boundaryC", "sourceH", "maximumRate")] + \
+               [("retries", C.c_int)]
+
+
+def edge_count(nx, ny):
+    return nx * (ny + 1) + (nx + 1) * ny
+
+
+def _flow(flow):
+    """flow: dict of arrays edge_stokes [E][q][2], edge_darcy, edge_phi [E][q],
+    cell_stokes [N][n*n][2], cell_darcy, cell_phi [N][n*n]."""
+    keep = {k: np.ascontiguousarray(flow[k], dtype=np.float64) for k in
+            ("edge_stokes", "edge_darcy", "edge_phi", "cell_stokes", "cell_darcy", "cell_phi")}
+    f = _Flow(**{k: v.ctypes.data_as(C.c_void_p) for k, v in keep.items()})
+    return f, keep
+
+
+def transport_velocities(solid, relative, phi, theta, a, b):
+    """PhaseTransportVelocities (src/couple/evolve.cpp:36-52); a, b = (cl, cs)."""
+    m = (C.c_double * 2)()
+    e = (C.c_double * 2)()
+    lib().mpo_phase_transport_velocities((C.c_double * 2)(*solid), (C.c_double * 2)(*relative),
+                                         phi, theta, a[0], a[1], b[0], b[1], m, e)
+    return np.array(m[:]), np.array(e[:])
+
+
 def _declare(L):
     L.mpo_last_error.restype = C.c_char_p
     L.mpo_gauss_rule.argtypes = [C.c_int, _dp, _dp]
@@ -235,6 +295,18 @@ def _declare(L):
     L.mpo_transport_traces.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.mpo_transport_set_diffusion.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double,
                                               C.c_void_p, C.c_void_p]
+    L.mpo_phase_transport_velocities.argtypes = [C.c_double * 2, C.c_double * 2, C.c_double,
+                                                 C.c_double, C.c_double, C.c_double,
+                                                 C.c_double, C.c_double, C.c_double * 2,
+                                                 C.c_double * 2]
+    L.mpo_phase_transport_velocities.restype = None
+    L.mpo_transport_set_composition_boundary.argtypes = [C.c_void_p] + [C.c_void_p] * 4
+    L.mpo_coupled_rhs.argtypes = [C.c_void_p, C.POINTER(_PhaseModel), C.POINTER(CoupledParams),
+                                  C.POINTER(_Flow), _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                  C.POINTER(_CoupledRates)]
+    L.mpo_coupled_run.argtypes = [C.c_void_p, C.POINTER(_PhaseModel), C.POINTER(CoupledParams),
+                                  C.POINTER(_Flow), C.POINTER(_CoupledTime), _dp, _dp,
+                                  C.POINTER(C.c_double), C.c_int, C.POINTER(_CoupledStep)]
     L.mpo_mix64.argtypes = [C.c_uint64]
     L.mpo_mix64.restype = C.c_uint64
 
@@ -609,6 +681,45 @@ class Transport:
             self._h, kappa, samples, extent,
             t.ctypes.data_as(C.c_void_p) if t is not None else None,
             v.ctypes.data_as(C.c_void_p) if v is not None else None))
+
+    def set_composition_boundary(self, bc):
+        keep, ptrs = [], []
+        for side in ("bottom", "right", "top", "left"):
+            arr, ptr = _u8_or_none(bc.get(side))
+            keep.append(arr)
+            ptrs.append(ptr)
+        self._bcC_keep = keep
+        _check(lib().mpo_transport_set_composition_boundary(self._h, *ptrs))
+
+    def coupled_rhs(self, phase, params, flow, H, Cc):
+        """One coupled RHS (src/couple/evolve.cpp:279-384); phase: PhaseModel."""
+        f, keep = _flow(flow)
+        out = {k: np.zeros(self.N) for k in ("rhsH", "rhsC", "rate", "div", "T", "phi")}
+        r = _CoupledRates()
+        _check(lib().mpo_coupled_rhs(self._h, C.byref(phase._m), C.byref(params), C.byref(f),
+                                     np.ascontiguousarray(H, dtype=np.float64).ravel(),
+                                     np.ascontiguousarray(Cc, dtype=np.float64).ravel(),
+                                     out["rhsH"], out["rhsC"], out["rate"], out["div"],
+                                     out["T"], out["phi"], C.byref(r)))
+        del keep
+        res = {k: v.reshape(self.ny, self.nx) for k, v in out.items()}
+        res.update({n: getattr(r, n) for n, _ in _CoupledRates._fields_})
+        return res
+
+    def coupled_run(self, phase, params, flow, H, Cc, nsteps, cfl=0.5, initial=1e-2,
+                    maximum=1e-2, minimum=1e-12, end=1e30, fixed=False, time=0.0):
+        """SSPRK2 steps with retries (src/couple/evolve.cpp:411-507), flow fixed."""
+        f, keep = _flow(flow)
+        tc = _CoupledTime(cfl, initial, maximum, minimum, end, int(fixed))
+        H = np.array(H, dtype=np.float64).ravel()
+        Cc = np.array(Cc, dtype=np.float64).ravel()
+        t = C.c_double(time)
+        steps = (_CoupledStep * max(nsteps, 1))()
+        _check(lib().mpo_coupled_run(self._h, C.byref(phase._m), C.byref(params), C.byref(f),
+                                     C.byref(tc), H, Cc, C.byref(t), nsteps, steps))
+        del keep
+        rec = [{n: getattr(st, n) for n, _ in _CoupledStep._fields_} for st in steps[:nsteps]]
+        return H.reshape(self.ny, self.nx), Cc.reshape(self.ny, self.nx), t.value, rec
 
     def traces(self, u, field=0):
         tr = np.zeros(self.N * 4 * self.q)
