@@ -110,6 +110,7 @@ typedef struct {
 typedef struct mp_transport mp_transport;
 typedef struct mp_batch mp_batch;
 typedef struct mp_server mp_server;
+typedef struct mp_coupled mp_coupled;
 
 /* Defaults: production options and the reference's default material. */
 void mp_default_weno_options(mp_weno_options* o);
@@ -361,6 +362,75 @@ typedef struct {
  * sample of an (edge, q) wins.  Every face point must be sampled. */
 mp_status mp_face_velocity_from_samples(int32_t nx, int32_t ny, int32_t q, int64_t nsamples,
                                         const mp_edge_sample* samples, double* vnx, double* vny);
+
+/* ---------------------------------------------------- coupled transport --- */
+/* The phase-coupled transport law of MantlePar's evolution (one rank's share
+ * of a uniform mesh, FP64): the RHS of src/couple/evolve.cpp:279-384 and the
+ * SSPRK2 loop with admissibility/CFL retries of :411-507.  Traces come from
+ * the limited reconstruction (StateReconstruction::Update + Limit); H carries
+ * the temperature flux with the latent term -LD*(1-phi)*v_s.n, C the
+ * effective velocity of PhaseTransportVelocities (:36-52); T diffuses by
+ * normal-line samples of the reconstruction; the cell Gauss points carry the
+ * cooling source.  The Stokes-Darcy solve stays with the caller: the flow
+ * samples are inputs, held fixed over a call (the reference re-solves the
+ * flow inside every RHS, SolveReconstructedFlow at :282). */
+typedef struct {
+    double theta;                 /* c.flow.material.theta: Darcy flux phi^(1+theta) u_d */
+    double cooling;               /* alpha0*l0*g/cp (evolve.cpp:285) */
+    double thermal_diffusivity;   /* c.thermalDiffusivity */
+    int32_t cell_points_per_axis; /* c.input.quadrature.cellPointsPerAxis (5) */
+    int32_t diffusion_samples;    /* c.sampling.numberOfSamples (4) */
+    double extent_fraction;       /* c.sampling.extentFraction (0.9) */
+    int32_t lf_mode;              /* c.stabilizer.mode: 0 local, 1 global */
+    double global_speed;          /* c.stabilizer.globalSpeed */
+} mp_coupled_params;
+
+typedef struct {
+    double boundary_H, boundary_C;   /* outward boundary flux totals */
+    double source_H;                 /* cooling source total */
+    double maximum_rate;             /* max per-cell CFL rate */
+    double divergence;               /* max |mixture divergence| */
+} mp_coupled_rates;
+
+/* input time controls (src/input/input.h: time.step.*) */
+typedef struct {
+    double cfl, initial_step, maximum_step, minimum_step, end_time;
+    int32_t fixed;                /* control "fixed" (1) or "cfl" (0) */
+} mp_coupled_time;
+
+/* One accepted step, as the history row of src/couple/evolve.cpp:394-409. */
+typedef struct {
+    double dt, courant, balance_H, balance_C, boundary_H, boundary_C, source_H, maximum_rate;
+    int32_t retries, pad;
+} mp_coupled_step;
+
+void mp_default_coupled_params(mp_coupled_params* p);
+mp_status mp_coupled_create(const mp_mesh_desc* mesh, const mp_phase_params* phase,
+                            const mp_coupled_params* params, void* stream, mp_coupled** out);
+mp_status mp_coupled_destroy(mp_coupled* c);
+/* FlowPointSample arrays (src/couple/visualization.h:8-12): edges by
+ * reference edge id (horizontal first, j*nx+i; then vertical) and canonical q,
+ * stokes/darcy [e][q][2], phi [e][q]; cell points [cell][b*n+a][2] / [..]. */
+mp_status mp_coupled_set_flow(mp_coupled* c, const double* edge_stokes, const double* edge_darcy,
+                              const double* edge_phi, const double* cell_stokes,
+                              const double* cell_darcy, const double* cell_phi);
+/* Advection policies (0 zero flux, 1 outflow only, 2 inflow/outflow) of H
+ * (field 0, c.boundary.enthalpy) or C (field 1, c.boundary.composition). */
+mp_status mp_coupled_set_boundary(mp_coupled* c, int32_t field, const uint8_t* bottom,
+                                  const uint8_t* right, const uint8_t* top, const uint8_t* left);
+mp_status mp_coupled_set_inflow(mp_coupled* c, const double* inflow_H, const double* inflow_C);
+/* EvaluateThermalBoundary data per boundary point, as mp_transport_set_thermal_boundary. */
+mp_status mp_coupled_set_thermal_boundary(mp_coupled* c, const uint8_t* types, const double* values);
+mp_status mp_coupled_upload_state(mp_coupled* c, const double* H, const double* C);
+mp_status mp_coupled_download_state(mp_coupled* c, double* H, double* C);
+/* RHS of the current state: rhs_H, rhs_C, rate, divergence, cell-averaged T
+ * and phi [N] (any may be NULL) and the totals (RHS, evolve.cpp:279-384). */
+mp_status mp_coupled_rhs(mp_coupled* c, double* rhsH, double* rhsC, double* rate, double* div,
+                         double* T, double* phi, mp_coupled_rates* totals);
+/* Up to nsteps accepted SSPRK2 steps from *time (Run, evolve.cpp:411-507):
+ * on an error the state and *time are those of the last accepted step. */
+mp_status mp_coupled_run(mp_coupled* c, const mp_coupled_time* time_controls, double* time,
+                         int32_t nsteps, mp_coupled_step* steps, int32_t* accepted);
 
 /* ------------------------------------------------------------- phase only --- */
 /* EutecticModel::evaluate over n cells on the device (host arrays in/out).

@@ -99,6 +99,29 @@ class EdgeSample(C.Structure):
     _fields_ = [("edge_id", C.c_int64), ("q", C.c_int32), ("vx", C.c_double), ("vy", C.c_double)]
 
 
+class CoupledParams(C.Structure):
+    _fields_ = [("theta", C.c_double), ("cooling", C.c_double),
+                ("thermal_diffusivity", C.c_double), ("cell_points_per_axis", C.c_int32),
+                ("diffusion_samples", C.c_int32), ("extent_fraction", C.c_double),
+                ("lf_mode", C.c_int32), ("global_speed", C.c_double)]
+
+
+class CoupledRates(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("boundary_H", "boundary_C", "source_H",
+                                          "maximum_rate", "divergence")]
+
+
+class CoupledTime(C.Structure):
+    _fields_ = [("cfl", C.c_double), ("initial_step", C.c_double), ("maximum_step", C.c_double),
+                ("minimum_step", C.c_double), ("end_time", C.c_double), ("fixed", C.c_int32)]
+
+
+class CoupledStep(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("dt", "courant", "balance_H", "balance_C", "boundary_H",
+                                          "boundary_C", "source_H", "maximum_rate")] + \
+               [("retries", C.c_int32), ("pad", C.c_int32)]
+
+
 _dp = C.POINTER(C.c_double)
 _u8p = C.POINTER(C.c_uint8)
 _vp = C.c_void_p
@@ -130,6 +153,19 @@ SIGNATURES = [
     ("mp_transport_set_limiter", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_double]),
     ("mp_transport_limits", C.c_int32, [_vp, _dp, _dp]),
     ("mp_transport_set_diffusion", C.c_int32, [_vp, C.c_double, C.c_int32, C.c_double]),
+    ("mp_default_coupled_params", None, [C.POINTER(CoupledParams)]),
+    ("mp_coupled_create", C.c_int32, [C.POINTER(MeshDesc), C.POINTER(PhaseParams),
+                                      C.POINTER(CoupledParams), _vp, C.POINTER(_vp)]),
+    ("mp_coupled_destroy", C.c_int32, [_vp]),
+    ("mp_coupled_set_flow", C.c_int32, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    ("mp_coupled_set_boundary", C.c_int32, [_vp, C.c_int32, _u8p, _u8p, _u8p, _u8p]),
+    ("mp_coupled_set_inflow", C.c_int32, [_vp, _dp, _dp]),
+    ("mp_coupled_set_thermal_boundary", C.c_int32, [_vp, _u8p, _dp]),
+    ("mp_coupled_upload_state", C.c_int32, [_vp, _dp, _dp]),
+    ("mp_coupled_download_state", C.c_int32, [_vp, _dp, _dp]),
+    ("mp_coupled_rhs", C.c_int32, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(CoupledRates)]),
+    ("mp_coupled_run", C.c_int32, [_vp, C.POINTER(CoupledTime), _dp, C.c_int32,
+                                   C.POINTER(CoupledStep), C.POINTER(C.c_int32)]),
     ("mp_transport_set_thermal_boundary", C.c_int32, [_vp, _u8p, _dp]),
     ("mp_batch_create", C.c_int32, [C.c_int32, C.POINTER(WenoOptions), C.POINTER(PhaseParams),
                                     C.c_int32, C.c_uint64, _vp, C.POINTER(_vp)]),

@@ -25,6 +25,8 @@ enum : uint32_t {
     kErrWeights = 1u << 5,        // no finite active log weight
     kErrPhaseRange = 1u << 6,     // phase boundaries not resolvable
     kErrInadmissible = 1u << 7,   // limiter: average outside H >= 0, 0 <= C <= Xe
+    kErrCoupledRange = 1u << 8,   // coupled law: T < 0 or phi >= 1 at a point
+    kErrLFSpeed = 1u << 9,        // coupled law: global LF speed below a flow speed
 };
 
 // Binary eutectic parameters as the device uses them.  Tep/Tmp are derived

@@ -1,0 +1,560 @@
+// Host side of the coupled transport law: the mp_coupled_* C ABI and the
+// SSPRK2 step loop with admissibility/CFL retries of src/couple/evolve.cpp:411-507.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mp_coupled.h"
+#include "mp_host.h"
+
+using namespace mp;
+
+struct mp_coupled {
+    mp_mesh_desc mesh{};
+    mp_phase_params phase_params{};
+    mp_coupled_params params{};
+    PhaseHost phase{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    MeshDev md{};
+    CoupledDev cd{};
+    size_t N = 0;
+    int64_t nxf = 0, nyf = 0;
+    int npts = 0;                       // cell points
+    DeviceBuffer d_md, state, old, rbuf, first, rd, bnd, blend, cell_out, face, flowx, flowy, cmix;
+    DeviceBuffer bcH, bcC, inflow, tbc, tval, error, red, kinds;
+    bool have_flow = false;
+    double c_max = 0;                   // admissible C bound
+
+    ~mp_coupled()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+
+    mp_status init(const mp_mesh_desc& m, const mp_phase_params& p, const mp_coupled_params& cp,
+                   void* s);
+    mp_status sync_md();
+    mp_status set_flow(const double* es, const double* ed, const double* ep, const double* cs,
+                       const double* cdarcy, const double* cp);
+    mp_status set_boundary(int field, const uint8_t* const side[4]);
+    mp_status set_inflow(const double* h, const double* c);
+    mp_status set_thermal(const uint8_t* types, const double* values);
+    mp_status rhs_on(const double* H, const double* C, double* rH, double* rC,
+                     mp_coupled_rates& tot);
+    mp_status rhs(double* rH, double* rC, double* rate, double* div, double* T, double* phi,
+                  mp_coupled_rates* tot);
+    mp_status admissible(const double* S, bool& ok, double& intH, double& intC);
+    mp_status run(const mp_coupled_time& tc, double* time, int nsteps, mp_coupled_step* steps,
+                  int32_t* accepted);
+};
+
+mp_status mp_coupled::sync_md()
+{
+    MP_HIP(hipMemcpyAsync(d_md.ptr, &md, sizeof(MeshDev), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+mp_status mp_coupled::init(const mp_mesh_desc& m, const mp_phase_params& p,
+                           const mp_coupled_params& cp, void* s)
+{
+    if (!(std::isfinite(cp.theta) && std::isfinite(cp.cooling)))
+        return fail(MP_ERR_ARG, "theta and cooling must be finite");
+    if (!(std::isfinite(cp.thermal_diffusivity) && cp.thermal_diffusivity >= 0))
+        return fail(MP_ERR_ARG, "diffusivity must be nonnegative");
+    if (cp.lf_mode != 0 && cp.lf_mode != 1) return fail(MP_ERR_ARG, "unknown Lax-Friedrichs mode");
+    if (cp.lf_mode == 1 && !(std::isfinite(cp.global_speed) && cp.global_speed >= 0))
+        return fail(MP_ERR_ARG, "global LF speed must be finite and nonnegative");
+    MP_CALL(make_phase(p, phase));
+    mp_weno_options w;
+    mp_default_weno_options(&w);
+    w.lf_mode = cp.lf_mode;
+    w.global_speed = cp.global_speed;
+    MP_CALL(make_mesh_dev(m, w, phase, md));
+    c_max = p.hypereutectic ? 1.0 : p.Xe;
+    MP_CALL(set_limiter_tables(md, cp.cell_points_per_axis, cp.diffusion_samples,
+                               cp.extent_fraction, c_max));
+    mesh = m;
+    phase_params = p;
+    params = cp;
+    N = size_t(m.nx) * m.ny;
+    nxf = int64_t(m.ny) * (m.nx + 1);
+    nyf = int64_t(m.ny + 1) * m.nx;
+    npts = cp.cell_points_per_axis * cp.cell_points_per_axis;
+    if (s) {
+        stream = static_cast<hipStream_t>(s);
+    } else {
+        MP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        own_stream = true;
+    }
+    MP_CALL(d_md.alloc(sizeof(MeshDev)));
+    MP_CALL(state.alloc(2 * N * sizeof(double)));
+    MP_CALL(old.alloc(2 * N * sizeof(double)));
+    MP_CALL(rbuf.alloc(2 * N * sizeof(double)));
+    MP_CALL(first.alloc(2 * N * sizeof(double)));
+    MP_CALL(rd.alloc(2 * N * sizeof(double)));
+    MP_CALL(bnd.alloc(3 * N * sizeof(double)));
+    MP_CALL(blend.alloc(20 * N * sizeof(double)));
+    MP_CALL(cell_out.alloc(4 * N * sizeof(double)));
+    MP_CALL(face.alloc(4 * size_t(nxf + nyf) * sizeof(double)));
+    MP_CALL(flowx.alloc(3 * size_t(m.q) * nxf * sizeof(double)));
+    MP_CALL(flowy.alloc(3 * size_t(m.q) * nyf * sizeof(double)));
+    MP_CALL(cmix.alloc(size_t(npts) * N * sizeof(double)));
+    MP_CALL(error.alloc(sizeof(uint32_t)));
+    MP_CALL(red.alloc(8 * sizeof(double)));
+    MP_CALL(kinds.alloc(8 * sizeof(int)));
+    MP_HIP(hipMemsetAsync(state.ptr, 0, state.bytes, stream));
+    MP_HIP(hipMemsetAsync(flowx.ptr, 0, flowx.bytes, stream));
+    MP_HIP(hipMemsetAsync(flowy.ptr, 0, flowy.bytes, stream));
+    MP_HIP(hipMemsetAsync(cmix.ptr, 0, cmix.bytes, stream));
+
+    CoupledDev& d = cd;
+    std::memset(&d, 0, sizeof d);
+    d.m = d_md.as<MeshDev>();
+    d.blend = blend.as<double>();
+    d.cell_out = cell_out.as<double>();
+    d.face = face.as<double>();
+    d.error = error.as<uint32_t>();
+    const size_t q = m.q;
+    d.xvs = flowx.as<double>();
+    d.xqd = d.xvs + q * nxf;
+    d.xphi = d.xqd + q * nxf;
+    d.yvs = flowy.as<double>();
+    d.yqd = d.yvs + q * nyf;
+    d.yphi = d.yqd + q * nyf;
+    d.cmix = cmix.as<double>();
+    double xc[kMaxQ], wc[kMaxQ];
+    gauss_legendre(cp.cell_points_per_axis, xc, wc);
+    for (int k = 0; k < cp.cell_points_per_axis; ++k) d.cw[k] = wc[k];
+    d.cooling = cp.cooling;
+    d.kappa = cp.thermal_diffusivity;
+    d.global_speed = cp.global_speed;
+    d.lf_mode = cp.lf_mode;
+    // normal-line sampling on uniform rectangles (src/mlweno/diffusiveflux.cpp:256-338)
+    const int n = cp.diffusion_samples;
+    d.n = n;
+    const long double f = cp.extent_fraction;
+    d.dsx = (double)(f * md.dx * 2.0L / (n - 1));
+    d.dsy = (double)(f * md.dy * 2.0L / (n - 1));
+    d.dsbx = (double)(f * md.dx / n);
+    d.dsby = (double)(f * md.dy / n);
+    lagrange_weights(n, (n - 1) / 2.0L, d.cm);
+    lagrange_weights(n + 1, (long double)n, d.ce);
+    long double last = 0, am = 0, ae = 0;
+    for (int k = 0; k + 1 < n; ++k) {
+        last -= d.cm[k];
+        am += std::fabs(d.cm[k]);
+    }
+    am += fabsl(last);
+    for (int k = 0; k < n; ++k) ae += std::fabs(d.ce[k]);
+    d.cm_abs = (double)am;
+    d.ce_abs = (double)ae;
+    return sync_md();
+}
+
+// Flow samples by reference edge id (horizontal edges first) and canonical q;
+// only the normal components of a uniform rectangle's faces are kept, with
+// the Darcy flux phi^(1+theta)*u_d formed as PhaseTransportVelocities does
+// (src/couple/evolve.cpp:40-46).
+mp_status mp_coupled::set_flow(const double* es, const double* ed, const double* ep,
+                               const double* cs, const double* cdarcy, const double* cp)
+{
+    if (!es || !ed || !ep || !cs || !cdarcy || !cp) return fail(MP_ERR_ARG, "null flow sample array");
+    const int nx = mesh.nx, ny = mesh.ny, q = mesh.q;
+    const size_t nh = size_t(nx) * (ny + 1);
+    std::vector<double> hx(3 * size_t(q) * nxf), hy(3 * size_t(q) * nyf);
+    auto factor = [&](double phi) { return phi == 0 ? 0.0 : std::pow(phi, 1 + params.theta); };
+    for (size_t e = 0; e < nh + size_t(nxf); ++e) {
+        const bool vert = e >= nh;
+        const int comp = vert ? 0 : 1;
+        for (int k = 0; k < q; ++k) {
+            const size_t a = e * q + k;
+            const double phi = ep[a];
+            if (!(std::isfinite(phi) && phi >= 0 && phi <= 1))
+                return fail(MP_ERR_ARG, "flow porosity must lie in [0, 1]");
+            const double vs = es[2 * a + comp], dn = ed[2 * a + comp];
+            if (!std::isfinite(vs) || !std::isfinite(dn)) return fail(MP_ERR_FP, "flow samples must be finite");
+            const double qd = factor(phi) * dn;
+            if (vert) {
+                const size_t fl = e - nh;   // j*(nx+1) + i
+                hx[size_t(k) * nxf + fl] = vs;
+                hx[(q + k) * size_t(nxf) + fl] = qd;
+                hx[(2 * q + k) * size_t(nxf) + fl] = phi;
+            } else {
+                hy[size_t(k) * nyf + e] = vs;
+                hy[(q + k) * size_t(nyf) + e] = qd;
+                hy[(2 * q + k) * size_t(nyf) + e] = phi;
+            }
+        }
+    }
+    std::vector<double> mix(size_t(npts) * N);
+    for (size_t c = 0; c < N; ++c)
+        for (int k = 0; k < npts; ++k) {
+            const size_t a = c * npts + k;
+            const double phi = cp[a];
+            if (!(std::isfinite(phi) && phi >= 0 && phi <= 1))
+                return fail(MP_ERR_ARG, "flow porosity must lie in [0, 1]");
+            const double m = cs[2 * a + 1] + factor(phi) * cdarcy[2 * a + 1];
+            if (!std::isfinite(m)) return fail(MP_ERR_FP, "flow samples must be finite");
+            mix[size_t(k) * N + c] = m;
+        }
+    MP_HIP(hipMemcpyAsync(flowx.ptr, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(flowy.ptr, hy.data(), hy.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(cmix.ptr, mix.data(), mix.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    have_flow = true;
+    return MP_OK;
+}
+
+mp_status mp_coupled::set_boundary(int field, const uint8_t* const side[4])
+{
+    if (field != 0 && field != 1) return fail(MP_ERR_ARG, "field must be 0 (H) or 1 (C)");
+    const size_t n[4] = {size_t(mesh.nx), size_t(mesh.ny), size_t(mesh.nx), size_t(mesh.ny)};
+    DeviceBuffer& buf = field ? bcC : bcH;
+    const uint8_t** dst = field ? cd.bcC : cd.bcH;
+    std::vector<uint8_t> all;
+    for (int s = 0; s < 4; ++s) {
+        for (size_t k = 0; k < n[s]; ++k) {
+            const uint8_t v = side[s] ? side[s][k] : 0;
+            if (v > 2) return fail(MP_ERR_ARG, "unknown boundary policy %d", int(v));
+            all.push_back(v);
+        }
+    }
+    MP_CALL(buf.alloc(all.size()));
+    MP_HIP(hipMemcpyAsync(buf.ptr, all.data(), all.size(), hipMemcpyHostToDevice, stream));
+    size_t off = 0;
+    for (int s = 0; s < 4; ++s) {
+        dst[s] = static_cast<const uint8_t*>(buf.ptr) + off;
+        off += n[s];
+    }
+    MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+mp_status mp_coupled::set_inflow(const double* h, const double* c)
+{
+    const size_t total = size_t(2) * (mesh.nx + mesh.ny) * mesh.q;
+    const size_t off[4] = {0, size_t(mesh.nx) * mesh.q, size_t(mesh.nx + mesh.ny) * mesh.q,
+                           size_t(2 * mesh.nx + mesh.ny) * mesh.q};
+    for (const double* src : {h, c})
+        if (src && !all_finite(src, total))
+            return fail(MP_ERR_FP, "Inflow boundary callback must return a finite state");
+    MP_CALL(inflow.alloc(2 * total * sizeof(double)));
+    double* base = inflow.as<double>();
+    for (int f = 0; f < 2; ++f) {
+        const double* src = f ? c : h;
+        const double** dst = f ? cd.inC : cd.inH;
+        if (src) MP_HIP(hipMemcpyAsync(base + f * total, src, total * sizeof(double), hipMemcpyHostToDevice, stream));
+        for (int s = 0; s < 4; ++s) dst[s] = src ? base + f * total + off[s] : nullptr;
+    }
+    MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+mp_status mp_coupled::set_thermal(const uint8_t* types, const double* values)
+{
+    const size_t total = size_t(2) * (mesh.nx + mesh.ny) * mesh.q;
+    const size_t off[4] = {0, size_t(mesh.nx) * mesh.q, size_t(mesh.nx + mesh.ny) * mesh.q,
+                           size_t(2 * mesh.nx + mesh.ny) * mesh.q};
+    if (!types) {
+        for (int s = 0; s < 4; ++s) { md.tbc[s] = nullptr; md.tval[s] = nullptr; }
+        return sync_md();
+    }
+    std::vector<double> v(total, 0.0);
+    for (size_t k = 0; k < total; ++k) {
+        if (types[k] > 2) return fail(MP_ERR_ARG, "Unknown thermal boundary policy");
+        if (types[k] != 0) {
+            if (!values) return fail(MP_ERR_ARG, "Missing thermal boundary profile.");
+            if (!std::isfinite(values[k])) return fail(MP_ERR_ARG, "Thermal boundary data must be finite");
+            v[k] = values[k];
+        }
+    }
+    MP_CALL(tbc.alloc(total));
+    MP_CALL(tval.alloc(total * sizeof(double)));
+    MP_HIP(hipMemcpyAsync(tbc.ptr, types, total, hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(tval.ptr, v.data(), total * sizeof(double), hipMemcpyHostToDevice, stream));
+    for (int s = 0; s < 4; ++s) {
+        md.tbc[s] = static_cast<const uint8_t*>(tbc.ptr) + off[s];
+        md.tval[s] = tval.as<double>() + off[s];
+    }
+    return sync_md();
+}
+
+// One RHS of the state (H, C) into (rH, rC) and the totals; fails with the
+// device's error bits.
+mp_status mp_coupled::rhs_on(const double* H, const double* C, double* rH, double* rC,
+                             mp_coupled_rates& tot)
+{
+    if (!have_flow) return fail(MP_ERR_STATE, "set the flow samples first");
+    MP_HIP(hipMemsetAsync(error.ptr, 0, sizeof(uint32_t), stream));
+    double* r = rd.as<double>();
+    MP_HIP(launch_coupled_rhs(cd, mesh.nx, mesh.ny, H, C, rH, rC, r, r + N, bnd.as<double>(), stream));
+    const int k1[3] = {0, 0, 0}, k2[2] = {3, 1};
+    MP_HIP(hipMemcpyAsync(kinds.ptr, k1, sizeof k1, hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(kinds.as<int>() + 3, k2, sizeof k2, hipMemcpyHostToDevice, stream));
+    MP_HIP(launch_reduce(bnd.as<double>(), int64_t(N), 3, kinds.as<int>(), red.as<double>(), stream));
+    MP_HIP(launch_reduce(r, int64_t(N), 2, kinds.as<int>() + 3, red.as<double>() + 3, stream));
+    double h[5];
+    uint32_t bits = 0;
+    MP_HIP(hipMemcpyAsync(h, red.ptr, sizeof h, hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipMemcpyAsync(&bits, error.ptr, sizeof bits, hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    if (bits) return fail(status_from_bits(bits), "%s (device error bits 0x%x)", describe_bits(bits), bits);
+    if (!std::isfinite(h[3])) return fail(MP_ERR_FP, "Nonfinite coupled stage rate");
+    tot.boundary_H = h[0];
+    tot.boundary_C = h[1];
+    tot.source_H = h[2];
+    tot.maximum_rate = h[3];
+    tot.divergence = h[4];
+    return MP_OK;
+}
+
+mp_status mp_coupled::rhs(double* rH, double* rC, double* rate, double* div, double* T, double* phi,
+                          mp_coupled_rates* tot)
+{
+    mp_coupled_rates t{};
+    double* S = state.as<double>();
+    double* R = rbuf.as<double>();
+    MP_CALL(rhs_on(S, S + N, R, R + N, t));
+    const size_t nb = N * sizeof(double);
+    double* outs[6] = {rH, rC, rate, div, T, phi};
+    const double* srcs[6] = {R, R + N, rd.as<double>(), rd.as<double>() + N,
+                             cell_out.as<double>() + N, cell_out.as<double>() + 2 * N};
+    for (int k = 0; k < 6; ++k)
+        if (outs[k]) MP_HIP(hipMemcpyAsync(outs[k], srcs[k], nb, hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    if (tot) *tot = t;
+    return MP_OK;
+}
+
+// Admissible (src/couple/evolve.cpp:386-393) and the integrals sum |K| u.
+mp_status mp_coupled::admissible(const double* S, bool& ok, double& intH, double& intC)
+{
+    const int k[6] = {2, 2, 3, 3, 0, 0};
+    MP_HIP(hipMemcpyAsync(kinds.ptr, k, sizeof k, hipMemcpyHostToDevice, stream));
+    double* o = red.as<double>();
+    // min H, min C | max H, max C | sum H, sum C (H and C are contiguous)
+    MP_HIP(launch_reduce(S, int64_t(N), 2, kinds.as<int>(), o, stream));
+    MP_HIP(launch_reduce(S, int64_t(N), 2, kinds.as<int>() + 2, o + 2, stream));
+    MP_HIP(launch_reduce(S, int64_t(N), 2, kinds.as<int>() + 4, o + 4, stream));
+    double h[6];
+    MP_HIP(hipMemcpyAsync(h, o, sizeof h, hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    ok = std::isfinite(h[0]) && std::isfinite(h[2]) && std::isfinite(h[1]) && std::isfinite(h[3]) &&
+         h[0] >= 0 && h[1] >= 0 && h[3] <= c_max;
+    intH = h[4] * md.area;
+    intC = h[5] * md.area;
+    return MP_OK;
+}
+
+mp_status mp_coupled::run(const mp_coupled_time& tc, double* time, int nsteps,
+                          mp_coupled_step* steps, int32_t* accepted_out)
+{
+    if (!(tc.cfl > 0 && tc.cfl <= 1)) return fail(MP_ERR_ARG, "Coupled CFL must be in (0,1]");
+    if (!time) return fail(MP_ERR_ARG, "null time");
+    const size_t nb = 2 * N * sizeof(double);
+    double* S = state.as<double>();
+    double* O = old.as<double>();
+    double* R = rbuf.as<double>();
+    double* F = first.as<double>();
+    mp_coupled_rates current{}, second{};
+    MP_CALL(rhs_on(S, S + N, R, R + N, current));
+    MP_HIP(hipMemcpyAsync(O, S, nb, hipMemcpyDeviceToDevice, stream));
+    int accepted = 0;
+    mp_status st = MP_OK;
+    for (int s = 0; s < nsteps; ++s) {
+        const double remaining = tc.end_time - *time;
+        if (remaining <= 32 * 2.220446049250313e-16 * std::fmax(1.0, std::fabs(tc.end_time))) {
+            *time = tc.end_time;
+            break;
+        }
+        bool ok;
+        double oldH, oldC;
+        MP_CALL(admissible(S, ok, oldH, oldC));
+        const double start = *time;
+        double dt = std::fmin(remaining, tc.fixed ? tc.initial_step
+                                                  : (accepted == 0 ? tc.initial_step : tc.maximum_step));
+        const double limit = current.maximum_rate > 0 ? tc.cfl / current.maximum_rate : tc.maximum_step;
+        if (tc.fixed && !(dt <= limit * (1 + 1e-12)))
+            return fail(MP_ERR_ARG, "Fixed time step exceeds the phase-coupled CFL bound; use control: cfl");
+        if (!tc.fixed) dt = std::fmin(dt, limit);
+        MP_HIP(hipMemcpyAsync(F, R, nb, hipMemcpyDeviceToDevice, stream));
+        const mp_coupled_rates firstRates = current;
+        int retries = 0;
+        double courant = 0;
+        for (;;) {
+            if (!(dt >= std::fmin(remaining, tc.minimum_step) && retries < 40)) {
+                st = fail(MP_ERR_PHASE, "Coupled step cannot satisfy admissibility/CFL above minimum step");
+                break;
+            }
+            MP_HIP(launch_coupled_stage(int64_t(N), 1, dt, O, O + N, F, F + N, S, S + N, stream));
+            *time = start + dt;
+            double ih, ic;
+            MP_CALL(admissible(S, ok, ih, ic));
+            if (ok) {
+                st = rhs_on(S, S + N, R, R + N, second);
+                if (st) break;
+                courant = dt * std::fmax(current.maximum_rate, second.maximum_rate);
+                ok = courant <= tc.cfl * (1 + 1e-12);
+            }
+            if (ok) {
+                MP_HIP(launch_coupled_stage(int64_t(N), 2, dt, O, O + N, R, R + N, S, S + N, stream));
+                MP_CALL(admissible(S, ok, ih, ic));
+            }
+            if (ok) break;
+            if (tc.fixed) {
+                st = fail(MP_ERR_ARG, "Fixed coupled step failed stage admissibility/CFL");
+                break;
+            }
+            dt *= .5;
+            ++retries;
+        }
+        double balH = 0, balC = 0;
+        if (!st) {
+            bool dummy;
+            double Hi, Ci;
+            MP_CALL(admissible(S, dummy, Hi, Ci));
+            balH = Hi - oldH + .5 * dt * (current.boundary_H + second.boundary_H - current.source_H - second.source_H);
+            balC = Ci - oldC + .5 * dt * (current.boundary_C + second.boundary_C);
+            if (!(std::fabs(balH) < 1e-10 * std::fmax(1.0, std::fabs(oldH)) && std::fabs(balC) < 1e-10))
+                st = fail(MP_ERR_PHASE, "Coupled finite-volume conservation balance failed");
+        }
+        if (!st) st = rhs_on(S, S + N, R, R + N, current);
+        if (st) {
+            // roll back to the latest accepted state (src/couple/evolve.cpp:493-499)
+            *time = start;
+            MP_HIP(hipMemcpyAsync(S, O, nb, hipMemcpyDeviceToDevice, stream));
+            MP_HIP(hipStreamSynchronize(stream));
+            break;
+        }
+        ++accepted;
+        MP_HIP(hipMemcpyAsync(O, S, nb, hipMemcpyDeviceToDevice, stream));
+        if (steps) {
+            mp_coupled_step& r = steps[s];
+            r.dt = dt;
+            r.courant = courant;
+            r.balance_H = balH;
+            r.balance_C = balC;
+            r.boundary_H = .5 * (firstRates.boundary_H + second.boundary_H);
+            r.boundary_C = .5 * (firstRates.boundary_C + second.boundary_C);
+            r.source_H = .5 * (firstRates.source_H + second.source_H);
+            r.maximum_rate = current.maximum_rate;
+            r.retries = retries;
+            r.pad = 0;
+        }
+    }
+    MP_HIP(hipStreamSynchronize(stream));
+    if (accepted_out) *accepted_out = accepted;
+    return st;
+}
+
+// ===================================================================== C ABI
+
+extern "C" {
+
+void mp_default_coupled_params(mp_coupled_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->theta = 1.0;
+    p->cooling = 0.0;
+    p->thermal_diffusivity = 0.0;
+    p->cell_points_per_axis = 5;
+    p->diffusion_samples = 4;
+    p->extent_fraction = 0.9;
+    p->lf_mode = 0;
+    p->global_speed = 0.0;
+}
+
+mp_status mp_coupled_create(const mp_mesh_desc* mesh, const mp_phase_params* phase,
+                            const mp_coupled_params* params, void* stream, mp_coupled** out)
+{
+    MP_GUARD({
+        if (!mesh || !phase || !params || !out) return fail(MP_ERR_ARG, "null argument");
+        *out = nullptr;
+        auto* c = new mp_coupled();
+        const mp_status s = c->init(*mesh, *phase, *params, stream);
+        if (s) {
+            delete c;
+            return s;
+        }
+        *out = c;
+        return MP_OK;
+    })
+}
+
+mp_status mp_coupled_destroy(mp_coupled* c)
+{
+    delete c;
+    return MP_OK;
+}
+
+mp_status mp_coupled_set_flow(mp_coupled* c, const double* edge_stokes, const double* edge_darcy,
+                              const double* edge_phi, const double* cell_stokes,
+                              const double* cell_darcy, const double* cell_phi)
+{
+    if (!c) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return c->set_flow(edge_stokes, edge_darcy, edge_phi, cell_stokes, cell_darcy, cell_phi); })
+}
+
+mp_status mp_coupled_set_boundary(mp_coupled* c, int32_t field, const uint8_t* bottom,
+                                  const uint8_t* right, const uint8_t* top, const uint8_t* left)
+{
+    if (!c) return fail(MP_ERR_ARG, "null handle");
+    const uint8_t* sides[4] = {bottom, right, top, left};
+    MP_GUARD({ return c->set_boundary(field, sides); })
+}
+
+mp_status mp_coupled_set_inflow(mp_coupled* c, const double* inflow_H, const double* inflow_C)
+{
+    if (!c) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return c->set_inflow(inflow_H, inflow_C); })
+}
+
+mp_status mp_coupled_set_thermal_boundary(mp_coupled* c, const uint8_t* types, const double* values)
+{
+    if (!c) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return c->set_thermal(types, values); })
+}
+
+mp_status mp_coupled_upload_state(mp_coupled* c, const double* H, const double* C)
+{
+    if (!c || !H || !C) return fail(MP_ERR_ARG, "null argument");
+    MP_GUARD({
+        const size_t nb = c->N * sizeof(double);
+        MP_HIP(hipMemcpyAsync(c->state.ptr, H, nb, hipMemcpyHostToDevice, c->stream));
+        MP_HIP(hipMemcpyAsync(c->state.as<double>() + c->N, C, nb, hipMemcpyHostToDevice, c->stream));
+        MP_HIP(hipStreamSynchronize(c->stream));
+        return MP_OK;
+    })
+}
+
+mp_status mp_coupled_download_state(mp_coupled* c, double* H, double* C)
+{
+    if (!c) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({
+        const size_t nb = c->N * sizeof(double);
+        if (H) MP_HIP(hipMemcpyAsync(H, c->state.ptr, nb, hipMemcpyDeviceToHost, c->stream));
+        if (C) MP_HIP(hipMemcpyAsync(C, c->state.as<double>() + c->N, nb, hipMemcpyDeviceToHost, c->stream));
+        MP_HIP(hipStreamSynchronize(c->stream));
+        return MP_OK;
+    })
+}
+
+mp_status mp_coupled_rhs(mp_coupled* c, double* rhsH, double* rhsC, double* rate, double* div,
+                         double* T, double* phi, mp_coupled_rates* totals)
+{
+    if (!c) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return c->rhs(rhsH, rhsC, rate, div, T, phi, totals); })
+}
+
+mp_status mp_coupled_run(mp_coupled* c, const mp_coupled_time* time_controls, double* time,
+                         int32_t nsteps, mp_coupled_step* steps, int32_t* accepted)
+{
+    if (!c || !time_controls) return fail(MP_ERR_ARG, "null argument");
+    if (nsteps < 0) return fail(MP_ERR_ARG, "nsteps must be >= 0");
+    MP_GUARD({ return c->run(*time_controls, time, nsteps, steps, accepted); })
+}
+
+}  // extern "C"

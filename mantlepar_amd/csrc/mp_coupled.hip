@@ -1,0 +1,426 @@
+// Coupled transport law on the device: one RHS of MantlePar's phase-coupled
+// evolution (src/couple/evolve.cpp:279-384) on a uniform rectangular mesh.
+//
+// Three launches per RHS, all FP64:
+//   coupled_cell_kernel   one thread per cell: limited blends of H and C
+//                         (the reconstruction + StateReconstruction::Limit),
+//                         stored as 20 doubles per cell, and the cell
+//                         quadrature (cooling source, cell-averaged T and phi)
+//   coupled_face_kernel   one thread per face: phase at the face Gauss points
+//                         on both sides, PhaseTransportVelocities, the T flux
+//                         with its latent term and the C flux with the
+//                         effective velocity (LLF), boundary policies, the
+//                         CFL rate with the phase sensitivity, and the
+//                         normal-line diffusion of T from phase evaluations at
+//                         the in-cell samples
+//   coupled_gather_kernel one thread per cell: the face fluxes into rhs_H,
+//                         rhs_C, rate and mixture divergence (a gather, no
+//                         atomics on state)
+// followed by a fixed-order reduction of the totals.  Each face is evaluated
+// once, so every phase evaluation of the reference (about 85 per cell) is
+// made once.
+#include <hip/hip_runtime.h>
+
+#include "mp_common.h"
+#include "mp_coupled.h"
+#include "mp_device.h"
+#include "mp_limit.h"
+#include "mp_phase.h"
+#include "mp_recon.h"
+
+namespace mp {
+
+namespace {
+
+__device__ __forceinline__ double poly_at(const double* B, double x, double y)
+{
+    // B[b*3 + a]: coefficient of x^a y^b
+    const double q0 = B[0] + x * (B[1] + x * B[2]);
+    const double q1 = B[3] + x * (B[4] + x * B[5]);
+    const double q2 = B[6] + x * (B[7] + x * B[8]);
+    return q0 + y * (q1 + y * q2);
+}
+
+// Limited state of cell c at normalised offset (x, y): mean + theta*P.
+struct CellPoly {
+    double tH, tC, BH[9], BC[9];
+};
+
+__device__ __forceinline__ void load_cell(const CoupledDev& d, size_t c, size_t N, CellPoly& p)
+{
+    p.tH = d.blend[c];
+    p.tC = d.blend[N + c];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        p.BH[k] = d.blend[(2 + k) * N + c];
+        p.BC[k] = d.blend[(11 + k) * N + c];
+    }
+}
+
+// Phase of the limited reconstruction at a point, with the coupled law's
+// range check (src/couple/evolve.cpp:270-277).
+template <bool DERIV>
+__device__ __noinline__ PhaseOut point_phase(const PhaseDev& ph, const CellPoly& p, double x,
+                                             double y, double& H, double& C, uint32_t& err)
+{
+    H = p.tH + poly_at(p.BH, x, y);
+    C = p.tC + poly_at(p.BC, x, y);
+    const PhaseOut o = phase_eval<DERIV>(ph, H, C, err);
+    if (!(o.T >= 0.0 && o.phil < 1.0)) err |= kErrCoupledRange;
+    return o;
+}
+
+}  // namespace
+
+__global__ void coupled_cell_kernel(CoupledDev d, const double* __restrict__ H,
+                                    const double* __restrict__ C)
+{
+    const MeshDev& m = *d.m;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = m.nx, ny = m.ny;
+    if (c >= nx * ny) return;
+    const size_t N = size_t(nx) * ny;
+    const int i = c % nx, j = c / nx;
+    uint32_t err = 0;
+    const unsigned mask = candidate_mask(i, j, nx, ny);
+    const unsigned sides = (j == 0 ? 1u : 0u) | (i == nx - 1 ? 2u : 0u) | (j == ny - 1 ? 4u : 0u) |
+                           (i == 0 ? 8u : 0u);
+    CellPoly p;
+    for (int field = 0; field < 2; ++field) {
+        double w[3][3], B[3][3];
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+            for (int di = -1; di <= 1; ++di)
+                w[dj + 1][di + 1] = (field ? C : H)[size_t(clampi(j + dj, 0, ny - 1)) * nx +
+                                                   clampi(i + di, 0, nx - 1)];
+        blend<double>(m, w, mask, B, err);
+        const double th = limit_factor<0>(m, w, B, sides, field ? m.lim_cupper : INFINITY, err);
+        double* out = field ? p.BC : p.BH;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) out[b * 3 + a] = th * B[b][a];
+        (field ? p.tC : p.tH) = w[1][1];
+    }
+    d.blend[c] = p.tH;
+    d.blend[N + c] = p.tC;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        d.blend[(2 + k) * N + c] = p.BH[k];
+        d.blend[(11 + k) * N + c] = p.BC[k];
+    }
+    // cell quadrature (src/couple/evolve.cpp:357-370): b along y outer, a along x
+    double heat = 0, T = 0, phi = 0, srate = 0;
+    const int n = m.lim_ncp;
+#pragma unroll 1
+    for (int b = 0; b < n; ++b)
+#pragma unroll 1
+        for (int a = 0; a < n; ++a) {
+            double Hp, Cp;
+            const PhaseOut o = point_phase<false>(m.ph, p, m.lim_cx[a], m.lim_cy[b], Hp, Cp, err);
+            const double wab = d.cw[a] * d.cw[b] * 0.25;      // weight / area
+            const double mixy = d.cmix[size_t(b * n + a) * N + c];
+            const double source = -d.cooling * o.T * mixy;
+            heat += wab * source;
+            T += wab * o.T;
+            phi += wab * o.phil;
+            srate += wab * (d.cooling * fabs(mixy));
+        }
+    d.cell_out[c] = heat;
+    d.cell_out[N + c] = T;
+    d.cell_out[2 * N + c] = phi;
+    d.cell_out[3 * N + c] = srate;
+    if (err) atomicOr(d.error, err);
+}
+
+// One face.  Vertical faces f < ny*(nx+1) at x_i (i = f % (nx+1), j = f / (nx+1)),
+// horizontal faces after them at y_j.  Outputs in the canonical direction
+// (+x / +y): flux_H, flux_C, mixture flux, and the rate (same for both sides).
+__global__ void coupled_face_kernel(CoupledDev d)
+{
+    const MeshDev& m = *d.m;
+    const int nx = m.nx, ny = m.ny, nq = m.q;
+    const size_t N = size_t(nx) * ny;
+    const int64_t nxf = int64_t(ny) * (nx + 1), nyf = int64_t(ny + 1) * nx;
+    int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (f >= nxf + nyf) return;
+    const bool vert = f < nxf;
+    const int64_t fl = vert ? f : f - nxf;
+    int i, j;
+    if (vert) { i = int(fl % (nx + 1)); j = int(fl / (nx + 1)); }
+    else { i = int(fl % nx); j = int(fl / nx); }
+    const int along = vert ? i : j, limit = vert ? nx : ny;
+    const bool boundary = along == 0 || along == limit;
+    // boundary side (0 bottom, 1 right, 2 top, 3 left), its face index, the
+    // outward sign against the canonical direction and the interior cell
+    const int side = vert ? (i == 0 ? 3 : 1) : (j == 0 ? 0 : 2);
+    const int kface = vert ? j : i;
+    const double sgn = (along == 0) ? -1.0 : 1.0;
+    // cells: L behind the canonical normal, R ahead; on the boundary A is the
+    // interior cell
+    const int li = vert ? i - 1 : i, lj = vert ? j : j - 1;
+    uint32_t err = 0;
+    CellPoly P, Q;          // P: left (or the boundary's interior cell), Q: right
+    const int ai = boundary ? (vert ? (i == 0 ? 0 : nx - 1) : i) : li;
+    const int aj = boundary ? (vert ? j : (j == 0 ? 0 : ny - 1)) : lj;
+    load_cell(d, size_t(aj) * nx + ai, N, P);
+    if (!boundary) load_cell(d, size_t(j) * nx + i, N, Q);
+    // trace offsets: the face sits at +h (interior left cell, or boundary with
+    // outward +) / -h of the cell holding the point
+    const double hf = vert ? m.hx : m.hy;
+    const double pf = boundary ? sgn * hf : hf;     // in P's coordinates
+    const double* s = vert ? m.sv : m.sh;
+    const double* W = vert ? m.wv : m.wh;
+    const int64_t fstride = vert ? nxf : nyf;
+    const double* vsa = (vert ? d.xvs : d.yvs) + fl;
+    const double* qda = (vert ? d.xqd : d.yqd) + fl;
+    const double* pha = (vert ? d.xphi : d.yphi) + fl;
+    double fluxH = 0, fluxC = 0, fluxMix = 0, rate = 0;
+    uint8_t polH = 0, polC = 0;
+    const double *inH = nullptr, *inC = nullptr;
+    if (boundary) {
+        polH = d.bcH[side] ? d.bcH[side][kface] : 0;
+        polC = d.bcC[side] ? d.bcC[side][kface] : 0;
+        inH = d.inH[side] ? d.inH[side] + size_t(kface) * nq : nullptr;
+        inC = d.inC[side] ? d.inC[side] + size_t(kface) * nq : nullptr;
+    }
+#pragma unroll 1
+    for (int q = 0; q < nq; ++q) {
+        // outward (boundary) or canonical (interior) normal components
+        const double vs = (boundary ? sgn : 1.0) * vsa[q * fstride];
+        const double qd = (boundary ? sgn : 1.0) * qda[q * fstride];
+        const double ephi = pha[q * fstride];
+        const double xa = vert ? pf : s[q], ya = vert ? s[q] : pf;
+        double hL, cL, hR, cR;
+        const PhaseOut a = point_phase<true>(m.ph, P, xa, ya, hL, cL, err);
+        PhaseOut b = a;
+        cR = cL;
+        if (!boundary) {
+            const double xb = vert ? -hf : s[q], yb = vert ? s[q] : -hf;
+            b = point_phase<true>(m.ph, Q, xb, yb, hR, cR, err);
+        }
+        (void)hR;
+        // PhaseTransportVelocities (src/couple/evolve.cpp:36-52) along the normal
+        const double cl = .5 * (a.cl + b.cl), cs = .5 * (a.cs + b.cs);
+        const double bulk = ephi * cl + (1 - ephi) * cs;
+        const double vn = vs + qd;
+        const double ve = vs + (bulk > 0 ? cl / bulk * qd : 0);
+        const double alphaH = d.lf_mode ? d.global_speed : fabs(vn);
+        const double alphaC = d.lf_mode ? d.global_speed : fabs(ve);
+        if (!(alphaH + 1e-14 >= fabs(vn) && alphaC + 1e-14 >= fabs(ve))) err |= kErrLFSpeed;
+        const double LD = m.ph.LD;
+        double fh = .5 * (vn * (a.T + b.T) - alphaH * (b.T - a.T)) - LD * (1 - ephi) * vs;
+        double fc = .5 * (ve * (cL + cR) - alphaC * (cR - cL));
+        if (boundary) {
+            // EvaluateAdvectionBoundary (src/couple/initialization.cpp:469-489)
+            // for H along vn and for C along ve; 0 prescribed, 1 extrapolated, 2 zero flux
+            int tH = 2, tC = 2;
+            double valH = 0, valC = 0;
+            if (!(polH == 0 || vn == 0)) {
+                if (vn < 0) {
+                    if (polH != 2) err |= kErrBackflow;
+                    else if (!inH) err |= kErrInflow;
+                    else { tH = 0; valH = inH[q]; }
+                } else tH = 1;
+            }
+            if (!(polC == 0 || ve == 0)) {
+                if (ve < 0) {
+                    if (polC != 2) err |= kErrBackflow;
+                    else if (!inC) err |= kErrInflow;
+                    else { tC = 0; valC = inC[q]; }
+                } else tC = 1;
+            }
+            const double inflowC = tC == 0 ? valC : cL;
+            double T = a.T;
+            if (tH == 0) {
+                uint32_t e2 = 0;
+                T = phase_eval<false>(m.ph, valH, inflowC, e2).T;
+                err |= e2;
+            }
+            fh = tH == 2 ? 0 : vn * T - LD * (1 - ephi) * vs;
+            fc = tC == 2 ? 0 : ve * (tC == 0 ? valC : cL);
+        }
+        const double wq = W[q];
+        fluxH += wq * fh;
+        fluxC += wq * fc;
+        fluxMix += wq * vn;
+        const double sens = 1 + fmax(fabs(a.dTdC), fabs(b.dTdC));
+        if (!isfinite(sens)) err |= kErrNonFinite;
+        rate += wq * (alphaC + alphaH + 2 * fabs(vs)) * sens;
+    }
+    // normal-line diffusion of T (src/couple/evolve.cpp:334-350); the samples'
+    // phases are evaluated (and range-checked) also when kappa is 0
+    {
+        const int n = d.n;
+        const double ds = boundary ? (vert ? d.dsbx : d.dsby) : (vert ? d.dsx : d.dsy);
+        const double* depth = boundary ? (vert ? m.lim_xb : m.lim_yb) : (vert ? m.lim_xi : m.lim_yi);
+        double dflux = 0;
+#pragma unroll 1
+        for (int q = 0; q < nq; ++q) {
+            const double wq = W[q];
+            int pol = 1;
+            double bval = 0;
+            if (boundary) {
+                pol = m.tbc[side] ? m.tbc[side][size_t(kface) * nq + q] : 0;
+                bval = m.tval[side] ? m.tval[side][size_t(kface) * nq + q] : 0;
+                if (pol == 0) continue;
+                if (pol == 2) { dflux += wq * bval; continue; }
+            }
+            // samples k = 0..count-1 from the far side of the left (interior) cell
+            double Tk[2 * kMaxLimS];
+            const int nin = boundary ? n : n / 2;
+#pragma unroll 1
+            for (int k = 0; k < nin; ++k) {
+                const double off = boundary ? sgn * depth[k] : depth[k];
+                double h_, c_;
+                Tk[k] = point_phase<false>(m.ph, P, vert ? off : s[q], vert ? s[q] : off, h_, c_, err).T;
+            }
+            double last;
+            if (boundary) {
+                last = bval;
+            } else {
+#pragma unroll 1
+                for (int k = n / 2; k < n; ++k) {
+                    const double off = -depth[n - 1 - k];
+                    double h_, c_;
+                    Tk[k] = point_phase<false>(m.ph, Q, vert ? off : s[q], vert ? s[q] : off, h_, c_, err).T;
+                }
+                last = Tk[n - 1];
+            }
+            const double* cw = boundary ? d.ce : d.cm;
+            const int count = boundary ? n + 1 : n;
+            double g = 0;
+            for (int k = 0; k + 1 < count; ++k) g += cw[k] * (Tk[k] - last);
+            g /= ds;
+            dflux -= wq * d.kappa * g;
+            rate += wq * d.kappa / ds * (boundary ? d.ce_abs : d.cm_abs);
+        }
+        fluxH += dflux;
+    }
+    // canonical orientation for storage
+    const double o = boundary ? sgn : 1.0;
+    d.face[f] = o * fluxH;
+    d.face[(nxf + nyf) + f] = o * fluxC;
+    d.face[2 * (nxf + nyf) + f] = o * fluxMix;
+    d.face[3 * (nxf + nyf) + f] = rate;
+    if (err) atomicOr(d.error, err);
+}
+
+__global__ void coupled_gather_kernel(CoupledDev d, double* rhsH, double* rhsC, double* rate,
+                                      double* div, double* bnd)
+{
+    const MeshDev& m = *d.m;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = m.nx, ny = m.ny;
+    if (c >= nx * ny) return;
+    const size_t N = size_t(nx) * ny;
+    const int i = c % nx, j = c / nx;
+    const int64_t nxf = int64_t(ny) * (nx + 1), nf = nxf + int64_t(ny + 1) * nx;
+    const int64_t fl = int64_t(j) * (nx + 1) + i, fr = fl + 1;
+    const int64_t fb = nxf + int64_t(j) * nx + i, ft = fb + nx;
+    const double* F = d.face;
+    const double ia = m.inv_area;
+    rhsH[c] = (F[fl] - F[fr] + F[fb] - F[ft]) * ia + d.cell_out[c];
+    rhsC[c] = (F[nf + fl] - F[nf + fr] + F[nf + fb] - F[nf + ft]) * ia;
+    div[c] = (F[2 * nf + fr] - F[2 * nf + fl] + F[2 * nf + ft] - F[2 * nf + fb]) * ia;
+    rate[c] = (F[3 * nf + fl] + F[3 * nf + fr] + F[3 * nf + fb] + F[3 * nf + ft]) * ia +
+              d.cell_out[3 * N + c];
+    // outward boundary fluxes of this cell (H, C) and its source total
+    double bh = 0, bc = 0;
+    if (i == 0) { bh -= F[fl]; bc -= F[nf + fl]; }
+    if (i == nx - 1) { bh += F[fr]; bc += F[nf + fr]; }
+    if (j == 0) { bh -= F[fb]; bc -= F[nf + fb]; }
+    if (j == ny - 1) { bh += F[ft]; bc += F[nf + ft]; }
+    bnd[c] = bh;
+    bnd[N + c] = bc;
+    bnd[2 * N + c] = d.cell_out[c] * m.area;
+}
+
+// Fixed-order reductions of k arrays of n values: sum (Neumaier), or max |x|.
+// One block; thread t takes indices t, t+T, ...; then a fixed tree.
+__global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__ x, int64_t n,
+                                                      int k, const int* kind, double* out)
+{
+    __shared__ double ss[1024], se[1024];
+    for (int a = 0; a < k; ++a) {
+        const double* v = x + a * n;
+        const int mode = kind[a];          // 0 sum, 1 max |x|, 2 min, 3 max
+        double s = mode == 2 ? INFINITY : (mode == 3 ? -INFINITY : 0.0), e = 0.0;
+        for (int64_t t = threadIdx.x; t < n; t += blockDim.x) {
+            const double y = v[t];
+            if (mode == 0) two_sum_acc(s, e, y);
+            else if (mode == 1) s = fmax(s, fabs(y)) + 0.0 * y;   // a NaN propagates
+            else if (mode == 2) s = (y < s || y != y) ? y : s;
+            else s = (y > s || y != y) ? y : s;
+        }
+        ss[threadIdx.x] = s;
+        se[threadIdx.x] = e;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) {
+                const double o = ss[threadIdx.x + w];
+                double& r = ss[threadIdx.x];
+                if (mode == 0) {
+                    double s2 = r, e2 = se[threadIdx.x] + se[threadIdx.x + w];
+                    two_sum_acc(s2, e2, o);
+                    r = s2;
+                    se[threadIdx.x] = e2;
+                } else if (mode == 1) r = fmax(r, o) + 0.0 * o;
+                else if (mode == 2) r = (o < r || o != o) ? o : r;
+                else r = (o > r || o != o) ? o : r;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[a] = mode == 0 ? ss[0] + se[0] : ss[0];
+        __syncthreads();
+    }
+}
+
+// SSPRK2 stage combinations of src/couple/evolve.cpp:469-479:
+// stage 1: u = u_old + dt*r1;  stage 2: u = 0.5*u_old + 0.5*(u + dt*r2).
+__global__ void coupled_stage_kernel(int64_t n, int stage, double dt, const double* __restrict__ oldH,
+                                     const double* __restrict__ oldC, const double* __restrict__ rH,
+                                     const double* __restrict__ rC, double* H, double* C)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (stage == 1) {
+        H[k] = dt * rH[k] + oldH[k];
+        C[k] = dt * rC[k] + oldC[k];
+    } else {
+        const double h = H[k] + dt * rH[k], c = C[k] + dt * rC[k];
+        H[k] = .5 * oldH[k] + .5 * h;
+        C[k] = .5 * oldC[k] + .5 * c;
+    }
+}
+
+hipError_t launch_coupled_rhs(const CoupledDev& d, int nx, int ny, const double* H, const double* C,
+                              double* rhsH, double* rhsC, double* rate, double* div, double* bnd,
+                              hipStream_t stream)
+{
+    const int N = nx * ny;
+    const int64_t nf = int64_t(ny) * (nx + 1) + int64_t(ny + 1) * nx;
+    coupled_cell_kernel<<<(N + 127) / 128, 128, 0, stream>>>(d, H, C);
+    coupled_face_kernel<<<unsigned((nf + 127) / 128), 128, 0, stream>>>(d);
+    coupled_gather_kernel<<<(N + 255) / 256, 256, 0, stream>>>(d, rhsH, rhsC, rate, div, bnd);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const double* x, int64_t n, int k, const int* kind, double* out,
+                         hipStream_t stream)
+{
+    reduce_kernel<<<1, 1024, 0, stream>>>(x, n, k, kind, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_coupled_stage(int64_t n, int stage, double dt, const double* oldH,
+                                const double* oldC, const double* rH, const double* rC, double* H,
+                                double* C, hipStream_t stream)
+{
+    coupled_stage_kernel<<<unsigned((n + 255) / 256), 256, 0, stream>>>(n, stage, dt, oldH, oldC,
+                                                                        rH, rC, H, C);
+    return hipGetLastError();
+}
+
+}  // namespace mp

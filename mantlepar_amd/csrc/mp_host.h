@@ -63,6 +63,9 @@ mp_status validate_weno(const mp_weno_options& o);
 // the sampling of src/mlweno/diffusiveflux.cpp:256-338) for a uniform mesh.
 mp_status set_limiter_tables(MeshDev& m, int cell_points, int samples, double extent,
                              double c_upper);
+// First-derivative weights at x of the Lagrange interpolant through nodes
+// 0..npts-1 (src/mlweno/diffusiveflux.cpp:203-254), rounded to double.
+void lagrange_weights(int npts, long double x, double* w);
 // Thermal diffusion coefficients of the legacy preheat form
 // (src/couple/preheat.cpp:176-190, src/mlweno/diffusiveflux.cpp:256-338).
 mp_status set_diffusion_tables(MeshDev& m, double kappa, int samples, double extent);

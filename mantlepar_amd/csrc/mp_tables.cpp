@@ -36,6 +36,7 @@ mp_status status_from_bits(uint32_t bits)
     if (bits & kErrBackflow) return MP_ERR_BACKFLOW;
     // the limiter runs before any phase evaluation of the stage
     if (bits & kErrInadmissible) return MP_ERR_ARG;
+    if (bits & (kErrCoupledRange | kErrLFSpeed)) return MP_ERR_ARG;
     if (bits & kErrComposition) return MP_ERR_COMPOSITION;
     if (bits & kErrFraction) return MP_ERR_PHASE;
     if (bits & kErrInflow) return MP_ERR_ARG;
@@ -47,6 +48,8 @@ const char* describe_bits(uint32_t bits)
 {
     if (bits & kErrBackflow) return "Backflow encountered on an outflow-only boundary";
     if (bits & kErrInadmissible) return "Inadmissible cell average in coupled reconstruction";
+    if (bits & kErrCoupledRange) return "Coupled Darcy-Stokes requires nonnegative temperature and porosity < 1";
+    if (bits & kErrLFSpeed) return "Global LF speed is below the stage flow speed";
     if (bits & kErrComposition) return "CD must lie in [0, Xe]";
     if (bits & kErrFraction) return "Computed phase fraction is outside [0, 1]";
     if (bits & kErrInflow) return "Missing inflow profile";
@@ -287,7 +290,7 @@ mp_status set_limiter_tables(MeshDev& m, int cell_points, int samples, double ex
 // First-derivative weights at x of the Lagrange interpolant through the
 // nodes 0..npts-1 (unit spacing), each rounded to double as the reference
 // stores them (src/mlweno/diffusiveflux.cpp:203-254): w_i = L_i'(x).
-static void lagrange_derivative(int npts, long double x, double* w)
+void lagrange_weights(int npts, long double x, double* w)
 {
     for (int i = 0; i < npts; ++i) {
         long double d = 0;
@@ -315,12 +318,12 @@ mp_status set_diffusion_tables(MeshDev& m, double kappa, int samples, double ext
     // samples only the in-cell half differs from the last sample, so
     // F = -kappa * length * S / ds * (H_L - H_R), S = sum_{k<n/2} w_k (< 0).
     double wm[kMaxLimS + 1], we[kMaxLimS + 1];
-    lagrange_derivative(n, (n - 1) / 2.0L, wm);
+    lagrange_weights(n, (n - 1) / 2.0L, wm);
     long double S = 0;
     for (int k = 0; k < n / 2; ++k) S += wm[k];
     // Boundary face: n in-cell samples and the face node at the end, ds = f w/n;
     // F = kappa * Hn / ds * (H_in - b) per unit length, Hn = -sum_{k<n} w_k.
-    lagrange_derivative(n + 1, (long double)n, we);
+    lagrange_weights(n + 1, (long double)n, we);
     long double Hn = 0;
     for (int k = 0; k < n; ++k) Hn -= we[k];
     const double dsx = (double)((long double)extent * m.dx * 2.0L / (n - 1));

@@ -1,0 +1,115 @@
+"""Coupled transport law on the GPU against the oracle, through the C ABI.
+
+Bars (FP64 on both sides): the RHS fields (rhs_H, rhs_C, rate, mixture
+divergence, cell-averaged T and phi) within 1e-12 of the oracle relative to
+each field's maximum, the totals within 1e-12; the SSPRK2 loop takes the same
+number of retries on every step, the same dt to 1e-12, and ends within 1e-11
+of the oracle's state; errors carry the reference's messages.
+"""
+import numpy as np
+import pytest
+
+import mantlepar_amd as M
+import oracle as O
+from mantlepar_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+XE = 0.4
+THETA, COOLING, KAPPA = 1.0, 0.05, 2e-4
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def pair(spec, darcy=0.05, H_in=0.55, kappa=KAPPA, state=None):
+    flow = W.coupled_flow(spec, darcy=darcy)
+    bc, inflow, ttypes, tvals = W.coupled_boundary(spec, H_in=H_in)
+    H, Cc = W.coupled_state(spec) if state is None else state
+    g = M.Coupled(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q,
+                  phase=W.config_phase(hypereutectic=0),
+                  params=M.default_coupled(theta=THETA, cooling=COOLING,
+                                           thermal_diffusivity=kappa))
+    g.set_flow(flow)
+    g.set_boundary(0, bc)
+    g.set_boundary(1, bc)
+    g.set_inflow(*inflow)
+    g.set_thermal_boundary(ttypes, tvals)
+    g.upload(H, Cc)
+    vx = np.zeros((spec.q, spec.ny, spec.nx + 1))
+    vy = np.zeros((spec.q, spec.ny + 1, spec.nx))
+    o = O.Transport(W.vertex_coords(spec), spec.nx, spec.ny, vx, vy, q=spec.q, bc=bc,
+                    inflow_H=inflow[0], inflow_C=inflow[1])
+    o.set_limiter(True, 5, 4, 0.9, XE)
+    o.set_diffusion(kappa, 4, 0.9, types={"bottom": np.ones(spec.nx)}, values=tvals)
+    pm = O.PhaseModel(O.Material.defaults(Tm0=1600.0, Te0=600.0, cp=1200.0, L=1.2e6, Xe=XE))
+    cp = O.CoupledParams.make(LD=pm.derived["LD"], theta=THETA, cooling=COOLING, kappa=kappa,
+                              admissible_c_max=XE)
+    return g, o, pm, cp, flow, H, Cc
+
+
+@pytest.mark.parametrize("shape", [(16, 14), (40, 36)])
+def test_rhs_matches_oracle(shape):
+    spec = W.MeshSpec(*shape, seed=2)
+    g, o, pm, cp, flow, H, Cc = pair(spec)
+    got = g.rhs()
+    want = o.coupled_rhs(pm, cp, flow, H, Cc)
+    for k in ("rhsH", "rhsC", "rate", "div", "T", "phi"):
+        assert rel(got[k], want[k]) <= 1e-12, (k, rel(got[k], want[k]))
+    for a, b in (("boundary_H", "boundaryH"), ("boundary_C", "boundaryC"), ("source_H", "sourceH"),
+                 ("maximum_rate", "maximumRate"), ("divergence", "divergence")):
+        assert abs(got[a] - want[b]) <= 1e-12 * abs(want[b]), (a, got[a], want[b])
+    # the workload exercises every term
+    assert want["boundaryH"] != 0 and want["sourceH"] != 0 and np.abs(want["div"]).max() > 0
+    g.close()
+
+
+def test_step_loop_matches_oracle_with_retries():
+    spec = W.MeshSpec(16, 14, seed=2)
+    _, Cc = W.coupled_state(spec)
+    H = np.full_like(Cc, 0.599)
+    g, o, pm, cp, flow, H, Cc = pair(spec, darcy=0.5, H_in=0.9, kappa=0.0, state=(H, Cc))
+    tg, sg = g.run(6, cfl=0.5, initial=1.0, maximum=1.0)
+    Ho, Co, to, so = o.coupled_run(pm, cp, flow, H, Cc, 6, cfl=0.5, initial=1.0, maximum=1.0)
+    assert len(sg) == len(so) == 6
+    assert [s["retries"] for s in sg] == [s["retries"] for s in so]
+    assert sum(s["retries"] for s in sg) >= 3
+    for a, b in zip(sg, so):
+        assert abs(a["dt"] - b["dt"]) <= 1e-12 * b["dt"]
+        assert abs(a["balance_H"]) < 1e-10 and abs(a["balance_C"]) < 1e-10
+    assert abs(tg - to) <= 1e-12 * to
+    Hg, Cg = g.download()
+    assert rel(Hg, Ho) <= 1e-11 and rel(Cg, Co) <= 1e-11
+    g.close()
+
+
+def test_general_run_matches_oracle():
+    spec = W.MeshSpec(24, 20, seed=2)
+    g, o, pm, cp, flow, H, Cc = pair(spec)
+    tg, sg = g.run(5, cfl=0.5, initial=1e-3, maximum=1.0)
+    Ho, Co, to, so = o.coupled_run(pm, cp, flow, H, Cc, 5, cfl=0.5, initial=1e-3, maximum=1.0)
+    assert [s["retries"] for s in sg] == [s["retries"] for s in so]
+    Hg, Cg = g.download()
+    assert rel(Hg, Ho) <= 1e-11 and rel(Cg, Co) <= 1e-11
+    g.close()
+
+
+def test_errors_keep_the_state():
+    spec = W.MeshSpec(16, 14, seed=2)
+    H, Cc = W.coupled_state(spec)
+    bad = Cc.copy()
+    bad[3, 3] = XE + 0.01
+    g, *_ = pair(spec, state=(H, bad))
+    with pytest.raises(M.MantleError, match="Inadmissible cell average") as ei:
+        g.rhs()
+    assert ei.value.code == 1
+    g.close()
+    g, o, pm, cp, flow, H, Cc = pair(spec)
+    r = g.rhs()
+    big = 4.0 / r["maximum_rate"]
+    with pytest.raises(M.MantleError, match="exceeds the phase-coupled CFL bound"):
+        g.run(1, cfl=0.5, initial=big, maximum=big, fixed=True)
+    Hg, Cg = g.download()
+    assert np.array_equal(Hg, H) and np.array_equal(Cg, Cc)
+    g.close()
