@@ -40,6 +40,10 @@ def parse():
                    help="steps of the 8192x8192 full-chip run (0: skip it)")
     p.add_argument("--limiter-steps", type=int, default=20,
                    help="steps of the FP64 bound-limiter leg (0: skip)")
+    p.add_argument("--coupled-steps", type=int, default=3,
+                   help="SSPRK2 steps of the coupled-law leg (0: skip)")
+    p.add_argument("--coupled-nx", type=int, default=1024)
+    p.add_argument("--coupled-ny", type=int, default=2048)
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--tune-cache", default=os.path.join(HERE, "gpurun_out", "batch_tune.json"))
@@ -428,6 +432,61 @@ def bench_diffusion(args, spec, H, Cc, vx, vy, phase):
     return out
 
 
+def bench_coupled(args):
+    """The coupled transport law (FP64): SSPRK2 steps with the limiter, phase-based
+    fluxes, diffusion of T and the cooling source, flow samples held fixed; and
+    the oracle's restatement on a bounded sample of the same workload."""
+    import mantlepar_amd as M
+    from mantlepar_amd import workloads as W
+    spec = W.MeshSpec(args.coupled_nx, args.coupled_ny, seed=2)
+    flow = W.coupled_flow(spec)
+    bc, inflow, tt, tv = W.coupled_boundary(spec)
+    H, Cc = W.coupled_state(spec)
+    g = M.Coupled(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q,
+                  phase=W.config_phase(hypereutectic=0),
+                  params=M.default_coupled(theta=1.0, cooling=0.05, thermal_diffusivity=2e-4))
+    g.set_flow(flow)
+    g.set_boundary(0, bc)
+    g.set_boundary(1, bc)
+    g.set_inflow(*inflow)
+    g.set_thermal_boundary(tt, tv)
+    g.upload(H, Cc)
+    g.run(1, cfl=0.5, initial=1e-3, maximum=1.0)            # warm-up
+    g.upload(H, Cc)
+    t0 = time.perf_counter()
+    _, steps = g.run(args.coupled_steps, cfl=0.5, initial=1e-3, maximum=1.0)
+    el = time.perf_counter() - t0
+    g.close()
+    out = {"cells": spec.cells, "steps": len(steps), "seconds": el,
+           "cell_steps_per_s": spec.cells * len(steps) / el,
+           "rhs_per_step": 2, "retries": sum(s["retries"] for s in steps),
+           "workload": f"{spec.nx}x{spec.ny}, flow samples fixed, limiter on, kappa 2e-4, "
+                       "cooling 0.05, Dirichlet bottom temperature, bottom inflow, top outflow"}
+    log(f"coupled: {out['cell_steps_per_s']:.3e} cell-steps/s ({len(steps)} SSPRK2 steps)")
+    if not args.no_cpu:
+        try:
+            import oracle as O
+            sm = W.MeshSpec(64, 64, seed=2)
+            sflow = W.coupled_flow(sm)
+            sbc, sin_, _, stv = W.coupled_boundary(sm)
+            sH, sC = W.coupled_state(sm)
+            z = (np.zeros((sm.q, sm.ny, sm.nx + 1)), np.zeros((sm.q, sm.ny + 1, sm.nx)))
+            o = O.Transport(W.vertex_coords(sm), sm.nx, sm.ny, *z, q=sm.q, bc=sbc,
+                            inflow_H=sin_[0], inflow_C=sin_[1])
+            o.set_limiter(True, 5, 4, 0.9, 0.4)
+            o.set_diffusion(2e-4, 4, 0.9, types={"bottom": np.ones(sm.nx)}, values=stv)
+            pm = O.PhaseModel(O.Material.defaults(Tm0=1600.0, Te0=600.0, cp=1200.0, L=1.2e6, Xe=0.4))
+            cp = O.CoupledParams.make(LD=pm.derived["LD"], theta=1.0, cooling=0.05, kappa=2e-4)
+            t0 = time.perf_counter()
+            _, _, _, st = o.coupled_run(pm, cp, sflow, sH, sC, 2, cfl=0.5, initial=1e-3, maximum=1.0)
+            ce = time.perf_counter() - t0
+            out["cpu_port"] = {"cell_steps_per_s": sm.cells * len(st) / ce, "cores": 1,
+                               "sample": f"64x64, {len(st)} steps", "seconds": ce}
+        except Exception as e:   # reported, never required
+            out["cpu_port"] = {"error": str(e)}
+    return out
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -529,6 +588,8 @@ def main():
     if args.limiter_steps > 0:
         out["limiter_fp64"] = bench_limiter(args, spec, H, Cc, vx, vy, phase)
         out["diffusion"] = bench_diffusion(args, spec, H, Cc, vx, vy, phase)
+    if args.coupled_steps > 0:
+        out["coupled_fp64"] = bench_coupled(args)
     if args.full_steps > 0:
         out["full_chip"] = bench_full_chip(args)
     if not (args.no_batch and args.no_serve):

@@ -22,7 +22,7 @@ struct mp_coupled {
     int64_t nxf = 0, nyf = 0;
     int npts = 0;                       // cell points
     DeviceBuffer d_md, state, old, rbuf, first, rd, bnd, blend, cell_out, face, flowx, flowy, cmix;
-    DeviceBuffer bcH, bcC, inflow, tbc, tval, error, red, kinds;
+    DeviceBuffer bcH, bcC, inflow, tbc, tval, error, red, kinds, part;
     bool have_flow = false;
     double c_max = 0;                   // admissible C bound
 
@@ -104,6 +104,7 @@ mp_status mp_coupled::init(const mp_mesh_desc& m, const mp_phase_params& p,
     MP_CALL(error.alloc(sizeof(uint32_t)));
     MP_CALL(red.alloc(8 * sizeof(double)));
     MP_CALL(kinds.alloc(8 * sizeof(int)));
+    MP_CALL(part.alloc(2 * 3 * size_t(kReducePartials) * sizeof(double)));
     MP_HIP(hipMemsetAsync(state.ptr, 0, state.bytes, stream));
     MP_HIP(hipMemsetAsync(flowx.ptr, 0, flowx.bytes, stream));
     MP_HIP(hipMemsetAsync(flowy.ptr, 0, flowy.bytes, stream));
@@ -293,8 +294,8 @@ mp_status mp_coupled::rhs_on(const double* H, const double* C, double* rH, doubl
     const int k1[3] = {0, 0, 0}, k2[2] = {3, 1};
     MP_HIP(hipMemcpyAsync(kinds.ptr, k1, sizeof k1, hipMemcpyHostToDevice, stream));
     MP_HIP(hipMemcpyAsync(kinds.as<int>() + 3, k2, sizeof k2, hipMemcpyHostToDevice, stream));
-    MP_HIP(launch_reduce(bnd.as<double>(), int64_t(N), 3, kinds.as<int>(), red.as<double>(), stream));
-    MP_HIP(launch_reduce(r, int64_t(N), 2, kinds.as<int>() + 3, red.as<double>() + 3, stream));
+    MP_HIP(launch_reduce(bnd.as<double>(), int64_t(N), 3, kinds.as<int>(), red.as<double>(), part.as<double>(), stream));
+    MP_HIP(launch_reduce(r, int64_t(N), 2, kinds.as<int>() + 3, red.as<double>() + 3, part.as<double>(), stream));
     double h[5];
     uint32_t bits = 0;
     MP_HIP(hipMemcpyAsync(h, red.ptr, sizeof h, hipMemcpyDeviceToHost, stream));
@@ -335,9 +336,9 @@ mp_status mp_coupled::admissible(const double* S, bool& ok, double& intH, double
     MP_HIP(hipMemcpyAsync(kinds.ptr, k, sizeof k, hipMemcpyHostToDevice, stream));
     double* o = red.as<double>();
     // min H, min C | max H, max C | sum H, sum C (H and C are contiguous)
-    MP_HIP(launch_reduce(S, int64_t(N), 2, kinds.as<int>(), o, stream));
-    MP_HIP(launch_reduce(S, int64_t(N), 2, kinds.as<int>() + 2, o + 2, stream));
-    MP_HIP(launch_reduce(S, int64_t(N), 2, kinds.as<int>() + 4, o + 4, stream));
+    MP_HIP(launch_reduce(S, int64_t(N), 2, kinds.as<int>(), o, part.as<double>(), stream));
+    MP_HIP(launch_reduce(S, int64_t(N), 2, kinds.as<int>() + 2, o + 2, part.as<double>(), stream));
+    MP_HIP(launch_reduce(S, int64_t(N), 2, kinds.as<int>() + 4, o + 4, part.as<double>(), stream));
     double h[6];
     MP_HIP(hipMemcpyAsync(h, o, sizeof h, hipMemcpyDeviceToHost, stream));
     MP_HIP(hipStreamSynchronize(stream));

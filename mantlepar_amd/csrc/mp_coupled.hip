@@ -21,6 +21,8 @@
 // made once.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "mp_common.h"
 #include "mp_coupled.h"
 #include "mp_device.h"
@@ -337,43 +339,78 @@ __global__ void coupled_gather_kernel(CoupledDev d, double* rhsH, double* rhsC, 
     bnd[2 * N + c] = d.cell_out[c] * m.area;
 }
 
-// Fixed-order reductions of k arrays of n values: sum (Neumaier), or max |x|.
-// One block; thread t takes indices t, t+T, ...; then a fixed tree.
-__global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__ x, int64_t n,
-                                                      int k, const int* kind, double* out)
+// Fixed-order reductions of k arrays of n values: sum (Neumaier), or max |x|,
+// min, max.  Two passes: block b reduces the contiguous chunk b of every
+// array in a fixed tree, then one block combines the partials in order, so the
+// result does not depend on scheduling.
+__device__ __forceinline__ void red_combine(int mode, double& s, double& e, double o, double oe)
 {
-    __shared__ double ss[1024], se[1024];
+    if (mode == 0) {
+        e += oe;
+        two_sum_acc(s, e, o);
+    } else if (mode == 1) s = fmax(s, fabs(o)) + 0.0 * o;   // a NaN propagates
+    else if (mode == 2) s = (o < s || o != o) ? o : s;
+    else s = (o > s || o != o) ? o : s;
+}
+
+__device__ __forceinline__ double red_init(int mode)
+{
+    return mode == 2 ? INFINITY : (mode == 3 ? -INFINITY : 0.0);
+}
+
+__device__ void block_reduce(int mode, double& s, double& e, double* ss, double* se)
+{
+    ss[threadIdx.x] = s;
+    se[threadIdx.x] = e;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            double a = ss[threadIdx.x], ae = se[threadIdx.x];
+            red_combine(mode, a, ae, ss[threadIdx.x + w], se[threadIdx.x + w]);
+            ss[threadIdx.x] = a;
+            se[threadIdx.x] = ae;
+        }
+        __syncthreads();
+    }
+    s = ss[0];
+    e = se[0];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void reduce_partial_kernel(const double* __restrict__ x, int64_t n,
+                                                             int k, const int* kind, double* part)
+{
+    __shared__ double ss[256], se[256];
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = int64_t(blockIdx.x) * chunk, b1 = min(n, b0 + chunk);
     for (int a = 0; a < k; ++a) {
         const double* v = x + a * n;
-        const int mode = kind[a];          // 0 sum, 1 max |x|, 2 min, 3 max
-        double s = mode == 2 ? INFINITY : (mode == 3 ? -INFINITY : 0.0), e = 0.0;
-        for (int64_t t = threadIdx.x; t < n; t += blockDim.x) {
+        const int mode = kind[a];
+        double s = red_init(mode), e = 0.0;
+        for (int64_t t = b0 + threadIdx.x; t < b1; t += blockDim.x) {
             const double y = v[t];
             if (mode == 0) two_sum_acc(s, e, y);
-            else if (mode == 1) s = fmax(s, fabs(y)) + 0.0 * y;   // a NaN propagates
-            else if (mode == 2) s = (y < s || y != y) ? y : s;
-            else s = (y > s || y != y) ? y : s;
+            else red_combine(mode, s, e, y, 0.0);
         }
-        ss[threadIdx.x] = s;
-        se[threadIdx.x] = e;
-        __syncthreads();
-        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < w) {
-                const double o = ss[threadIdx.x + w];
-                double& r = ss[threadIdx.x];
-                if (mode == 0) {
-                    double s2 = r, e2 = se[threadIdx.x] + se[threadIdx.x + w];
-                    two_sum_acc(s2, e2, o);
-                    r = s2;
-                    se[threadIdx.x] = e2;
-                } else if (mode == 1) r = fmax(r, o) + 0.0 * o;
-                else if (mode == 2) r = (o < r || o != o) ? o : r;
-                else r = (o > r || o != o) ? o : r;
-            }
-            __syncthreads();
+        block_reduce(mode, s, e, ss, se);
+        if (threadIdx.x == 0) {
+            part[(size_t(a) * gridDim.x + blockIdx.x) * 2] = s;
+            part[(size_t(a) * gridDim.x + blockIdx.x) * 2 + 1] = e;
         }
-        if (threadIdx.x == 0) out[a] = mode == 0 ? ss[0] + se[0] : ss[0];
-        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restrict__ part, int nb,
+                                                           int k, const int* kind, double* out)
+{
+    __shared__ double ss[256], se[256];
+    for (int a = 0; a < k; ++a) {
+        const int mode = kind[a];
+        double s = red_init(mode), e = 0.0;
+        for (int t = threadIdx.x; t < nb; t += blockDim.x)
+            red_combine(mode, s, e, part[(size_t(a) * nb + t) * 2], part[(size_t(a) * nb + t) * 2 + 1]);
+        block_reduce(mode, s, e, ss, se);
+        if (threadIdx.x == 0) out[a] = mode == 0 ? s + e : s;
     }
 }
 
@@ -408,9 +445,11 @@ hipError_t launch_coupled_rhs(const CoupledDev& d, int nx, int ny, const double*
 }
 
 hipError_t launch_reduce(const double* x, int64_t n, int k, const int* kind, double* out,
-                         hipStream_t stream)
+                         double* part, hipStream_t stream)
 {
-    reduce_kernel<<<1, 1024, 0, stream>>>(x, n, k, kind, out);
+    const int nb = int(std::min<int64_t>(kReducePartials, std::max<int64_t>(1, (n + 4095) / 4096)));
+    reduce_partial_kernel<<<nb, 256, 0, stream>>>(x, n, k, kind, part);
+    reduce_final_kernel<<<1, 256, 0, stream>>>(part, nb, k, kind, out);
     return hipGetLastError();
 }
 
