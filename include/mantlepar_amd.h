@@ -366,7 +366,7 @@ mp_status mp_face_velocity_from_samples(int32_t nx, int32_t ny, int32_t q, int64
 /* ---------------------------------------------------- coupled transport --- */
 /* The phase-coupled transport law of MantlePar's evolution (one rank's share
  * of a uniform mesh, FP64): the RHS of src/couple/evolve.cpp:279-384 and the
- * SSPRK2 loop with admissibility/CFL retries of :411-507.  Traces come from
+ * SSPRK2 loop with admissibility/CFL retries of :428-511.  Traces come from
  * the limited reconstruction (StateReconstruction::Update + Limit); H carries
  * the temperature flux with the latent term -LD*(1-phi)*v_s.n, C the
  * effective velocity of PhaseTransportVelocities (:36-52); T diffuses by
@@ -427,7 +427,7 @@ mp_status mp_coupled_download_state(mp_coupled* c, double* H, double* C);
  * and phi [N] (any may be NULL) and the totals (RHS, evolve.cpp:279-384). */
 mp_status mp_coupled_rhs(mp_coupled* c, double* rhsH, double* rhsC, double* rate, double* div,
                          double* T, double* phi, mp_coupled_rates* totals);
-/* Up to nsteps accepted SSPRK2 steps from *time (Run, evolve.cpp:411-507):
+/* Up to nsteps accepted SSPRK2 steps from *time (Run, evolve.cpp:428-511):
  * on an error the state and *time are those of the last accepted step. */
 mp_status mp_coupled_run(mp_coupled* c, const mp_coupled_time* time_controls, double* time,
                          int32_t nsteps, mp_coupled_step* steps, int32_t* accepted);

@@ -1,5 +1,5 @@
 // Host side of the coupled transport law: the mp_coupled_* C ABI and the
-// SSPRK2 step loop with admissibility/CFL retries of src/couple/evolve.cpp:411-507.
+// SSPRK2 step loop with admissibility/CFL retries of src/couple/evolve.cpp:428-511.
 #include <cmath>
 #include <cstring>
 #include <vector>

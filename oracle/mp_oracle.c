@@ -2446,7 +2446,7 @@ static double coupled_integral(const mpo_transport* t, const double* u)
     return (double)s;
 }
 
-/* Run, src/couple/evolve.cpp:411-507: SSPRK2 steps with admissibility and CFL
+/* Run, src/couple/evolve.cpp:428-511: SSPRK2 steps with admissibility and CFL
  * retries (dt halved, at most 40) and the conservation balance check; the flow
  * samples stay fixed (the flow solve of every RHS is outside this path).
  * H, C advance in place over nsteps accepted steps from time *time. */

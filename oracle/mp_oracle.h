@@ -339,7 +339,7 @@ int mpo_coupled_rhs(mpo_transport* t, const mpo_phase_model* pm, const mpo_coupl
                     const mpo_flow* flow, const double* H, const double* C, double* rhsH,
                     double* rhsC, double* rate, double* div, double* T, double* phi,
                     mpo_coupled_rates* totals);
-/* SSPRK2 steps of src/couple/evolve.cpp:411-507 with the flow held fixed. */
+/* SSPRK2 steps of src/couple/evolve.cpp:428-511 with the flow held fixed. */
 int mpo_coupled_run(mpo_transport* t, const mpo_phase_model* pm, const mpo_coupled_params* cp,
                     const mpo_flow* flow, const mpo_coupled_time* tc, double* H, double* C,
                     double* time, int nsteps, mpo_coupled_step* steps);

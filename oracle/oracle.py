@@ -708,7 +708,7 @@ class Transport:
 
     def coupled_run(self, phase, params, flow, H, Cc, nsteps, cfl=0.5, initial=1e-2,
                     maximum=1e-2, minimum=1e-12, end=1e30, fixed=False, time=0.0):
-        """SSPRK2 steps with retries (src/couple/evolve.cpp:411-507), flow fixed."""
+        """SSPRK2 steps with retries (src/couple/evolve.cpp:428-511), flow fixed."""
         f, keep = _flow(flow)
         tc = _CoupledTime(cfl, initial, maximum, minimum, end, int(fixed))
         H = np.array(H, dtype=np.float64).ravel()

@@ -1,7 +1,7 @@
 """Oracle pins for the coupled transport law (CPU only).
 
 The reference's coupled RHS (src/couple/evolve.cpp:279-384) and step loop
-(:411-507) need the Stokes-Darcy solve and PETSc, so they do not run here.
+(:428-511) need the Stokes-Darcy solve and PETSc, so they do not run here.
 The oracle's restatement is pinned by:
 
 * PhaseTransportVelocities: the reference's own cases
