@@ -64,6 +64,34 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 3}))
 
 
+_COPY_PROBE = """
+import json, torch
+n = 1 << 27
+x = torch.ones(n, dtype=torch.float64, device="cuda")
+y = torch.empty_like(x)
+for _ in range(3):
+    torch.mul(x, 1.0, out=y)
+s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+s.record()
+for _ in range(10):
+    torch.mul(x, 1.0, out=y)
+e.record()
+torch.cuda.synchronize()
+print(json.dumps(2 * n * 8 / (s.elapsed_time(e) / 10 * 1e-3) / 1e9))
+"""
+
+
+def practical_copy_gbs():
+    """HBM bandwidth a plain device copy reaches on this GPU (1 GiB read + 1 GiB
+    written, 10 launches timed with events), measured in a child process
+    before this one touches the GPU: the practical ceiling beside the 8 TB/s
+    published peak."""
+    import subprocess
+    r = subprocess.run([sys.executable, "-c", _COPY_PROBE], capture_output=True, text=True,
+                       timeout=180)
+    return float(r.stdout.strip().splitlines()[-1])
+
+
 def algorithmic_bytes(q, state_b=8, vel_b=8, phase_b=8):
     """Bytes per cell of one step and of each stage launch.
     Stage 1 reads 2 fields and writes 2; stages 2 and 3 read 4 and write 2;
@@ -496,6 +524,11 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path shards by mesh strips; see the design notes).", file=sys.stderr)
         sys.exit(2)
+    try:
+        copy_gbs = practical_copy_gbs()
+    except Exception as e:  # informative only
+        copy_gbs = None
+        log(f"copy bandwidth not measured: {e}")
     import mantlepar_amd as M
     from mantlepar_amd import workloads as W
 
@@ -518,7 +551,7 @@ def main():
             for rows in (8, 12, 16, 24, 32):
                 t.set_tile(rows)
                 t.upload(H, Cc)
-                ms = min(t.profile(4, dt)["ms_total"] for _ in range(2))
+                ms = min(t.profile(4, dt)["ms_total"] for _ in range(3))
                 if best is None or ms < best[0]:
                     best = (ms, rows)
             t.set_tile(best[1])
@@ -585,6 +618,8 @@ def main():
                      "ms_per_launch": prof["ms_stage"], "rows_per_item": mixed["rows"]},
         "conservation": {"total_H": mixed["report"]["total_H"], "total_C": mixed["report"]["total_C"]},
     }
+    out["roofline"]["practical_copy_GBs"] = copy_gbs
+    out["roofline"]["frac_of_practical_copy"] = achieved / copy_gbs if copy_gbs else None
     if args.limiter_steps > 0:
         out["limiter_fp64"] = bench_limiter(args, spec, H, Cc, vx, vy, phase)
         out["diffusion"] = bench_diffusion(args, spec, H, Cc, vx, vy, phase)

@@ -11,6 +11,16 @@ namespace mp {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// The dispatcher places workgroup b on XCD b % 8.  Renumber the workgroups so
+// each XCD takes one contiguous run of work items (a slab of row bands): the
+// halo rows a band shares with the band above then sit in the same L2.  A
+// bijection of [0, G) for any G.
+__device__ __forceinline__ int xcd_slab_block(int b, int G)
+{
+    const int full = G / 8, rem = G % 8, x = b % 8, k = b / 8;
+    return x * full + min(x, rem) + k;
+}
+
 // Compensated accumulation s + e += x (Neumaier); contraction would break it.
 __device__ __forceinline__ void two_sum_acc(double& s, double& e, double x)
 {

@@ -346,7 +346,7 @@ template <int MODE, bool PHASE, int NQ, bool LIM = false>
 __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void stage_sweep_kernel(const MeshDev* __restrict__ mesh,
                                                           StagePtrs sp, int nitems)
 {
-    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int item = xcd_slab_block(int(blockIdx.x), int(gridDim.x)) * 4 + (threadIdx.x >> 6);
     if (item >= nitems) return;
     const MeshDev& m = *mesh;
     const int strip = item % sp.strips, band = item / sp.strips;

@@ -421,7 +421,7 @@ template <bool PHASE>
 __global__ __launch_bounds__(256, 3) void stage32_kernel(const MeshDev* __restrict__ mesh, StagePtrs sp, int nitems)
 {
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    const int item = blockIdx.x * 4 + wave;
+    const int item = xcd_slab_block(int(blockIdx.x), int(gridDim.x)) * 4 + wave;
     if (item >= nitems) return;
     const MeshDev& m = *mesh;
     const int strip = item % sp.strips, band = item / sp.strips;
