@@ -192,6 +192,14 @@ mp_status mp_transport_set_diffusion(mp_transport* t, double kappa, int32_t diff
  * (the default). */
 mp_status mp_transport_set_thermal_boundary(mp_transport* t, const uint8_t* types,
                                             const double* values);
+/* Lithostatic phase pressure, replacing the pressure model "lithostatic"
+ * (src/couple/initialization.cpp:155-162 with EutecticModel::staticPressure,
+ * src/phase/eutectic_rescaled.cpp:151-161): the phase of each cell is
+ * evaluated at P = rho*g*(surface_y - y_c)*l0, y_c the cell centre
+ * (nondimensional), instead of the constant phase.pressure.  surface_y must
+ * be at or above the mesh top y1 (MP_ERR_ARG otherwise).  enable = 0 returns
+ * to the constant pressure. */
+mp_status mp_transport_set_lithostatic(mp_transport* t, int32_t enable, double surface_y);
 /* Limiter factors theta[N] of H and C for the current state (diagnostics). */
 mp_status mp_transport_limits(mp_transport* t, double* thetaH, double* thetaC);
 
@@ -421,6 +429,13 @@ mp_status mp_coupled_set_boundary(mp_coupled* c, int32_t field, const uint8_t* b
 mp_status mp_coupled_set_inflow(mp_coupled* c, const double* inflow_H, const double* inflow_C);
 /* EvaluateThermalBoundary data per boundary point, as mp_transport_set_thermal_boundary. */
 mp_status mp_coupled_set_thermal_boundary(mp_coupled* c, const uint8_t* types, const double* values);
+/* Lithostatic phase pressure (src/couple/initialization.cpp:155-162): every
+ * phase evaluation of the RHS -- cell Gauss points, face Gauss points (both
+ * sides and the inflow temperature) and normal-line diffusion samples -- uses
+ * P = staticPressure(surface_y - y) at its own point, as c.pressure(p, t)
+ * does in src/couple/evolve.cpp:189, 273, 318.  surface_y >= mesh top.
+ * enable = 0 returns to the constant phase.pressure. */
+mp_status mp_coupled_set_lithostatic(mp_coupled* c, int32_t enable, double surface_y);
 mp_status mp_coupled_upload_state(mp_coupled* c, const double* H, const double* C);
 mp_status mp_coupled_download_state(mp_coupled* c, double* H, double* C);
 /* RHS of the current state: rhs_H, rhs_C, rate, divergence, cell-averaged T

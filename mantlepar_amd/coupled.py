@@ -70,6 +70,13 @@ class Coupled:
         v = None if values is None else _f64(values).ravel()
         check(lib().mp_coupled_set_thermal_boundary(self._h, u8ptr(t), dptr(v)))
 
+    def set_lithostatic(self, surface_y=None):
+        """Pressure model "lithostatic" (src/couple/initialization.cpp:155-162): every
+        phase evaluation at P = staticPressure(surface_y - y) of its own point;
+        None returns to the constant phase pressure."""
+        check(lib().mp_coupled_set_lithostatic(self._h, int(surface_y is not None),
+                                               0.0 if surface_y is None else surface_y))
+
     def upload(self, H, Cd):
         check(lib().mp_coupled_upload_state(self._h, dptr(_f64(H).ravel()), dptr(_f64(Cd).ravel())))
 

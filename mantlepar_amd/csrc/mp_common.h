@@ -31,9 +31,10 @@ enum : uint32_t {
 
 // Binary eutectic parameters as the device uses them.  Tep/Tmp are derived
 // on the host in long double exactly as EutecticModel::boundaries does
-// (src/phase/eutectic_rescaled.cpp:163-170) because P is uniform per mesh.
-// Reciprocals are double-doubles (hi + lo) formed on the host, so the device
-// multiplies where the reference divides, at double-double accuracy.
+// (src/phase/eutectic_rescaled.cpp:163-170): P is uniform per mesh, or, for a
+// lithostatic pressure, uniform per table entry (TepPair).  Reciprocals are
+// double-doubles (hi + lo) formed on the host, so the device multiplies where
+// the reference divides, at double-double accuracy.
 struct PhaseDev {
     double LD;        // L/h0
     double Xe;        // eutectic composition
@@ -44,6 +45,15 @@ struct PhaseDev {
     double ixh, ixl;  // 1/Xe
     double imh, iml;  // 1/(1 - Xe), of the double 1 - Xe the mirror uses
     double ilh, ill;  // 1/LD
+};
+
+// Pressure-corrected eutectic and melting temperatures of one lithostatic
+// pressure P = staticPressure(surface_y - y) (src/phase/eutectic_rescaled.cpp:
+// 143-161, src/couple/initialization.cpp:155-162), rounded on the host as the
+// reference rounds them; they replace PhaseDev's Tep and Tmp where the phase
+// is evaluated at that ordinate.
+struct __attribute__((aligned(16))) TepPair {
+    double tep, tmp;
 };
 
 // Per-face flux moments of the mixed path, built once per velocity refresh.
@@ -85,6 +95,9 @@ struct MeshDev {
     double wv[kMaxQ];           // w_q * dy/2
     double wh[kMaxQ];           // w_q * dx/2
     PhaseDev ph;
+    // Lithostatic pressure: {Tep, Tmp} of each cell row at the cell centre
+    // (null: the constant pressure of ph).
+    const TepPair* tep_row;
     // Bound limiter of the reconstruction (limit = 0: off).  Point offsets in
     // the same normalised units as hx, hy: cell Gauss points lim_cx x lim_cy;
     // in-cell normal-line samples of the right face at x = lim_x*[k] and of the

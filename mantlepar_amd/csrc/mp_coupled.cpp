@@ -1,5 +1,6 @@
 // Host side of the coupled transport law: the mp_coupled_* C ABI and the
 // SSPRK2 step loop with admissibility/CFL retries of src/couple/evolve.cpp:428-511.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -23,6 +24,7 @@ struct mp_coupled {
     int npts = 0;                       // cell points
     DeviceBuffer d_md, state, old, rbuf, first, rd, bnd, blend, cell_out, face, flowx, flowy, cmix;
     DeviceBuffer bcH, bcC, inflow, tbc, tval, error, red, kinds, part;
+    DeviceBuffer tep;                   // lithostatic {Tep, Tmp} tables
     bool have_flow = false;
     double c_max = 0;                   // admissible C bound
 
@@ -40,6 +42,7 @@ struct mp_coupled {
     mp_status set_boundary(int field, const uint8_t* const side[4]);
     mp_status set_inflow(const double* h, const double* c);
     mp_status set_thermal(const uint8_t* types, const double* values);
+    mp_status set_lithostatic(int enable, double surface_y);
     mp_status rhs_on(const double* H, const double* C, double* rH, double* rC,
                      mp_coupled_rates& tot);
     mp_status rhs(double* rH, double* rC, double* rate, double* div, double* T, double* phi,
@@ -282,6 +285,76 @@ mp_status mp_coupled::set_thermal(const uint8_t* types, const double* values)
     return sync_md();
 }
 
+// Lithostatic pressure at every phase evaluation point
+// (src/couple/initialization.cpp:155-162; c.pressure(p, t) at the point p of
+// src/couple/evolve.cpp:189, 273, 318).  Point ordinates follow the reference's
+// construction on the uniform vertices: MapCellPoint for cell points,
+// MapEdgePoint for face points (src/core/integral.cpp:91-106), and
+// ShiftPoint along the unit normal by offset*spacing for the normal-line
+// samples (src/mlweno/diffusiveflux.cpp:76-86, 256-338) -- the samples of a
+// vertical face keep their point's ordinate.
+mp_status mp_coupled::set_lithostatic(int enable, double surface_y)
+{
+    if (!enable) {
+        tep.reset();
+        cd.tep_cell = cd.tep_v = cd.tep_h = cd.tep_hs = nullptr;
+        return MP_OK;
+    }
+    if (!std::isfinite(surface_y)) return fail(MP_ERR_ARG, "surface_y must be finite");
+    if (surface_y < mesh.y1) return fail(MP_ERR_ARG, "surface_y must be at or above the mesh top");
+    const int ny = mesh.ny, nq = mesh.q, n = params.cell_points_per_axis;
+    const int ns = params.diffusion_samples;
+    std::vector<double> y(size_t(ny) + 1);
+    uniform_vertices(mesh.y0, mesh.y1, ny, y.data());
+    double xc[kMaxQ], wc[kMaxQ], xq[kMaxQ], wq[kMaxQ];
+    gauss_legendre(n, xc, wc);
+    gauss_legendre(nq, xq, wq);
+    const size_t ncell = size_t(ny) * n * n, nv = size_t(ny) * nq, nh = size_t(ny + 1) * nq;
+    std::vector<TepPair> tab(ncell + nv + nh + nh * ns);
+    TepPair *tc = tab.data(), *tv = tc + ncell, *th = tv + nv, *ths = th + nh;
+    for (int j = 0; j < ny; ++j)
+        for (int b = 0; b < n; ++b)
+            for (int a = 0; a < n; ++a) {
+                const double lower = map_edge(xc[a], y[j], y[j]);
+                const double upper = map_edge(xc[a], y[j + 1], y[j + 1]);
+                MP_CALL(lithostatic_pair(phase, surface_y, map_edge(xc[b], lower, upper),
+                                         tc[(size_t(j) * n + b) * n + a]));
+            }
+    for (int j = 0; j < ny; ++j)
+        for (int q = 0; q < nq; ++q)
+            MP_CALL(lithostatic_pair(phase, surface_y, map_edge(xq[q], y[j], y[j + 1]),
+                                     tv[size_t(j) * nq + q]));
+    const long double f = params.extent_fraction;
+    for (int j = 0; j <= ny; ++j)
+        for (int q = 0; q < nq; ++q) {
+            const double py = map_edge(xq[q], y[j], y[j]);
+            const size_t fq = size_t(j) * nq + q;
+            MP_CALL(lithostatic_pair(phase, surface_y, py, th[fq]));
+            // available normal distance: to the far face of the cell(s) beside the face
+            const bool bottom = j == 0, top = j == ny;
+            double avail = bottom ? y[1] - py : py - y[j - 1];
+            if (!bottom && !top) avail = std::min(avail, y[j + 1] - py);
+            const double spacing =
+                (double)(f * avail * ((bottom || top) ? 1.0L / ns : 2.0L / (ns - 1)));
+            const long double normal = bottom ? -1.0L : 1.0L;      // outward, or lower -> upper
+            for (int k = 0; k < ns; ++k) {
+                const long double offset = (bottom || top) ? (long double)k - ns
+                                                           : (long double)k - (ns - 1) / 2.0L;
+                const double ys = (double)((long double)py + offset * spacing * normal);
+                MP_CALL(lithostatic_pair(phase, surface_y, ys, ths[fq * ns + k]));
+            }
+        }
+    MP_CALL(tep.alloc(tab.size() * sizeof(TepPair)));
+    MP_HIP(hipMemcpyAsync(tep.ptr, tab.data(), tep.bytes, hipMemcpyHostToDevice, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    TepPair* base = tep.as<TepPair>();
+    cd.tep_cell = base;
+    cd.tep_v = base + ncell;
+    cd.tep_h = base + ncell + nv;
+    cd.tep_hs = base + ncell + nv + nh;
+    return MP_OK;
+}
+
 // One RHS of the state (H, C) into (rH, rC) and the totals; fails with the
 // device's error bits.
 mp_status mp_coupled::rhs_on(const double* H, const double* C, double* rH, double* rC,
@@ -517,6 +590,12 @@ mp_status mp_coupled_set_thermal_boundary(mp_coupled* c, const uint8_t* types, c
 {
     if (!c) return fail(MP_ERR_ARG, "null handle");
     MP_GUARD({ return c->set_thermal(types, values); })
+}
+
+mp_status mp_coupled_set_lithostatic(mp_coupled* c, int32_t enable, double surface_y)
+{
+    if (!c) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return c->set_lithostatic(enable, surface_y); })
 }
 
 mp_status mp_coupled_upload_state(mp_coupled* c, const double* H, const double* C)

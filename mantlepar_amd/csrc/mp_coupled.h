@@ -32,6 +32,14 @@ struct CoupledDev {
     const uint8_t* bcC[4];
     const double* inH[4];        // inflow states [face][q]
     const double* inC[4];
+    // Lithostatic pressure (null: the constant pressure of m->ph): {Tep, Tmp}
+    // at each point's ordinate -- cell points [(j*n + b)*n + a], vertical face
+    // points (and their normal-line samples) [j*q + qi], horizontal face points
+    // [j*q + qi] (j = 0..ny) and their samples [(j*q + qi)*n + k]
+    const TepPair* tep_cell;
+    const TepPair* tep_v;
+    const TepPair* tep_h;
+    const TepPair* tep_hs;
 };
 
 hipError_t launch_coupled_rhs(const CoupledDev& d, int nx, int ny, const double* H, const double* C,

@@ -72,6 +72,14 @@ __device__ __noinline__ PhaseOut point_phase(const PhaseDev& ph, const CellPoly&
     return o;
 }
 
+// Phase parameters of normal-line sample k of face point fq: a vertical face's
+// samples share the point's ordinate, a horizontal face's have their own.
+__device__ __forceinline__ PhaseDev sample_phase(const CoupledDev& d, const MeshDev& m, bool vert,
+                                                 size_t fq, int k)
+{
+    return vert ? phase_at(m.ph, d.tep_v, fq) : phase_at(m.ph, d.tep_hs, fq * d.n + k);
+}
+
 }  // namespace
 
 __global__ void coupled_cell_kernel(CoupledDev d, const double* __restrict__ H,
@@ -120,7 +128,9 @@ __global__ void coupled_cell_kernel(CoupledDev d, const double* __restrict__ H,
 #pragma unroll 1
         for (int a = 0; a < n; ++a) {
             double Hp, Cp;
-            const PhaseOut o = point_phase<false>(m.ph, p, m.lim_cx[a], m.lim_cy[b], Hp, Cp, err);
+            const PhaseOut o = point_phase<false>(
+                phase_at(m.ph, d.tep_cell, (size_t(j) * n + b) * n + a), p, m.lim_cx[a], m.lim_cy[b],
+                Hp, Cp, err);
             const double wab = d.cw[a] * d.cw[b] * 0.25;      // weight / area
             const double mixy = d.cmix[size_t(b * n + a) * N + c];
             const double source = -d.cooling * o.T * mixy;
@@ -178,6 +188,9 @@ __global__ void coupled_face_kernel(CoupledDev d)
     const double* vsa = (vert ? d.xvs : d.yvs) + fl;
     const double* qda = (vert ? d.xqd : d.yqd) + fl;
     const double* pha = (vert ? d.xphi : d.yphi) + fl;
+    // lithostatic tables of this face's points (both sides share a point)
+    const TepPair* tpt = vert ? d.tep_v : d.tep_h;
+    const size_t trow = size_t(j) * nq;
     double fluxH = 0, fluxC = 0, fluxMix = 0, rate = 0;
     uint8_t polH = 0, polC = 0;
     const double *inH = nullptr, *inC = nullptr;
@@ -195,12 +208,13 @@ __global__ void coupled_face_kernel(CoupledDev d)
         const double ephi = pha[q * fstride];
         const double xa = vert ? pf : s[q], ya = vert ? s[q] : pf;
         double hL, cL, hR, cR;
-        const PhaseOut a = point_phase<true>(m.ph, P, xa, ya, hL, cL, err);
+        const PhaseDev php = phase_at(m.ph, tpt, trow + q);
+        const PhaseOut a = point_phase<true>(php, P, xa, ya, hL, cL, err);
         PhaseOut b = a;
         cR = cL;
         if (!boundary) {
             const double xb = vert ? -hf : s[q], yb = vert ? s[q] : -hf;
-            b = point_phase<true>(m.ph, Q, xb, yb, hR, cR, err);
+            b = point_phase<true>(php, Q, xb, yb, hR, cR, err);
         }
         (void)hR;
         // PhaseTransportVelocities (src/couple/evolve.cpp:36-52) along the normal
@@ -237,7 +251,7 @@ __global__ void coupled_face_kernel(CoupledDev d)
             double T = a.T;
             if (tH == 0) {
                 uint32_t e2 = 0;
-                T = phase_eval<false>(m.ph, valH, inflowC, e2).T;
+                T = phase_eval<false>(php, valH, inflowC, e2).T;
                 err |= e2;
             }
             fh = tH == 2 ? 0 : vn * T - LD * (1 - ephi) * vs;
@@ -276,7 +290,8 @@ __global__ void coupled_face_kernel(CoupledDev d)
             for (int k = 0; k < nin; ++k) {
                 const double off = boundary ? sgn * depth[k] : depth[k];
                 double h_, c_;
-                Tk[k] = point_phase<false>(m.ph, P, vert ? off : s[q], vert ? s[q] : off, h_, c_, err).T;
+                Tk[k] = point_phase<false>(sample_phase(d, m, vert, trow + q, k), P, vert ? off : s[q],
+                                           vert ? s[q] : off, h_, c_, err).T;
             }
             double last;
             if (boundary) {
@@ -286,7 +301,8 @@ __global__ void coupled_face_kernel(CoupledDev d)
                 for (int k = n / 2; k < n; ++k) {
                     const double off = -depth[n - 1 - k];
                     double h_, c_;
-                    Tk[k] = point_phase<false>(m.ph, Q, vert ? off : s[q], vert ? s[q] : off, h_, c_, err).T;
+                    Tk[k] = point_phase<false>(sample_phase(d, m, vert, trow + q, k), Q,
+                                               vert ? off : s[q], vert ? s[q] : off, h_, c_, err).T;
                 }
                 last = Tk[n - 1];
             }

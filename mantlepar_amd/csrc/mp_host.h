@@ -53,8 +53,24 @@ void gauss_legendre(int n, double* x, double* w);
 struct PhaseHost {
     PhaseDev dev;
     double TDe0, nu, dT;
+    double rho, g, l0;      // staticPressure's factors (l0 = sqrt(mus*k0/mul))
 };
 mp_status make_phase(const mp_phase_params& p, PhaseHost& out);
+
+// Lithostatic {Tep, Tmp} at nondimensional ordinate y below the surface
+// surface_y: P = staticPressure(surface_y - y) (src/phase/eutectic_rescaled.cpp:
+// 151-161, src/couple/initialization.cpp:159-161), then the boundaries' pressure
+// shift (:143-149, :166-169), each rounded once from long double.
+mp_status lithostatic_pair(const PhaseHost& h, double surface_y, double y, TepPair& out);
+// Vertex ordinates of a uniform axis a0 + k*(a1 - a0)/n with the ends snapped
+// (src/core/mesh.cpp:27-33, 86-114); v holds n + 1 values.
+void uniform_vertices(double a0, double a1, int n, double* v);
+// MapEdgePoint's combination of two end values at reference t in [-1, 1]
+// (src/core/integral.cpp:91-97).
+inline double map_edge(double t, double a, double b)
+{
+    return (0.5 * (1.0 - t)) * a + (0.5 * (1.0 + t)) * b;
+}
 
 mp_status validate_mesh(const mp_mesh_desc& m);
 mp_status validate_weno(const mp_weno_options& o);

@@ -270,7 +270,7 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
                 sp.Ho[k] = oH;
                 sp.Co[k] = oC;
                 if constexpr (PHASE) {
-                    const PhaseOut po = phase_eval<false>(m.ph, oH, oC, err);
+                    const PhaseOut po = phase_eval<false>(phase_at(m.ph, m.tep_row, r - 1), oH, oC, err);
                     sp.T[k] = po.T;
                     sp.phi[k] = po.phil;
                     sp.Cl[k] = po.cl;
@@ -729,7 +729,7 @@ __global__ void state_phase_kernel(const MeshDev* __restrict__ mesh, const doubl
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nx * m.ny) return;
     uint32_t err = 0;
-    const PhaseOut o = phase_eval(m.ph, H[c], C[c], err);
+    const PhaseOut o = phase_eval(phase_at(m.ph, m.tep_row, c / m.nx), H[c], C[c], err);
     T[c] = o.T; phi[c] = o.phil; Cl[c] = o.cl; Cs[c] = o.cs; region[c] = uint8_t(o.region);
     if (err) atomicOr(error, err);
 }

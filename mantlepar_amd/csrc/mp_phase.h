@@ -88,6 +88,18 @@ __device__ __forceinline__ DD dd_mul(DD a, DD b)
     return dd_two_sum(p, e);
 }
 
+// The phase parameters at entry k of a lithostatic table (tab null: p).
+__device__ __forceinline__ PhaseDev phase_at(const PhaseDev& p, const TepPair* tab, size_t k)
+{
+    PhaseDev q = p;
+    if (tab) {
+        const TepPair t = tab[k];
+        q.Tep = t.tep;
+        q.Tmp = t.tmp;
+    }
+    return q;
+}
+
 // unitFraction (src/phase/eutectic_rescaled.cpp:108-114)
 __device__ __forceinline__ double unit_fraction(double v, uint32_t& err)
 {

@@ -352,7 +352,7 @@ __device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, in
             sp.Ho[kk] = outH;
             sp.Co[kk] = outC;
             if constexpr (PHASE) {
-                const PhaseOut po = phase_eval<false>(m.ph, outH, outC, err);
+                const PhaseOut po = phase_eval<false>(phase_at(m.ph, m.tep_row, r - 1), outH, outC, err);
                 sp.T[kk] = po.T;
                 sp.phi[kk] = po.phil;
                 sp.Cl[kk] = po.cl;

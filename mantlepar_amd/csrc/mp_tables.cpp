@@ -124,11 +124,16 @@ mp_status make_phase(const mp_phase_params& p, PhaseHost& out)
     if (!finite_ld(deltaT) || !finite_ld(h0) || !finite_ld(LD) || !finite_ld(TDe0) ||
         (double)LD <= 0.0 || (double)TDe0 <= 0.0 || (double)deltaT <= 0.0)
         return fail(MP_ERR_FP, "phase scales are not representable");
+    const long double l0 = sqrtl((long double)p.mus * p.k0 / p.mul);
+    if (!finite_ld(l0) || (double)l0 <= 0.0) return fail(MP_ERR_FP, "l0 is not a positive scale");
     PhaseHost h;
     std::memset(&h, 0, sizeof h);
     h.dT = (double)deltaT;
     h.TDe0 = (double)TDe0;
     h.nu = p.nu;
+    h.rho = p.rho;
+    h.g = p.g;
+    h.l0 = (double)l0;
     h.dev.LD = (double)LD;
     h.dev.Xe = p.Xe;
     // boundaries(): Tep and Tmp rounded once (eutectic_rescaled.cpp:143-149, 166-169)
@@ -151,6 +156,31 @@ mp_status make_phase(const mp_phase_params& p, PhaseHost& out)
     recip(h.dev.LD, h.dev.ilh, h.dev.ill);
     out = h;
     return MP_OK;
+}
+
+mp_status lithostatic_pair(const PhaseHost& h, double surface_y, double y, TepPair& out)
+{
+    const double zD = surface_y - y;
+    if (!std::isfinite(zD)) return fail(MP_ERR_ARG, "zD must be finite");
+    const long double P = (long double)h.rho * h.g * zD * h.l0;
+    if (!finite_ld(P)) return fail(MP_ERR_FP, "static pressure is not representable");
+    const long double shift = (long double)(double)P / h.nu / h.dT;
+    const long double tep = (long double)h.TDe0 + shift;
+    if (!finite_ld(tep)) return fail(MP_ERR_FP, "pressure-corrected temperature is not representable");
+    const long double tmp = (long double)(double)tep + 1.0L;
+    if (!finite_ld(tmp)) return fail(MP_ERR_FP, "Tmp is not representable");
+    out.tep = (double)tep;
+    out.tmp = (double)tmp;
+    if (!(out.tmp > out.tep)) return fail(MP_ERR_FP, "Phase temperature intervals are not resolvable");
+    return MP_OK;
+}
+
+void uniform_vertices(double a0, double a1, int n, double* v)
+{
+    const double step = (a1 - a0) / n;
+    for (int k = 0; k <= n; ++k) v[k] = a0 + k * step;
+    v[0] = a0;
+    v[n] = a1;
 }
 
 mp_status validate_mesh(const mp_mesh_desc& m)

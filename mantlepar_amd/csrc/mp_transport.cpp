@@ -544,6 +544,31 @@ mp_status mp_transport::set_thermal_boundary(const uint8_t* types, const double*
     return MP_OK;
 }
 
+mp_status mp_transport::set_lithostatic(int enable, double surface_y)
+{
+    if (!enable) {
+        tep.reset();
+        md.tep_row = nullptr;
+    } else {
+        // src/couple/initialization.cpp:155-162: the phase at a cell centre
+        // (MapCellPoint of (0, 0)) sees P = staticPressure(surface_y - y_c)
+        if (!std::isfinite(surface_y)) return fail(MP_ERR_ARG, "surface_y must be finite");
+        if (surface_y < mesh.y1) return fail(MP_ERR_ARG, "surface_y must be at or above the mesh top");
+        std::vector<double> y(size_t(mesh.ny) + 1);
+        uniform_vertices(mesh.y0, mesh.y1, mesh.ny, y.data());
+        std::vector<TepPair> rows(mesh.ny);
+        for (int j = 0; j < mesh.ny; ++j) {
+            const double lower = map_edge(0.0, y[j], y[j]), upper = map_edge(0.0, y[j + 1], y[j + 1]);
+            MP_CALL(lithostatic_pair(phase, surface_y, map_edge(0.0, lower, upper), rows[j]));
+        }
+        MP_CALL(tep.alloc(rows.size() * sizeof(TepPair)));
+        MP_HIP(hipMemcpyAsync(tep.ptr, rows.data(), tep.bytes, hipMemcpyHostToDevice, stream));
+        md.tep_row = tep.as<TepPair>();
+    }
+    phase_valid = false;
+    return sync_md();
+}
+
 // ===================================================================== C ABI
 
 extern "C" {
@@ -712,6 +737,12 @@ mp_status mp_transport_set_thermal_boundary(mp_transport* t, const uint8_t* type
 {
     if (!t) return fail(MP_ERR_ARG, "null handle");
     MP_GUARD({ return t->set_thermal_boundary(types, values); })
+}
+
+mp_status mp_transport_set_lithostatic(mp_transport* t, int32_t enable, double surface_y)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->set_lithostatic(enable, surface_y); })
 }
 
 mp_status mp_transport_limits(mp_transport* t, double* thetaH, double* thetaC)

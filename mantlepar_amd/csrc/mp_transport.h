@@ -26,6 +26,7 @@ struct mp_transport {
     double max_rate = 0;
     mp::DeviceBuffer bc[4], inflowH, inflowC;
     mp::DeviceBuffer tbc, tval;   // thermal boundary policies and data
+    mp::DeviceBuffer tep;         // lithostatic {Tep, Tmp} per cell row
     // phase outputs: T, phi, Cl, Cs (N doubles each) then region (N bytes)
     mp::DeviceBuffer phase_buf[2];
     int phase_cur = 0;
@@ -66,4 +67,5 @@ struct mp_transport {
     mp_status limits(double* thetaH, double* thetaC);
     mp_status set_diffusion(double kappa, int samples, double extent);
     mp_status set_thermal_boundary(const uint8_t* types, const double* values);
+    mp_status set_lithostatic(int enable, double surface_y);
 };

@@ -194,6 +194,13 @@ class Transport:
         v = None if values is None else _f64(values).ravel()
         check(lib().mp_transport_set_thermal_boundary(self._h, u8ptr(t), dptr(v)))
 
+    def set_lithostatic(self, surface_y=None):
+        """Pressure model "lithostatic" (src/couple/initialization.cpp:155-162): the
+        phase of each cell at P = staticPressure(surface_y - y_centre); None
+        returns to the constant phase pressure."""
+        check(lib().mp_transport_set_lithostatic(self._h, int(surface_y is not None),
+                                                 0.0 if surface_y is None else surface_y))
+
     def limits(self):
         """Limiter factors (theta_H, theta_C) of the current state."""
         tH = np.empty(self.N)

@@ -1289,6 +1289,26 @@ int mpo_phase_classify(const mpo_phase_model* m, double H, double C, double P, i
     return MPO_OK;
 }
 
+/* staticPressure, src/phase/eutectic_rescaled.cpp:151-161 (the one-argument
+ * form passes the model's l0; initialization.cpp:155-162 feeds it
+ * surface_y - y and rejects a non-finite result). */
+int mpo_static_pressure_scaled(const mpo_phase_model* m, double zD, double length, double* P)
+{
+    CHECK(finite_d(zD), MPO_ERR_ARG, "zD must be finite");
+    CHECK(finite_d(length), MPO_ERR_ARG, "length_scale must be finite");
+    CHECK(length > 0.0, MPO_ERR_ARG, "length_scale must be positive");
+    const long double v = (long double)m->p.rho * m->p.g * zD * length;
+    CHECK(isfinite(v) && fabsl(v) <= (long double)DBL_MAX, MPO_ERR_FP,
+          "static pressure is not representable");
+    *P = (double)v;
+    return MPO_OK;
+}
+
+int mpo_static_pressure(const mpo_phase_model* m, double zD, double* P)
+{
+    return mpo_static_pressure_scaled(m, zD, m->d.l0, P);
+}
+
 /* Hypereutectic mirror (extension; the reference rejects C > Xe,
  * src/phase/eutectic_rescaled.cpp:88-93).  Component 2 plays the role of
  * component 1 on the mirrored axis C' = 1 - C with eutectic Xe' = 1 - Xe. */
@@ -2132,10 +2152,26 @@ typedef struct {
     double* thC;
 } coupled_ctx;
 
-static int coupled_phase_at(const coupled_ctx* x, double H, double C, mpo_phase_state* s)
+/* c.pressure(p, t): constant, or lithostatic staticPressure(surface_y - y)
+ * with a finite result (src/couple/initialization.cpp:148-163) */
+static int coupled_pressure(const coupled_ctx* x, mpo_point p, double* P)
 {
-    return x->cp->hypereutectic ? mpo_phase_evaluate_ext(x->pm, H, C, x->cp->pressure, s)
-                                : mpo_phase_evaluate(x->pm, H, C, x->cp->pressure, s);
+    if (!x->cp->lithostatic) {
+        *P = x->cp->pressure;
+        return MPO_OK;
+    }
+    CALL(mpo_static_pressure(x->pm, x->cp->surface_y - p.p[1], P));
+    CHECK(finite_d(*P), MPO_ERR_FP, "static pressure is not finite");
+    return MPO_OK;
+}
+
+static int coupled_phase_at(const coupled_ctx* x, mpo_point p, double H, double C,
+                            mpo_phase_state* s)
+{
+    double P;
+    CALL(coupled_pressure(x, p, &P));
+    return x->cp->hypereutectic ? mpo_phase_evaluate_ext(x->pm, H, C, P, s)
+                                : mpo_phase_evaluate(x->pm, H, C, P, s);
 }
 
 /* Phase of the limited reconstruction at p, src/couple/evolve.cpp:270-277 with
@@ -2149,7 +2185,7 @@ static int coupled_phase(const coupled_ctx* x, int i, int j, mpo_point p, mpo_ph
     CALL(mpo_recon_eval(&x->rc[c], p, 0, 0, &cc));
     h = x->H[c] + x->thH[c] * (h - x->H[c]);
     cc = x->C[c] + x->thC[c] * (cc - x->C[c]);
-    CALL(coupled_phase_at(x, h, cc, s));
+    CALL(coupled_phase_at(x, p, h, cc, s));
     CHECK(s->TDp >= 0 && s->phil < 1, MPO_ERR_ARG,
           "Coupled Darcy-Stokes requires nonnegative temperature and porosity < 1");
     if (Hp) *Hp = h;
@@ -2226,7 +2262,7 @@ static int coupled_face(const coupled_ctx* x, const mpo_flow* flow, const mpo_po
             double T = a.TDp;
             if (typeH == 0) {
                 mpo_phase_state sb;
-                CALL(coupled_phase_at(x, valH, inflowC, &sb));
+                CALL(coupled_phase_at(x, p, valH, inflowC, &sb));
                 T = sb.TDp;
             }
             fh = typeH == 2 ? 0 : vn * T - cp->LD * (1 - ephi) * vs;
@@ -2296,6 +2332,16 @@ int mpo_coupled_rhs(mpo_transport* t, const mpo_phase_model* pm, const mpo_coupl
     CHECK(t->limit, MPO_ERR_STATE, "the coupled law needs the bound limiter (set_limiter)");
     const int nx = t->nx, ny = t->ny;
     const size_t N = (size_t)nx * ny;
+    if (cp->lithostatic) {
+        /* src/couple/initialization.cpp:155-158 */
+        CHECK(finite_d(cp->surface_y), MPO_ERR_ARG, "surface_y must be finite");
+        for (int i = 0; i < nx; ++i) {
+            mpo_point c4[4];
+            mpo_cell_corners(&t->mesh, i, ny - 1, c4);
+            CHECK(cp->surface_y >= c4[2].p[1] && cp->surface_y >= c4[3].p[1], MPO_ERR_ARG,
+                  "surface_y must be at or above the mesh top");
+        }
+    }
     coupled_ctx x = {t, pm, cp, NULL, NULL, H, C, NULL, NULL};
     x.rh = (mpo_recon*)malloc(N * sizeof(mpo_recon));
     x.rc = (mpo_recon*)malloc(N * sizeof(mpo_recon));

@@ -191,6 +191,10 @@ int mpo_phase_boundaries(const mpo_phase_model* m, double C, double P, mpo_phase
 int mpo_phase_evaluate(const mpo_phase_model* m, double H, double C, double P,
                        mpo_phase_state* s);
 int mpo_phase_classify(const mpo_phase_model* m, double H, double C, double P, int* region);
+/* staticPressure, src/phase/eutectic_rescaled.cpp:151-161: rho*g*zD*length
+ * in long double, rounded once; the unscaled form uses the model's l0. */
+int mpo_static_pressure(const mpo_phase_model* m, double zD, double* P);
+int mpo_static_pressure_scaled(const mpo_phase_model* m, double zD, double length, double* P);
 /* Hypereutectic extension (no reference behaviour): for C > Xe evaluates the
  * mirrored diagram C -> 1-C, Xe -> 1-Xe and maps phases back. */
 int mpo_phase_evaluate_ext(const mpo_phase_model* m, double H, double C, double P,
@@ -297,6 +301,8 @@ typedef struct {
     int samples;             /* c.sampling.numberOfSamples */
     double extent;           /* c.sampling.extentFraction */
     double pressure;         /* constant phase pressure [Pa] */
+    int lithostatic;         /* 1: P = staticPressure(surface_y - y) at each point */
+    double surface_y;        /* nondimensional surface ordinate (>= mesh top) */
     int hypereutectic;       /* evaluate C > Xe on the mirrored diagram */
     double admissible_c_max; /* Admissible: C <= this (Xe) */
 } mpo_coupled_params;

@@ -11,7 +11,7 @@ REF_PATH = os.path.join(HERE, "_ref", "libref_phase.so")
 
 __all__ = [
     "lib", "OracleError", "build", "Material", "PhaseModel", "Transport", "gauss_rule",
-    "ref_phase_available", "ref_phase_evaluate", "ref_phase_derived", "WenoOptions",
+    "ref_phase_available", "ref_phase_evaluate", "ref_phase_derived", "ref_static_pressure", "WenoOptions",
     "StencilPoly", "Recon", "llf_point", "integrate_edge_flux", "REGION",
     "lagrange_weights", "Sampling", "diffusive_sampling", "diffusive_flux", "LAGRANGE",
     "SIDE", "DIFF_BC", "MAX_SAMPLES", "CoupledParams", "transport_velocities", "edge_count",
@@ -182,14 +182,14 @@ class CoupledParams(C.Structure):
     _fields_ = [("LD", C.c_double), ("theta", C.c_double), ("cooling", C.c_double),
                 ("kappa", C.c_double), ("lf_mode", C.c_int), ("global_speed", C.c_double),
                 ("cell_points", C.c_int), ("samples", C.c_int), ("extent", C.c_double),
-                ("pressure", C.c_double), ("hypereutectic", C.c_int),
-                ("admissible_c_max", C.c_double)]
+                ("pressure", C.c_double), ("lithostatic", C.c_int), ("surface_y", C.c_double),
+                ("hypereutectic", C.c_int), ("admissible_c_max", C.c_double)]
 
     @classmethod
     def make(cls, **kw):
         d = dict(LD=1.0, theta=1.0, cooling=0.0, kappa=0.0, lf_mode=0, global_speed=0.0,
-                 cell_points=5, samples=4, extent=0.9, pressure=0.0, hypereutectic=0,
-                 admissible_c_max=0.4)
+                 cell_points=5, samples=4, extent=0.9, pressure=0.0, lithostatic=0,
+                 surface_y=0.0, hypereutectic=0, admissible_c_max=0.4)
         d.update(kw)
         return cls(**d)
 
@@ -249,6 +249,9 @@ def _declare(L):
     L.mpo_phase_evaluate_ext.argtypes = L.mpo_phase_evaluate.argtypes
     L.mpo_phase_classify.argtypes = [C.POINTER(_PhaseModel), C.c_double, C.c_double,
                                      C.c_double, C.POINTER(C.c_int)]
+    L.mpo_static_pressure.argtypes = [C.POINTER(_PhaseModel), C.c_double, C.POINTER(C.c_double)]
+    L.mpo_static_pressure_scaled.argtypes = [C.POINTER(_PhaseModel), C.c_double, C.c_double,
+                                             C.POINTER(C.c_double)]
     L.mpo_phase_batch.argtypes = [C.POINTER(_PhaseModel), C.c_int64, _dp, _dp, C.c_double,
                                   C.c_int, _dp, _dp, _dp, _dp, _u8p]
     L.mpo_production_options.argtypes = [C.POINTER(WenoOptions)]
@@ -346,6 +349,15 @@ class PhaseModel:
         _check(f(C.byref(self._m), H, Cd, P, C.byref(s)))
         return s.as_dict()
 
+    def static_pressure(self, zD, length=None):
+        """staticPressure(zD[, length_scale]) (eutectic_rescaled.cpp:151-161)."""
+        out = C.c_double()
+        if length is None:
+            _check(lib().mpo_static_pressure(C.byref(self._m), zD, C.byref(out)))
+        else:
+            _check(lib().mpo_static_pressure_scaled(C.byref(self._m), zD, length, C.byref(out)))
+        return out.value
+
     def classify(self, H, Cd, P=0.0):
         r = C.c_int()
         _check(lib().mpo_phase_classify(C.byref(self._m), H, Cd, P, C.byref(r)))
@@ -376,6 +388,8 @@ def _reflib():
         _ref.ref_phase_evaluate.argtypes = [_dp, C.c_long, _dp, _dp, _dp, _dp,
                                             np.ctypeslib.ndpointer(dtype=np.int32)]
         _ref.ref_phase_derived.argtypes = [_dp, _dp]
+        _ref.ref_static_pressure.argtypes = [_dp, C.c_long, _dp, _dp, C.c_int, _dp,
+                                             np.ctypeslib.ndpointer(dtype=np.int32)]
     return _ref
 
 
@@ -397,6 +411,19 @@ def ref_phase_derived(material):
     rc = _reflib().ref_phase_derived(material.as_array(), d)
     return rc, d
 
+
+def ref_static_pressure(material, zD, length=None):
+    """The reference's own EutecticModel::staticPressure on arrays (oracle/_ref)."""
+    zD = np.ascontiguousarray(zD, dtype=np.float64).ravel()
+    ln = np.ascontiguousarray(np.broadcast_to(0.0 if length is None else length, zD.shape),
+                              dtype=np.float64)
+    out = np.zeros(zD.size)
+    status = np.zeros(zD.size, dtype=np.int32)
+    rc = _reflib().ref_static_pressure(material.as_array(), zD.size, zD, ln,
+                                       0 if length is None else 1, out, status)
+    if rc:
+        raise OracleError(rc, "reference model construction failed")
+    return out, status
 
 
 

@@ -46,6 +46,32 @@ int ref_phase_evaluate(const double* params, long n, const double* H, const doub
     return 0;
 }
 
+// staticPressure(zD[k]) (scaled == 0) or staticPressure(zD[k], length[k]);
+// status[k] as in ref_phase_evaluate
+int ref_static_pressure(const double* params, long n, const double* zD, const double* length,
+                        int scaled, double* out, int* status)
+{
+    MaterialParameters p;
+    p.Tm0 = params[0]; p.Te0 = params[1]; p.nu = params[2]; p.L = params[3];
+    p.cp = params[4]; p.Xe = params[5]; p.rho = params[6]; p.rhor = params[7];
+    p.mus = params[8]; p.mul = params[9]; p.k0 = params[10]; p.g = params[11];
+    p.alpha0 = params[12];
+    try {
+        const EutecticModel model(p);
+        for (long k = 0; k < n; ++k) {
+            try {
+                out[k] = scaled ? model.staticPressure(zD[k], length[k]) : model.staticPressure(zD[k]);
+                status[k] = 0;
+            } catch (const std::invalid_argument&) { status[k] = 1; }
+            catch (const std::overflow_error&) { status[k] = 2; }
+            catch (const std::runtime_error&) { status[k] = 3; }
+        }
+    } catch (const std::exception&) {
+        return -1;
+    }
+    return 0;
+}
+
 // derived: dT, h0, gamma, LD, TDm0, TDe0, invk0, l0, u0, p0, t0
 int ref_phase_derived(const double* params, double* derived)
 {

@@ -23,12 +23,15 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-def pair(spec, darcy=0.05, H_in=0.55, kappa=KAPPA, state=None):
+def pair(spec, darcy=0.05, H_in=0.55, kappa=KAPPA, state=None, surface=None):
+    """Device handle and oracle on the same workload; surface: lithostatic
+    pressure below surface_y (rho = 600) instead of P = 0."""
     flow = W.coupled_flow(spec, darcy=darcy)
     bc, inflow, ttypes, tvals = W.coupled_boundary(spec, H_in=H_in)
     H, Cc = W.coupled_state(spec) if state is None else state
+    mat = {} if surface is None else {"rho": 600.0}
     g = M.Coupled(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q,
-                  phase=W.config_phase(hypereutectic=0),
+                  phase=W.config_phase(hypereutectic=0, **mat),
                   params=M.default_coupled(theta=THETA, cooling=COOLING,
                                            thermal_diffusivity=kappa))
     g.set_flow(flow)
@@ -43,9 +46,12 @@ def pair(spec, darcy=0.05, H_in=0.55, kappa=KAPPA, state=None):
                     inflow_H=inflow[0], inflow_C=inflow[1])
     o.set_limiter(True, 5, 4, 0.9, XE)
     o.set_diffusion(kappa, 4, 0.9, types={"bottom": np.ones(spec.nx)}, values=tvals)
-    pm = O.PhaseModel(O.Material.defaults(Tm0=1600.0, Te0=600.0, cp=1200.0, L=1.2e6, Xe=XE))
+    pm = O.PhaseModel(O.Material.defaults(Tm0=1600.0, Te0=600.0, cp=1200.0, L=1.2e6, Xe=XE, **mat))
     cp = O.CoupledParams.make(LD=pm.derived["LD"], theta=THETA, cooling=COOLING, kappa=kappa,
-                              admissible_c_max=XE)
+                              admissible_c_max=XE, lithostatic=int(surface is not None),
+                              surface_y=surface or 0.0)
+    if surface is not None:
+        g.set_lithostatic(surface)
     return g, o, pm, cp, flow, H, Cc
 
 
@@ -112,4 +118,76 @@ def test_errors_keep_the_state():
         g.run(1, cfl=0.5, initial=big, maximum=big, fixed=True)
     Hg, Cg = g.download()
     assert np.array_equal(Hg, H) and np.array_equal(Cg, Cc)
+    g.close()
+
+
+# ------------------------------------------------------- lithostatic pressure
+@pytest.mark.parametrize("shape", [(16, 14), (40, 36)])
+def test_lithostatic_rhs_matches_oracle(shape):
+    """Every phase evaluation at its own point's static pressure: cell points,
+    both sides of each face point, inflow temperatures and diffusion samples."""
+    spec = W.MeshSpec(*shape, seed=2)
+    g, o, pm, cp, flow, H, Cc = pair(spec, surface=1.2)
+    got = g.rhs()
+    want = o.coupled_rhs(pm, cp, flow, H, Cc)
+    for k in ("rhsH", "rhsC", "rate", "div", "T", "phi"):
+        assert rel(got[k], want[k]) <= 1e-12, (k, rel(got[k], want[k]))
+    for a, b in (("boundary_H", "boundaryH"), ("boundary_C", "boundaryC"), ("source_H", "sourceH"),
+                 ("maximum_rate", "maximumRate")):
+        assert abs(got[a] - want[b]) <= 1e-12 * abs(want[b]), (a, got[a], want[b])
+    # the pressure matters here
+    flat = o.coupled_rhs(pm, O.CoupledParams.make(LD=pm.derived["LD"], theta=THETA, cooling=COOLING,
+                                                 kappa=KAPPA, admissible_c_max=XE), flow, H, Cc)
+    assert rel(flat["T"], want["T"]) > 1e-3
+    g.close()
+
+
+def test_lithostatic_run_matches_oracle():
+    spec = W.MeshSpec(24, 20, seed=2)
+    g, o, pm, cp, flow, H, Cc = pair(spec, surface=1.2)
+    tg, sg = g.run(5, cfl=0.5, initial=1e-3, maximum=1.0)
+    Ho, Co, to, so = o.coupled_run(pm, cp, flow, H, Cc, 5, cfl=0.5, initial=1e-3, maximum=1.0)
+    assert [s["retries"] for s in sg] == [s["retries"] for s in so]
+    for a, b in zip(sg, so):
+        assert abs(a["dt"] - b["dt"]) <= 1e-12 * b["dt"]
+    Hg, Cg = g.download()
+    assert rel(Hg, Ho) <= 1e-11 and rel(Cg, Co) <= 1e-11
+    g.close()
+
+
+def test_lithostatic_eutectic_layer_closed_form():
+    """H = Tep(y) + 0.25, C = 0.2: T = Tep(P(y)) at every point, linear in
+    depth with slope B.  Uniform upward flow v entering at the bottom with
+    H_in = Tep(y0) + 0.25 and diffusion kappa with insulated sides give
+    rhs_H = v*B everywhere plus kappa*B/dy in the top row and -kappa*B/dy in
+    the bottom row; rhs_C = 0."""
+    spec = W.MeshSpec(10, 12, seed=1)
+    surface, v, kappa = 1.2, 0.05, 1e-2
+    g, o, pm, cp, flow, _, _ = pair(spec, surface=surface, kappa=kappa)
+    B = pm.static_pressure(1.0) / pm.material.nu / pm.derived["dT"]
+
+    def tep(y):
+        return pm.boundaries(0.2, pm.static_pressure(surface - y))["Tep"]
+
+    yc = spec.y0 + (np.arange(spec.ny) + 0.5) * spec.dy
+    H = np.repeat(np.array([tep(y) + 0.25 for y in yc])[:, None], spec.nx, axis=1)
+    Cc = np.full_like(H, 0.2)
+    for k in flow:
+        flow[k][:] = 0.0
+    flow["edge_stokes"][..., 1] = v
+    flow["cell_stokes"][..., 1] = v
+    g.set_flow(flow)
+    n = 2 * (spec.nx + spec.ny) * spec.q
+    g.set_inflow(np.full(n, tep(spec.y0) + 0.25), np.full(n, 0.2))
+    g.set_thermal_boundary(None)
+    g.upload(H, Cc)
+    got = g.rhs()
+    want = np.full_like(H, v * B)
+    want[-1] += kappa * B / spec.dy
+    want[0] -= kappa * B / spec.dy
+    # the workload's cooling source -cooling*T*v_y, averaged over the cell's
+    # points: T is linear in y, so its average is Tep at the centre
+    cool = -COOLING * v * np.array([tep(y) for y in yc])[:, None]
+    assert np.abs(got["rhsH"] - (want + cool)).max() <= 1e-11 * np.abs(want).max()
+    assert np.abs(got["rhsC"]).max() <= 1e-13
     g.close()

@@ -190,3 +190,49 @@ def test_full_size_config2_properties():
     (h64, c64), _ = res[M.PREC_FP64]
     (hm, cm), _ = res[M.PREC_MIXED]
     assert rel(hm, h64) <= 1e-6 and rel(cm, c64) <= 1e-6
+
+
+def row_pressure(om, spec, surface):
+    """staticPressure(surface - y_c) per row, y_c = MapCellPoint((0, 0)) over the
+    uniform vertices (src/core/integral.cpp:91-106)."""
+    y = W.vertex_coords(spec)[:, 0, 1]
+    lower, upper = 0.5 * y[:-1] + 0.5 * y[:-1], 0.5 * y[1:] + 0.5 * y[1:]
+    yc = 0.5 * lower + 0.5 * upper
+    return np.array([om.static_pressure(surface - v) for v in yc])
+
+
+def oracle_phase_rows(om, H, Cc, P):
+    out = [om.batch(H[j], Cc[j], P[j], hyper=True) for j in range(H.shape[0])]
+    return [np.stack([o[k] for o in out]) for k in range(5)]
+
+
+@pytest.mark.parametrize("prec", [M.PREC_FP64, M.PREC_MIXED], ids=["fp64", "mixed"])
+def test_lithostatic_phase_per_row(prec):
+    """Pressure model lithostatic: each cell's phase at the static pressure of
+    its centre, before stepping (state phase kernel) and fused into the last
+    stage of a step; regions exact, T and phi within 1e-13."""
+    spec = W.MeshSpec(40, 36, seed=3)
+    t, o, H, Cc = setup(spec, prec, phase=W.config_phase(rho=600.0))
+    om = O.PhaseModel(O.Material.defaults(Tm0=1600.0, Te0=600.0, cp=1200.0, L=1.2e6, Xe=0.4,
+                                          rho=600.0))
+    P = row_pressure(om, spec, 1.2)
+    flat = t.phase_state()
+    t.set_lithostatic(1.2)
+    for stepped in (False, True):
+        if stepped:
+            t.step(5, t.stable_dt(0.4))
+        Hs, Cs = t.download()
+        ph = t.phase_state()
+        T, phi, cl, cs, reg = oracle_phase_rows(om, Hs, Cs, P)
+        assert np.array_equal(ph["region"], reg), stepped
+        assert np.abs(ph["T"] - T).max() <= 1e-13
+        assert np.abs(ph["phi"] - phi).max() <= 1e-13
+    # the pressure moves the eutectic: many cells change region
+    assert (flat["region"] != oracle_phase_rows(om, H, Cc, P)[4]).sum() > 50
+    with pytest.raises(M.MantleError, match="surface_y must be at or above the mesh top"):
+        t.set_lithostatic(0.9)
+    t.set_lithostatic(None)
+    Hs, Cs = t.download()
+    T0, _, _, _, reg0 = om.batch(Hs.ravel(), Cs.ravel(), 0.0, hyper=True)
+    assert np.array_equal(t.phase_state()["region"].ravel(), reg0)
+    t.close()

@@ -201,3 +201,142 @@ def test_step_loop_retries_and_balance():
     bad[3, 3] = XE + 0.01
     with pytest.raises(O.OracleError, match="Inadmissible cell average"):
         t.coupled_rhs(pm, cp, flow, H, bad)
+
+
+# ------------------------------------------------------- lithostatic pressure
+# P = staticPressure(surface_y - y) at each point where the phase is evaluated
+# (src/couple/initialization.cpp:155-162, evolve.cpp:273 and :318).  With
+# rho = 600 the eutectic temperature rises by B = rho*g*l0/(nu*dT) = 0.29 per
+# unit depth; surface 1.2 over a mesh spanning y in [0, 1].
+SURFACE = 1.2
+
+
+def litho_model():
+    return O.PhaseModel(O.Material.defaults(Tm0=1600.0, Te0=600.0, cp=1200.0, L=1.2e6, Xe=XE,
+                                            rho=600.0))
+
+
+def litho_slope(pm):
+    return pm.static_pressure(1.0) / pm.material.nu / pm.derived["dT"]
+
+
+def tep_at(pm, y, c=0.2):
+    return pm.boundaries(c, pm.static_pressure(SURFACE - y))["Tep"]
+
+
+def still_flow(spec):
+    flow = W.coupled_flow(spec)
+    for k in flow:
+        flow[k][:] = 0.0
+    return flow
+
+
+def eutectic_layer(spec, pm):
+    """H = Tep(y) + 0.25 at the cell centres, C = 0.2: the eutectic region at
+    every depth (H_eutectic_end - Tep = LD*C/Xe = 0.5), so T = Tep(P(y)),
+    linear in y."""
+    yc = spec.y0 + (np.arange(spec.ny) + 0.5) * spec.dy
+    H = np.repeat(np.array([tep_at(pm, y) + 0.25 for y in yc])[:, None], spec.nx, axis=1)
+    return H, np.full_like(H, 0.2)
+
+
+def test_lithostatic_diffusion_of_the_eutectic_layer():
+    """No flow: the normal-line diffusion of T = Tep(y) carries kappa*B upward
+    through every horizontal face, so interior rows have zero rhs and the
+    insulated top and bottom rows gain and lose kappa*B/dy; the cell-averaged T
+    is Tep at the cell centre."""
+    spec = W.MeshSpec(10, 12, seed=1)
+    pm = litho_model()
+    B, kappa = litho_slope(pm), 1e-2
+    H, Cc = eutectic_layer(spec, pm)
+    flow = still_flow(spec)
+    t = transport(spec, flow)
+    t.set_diffusion(kappa, 4, 0.9)
+    cp = O.CoupledParams.make(LD=pm.derived["LD"], kappa=kappa, lithostatic=1, surface_y=SURFACE)
+    r = t.coupled_rhs(pm, cp, flow, H, Cc)
+    scale = kappa * B / spec.dy
+    want = np.zeros_like(H)
+    want[-1] = scale
+    want[0] = -scale
+    assert np.abs(r["rhsH"] - want).max() <= 1e-11 * scale
+    yc = spec.y0 + (np.arange(spec.ny) + 0.5) * spec.dy
+    assert np.abs(r["T"] - np.array([tep_at(pm, y) for y in yc])[:, None]).max() <= 1e-14
+    # eutectic melting: phi = (H - Tep)/LD at every point
+    assert np.abs(r["phi"] - 0.25 / pm.derived["LD"]).max() <= 1e-14
+    # at constant pressure the same state is no longer one layer
+    r0 = t.coupled_rhs(pm, O.CoupledParams.make(LD=pm.derived["LD"], kappa=kappa), flow, H, Cc)
+    assert np.abs(r0["rhsH"] - want).max() > 1e3 * 1e-11 * scale
+
+
+def test_lithostatic_vertical_flow_through_the_eutectic_layer():
+    """Uniform upward Stokes flow v through the layer, entering at the bottom
+    with H_in = Tep(y0) + 0.25: every horizontal face carries v*Tep(y_face)
+    minus the latent term, both sides of a face see the same point and
+    pressure, and the inflow temperature is taken at the boundary point's
+    pressure, so rhs_H = v*B in every cell and rhs_C = 0."""
+    spec = W.MeshSpec(10, 12, seed=1)
+    pm = litho_model()
+    B, v = litho_slope(pm), 0.05
+    H, Cc = eutectic_layer(spec, pm)
+    flow = still_flow(spec)
+    flow["edge_stokes"][..., 1] = v
+    flow["cell_stokes"][..., 1] = v
+    bc, _, _, _ = W.coupled_boundary(spec)
+    n = 2 * (spec.nx + spec.ny) * spec.q
+    inflow = (np.full(n, tep_at(pm, spec.y0) + 0.25), np.full(n, 0.2))
+    t = transport(spec, flow, bc, inflow)
+    cp = O.CoupledParams.make(LD=pm.derived["LD"], lithostatic=1, surface_y=SURFACE)
+    r = t.coupled_rhs(pm, cp, flow, H, Cc)
+    assert np.abs(r["rhsH"] - v * B).max() <= 1e-11 * v * B
+    assert np.abs(r["rhsC"]).max() <= 1e-14
+
+
+def test_lithostatic_inflow_through_vertical_faces():
+    """Uniform H = 1.8, C = 0.2 in the solid-liquid region at every depth, where
+    T depends on depth nonlinearly; uniform flow u in x enters on the left with
+    H_in = 1.6.  Only the left column changes:
+    rhs_H = -u/(dx dy) * sum_q w_q dy/2 (T(1.8, P(y_q)) - T(1.6, P(y_q))),
+    the sum over the face's Gauss points at their own depths."""
+    spec = W.MeshSpec(10, 12, seed=1)
+    pm = litho_model()
+    u = 0.05
+    H = np.full((spec.ny, spec.nx), 1.8)
+    Cc = np.full_like(H, 0.2)
+    flow = still_flow(spec)
+    flow["edge_stokes"][..., 0] = u
+    flow["cell_stokes"][..., 0] = u
+    bc = {"bottom": np.zeros(spec.nx, np.uint8), "right": np.ones(spec.ny, np.uint8),
+          "top": np.zeros(spec.nx, np.uint8), "left": np.full(spec.ny, 2, np.uint8)}
+    n = 2 * (spec.nx + spec.ny) * spec.q
+    t = transport(spec, flow, bc, (np.full(n, 1.6), np.full(n, 0.2)))
+    cp = O.CoupledParams.make(LD=pm.derived["LD"], lithostatic=1, surface_y=SURFACE)
+    r = t.coupled_rhs(pm, cp, flow, H, Cc)
+    g, w = np.polynomial.legendre.leggauss(spec.q)
+
+    def T(h, y):
+        return pm.evaluate(h, 0.2, pm.static_pressure(SURFACE - y))["TDp"]
+
+    want = np.zeros_like(H)
+    centre = np.zeros(spec.ny)
+    for j in range(spec.ny):
+        ys = spec.y0 + (j + 0.5 * (1 + g)) * spec.dy
+        d = sum(wq * spec.dy / 2 * (T(1.8, y) - T(1.6, y)) for wq, y in zip(w, ys))
+        want[j, 0] = -u * d / (spec.dx * spec.dy)
+        yc = spec.y0 + (j + 0.5) * spec.dy
+        centre[j] = -u * spec.dy * (T(1.8, yc) - T(1.6, yc)) / (spec.dx * spec.dy)
+    scale = np.abs(want).max()
+    assert np.abs(r["rhsH"] - want).max() <= 1e-11 * scale
+    assert np.abs(r["rhsC"]).max() <= 1e-14
+    # the Gauss points' own depths matter: the face-centre pressure misses by far more
+    assert np.abs(centre - want[:, 0]).max() > 1e3 * 1e-11 * scale
+
+
+def test_lithostatic_surface_below_the_mesh_top_is_rejected():
+    spec = W.MeshSpec(10, 12, seed=1)
+    pm = litho_model()
+    H, Cc = eutectic_layer(spec, pm)
+    flow = still_flow(spec)
+    t = transport(spec, flow)
+    cp = O.CoupledParams.make(LD=pm.derived["LD"], lithostatic=1, surface_y=0.99)
+    with pytest.raises(O.OracleError, match="surface_y must be at or above the mesh top"):
+        t.coupled_rhs(pm, cp, flow, H, Cc)

@@ -11,6 +11,7 @@
   diagram.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -260,3 +261,44 @@ def test_oracle_matches_reference_build_on_fresh_vectors():
         for k in range(0, n, 3):
             s = m.evaluate(H[k], C[k], 0.0)
             assert [s[c] for c in COLS] == list(out[k, :10])
+
+
+# --------------------------------------------------------- static pressure
+def test_static_pressure_matches_reference_golden_bitwise():
+    """EutecticModel::staticPressure (eutectic_rescaled.cpp:151-161) against the
+    reference's own values (tests/golden/make_static_pressure_golden.py)."""
+    g = dict(np.load(os.path.join(os.path.dirname(__file__), "golden",
+                                  "static_pressure_golden.npz"), allow_pickle=False))
+    models = [O.PhaseModel(material(r)) for r in g["params"]]
+    ok = 0
+    for k in range(g["zD"].size):
+        m = models[g["set"][k]]
+        length = float(g["length"][k]) if g["scaled"][k] else None
+        if g["status"][k] != 0:
+            with pytest.raises(O.OracleError):
+                m.static_pressure(g["zD"][k], length)
+            continue
+        assert m.static_pressure(g["zD"][k], length) == g["P"][k], k
+        ok += 1
+    assert ok > 1000 and (g["status"] == 1).sum() > 100 and (g["status"] == 2).sum() > 0
+
+
+def test_static_pressure_known_answers():
+    """src/phase/test/test_eutectic.cpp:337-341, 455-463."""
+    for mat in (O.Material.defaults(), O.Material.defaults(rho=3100.0, g=9.81, mus=3e19, k0=2e-8,
+                                                           mul=2.0)):
+        m = O.PhaseModel(mat)
+        assert m.static_pressure(0.0) == 0.0
+        assert math.isclose(m.static_pressure(0.5), mat.rho * mat.g * 0.5 * m.derived["l0"],
+                            rel_tol=1e-15)
+        assert math.isclose(m.static_pressure(0.5, 1000.0), mat.rho * mat.g * 500.0, rel_tol=1e-15)
+        for bad in (math.nan, math.inf, -math.inf):
+            with pytest.raises(O.OracleError, match="zD"):
+                m.static_pressure(bad)
+        for length in (0.0, -1.0, math.inf, math.nan):
+            with pytest.raises(O.OracleError, match="length_scale"):
+                m.static_pressure(0.5, length)
+    # linear in depth: P(a) + P(b) = P(a + b) to rounding
+    m = O.PhaseModel()
+    assert math.isclose(m.static_pressure(0.25) + m.static_pressure(0.5), m.static_pressure(0.75),
+                        rel_tol=4e-16)
