@@ -60,24 +60,30 @@ __device__ __forceinline__ void load_cell(const CoupledDev& d, size_t c, size_t 
 }
 
 // Phase of the limited reconstruction at a point, with the coupled law's
-// range check (src/couple/evolve.cpp:270-277).
+// range check (src/couple/evolve.cpp:270-277); (tab, k): the point's
+// lithostatic entry (tab null: the constant pressure of ph).  The override is
+// formed here, in registers, not at the call site.
 template <bool DERIV>
-__device__ __noinline__ PhaseOut point_phase(const PhaseDev& ph, const CellPoly& p, double x,
-                                             double y, double& H, double& C, uint32_t& err)
+__device__ __noinline__ PhaseOut point_phase(const PhaseDev& ph, const TepPair* tab, size_t k,
+                                             const CellPoly& p, double x, double y, double& H,
+                                             double& C, uint32_t& err)
 {
     H = p.tH + poly_at(p.BH, x, y);
     C = p.tC + poly_at(p.BC, x, y);
-    const PhaseOut o = phase_eval<DERIV>(ph, H, C, err);
+    const PhaseOut o = phase_eval<DERIV>(phase_at(ph, tab, k), H, C, err);
     if (!(o.T >= 0.0 && o.phil < 1.0)) err |= kErrCoupledRange;
     return o;
 }
 
-// Phase parameters of normal-line sample k of face point fq: a vertical face's
-// samples share the point's ordinate, a horizontal face's have their own.
-__device__ __forceinline__ PhaseDev sample_phase(const CoupledDev& d, const MeshDev& m, bool vert,
-                                                 size_t fq, int k)
+// Lithostatic entry of normal-line sample k of face point fq: a vertical
+// face's samples share the point's ordinate, a horizontal face's have their own.
+__device__ __forceinline__ const TepPair* sample_table(const CoupledDev& d, bool vert)
 {
-    return vert ? phase_at(m.ph, d.tep_v, fq) : phase_at(m.ph, d.tep_hs, fq * d.n + k);
+    return vert ? d.tep_v : d.tep_hs;
+}
+__device__ __forceinline__ size_t sample_index(const CoupledDev& d, bool vert, size_t fq, int k)
+{
+    return vert ? fq : fq * d.n + k;
 }
 
 }  // namespace
@@ -128,9 +134,8 @@ __global__ void coupled_cell_kernel(CoupledDev d, const double* __restrict__ H,
 #pragma unroll 1
         for (int a = 0; a < n; ++a) {
             double Hp, Cp;
-            const PhaseOut o = point_phase<false>(
-                phase_at(m.ph, d.tep_cell, (size_t(j) * n + b) * n + a), p, m.lim_cx[a], m.lim_cy[b],
-                Hp, Cp, err);
+            const PhaseOut o = point_phase<false>(m.ph, d.tep_cell, (size_t(j) * n + b) * n + a, p,
+                                                  m.lim_cx[a], m.lim_cy[b], Hp, Cp, err);
             const double wab = d.cw[a] * d.cw[b] * 0.25;      // weight / area
             const double mixy = d.cmix[size_t(b * n + a) * N + c];
             const double source = -d.cooling * o.T * mixy;
@@ -208,13 +213,12 @@ __global__ void coupled_face_kernel(CoupledDev d)
         const double ephi = pha[q * fstride];
         const double xa = vert ? pf : s[q], ya = vert ? s[q] : pf;
         double hL, cL, hR, cR;
-        const PhaseDev php = phase_at(m.ph, tpt, trow + q);
-        const PhaseOut a = point_phase<true>(php, P, xa, ya, hL, cL, err);
+        const PhaseOut a = point_phase<true>(m.ph, tpt, trow + q, P, xa, ya, hL, cL, err);
         PhaseOut b = a;
         cR = cL;
         if (!boundary) {
             const double xb = vert ? -hf : s[q], yb = vert ? s[q] : -hf;
-            b = point_phase<true>(php, Q, xb, yb, hR, cR, err);
+            b = point_phase<true>(m.ph, tpt, trow + q, Q, xb, yb, hR, cR, err);
         }
         (void)hR;
         // PhaseTransportVelocities (src/couple/evolve.cpp:36-52) along the normal
@@ -251,7 +255,7 @@ __global__ void coupled_face_kernel(CoupledDev d)
             double T = a.T;
             if (tH == 0) {
                 uint32_t e2 = 0;
-                T = phase_eval<false>(php, valH, inflowC, e2).T;
+                T = phase_eval<false>(phase_at(m.ph, tpt, trow + q), valH, inflowC, e2).T;
                 err |= e2;
             }
             fh = tH == 2 ? 0 : vn * T - LD * (1 - ephi) * vs;
@@ -290,8 +294,8 @@ __global__ void coupled_face_kernel(CoupledDev d)
             for (int k = 0; k < nin; ++k) {
                 const double off = boundary ? sgn * depth[k] : depth[k];
                 double h_, c_;
-                Tk[k] = point_phase<false>(sample_phase(d, m, vert, trow + q, k), P, vert ? off : s[q],
-                                           vert ? s[q] : off, h_, c_, err).T;
+                Tk[k] = point_phase<false>(m.ph, sample_table(d, vert), sample_index(d, vert, trow + q, k),
+                                           P, vert ? off : s[q], vert ? s[q] : off, h_, c_, err).T;
             }
             double last;
             if (boundary) {
@@ -301,7 +305,8 @@ __global__ void coupled_face_kernel(CoupledDev d)
                 for (int k = n / 2; k < n; ++k) {
                     const double off = -depth[n - 1 - k];
                     double h_, c_;
-                    Tk[k] = point_phase<false>(sample_phase(d, m, vert, trow + q, k), Q,
+                    Tk[k] = point_phase<false>(m.ph, sample_table(d, vert),
+                                               sample_index(d, vert, trow + q, k), Q,
                                                vert ? off : s[q], vert ? s[q] : off, h_, c_, err).T;
                 }
                 last = Tk[n - 1];

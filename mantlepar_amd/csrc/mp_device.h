@@ -79,6 +79,9 @@ __device__ __forceinline__ double thermal_flux(const MeshDev& m, int side, int k
 // Wave reduction of one work item's partial sums in a fixed butterfly order
 // (compensated for the state sums), stored by lane 0; error bits are or-ed
 // into the launch's error word.
+// SUMS_ONLY writes the state sums (slots 0-3) and leaves the boundary fluxes
+// (4-5) of the item's stage as they are.
+template <bool SUMS_ONLY = false>
 __device__ __forceinline__ void item_partials(const StagePtrs& sp, int item, double sH, double eH,
                                               double sC, double eC, double bH, double bC,
                                               uint32_t err)
@@ -99,7 +102,7 @@ __device__ __forceinline__ void item_partials(const StagePtrs& sp, int item, dou
     if ((threadIdx.x & (kWave - 1)) == 0 && sp.partials) {
         double* p = sp.partials + size_t(item) * kPartials;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) p[k] = v[k];
+        for (int k = 0; k < (SUMS_ONLY ? 4 : 6); ++k) p[k] = v[k];
     }
     if (err) atomicOr(sp.error, err);
 }

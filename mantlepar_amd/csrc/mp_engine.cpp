@@ -485,11 +485,17 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
         }
         const auto& L = lists.at(p);
         for (int st = 0; st < scheme; ++st) {
+            // the mixed sweep leaves the phase to its own streaming kernel, as
+            // the single-mesh handle does
+            const bool last = st == scheme - 1, split = last && precision == MP_PREC_MIXED;
+            const StagePtrs* sps = base + size_t(set * 3 + st) * capacity;
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
-            MP_HIP(launch_batch_stage(precision, st == scheme - 1, q_common, d_meshes.as<MeshDev>(),
-                                      base + size_t(set * 3 + st) * capacity,
+            MP_HIP(launch_batch_stage(precision, last && !split, q_common, d_meshes.as<MeshDev>(), sps,
                                       d_items.as<int2>() + L.first, L.second, blocks,
                                       d_queues.as<int>() + 3 * s + st, stream));
+            if (split)
+                MP_HIP(launch_batch_phase_items(d_meshes.as<MeshDev>(), sps, d_items.as<int2>() + L.first,
+                                                L.second, stream));
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
         }
         MP_HIP(launch_batch_reduce(base + size_t(set * 3) * capacity, d_nitems.as<int>(),

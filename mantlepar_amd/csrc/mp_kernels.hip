@@ -356,25 +356,47 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void stage_sweep_kernel(con
 }
 
 
-// Step totals from the stage partials in a fixed order: 1024 threads take
-// strided slices, then a fixed LDS tree combines them (compensated for the
-// state sums).  out[0..3] = sum H, sum C (times area), stage-weighted
-// boundary H, C.
-constexpr int kReduceThreads = 1024;
+// Step totals from the stage partials in a fixed order, in two passes so the
+// partials (2.5 MB at 2048x4096) stream through many CUs: block b of
+// kStepBlocks takes a contiguous slice of items, its threads strided slices
+// of that, and a fixed LDS tree (compensated for the state sums) combines
+// them; one block then combines the blocks' results the same way.
+// out[0..3] = sum H, sum C (times area), stage-weighted boundary H, C.
+constexpr int kStepThreads = 256;
 
-__global__ __launch_bounds__(kReduceThreads) void reduce_step_kernel(
-    const double* __restrict__ partials, int nitems, int nstages, double w0, double w1,
-    double w2, double area, double* out)
+// Fixed-order tree over sh[6][T]: (0,1) and (2,3) compensated pairs, 4 and 5 plain.
+template <int T>
+__device__ __forceinline__ void step_tree(double (&sh)[6][T], int tid)
 {
-    __shared__ double sh[6][kReduceThreads];
+    for (int half = T / 2; half >= 1; half >>= 1) {
+        if (tid < half) {
+            double s0 = sh[0][tid], e0 = sh[1][tid] + sh[1][tid + half];
+            two_sum_acc(s0, e0, sh[0][tid + half]);
+            double s1 = sh[2][tid], e1 = sh[3][tid] + sh[3][tid + half];
+            two_sum_acc(s1, e1, sh[2][tid + half]);
+            sh[0][tid] = s0; sh[1][tid] = e0; sh[2][tid] = s1; sh[3][tid] = e1;
+            sh[4][tid] += sh[4][tid + half];
+            sh[5][tid] += sh[5][tid + half];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kStepThreads) void reduce_step_kernel(
+    const double* __restrict__ partials, int nitems, int nstages, double w0, double w1,
+    double w2, double* __restrict__ part)
+{
+    __shared__ double sh[6][kStepThreads];
     const int tid = threadIdx.x;
+    const int chunk = (nitems + int(gridDim.x) - 1) / int(gridDim.x);
+    const int lo = int(blockIdx.x) * chunk, hi = min(nitems, lo + chunk);
     double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
     const double wst[3] = {w0, w1, w2};
     for (int st = 0; st < nstages; ++st) {
         const double* p = partials + size_t(st) * nitems * kPartials;
         const double w = wst[st];
         const bool last = st == nstages - 1;
-        for (int it = tid; it < nitems; it += kReduceThreads) {
+        for (int it = lo + tid; it < hi; it += kStepThreads) {
             const double* q = p + size_t(it) * kPartials;
             if (last) {
                 two_sum_acc(sH, eH, q[0]);
@@ -388,18 +410,19 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_step_kernel(
     }
     sh[0][tid] = sH; sh[1][tid] = eH; sh[2][tid] = sC; sh[3][tid] = eC; sh[4][tid] = bH; sh[5][tid] = bC;
     __syncthreads();
-    for (int half = kReduceThreads / 2; half >= 1; half >>= 1) {
-        if (tid < half) {
-            double s0 = sh[0][tid], e0 = sh[1][tid] + sh[1][tid + half];
-            two_sum_acc(s0, e0, sh[0][tid + half]);
-            double s1 = sh[2][tid], e1 = sh[3][tid] + sh[3][tid + half];
-            two_sum_acc(s1, e1, sh[2][tid + half]);
-            sh[0][tid] = s0; sh[1][tid] = e0; sh[2][tid] = s1; sh[3][tid] = e1;
-            sh[4][tid] += sh[4][tid + half];
-            sh[5][tid] += sh[5][tid + half];
-        }
-        __syncthreads();
-    }
+    step_tree<kStepThreads>(sh, tid);
+    if (tid < 6) part[size_t(blockIdx.x) * 6 + tid] = sh[tid][0];
+}
+
+__global__ __launch_bounds__(kStepBlocks) void reduce_step_final_kernel(const double* __restrict__ part,
+                                                                       double area, double* out)
+{
+    __shared__ double sh[6][kStepBlocks];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sh[k][tid] = part[size_t(tid) * 6 + k];
+    __syncthreads();
+    step_tree<kStepBlocks>(sh, tid);
     if (tid == 0) {
         out[0] = (sh[0][0] + sh[1][0]) * area;
         out[1] = (sh[2][0] + sh[3][0]) * area;
@@ -448,58 +471,57 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
 }
 
 // Per-mesh step totals: one block per mesh, the same fixed order as the
-// single-mesh reduction, so each mesh's totals are bitwise those of the
-// single-mesh path.  Partials of mesh k start at stages[k].partials with
-// nitems[k] items per stage; out[k*4 + 0..3] as in reduce_step_kernel.
-__global__ __launch_bounds__(kReduceThreads) void batch_reduce_kernel(
+// single-mesh reduction (kStepBlocks slices, each a strided 256-thread
+// accumulation and tree, then a tree over the slices), so each mesh's totals
+// are bitwise those of the single-mesh path.  Partials of mesh k start at
+// stages[k].partials with nitems[k] items per stage; out[k*4 + 0..3] as in
+// reduce_step_final_kernel.
+__global__ __launch_bounds__(kStepThreads) void batch_reduce_kernel(
     const StagePtrs* __restrict__ stages, const int* __restrict__ nitems,
     const MeshDev* __restrict__ meshes, const int* __restrict__ slots, int nstages, double w0,
     double w1, double w2, double* out)
 {
-    __shared__ double sh[6][kReduceThreads];
+    __shared__ double sh[6][kStepThreads];
+    __shared__ double blk[6][kStepBlocks];
     const int k = slots[blockIdx.x];
     const int tid = threadIdx.x;
     const int n = nitems[k];
     const double* base = stages[k].partials;
-    double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
     const double wst[3] = {w0, w1, w2};
-    for (int st = 0; st < nstages; ++st) {
-        const double* p = base + size_t(st) * n * kPartials;
-        const double w = wst[st];
-        const bool last = st == nstages - 1;
-        for (int it = tid; it < n; it += kReduceThreads) {
-            const double* q = p + size_t(it) * kPartials;
-            if (last) {
-                two_sum_acc(sH, eH, q[0]);
-                eH += q[1];
-                two_sum_acc(sC, eC, q[2]);
-                eC += q[3];
+    const int chunk = (n + kStepBlocks - 1) / kStepBlocks;
+    for (int c = 0; c < kStepBlocks; ++c) {
+        const int lo = c * chunk, hi = min(n, lo + chunk);
+        double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
+        for (int st = 0; st < nstages; ++st) {
+            const double* p = base + size_t(st) * n * kPartials;
+            const double w = wst[st];
+            const bool last = st == nstages - 1;
+            for (int it = lo + tid; it < hi; it += kStepThreads) {
+                const double* q = p + size_t(it) * kPartials;
+                if (last) {
+                    two_sum_acc(sH, eH, q[0]);
+                    eH += q[1];
+                    two_sum_acc(sC, eC, q[2]);
+                    eC += q[3];
+                }
+                bH += w * q[4];
+                bC += w * q[5];
             }
-            bH += w * q[4];
-            bC += w * q[5];
         }
-    }
-    sh[0][tid] = sH; sh[1][tid] = eH; sh[2][tid] = sC; sh[3][tid] = eC; sh[4][tid] = bH; sh[5][tid] = bC;
-    __syncthreads();
-    for (int half = kReduceThreads / 2; half >= 1; half >>= 1) {
-        if (tid < half) {
-            double s0 = sh[0][tid], e0 = sh[1][tid] + sh[1][tid + half];
-            two_sum_acc(s0, e0, sh[0][tid + half]);
-            double s1 = sh[2][tid], e1 = sh[3][tid] + sh[3][tid + half];
-            two_sum_acc(s1, e1, sh[2][tid + half]);
-            sh[0][tid] = s0; sh[1][tid] = e0; sh[2][tid] = s1; sh[3][tid] = e1;
-            sh[4][tid] += sh[4][tid + half];
-            sh[5][tid] += sh[5][tid + half];
-        }
+        sh[0][tid] = sH; sh[1][tid] = eH; sh[2][tid] = sC; sh[3][tid] = eC; sh[4][tid] = bH; sh[5][tid] = bC;
+        __syncthreads();
+        step_tree<kStepThreads>(sh, tid);
+        if (tid < 6) blk[tid][c] = sh[tid][0];
         __syncthreads();
     }
+    step_tree<kStepBlocks>(blk, tid);
     if (tid == 0) {
         const double area = meshes[k].area;
         double* o = out + size_t(k) * 4;
-        o[0] = (sh[0][0] + sh[1][0]) * area;
-        o[1] = (sh[2][0] + sh[3][0]) * area;
-        o[2] = sh[4][0];
-        o[3] = sh[5][0];
+        o[0] = (blk[0][0] + blk[1][0]) * area;
+        o[1] = (blk[2][0] + blk[3][0]) * area;
+        o[2] = blk[4][0];
+        o[3] = blk[5][0];
     }
 }
 
@@ -734,7 +756,85 @@ __global__ void state_phase_kernel(const MeshDev* __restrict__ mesh, const doubl
     if (err) atomicOr(error, err);
 }
 
+// Phase of a stage's output and its state sums, per work item of the stage
+// sweep: lane = column, rows in order, the same butterfly -- the values the
+// phase-fused sweep produces, bit for bit.  The mixed path runs its last
+// stage without the phase epilogue and this kernel after it: the sweep keeps
+// its fast interior form, and the phase work streams at full occupancy.
+__device__ __forceinline__ void phase_item(const MeshDev& m, const StagePtrs& sp, int item,
+                                           int strip, int j0, int j1)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int i = strip * kStripCells - 1 + lane;
+    const bool owned = i >= 0 && i < m.nx && lane >= 1 && lane <= kStripCells;
+    double sH = 0, eH = 0, sC = 0, eC = 0;
+    uint32_t err = 0;
+    if (owned) {
+        for (int r = j0; r < j1; ++r) {
+            const size_t k = size_t(r) * m.nx + i;
+            const double h = sp.Ho[k], c = sp.Co[k];
+            const PhaseOut po = phase_eval<false>(phase_at(m.ph, m.tep_row, r), h, c, err);
+            sp.T[k] = po.T;
+            sp.phi[k] = po.phil;
+            sp.Cl[k] = po.cl;
+            sp.Cs[k] = po.cs;
+            sp.region[k] = uint8_t(po.region);
+            two_sum_acc(sH, eH, h);
+            two_sum_acc(sC, eC, c);
+        }
+    }
+    item_partials<true>(sp, item, sH, eH, sC, eC, 0.0, 0.0, err);
+}
+
+__global__ __launch_bounds__(256) void phase_items_kernel(const MeshDev* __restrict__ mesh,
+                                                          StagePtrs sp, int nitems)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int item = xcd_slab_block(int(blockIdx.x), int(gridDim.x)) * 4 + wave;
+    if (item >= nitems) return;
+    const MeshDev& m = *mesh;
+    const int strip = item % sp.strips, band = item / sp.strips;
+    const int j0 = band * sp.rows;
+    phase_item(m, sp, item, strip, j0, min(j0 + sp.rows, m.ny));
+}
+
+// The same over a batch work list of (mesh slot, item) pairs.
+__global__ __launch_bounds__(256) void batch_phase_items_kernel(const MeshDev* __restrict__ meshes,
+                                                                const StagePtrs* __restrict__ stages,
+                                                                const int2* __restrict__ items,
+                                                                int nitems)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int it = int(blockIdx.x) * 4 + wave;
+    if (it >= nitems) return;
+    const int2 w = items[it];
+    const int slot = __builtin_amdgcn_readfirstlane(w.x);
+    const int local = __builtin_amdgcn_readfirstlane(w.y);
+    const MeshDev& m = meshes[slot];
+    const StagePtrs& sp = stages[slot];
+    const int strip = local % sp.strips, band = local / sp.strips;
+    const int j0 = band * sp.rows;
+    if (j0 < m.ny) phase_item(m, sp, local, strip, j0, min(j0 + sp.rows, m.ny));
+}
+
 // ---------------------------------------------------------------- launchers
+
+hipError_t launch_phase_items(const MeshDev* d_mesh, const StagePtrs& sp, int nitems,
+                              hipStream_t stream)
+{
+    const dim3 grid((nitems + 3) / 4), block(256);
+    phase_items_kernel<<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_phase_items(const MeshDev* meshes, const StagePtrs* stages,
+                                    const int2* items, int nitems, hipStream_t stream)
+{
+    if (nitems <= 0) return hipSuccess;
+    const dim3 grid((nitems + 3) / 4), block(256);
+    batch_phase_items_kernel<<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
+    return hipGetLastError();
+}
 
 hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
                         int nitems, hipStream_t stream, bool limit)
@@ -828,9 +928,11 @@ hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H
 }
 
 hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, const double* w,
-                              double area, double* out, hipStream_t stream)
+                              double area, double* out, double* scratch, hipStream_t stream)
 {
-    reduce_step_kernel<<<1, kReduceThreads, 0, stream>>>(partials, nitems, nstages, w[0], w[1], w[2], area, out);
+    reduce_step_kernel<<<kStepBlocks, kStepThreads, 0, stream>>>(partials, nitems, nstages, w[0], w[1],
+                                                                  w[2], scratch);
+    reduce_step_final_kernel<<<1, kStepBlocks, 0, stream>>>(scratch, area, out);
     return hipGetLastError();
 }
 
@@ -867,7 +969,7 @@ hipError_t launch_batch_reduce(const StagePtrs* stages, const int* nitems, const
                                double* out, hipStream_t stream)
 {
     if (nslots <= 0) return hipSuccess;
-    batch_reduce_kernel<<<nslots, kReduceThreads, 0, stream>>>(stages, nitems, meshes, slots,
+    batch_reduce_kernel<<<nslots, kStepThreads, 0, stream>>>(stages, nitems, meshes, slots,
                                                                nstages, w[0], w[1], w[2], out);
     return hipGetLastError();
 }

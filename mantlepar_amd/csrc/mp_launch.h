@@ -48,7 +48,15 @@ hipError_t launch_stage32(bool phase, const MeshDev* d_mesh, const StagePtrs& sp
 hipError_t launch_batch_stage32(bool phase, const MeshDev* meshes, const StagePtrs* stages,
                                 const int2* items, int nitems, int blocks, int* queue,
                                 hipStream_t stream);
+// Phase and state sums of a stage's output over the stage's work items
+// (sp.Ho/sp.Co in, sp.T..region and partial slots 0-3 out).
+hipError_t launch_phase_items(const MeshDev* d_mesh, const StagePtrs& sp, int nitems,
+                              hipStream_t stream);
+hipError_t launch_batch_phase_items(const MeshDev* meshes, const StagePtrs* stages,
+                                    const int2* items, int nitems, hipStream_t stream);
+// Step totals (two passes); scratch: kStepBlocks * 6 doubles.
+constexpr int kStepBlocks = 64;
 hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, const double* w,
-                              double area, double* out, hipStream_t stream);
+                              double area, double* out, double* scratch, hipStream_t stream);
 
 }  // namespace mp

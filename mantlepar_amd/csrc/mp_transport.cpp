@@ -118,7 +118,8 @@ void mp_transport::set_tile(int r)
 
 mp_status mp_transport::ensure_scratch(int nsteps)
 {
-    const size_t pbytes = size_t(3) * nitems * kPartials * sizeof(double);
+    // stage partials, then the step reduction's block results
+    const size_t pbytes = (size_t(3) * nitems * kPartials + size_t(kStepBlocks) * 6) * sizeof(double);
     if (partials.bytes < pbytes) MP_CALL(partials.alloc(pbytes));
     const size_t tbytes = size_t(nsteps > 0 ? nsteps : 1) * 4 * sizeof(double);
     if (totals.bytes < tbytes) MP_CALL(totals.alloc(tbytes));
@@ -267,14 +268,18 @@ mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& fi
             const bool last = st == scheme - 1;
             const int out = last ? dst : 3 + st;
             const StagePtrs sp = stage_ptrs(src, kin, out, scheme, st, dt, last, phase_set, st);
+            // the mixed sweep leaves the phase to its own streaming kernel
+            const bool split = last && precision == MP_PREC_MIXED;
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
-            MP_HIP(launch_stage(precision, last, mesh.q, d_md.as<MeshDev>(), sp, nitems, stream,
-                                md.limit != 0));
+            MP_HIP(launch_stage(precision, last && !split, mesh.q, d_md.as<MeshDev>(), sp, nitems,
+                                stream, md.limit != 0));
+            if (split) MP_HIP(launch_phase_items(d_md.as<MeshDev>(), sp, nitems, stream));
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
             kin = out;
         }
         MP_HIP(launch_reduce_step(partials.as<double>(), nitems, scheme, kW[scheme - 1], md.area,
-                                  totals.as<double>() + size_t(s) * 4, stream));
+                                  totals.as<double>() + size_t(s) * 4,
+                                  partials.as<double>() + size_t(3) * nitems * kPartials, stream));
         src = dst;
         final_state = dst;
     }
