@@ -127,8 +127,9 @@ __global__ void face_prep_kernel(const MeshDev* __restrict__ mesh, FaceMom* fx, 
     (vert ? fx : fy)[f] = o;
 }
 
-// Flux of both fields through one interior face.
-template <int MODE, int NQ>
+// Flux of both fields through one interior face; DIFF adds the thermal
+// diffusion of H.
+template <int MODE, int NQ, bool DIFF>
 __device__ __forceinline__ void interior_pair(const MeshDev& m, bool vertical, size_t face,
                                               double tLH, const RealT<MODE> (&cLH)[3], double tRH,
                                               const RealT<MODE> (&cRH)[3], double tLC,
@@ -146,12 +147,15 @@ __device__ __forceinline__ void interior_pair(const MeshDev& m, bool vertical, s
         moment_pair(f.vp, f.vm, f.p1, f.p2, f.m1, f.m2, tLH, cLH, tRH, cRH, tLC, cLC, tRC, cRC,
                     FH, FC);
     }
-    FH = fma(vertical ? m.dif_x : m.dif_y, tLH - tRH, FH);   // thermal diffusion of H
+    if constexpr (DIFF) FH = fma(vertical ? m.dif_x : m.dif_y, tLH - tRH, FH);
 }
 
 // LIM: apply the bound limiter to both fields' reconstructions (FP64 only).
-template <int MODE, bool PHASE, int NQ, bool LIM = false>
-__device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int strip, int j0,
+// DIFF: the mesh has thermal diffusion or thermal boundary data; without it
+// the diffusion terms are compiled out, which keeps the sweep within its
+// register budget (with them it spills 64 B more per lane).
+template <int MODE, bool PHASE, int NQ, bool LIM = false, bool DIFF = true>
+__device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int strip, int j0,
                            int j1)
 {
     using R = RealT<MODE>;
@@ -234,7 +238,7 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
                 const double* inC = m.inflowC[0] ? m.inflowC[0] + size_t(ic) * m.q : nullptr;
                 const uint8_t pol = m.bc[0] ? m.bc[0][ic] : 0;
                 const double oH = boundary_flux<R>(m, vn, vys, -1.0, tH, LH.c[0], m.sh, m.wh, pol, inH, err) +
-                                  thermal_flux(m, 0, ic, tH, m.wh, m.dif_by);
+                                  (DIFF ? thermal_flux(m, 0, ic, tH, m.wh, m.dif_by) : 0.0);
                 const double oC = boundary_flux<R>(m, vn, vys, -1.0, tC, LC.c[0], m.sh, m.wh, pol, inC, err);
                 FvH = -oH; FvC = -oC;
                 if (owned) { bH += oH; bC += oC; }
@@ -243,12 +247,12 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
                 const double* inC = m.inflowC[2] ? m.inflowC[2] + size_t(ic) * m.q : nullptr;
                 const uint8_t pol = m.bc[2] ? m.bc[2][ic] : 0;
                 const double oH = boundary_flux<R>(m, vn, vys, 1.0, tTopH, topH, m.sh, m.wh, pol, inH, err) +
-                                  thermal_flux(m, 2, ic, tTopH, m.wh, m.dif_by);
+                                  (DIFF ? thermal_flux(m, 2, ic, tTopH, m.wh, m.dif_by) : 0.0);
                 const double oC = boundary_flux<R>(m, vn, vys, 1.0, tTopC, topC, m.sh, m.wh, pol, inC, err);
                 FvH = oH; FvC = oC;
                 if (owned) { bH += oH; bC += oC; }
             } else {
-                interior_pair<MODE, NQ>(m, false, size_t(r) * nx + ic, tTopH, topH, tH, LH.c[0],
+                interior_pair<MODE, NQ, DIFF>(m, false, size_t(r) * nx + ic, tTopH, topH, tH, LH.c[0],
                                         tTopC, topC, tC, LC.c[0], FvH, FvC);
             }
         }
@@ -301,7 +305,7 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
                     const double* inC = m.inflowC[3] ? m.inflowC[3] + size_t(r) * m.q : nullptr;
                     const uint8_t pol = m.bc[3] ? m.bc[3][r] : 0;
                     const double oH = boundary_flux<R>(m, vn, vxs, -1.0, nbtH, nbH, m.sv, m.wv, pol, inH, err) +
-                                      thermal_flux(m, 3, r, nbtH, m.wv, m.dif_bx);
+                                      (DIFF ? thermal_flux(m, 3, r, nbtH, m.wv, m.dif_bx) : 0.0);
                     const double oC = boundary_flux<R>(m, vn, vxs, -1.0, nbtC, nbC, m.sv, m.wv, pol, inC, err);
                     FrH = -oH; FrC = -oC;
                     bH += oH; bC += oC;
@@ -310,12 +314,12 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
                     const double* inC = m.inflowC[1] ? m.inflowC[1] + size_t(r) * m.q : nullptr;
                     const uint8_t pol = m.bc[1] ? m.bc[1][r] : 0;
                     const double oH = boundary_flux<R>(m, vn, vxs, 1.0, tH, LH.c[1], m.sv, m.wv, pol, inH, err) +
-                                      thermal_flux(m, 1, r, tH, m.wv, m.dif_bx);
+                                      (DIFF ? thermal_flux(m, 1, r, tH, m.wv, m.dif_bx) : 0.0);
                     const double oC = boundary_flux<R>(m, vn, vxs, 1.0, tC, LC.c[1], m.sv, m.wv, pol, inC, err);
                     FrH = oH; FrC = oC;
                     if (owned) { bH += oH; bC += oC; }
                 } else {
-                    interior_pair<MODE, NQ>(m, true, face, tH, LH.c[1], nbtH, nbH, tC, LC.c[1],
+                    interior_pair<MODE, NQ, DIFF>(m, true, face, tH, LH.c[1], nbtH, nbH, tC, LC.c[1],
                                             nbtC, nbC, FrH, FrC);
                 }
             }
@@ -342,7 +346,7 @@ __device__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int 
 
 // Register budget: two waves per SIMD (256 registers).  The mixed path has its
 // own sweep (mp_sweep32.hip).
-template <int MODE, bool PHASE, int NQ, bool LIM = false>
+template <int MODE, bool PHASE, int NQ, bool LIM, bool DIFF>
 __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void stage_sweep_kernel(const MeshDev* __restrict__ mesh,
                                                           StagePtrs sp, int nitems)
 {
@@ -352,23 +356,71 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void stage_sweep_kernel(con
     const int strip = item % sp.strips, band = item / sp.strips;
     const int j0 = band * sp.rows;
     const int j1 = min(j0 + sp.rows, m.ny);
-    sweep_item<MODE, PHASE, NQ, LIM>(m, sp, item, strip, j0, j1);
+    sweep_item<MODE, PHASE, NQ, LIM, DIFF>(m, sp, item, strip, j0, j1);
 }
 
 
-// Step totals from the stage partials in a fixed order, in two passes so the
-// partials (2.5 MB at 2048x4096) stream through many CUs: block b of
-// kStepBlocks takes a contiguous slice of items, its threads strided slices
-// of that, and a fixed LDS tree (compensated for the state sums) combines
-// them; one block then combines the blocks' results the same way.
+// Step totals from the stage partials in a fixed order.  The item list is cut
+// into contiguous slices (step_chunk); one wave reduces a slice (lanes take
+// strided items, then the item_partials butterfly, compensated for the state
+// sums), and a fixed LDS tree combines the slices.  The single-mesh form runs
+// the slices as waves of one launch, so the partials (2.5 MB at 2048x4096)
+// stream through many CUs; the batch form runs one mesh per block, its waves
+// taking the slices in turn.  Both make the same operations in the same
+// order, so a mesh's totals in a batch are bitwise its single-mesh totals.
 // out[0..3] = sum H, sum C (times area), stage-weighted boundary H, C.
-constexpr int kStepThreads = 256;
 
-// Fixed-order tree over sh[6][T]: (0,1) and (2,3) compensated pairs, 4 and 5 plain.
-template <int T>
-__device__ __forceinline__ void step_tree(double (&sh)[6][T], int tid)
+// Slices of an n-item list: about 64 items each (one per lane and stage), at
+// least 4 and at most kStepSlices; slice c covers [c*chunk, (c+1)*chunk).
+__device__ __forceinline__ int step_chunk(int n)
 {
-    for (int half = T / 2; half >= 1; half >>= 1) {
+    const int slices = min(kStepSlices, max(4, (n + kWave - 1) / kWave));
+    return max(1, (n + slices - 1) / slices);
+}
+
+// Slice [lo, hi) of the item list; every lane returns the slice's six sums.
+__device__ __forceinline__ void step_slice(const double* __restrict__ partials, int nitems,
+                                           int nstages, const double (&wst)[3], int lo, int hi,
+                                           double (&v)[6])
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
+    for (int st = 0; st < nstages; ++st) {
+        const double* p = partials + size_t(st) * nitems * kPartials;
+        const double w = wst[st];
+        const bool last = st == nstages - 1;
+        for (int it = lo + lane; it < hi; it += kWave) {
+            const double* q = p + size_t(it) * kPartials;
+            if (last) {
+                two_sum_acc(sH, eH, q[0]);
+                eH += q[1];
+                two_sum_acc(sC, eC, q[2]);
+                eC += q[3];
+            }
+            bH += w * q[4];
+            bC += w * q[5];
+        }
+    }
+    v[0] = sH; v[1] = eH; v[2] = sC; v[3] = eC; v[4] = bH; v[5] = bC;
+    for (int off = 32; off >= 1; off >>= 1) {
+        double o[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) o[k] = __shfl_xor(v[k], off);
+        double s0 = v[0], e0 = v[1] + o[1];
+        two_sum_acc(s0, e0, o[0]);
+        double s1 = v[2], e1 = v[3] + o[3];
+        two_sum_acc(s1, e1, o[2]);
+        v[0] = s0; v[1] = e0; v[2] = s1; v[3] = e1;
+        v[4] += o[4];
+        v[5] += o[5];
+    }
+}
+
+// Fixed-order tree over sh[6][kStepSlices] (all threads of the block call it):
+// (0,1) and (2,3) compensated pairs, 4 and 5 plain; the result is in column 0.
+__device__ __forceinline__ void step_tree(double (&sh)[6][kStepSlices], int tid)
+{
+    for (int half = kStepSlices / 2; half >= 1; half >>= 1) {
         if (tid < half) {
             double s0 = sh[0][tid], e0 = sh[1][tid] + sh[1][tid + half];
             two_sum_acc(s0, e0, sh[0][tid + half]);
@@ -382,53 +434,40 @@ __device__ __forceinline__ void step_tree(double (&sh)[6][T], int tid)
     }
 }
 
-__global__ __launch_bounds__(kStepThreads) void reduce_step_kernel(
+__device__ __forceinline__ void step_out(const double (&sh)[6][kStepSlices], double area, double* out)
+{
+    out[0] = (sh[0][0] + sh[1][0]) * area;
+    out[1] = (sh[2][0] + sh[3][0]) * area;
+    out[2] = sh[4][0];
+    out[3] = sh[5][0];
+}
+
+// One wave per slice: kStepSlices / 4 blocks of 256 threads.
+__global__ __launch_bounds__(256) void reduce_step_kernel(
     const double* __restrict__ partials, int nitems, int nstages, double w0, double w1,
     double w2, double* __restrict__ part)
 {
-    __shared__ double sh[6][kStepThreads];
-    const int tid = threadIdx.x;
-    const int chunk = (nitems + int(gridDim.x) - 1) / int(gridDim.x);
-    const int lo = int(blockIdx.x) * chunk, hi = min(nitems, lo + chunk);
-    double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
+    const int slice = int(blockIdx.x) * 4 + int(threadIdx.x >> 6);
+    const int chunk = step_chunk(nitems);
+    const int lo = min(nitems, slice * chunk), hi = min(nitems, lo + chunk);
     const double wst[3] = {w0, w1, w2};
-    for (int st = 0; st < nstages; ++st) {
-        const double* p = partials + size_t(st) * nitems * kPartials;
-        const double w = wst[st];
-        const bool last = st == nstages - 1;
-        for (int it = lo + tid; it < hi; it += kStepThreads) {
-            const double* q = p + size_t(it) * kPartials;
-            if (last) {
-                two_sum_acc(sH, eH, q[0]);
-                eH += q[1];
-                two_sum_acc(sC, eC, q[2]);
-                eC += q[3];
-            }
-            bH += w * q[4];
-            bC += w * q[5];
-        }
-    }
-    sh[0][tid] = sH; sh[1][tid] = eH; sh[2][tid] = sC; sh[3][tid] = eC; sh[4][tid] = bH; sh[5][tid] = bC;
-    __syncthreads();
-    step_tree<kStepThreads>(sh, tid);
-    if (tid < 6) part[size_t(blockIdx.x) * 6 + tid] = sh[tid][0];
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (lo < hi) step_slice(partials, nitems, nstages, wst, lo, hi, v);
+    if ((threadIdx.x & (kWave - 1)) == 0)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) part[size_t(slice) * 6 + k] = v[k];
 }
 
-__global__ __launch_bounds__(kStepBlocks) void reduce_step_final_kernel(const double* __restrict__ part,
+__global__ __launch_bounds__(kStepSlices) void reduce_step_final_kernel(const double* __restrict__ part,
                                                                        double area, double* out)
 {
-    __shared__ double sh[6][kStepBlocks];
+    __shared__ double sh[6][kStepSlices];
     const int tid = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < 6; ++k) sh[k][tid] = part[size_t(tid) * 6 + k];
     __syncthreads();
-    step_tree<kStepBlocks>(sh, tid);
-    if (tid == 0) {
-        out[0] = (sh[0][0] + sh[1][0]) * area;
-        out[1] = (sh[2][0] + sh[3][0]) * area;
-        out[2] = sh[4][0];
-        out[3] = sh[5][0];
-    }
+    step_tree(sh, tid);
+    if (tid == 0) step_out(sh, area, out);
 }
 
 // ----------------------------------------------------------- batched sweep
@@ -465,64 +504,38 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
         const int strip = local % sp.strips, band = local / sp.strips;
         const int j0 = band * sp.rows;
         if (j0 < m.ny)                 // always true with a current work list
-            sweep_item<MODE, PHASE, NQ>(m, sp, local, strip, j0, min(j0 + sp.rows, m.ny));
+            sweep_item<MODE, PHASE, NQ, false, false>(m, sp, local, strip, j0, min(j0 + sp.rows, m.ny));
         if (!queue) break;
     }
 }
 
-// Per-mesh step totals: one block per mesh, the same fixed order as the
-// single-mesh reduction (kStepBlocks slices, each a strided 256-thread
-// accumulation and tree, then a tree over the slices), so each mesh's totals
-// are bitwise those of the single-mesh path.  Partials of mesh k start at
-// stages[k].partials with nitems[k] items per stage; out[k*4 + 0..3] as in
-// reduce_step_final_kernel.
-__global__ __launch_bounds__(kStepThreads) void batch_reduce_kernel(
+// Per-mesh step totals: one block per mesh, its four waves taking the slices
+// of the single-mesh reduction in turn, then the same tree.  Partials of mesh
+// k start at stages[k].partials with nitems[k] items per stage; out[k*4 + 0..3]
+// as in reduce_step_final_kernel.
+__global__ __launch_bounds__(256) void batch_reduce_kernel(
     const StagePtrs* __restrict__ stages, const int* __restrict__ nitems,
     const MeshDev* __restrict__ meshes, const int* __restrict__ slots, int nstages, double w0,
     double w1, double w2, double* out)
 {
-    __shared__ double sh[6][kStepThreads];
-    __shared__ double blk[6][kStepBlocks];
+    __shared__ double sh[6][kStepSlices];
     const int k = slots[blockIdx.x];
     const int tid = threadIdx.x;
     const int n = nitems[k];
     const double* base = stages[k].partials;
     const double wst[3] = {w0, w1, w2};
-    const int chunk = (n + kStepBlocks - 1) / kStepBlocks;
-    for (int c = 0; c < kStepBlocks; ++c) {
-        const int lo = c * chunk, hi = min(n, lo + chunk);
-        double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
-        for (int st = 0; st < nstages; ++st) {
-            const double* p = base + size_t(st) * n * kPartials;
-            const double w = wst[st];
-            const bool last = st == nstages - 1;
-            for (int it = lo + tid; it < hi; it += kStepThreads) {
-                const double* q = p + size_t(it) * kPartials;
-                if (last) {
-                    two_sum_acc(sH, eH, q[0]);
-                    eH += q[1];
-                    two_sum_acc(sC, eC, q[2]);
-                    eC += q[3];
-                }
-                bH += w * q[4];
-                bC += w * q[5];
-            }
-        }
-        sh[0][tid] = sH; sh[1][tid] = eH; sh[2][tid] = sC; sh[3][tid] = eC; sh[4][tid] = bH; sh[5][tid] = bC;
-        __syncthreads();
-        step_tree<kStepThreads>(sh, tid);
-        if (tid < 6) blk[tid][c] = sh[tid][0];
-        __syncthreads();
+    const int chunk = step_chunk(n);
+    for (int slice = tid >> 6; slice < kStepSlices; slice += 4) {
+        const int lo = min(n, slice * chunk), hi = min(n, lo + chunk);
+        double v[6] = {0, 0, 0, 0, 0, 0};
+        if (lo < hi) step_slice(base, n, nstages, wst, lo, hi, v);
+        if ((tid & (kWave - 1)) == 0)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) sh[q][slice] = v[q];
     }
-    step_tree<kStepBlocks>(blk, tid);
-    if (tid == 0) {
-        const double area = meshes[k].area;
-        double* o = out + size_t(k) * 4;
-        o[0] = (blk[0][0] + blk[1][0]) * area;
-        o[1] = (blk[2][0] + blk[3][0]) * area;
-        o[2] = blk[4][0];
-        o[3] = blk[5][0];
-    }
+    __syncthreads();
+    step_tree(sh, tid);
+    if (tid == 0) step_out(sh, meshes[k].area, out + size_t(k) * 4);
 }
 
 // Zero a small device array with a kernel (graph-safe: no memset node).
@@ -836,30 +849,37 @@ hipError_t launch_batch_phase_items(const MeshDev* meshes, const StagePtrs* stag
     return hipGetLastError();
 }
 
-hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
-                        int nitems, hipStream_t stream, bool limit)
+// The FP64 sweep's instances: (LIM, DIFF) in {(false, false), (false, true),
+// (true, true)} -- the limiter, which the coupled set-up pairs with diffusion,
+// keeps the diffusion terms.
+template <bool PHASE, int NQ>
+static void launch_fp64(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
+                        bool limit, bool diffusion)
 {
     const dim3 grid((nitems + 3) / 4), block(256);
-    if (limit) {
-        if (mode != 0) return hipErrorInvalidValue;
-        if (q == 5) {
-            if (phase) stage_sweep_kernel<0, true, 5, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-            else stage_sweep_kernel<0, false, 5, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-        } else {
-            if (phase) stage_sweep_kernel<0, true, 0, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-            else stage_sweep_kernel<0, false, 0, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-        }
-    } else if (mode == 0 && q == 5) {
-        if (phase) stage_sweep_kernel<0, true, 5><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-        else stage_sweep_kernel<0, false, 5><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-    } else if (mode == 0) {
-        if (phase) stage_sweep_kernel<0, true, 0><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-        else stage_sweep_kernel<0, false, 0><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    if (limit) stage_sweep_kernel<0, PHASE, NQ, true, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    else if (diffusion) stage_sweep_kernel<0, PHASE, NQ, false, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    else stage_sweep_kernel<0, PHASE, NQ, false, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+}
+
+hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
+                        int nitems, hipStream_t stream, bool limit, bool diffusion)
+{
+    if (mode != 0) {
+        if (phase || limit) return hipErrorInvalidValue;   // mixed: phase_items_kernel follows
+        return launch_stage32(d_mesh, sp, nitems, stream, diffusion);
+    }
+    if (q == 5) {
+        if (phase) launch_fp64<true, 5>(d_mesh, sp, nitems, stream, limit, diffusion);
+        else launch_fp64<false, 5>(d_mesh, sp, nitems, stream, limit, diffusion);
     } else {
-        return launch_stage32(phase, d_mesh, sp, nitems, stream);
+        if (phase) launch_fp64<true, 0>(d_mesh, sp, nitems, stream, limit, diffusion);
+        else launch_fp64<false, 0>(d_mesh, sp, nitems, stream, limit, diffusion);
     }
     return hipGetLastError();
 }
+
+
 
 hipError_t launch_limit_diag(const MeshDev* d_mesh, int ncells, const double* H, const double* C,
                              double* thetaH, double* thetaC, uint32_t* error, hipStream_t stream)
@@ -930,9 +950,9 @@ hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H
 hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, const double* w,
                               double area, double* out, double* scratch, hipStream_t stream)
 {
-    reduce_step_kernel<<<kStepBlocks, kStepThreads, 0, stream>>>(partials, nitems, nstages, w[0], w[1],
-                                                                  w[2], scratch);
-    reduce_step_final_kernel<<<1, kStepBlocks, 0, stream>>>(scratch, area, out);
+    reduce_step_kernel<<<kStepSlices / 4, 256, 0, stream>>>(partials, nitems, nstages, w[0], w[1], w[2],
+                                                             scratch);
+    reduce_step_final_kernel<<<1, kStepSlices, 0, stream>>>(scratch, area, out);
     return hipGetLastError();
 }
 
@@ -950,7 +970,8 @@ hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes
         if (phase) batch_sweep_kernel<0, true, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp);
         else batch_sweep_kernel<0, false, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp);
     } else {
-        return launch_batch_stage32(phase, meshes, stages, items, nitems, blocks, queue, stream);
+        if (phase) return hipErrorInvalidValue;   // mixed: batch_phase_items_kernel follows
+        return launch_batch_stage32(meshes, stages, items, nitems, blocks, queue, stream);
     }
     return hipGetLastError();
 }
@@ -969,7 +990,7 @@ hipError_t launch_batch_reduce(const StagePtrs* stages, const int* nitems, const
                                double* out, hipStream_t stream)
 {
     if (nslots <= 0) return hipSuccess;
-    batch_reduce_kernel<<<nslots, kStepThreads, 0, stream>>>(stages, nitems, meshes, slots,
+    batch_reduce_kernel<<<nslots, 256, 0, stream>>>(stages, nitems, meshes, slots,
                                                                nstages, w[0], w[1], w[2], out);
     return hipGetLastError();
 }

@@ -8,8 +8,10 @@
 namespace mp {
 
 // limit: apply the bound limiter (FP64 sweep only).
+// diffusion: the mesh has thermal diffusion or thermal boundary data (the
+// mixed sweep compiles the terms out otherwise).
 hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
-                        int nitems, hipStream_t stream, bool limit = false);
+                        int nitems, hipStream_t stream, bool limit = false, bool diffusion = true);
 hipError_t launch_limit_diag(const MeshDev* d_mesh, int ncells, const double* H, const double* C,
                              double* thetaH, double* thetaC, uint32_t* error, hipStream_t stream);
 hipError_t launch_face_prep(const MeshDev* d_mesh, int nx, int ny, FaceMom* fx, FaceMom* fy,
@@ -43,19 +45,21 @@ hipError_t launch_clear(uint32_t* p, int n, hipStream_t stream);
 int batch_blocks_per_cu(int mode);
 // Mixed-precision sweeps (mp_sweep32.hip).
 int batch_blocks_per_cu32();
-hipError_t launch_stage32(bool phase, const MeshDev* d_mesh, const StagePtrs& sp, int nitems,
-                          hipStream_t stream);
-hipError_t launch_batch_stage32(bool phase, const MeshDev* meshes, const StagePtrs* stages,
-                                const int2* items, int nitems, int blocks, int* queue,
-                                hipStream_t stream);
+// The mixed sweep has no phase-fused form: phase_items_kernel follows its last stage.
+// diffusion: the mesh has thermal diffusion or thermal boundary data.
+hipError_t launch_stage32(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
+                          bool diffusion);
+hipError_t launch_batch_stage32(const MeshDev* meshes, const StagePtrs* stages, const int2* items,
+                                int nitems, int blocks, int* queue, hipStream_t stream);
 // Phase and state sums of a stage's output over the stage's work items
 // (sp.Ho/sp.Co in, sp.T..region and partial slots 0-3 out).
 hipError_t launch_phase_items(const MeshDev* d_mesh, const StagePtrs& sp, int nitems,
                               hipStream_t stream);
 hipError_t launch_batch_phase_items(const MeshDev* meshes, const StagePtrs* stages,
                                     const int2* items, int nitems, hipStream_t stream);
-// Step totals (two passes); scratch: kStepBlocks * 6 doubles.
-constexpr int kStepBlocks = 64;
+// Step totals (two passes over kStepSlices wave-sized slices of the item
+// list); scratch: kStepSlices * 6 doubles.
+constexpr int kStepSlices = 128;
 hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, const double* w,
                               double area, double* out, double* scratch, hipStream_t stream);
 

@@ -1,5 +1,5 @@
 // Mixed-precision stage sweep (FP32 reconstruction and flux corrections, FP64
-// state, first-order flux part, update, phase and sums), written for gfx950.
+// state, first-order flux part and update), written for gfx950.
 //
 // Layout as the FP64 sweep: one wavefront owns a strip of 62 columns plus a
 // halo lane on each side and walks up a band of rows.  What differs is how
@@ -24,7 +24,6 @@
 #include "mp_common.h"
 #include "mp_device.h"
 #include "mp_launch.h"
-#include "mp_phase.h"
 #include "mp_recon.h"
 
 namespace mp {
@@ -214,7 +213,8 @@ __device__ __forceinline__ void recon32(const K32& k, const Row32& R0, const Row
 
 // Both fields through one interior face from its moment record: first-order
 // part in FP64, high-order correction in FP32.
-// D: thermal diffusion coefficient of H on this face (0: off).
+// D: thermal diffusion coefficient of H on this face (DIFF = false: none).
+template <bool DIFF>
 __device__ __forceinline__ void face_flux(const FaceMom& f, double D, double tLH,
                                           const float (&cLH)[3], double tRH,
                                           const float (&cRH)[3], double tLC,
@@ -224,12 +224,19 @@ __device__ __forceinline__ void face_flux(const FaceMom& f, double D, double tLH
     const float vp = float(f.vp), vm = float(f.vm);
     const float ch = cLH[0] * vp + cLH[1] * f.p1 + cLH[2] * f.p2 + cRH[0] * vm + cRH[1] * f.m1 + cRH[2] * f.m2;
     const float cc = cLC[0] * vp + cLC[1] * f.p1 + cLC[2] * f.p2 + cRC[0] * vm + cRC[1] * f.m1 + cRC[2] * f.m2;
-    FH = fma(D, tLH - tRH, fma(tLH, f.vp, tRH * f.vm) + double(ch));
+    const double adv = fma(tLH, f.vp, tRH * f.vm) + double(ch);
+    FH = DIFF ? fma(D, tLH - tRH, adv) : adv;
     FC = fma(tLC, f.vp, tRC * f.vm) + double(cc);
 }
 
-template <bool PHASE>
-__device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, int strip, int j0, int j1)
+// One work item of a stage.  The stage's phase and state sums come from
+// phase_items_kernel after the last stage, so this sweep only writes the
+// stage's boundary-flux partials (state-sum slots zero).  DIFF: the mesh has
+// thermal diffusion or thermal boundary data; without it the diffusion terms
+// are compiled out, which keeps the sweep within its register budget.
+template <bool DIFF>
+__device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, int strip, int j0,
+                                             int j1)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int nx = m.nx, ny = m.ny;
@@ -243,11 +250,8 @@ __device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, in
     const size_t vys = size_t(ny + 1) * nx;
     const K32 k = make_k32(m);
     uint32_t err = 0;
-    // every lane's 3x3 neighbourhood inside the mesh (wave-uniform).  The
-    // phase-fused kernel keeps the single general path: the second copy of the
-    // reconstruction pushed it past its register budget.
-    const bool inner_strip = !PHASE && strip >= 1 && strip * kStripCells + kWave <= nx &&
-                             !k.use_constant;
+    // every lane's 3x3 neighbourhood inside the mesh (wave-uniform)
+    const bool inner_strip = strip >= 1 && strip * kStripCells + kWave <= nx && !k.use_constant;
 
     auto load = [&](const double* base, int row, double& u, double& o) {
         const double* p = base + size_t(clampi(row, 0, ny - 1)) * nx;
@@ -285,7 +289,7 @@ __device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, in
     // (left - right + bottom) face fluxes of the row waiting for its top face,
     // summed in that order as soon as they are known.
     double accH = 0, accC = 0;
-    double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
+    double bH = 0, bC = 0;
 
     for (; r <= j1; ++r) {
         const bool have = r < ny;
@@ -297,7 +301,7 @@ __device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, in
             load(sp.Ck, r + 2, nC, oC);
         }
         FaceLines<float> LH, LC;
-        if (!PHASE && have && inner_strip && r >= 1 && r <= ny - 2) {
+        if (have && inner_strip && r >= 1 && r <= ny - 2) {
             recon32<false>(k, H0, H1, H2, 31u, LH, err);
             recon32<false>(k, C0, C1, C2, 31u, LC, err);
         } else if (have) {
@@ -326,14 +330,14 @@ __device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, in
                 const uint8_t pol = m.bc[side] ? m.bc[side][ic] : 0;
                 const double oHf = (r == 0 ? boundary_flux<float>(m, vn, vys, sign, tH, LH.c[0], m.sh, m.wh, pol, inH, err)
                                            : boundary_flux<float>(m, vn, vys, sign, H0.u, topH, m.sh, m.wh, pol, inH, err)) +
-                                   thermal_flux(m, side, ic, r == 0 ? tH : H0.u, m.wh, m.dif_by);
+                                   (DIFF ? thermal_flux(m, side, ic, r == 0 ? tH : H0.u, m.wh, m.dif_by) : 0.0);
                 const double oCf = r == 0 ? boundary_flux<float>(m, vn, vys, sign, tC, LC.c[0], m.sh, m.wh, pol, inC, err)
                                           : boundary_flux<float>(m, vn, vys, sign, C0.u, topC, m.sh, m.wh, pol, inC, err);
                 FvH = sign * oHf;
                 FvC = sign * oCf;
                 if (owned) { bH += oHf; bC += oCf; }
             } else {
-                face_flux(fyc, m.dif_y, H0.u, topH, tH, LH.c[0], C0.u, topC, tC, LC.c[0], FvH, FvC);
+                face_flux<DIFF>(fyc, m.dif_y, H0.u, topH, tH, LH.c[0], C0.u, topC, tC, LC.c[0], FvH, FvC);
             }
         }
         if (shift) fyc = load_fy(r + 1);   // used next iteration
@@ -351,16 +355,6 @@ __device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, in
             if (!isfinite(outH) || !isfinite(outC)) err |= kErrNonFinite;
             sp.Ho[kk] = outH;
             sp.Co[kk] = outC;
-            if constexpr (PHASE) {
-                const PhaseOut po = phase_eval<false>(phase_at(m.ph, m.tep_row, r - 1), outH, outC, err);
-                sp.T[kk] = po.T;
-                sp.phi[kk] = po.phil;
-                sp.Cl[kk] = po.cl;
-                sp.Cs[kk] = po.cs;
-                sp.region[kk] = uint8_t(po.region);
-                two_sum_acc(sH, eH, outH);
-                two_sum_acc(sC, eC, outC);
-            }
         }
 
         if (need_n && have) {               // u_n of row r, finished next iteration
@@ -390,14 +384,14 @@ __device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, in
                     const uint8_t pol = m.bc[side] ? m.bc[side][r] : 0;
                     const double oHf = (left ? boundary_flux<float>(m, vn, vxs, sign, nbtH, nbH, m.sv, m.wv, pol, inH, err)
                                              : boundary_flux<float>(m, vn, vxs, sign, tH, LH.c[1], m.sv, m.wv, pol, inH, err)) +
-                                       thermal_flux(m, side, r, left ? nbtH : tH, m.wv, m.dif_bx);
+                                       (DIFF ? thermal_flux(m, side, r, left ? nbtH : tH, m.wv, m.dif_bx) : 0.0);
                     const double oCf = left ? boundary_flux<float>(m, vn, vxs, sign, nbtC, nbC, m.sv, m.wv, pol, inC, err)
                                             : boundary_flux<float>(m, vn, vxs, sign, tC, LC.c[1], m.sv, m.wv, pol, inC, err);
                     FrH = sign * oHf;
                     FrC = sign * oCf;
                     if (left || owned) { bH += oHf; bC += oCf; }
                 } else {
-                    face_flux(fxc, m.dif_x, tH, LH.c[1], nbtH, nbH, tC, LC.c[1], nbtC, nbC, FrH, FrC);
+                    face_flux<DIFF>(fxc, m.dif_x, tH, LH.c[1], nbtH, nbH, tC, LC.c[1], nbtC, nbC, FrH, FrC);
                 }
             }
             accH = shr1(FrH) - FrH + FvH;
@@ -414,10 +408,10 @@ __device__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, in
             C2 = make_row(nC, oC, C1.u, first, last, k);
         }
     }
-    item_partials(sp, item, sH, eH, sC, eC, bH, bC, err);
+    item_partials(sp, item, 0.0, 0.0, 0.0, 0.0, bH, bC, err);
 }
 
-template <bool PHASE>
+template <bool DIFF>
 __global__ __launch_bounds__(256, 3) void stage32_kernel(const MeshDev* __restrict__ mesh, StagePtrs sp, int nitems)
 {
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
@@ -427,12 +421,12 @@ __global__ __launch_bounds__(256, 3) void stage32_kernel(const MeshDev* __restri
     const int strip = item % sp.strips, band = item / sp.strips;
     const int j0 = band * sp.rows;
     const int j1 = min(j0 + sp.rows, m.ny);
-    sweep32_item<PHASE>(m, sp, item, strip, j0, j1);
+    sweep32_item<DIFF>(m, sp, item, strip, j0, j1);
 }
 
 // Persistent form (queue != nullptr): every wave takes the next work item
 // from a counter until the list is exhausted; otherwise one wave per item.
-template <bool PHASE>
+// Batch meshes carry no thermal diffusion.
 __global__ __launch_bounds__(256, 3) void batch32_kernel(const MeshDev* __restrict__ meshes,
                                                          const StagePtrs* __restrict__ stages,
                                                          const int2* __restrict__ items, int nitems,
@@ -455,18 +449,18 @@ __global__ __launch_bounds__(256, 3) void batch32_kernel(const MeshDev* __restri
         const int strip = local % sp.strips, band = local / sp.strips;
         const int j0 = band * sp.rows;
         if (j0 < m.ny)                 // always true with a current work list
-            sweep32_item<PHASE>(m, sp, local, strip, j0, min(j0 + sp.rows, m.ny));
+            sweep32_item<false>(m, sp, local, strip, j0, min(j0 + sp.rows, m.ny));
         if (!queue) break;
     }
 }
 
 }  // namespace
 
-hipError_t launch_stage32(bool phase, const MeshDev* d_mesh, const StagePtrs& sp, int nitems,
-                          hipStream_t stream)
+hipError_t launch_stage32(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
+                          bool diffusion)
 {
     const dim3 grid((nitems + 3) / 4), block(256);
-    if (phase) stage32_kernel<true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    if (diffusion) stage32_kernel<true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
     else stage32_kernel<false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
     return hipGetLastError();
 }
@@ -474,19 +468,17 @@ hipError_t launch_stage32(bool phase, const MeshDev* d_mesh, const StagePtrs& sp
 int batch_blocks_per_cu32()
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch32_kernel<false>, 256, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch32_kernel, 256, 0) != hipSuccess) return 0;
     return n;
 }
 
-hipError_t launch_batch_stage32(bool phase, const MeshDev* meshes, const StagePtrs* stages,
-                                const int2* items, int nitems, int blocks, int* queue,
-                                hipStream_t stream)
+hipError_t launch_batch_stage32(const MeshDev* meshes, const StagePtrs* stages, const int2* items,
+                                int nitems, int blocks, int* queue, hipStream_t stream)
 {
     if (nitems <= 0) return hipSuccess;
     int* q = blocks > 0 ? queue : nullptr;
     const dim3 grid(blocks > 0 ? blocks : (nitems + 3) / 4), block(256);
-    if (phase) batch32_kernel<true><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q);
-    else batch32_kernel<false><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q);
+    batch32_kernel<<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q);
     return hipGetLastError();
 }
 

@@ -119,7 +119,7 @@ void mp_transport::set_tile(int r)
 mp_status mp_transport::ensure_scratch(int nsteps)
 {
     // stage partials, then the step reduction's block results
-    const size_t pbytes = (size_t(3) * nitems * kPartials + size_t(kStepBlocks) * 6) * sizeof(double);
+    const size_t pbytes = (size_t(3) * nitems * kPartials + size_t(kStepSlices) * 6) * sizeof(double);
     if (partials.bytes < pbytes) MP_CALL(partials.alloc(pbytes));
     const size_t tbytes = size_t(nsteps > 0 ? nsteps : 1) * 4 * sizeof(double);
     if (totals.bytes < tbytes) MP_CALL(totals.alloc(tbytes));
@@ -272,7 +272,7 @@ mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& fi
             const bool split = last && precision == MP_PREC_MIXED;
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
             MP_HIP(launch_stage(precision, last && !split, mesh.q, d_md.as<MeshDev>(), sp, nitems,
-                                stream, md.limit != 0));
+                                stream, md.limit != 0, has_diffusion()));
             if (split) MP_HIP(launch_phase_items(d_md.as<MeshDev>(), sp, nitems, stream));
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
             kin = out;
@@ -284,6 +284,13 @@ mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& fi
         final_state = dst;
     }
     return MP_OK;
+}
+
+bool mp_transport::has_diffusion() const
+{
+    bool on = md.dif_x != 0.0 || md.dif_y != 0.0 || md.dif_bx != 0.0 || md.dif_by != 0.0;
+    for (int k = 0; k < 4; ++k) on = on || md.tbc[k] != nullptr;
+    return on;
 }
 
 mp_status mp_transport::check_step_args(int nsteps, int scheme, double dt)

@@ -68,4 +68,6 @@ struct mp_transport {
     mp_status set_diffusion(double kappa, int samples, double extent);
     mp_status set_thermal_boundary(const uint8_t* types, const double* values);
     mp_status set_lithostatic(int enable, double surface_y);
+    // thermal diffusion or thermal boundary data are set
+    bool has_diffusion() const;
 };
