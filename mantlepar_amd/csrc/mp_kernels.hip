@@ -15,6 +15,7 @@
 // launch order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "mp_common.h"
@@ -682,42 +683,55 @@ __global__ void rhs_diag_kernel(const MeshDev* __restrict__ mesh, const double* 
     if (err) atomicOr(error, err);
 }
 
-// Per-cell advective rate (src/couple/preheat.cpp:125-152); max via the
-// ordered bit pattern of non-negative doubles.
-__global__ void rate_kernel(const MeshDev* __restrict__ mesh, unsigned long long* max_bits)
+// Per-cell advective rate (src/couple/preheat.cpp:125-152) of cell c.
+__device__ __forceinline__ double cell_rate(const MeshDev& m, int c)
 {
-    const MeshDev& m = *mesh;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    double rate = 0.0;
-    if (c < m.nx * m.ny) {
-        const int nx = m.nx, ny = m.ny, i = c % nx, j = c / nx;
-        const size_t vxs = size_t(ny) * (nx + 1), vys = size_t(ny + 1) * nx;
-        double sx = 0.0, sy = 0.0;
-        for (int q = 0; q < m.q; ++q) {
-            const double a = m.wv[q] * fabs(m.vnx[q * vxs + size_t(j) * (nx + 1) + i]);
-            const double b = m.wv[q] * fabs(m.vnx[q * vxs + size_t(j) * (nx + 1) + i + 1]);
-            const double cc = m.wh[q] * fabs(m.vny[q * vys + size_t(j) * nx + i]);
-            const double d = m.wh[q] * fabs(m.vny[q * vys + size_t(j + 1) * nx + i]);
-            sx += a + b;
-            sy += cc + d;
-        }
-        // thermal diffusion: |sum of sample partials| per face side
-        // (src/couple/preheat.cpp:137-148)
-        double dif = (i > 0 ? m.dif_x : 0.0) + (i < nx - 1 ? m.dif_x : 0.0) +
-                     (j > 0 ? m.dif_y : 0.0) + (j < ny - 1 ? m.dif_y : 0.0);
-        const int bside[4] = {j == 0 ? 0 : -1, i == nx - 1 ? 1 : -1, j == ny - 1 ? 2 : -1, i == 0 ? 3 : -1};
-        for (int f = 0; f < 4; ++f) {
-            if (bside[f] < 0 || !m.tbc[f]) continue;
-            const int k = (f & 1) ? j : i;
-            const double* W = (f & 1) ? m.wv : m.wh;
-            const double coef = (f & 1) ? m.dif_bx : m.dif_by;
-            for (int q = 0; q < m.q; ++q)
-                if (m.tbc[f][size_t(k) * m.q + q] == 1) dif += W[q] * coef;
-        }
-        rate = (sx + sy + dif) * m.inv_area;
+    const int nx = m.nx, ny = m.ny, i = c % nx, j = c / nx;
+    const size_t vxs = size_t(ny) * (nx + 1), vys = size_t(ny + 1) * nx;
+    double sx = 0.0, sy = 0.0;
+    for (int q = 0; q < m.q; ++q) {
+        const double a = m.wv[q] * fabs(m.vnx[q * vxs + size_t(j) * (nx + 1) + i]);
+        const double b = m.wv[q] * fabs(m.vnx[q * vxs + size_t(j) * (nx + 1) + i + 1]);
+        const double cc = m.wh[q] * fabs(m.vny[q * vys + size_t(j) * nx + i]);
+        const double d = m.wh[q] * fabs(m.vny[q * vys + size_t(j + 1) * nx + i]);
+        sx += a + b;
+        sy += cc + d;
     }
+    // thermal diffusion: |sum of sample partials| per face side
+    // (src/couple/preheat.cpp:137-148)
+    double dif = (i > 0 ? m.dif_x : 0.0) + (i < nx - 1 ? m.dif_x : 0.0) +
+                 (j > 0 ? m.dif_y : 0.0) + (j < ny - 1 ? m.dif_y : 0.0);
+    const int bside[4] = {j == 0 ? 0 : -1, i == nx - 1 ? 1 : -1, j == ny - 1 ? 2 : -1, i == 0 ? 3 : -1};
+    for (int f = 0; f < 4; ++f) {
+        if (bside[f] < 0 || !m.tbc[f]) continue;
+        const int k = (f & 1) ? j : i;
+        const double* W = (f & 1) ? m.wv : m.wh;
+        const double coef = (f & 1) ? m.dif_bx : m.dif_by;
+        for (int q = 0; q < m.q; ++q)
+            if (m.tbc[f][size_t(k) * m.q + q] == 1) dif += W[q] * coef;
+    }
+    return (sx + sy + dif) * m.inv_area;
+}
+
+// Maximum cell rate: a grid-stride loop, then one atomic per block on the
+// ordered bit pattern of the non-negative double (one atomic per wave on a
+// single address serialised at 12 ms for 67M cells).
+__global__ __launch_bounds__(256) void rate_kernel(const MeshDev* __restrict__ mesh,
+                                                   unsigned long long* max_bits)
+{
+    __shared__ double wmax[4];
+    const MeshDev& m = *mesh;
+    const int n = m.nx * m.ny;
+    double rate = 0.0;
+    for (int c = int(blockIdx.x) * 256 + int(threadIdx.x); c < n; c += int(gridDim.x) * 256)
+        rate = fmax(rate, cell_rate(m, c));
     for (int off = 32; off >= 1; off >>= 1) rate = fmax(rate, __shfl_xor(rate, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(max_bits, (unsigned long long)__double_as_longlong(rate));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = rate;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        rate = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+        atomicMax(max_bits, (unsigned long long)__double_as_longlong(rate));
+    }
 }
 
 // EutecticModel::evaluate over arbitrary arrays.  P (optional) per element:
@@ -921,7 +935,7 @@ hipError_t launch_rhs_diag(int mode, const MeshDev* d_mesh, int ncells, const do
 hipError_t launch_rate(const MeshDev* d_mesh, int ncells, unsigned long long* max_bits,
                        hipStream_t stream)
 {
-    const dim3 grid((ncells + 255) / 256), block(256);
+    const dim3 grid(std::min((ncells + 255) / 256, 2048)), block(256);
     rate_kernel<<<grid, block, 0, stream>>>(d_mesh, max_bits);
     return hipGetLastError();
 }
