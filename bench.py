@@ -613,7 +613,7 @@ def main():
                           "region_cells_differing": region_diff, "steps": K},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "stage32_kernel (mixed sweep, stages 1-2)",
+                     "kernel": "stage32_kernel<false> (mixed sweep without diffusion terms, stages 1-2)",
                      "bytes_per_cell_launch": 0.5 * (b["stage1"] + b["stage2"]),
                      "ms_per_launch": prof["ms_stage"], "rows_per_item": mixed["rows"]},
         "conservation": {"total_H": mixed["report"]["total_H"], "total_C": mixed["report"]["total_C"]},
