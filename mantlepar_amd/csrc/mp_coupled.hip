@@ -62,9 +62,11 @@ __device__ __forceinline__ void load_cell(const CoupledDev& d, size_t c, size_t 
 // Phase of the limited reconstruction at a point, with the coupled law's
 // range check (src/couple/evolve.cpp:270-277); (tab, k): the point's
 // lithostatic entry (tab null: the constant pressure of ph).  The override is
-// formed here, in registers, not at the call site.
+// formed here, in registers, not at the call site.  Inlined: as a call, every
+// evaluation saved and restored the caller's registers through 512 B of
+// scratch per lane, which bound the face kernel (VALU busy ~10%).
 template <bool DERIV>
-__device__ __noinline__ PhaseOut point_phase(const PhaseDev& ph, const TepPair* tab, size_t k,
+__device__ __forceinline__ PhaseOut point_phase(const PhaseDev& ph, const TepPair* tab, size_t k,
                                              const CellPoly& p, double x, double y, double& H,
                                              double& C, uint32_t& err)
 {
@@ -88,7 +90,10 @@ __device__ __forceinline__ size_t sample_index(const CoupledDev& d, bool vert, s
 
 }  // namespace
 
-__global__ void coupled_cell_kernel(CoupledDev d, const double* __restrict__ H,
+// Launched with 128 threads per block; the bound lets the compiler use the
+// registers that a 1024-thread default would deny (202 VGPRs in the face
+// kernel, no scratch).
+__global__ __launch_bounds__(128) void coupled_cell_kernel(CoupledDev d, const double* __restrict__ H,
                                     const double* __restrict__ C)
 {
     const MeshDev& m = *d.m;
@@ -154,7 +159,7 @@ __global__ void coupled_cell_kernel(CoupledDev d, const double* __restrict__ H,
 // One face.  Vertical faces f < ny*(nx+1) at x_i (i = f % (nx+1), j = f / (nx+1)),
 // horizontal faces after them at y_j.  Outputs in the canonical direction
 // (+x / +y): flux_H, flux_C, mixture flux, and the rate (same for both sides).
-__global__ void coupled_face_kernel(CoupledDev d)
+__global__ __launch_bounds__(128) void coupled_face_kernel(CoupledDev d)
 {
     const MeshDev& m = *d.m;
     const int nx = m.nx, ny = m.ny, nq = m.q;
