@@ -22,7 +22,8 @@ struct mp_coupled {
     size_t N = 0;
     int64_t nxf = 0, nyf = 0;
     int npts = 0;                       // cell points
-    DeviceBuffer d_md, state, old, rbuf, first, rd, bnd, blend, cell_out, face, flowx, flowy, cmix;
+    // bnd: [5][N] per-cell boundary flux H, C, heat source, then rate and divergence
+    DeviceBuffer d_md, state, old, rbuf, first, bnd, blend, cell_out, face, flowx, flowy, cmix;
     DeviceBuffer bcH, bcC, inflow, tbc, tval, error, red, kinds, part;
     DeviceBuffer tep;                   // lithostatic {Tep, Tmp} tables
     bool have_flow = false;
@@ -96,8 +97,7 @@ mp_status mp_coupled::init(const mp_mesh_desc& m, const mp_phase_params& p,
     MP_CALL(old.alloc(2 * N * sizeof(double)));
     MP_CALL(rbuf.alloc(2 * N * sizeof(double)));
     MP_CALL(first.alloc(2 * N * sizeof(double)));
-    MP_CALL(rd.alloc(2 * N * sizeof(double)));
-    MP_CALL(bnd.alloc(3 * N * sizeof(double)));
+    MP_CALL(bnd.alloc(5 * N * sizeof(double)));
     MP_CALL(blend.alloc(20 * N * sizeof(double)));
     MP_CALL(cell_out.alloc(4 * N * sizeof(double)));
     MP_CALL(face.alloc(4 * size_t(nxf + nyf) * sizeof(double)));
@@ -106,8 +106,15 @@ mp_status mp_coupled::init(const mp_mesh_desc& m, const mp_phase_params& p,
     MP_CALL(cmix.alloc(size_t(npts) * N * sizeof(double)));
     MP_CALL(error.alloc(sizeof(uint32_t)));
     MP_CALL(red.alloc(8 * sizeof(double)));
-    MP_CALL(kinds.alloc(8 * sizeof(int)));
-    MP_CALL(part.alloc(2 * 3 * size_t(kReducePartials) * sizeof(double)));
+    // reduction kinds and array maps, fixed: the RHS totals (sums of the three
+    // boundary/source fields, max rate, max |divergence|) and the state checks
+    // (min, max and sum of H and C)
+    static const int kKinds[17] = {0, 0, 0, 3, 1,               // RHS kinds, arrays 0..4
+                                   2, 2, 3, 3, 0, 0,            // state kinds
+                                   0, 1, 0, 1, 0, 1};           // state arrays: H, C, H, C, H, C
+    MP_CALL(kinds.alloc(sizeof kKinds));
+    MP_HIP(hipMemcpyAsync(kinds.ptr, kKinds, sizeof kKinds, hipMemcpyHostToDevice, stream));
+    MP_CALL(part.alloc(2 * 6 * size_t(kReducePartials) * sizeof(double)));
     MP_HIP(hipMemsetAsync(state.ptr, 0, state.bytes, stream));
     MP_HIP(hipMemsetAsync(flowx.ptr, 0, flowx.bytes, stream));
     MP_HIP(hipMemsetAsync(flowy.ptr, 0, flowy.bytes, stream));
@@ -362,13 +369,9 @@ mp_status mp_coupled::rhs_on(const double* H, const double* C, double* rH, doubl
 {
     if (!have_flow) return fail(MP_ERR_STATE, "set the flow samples first");
     MP_HIP(hipMemsetAsync(error.ptr, 0, sizeof(uint32_t), stream));
-    double* r = rd.as<double>();
-    MP_HIP(launch_coupled_rhs(cd, mesh.nx, mesh.ny, H, C, rH, rC, r, r + N, bnd.as<double>(), stream));
-    const int k1[3] = {0, 0, 0}, k2[2] = {3, 1};
-    MP_HIP(hipMemcpyAsync(kinds.ptr, k1, sizeof k1, hipMemcpyHostToDevice, stream));
-    MP_HIP(hipMemcpyAsync(kinds.as<int>() + 3, k2, sizeof k2, hipMemcpyHostToDevice, stream));
-    MP_HIP(launch_reduce(bnd.as<double>(), int64_t(N), 3, kinds.as<int>(), red.as<double>(), part.as<double>(), stream));
-    MP_HIP(launch_reduce(r, int64_t(N), 2, kinds.as<int>() + 3, red.as<double>() + 3, part.as<double>(), stream));
+    double* b = bnd.as<double>();
+    MP_HIP(launch_coupled_rhs(cd, mesh.nx, mesh.ny, H, C, rH, rC, b + 3 * N, b + 4 * N, b, stream));
+    MP_HIP(launch_reduce(b, int64_t(N), 5, kinds.as<int>(), nullptr, red.as<double>(), part.as<double>(), stream));
     double h[5];
     uint32_t bits = 0;
     MP_HIP(hipMemcpyAsync(h, red.ptr, sizeof h, hipMemcpyDeviceToHost, stream));
@@ -393,7 +396,7 @@ mp_status mp_coupled::rhs(double* rH, double* rC, double* rate, double* div, dou
     MP_CALL(rhs_on(S, S + N, R, R + N, t));
     const size_t nb = N * sizeof(double);
     double* outs[6] = {rH, rC, rate, div, T, phi};
-    const double* srcs[6] = {R, R + N, rd.as<double>(), rd.as<double>() + N,
+    const double* srcs[6] = {R, R + N, bnd.as<double>() + 3 * N, bnd.as<double>() + 4 * N,
                              cell_out.as<double>() + N, cell_out.as<double>() + 2 * N};
     for (int k = 0; k < 6; ++k)
         if (outs[k]) MP_HIP(hipMemcpyAsync(outs[k], srcs[k], nb, hipMemcpyDeviceToHost, stream));
@@ -405,13 +408,10 @@ mp_status mp_coupled::rhs(double* rH, double* rC, double* rate, double* div, dou
 // Admissible (src/couple/evolve.cpp:386-393) and the integrals sum |K| u.
 mp_status mp_coupled::admissible(const double* S, bool& ok, double& intH, double& intC)
 {
-    const int k[6] = {2, 2, 3, 3, 0, 0};
-    MP_HIP(hipMemcpyAsync(kinds.ptr, k, sizeof k, hipMemcpyHostToDevice, stream));
     double* o = red.as<double>();
-    // min H, min C | max H, max C | sum H, sum C (H and C are contiguous)
-    MP_HIP(launch_reduce(S, int64_t(N), 2, kinds.as<int>(), o, part.as<double>(), stream));
-    MP_HIP(launch_reduce(S, int64_t(N), 2, kinds.as<int>() + 2, o + 2, part.as<double>(), stream));
-    MP_HIP(launch_reduce(S, int64_t(N), 2, kinds.as<int>() + 4, o + 4, part.as<double>(), stream));
+    // min H, min C | max H, max C | sum H, sum C (H and C are contiguous), one pass
+    MP_HIP(launch_reduce(S, int64_t(N), 6, kinds.as<int>() + 5, kinds.as<int>() + 11, o, part.as<double>(),
+                         stream));
     double h[6];
     MP_HIP(hipMemcpyAsync(h, o, sizeof h, hipMemcpyDeviceToHost, stream));
     MP_HIP(hipStreamSynchronize(stream));

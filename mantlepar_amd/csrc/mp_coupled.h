@@ -45,11 +45,13 @@ struct CoupledDev {
 hipError_t launch_coupled_rhs(const CoupledDev& d, int nx, int ny, const double* H, const double* C,
                               double* rhsH, double* rhsC, double* rate, double* div, double* bnd,
                               hipStream_t stream);
-// kind per array: 0 compensated sum, 1 max |x|, 2 min, 3 max (NaN propagates);
-// part: scratch of 2 * k * kReducePartials doubles
+// k reductions of length-n arrays: array a is x + src[a]*n (src null: x + a*n),
+// reduced by kind[a]: 0 compensated sum, 1 max |x|, 2 min, 3 max (NaN
+// propagates); kind and src in device memory; part: scratch of
+// 2 * k * kReducePartials doubles.
 constexpr int kReducePartials = 1024;
-hipError_t launch_reduce(const double* x, int64_t n, int k, const int* kind, double* out,
-                         double* part, hipStream_t stream);
+hipError_t launch_reduce(const double* x, int64_t n, int k, const int* kind, const int* src,
+                         double* out, double* part, hipStream_t stream);
 hipError_t launch_coupled_stage(int64_t n, int stage, double dt, const double* oldH,
                                 const double* oldC, const double* rH, const double* rC, double* H,
                                 double* C, hipStream_t stream);

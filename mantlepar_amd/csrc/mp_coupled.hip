@@ -404,13 +404,14 @@ __device__ void block_reduce(int mode, double& s, double& e, double* ss, double*
 }
 
 __global__ __launch_bounds__(256) void reduce_partial_kernel(const double* __restrict__ x, int64_t n,
-                                                             int k, const int* kind, double* part)
+                                                             int k, const int* kind, const int* src,
+                                                             double* part)
 {
     __shared__ double ss[256], se[256];
     const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const int64_t b0 = int64_t(blockIdx.x) * chunk, b1 = min(n, b0 + chunk);
     for (int a = 0; a < k; ++a) {
-        const double* v = x + a * n;
+        const double* v = x + (src ? src[a] : a) * n;
         const int mode = kind[a];
         double s = red_init(mode), e = 0.0;
         for (int64_t t = b0 + threadIdx.x; t < b1; t += blockDim.x) {
@@ -470,11 +471,11 @@ hipError_t launch_coupled_rhs(const CoupledDev& d, int nx, int ny, const double*
     return hipGetLastError();
 }
 
-hipError_t launch_reduce(const double* x, int64_t n, int k, const int* kind, double* out,
-                         double* part, hipStream_t stream)
+hipError_t launch_reduce(const double* x, int64_t n, int k, const int* kind, const int* src,
+                         double* out, double* part, hipStream_t stream)
 {
     const int nb = int(std::min<int64_t>(kReducePartials, std::max<int64_t>(1, (n + 4095) / 4096)));
-    reduce_partial_kernel<<<nb, 256, 0, stream>>>(x, n, k, kind, part);
+    reduce_partial_kernel<<<nb, 256, 0, stream>>>(x, n, k, kind, src, part);
     reduce_final_kernel<<<1, 256, 0, stream>>>(part, nb, k, kind, out);
     return hipGetLastError();
 }
