@@ -241,6 +241,9 @@ mp_status mp_batch_set_dt(mp_batch* b, int32_t id, double dt);
  * status is that of the first failed mesh. */
 mp_status mp_batch_step(mp_batch* b, int32_t nsteps, int32_t scheme, int32_t flags,
                         mp_step_report* reports);
+/* Lithostatic phase pressure of mesh id, as mp_transport_set_lithostatic
+ * (src/couple/initialization.cpp:155-162). */
+mp_status mp_batch_set_lithostatic(mp_batch* b, int32_t id, int32_t enable, double surface_y);
 mp_status mp_batch_phase(mp_batch* b, int32_t id, double* T, double* phi, double* Cl,
                          double* Cs, uint8_t* region);
 /* Schedule: rows per work item (>= 8) and persistent grid size in blocks of

@@ -169,6 +169,7 @@ SIGNATURES = [
     ("mp_transport_set_thermal_boundary", C.c_int32, [_vp, _u8p, _dp]),
     ("mp_transport_set_lithostatic", C.c_int32, [_vp, C.c_int32, C.c_double]),
     ("mp_coupled_set_lithostatic", C.c_int32, [_vp, C.c_int32, C.c_double]),
+    ("mp_batch_set_lithostatic", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double]),
     ("mp_batch_create", C.c_int32, [C.c_int32, C.POINTER(WenoOptions), C.POINTER(PhaseParams),
                                     C.c_int32, C.c_uint64, _vp, C.POINTER(_vp)]),
     ("mp_batch_destroy", C.c_int32, [_vp]),

@@ -92,6 +92,12 @@ class Batch:
         b = None if inflow_C is None else _f64(inflow_C)
         check(lib().mp_batch_set_inflow(self._h, k, dptr(a), dptr(b)))
 
+    def set_lithostatic(self, k, surface_y=None):
+        """Pressure model "lithostatic" for mesh k (src/couple/initialization.cpp:155-162);
+        None returns to the constant phase pressure."""
+        check(lib().mp_batch_set_lithostatic(self._h, k, int(surface_y is not None),
+                                             0.0 if surface_y is None else surface_y))
+
     def stable_dt(self, k, cfl):
         dt = C.c_double()
         check(lib().mp_batch_stable_dt(self._h, k, cfl, C.byref(dt)))

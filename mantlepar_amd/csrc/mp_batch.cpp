@@ -425,6 +425,13 @@ mp_status mp_batch_step(mp_batch* b, int32_t nsteps, int32_t scheme, int32_t fla
     MP_GUARD({ return b->step(nsteps, scheme, flags, reports); })
 }
 
+mp_status mp_batch_set_lithostatic(mp_batch* b, int32_t id, int32_t enable, double surface_y)
+{
+    if (!b) return fail(MP_ERR_ARG, "null handle");
+    MP_CALL(b->check_id(id));
+    MP_GUARD({ return b->eng.set_lithostatic(id, enable, surface_y); })
+}
+
 mp_status mp_batch_phase(mp_batch* b, int32_t id, double* T, double* phi, double* Cl, double* Cs,
                          uint8_t* region)
 {

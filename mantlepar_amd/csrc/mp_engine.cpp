@@ -147,6 +147,7 @@ size_t Engine::slot_bytes(const mp_mesh_desc& d, int rows_min)
     b += Arena::round(size_t(3) * items_for(d, rows_min) * kPartials * sizeof(double));
     b += Arena::round(size_t(2) * (d.nx + d.ny));
     b += Arena::round(size_t(4) * (d.nx + d.ny) * d.q * sizeof(double));
+    b += Arena::round(size_t(d.ny) * sizeof(TepPair));
     return b;
 }
 
@@ -178,6 +179,7 @@ mp_status Engine::add_mesh(const mp_mesh_desc& d, int* out)
     sl.partials = reinterpret_cast<double*>(take(size_t(3) * items_for(d, 8) * kPartials * sizeof(double)));
     sl.bc = reinterpret_cast<uint8_t*>(take(size_t(2) * (d.nx + d.ny)));
     sl.inflow = reinterpret_cast<double*>(take(size_t(4) * (d.nx + d.ny) * d.q * sizeof(double)));
+    sl.tep = reinterpret_cast<TepPair*>(take(size_t(d.ny) * sizeof(TepPair)));
     sl.live = true;
     slots[k] = sl;
     set_rows(k, 64);
@@ -326,6 +328,25 @@ mp_status Engine::set_inflow(int k, const double* h, const double* c)
     }
     MP_HIP(hipMemcpyAsync(d_meshes.as<MeshDev>() + k, &s.md, sizeof(MeshDev), hipMemcpyHostToDevice, stream));
     MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+// Pressure model "lithostatic" for one mesh (src/couple/initialization.cpp:155-162):
+// the phase of each cell at the static pressure of its centre.
+mp_status Engine::set_lithostatic(int k, int enable, double surface_y)
+{
+    Slot& s = slots[k];
+    if (enable) {
+        std::vector<TepPair> rows;
+        MP_CALL(lithostatic_rows(s.desc, phase, surface_y, rows));
+        MP_HIP(hipMemcpyAsync(s.tep, rows.data(), rows.size() * sizeof(TepPair), hipMemcpyHostToDevice,
+                              stream));
+    }
+    s.md.tep_row = enable ? s.tep : nullptr;
+    MP_HIP(hipMemcpyAsync(d_meshes.as<MeshDev>() + k, &s.md, sizeof(MeshDev), hipMemcpyHostToDevice,
+                          stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    s.phase_valid = false;
     return MP_OK;
 }
 

@@ -58,6 +58,7 @@ struct Slot {
     double* partials = nullptr;
     uint8_t* bc = nullptr;     // bottom(nx) right(ny) top(nx) left(ny) policies
     double* inflow = nullptr;  // H then C, each [face][q] in side order
+    TepPair* tep = nullptr;    // lithostatic {Tep, Tmp} per row (md.tep_row when enabled)
     int cur = 0, phase_cur = 0;
     bool phase_valid = false, have_velocity = false;
     double max_rate = 0, dt = 0;
@@ -83,6 +84,7 @@ public:
     mp_status set_velocity(int slot, const double* vnx, const double* vny, bool wait = true);
     mp_status set_boundary(int slot, const uint8_t* const side[4]);
     mp_status set_inflow(int slot, const double* H, const double* C);
+    mp_status set_lithostatic(int slot, int enable, double surface_y);
     mp_status phase_out(int slot, double* T, double* phi, double* Cl, double* Cs, uint8_t* region,
                         bool async = false);
     void set_rows(int slot, int rows);

@@ -62,6 +62,11 @@ mp_status make_phase(const mp_phase_params& p, PhaseHost& out);
 // 151-161, src/couple/initialization.cpp:159-161), then the boundaries' pressure
 // shift (:143-149, :166-169), each rounded once from long double.
 mp_status lithostatic_pair(const PhaseHost& h, double surface_y, double y, TepPair& out);
+// {Tep, Tmp} of each cell row at its centre (MapCellPoint of (0, 0) over the
+// uniform vertices) for the pressure model "lithostatic"
+// (src/couple/initialization.cpp:155-162); rejects a surface below the mesh top.
+mp_status lithostatic_rows(const mp_mesh_desc& mesh, const PhaseHost& h, double surface_y,
+                           std::vector<TepPair>& rows);
 // Vertex ordinates of a uniform axis a0 + k*(a1 - a0)/n with the ends snapped
 // (src/core/mesh.cpp:27-33, 86-114); v holds n + 1 values.
 void uniform_vertices(double a0, double a1, int n, double* v);

@@ -175,6 +175,21 @@ mp_status lithostatic_pair(const PhaseHost& h, double surface_y, double y, TepPa
     return MP_OK;
 }
 
+mp_status lithostatic_rows(const mp_mesh_desc& mesh, const PhaseHost& h, double surface_y,
+                           std::vector<TepPair>& rows)
+{
+    if (!std::isfinite(surface_y)) return fail(MP_ERR_ARG, "surface_y must be finite");
+    if (surface_y < mesh.y1) return fail(MP_ERR_ARG, "surface_y must be at or above the mesh top");
+    std::vector<double> y(size_t(mesh.ny) + 1);
+    uniform_vertices(mesh.y0, mesh.y1, mesh.ny, y.data());
+    rows.assign(size_t(mesh.ny), TepPair{});
+    for (int j = 0; j < mesh.ny; ++j) {
+        const double lower = map_edge(0.0, y[j], y[j]), upper = map_edge(0.0, y[j + 1], y[j + 1]);
+        MP_CALL(lithostatic_pair(h, surface_y, map_edge(0.0, lower, upper), rows[j]));
+    }
+    return MP_OK;
+}
+
 void uniform_vertices(double a0, double a1, int n, double* v)
 {
     const double step = (a1 - a0) / n;

@@ -562,17 +562,9 @@ mp_status mp_transport::set_lithostatic(int enable, double surface_y)
         tep.reset();
         md.tep_row = nullptr;
     } else {
-        // src/couple/initialization.cpp:155-162: the phase at a cell centre
-        // (MapCellPoint of (0, 0)) sees P = staticPressure(surface_y - y_c)
-        if (!std::isfinite(surface_y)) return fail(MP_ERR_ARG, "surface_y must be finite");
-        if (surface_y < mesh.y1) return fail(MP_ERR_ARG, "surface_y must be at or above the mesh top");
-        std::vector<double> y(size_t(mesh.ny) + 1);
-        uniform_vertices(mesh.y0, mesh.y1, mesh.ny, y.data());
-        std::vector<TepPair> rows(mesh.ny);
-        for (int j = 0; j < mesh.ny; ++j) {
-            const double lower = map_edge(0.0, y[j], y[j]), upper = map_edge(0.0, y[j + 1], y[j + 1]);
-            MP_CALL(lithostatic_pair(phase, surface_y, map_edge(0.0, lower, upper), rows[j]));
-        }
+        // the phase at a cell centre sees P = staticPressure(surface_y - y_c)
+        std::vector<TepPair> rows;
+        MP_CALL(lithostatic_rows(mesh, phase, surface_y, rows));
         MP_CALL(tep.alloc(rows.size() * sizeof(TepPair)));
         MP_HIP(hipMemcpyAsync(tep.ptr, rows.data(), tep.bytes, hipMemcpyHostToDevice, stream));
         md.tep_row = tep.as<TepPair>();

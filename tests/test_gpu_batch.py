@@ -40,7 +40,8 @@ def open_bc(spec):
     return (vx, vy), bc, (0.5 + 0.2 * rng.random(n), 0.2 + 0.2 * rng.random(n))
 
 
-def single_run(spec, prec, nsteps, cfl=0.4, vel=None, bc=None, inflow=None, scheme=3, rows=None):
+def single_run(spec, prec, nsteps, cfl=0.4, vel=None, bc=None, inflow=None, scheme=3, rows=None,
+               surface=None):
     """The single-mesh handle on the same inputs.  States and phase outputs do
     not depend on the tile shape; the reported boundary sums are plain sums of
     per-item partials, so they match bitwise only under the same rows per item."""
@@ -56,6 +57,8 @@ def single_run(spec, prec, nsteps, cfl=0.4, vel=None, bc=None, inflow=None, sche
         t.set_boundary(bc["bottom"], bc["right"], bc["top"], bc["left"])
     if inflow is not None:
         t.set_inflow(*inflow)
+    if surface is not None:
+        t.set_lithostatic(surface)
     dt = t.stable_dt(cfl)
     rep = t.step(nsteps, dt, scheme) if nsteps else None
     H1, C1 = t.download()
@@ -107,6 +110,41 @@ def test_batch_matches_single_mesh_bitwise(prec, graph):
         for n in ("dt", "total_H", "total_C", "boundary_H", "boundary_C"):
             assert r[n] == rr[n], (i, n, r[n], rr[n])
         assert r["steps_done"] == 5 and r["error_bits"] == 0
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [M.PREC_FP64, M.PREC_MIXED], ids=["fp64", "mixed"])
+def test_batch_lithostatic_matches_single_mesh_bitwise(prec):
+    """Lithostatic phase pressure on two meshes of a batch (different surfaces),
+    the constant pressure on the others: phase outputs and states bit for bit
+    those of single-mesh handles with the same settings."""
+    b = M.Batch(8, phase=W.config_phase(), precision=prec)
+    ids = fill_batch(b, SPECS[:4])
+    surfaces = {0: 1.3, 2: 1.0}
+    for i, sy in surfaces.items():
+        b.set_lithostatic(ids[i], sy)
+    with pytest.raises(M.MantleError, match="surface_y must be at or above the mesh top"):
+        b.set_lithostatic(ids[1], 0.5)
+    b.set_schedule(32, 0)
+    b.step(4)
+    for i, k in enumerate(ids):
+        ref = single_run(SPECS[i], prec, 4, rows=32, surface=surfaces.get(i))
+        assert_same(b, k, ref)
+    # switching off returns to the constant pressure: the phase of the current
+    # state equals a fresh handle's on the same state
+    litho = b.phase_state(ids[0])
+    b.set_lithostatic(ids[0], None)
+    spec = SPECS[0]
+    Hb, Cb = b.download(ids[0])
+    t = M.Transport(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q,
+                    phase=W.config_phase(), precision=prec)
+    t.upload(Hb, Cb)
+    flat = t.phase_state()
+    t.close()
+    bp = b.phase_state(ids[0])
+    for n in ("T", "phi", "region"):
+        assert np.array_equal(bp[n], flat[n]), n
+    assert not np.array_equal(bp["T"], litho["T"])
     b.close()
 
 
