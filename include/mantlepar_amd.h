@@ -290,6 +290,10 @@ typedef struct {
     double* Cl;
     double* Cs;
     uint8_t* region;
+    /* phase pressure model: 0 constant (mp_phase_params.pressure), 1 lithostatic below
+     * surface_y, as mp_transport_set_lithostatic */
+    int32_t lithostatic;
+    double surface_y;
 } mp_request;
 
 typedef struct {

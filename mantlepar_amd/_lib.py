@@ -63,7 +63,7 @@ class Request(C.Structure):
                 ("vny", C.c_void_p), ("bc", C.c_void_p * 4), ("cfl", C.c_double),
                 ("nsteps", C.c_int32), ("H_out", C.c_void_p), ("C_out", C.c_void_p),
                 ("T", C.c_void_p), ("phi", C.c_void_p), ("Cl", C.c_void_p), ("Cs", C.c_void_p),
-                ("region", C.c_void_p)]
+                ("region", C.c_void_p), ("lithostatic", C.c_int32), ("surface_y", C.c_double)]
 
 
 class RequestResult(C.Structure):

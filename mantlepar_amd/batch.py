@@ -177,8 +177,9 @@ class Server:
         return lib().mp_server_now(self._h)
 
     def submit(self, nx, ny, H, Cd, vnx, vny, nsteps, cfl, x0=0.0, x1=0.5, y0=0.0, y1=1.0, q=5,
-               bc=None, outputs=("H", "C", "T", "phi", "Cl", "Cs", "region")):
-        """Queue one run; returns its id.  Inputs are kept referenced until done."""
+               bc=None, outputs=("H", "C", "T", "phi", "Cl", "Cs", "region"), surface_y=None):
+        """Queue one run; returns its id.  Inputs are kept referenced until done.
+        surface_y: lithostatic phase pressure below that surface (None: constant)."""
         ins = [_f64(H), _f64(Cd), _f64(vnx), _f64(vny)]
         n = nx * ny
         assert ins[0].size == n and ins[1].size == n
@@ -193,6 +194,8 @@ class Server:
             r.bc[i] = None if a is None else a.ctypes.data
         r.cfl = cfl
         r.nsteps = nsteps
+        r.lithostatic = int(surface_y is not None)
+        r.surface_y = 0.0 if surface_y is None else surface_y
         names = {"H": "H_out", "C": "C_out", "T": "T", "phi": "phi", "Cl": "Cl", "Cs": "Cs",
                  "region": "region"}
         for k, a in outs.items():

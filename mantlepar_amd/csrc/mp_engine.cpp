@@ -337,15 +337,13 @@ mp_status Engine::set_lithostatic(int k, int enable, double surface_y)
 {
     Slot& s = slots[k];
     if (enable) {
-        std::vector<TepPair> rows;
-        MP_CALL(lithostatic_rows(s.desc, phase, surface_y, rows));
-        MP_HIP(hipMemcpyAsync(s.tep, rows.data(), rows.size() * sizeof(TepPair), hipMemcpyHostToDevice,
-                              stream));
+        MP_CALL(lithostatic_rows(s.desc, phase, surface_y, s.tep_host));
+        MP_HIP(hipMemcpyAsync(s.tep, s.tep_host.data(), s.tep_host.size() * sizeof(TepPair),
+                              hipMemcpyHostToDevice, stream));
     }
     s.md.tep_row = enable ? s.tep : nullptr;
     MP_HIP(hipMemcpyAsync(d_meshes.as<MeshDev>() + k, &s.md, sizeof(MeshDev), hipMemcpyHostToDevice,
                           stream));
-    MP_HIP(hipStreamSynchronize(stream));
     s.phase_valid = false;
     return MP_OK;
 }

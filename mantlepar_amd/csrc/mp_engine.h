@@ -59,6 +59,7 @@ struct Slot {
     uint8_t* bc = nullptr;     // bottom(nx) right(ny) top(nx) left(ny) policies
     double* inflow = nullptr;  // H then C, each [face][q] in side order
     TepPair* tep = nullptr;    // lithostatic {Tep, Tmp} per row (md.tep_row when enabled)
+    std::vector<TepPair> tep_host;   // its host source, kept until the slot is reused
     int cur = 0, phase_cur = 0;
     bool phase_valid = false, have_velocity = false;
     double max_rate = 0, dt = 0;

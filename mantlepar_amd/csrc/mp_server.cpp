@@ -125,6 +125,7 @@ void mp_server::admit(std::vector<Request*>& batch)
         }
         if (st == MP_OK) st = eng.upload_async(k, q->r.H, q->r.C);
         if (st == MP_OK) st = eng.set_velocity(k, q->r.vnx, q->r.vny, false);
+        if (st == MP_OK && q->r.lithostatic) st = eng.set_lithostatic(k, 1, q->r.surface_y);
         if (st != MP_OK) {
             finish(q, st, last_error());
             continue;

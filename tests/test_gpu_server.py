@@ -34,7 +34,7 @@ def make_requests(count, seed=21, max_steps=24):
 def submit(srv, r, cfl=0.4):
     s = r["spec"]
     return srv.submit(s.nx, s.ny, r["H"], r["C"], r["vx"], r["vy"], r["nsteps"], cfl,
-                      s.x0, s.x1, s.y0, s.y1, q=s.q, bc=r.get("bc"))
+                      s.x0, s.x1, s.y0, s.y1, q=s.q, bc=r.get("bc"), surface_y=r.get("surface"))
 
 
 OUT = ("H", "C", "T", "phi", "Cl", "Cs", "region")
@@ -88,6 +88,42 @@ def test_alone_equals_mixed(prec):
             assert np.array_equal(ph[n], oa[n]), n
         assert rep["total_H"] == ra["total_H"] and rep["boundary_H"] == ra["boundary_H"]
         t.close()
+
+
+@pytest.mark.parametrize("prec", [M.PREC_FP64, M.PREC_MIXED], ids=["fp64", "mixed"])
+def test_lithostatic_requests_equal_single_mesh(prec):
+    """Requests with the lithostatic pressure model, mixed with constant-pressure
+    ones, give the single-mesh handle's outputs bit for bit; a surface below the
+    mesh top fails that request alone."""
+    reqs = make_requests(8, seed=41, max_steps=10)
+    for i, r in enumerate(reqs):
+        if i % 2 == 0:
+            r["surface"] = r["spec"].y1 + 0.1 * (i + 1)
+    bad = dict(reqs[1], surface=0.5 * reqs[1]["spec"].y1)
+    srv = M.Server(max_slots=4, phase=W.config_phase(), precision=prec, max_epoch_steps=4)
+    ids = [submit(srv, r) for r in reqs]
+    bad_id = submit(srv, bad)
+    srv.drain()
+    with pytest.raises(M.MantleError, match="surface_y must be at or above the mesh top"):
+        srv.wait(bad_id)
+    for r, rid in zip(reqs, ids):
+        res, out = srv.wait(rid)
+        s = r["spec"]
+        t = M.Transport(s.nx, s.ny, s.x0, s.x1, s.y0, s.y1, q=s.q, phase=W.config_phase(),
+                        precision=prec)
+        t.set_tile(8)
+        t.upload(r["H"], r["C"])
+        t.set_face_velocity(r["vx"], r["vy"])
+        if "surface" in r:
+            t.set_lithostatic(r["surface"])
+        t.step(r["nsteps"], t.stable_dt(0.4))
+        H, Cc = t.download()
+        ph = t.phase_state()
+        assert np.array_equal(H, out["H"]) and np.array_equal(Cc, out["C"])
+        for n in ("T", "phi", "Cl", "Cs", "region"):
+            assert np.array_equal(ph[n], out[n]), n
+        t.close()
+    srv.close()
 
 
 def test_budget_limits_residency():
