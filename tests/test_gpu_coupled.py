@@ -191,3 +191,97 @@ def test_lithostatic_eutectic_layer_closed_form():
     assert np.abs(got["rhsH"] - (want + cool)).max() <= 1e-11 * np.abs(want).max()
     assert np.abs(got["rhsC"]).max() <= 1e-13
     g.close()
+
+
+# --------------------------------------------- options of the coupled RHS
+def options_pair(spec, hyper=0, lf_mode=0, global_speed=0.0, bcC=None, state=None, inflow_C=0.2):
+    """pair() with the hypereutectic mirror, the global LF speed and separate
+    composition policies; the same settings on both sides."""
+    flow = W.coupled_flow(spec)
+    bc, inflow, ttypes, tvals = W.coupled_boundary(spec, C_in=inflow_C)
+    H, Cc = W.coupled_state(spec) if state is None else state
+    c_max = 1.0 if hyper else XE
+    g = M.Coupled(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q,
+                  phase=W.config_phase(hypereutectic=hyper),
+                  params=M.default_coupled(theta=THETA, cooling=COOLING, thermal_diffusivity=KAPPA,
+                                           lf_mode=lf_mode, global_speed=global_speed))
+    g.set_flow(flow)
+    g.set_boundary(0, bc)
+    g.set_boundary(1, bc if bcC is None else bcC)
+    g.set_inflow(*inflow)
+    g.set_thermal_boundary(ttypes, tvals)
+    g.upload(H, Cc)
+    vx = np.zeros((spec.q, spec.ny, spec.nx + 1))
+    vy = np.zeros((spec.q, spec.ny + 1, spec.nx))
+    o = O.Transport(W.vertex_coords(spec), spec.nx, spec.ny, vx, vy, q=spec.q, bc=bc,
+                    inflow_H=inflow[0], inflow_C=inflow[1])
+    if bcC is not None:
+        o.set_composition_boundary(bcC)
+    o.set_limiter(True, 5, 4, 0.9, c_max)
+    o.set_diffusion(KAPPA, 4, 0.9, types={"bottom": np.ones(spec.nx)}, values=tvals)
+    pm = O.PhaseModel(O.Material.defaults(Tm0=1600.0, Te0=600.0, cp=1200.0, L=1.2e6, Xe=XE))
+    cp = O.CoupledParams.make(LD=pm.derived["LD"], theta=THETA, cooling=COOLING, kappa=KAPPA,
+                              lf_mode=lf_mode, global_speed=global_speed, hypereutectic=hyper,
+                              admissible_c_max=c_max)
+    return g, o, pm, cp, flow, H, Cc
+
+
+def check_rhs(g, o, pm, cp, flow, H, Cc, spec):
+    """Fields within 1e-12 of their maximum; totals within 1e-12 relative, or
+    within 1e-15 of the field's sum |K| |rhs| where the total is a cancelling
+    sum of terms of both signs (the mirror case's cooling source sums to 1e-9
+    from terms near 1e-5)."""
+    got = g.rhs()
+    want = o.coupled_rhs(pm, cp, flow, H, Cc)
+    for k in ("rhsH", "rhsC", "rate", "div", "T", "phi"):
+        assert rel(got[k], want[k]) <= 1e-12, (k, rel(got[k], want[k]))
+    area = spec.dx * spec.dy
+    for a, b, f in (("boundary_H", "boundaryH", "rhsH"), ("boundary_C", "boundaryC", "rhsC"),
+                    ("source_H", "sourceH", "rhsH"), ("maximum_rate", "maximumRate", "rate")):
+        tol = 1e-12 * abs(want[b]) + 1e-15 * area * np.abs(want[f]).sum()
+        assert abs(got[a] - want[b]) <= tol, (a, got[a], want[b])
+    return got, want
+
+
+def test_global_lf_and_separate_composition_policies():
+    """Global Lax-Friedrichs speed (alpha fixed above every |v.n|) and a
+    composition that may not enter through the bottom (zero-flux C policy there,
+    inflow for H)."""
+    spec = W.MeshSpec(24, 20, seed=2)
+    bcC = {"bottom": np.zeros(spec.nx, np.uint8), "right": np.zeros(spec.ny, np.uint8),
+           "top": np.ones(spec.nx, np.uint8), "left": np.zeros(spec.ny, np.uint8)}
+    g, o, pm, cp, flow, H, Cc = options_pair(spec, lf_mode=1, global_speed=2.0, bcC=bcC)
+    got, want = check_rhs(g, o, pm, cp, flow, H, Cc, spec)
+    # the zero-flux top changes C's boundary flux against the shared policies
+    g2, o2, pm2, cp2, flow2, _, _ = options_pair(spec, lf_mode=1, global_speed=2.0)
+    assert abs(g2.rhs()["boundary_C"] - got["boundary_C"]) > 1e-6 * abs(got["boundary_C"])
+    tg, sg = g.run(3, cfl=0.5, initial=1e-3, maximum=1.0)
+    Ho, Co, to, so = o.coupled_run(pm, cp, flow, H, Cc, 3, cfl=0.5, initial=1e-3, maximum=1.0)
+    Hg, Cg = g.download()
+    assert rel(Hg, Ho) <= 1e-11 and rel(Cg, Co) <= 1e-11
+    g.close()
+    g2.close()
+    # a global speed below a flow speed is rejected
+    g3, *_ = options_pair(spec, lf_mode=1, global_speed=1e-4)
+    with pytest.raises(M.MantleError):
+        g3.rhs()
+    g3.close()
+
+
+def test_hypereutectic_mirror():
+    """C on both sides of the eutectic (0.2-0.6 about Xe = 0.4), evaluated on the
+    mirrored diagram above Xe; admissible up to C = 1."""
+    spec = W.MeshSpec(20, 18, seed=3)
+    H, _ = W.coupled_state(spec)
+    xc = (np.arange(spec.nx) + 0.5) / spec.nx
+    yc = (np.arange(spec.ny) + 0.5) / spec.ny
+    Cc = 0.2 + 0.4 * (0.5 + 0.5 * np.outer(np.cos(np.pi * yc), np.sin(2 * np.pi * xc)))
+    g, o, pm, cp, flow, H, Cc = options_pair(spec, hyper=1, state=(H, Cc), inflow_C=0.5)
+    assert Cc.max() > XE and Cc.min() < XE
+    check_rhs(g, o, pm, cp, flow, H, Cc, spec)
+    tg, sg = g.run(3, cfl=0.5, initial=1e-3, maximum=1.0)
+    Ho, Co, to, so = o.coupled_run(pm, cp, flow, H, Cc, 3, cfl=0.5, initial=1e-3, maximum=1.0)
+    assert [s["retries"] for s in sg] == [s["retries"] for s in so]
+    Hg, Cg = g.download()
+    assert rel(Hg, Ho) <= 1e-11 and rel(Cg, Co) <= 1e-11
+    g.close()
