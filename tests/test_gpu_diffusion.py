@@ -44,7 +44,7 @@ def thermal(spec):
     return types, flat_t, flat_v
 
 
-def build(spec, prec=M.PREC_FP64, kappa=None, with_bc=True):
+def build(spec, prec=M.PREC_FP64, kappa=None, with_bc=True, samples=4, extent=0.9):
     H, Cc = W.enthalpy(spec), W.composition(spec)
     vx, vy = velocities(spec)
     bc, inflow = W.throughflow_boundary(spec)
@@ -62,8 +62,8 @@ def build(spec, prec=M.PREC_FP64, kappa=None, with_bc=True):
         kappa = 0.5 * adv * 0.6 * spec.dx ** 2 * 12 / 13
     types, ft, fv = thermal(spec)
     if kappa > 0:
-        t.set_diffusion(kappa, 4, 0.9)
-        o.set_diffusion(kappa, 4, 0.9, types if with_bc else None, fv if with_bc else None)
+        t.set_diffusion(kappa, samples, extent)
+        o.set_diffusion(kappa, samples, extent, types if with_bc else None, fv if with_bc else None)
         if with_bc:
             t.set_thermal_boundary(ft, fv)
     return t, o, H, Cc, kappa
@@ -73,9 +73,11 @@ def rel(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
-def test_rhs_and_rate_match_oracle():
-    spec = W.MeshSpec(40, 48, seed=4)
-    t, o, H, Cc, kappa = build(spec)
+@pytest.mark.parametrize("q,samples,extent", [(5, 4, 0.9), (3, 6, 0.75), (8, 2, 1.0)],
+                         ids=["q5-4samples", "q3-6samples", "q8-2samples"])
+def test_rhs_and_rate_match_oracle(q, samples, extent):
+    spec = W.MeshSpec(40, 48, seed=4, q=q)
+    t, o, H, Cc, kappa = build(spec, samples=samples, extent=extent)
     gH, gC, gbH, gbC = t.rhs()
     oH, oC, obH, obC = o.rhs(H, Cc)
     assert rel(gH, oH) <= 1e-12 and rel(gC, oC) <= 1e-12

@@ -54,17 +54,17 @@ def interior_fields(spec, seed=7):
     return H, Cc
 
 
-def make(spec, H, Cc, limiter=True):
+def make(spec, H, Cc, limiter=True, points=5, samples=4, extent=0.9):
     vx, vy = W.face_velocity(spec)
     t = M.Transport(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q,
                     phase=phase(), precision=M.PREC_FP64)
     t.upload(H, Cc)
     t.set_face_velocity(vx, vy)
     if limiter:
-        t.set_limiter(True, 5, 4, 0.9)
+        t.set_limiter(True, points, samples, extent)
     o = O.Transport(W.vertex_coords(spec), spec.nx, spec.ny, vx, vy, q=spec.q)
     if limiter:
-        o.set_limiter(True, 5, 4, 0.9, XE)
+        o.set_limiter(True, points, samples, extent, XE)
     return t, o
 
 
@@ -85,10 +85,14 @@ def test_factors_and_traces_match_oracle():
     t.close()
 
 
-def test_limited_steps_match_oracle():
-    spec = W.MeshSpec(48, 40, seed=3)
+@pytest.mark.parametrize("q,points,samples,extent", [(5, 5, 4, 0.9), (3, 3, 6, 0.75)],
+                         ids=["q5-5pts-4samples", "q3-3pts-6samples"])
+def test_limited_steps_match_oracle(q, points, samples, extent):
+    """Ten limited SSP-RK3 steps; the second case runs the sweep's generic edge-rule
+    instance with other cell-point and normal-line sample tables."""
+    spec = W.MeshSpec(48, 40, seed=3, q=q)
     H, Cc = interior_fields(spec)
-    t, o = make(spec, H, Cc)
+    t, o = make(spec, H, Cc, points=points, samples=samples, extent=extent)
     dt = t.stable_dt(0.4)
     for chunk in range(2):
         t.step(5, dt)
