@@ -28,11 +28,20 @@ struct mp_coupled {
     DeviceBuffer tep;                   // lithostatic {Tep, Tmp} tables
     bool have_flow = false;
     double c_max = 0;                   // admissible C bound
+    // Read-back of one step attempt, in pinned host memory: the totals in red
+    // (offsets below) and the error words of the RHS evaluations (slots).
+    double* host_red = nullptr;         // [kRedSize]
+    uint32_t* host_err = nullptr;       // [kErrSlots]
+    static constexpr int kRedRhs0 = 0, kRedAdm0 = 8, kRedAdm1 = 16, kRedRhs1 = 24, kRedAdm2 = 32,
+                         kRedRhs2 = 40, kRedSize = 48;
+    static constexpr int kErrSlots = 3;
 
     ~mp_coupled()
     {
         if (stream) (void)hipStreamSynchronize(stream);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
+        if (host_red) (void)hipHostFree(host_red);
+        if (host_err) (void)hipHostFree(host_err);
     }
 
     mp_status init(const mp_mesh_desc& m, const mp_phase_params& p, const mp_coupled_params& cp,
@@ -44,11 +53,16 @@ struct mp_coupled {
     mp_status set_inflow(const double* h, const double* c);
     mp_status set_thermal(const uint8_t* types, const double* values);
     mp_status set_lithostatic(int enable, double surface_y);
+    mp_status enqueue_rhs(const double* H, const double* C, double* rH, double* rC, int slot,
+                          int off);
+    mp_status enqueue_admissible(const double* S, int off);
+    mp_status read_back();
+    mp_status rhs_result(int slot, int off, mp_coupled_rates& tot) const;
+    void admissible_result(int off, bool& ok, double& intH, double& intC) const;
     mp_status rhs_on(const double* H, const double* C, double* rH, double* rC,
                      mp_coupled_rates& tot);
     mp_status rhs(double* rH, double* rC, double* rate, double* div, double* T, double* phi,
                   mp_coupled_rates* tot);
-    mp_status admissible(const double* S, bool& ok, double& intH, double& intC);
     mp_status run(const mp_coupled_time& tc, double* time, int nsteps, mp_coupled_step* steps,
                   int32_t* accepted);
 };
@@ -104,8 +118,10 @@ mp_status mp_coupled::init(const mp_mesh_desc& m, const mp_phase_params& p,
     MP_CALL(flowx.alloc(3 * size_t(m.q) * nxf * sizeof(double)));
     MP_CALL(flowy.alloc(3 * size_t(m.q) * nyf * sizeof(double)));
     MP_CALL(cmix.alloc(size_t(npts) * N * sizeof(double)));
-    MP_CALL(error.alloc(sizeof(uint32_t)));
-    MP_CALL(red.alloc(8 * sizeof(double)));
+    MP_CALL(error.alloc(kErrSlots * sizeof(uint32_t)));
+    MP_CALL(red.alloc(kRedSize * sizeof(double)));
+    MP_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_red), kRedSize * sizeof(double), hipHostMallocDefault));
+    MP_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_err), kErrSlots * sizeof(uint32_t), hipHostMallocDefault));
     // reduction kinds and array maps, fixed: the RHS totals (sums of the three
     // boundary/source fields, max rate, max |divergence|) and the state checks
     // (min, max and sum of H and C)
@@ -362,21 +378,46 @@ mp_status mp_coupled::set_lithostatic(int enable, double surface_y)
     return MP_OK;
 }
 
-// One RHS of the state (H, C) into (rH, rC) and the totals; fails with the
-// device's error bits.
-mp_status mp_coupled::rhs_on(const double* H, const double* C, double* rH, double* rC,
-                             mp_coupled_rates& tot)
+// One RHS of the state (H, C) into (rH, rC), its error bits into error word
+// `slot` and its totals (5 doubles) into red + off; nothing is waited for.
+mp_status mp_coupled::enqueue_rhs(const double* H, const double* C, double* rH, double* rC,
+                                  int slot, int off)
 {
     if (!have_flow) return fail(MP_ERR_STATE, "set the flow samples first");
-    MP_HIP(hipMemsetAsync(error.ptr, 0, sizeof(uint32_t), stream));
+    MP_HIP(hipMemsetAsync(error.as<uint32_t>() + slot, 0, sizeof(uint32_t), stream));
+    CoupledDev d = cd;
+    d.error = error.as<uint32_t>() + slot;
     double* b = bnd.as<double>();
-    MP_HIP(launch_coupled_rhs(cd, mesh.nx, mesh.ny, H, C, rH, rC, b + 3 * N, b + 4 * N, b, stream));
-    MP_HIP(launch_reduce(b, int64_t(N), 5, kinds.as<int>(), nullptr, red.as<double>(), part.as<double>(), stream));
-    double h[5];
-    uint32_t bits = 0;
-    MP_HIP(hipMemcpyAsync(h, red.ptr, sizeof h, hipMemcpyDeviceToHost, stream));
-    MP_HIP(hipMemcpyAsync(&bits, error.ptr, sizeof bits, hipMemcpyDeviceToHost, stream));
+    MP_HIP(launch_coupled_rhs(d, mesh.nx, mesh.ny, H, C, rH, rC, b + 3 * N, b + 4 * N, b, stream));
+    MP_HIP(launch_reduce(b, int64_t(N), 5, kinds.as<int>(), nullptr, red.as<double>() + off,
+                         part.as<double>(), stream));
+    return MP_OK;
+}
+
+// The state checks of Admissible (src/couple/evolve.cpp:386-393) and the
+// integrals sum |K| u: min H, min C | max H, max C | sum H, sum C (H and C
+// are contiguous) in one pass, into red + off.
+mp_status mp_coupled::enqueue_admissible(const double* S, int off)
+{
+    MP_HIP(launch_reduce(S, int64_t(N), 6, kinds.as<int>() + 5, kinds.as<int>() + 11,
+                         red.as<double>() + off, part.as<double>(), stream));
+    return MP_OK;
+}
+
+// Copies every total and error word to the host and waits for the stream.
+mp_status mp_coupled::read_back()
+{
+    MP_HIP(hipMemcpyAsync(host_red, red.ptr, kRedSize * sizeof(double), hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipMemcpyAsync(host_err, error.ptr, kErrSlots * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+// Totals of an RHS after read_back(); fails with its device error bits.
+mp_status mp_coupled::rhs_result(int slot, int off, mp_coupled_rates& tot) const
+{
+    const uint32_t bits = host_err[slot];
+    const double* h = host_red + off;
     if (bits) return fail(status_from_bits(bits), "%s (device error bits 0x%x)", describe_bits(bits), bits);
     if (!std::isfinite(h[3])) return fail(MP_ERR_FP, "Nonfinite coupled stage rate");
     tot.boundary_H = h[0];
@@ -385,6 +426,26 @@ mp_status mp_coupled::rhs_on(const double* H, const double* C, double* rH, doubl
     tot.maximum_rate = h[3];
     tot.divergence = h[4];
     return MP_OK;
+}
+
+// Admissible and the integrals after read_back().
+void mp_coupled::admissible_result(int off, bool& ok, double& intH, double& intC) const
+{
+    const double* h = host_red + off;
+    ok = std::isfinite(h[0]) && std::isfinite(h[2]) && std::isfinite(h[1]) && std::isfinite(h[3]) &&
+         h[0] >= 0 && h[1] >= 0 && h[3] <= c_max;
+    intH = h[4] * md.area;
+    intC = h[5] * md.area;
+}
+
+// One RHS of the state (H, C) into (rH, rC) and the totals; fails with the
+// device's error bits.
+mp_status mp_coupled::rhs_on(const double* H, const double* C, double* rH, double* rC,
+                             mp_coupled_rates& tot)
+{
+    MP_CALL(enqueue_rhs(H, C, rH, rC, 0, kRedRhs0));
+    MP_CALL(read_back());
+    return rhs_result(0, kRedRhs0, tot);
 }
 
 mp_status mp_coupled::rhs(double* rH, double* rC, double* rate, double* div, double* T, double* phi,
@@ -405,23 +466,15 @@ mp_status mp_coupled::rhs(double* rH, double* rC, double* rate, double* div, dou
     return MP_OK;
 }
 
-// Admissible (src/couple/evolve.cpp:386-393) and the integrals sum |K| u.
-mp_status mp_coupled::admissible(const double* S, bool& ok, double& intH, double& intC)
-{
-    double* o = red.as<double>();
-    // min H, min C | max H, max C | sum H, sum C (H and C are contiguous), one pass
-    MP_HIP(launch_reduce(S, int64_t(N), 6, kinds.as<int>() + 5, kinds.as<int>() + 11, o, part.as<double>(),
-                         stream));
-    double h[6];
-    MP_HIP(hipMemcpyAsync(h, o, sizeof h, hipMemcpyDeviceToHost, stream));
-    MP_HIP(hipStreamSynchronize(stream));
-    ok = std::isfinite(h[0]) && std::isfinite(h[2]) && std::isfinite(h[1]) && std::isfinite(h[3]) &&
-         h[0] >= 0 && h[1] >= 0 && h[3] <= c_max;
-    intH = h[4] * md.area;
-    intC = h[5] * md.area;
-    return MP_OK;
-}
-
+// The step loop of src/couple/evolve.cpp:428-511.  One attempt at a step size
+// is enqueued whole -- stage 1, its state check, the second RHS, stage 2, its
+// state check and the RHS of the new state -- and read back with one wait; the
+// host then takes the reference's decisions in the reference's order and
+// ignores whatever the attempt computed past the first failed check (a
+// rejected attempt's later results, error bits included, are discarded, and a
+// retry restarts from the saved state and first RHS).  The state checks of an
+// accepted step serve as the next step's starting integrals, which the
+// reference recomputes from the same state.
 mp_status mp_coupled::run(const mp_coupled_time& tc, double* time, int nsteps,
                           mp_coupled_step* steps, int32_t* accepted_out)
 {
@@ -433,8 +486,14 @@ mp_status mp_coupled::run(const mp_coupled_time& tc, double* time, int nsteps,
     double* R = rbuf.as<double>();
     double* F = first.as<double>();
     mp_coupled_rates current{}, second{};
-    MP_CALL(rhs_on(S, S + N, R, R + N, current));
+    MP_CALL(enqueue_rhs(S, S + N, R, R + N, 0, kRedRhs0));
+    MP_CALL(enqueue_admissible(S, kRedAdm0));
     MP_HIP(hipMemcpyAsync(O, S, nb, hipMemcpyDeviceToDevice, stream));
+    MP_CALL(read_back());
+    MP_CALL(rhs_result(0, kRedRhs0, current));
+    bool ok;
+    double oldH, oldC;
+    admissible_result(kRedAdm0, ok, oldH, oldC);
     int accepted = 0;
     mp_status st = MP_OK;
     for (int s = 0; s < nsteps; ++s) {
@@ -443,9 +502,6 @@ mp_status mp_coupled::run(const mp_coupled_time& tc, double* time, int nsteps,
             *time = tc.end_time;
             break;
         }
-        bool ok;
-        double oldH, oldC;
-        MP_CALL(admissible(S, ok, oldH, oldC));
         const double start = *time;
         double dt = std::fmin(remaining, tc.fixed ? tc.initial_step
                                                   : (accepted == 0 ? tc.initial_step : tc.maximum_step));
@@ -456,26 +512,29 @@ mp_status mp_coupled::run(const mp_coupled_time& tc, double* time, int nsteps,
         MP_HIP(hipMemcpyAsync(F, R, nb, hipMemcpyDeviceToDevice, stream));
         const mp_coupled_rates firstRates = current;
         int retries = 0;
-        double courant = 0;
+        double courant = 0, newH = 0, newC = 0;
         for (;;) {
             if (!(dt >= std::fmin(remaining, tc.minimum_step) && retries < 40)) {
                 st = fail(MP_ERR_PHASE, "Coupled step cannot satisfy admissibility/CFL above minimum step");
                 break;
             }
             MP_HIP(launch_coupled_stage(int64_t(N), 1, dt, O, O + N, F, F + N, S, S + N, stream));
+            MP_CALL(enqueue_admissible(S, kRedAdm1));
+            MP_CALL(enqueue_rhs(S, S + N, R, R + N, 1, kRedRhs1));
+            MP_HIP(launch_coupled_stage(int64_t(N), 2, dt, O, O + N, R, R + N, S, S + N, stream));
+            MP_CALL(enqueue_admissible(S, kRedAdm2));
+            MP_CALL(enqueue_rhs(S, S + N, R, R + N, 2, kRedRhs2));
+            MP_CALL(read_back());
             *time = start + dt;
             double ih, ic;
-            MP_CALL(admissible(S, ok, ih, ic));
+            admissible_result(kRedAdm1, ok, ih, ic);
             if (ok) {
-                st = rhs_on(S, S + N, R, R + N, second);
+                st = rhs_result(1, kRedRhs1, second);
                 if (st) break;
                 courant = dt * std::fmax(current.maximum_rate, second.maximum_rate);
                 ok = courant <= tc.cfl * (1 + 1e-12);
             }
-            if (ok) {
-                MP_HIP(launch_coupled_stage(int64_t(N), 2, dt, O, O + N, R, R + N, S, S + N, stream));
-                MP_CALL(admissible(S, ok, ih, ic));
-            }
+            if (ok) admissible_result(kRedAdm2, ok, newH, newC);
             if (ok) break;
             if (tc.fixed) {
                 st = fail(MP_ERR_ARG, "Fixed coupled step failed stage admissibility/CFL");
@@ -486,15 +545,12 @@ mp_status mp_coupled::run(const mp_coupled_time& tc, double* time, int nsteps,
         }
         double balH = 0, balC = 0;
         if (!st) {
-            bool dummy;
-            double Hi, Ci;
-            MP_CALL(admissible(S, dummy, Hi, Ci));
-            balH = Hi - oldH + .5 * dt * (current.boundary_H + second.boundary_H - current.source_H - second.source_H);
-            balC = Ci - oldC + .5 * dt * (current.boundary_C + second.boundary_C);
+            balH = newH - oldH + .5 * dt * (current.boundary_H + second.boundary_H - current.source_H - second.source_H);
+            balC = newC - oldC + .5 * dt * (current.boundary_C + second.boundary_C);
             if (!(std::fabs(balH) < 1e-10 * std::fmax(1.0, std::fabs(oldH)) && std::fabs(balC) < 1e-10))
                 st = fail(MP_ERR_PHASE, "Coupled finite-volume conservation balance failed");
         }
-        if (!st) st = rhs_on(S, S + N, R, R + N, current);
+        if (!st) st = rhs_result(2, kRedRhs2, current);
         if (st) {
             // roll back to the latest accepted state (src/couple/evolve.cpp:493-499)
             *time = start;
@@ -503,6 +559,8 @@ mp_status mp_coupled::run(const mp_coupled_time& tc, double* time, int nsteps,
             break;
         }
         ++accepted;
+        oldH = newH;
+        oldC = newC;
         MP_HIP(hipMemcpyAsync(O, S, nb, hipMemcpyDeviceToDevice, stream));
         if (steps) {
             mp_coupled_step& r = steps[s];

@@ -90,6 +90,33 @@ def test_step_loop_matches_oracle_with_retries():
     g.close()
 
 
+def test_failed_fixed_step_rolls_back():
+    """A fixed step at the CFL bound that the retry workload cannot take: both
+    sides fail with the reference's message, and the device state is the
+    state before the step, although the attempt ran both stages."""
+    spec = W.MeshSpec(16, 14, seed=2)
+    _, Cc = W.coupled_state(spec)
+    H = np.full_like(Cc, 0.599)
+    g, o, pm, cp, flow, H, Cc = pair(spec, darcy=0.5, H_in=0.9, kappa=0.0, state=(H, Cc))
+    _, _, _, so = o.coupled_run(pm, cp, flow, H, Cc, 1, cfl=0.5, initial=1.0, maximum=1.0)
+    assert so[0]["retries"] >= 1          # the step at the bound is rejected
+    dt = 0.5 / g.rhs()["maximum_rate"]
+    msg = "Fixed coupled step failed stage admissibility/CFL"
+    with pytest.raises(O.OracleError, match=msg):
+        o.coupled_run(pm, cp, flow, H, Cc, 1, cfl=0.5, initial=dt, maximum=dt, fixed=True)
+    with pytest.raises(M.MantleError, match=msg):
+        g.run(1, cfl=0.5, initial=dt, maximum=dt, fixed=True)
+    Hg, Cg = g.download()
+    assert np.array_equal(Hg, H) and np.array_equal(Cg, Cc)
+    # the handle goes on from the kept state as a fresh one would
+    tg, sg = g.run(3, cfl=0.5, initial=1.0, maximum=1.0)
+    Ho, Co, to, so = o.coupled_run(pm, cp, flow, H, Cc, 3, cfl=0.5, initial=1.0, maximum=1.0)
+    assert [s["retries"] for s in sg] == [s["retries"] for s in so]
+    Hg, Cg = g.download()
+    assert rel(Hg, Ho) <= 1e-11 and rel(Cg, Co) <= 1e-11
+    g.close()
+
+
 def test_general_run_matches_oracle():
     spec = W.MeshSpec(24, 20, seed=2)
     g, o, pm, cp, flow, H, Cc = pair(spec)
