@@ -69,6 +69,18 @@ struct __attribute__((aligned(16))) FaceMom {
     float p1, p2, m1, m2;
 };
 
+// The same moments of a face whose LF weights are one-sided, in 16 bytes.
+// With the local LF speed alpha_q = |v_q|, (v_q + alpha_q)/2 = max(v_q, 0) and
+// (v_q - alpha_q)/2 = min(v_q, 0), so where no point of the face changes sign
+// one side's moments are exactly +0: v >= 0 holds (vp, p1, p2) = (v, a, b),
+// v < 0 holds (vm, m1, m2) = (v, a, b), and the other three are +0.  A face
+// whose points change sign, and every face under the global LF speed, has
+// b = NaN: its moments are the FaceMom record at the same face index.
+struct __attribute__((aligned(16))) FaceRec {
+    double v;
+    float a, b;
+};
+
 // One uniform rectangular mesh.  All reconstruction tables are the closed
 // forms of the reference's moment / smoothness construction on rectangles,
 // in coordinates normalised by h = sqrt(dx*dy) about the target centre.
@@ -123,9 +135,12 @@ struct MeshDev {
     // face velocities [q][ny][nx+1] and [q][ny+1][nx]
     const double* vnx;
     const double* vny;
-    // Per-face flux moments (mixed path): [ny][nx+1] and [ny+1][nx].
+    // Per-face flux moments (mixed path): [ny][nx+1] and [ny+1][nx], full
+    // records and their 16-byte one-sided form.
     const FaceMom* fx;
     const FaceMom* fy;
+    const FaceRec* rx;
+    const FaceRec* ry;
 };
 
 // Per launch, per mesh: the stage u_out = a*u_n + b*(u_k + dt*L(u_k)).

@@ -143,6 +143,7 @@ size_t Engine::slot_bytes(const mp_mesh_desc& d, int rows_min)
     b += Arena::round(size_t(d.q) * faces_x(d) * sizeof(double));
     b += Arena::round(size_t(d.q) * faces_y(d) * sizeof(double));
     b += Arena::round(faces_x(d) * sizeof(FaceMom)) + Arena::round(faces_y(d) * sizeof(FaceMom));
+    b += Arena::round(faces_x(d) * sizeof(FaceRec)) + Arena::round(faces_y(d) * sizeof(FaceRec));
     b += 2 * Arena::round(N * (4 * sizeof(double) + 1));
     b += Arena::round(size_t(3) * items_for(d, rows_min) * kPartials * sizeof(double));
     b += Arena::round(size_t(2) * (d.nx + d.ny));
@@ -174,6 +175,8 @@ mp_status Engine::add_mesh(const mp_mesh_desc& d, int* out)
     sl.md.vny = reinterpret_cast<double*>(take(size_t(d.q) * faces_y(d) * sizeof(double)));
     sl.md.fx = reinterpret_cast<FaceMom*>(take(faces_x(d) * sizeof(FaceMom)));
     sl.md.fy = reinterpret_cast<FaceMom*>(take(faces_y(d) * sizeof(FaceMom)));
+    sl.md.rx = reinterpret_cast<FaceRec*>(take(faces_x(d) * sizeof(FaceRec)));
+    sl.md.ry = reinterpret_cast<FaceRec*>(take(faces_y(d) * sizeof(FaceRec)));
     sl.phase[0] = reinterpret_cast<double*>(take(N * (4 * sizeof(double) + 1)));
     sl.phase[1] = reinterpret_cast<double*>(take(N * (4 * sizeof(double) + 1)));
     sl.partials = reinterpret_cast<double*>(take(size_t(3) * items_for(d, 8) * kPartials * sizeof(double)));
@@ -266,7 +269,7 @@ mp_status Engine::set_velocity(int k, const double* vx, const double* vy, bool w
     MP_HIP(hipMemcpyAsync(const_cast<double*>(s.md.vny), vy, nyv * sizeof(double), hipMemcpyHostToDevice, stream));
     MeshDev* dm = d_meshes.as<MeshDev>() + k;
     MP_HIP(launch_face_prep(dm, d.nx, d.ny, const_cast<FaceMom*>(s.md.fx), const_cast<FaceMom*>(s.md.fy),
-                            stream));
+                            const_cast<FaceRec*>(s.md.rx), const_cast<FaceRec*>(s.md.ry), stream));
     unsigned long long* bits = d_maxbits.as<unsigned long long>() + k;
     MP_CALL(launch_clear(reinterpret_cast<uint32_t*>(bits), 2, stream) == hipSuccess
                 ? MP_OK : fail(MP_ERR_DEVICE, "clear failed"));
@@ -511,7 +514,7 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
             MP_HIP(launch_batch_stage(precision, last && !split, q_common, d_meshes.as<MeshDev>(), sps,
                                       d_items.as<int2>() + L.first, L.second, blocks,
-                                      d_queues.as<int>() + 3 * s + st, stream));
+                                      d_queues.as<int>() + 3 * s + st, stream, weno.lf_mode != 0));
             if (split)
                 MP_HIP(launch_batch_phase_items(d_meshes.as<MeshDev>(), sps, d_items.as<int2>() + L.first,
                                                 L.second, stream));

@@ -93,8 +93,10 @@ __device__ __forceinline__ void moment_pair(double Vp, double Vm, float Mp1, flo
     FC = fma(tLC, Vp, tRC * Vm) + double(cc);
 }
 
-// Flux moments of every face (one thread per face), once per velocity refresh.
-__global__ void face_prep_kernel(const MeshDev* __restrict__ mesh, FaceMom* fx, FaceMom* fy)
+// Flux moments of every face (one thread per face), once per velocity refresh:
+// the full record and its one-sided 16-byte form.
+__global__ void face_prep_kernel(const MeshDev* __restrict__ mesh, FaceMom* fx, FaceMom* fy,
+                                 FaceRec* rx, FaceRec* ry)
 {
     const MeshDev& m = *mesh;
     const int64_t nxf = int64_t(m.ny) * (m.nx + 1), nyf = int64_t(m.ny + 1) * m.nx;
@@ -126,6 +128,18 @@ __global__ void face_prep_kernel(const MeshDev* __restrict__ mesh, FaceMom* fx, 
     o.m1 = float(m1);
     o.m2 = float(m2);
     (vert ? fx : fy)[f] = o;
+    // vp and the p moments are sums of +0 and positive terms, vm and the m
+    // moments of +0 and negative ones, so "vm == 0" means the m side is +0
+    // throughout (likewise vp); under the global LF speed both sides carry weight.
+    FaceRec c;
+    if (!m.lf_mode && vm == 0.0) {
+        c.v = vp; c.a = o.p1; c.b = o.p2;
+    } else if (!m.lf_mode && vp == 0.0) {
+        c.v = vm; c.a = o.m1; c.b = o.m2;
+    } else {
+        c.v = 0.0; c.a = 0.0f; c.b = __builtin_nanf("");
+    }
+    (vert ? rx : ry)[f] = c;
 }
 
 // Flux of both fields through one interior face; DIFF adds the thermal
@@ -877,11 +891,11 @@ static void launch_fp64(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, 
 }
 
 hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
-                        int nitems, hipStream_t stream, bool limit, bool diffusion)
+                        int nitems, hipStream_t stream, bool limit, bool diffusion, bool global_lf)
 {
     if (mode != 0) {
         if (phase || limit) return hipErrorInvalidValue;   // mixed: phase_items_kernel follows
-        return launch_stage32(d_mesh, sp, nitems, stream, diffusion);
+        return launch_stage32(d_mesh, sp, nitems, stream, diffusion, global_lf);
     }
     if (q == 5) {
         if (phase) launch_fp64<true, 5>(d_mesh, sp, nitems, stream, limit, diffusion);
@@ -904,11 +918,11 @@ hipError_t launch_limit_diag(const MeshDev* d_mesh, int ncells, const double* H,
 }
 
 hipError_t launch_face_prep(const MeshDev* d_mesh, int nx, int ny, FaceMom* fx, FaceMom* fy,
-                            hipStream_t stream)
+                            FaceRec* rx, FaceRec* ry, hipStream_t stream)
 {
     const int64_t n = int64_t(ny) * (nx + 1) + int64_t(ny + 1) * nx;
     const dim3 grid(unsigned((n + 255) / 256)), block(256);
-    face_prep_kernel<<<grid, block, 0, stream>>>(d_mesh, fx, fy);
+    face_prep_kernel<<<grid, block, 0, stream>>>(d_mesh, fx, fy, rx, ry);
     return hipGetLastError();
 }
 
@@ -972,7 +986,7 @@ hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, c
 
 hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes,
                               const StagePtrs* stages, const int2* items, int nitems, int blocks,
-                              int* queue, hipStream_t stream)
+                              int* queue, hipStream_t stream, bool global_lf)
 {
     if (nitems <= 0) return hipSuccess;
     int* qp = blocks > 0 ? queue : nullptr;
@@ -985,7 +999,7 @@ hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes
         else batch_sweep_kernel<0, false, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp);
     } else {
         if (phase) return hipErrorInvalidValue;   // mixed: batch_phase_items_kernel follows
-        return launch_batch_stage32(meshes, stages, items, nitems, blocks, queue, stream);
+        return launch_batch_stage32(meshes, stages, items, nitems, blocks, queue, global_lf, stream);
     }
     return hipGetLastError();
 }

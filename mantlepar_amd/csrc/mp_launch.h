@@ -10,12 +10,14 @@ namespace mp {
 // limit: apply the bound limiter (FP64 sweep only).
 // diffusion: the mesh has thermal diffusion or thermal boundary data (the
 // mixed sweep compiles the terms out otherwise).
+// global_lf: the mesh uses the global LF speed (mixed sweep's record form).
 hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
-                        int nitems, hipStream_t stream, bool limit = false, bool diffusion = true);
+                        int nitems, hipStream_t stream, bool limit = false, bool diffusion = true,
+                        bool global_lf = false);
 hipError_t launch_limit_diag(const MeshDev* d_mesh, int ncells, const double* H, const double* C,
                              double* thetaH, double* thetaC, uint32_t* error, hipStream_t stream);
 hipError_t launch_face_prep(const MeshDev* d_mesh, int nx, int ny, FaceMom* fx, FaceMom* fy,
-                            hipStream_t stream);
+                            FaceRec* rx, FaceRec* ry, hipStream_t stream);
 hipError_t launch_recon_diag(int mode, const MeshDev* d_mesh, int ncells, const double* u,
                              int field, double* sigma, double* omega, double* traces,
                              uint32_t* error, hipStream_t stream);
@@ -36,7 +38,7 @@ hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H
 // (an int that must be zero at launch); blocks == 0: one wave per item.
 hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes,
                               const StagePtrs* stages, const int2* items, int nitems, int blocks,
-                              int* queue, hipStream_t stream);
+                              int* queue, hipStream_t stream, bool global_lf = false);
 hipError_t launch_batch_reduce(const StagePtrs* stages, const int* nitems, const MeshDev* meshes,
                                const int* slots, int nslots, int nstages, const double* w,
                                double* out, hipStream_t stream);
@@ -47,10 +49,12 @@ int batch_blocks_per_cu(int mode);
 int batch_blocks_per_cu32();
 // The mixed sweep has no phase-fused form: phase_items_kernel follows its last stage.
 // diffusion: the mesh has thermal diffusion or thermal boundary data.
+// global_lf: the mesh uses the global LF speed, so the sweep fetches full face
+// records instead of their one-sided form.
 hipError_t launch_stage32(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
-                          bool diffusion);
+                          bool diffusion, bool global_lf);
 hipError_t launch_batch_stage32(const MeshDev* meshes, const StagePtrs* stages, const int2* items,
-                                int nitems, int blocks, int* queue, hipStream_t stream);
+                                int nitems, int blocks, int* queue, bool global_lf, hipStream_t stream);
 // Phase and state sums of a stage's output over the stage's work items
 // (sp.Ho/sp.Co in, sp.T..region and partial slots 0-3 out).
 hipError_t launch_phase_items(const MeshDev* d_mesh, const StagePtrs& sp, int nitems,

@@ -12,7 +12,8 @@
 //    x-direction quadratic (a1, a2), and consecutive rows share the vertical
 //    difference e.  The three cells that use a row (rows r-1, r, r+1 around
 //    it) read these instead of recomputing eight differences each;
-//  * face flux moments arrive as one 32-byte record per face.
+//  * face flux moments arrive as one 16-byte record per face (the one-sided
+//    form of the local LF speed), 32 bytes where both sides carry weight.
 //
 // The arithmetic is the production ML-WENO of the reference
 // (src/mlweno/reconstruction.cpp:212-265, 489-496) in the closed forms of
@@ -211,6 +212,27 @@ __device__ __forceinline__ void recon32(const K32& k, const Row32& R0, const Row
     }
 }
 
+// The face's full moments from its one-sided record: the side that v's sign
+// names, +0 on the other; a flagged face (b NaN) reads its full record.
+__device__ __forceinline__ FaceMom expand_rec(const FaceRec& h, const FaceMom* full, size_t face)
+{
+    if (__builtin_isnan(h.b)) return full[face];
+    const bool pos = h.v >= 0.0;
+    FaceMom o;
+    o.vp = pos ? h.v : 0.0;
+    o.vm = pos ? 0.0 : h.v;
+    o.p1 = pos ? h.a : 0.f;
+    o.p2 = pos ? h.b : 0.f;
+    o.m1 = pos ? 0.f : h.a;
+    o.m2 = pos ? 0.f : h.b;
+    return o;
+}
+
+// The record a sweep fetches per face: the one-sided form under the local LF
+// speed (FULL = false), the full record under the global speed.
+template <bool FULL> struct FaceFetch { using type = FaceRec; };
+template <> struct FaceFetch<true> { using type = FaceMom; };
+
 // Both fields through one interior face from its moment record: first-order
 // part in FP64, high-order correction in FP32.
 // D: thermal diffusion coefficient of H on this face (DIFF = false: none).
@@ -234,7 +256,7 @@ __device__ __forceinline__ void face_flux(const FaceMom& f, double D, double tLH
 // stage's boundary-flux partials (state-sum slots zero).  DIFF: the mesh has
 // thermal diffusion or thermal boundary data; without it the diffusion terms
 // are compiled out, which keeps the sweep within its register budget.
-template <bool DIFF>
+template <bool DIFF, bool FULL>
 __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, int strip, int j0,
                                              int j1)
 {
@@ -277,12 +299,27 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
     }
 
     // Face records and u_n are fetched one row ahead, like the averages, so
-    // no load is consumed in the iteration that issues it.
+    // no load is consumed in the iteration that issues it.  Under the local LF
+    // speed a lane fetches the 16-byte one-sided record and expands it where
+    // it is used; under the global speed it fetches the full record.
+    using Rec = typename FaceFetch<FULL>::type;
     const int ix = clampi(i + 1, 0, nx);           // vertical face x_{i+1}
-    auto load_fy = [&](int row) { return m.fy[size_t(row) * nx + ic]; };
-    auto load_fx = [&](int row) { return m.fx[size_t(clampi(row, 0, ny - 1)) * (nx + 1) + ix]; };
+    auto fy_index = [&](int row) { return size_t(row) * nx + ic; };
+    auto fx_index = [&](int row) { return size_t(clampi(row, 0, ny - 1)) * (nx + 1) + ix; };
+    auto load_fy = [&](int row) -> Rec {
+        if constexpr (FULL) return m.fy[fy_index(row)];
+        else return m.ry[fy_index(row)];
+    };
+    auto load_fx = [&](int row) -> Rec {
+        if constexpr (FULL) return m.fx[fx_index(row)];
+        else return m.rx[fx_index(row)];
+    };
+    auto moments = [&](const Rec& h, const FaceMom* full, size_t face) -> FaceMom {
+        if constexpr (FULL) return h;
+        else return expand_rec(h, full, face);
+    };
     const bool need_n = sp.a != 0.0;
-    FaceMom fyc = load_fy(r), fxc = load_fx(r);
+    Rec fyc = load_fy(r), fxc = load_fx(r);
     double hn = 0.0, cn = 0.0;                     // u_n of the row being finished
 
     float topH[3] = {0.f, 0.f, 0.f}, topC[3] = {0.f, 0.f, 0.f};
@@ -337,7 +374,8 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
                 FvC = sign * oCf;
                 if (owned) { bH += oHf; bC += oCf; }
             } else {
-                face_flux<DIFF>(fyc, m.dif_y, H0.u, topH, tH, LH.c[0], C0.u, topC, tC, LC.c[0], FvH, FvC);
+                const FaceMom f = moments(fyc, m.fy, fy_index(r));
+                face_flux<DIFF>(f, m.dif_y, H0.u, topH, tH, LH.c[0], C0.u, topC, tC, LC.c[0], FvH, FvC);
             }
         }
         if (shift) fyc = load_fy(r + 1);   // used next iteration
@@ -391,7 +429,8 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
                     FrC = sign * oCf;
                     if (left || owned) { bH += oHf; bC += oCf; }
                 } else {
-                    face_flux<DIFF>(fxc, m.dif_x, tH, LH.c[1], nbtH, nbH, tC, LC.c[1], nbtC, nbC, FrH, FrC);
+                    const FaceMom f = moments(fxc, m.fx, fx_index(r));
+                    face_flux<DIFF>(f, m.dif_x, tH, LH.c[1], nbtH, nbH, tC, LC.c[1], nbtC, nbC, FrH, FrC);
                 }
             }
             accH = shr1(FrH) - FrH + FvH;
@@ -411,7 +450,7 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
     item_partials(sp, item, 0.0, 0.0, 0.0, 0.0, bH, bC, err);
 }
 
-template <bool DIFF>
+template <bool DIFF, bool FULL>
 __global__ __launch_bounds__(256, 3) void stage32_kernel(const MeshDev* __restrict__ mesh, StagePtrs sp, int nitems)
 {
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
@@ -421,12 +460,13 @@ __global__ __launch_bounds__(256, 3) void stage32_kernel(const MeshDev* __restri
     const int strip = item % sp.strips, band = item / sp.strips;
     const int j0 = band * sp.rows;
     const int j1 = min(j0 + sp.rows, m.ny);
-    sweep32_item<DIFF>(m, sp, item, strip, j0, j1);
+    sweep32_item<DIFF, FULL>(m, sp, item, strip, j0, j1);
 }
 
 // Persistent form (queue != nullptr): every wave takes the next work item
 // from a counter until the list is exhausted; otherwise one wave per item.
-// Batch meshes carry no thermal diffusion.
+// Batch meshes carry no thermal diffusion and share one LF mode (FULL: global).
+template <bool FULL>
 __global__ __launch_bounds__(256, 3) void batch32_kernel(const MeshDev* __restrict__ meshes,
                                                          const StagePtrs* __restrict__ stages,
                                                          const int2* __restrict__ items, int nitems,
@@ -449,7 +489,7 @@ __global__ __launch_bounds__(256, 3) void batch32_kernel(const MeshDev* __restri
         const int strip = local % sp.strips, band = local / sp.strips;
         const int j0 = band * sp.rows;
         if (j0 < m.ny)                 // always true with a current work list
-            sweep32_item<false>(m, sp, local, strip, j0, min(j0 + sp.rows, m.ny));
+            sweep32_item<false, FULL>(m, sp, local, strip, j0, min(j0 + sp.rows, m.ny));
         if (!queue) break;
     }
 }
@@ -457,28 +497,34 @@ __global__ __launch_bounds__(256, 3) void batch32_kernel(const MeshDev* __restri
 }  // namespace
 
 hipError_t launch_stage32(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
-                          bool diffusion)
+                          bool diffusion, bool global_lf)
 {
     const dim3 grid((nitems + 3) / 4), block(256);
-    if (diffusion) stage32_kernel<true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-    else stage32_kernel<false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    if (global_lf) {
+        if (diffusion) stage32_kernel<true, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        else stage32_kernel<false, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    } else {
+        if (diffusion) stage32_kernel<true, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        else stage32_kernel<false, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    }
     return hipGetLastError();
 }
 
 int batch_blocks_per_cu32()
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch32_kernel, 256, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch32_kernel<false>, 256, 0) != hipSuccess) return 0;
     return n;
 }
 
 hipError_t launch_batch_stage32(const MeshDev* meshes, const StagePtrs* stages, const int2* items,
-                                int nitems, int blocks, int* queue, hipStream_t stream)
+                                int nitems, int blocks, int* queue, bool global_lf, hipStream_t stream)
 {
     if (nitems <= 0) return hipSuccess;
     int* q = blocks > 0 ? queue : nullptr;
     const dim3 grid(blocks > 0 ? blocks : (nitems + 3) / 4), block(256);
-    batch32_kernel<<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q);
+    if (global_lf) batch32_kernel<true><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q);
+    else batch32_kernel<false><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q);
     return hipGetLastError();
 }
 

@@ -63,8 +63,12 @@ mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
     const size_t nxf = size_t(m.ny) * (m.nx + 1), nyf = size_t(m.ny + 1) * m.nx;
     MP_CALL(fx.alloc(nxf * sizeof(FaceMom)));
     MP_CALL(fy.alloc(nyf * sizeof(FaceMom)));
+    MP_CALL(rx.alloc(nxf * sizeof(FaceRec)));
+    MP_CALL(ry.alloc(nyf * sizeof(FaceRec)));
     md.fx = fx.as<FaceMom>();
     md.fy = fy.as<FaceMom>();
+    md.rx = rx.as<FaceRec>();
+    md.ry = ry.as<FaceRec>();
     MP_CALL(error.alloc(sizeof(uint64_t) * 2));
     MP_CALL(d_md.alloc(sizeof(MeshDev)));
     MP_HIP(hipMemsetAsync(H[0].ptr, 0, N * sizeof(double), stream));
@@ -161,6 +165,7 @@ mp_status mp_transport::set_velocity(const double* x, const double* y)
     MP_HIP(hipMemcpyAsync(vnx.ptr, x, nxv * sizeof(double), hipMemcpyHostToDevice, stream));
     MP_HIP(hipMemcpyAsync(vny.ptr, y, nyv * sizeof(double), hipMemcpyHostToDevice, stream));
     MP_HIP(launch_face_prep(d_md.as<MeshDev>(), mesh.nx, mesh.ny, fx.as<FaceMom>(), fy.as<FaceMom>(),
+                            rx.as<FaceRec>(), ry.as<FaceRec>(),
                             stream));
     MP_CALL(compute_rate());
     have_velocity = true;
@@ -272,7 +277,7 @@ mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& fi
             const bool split = last && precision == MP_PREC_MIXED;
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
             MP_HIP(launch_stage(precision, last && !split, mesh.q, d_md.as<MeshDev>(), sp, nitems,
-                                stream, md.limit != 0, has_diffusion()));
+                                stream, md.limit != 0, has_diffusion(), md.lf_mode != 0));
             if (split) MP_HIP(launch_phase_items(d_md.as<MeshDev>(), sp, nitems, stream));
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
             kin = out;

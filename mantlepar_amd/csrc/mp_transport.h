@@ -22,6 +22,7 @@ struct mp_transport {
     int cur = 0;
     mp::DeviceBuffer vnx, vny;
     mp::DeviceBuffer fx, fy;   // per-face flux moments
+    mp::DeviceBuffer rx, ry;   // and their one-sided 16-byte form
     bool have_velocity = false;
     double max_rate = 0;
     mp::DeviceBuffer bc[4], inflowH, inflowC;

@@ -14,7 +14,8 @@
  *                        direct-formula weights, extreme weights, boundary
  *                        candidate counts (src/mlweno/test/test_reconstruction.cpp).
  *   flux, operator       LLF closed form, constant-state preservation, global
- *                        balance, brute force on tiny meshes.
+ *                        balance, brute force on tiny meshes (local and global
+ *                        LF speed), the global speed's closed forms.
  *   SSP-RK3              stability polynomial and third-order convergence
  *                        (not in the reference: it uses SSPRK2, evolve.cpp:469-479).
  * The mlweno sources need PETSc and LAPACKE, which this image lacks, so they are
@@ -1384,6 +1385,8 @@ struct mpo_transport {
     uint8_t* tbc[4];                  /* [face][q]: 0 zero flux, 1 temperature, 2 flux */
     double* tval[4];                  /* [face][q] canonical order */
     uint8_t* bcC[4];                  /* advection policies of C (NULL: those of H) */
+    int lf_mode;                      /* interior faces: 0 local, 1 global LF speed */
+    double global_speed;
 };
 
 static size_t cidx(const mpo_transport* t, int i, int j) { return (size_t)j * t->nx + i; }
@@ -1527,6 +1530,20 @@ static void canonical_face(const mpo_transport* t, int i, int j, int f, mpo_poin
     case 2: e[0] = c[3]; e[1] = c[2]; break;
     default: e[0] = c[0]; e[1] = c[3]; break;
     }
+}
+
+/* Lax-Friedrichs speed of the interior faces (FluxImpl's mode,
+ * src/mlweno/advectiveflux.cpp:129-164): 0 local |v.n| per point, 1 the
+ * global speed, which every point checks against its own bound. */
+int mpo_transport_set_lf(mpo_transport* t, int mode, double global_speed)
+{
+    CHECK(mode == 0 || mode == 1, MPO_ERR_ARG, "Unknown LF mode");
+    if (mode == 1)
+        CHECK(finite_d(global_speed) && global_speed >= 0, MPO_ERR_ARG,
+              "Global LF speed must be finite and nonnegative");
+    t->lf_mode = mode;
+    t->global_speed = mode == 1 ? global_speed : 0.0;
+    return MPO_OK;
 }
 
 int mpo_transport_set_limiter(mpo_transport* t, int enable, int cell_points, int samples,
@@ -1790,7 +1807,8 @@ static int face_fluxes(mpo_transport* t, int field, const double* u)
                     left[q] = t->traces[(cidx(t, i - 1, j) * 4 + 1) * nq + q];
                     right[q] = t->traces[(cidx(t, i, j) * 4 + 3) * nq + q];
                 }
-                e = mpo_integrate_edge_flux(edge, nq, t->qp, t->qw, vel, left, right, 0, 0, &flux);
+                e = mpo_integrate_edge_flux(edge, nq, t->qp, t->qw, vel, left, right, t->lf_mode,
+                                            t->global_speed, &flux);
                 if (!e && diffuse) {
                     double d;
                     e = face_diffusion(t, edge, i - 1, j, i, j, u, 0, 0, 0, &d, NULL, NULL);
@@ -1857,7 +1875,8 @@ static int face_fluxes(mpo_transport* t, int field, const double* u)
                     left[q] = t->traces[(cidx(t, i, j - 1) * 4 + 2) * nq + qc];
                     right[q] = t->traces[(cidx(t, i, j) * 4 + 0) * nq + qc];
                 }
-                e = mpo_integrate_edge_flux(edge, nq, t->qp, t->qw, vel, left, right, 0, 0, &flux);
+                e = mpo_integrate_edge_flux(edge, nq, t->qp, t->qw, vel, left, right, t->lf_mode,
+                                            t->global_speed, &flux);
                 if (!e && diffuse) {
                     double d;
                     e = face_diffusion(t, edge, i, j - 1, i, j, u, 0, 0, 0, &d, NULL, NULL);

@@ -261,6 +261,9 @@ int mpo_transport_rhs(mpo_transport* t, const double* H, const double* C, double
  * points [N][4][q] in face order bottom,right,top,left, Gauss order along +x/+y. */
 int mpo_transport_reconstruct(mpo_transport* t, const double* u, double* sigma,
                               double* omega, double* traces);
+/* Lax-Friedrichs speed of the interior faces (src/mlweno/advectiveflux.cpp:129-164):
+ * mode 0 local |v.n| per point (the default), 1 global_speed at every point. */
+int mpo_transport_set_lf(mpo_transport* t, int mode, double global_speed);
 /* Bound limiter of the coupled reconstruction (src/couple/state_reconstruction.cpp:136-192):
  * cell_points Gauss points per axis, `samples` normal-line diffusion samples
  * per edge point with extent fraction `extent`, composition bounded by c_upper

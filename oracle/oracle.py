@@ -294,6 +294,7 @@ def _declare(L):
                                      C.POINTER(C.c_double)]
     L.mpo_transport_set_limiter.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
                                             C.c_double]
+    L.mpo_transport_set_lf.argtypes = [C.c_void_p, C.c_int, C.c_double]
     L.mpo_transport_limits.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     L.mpo_transport_traces.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     L.mpo_transport_set_diffusion.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double,
@@ -682,6 +683,11 @@ class Transport:
                                            totals.ctypes.data_as(C.c_void_p)))
         return (H.reshape(self.ny, self.nx), Cc.reshape(self.ny, self.nx),
                 totals.reshape(-1, 4)[:nsteps])
+
+    def set_lf(self, mode, global_speed=0.0):
+        """LF speed of the interior faces (src/mlweno/advectiveflux.cpp:129-164):
+        0 local |v.n|, 1 the global speed."""
+        _check(lib().mpo_transport_set_lf(self._h, int(mode), float(global_speed)))
 
     def set_limiter(self, enable=True, cell_points=5, samples=4, extent=0.9, c_upper=0.4):
         """Bound limiter of src/couple/state_reconstruction.cpp:136-192 on every trace."""

@@ -3,7 +3,9 @@
 Pins: constant-state preservation and global mass/boundary balance
 (src/mlweno/test/test_advection_diffusion.cpp:125-148,
 src/mlweno/test/transport_test_support.h:342-344); the LLF closed form; a brute
-force face loop on a tiny mesh built from the per-object API; the per-step
+force face loop on a tiny mesh built from the per-object API, with the local and
+a global LF speed, and the global speed's closed forms (the local rule where
+every point's speed equals it, affine in the speed otherwise); the per-step
 conservation balance (src/couple/evolve.cpp:486-490); the SSP stability
 polynomials 1+z, 1+z+z^2/2, 1+z+z^2/2+z^3/6 on a linear operator; third-order
 temporal convergence of SSP-RK3; backflow rejection
@@ -85,14 +87,18 @@ def test_llf_closed_form():
         O.llf_point(1.0, 2.0, (2.0, 0.0), (1.0, 0.0), mode=1, global_speed=1.0)
 
 
-def test_brute_force_face_loop_on_tiny_mesh():
+@pytest.mark.parametrize("lf", ["local", "global"])
+def test_brute_force_face_loop_on_tiny_mesh(lf):
     """Per-object API (Recon + integrate_edge_flux) assembled face by face in the
-    order of src/mlweno/test/transport_test_support.h:253-269, against rhs()."""
+    order of src/mlweno/test/transport_test_support.h:253-269, against rhs(),
+    with the local LF speed and with a global one (1.5 max |v|)."""
     spec = W.MeshSpec(4, 3, seed=6)
     vnx, vny = W.face_velocity(spec)
     coords = W.vertex_coords(spec)
     H = W.enthalpy(spec)
     t = make(spec, vnx, vny)
+    mode, a = (0, 0.0) if lf == "local" else (1, 1.5 * max(np.abs(vnx).max(), np.abs(vny).max()))
+    t.set_lf(mode, a)
     LH, _, _, _ = t.rhs(H, W.composition(spec))
     nx, ny, nq = spec.nx, spec.ny, spec.q
     recon = {}
@@ -122,10 +128,49 @@ def test_brute_force_face_loop_on_tiny_mesh():
                     vel = [(0.0, vny[nq - 1 - q, j + 1, i]) for q in range(nq)]
                 left = [recon[(i, j)].evaluate(tuple(p)) for p in pts]
                 right = [recon[nb].evaluate(tuple(p)) for p in pts]
-                F = O.integrate_edge_flux(edge, vel, left, right, nq)
+                F = O.integrate_edge_flux(edge, vel, left, right, nq, mode=mode, global_speed=a)
                 rhs[j, i] -= F / (spec.dx * spec.dy)
                 rhs[nb[1], nb[0]] += F / (spec.dx * spec.dy)
     assert np.abs(rhs - LH).max() <= 1e-13 * max(1.0, np.abs(LH).max())
+
+
+def test_global_lf_closed_forms():
+    """The global LF speed on the transport operator: on a flow whose normal
+    speed is a at every face point it is the local rule bit for bit; otherwise
+    the RHS is affine in the speed, rhs(a2) - rhs(a1) = (a2 - a1) * rhs(v = 0,
+    speed 1), the pure jump dissipation, whatever the flow; a speed below a
+    point's own bound is rejected."""
+    spec = W.MeshSpec(9, 7, seed=3)
+    H, Cc = W.enthalpy(spec), W.composition(spec)
+    rng = np.random.default_rng(5)
+    a = 0.3
+    vnx = a * rng.choice([-1.0, 1.0], size=(1, spec.ny, spec.nx + 1)).repeat(spec.q, 0)
+    vny = a * rng.choice([-1.0, 1.0], size=(1, spec.ny + 1, spec.nx)).repeat(spec.q, 0)
+    t = make(spec, vnx, vny)
+    loc = t.rhs(H, Cc)
+    t.set_lf(1, a)
+    glo = t.rhs(H, Cc)
+    for x, y in zip(loc[:2], glo[:2]):
+        assert np.array_equal(x, y)
+    vnx, vny = W.face_velocity(spec)
+    vmax = max(np.abs(vnx).max(), np.abs(vny).max())
+    t = make(spec, vnx, vny)
+    t.set_lf(1, 1.2 * vmax)
+    r1 = t.rhs(H, Cc)
+    t.set_lf(1, 2.7 * vmax)
+    r2 = t.rhs(H, Cc)
+    z = make(spec, 0 * vnx, 0 * vny)
+    z.set_lf(1, 1.0)
+    d = z.rhs(H, Cc)
+    for k in range(2):
+        want = 1.5 * vmax * d[k]
+        assert np.abs((r2[k] - r1[k]) - want).max() <= 1e-12 * np.abs(r1[k]).max()
+        assert np.abs(d[k]).max() > 0
+    t.set_lf(1, 0.5 * vmax)
+    with pytest.raises(O.OracleError, match="smaller than the current local bound"):
+        t.rhs(H, Cc)
+    with pytest.raises(O.OracleError, match="Unknown LF mode"):
+        t.set_lf(2, 1.0)
 
 
 @pytest.mark.parametrize("scheme", [1, 2, 3])
