@@ -82,21 +82,28 @@ mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
     return MP_OK;
 }
 
-// Rows per work item when the caller sets none: 8, or 16 for the FP64 sweep
-// on a mesh large enough to give every wave slot of the chip two 16-row items.
-// These were the fastest measured on MI355X (2048x4096: 8 rows for the mixed
-// sweep, 16 for FP64; longer items, even ones sized to fill the chip in a
-// single round, ran slower), and a small mesh's stage time is the time of one
-// item, so short items also serve small meshes best.
+// Rows per work item when the caller sets none, by how many items of a height
+// each wave slot of the chip gets.  FP64 sweep (2 waves per SIMD): 16 rows
+// when every slot gets two 16-row items, else 8.  Mixed sweep (3 waves per
+// SIMD): 16 rows at four items per slot, 12 at two, else 8.  These were the
+// fastest measured on MI355X (mixed: 2048x4096 12 rows, 4096x4096 and larger
+// 16; FP64: 2048x4096 16; items sized to fill the chip in a single round ran
+// slower), and a small mesh's stage time is the time of one item, so short
+// items serve small meshes best.
 int mp_transport::default_rows() const
 {
-    if (precision == MP_PREC_MIXED) return 8;
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const long long slots = (long long)cus * 4 * 2;
     const long long strips = (mesh.nx + kStripCells - 1) / kStripCells;
-    return strips * ((mesh.ny + 15) / 16) >= 2 * slots ? 16 : 8;
+    auto items = [&](int rows) { return strips * ((mesh.ny + rows - 1) / rows); };
+    if (precision == MP_PREC_MIXED) {
+        const long long slots = (long long)cus * 4 * 3;
+        if (items(16) >= 4 * slots) return 16;
+        return items(12) >= 2 * slots ? 12 : 8;
+    }
+    const long long slots = (long long)cus * 4 * 2;
+    return items(16) >= 2 * slots ? 16 : 8;
 }
 
 mp_transport::~mp_transport()
