@@ -504,7 +504,7 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
     // The work index is wave-uniform; saying so keeps every per-mesh table
     // load scalar (SGPRs) instead of replicating it across lanes.
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    int it = blockIdx.x * 4 + wave;
+    int it = blockIdx.x * int(blockDim.x >> 6) + wave;
     for (;;) {
         if (queue) {
             int t = 0;
@@ -990,8 +990,10 @@ hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes
                               int* queue, hipStream_t stream, bool global_lf)
 {
     if (nitems <= 0) return hipSuccess;
+    // A persistent grid (blocks > 0) runs 4-wave workgroups that take items from
+    // the queue; otherwise one wave per workgroup and per item.
     int* qp = blocks > 0 ? queue : nullptr;
-    const dim3 grid(blocks > 0 ? blocks : (nitems + 3) / 4), block(256);
+    const dim3 grid(blocks > 0 ? blocks : nitems), block(blocks > 0 ? 256 : kWave);
     if (mode == 0 && q == 5) {
         if (phase) batch_sweep_kernel<0, true, 5><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp);
         else batch_sweep_kernel<0, false, 5><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp);

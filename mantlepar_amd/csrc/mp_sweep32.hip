@@ -466,7 +466,8 @@ __global__ __launch_bounds__(64, 3) void stage32_kernel(const MeshDev* __restric
 }
 
 // Persistent form (queue != nullptr): every wave takes the next work item
-// from a counter until the list is exhausted; otherwise one wave per item.
+// from a counter until the list is exhausted; otherwise one wave per item
+// (and per workgroup).
 // Batch meshes carry no thermal diffusion and share one LF mode (FULL: global).
 template <bool FULL>
 __global__ __launch_bounds__(256, 3) void batch32_kernel(const MeshDev* __restrict__ meshes,
@@ -475,7 +476,7 @@ __global__ __launch_bounds__(256, 3) void batch32_kernel(const MeshDev* __restri
                                                          int* queue)
 {
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    int it = blockIdx.x * 4 + wave;
+    int it = blockIdx.x * int(blockDim.x >> 6) + wave;
     for (;;) {
         if (queue) {
             int t = 0;
@@ -524,7 +525,7 @@ hipError_t launch_batch_stage32(const MeshDev* meshes, const StagePtrs* stages, 
 {
     if (nitems <= 0) return hipSuccess;
     int* q = blocks > 0 ? queue : nullptr;
-    const dim3 grid(blocks > 0 ? blocks : (nitems + 3) / 4), block(256);
+    const dim3 grid(blocks > 0 ? blocks : nitems), block(blocks > 0 ? 256 : kWave);
     if (global_lf) batch32_kernel<true><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q);
     else batch32_kernel<false><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q);
     return hipGetLastError();
