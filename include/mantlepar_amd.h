@@ -208,7 +208,7 @@ mp_status mp_transport_limits(mp_transport* t, double* thetaH, double* thetaC);
  * rank, each with its own StateReconstruction and stepping loop
  * (src/couple/state_reconstruction.cpp:35-120, src/couple/preheat.cpp:234-287);
  * a batch holds many such shares on one device and runs every stage as ONE
- * persistent launch over all of them.  Each mesh keeps its own dt, boundary
+ * launch over all of them.  Each mesh keeps its own dt, boundary
  * data, error word and report, and its results are bitwise those of an
  * mp_transport handle on the same inputs.  No reference counterpart for the
  * batching itself.  Mesh ids are slot numbers in [0, capacity). */
@@ -247,7 +247,7 @@ mp_status mp_batch_set_lithostatic(mp_batch* b, int32_t id, int32_t enable, doub
 mp_status mp_batch_phase(mp_batch* b, int32_t id, double* T, double* phi, double* Cl,
                          double* Cs, uint8_t* region);
 /* Schedule: rows per work item (>= 8) and persistent grid size in blocks of
- * four waves (0: one wave per work item). */
+ * four waves (0: one wave per work item, the default with 12 rows). */
 mp_status mp_batch_set_schedule(mp_batch* b, int32_t rows_per_item, int32_t blocks);
 mp_status mp_batch_get_schedule(mp_batch* b, int32_t* rows_per_item, int32_t* blocks);
 /* Time candidate schedules on the resident meshes (state is left unchanged),

@@ -319,13 +319,11 @@ mp_status mp_batch_create(int32_t capacity, const mp_weno_options* weno, const m
         const size_t budget = hbm_budget_bytes ? size_t(hbm_budget_bytes) : ~size_t(0) / 2;
         mp_status s = b->eng.init(capacity, precision, w, p, stream, budget);
         if (s) { delete b; return s; }
-        // Defaults before any tuning: short items on a persistent grid that
-        // fills every resident wave slot.
-        int dev = 0, cus = 0;
-        if (hipGetDevice(&dev) == hipSuccess)
-            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        b->rows = precision == MP_PREC_MIXED ? 8 : 16;
-        b->eng.blocks = std::max(1, cus) * std::max(1, batch_blocks_per_cu(precision));
+        // Defaults before any tuning: 12-row items, one wave per item and per
+        // workgroup (the autotuner's choice for the 256-mesh batch on MI355X,
+        // in both precisions).
+        b->rows = 12;
+        b->eng.blocks = 0;
         *out = b;
         return MP_OK;
     })
