@@ -159,8 +159,12 @@ struct StagePtrs {
     uint8_t* region;
     double* partials;   // [item][kPartials]: sumH, errH, sumC, errC, boundaryH, boundaryC
     uint32_t* error;
-    int32_t rows;       // rows per work item
+    int32_t rows;       // rows per work item below row `split`
     int32_t strips;     // work items per row band
+    int32_t split;      // first row of the tail bands (ny: none)
+    int32_t tail_rows;  // rows per work item from `split` up
+    int32_t nlong;      // work items below `split`
+    int32_t pad;
 };
 
 }  // namespace mp

@@ -21,6 +21,34 @@ __device__ __forceinline__ int xcd_slab_block(int b, int G)
     return x * full + min(x, rem) + k;
 }
 
+// Strip and rows [j0, j1) of work item `item` of a sweep.  Items below row
+// sp.split are sp.rows tall, the tail items above it sp.tail_rows, each band
+// of items ordered by strip.
+__device__ __forceinline__ void item_region(const StagePtrs& sp, int ny, int item, int& strip, int& j0,
+                                            int& j1)
+{
+    const int S = sp.strips;
+    if (item < sp.nlong) {
+        strip = item % S;
+        j0 = (item / S) * sp.rows;
+        j1 = min(j0 + sp.rows, sp.split);
+    } else {
+        const int k = item - sp.nlong;
+        strip = k % S;
+        j0 = sp.split + (k / S) * sp.tail_rows;
+        j1 = min(j0 + sp.tail_rows, ny);
+    }
+}
+
+// Work item of workgroup b in a one-wave-per-workgroup launch over one mesh:
+// each segment in XCD slabs, and the short tail items after the others, so
+// they are the last to be dispatched and the launch drains on short items.
+__device__ __forceinline__ int item_of_block(const StagePtrs& sp, int b, int nitems)
+{
+    return b < sp.nlong ? xcd_slab_block(b, sp.nlong)
+                        : sp.nlong + xcd_slab_block(b - sp.nlong, nitems - sp.nlong);
+}
+
 // Compensated accumulation s + e += x (Neumaier); contraction would break it.
 __device__ __forceinline__ void two_sum_acc(double& s, double& e, double x)
 {

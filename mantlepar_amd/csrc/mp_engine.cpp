@@ -129,10 +129,13 @@ mp_status Engine::init(int cap, int32_t prec, const mp_weno_options& w, const mp
 static size_t faces_x(const mp_mesh_desc& d) { return size_t(d.ny) * (d.nx + 1); }
 static size_t faces_y(const mp_mesh_desc& d) { return size_t(d.ny + 1) * d.nx; }
 
+// An upper bound on the work items of any layout with at least `rows` rows
+// per item: tail items are 4 rows tall and number at most two bands' worth per
+// wave slot, fewer than a 4-row layout of the whole mesh has.
 static int items_for(const mp_mesh_desc& d, int rows)
 {
     const int strips = (d.nx + kStripCells - 1) / kStripCells;
-    return strips * ((d.ny + rows - 1) / rows);
+    return strips * ((d.ny + rows - 1) / rows) + strips * ((d.ny + 3) / 4);
 }
 
 size_t Engine::slot_bytes(const mp_mesh_desc& d, int rows_min)
@@ -213,9 +216,9 @@ void Engine::set_rows(int k, int rows)
     Slot& s = slots[k];
     s.rows = rows < 8 ? 8 : (rows > s.desc.ny ? s.desc.ny : rows);
     if (s.rows < 8) s.rows = 8;
-    s.strips = (s.desc.nx + kStripCells - 1) / kStripCells;
-    s.bands = (s.desc.ny + s.rows - 1) / s.rows;
-    s.nitems = s.strips * s.bands;
+    s.layout = item_layout(s.desc.nx, s.desc.ny, s.rows, precision);
+    s.strips = s.layout.strips;
+    s.nitems = s.layout.nitems;
 }
 
 mp_status Engine::upload_async(int k, const double* H, const double* C)
@@ -467,8 +470,7 @@ mp_status Engine::write_tables(int scheme)
                 }
                 sp.partials = s.partials + size_t(st) * s.nitems * kPartials;
                 sp.error = d_errors.as<uint32_t>() + k;
-                sp.rows = s.rows;
-                sp.strips = s.strips;
+                apply_layout(s.layout, sp);
                 h_stages[size_t(set * 3 + st) * capacity + k] = sp;
             }
     }

@@ -63,7 +63,8 @@ struct Slot {
     int cur = 0, phase_cur = 0;
     bool phase_valid = false, have_velocity = false;
     double max_rate = 0, dt = 0;
-    int rows = 64, strips = 1, bands = 1, nitems = 1;
+    int rows = 64, strips = 1, nitems = 1;
+    ItemLayout layout{};
 };
 
 class Engine {

@@ -70,6 +70,19 @@ mp_status lithostatic_rows(const mp_mesh_desc& mesh, const PhaseHost& h, double 
 // Vertex ordinates of a uniform axis a0 + k*(a1 - a0)/n with the ends snapped
 // (src/core/mesh.cpp:27-33, 86-114); v holds n + 1 values.
 void uniform_vertices(double a0, double a1, int n, double* v);
+
+// Work items of one mesh's sweep: `rows` per item over rows [0, split) and
+// tail_rows per item above, nlong + tail items in all (nitems).  A mesh that
+// gives every wave slot of the chip three or more items keeps its last rows
+// for 4-row tail items, about two per slot, which the single-mesh launch
+// dispatches last; otherwise split = ny and every item is `rows` tall.  The
+// single-mesh handle and the batch engine share it, so a mesh's step totals,
+// sums over its items in item order, are the same in both.
+struct ItemLayout {
+    int strips, rows, split, tail_rows, nlong, nitems;
+};
+ItemLayout item_layout(int nx, int ny, int rows, int32_t precision);
+void apply_layout(const ItemLayout& L, StagePtrs& sp);
 // MapEdgePoint's combination of two end values at reference t in [-1, 1]
 // (src/core/integral.cpp:91-97).
 inline double map_edge(double t, double a, double b)

@@ -366,12 +366,11 @@ template <int MODE, bool PHASE, int NQ, bool LIM, bool DIFF>
 __global__ __launch_bounds__(64, MODE == 0 ? 2 : 3) void stage_sweep_kernel(const MeshDev* __restrict__ mesh,
                                                           StagePtrs sp, int nitems)
 {
-    const int item = xcd_slab_block(int(blockIdx.x), int(gridDim.x));
-    if (item >= nitems) return;
+    if (int(blockIdx.x) >= nitems) return;
+    const int item = item_of_block(sp, int(blockIdx.x), nitems);
     const MeshDev& m = *mesh;
-    const int strip = item % sp.strips, band = item / sp.strips;
-    const int j0 = band * sp.rows;
-    const int j1 = min(j0 + sp.rows, m.ny);
+    int strip, j0, j1;
+    item_region(sp, m.ny, item, strip, j0, j1);
     sweep_item<MODE, PHASE, NQ, LIM, DIFF>(m, sp, item, strip, j0, j1);
 }
 
@@ -517,10 +516,10 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
         const int local = __builtin_amdgcn_readfirstlane(w.y);
         const MeshDev& m = meshes[slot];
         const StagePtrs& sp = stages[slot];
-        const int strip = local % sp.strips, band = local / sp.strips;
-        const int j0 = band * sp.rows;
+        int strip, j0, j1;
+        item_region(sp, m.ny, local, strip, j0, j1);
         if (j0 < m.ny)                 // always true with a current work list
-            sweep_item<MODE, PHASE, NQ, false, false>(m, sp, local, strip, j0, min(j0 + sp.rows, m.ny));
+            sweep_item<MODE, PHASE, NQ, false, false>(m, sp, local, strip, j0, j1);
         if (!queue) break;
     }
 }
@@ -835,9 +834,9 @@ __global__ __launch_bounds__(256) void phase_items_kernel(const MeshDev* __restr
     const int item = xcd_slab_block(int(blockIdx.x), int(gridDim.x)) * 4 + wave;
     if (item >= nitems) return;
     const MeshDev& m = *mesh;
-    const int strip = item % sp.strips, band = item / sp.strips;
-    const int j0 = band * sp.rows;
-    phase_item(m, sp, item, strip, j0, min(j0 + sp.rows, m.ny));
+    int strip, j0, j1;
+    item_region(sp, m.ny, item, strip, j0, j1);
+    phase_item(m, sp, item, strip, j0, j1);
 }
 
 // The same over a batch work list of (mesh slot, item) pairs.
@@ -854,9 +853,9 @@ __global__ __launch_bounds__(256) void batch_phase_items_kernel(const MeshDev* _
     const int local = __builtin_amdgcn_readfirstlane(w.y);
     const MeshDev& m = meshes[slot];
     const StagePtrs& sp = stages[slot];
-    const int strip = local % sp.strips, band = local / sp.strips;
-    const int j0 = band * sp.rows;
-    if (j0 < m.ny) phase_item(m, sp, local, strip, j0, min(j0 + sp.rows, m.ny));
+    int strip, j0, j1;
+    item_region(sp, m.ny, local, strip, j0, j1);
+    if (j0 < m.ny) phase_item(m, sp, local, strip, j0, j1);
 }
 
 // ---------------------------------------------------------------- launchers

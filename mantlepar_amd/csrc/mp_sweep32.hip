@@ -456,12 +456,11 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
 template <bool DIFF, bool FULL>
 __global__ __launch_bounds__(64, 3) void stage32_kernel(const MeshDev* __restrict__ mesh, StagePtrs sp, int nitems)
 {
-    const int item = xcd_slab_block(int(blockIdx.x), int(gridDim.x));
-    if (item >= nitems) return;
+    if (int(blockIdx.x) >= nitems) return;
+    const int item = item_of_block(sp, int(blockIdx.x), nitems);
     const MeshDev& m = *mesh;
-    const int strip = item % sp.strips, band = item / sp.strips;
-    const int j0 = band * sp.rows;
-    const int j1 = min(j0 + sp.rows, m.ny);
+    int strip, j0, j1;
+    item_region(sp, m.ny, item, strip, j0, j1);
     sweep32_item<DIFF, FULL>(m, sp, item, strip, j0, j1);
 }
 
@@ -489,10 +488,10 @@ __global__ __launch_bounds__(256, 3) void batch32_kernel(const MeshDev* __restri
         const int local = __builtin_amdgcn_readfirstlane(w.y);
         const MeshDev& m = meshes[slot];
         const StagePtrs& sp = stages[slot];
-        const int strip = local % sp.strips, band = local / sp.strips;
-        const int j0 = band * sp.rows;
+        int strip, j0, j1;
+        item_region(sp, m.ny, local, strip, j0, j1);
         if (j0 < m.ny)                 // always true with a current work list
-            sweep32_item<false, FULL>(m, sp, local, strip, j0, min(j0 + sp.rows, m.ny));
+            sweep32_item<false, FULL>(m, sp, local, strip, j0, j1);
         if (!queue) break;
     }
 }

@@ -409,4 +409,40 @@ bool all_finite(const double* v, size_t n)
     return ok;
 }
 
+ItemLayout item_layout(int nx, int ny, int rows, int32_t precision)
+{
+    ItemLayout L;
+    L.strips = (nx + kStripCells - 1) / kStripCells;
+    L.rows = rows;
+    L.split = ny;
+    L.tail_rows = rows;
+    constexpr int kTailRows = 4;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const long long slots = (long long)cus * 4 * (precision == MP_PREC_MIXED ? 3 : 2);
+    const long long uniform = (long long)L.strips * ((ny + rows - 1) / rows);
+    if (rows > kTailRows && uniform >= 3 * slots) {
+        const long long bands = (2 * slots + L.strips - 1) / L.strips;
+        const long long split = ny - bands * kTailRows;
+        if (split >= ny / 2) {
+            L.split = int(split);
+            L.tail_rows = kTailRows;
+        }
+    }
+    L.nlong = L.strips * ((L.split + rows - 1) / rows);
+    L.nitems = L.nlong + L.strips * ((ny - L.split + L.tail_rows - 1) / L.tail_rows);
+    return L;
+}
+
+void apply_layout(const ItemLayout& L, StagePtrs& sp)
+{
+    sp.rows = L.rows;
+    sp.strips = L.strips;
+    sp.split = L.split;
+    sp.tail_rows = L.tail_rows;
+    sp.nlong = L.nlong;
+    sp.pad = 0;
+}
+
 }  // namespace mp

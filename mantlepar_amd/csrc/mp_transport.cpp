@@ -122,9 +122,9 @@ mp_status mp_transport::sync_md()
 void mp_transport::set_tile(int r)
 {
     rows = r < 4 ? 4 : r;
-    strips = (mesh.nx + kStripCells - 1) / kStripCells;
-    bands = (mesh.ny + rows - 1) / rows;
-    nitems = strips * bands;
+    layout = item_layout(mesh.nx, mesh.ny, rows, precision);
+    strips = layout.strips;
+    nitems = layout.nitems;
 }
 
 mp_status mp_transport::ensure_scratch(int nsteps)
@@ -258,8 +258,7 @@ StagePtrs mp_transport::stage_ptrs(int src, int kin, int out, int scheme, int st
     }
     sp.partials = partials.as<double>() + size_t(part_stage) * nitems * kPartials;
     sp.error = reinterpret_cast<uint32_t*>(error.ptr);
-    sp.rows = rows;
-    sp.strips = strips;
+    apply_layout(layout, sp);
     return sp;
 }
 

@@ -33,7 +33,8 @@ struct mp_transport {
     int phase_cur = 0;
     bool phase_valid = false;
     mp::DeviceBuffer partials, totals, error;
-    int rows = 64, strips = 1, bands = 1, nitems = 1;
+    int rows = 64, strips = 1, nitems = 1;
+    mp::ItemLayout layout{};
 
     mp_transport() = default;
     mp_transport(const mp_transport&) = delete;

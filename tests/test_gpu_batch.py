@@ -114,6 +114,24 @@ def test_batch_matches_single_mesh_bitwise(prec, graph):
 
 
 @pytest.mark.parametrize("prec", [M.PREC_FP64, M.PREC_MIXED], ids=["fp64", "mixed"])
+def test_batch_tail_items_match_single_mesh_bitwise(prec):
+    """A mesh large enough to give every wave slot three items keeps its top rows
+    for 4-row tail items; the batch and the single-mesh handle share that layout,
+    so states, phase outputs and step totals agree bit for bit."""
+    specs = [W.MeshSpec(2048, 3400, seed=19), W.MeshSpec(130, 97, seed=13)]
+    b = M.Batch(4, phase=W.config_phase(), precision=prec)
+    ids = fill_batch(b, specs)
+    b.set_schedule(12, 0)
+    reps = b.step(3)
+    for k, spec in zip(ids, specs):
+        ref = single_run(spec, prec, 3, rows=12)
+        assert_same(b, k, ref)
+        for n in ("dt", "total_H", "total_C", "boundary_H", "boundary_C"):
+            assert reps[k][n] == ref[3][n], (k, n)
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [M.PREC_FP64, M.PREC_MIXED], ids=["fp64", "mixed"])
 def test_batch_lithostatic_matches_single_mesh_bitwise(prec):
     """Lithostatic phase pressure on two meshes of a batch (different surfaces),
     the constant pressure on the others: phase outputs and states bit for bit

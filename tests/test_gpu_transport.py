@@ -139,6 +139,27 @@ def test_results_do_not_depend_on_tile_shape():
         assert np.array_equal(h, out[0][0]) and np.array_equal(c, out[0][1])
 
 
+@pytest.mark.parametrize("prec", [M.PREC_FP64, M.PREC_MIXED], ids=["fp64", "mixed"])
+def test_tail_items_leave_results_unchanged(prec):
+    """On a mesh that fills the chip, 12-row items leave the top rows to 4-row
+    tail items dispatched last; 4-row items have no tail.  The states are the
+    same bit for bit."""
+    spec = W.MeshSpec(2048, 3400, seed=19)
+    H, Cc = W.enthalpy(spec), W.composition(spec)
+    out = []
+    for rows in (4, 12):
+        t = M.Transport(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q,
+                        phase=W.config_phase(), precision=prec)
+        t.upload(H, Cc)
+        t.set_face_velocity(*W.face_velocity(spec))
+        t.set_tile(rows)
+        t.step(2, t.stable_dt(0.4))
+        out.append(t.download() + (t.phase_state()["T"],))
+        t.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
 def test_backflow_error_leaves_state_unchanged():
     spec = W.MeshSpec(16, 8, seed=3)
     vx = np.full((spec.q, spec.ny, spec.nx + 1), 0.3)     # inflow through the left side
