@@ -51,6 +51,13 @@ mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
         MP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         own_stream = true;
     }
+    if (prec == MP_PREC_MIXED) {
+        MP_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        for (int k = 0; k < 2; ++k) {
+            MP_HIP(hipEventCreateWithFlags(&stage3[k], hipEventDisableTiming));
+            MP_HIP(hipEventCreateWithFlags(&side_done[k], hipEventDisableTiming));
+        }
+    }
     for (int k = 0; k < 5; ++k) {
         MP_CALL(H[k].alloc(N * sizeof(double)));
         MP_CALL(C[k].alloc(N * sizeof(double)));
@@ -109,6 +116,12 @@ int mp_transport::default_rows() const
 mp_transport::~mp_transport()
 {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (side) (void)hipStreamSynchronize(side);
+    for (int k = 0; k < 2; ++k) {
+        if (stage3[k]) (void)hipEventDestroy(stage3[k]);
+        if (side_done[k]) (void)hipEventDestroy(side_done[k]);
+    }
+    if (side) (void)hipStreamDestroy(side);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -129,8 +142,9 @@ void mp_transport::set_tile(int r)
 
 mp_status mp_transport::ensure_scratch(int nsteps)
 {
-    // stage partials, then the step reduction's block results
-    const size_t pbytes = (size_t(3) * nitems * kPartials + size_t(kStepSlices) * 6) * sizeof(double);
+    // stage partials of two consecutive steps, then the step reduction's
+    // block results
+    const size_t pbytes = (size_t(6) * nitems * kPartials + size_t(kStepSlices) * 6) * sizeof(double);
     if (partials.bytes < pbytes) MP_CALL(partials.alloc(pbytes));
     const size_t tbytes = size_t(nsteps > 0 ? nsteps : 1) * 4 * sizeof(double);
     if (totals.bytes < tbytes) MP_CALL(totals.alloc(tbytes));
@@ -263,6 +277,15 @@ StagePtrs mp_transport::stage_ptrs(int src, int kin, int out, int scheme, int st
 }
 
 // Enqueue nsteps; returns the index of the final state buffer.
+//
+// The mixed path runs each step's phase kernel and step totals on the side
+// stream, beside the next step's stages: the phase reads the step's output
+// state, which the next step only reads, and writes the phase outputs and the
+// sum slots of the step's partials.  Partials alternate between two sets by
+// step parity, and step s + 2, which writes step s's output buffer and
+// partial set again, first waits for step s's phase and totals.  The call
+// ends with the main stream waiting for the side stream.  Profiling (events)
+// keeps everything on the main stream so each launch is timed alone.
 mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& final_state,
                                       std::vector<hipEvent_t>* events)
 {
@@ -272,28 +295,41 @@ mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& fi
     int src = cur;
     final_state = cur;
     const int phase_set = 1 - phase_cur;
+    const bool overlap = precision == MP_PREC_MIXED && side && !events;
+    const size_t set_doubles = size_t(nitems) * kPartials;
+    double* scratch = partials.as<double>() + 6 * set_doubles;
     for (int s = 0; s < nsteps; ++s) {
         const int dst = others[s % 2];
+        const int par = overlap ? s % 2 : 0;
+        if (overlap && s >= 2) MP_HIP(hipStreamWaitEvent(stream, side_done[par], 0));
         int kin = src;
         for (int st = 0; st < scheme; ++st) {
             const bool last = st == scheme - 1;
             const int out = last ? dst : 3 + st;
-            const StagePtrs sp = stage_ptrs(src, kin, out, scheme, st, dt, last, phase_set, st);
+            const StagePtrs sp = stage_ptrs(src, kin, out, scheme, st, dt, last, phase_set, par * 3 + st);
             // the mixed sweep leaves the phase to its own streaming kernel
             const bool split = last && precision == MP_PREC_MIXED;
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
             MP_HIP(launch_stage(precision, last && !split, mesh.q, d_md.as<MeshDev>(), sp, nitems,
                                 stream, md.limit != 0, has_diffusion(), md.lf_mode != 0));
-            if (split) MP_HIP(launch_phase_items(d_md.as<MeshDev>(), sp, nitems, stream));
+            if (split && overlap) {
+                MP_HIP(hipEventRecord(stage3[par], stream));
+                MP_HIP(hipStreamWaitEvent(side, stage3[par], 0));
+                MP_HIP(launch_phase_items(d_md.as<MeshDev>(), sp, nitems, side));
+            } else if (split) {
+                MP_HIP(launch_phase_items(d_md.as<MeshDev>(), sp, nitems, stream));
+            }
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
             kin = out;
         }
-        MP_HIP(launch_reduce_step(partials.as<double>(), nitems, scheme, kW[scheme - 1], md.area,
-                                  totals.as<double>() + size_t(s) * 4,
-                                  partials.as<double>() + size_t(3) * nitems * kPartials, stream));
+        MP_HIP(launch_reduce_step(partials.as<double>() + size_t(par) * 3 * set_doubles, nitems, scheme,
+                                  kW[scheme - 1], md.area, totals.as<double>() + size_t(s) * 4, scratch,
+                                  overlap ? side : stream));
+        if (overlap) MP_HIP(hipEventRecord(side_done[par], side));
         src = dst;
         final_state = dst;
     }
+    if (overlap && nsteps > 0) MP_HIP(hipStreamWaitEvent(stream, side_done[(nsteps - 1) % 2], 0));
     return MP_OK;
 }
 

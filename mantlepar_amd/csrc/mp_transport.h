@@ -33,6 +33,11 @@ struct mp_transport {
     int phase_cur = 0;
     bool phase_valid = false;
     mp::DeviceBuffer partials, totals, error;
+    // Mixed path: each step's phase kernel and totals run on `side`, beside the
+    // next step's stages on `stream` (stage3: the step's last stage is done;
+    // side_done: its phase and totals are done; by step parity).
+    hipStream_t side = nullptr;
+    hipEvent_t stage3[2] = {nullptr, nullptr}, side_done[2] = {nullptr, nullptr};
     int rows = 64, strips = 1, nitems = 1;
     mp::ItemLayout layout{};
 
