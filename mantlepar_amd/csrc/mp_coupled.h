@@ -45,7 +45,7 @@ struct CoupledDev {
 hipError_t launch_coupled_rhs(const CoupledDev& d, int nx, int ny, const double* H, const double* C,
                               double* rhsH, double* rhsC, double* rate, double* div, double* bnd,
                               hipStream_t stream);
-// k reductions of length-n arrays: array a is x + src[a]*n (src null: x + a*n),
+// k reductions (1 <= k <= 6) of length-n arrays: array a is x + src[a]*n (src null: x + a*n),
 // reduced by kind[a]: 0 compensated sum, 1 max |x|, 2 min, 3 max (NaN
 // propagates); kind and src in device memory; part: scratch of
 // 2 * k * kReducePartials doubles.

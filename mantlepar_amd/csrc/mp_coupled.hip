@@ -403,6 +403,10 @@ __device__ void block_reduce(int mode, double& s, double& e, double* ss, double*
     __syncthreads();
 }
 
+// Every array of the chunk is read in the same loop, so a thread has loads
+// of all k arrays in flight at once (k <= kReduceMax).
+constexpr int kReduceMax = 6;
+
 __global__ __launch_bounds__(256) void reduce_partial_kernel(const double* __restrict__ x, int64_t n,
                                                              int k, const int* kind, const int* src,
                                                              double* part)
@@ -410,19 +414,36 @@ __global__ __launch_bounds__(256) void reduce_partial_kernel(const double* __res
     __shared__ double ss[256], se[256];
     const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const int64_t b0 = int64_t(blockIdx.x) * chunk, b1 = min(n, b0 + chunk);
-    for (int a = 0; a < k; ++a) {
-        const double* v = x + (src ? src[a] : a) * n;
-        const int mode = kind[a];
-        double s = red_init(mode), e = 0.0;
-        for (int64_t t = b0 + threadIdx.x; t < b1; t += blockDim.x) {
-            const double y = v[t];
-            if (mode == 0) two_sum_acc(s, e, y);
-            else red_combine(mode, s, e, y, 0.0);
+    const double* v[kReduceMax];
+    int mode[kReduceMax];
+    double s[kReduceMax], e[kReduceMax];
+#pragma unroll
+    for (int a = 0; a < kReduceMax; ++a) {
+        const int aa = a < k ? a : 0;
+        v[a] = x + (src ? src[aa] : aa) * n;
+        mode[a] = kind[aa];
+        s[a] = red_init(mode[a]);
+        e[a] = 0.0;
+    }
+    for (int64_t t = b0 + threadIdx.x; t < b1; t += blockDim.x) {
+        double y[kReduceMax];
+#pragma unroll
+        for (int a = 0; a < kReduceMax; ++a) y[a] = a < k ? v[a][t] : 0.0;
+#pragma unroll
+        for (int a = 0; a < kReduceMax; ++a) {
+            if (a >= k) break;
+            if (mode[a] == 0) two_sum_acc(s[a], e[a], y[a]);
+            else red_combine(mode[a], s[a], e[a], y[a], 0.0);
         }
-        block_reduce(mode, s, e, ss, se);
+    }
+#pragma unroll
+    for (int a = 0; a < kReduceMax; ++a) {
+        if (a >= k) break;
+        double sa = s[a], ea = e[a];
+        block_reduce(mode[a], sa, ea, ss, se);
         if (threadIdx.x == 0) {
-            part[(size_t(a) * gridDim.x + blockIdx.x) * 2] = s;
-            part[(size_t(a) * gridDim.x + blockIdx.x) * 2 + 1] = e;
+            part[(size_t(a) * gridDim.x + blockIdx.x) * 2] = sa;
+            part[(size_t(a) * gridDim.x + blockIdx.x) * 2 + 1] = ea;
         }
     }
 }
@@ -431,13 +452,27 @@ __global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restr
                                                            int k, const int* kind, double* out)
 {
     __shared__ double ss[256], se[256];
-    for (int a = 0; a < k; ++a) {
-        const int mode = kind[a];
-        double s = red_init(mode), e = 0.0;
-        for (int t = threadIdx.x; t < nb; t += blockDim.x)
-            red_combine(mode, s, e, part[(size_t(a) * nb + t) * 2], part[(size_t(a) * nb + t) * 2 + 1]);
-        block_reduce(mode, s, e, ss, se);
-        if (threadIdx.x == 0) out[a] = mode == 0 ? s + e : s;
+    int mode[kReduceMax];
+    double s[kReduceMax], e[kReduceMax];
+#pragma unroll
+    for (int a = 0; a < kReduceMax; ++a) {
+        mode[a] = kind[a < k ? a : 0];
+        s[a] = red_init(mode[a]);
+        e[a] = 0.0;
+    }
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+#pragma unroll
+        for (int a = 0; a < kReduceMax; ++a) {
+            if (a >= k) break;
+            red_combine(mode[a], s[a], e[a], part[(size_t(a) * nb + t) * 2], part[(size_t(a) * nb + t) * 2 + 1]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < kReduceMax; ++a) {
+        if (a >= k) break;
+        double sa = s[a], ea = e[a];
+        block_reduce(mode[a], sa, ea, ss, se);
+        if (threadIdx.x == 0) out[a] = mode[a] == 0 ? sa + ea : sa;
     }
 }
 
@@ -474,7 +509,8 @@ hipError_t launch_coupled_rhs(const CoupledDev& d, int nx, int ny, const double*
 hipError_t launch_reduce(const double* x, int64_t n, int k, const int* kind, const int* src,
                          double* out, double* part, hipStream_t stream)
 {
-    const int nb = int(std::min<int64_t>(kReducePartials, std::max<int64_t>(1, (n + 4095) / 4096)));
+    if (k < 1 || k > kReduceMax) return hipErrorInvalidValue;
+    const int nb = int(std::min<int64_t>(kReducePartials, std::max<int64_t>(1, (n + 2047) / 2048)));
     reduce_partial_kernel<<<nb, 256, 0, stream>>>(x, n, k, kind, src, part);
     reduce_final_kernel<<<1, 256, 0, stream>>>(part, nb, k, kind, out);
     return hipGetLastError();
