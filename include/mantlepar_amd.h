@@ -268,7 +268,7 @@ mp_status mp_batch_info(mp_batch* b, int32_t* meshes, int64_t* cells, int64_t* i
 /* Continuous batching of transport runs.  Requests arrive at any time from any
  * thread; a server thread admits them into slots of one device engine while
  * slots and the HBM budget allow, advances all resident runs together in
- * epochs of at most max_epoch_steps steps (one persistent launch per stage),
+ * epochs of at most max_epoch_steps steps (one launch per stage),
  * and retires each run after its last step: outputs are written to the
  * caller's arrays and the slot and memory are released.  A run's results do
  * not depend on which other runs shared its launches.  No reference
