@@ -13,6 +13,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -46,7 +47,8 @@ def parse():
     p.add_argument("--coupled-ny", type=int, default=2048)
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--no-serve", action="store_true")
-    p.add_argument("--tune-cache", default=os.path.join(HERE, "gpurun_out", "batch_tune.json"))
+    p.add_argument("--tune-cache", default=os.path.join(tempfile.gettempdir(), "mantlepar_batch_tune.json"),
+                   help="JSON file where the batch autotuner caches its choices")
     p.add_argument("--copy-calibration", action="store_true",
                    help="run only a plain 1 GiB device copy (read 1 GiB, write 1 GiB) so that "
                         "FETCH_SIZE/WRITE_SIZE counters can be scaled")

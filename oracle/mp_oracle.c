@@ -2053,7 +2053,7 @@ double mpo_transport_integral(const mpo_transport* t, const double* u)
 
 /* Stage combination u_out = a*u_n + b*(u_k + dt*L).  Forward Euler and SSPRK2
  * follow src/couple/preheat.cpp:267 and src/couple/evolve.cpp:469-479; SSPRK3
- * (Shu-Osher) is this build's addition. */
+ * (Shu-Osher) has no counterpart in the reference, which steps with SSPRK2. */
 static void combine(size_t N, double a, const double* un, double b, const double* uk,
                     double dt, const double* L, double* out)
 {
