@@ -57,6 +57,7 @@ mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
             MP_HIP(hipEventCreateWithFlags(&stage3[k], hipEventDisableTiming));
             MP_HIP(hipEventCreateWithFlags(&side_done[k], hipEventDisableTiming));
         }
+        MP_HIP(hipEventCreateWithFlags(&side_join, hipEventDisableTiming));
     }
     for (int k = 0; k < 5; ++k) {
         MP_CALL(H[k].alloc(N * sizeof(double)));
@@ -121,6 +122,7 @@ mp_transport::~mp_transport()
         if (stage3[k]) (void)hipEventDestroy(stage3[k]);
         if (side_done[k]) (void)hipEventDestroy(side_done[k]);
     }
+    if (side_join) (void)hipEventDestroy(side_join);
     if (side) (void)hipStreamDestroy(side);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
@@ -296,6 +298,17 @@ mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& fi
     final_state = cur;
     const int phase_set = 1 - phase_cur;
     const bool overlap = precision == MP_PREC_MIXED && side && !events;
+    // However the call ends, error returns included, the main stream waits for
+    // everything enqueued on the side stream.
+    struct Join {
+        hipStream_t main, side;
+        hipEvent_t ev;
+        bool on;
+        ~Join()
+        {
+            if (on && hipEventRecord(ev, side) == hipSuccess) (void)hipStreamWaitEvent(main, ev, 0);
+        }
+    } join{stream, side, side_join, overlap && nsteps > 0};
     const size_t set_doubles = size_t(nitems) * kPartials;
     double* scratch = partials.as<double>() + 6 * set_doubles;
     for (int s = 0; s < nsteps; ++s) {
@@ -329,7 +342,6 @@ mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& fi
         src = dst;
         final_state = dst;
     }
-    if (overlap && nsteps > 0) MP_HIP(hipStreamWaitEvent(stream, side_done[(nsteps - 1) % 2], 0));
     return MP_OK;
 }
 

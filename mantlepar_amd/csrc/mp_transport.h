@@ -38,6 +38,7 @@ struct mp_transport {
     // side_done: its phase and totals are done; by step parity).
     hipStream_t side = nullptr;
     hipEvent_t stage3[2] = {nullptr, nullptr}, side_done[2] = {nullptr, nullptr};
+    hipEvent_t side_join = nullptr;   // the main stream waits on it when a call ends
     int rows = 64, strips = 1, nitems = 1;
     mp::ItemLayout layout{};
 
