@@ -138,15 +138,17 @@ static int items_for(const mp_mesh_desc& d, int rows)
     return strips * ((d.ny + rows - 1) / rows) + strips * ((d.ny + 3) / 4);
 }
 
-size_t Engine::slot_bytes(const mp_mesh_desc& d, int rows_min)
+size_t Engine::slot_bytes(const mp_mesh_desc& d, int32_t precision, int rows_min)
 {
     const size_t N = size_t(d.nx) * d.ny;
     size_t b = 0;
     b += Arena::round(10 * N * sizeof(double));
     b += Arena::round(size_t(d.q) * faces_x(d) * sizeof(double));
     b += Arena::round(size_t(d.q) * faces_y(d) * sizeof(double));
-    b += Arena::round(faces_x(d) * sizeof(FaceMom)) + Arena::round(faces_y(d) * sizeof(FaceMom));
-    b += Arena::round(faces_x(d) * sizeof(FaceRec)) + Arena::round(faces_y(d) * sizeof(FaceRec));
+    if (precision == MP_PREC_MIXED) {   // face moment records, full and one-sided
+        b += Arena::round(faces_x(d) * sizeof(FaceMom)) + Arena::round(faces_y(d) * sizeof(FaceMom));
+        b += Arena::round(faces_x(d) * sizeof(FaceRec)) + Arena::round(faces_y(d) * sizeof(FaceRec));
+    }
     b += 2 * Arena::round(N * (4 * sizeof(double) + 1));
     b += Arena::round(size_t(3) * items_for(d, rows_min) * kPartials * sizeof(double));
     b += Arena::round(size_t(2) * (d.nx + d.ny));
@@ -165,7 +167,7 @@ mp_status Engine::add_mesh(const mp_mesh_desc& d, int* out)
     MP_CALL(make_mesh_dev(d, weno, phase, sl.md));
     sl.desc = d;
     sl.N = size_t(d.nx) * d.ny;
-    MP_CALL(arena.alloc(slot_bytes(d), &sl.block));
+    MP_CALL(arena.alloc(slot_bytes(d, precision), &sl.block));
     char* p = static_cast<char*>(sl.block);
     auto take = [&](size_t bytes) { char* r = p; p += Arena::round(bytes); return r; };
     const size_t N = sl.N;
@@ -176,10 +178,12 @@ mp_status Engine::add_mesh(const mp_mesh_desc& d, int* out)
     }
     sl.md.vnx = reinterpret_cast<double*>(take(size_t(d.q) * faces_x(d) * sizeof(double)));
     sl.md.vny = reinterpret_cast<double*>(take(size_t(d.q) * faces_y(d) * sizeof(double)));
-    sl.md.fx = reinterpret_cast<FaceMom*>(take(faces_x(d) * sizeof(FaceMom)));
-    sl.md.fy = reinterpret_cast<FaceMom*>(take(faces_y(d) * sizeof(FaceMom)));
-    sl.md.rx = reinterpret_cast<FaceRec*>(take(faces_x(d) * sizeof(FaceRec)));
-    sl.md.ry = reinterpret_cast<FaceRec*>(take(faces_y(d) * sizeof(FaceRec)));
+    if (precision == MP_PREC_MIXED) {
+        sl.md.fx = reinterpret_cast<FaceMom*>(take(faces_x(d) * sizeof(FaceMom)));
+        sl.md.fy = reinterpret_cast<FaceMom*>(take(faces_y(d) * sizeof(FaceMom)));
+        sl.md.rx = reinterpret_cast<FaceRec*>(take(faces_x(d) * sizeof(FaceRec)));
+        sl.md.ry = reinterpret_cast<FaceRec*>(take(faces_y(d) * sizeof(FaceRec)));
+    }
     sl.phase[0] = reinterpret_cast<double*>(take(N * (4 * sizeof(double) + 1)));
     sl.phase[1] = reinterpret_cast<double*>(take(N * (4 * sizeof(double) + 1)));
     sl.partials = reinterpret_cast<double*>(take(size_t(3) * items_for(d, 8) * kPartials * sizeof(double)));
@@ -271,8 +275,9 @@ mp_status Engine::set_velocity(int k, const double* vx, const double* vy, bool w
     MP_HIP(hipMemcpyAsync(const_cast<double*>(s.md.vnx), vx, nxv * sizeof(double), hipMemcpyHostToDevice, stream));
     MP_HIP(hipMemcpyAsync(const_cast<double*>(s.md.vny), vy, nyv * sizeof(double), hipMemcpyHostToDevice, stream));
     MeshDev* dm = d_meshes.as<MeshDev>() + k;
-    MP_HIP(launch_face_prep(dm, d.nx, d.ny, const_cast<FaceMom*>(s.md.fx), const_cast<FaceMom*>(s.md.fy),
-                            const_cast<FaceRec*>(s.md.rx), const_cast<FaceRec*>(s.md.ry), stream));
+    if (precision == MP_PREC_MIXED)
+        MP_HIP(launch_face_prep(dm, d.nx, d.ny, const_cast<FaceMom*>(s.md.fx), const_cast<FaceMom*>(s.md.fy),
+                                const_cast<FaceRec*>(s.md.rx), const_cast<FaceRec*>(s.md.ry), stream));
     unsigned long long* bits = d_maxbits.as<unsigned long long>() + k;
     MP_CALL(launch_clear(reinterpret_cast<uint32_t*>(bits), 2, stream) == hipSuccess
                 ? MP_OK : fail(MP_ERR_DEVICE, "clear failed"));

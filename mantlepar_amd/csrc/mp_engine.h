@@ -76,7 +76,7 @@ public:
 
     mp_status init(int capacity, int32_t precision, const mp_weno_options& weno,
                    const mp_phase_params& phase, void* stream, size_t budget_bytes);
-    static size_t slot_bytes(const mp_mesh_desc& d, int rows_min = 8);
+    static size_t slot_bytes(const mp_mesh_desc& d, int32_t precision, int rows_min = 8);
     mp_status add_mesh(const mp_mesh_desc& d, int* slot);
     void remove_mesh(int slot);
     mp_status upload(int slot, const double* H, const double* C);

@@ -335,7 +335,7 @@ mp_status mp_server_submit(mp_server* s, const mp_request* req, int64_t* id)
         if (req->nsteps < 0) return fail(MP_ERR_ARG, "nsteps must be nonnegative");
         if (!std::isfinite(req->cfl) || !(req->cfl > 0)) return fail(MP_ERR_ARG, "CFL must be finite and positive");
         MP_CALL(validate_mesh(req->mesh));
-        const size_t bytes = Engine::slot_bytes(req->mesh);
+        const size_t bytes = Engine::slot_bytes(req->mesh, s->eng.precision);
         if (bytes > s->eng.arena.budget())
             return fail(MP_ERR_MEMORY, "request needs %zu bytes, above the HBM budget %zu", bytes,
                         s->eng.arena.budget());

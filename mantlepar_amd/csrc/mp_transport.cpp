@@ -68,15 +68,17 @@ mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
     }
     MP_CALL(vnx.alloc(size_t(m.q) * m.ny * (m.nx + 1) * sizeof(double)));
     MP_CALL(vny.alloc(size_t(m.q) * (m.ny + 1) * m.nx * sizeof(double)));
-    const size_t nxf = size_t(m.ny) * (m.nx + 1), nyf = size_t(m.ny + 1) * m.nx;
-    MP_CALL(fx.alloc(nxf * sizeof(FaceMom)));
-    MP_CALL(fy.alloc(nyf * sizeof(FaceMom)));
-    MP_CALL(rx.alloc(nxf * sizeof(FaceRec)));
-    MP_CALL(ry.alloc(nyf * sizeof(FaceRec)));
-    md.fx = fx.as<FaceMom>();
-    md.fy = fy.as<FaceMom>();
-    md.rx = rx.as<FaceRec>();
-    md.ry = ry.as<FaceRec>();
+    if (prec == MP_PREC_MIXED) {   // face moment records: the mixed sweep's flux input
+        const size_t nxf = size_t(m.ny) * (m.nx + 1), nyf = size_t(m.ny + 1) * m.nx;
+        MP_CALL(fx.alloc(nxf * sizeof(FaceMom)));
+        MP_CALL(fy.alloc(nyf * sizeof(FaceMom)));
+        MP_CALL(rx.alloc(nxf * sizeof(FaceRec)));
+        MP_CALL(ry.alloc(nyf * sizeof(FaceRec)));
+        md.fx = fx.as<FaceMom>();
+        md.fy = fy.as<FaceMom>();
+        md.rx = rx.as<FaceRec>();
+        md.ry = ry.as<FaceRec>();
+    }
     MP_CALL(error.alloc(sizeof(uint64_t) * 2));
     MP_CALL(d_md.alloc(sizeof(MeshDev)));
     MP_HIP(hipMemsetAsync(H[0].ptr, 0, N * sizeof(double), stream));
@@ -187,9 +189,9 @@ mp_status mp_transport::set_velocity(const double* x, const double* y)
     }
     MP_HIP(hipMemcpyAsync(vnx.ptr, x, nxv * sizeof(double), hipMemcpyHostToDevice, stream));
     MP_HIP(hipMemcpyAsync(vny.ptr, y, nyv * sizeof(double), hipMemcpyHostToDevice, stream));
-    MP_HIP(launch_face_prep(d_md.as<MeshDev>(), mesh.nx, mesh.ny, fx.as<FaceMom>(), fy.as<FaceMom>(),
-                            rx.as<FaceRec>(), ry.as<FaceRec>(),
-                            stream));
+    if (precision == MP_PREC_MIXED)
+        MP_HIP(launch_face_prep(d_md.as<MeshDev>(), mesh.nx, mesh.ny, fx.as<FaceMom>(), fy.as<FaceMom>(),
+                                rx.as<FaceRec>(), ry.as<FaceRec>(), stream));
     MP_CALL(compute_rate());
     have_velocity = true;
     return MP_OK;
