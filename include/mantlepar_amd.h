@@ -139,6 +139,21 @@ mp_status mp_transport_stable_dt(mp_transport* t, double cfl, double* dt);
  * are asynchronous; the report (may be NULL) synchronises. */
 mp_status mp_transport_step(mp_transport* t, int32_t nsteps, int32_t scheme, double dt,
                             mp_step_report* report);
+/* mp_transport_step without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The step's work memory (stage partials, the step reduction's block results,
+ * per-step totals and the error word) lives in a static 32 MiB device array of
+ * the library instead of buffers the handle allocates; the states, face data
+ * and phase outputs are the handle's, allocated at create time.  Results are
+ * bitwise those of mp_transport_step.  A call needing more than 32 MiB (384
+ * bytes per work item plus 32 per step: an 8192 x 8192 mesh fits) fails with
+ * MP_ERR_MEMORY.  The
+ * handle's device must be current.  Each GPU holds one such array, shared by
+ * every call on it: calls that overlap on one GPU (several handles on
+ * different threads or streams) race; serialise them. */
+mp_status mp_transport_step_noalloc(mp_transport* t, int32_t nsteps, int32_t scheme, double dt,
+                                    mp_step_report* report);
 /* Phase of the latest step (or of the uploaded state, computed on demand). */
 mp_status mp_transport_phase(mp_transport* t, double* T, double* phi, double* Cl, double* Cs,
                              uint8_t* region);

@@ -143,6 +143,8 @@ SIGNATURES = [
     ("mp_transport_stable_dt", C.c_int32, [_vp, C.c_double, _dp]),
     ("mp_transport_step", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double,
                                       C.POINTER(StepReport)]),
+    ("mp_transport_step_noalloc", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double,
+                                              C.POINTER(StepReport)]),
     ("mp_transport_phase", C.c_int32, [_vp, _dp, _dp, _dp, _dp, _u8p]),
     ("mp_transport_rhs", C.c_int32, [_vp, _dp, _dp, _dp]),
     ("mp_transport_reconstruct", C.c_int32, [_vp, C.c_int32, _dp, _dp, _dp]),

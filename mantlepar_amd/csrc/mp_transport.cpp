@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -44,6 +45,7 @@ mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
     phase_params = p;
     precision = prec;
     N = size_t(m.nx) * size_t(m.ny);
+    MP_HIP(hipGetDevice(&device));
     if (s) {
         stream = static_cast<hipStream_t>(s);
         own_stream = false;
@@ -144,6 +146,17 @@ void mp_transport::set_tile(int r)
     nitems = layout.nitems;
 }
 
+// Work memory of a step over nitems work items and nsteps steps: two sets of
+// three stage partials, the step reduction's block results, the per-step
+// totals and the error word (the last two 256-byte aligned).
+size_t mp_transport::work_bytes(int nitems, int nsteps)
+{
+    const size_t part = (size_t(6) * nitems * kPartials + size_t(kStepSlices) * 6) * sizeof(double);
+    const size_t tot = size_t(nsteps > 0 ? nsteps : 1) * 4 * sizeof(double);
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    return up(part) + up(tot) + 256;
+}
+
 mp_status mp_transport::ensure_scratch(int nsteps)
 {
     // stage partials of two consecutive steps, then the step reduction's
@@ -153,6 +166,11 @@ mp_status mp_transport::ensure_scratch(int nsteps)
     const size_t tbytes = size_t(nsteps > 0 ? nsteps : 1) * 4 * sizeof(double);
     if (totals.bytes < tbytes) MP_CALL(totals.alloc(tbytes));
     return MP_OK;
+}
+
+mp_transport::StepWork mp_transport::own_work() const
+{
+    return StepWork{partials.as<double>(), totals.as<double>(), reinterpret_cast<uint32_t*>(error.ptr)};
 }
 
 mp_status mp_transport::upload(const double* h, const double* c)
@@ -252,8 +270,8 @@ mp_status mp_transport::set_inflow(const double* h, const double* c)
     return sync_md();
 }
 
-StagePtrs mp_transport::stage_ptrs(int src, int kin, int out, int scheme, int st, double dt,
-                                   bool phase_stage, int phase_set, int part_stage) const
+StagePtrs mp_transport::stage_ptrs(const StepWork& w, int src, int kin, int out, int scheme, int st,
+                                   double dt, bool phase_stage, int phase_set, int part_stage) const
 {
     StagePtrs sp;
     std::memset(&sp, 0, sizeof sp);
@@ -274,8 +292,8 @@ StagePtrs mp_transport::stage_ptrs(int src, int kin, int out, int scheme, int st
         sp.Cs = base + 3 * N;
         sp.region = reinterpret_cast<uint8_t*>(base + 4 * N);
     }
-    sp.partials = partials.as<double>() + size_t(part_stage) * nitems * kPartials;
-    sp.error = reinterpret_cast<uint32_t*>(error.ptr);
+    sp.partials = w.partials + size_t(part_stage) * nitems * kPartials;
+    sp.error = w.error;
     apply_layout(layout, sp);
     return sp;
 }
@@ -290,8 +308,8 @@ StagePtrs mp_transport::stage_ptrs(int src, int kin, int out, int scheme, int st
 // partial set again, first waits for step s's phase and totals.  The call
 // ends with the main stream waiting for the side stream.  Profiling (events)
 // keeps everything on the main stream so each launch is timed alone.
-mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& final_state,
-                                      std::vector<hipEvent_t>* events)
+mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme, double dt,
+                                      int& final_state, std::vector<hipEvent_t>* events)
 {
     int others[2], n = 0;
     for (int k = 0; k < 3; ++k)
@@ -312,7 +330,7 @@ mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& fi
         }
     } join{stream, side, side_join, overlap && nsteps > 0};
     const size_t set_doubles = size_t(nitems) * kPartials;
-    double* scratch = partials.as<double>() + 6 * set_doubles;
+    double* scratch = w.partials + 6 * set_doubles;
     for (int s = 0; s < nsteps; ++s) {
         const int dst = others[s % 2];
         const int par = overlap ? s % 2 : 0;
@@ -321,7 +339,7 @@ mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& fi
         for (int st = 0; st < scheme; ++st) {
             const bool last = st == scheme - 1;
             const int out = last ? dst : 3 + st;
-            const StagePtrs sp = stage_ptrs(src, kin, out, scheme, st, dt, last, phase_set, par * 3 + st);
+            const StagePtrs sp = stage_ptrs(w, src, kin, out, scheme, st, dt, last, phase_set, par * 3 + st);
             // the mixed sweep leaves the phase to its own streaming kernel
             const bool split = last && precision == MP_PREC_MIXED;
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
@@ -337,8 +355,8 @@ mp_status mp_transport::enqueue_steps(int nsteps, int scheme, double dt, int& fi
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
             kin = out;
         }
-        MP_HIP(launch_reduce_step(partials.as<double>() + size_t(par) * 3 * set_doubles, nitems, scheme,
-                                  kW[scheme - 1], md.area, totals.as<double>() + size_t(s) * 4, scratch,
+        MP_HIP(launch_reduce_step(w.partials + size_t(par) * 3 * set_doubles, nitems, scheme,
+                                  kW[scheme - 1], md.area, w.totals + size_t(s) * 4, scratch,
                                   overlap ? side : stream));
         if (overlap) MP_HIP(hipEventRecord(side_done[par], side));
         src = dst;
@@ -367,16 +385,22 @@ mp_status mp_transport::step(int nsteps, int scheme, double dt, mp_step_report* 
 {
     MP_CALL(check_step_args(nsteps, scheme, dt));
     MP_CALL(ensure_scratch(nsteps));
-    MP_HIP(hipMemsetAsync(error.ptr, 0, sizeof(uint32_t), stream));
+    const StepWork w = own_work();
+    MP_HIP(launch_clear(w.error, 1, stream));
     int final_state = cur;
-    MP_CALL(enqueue_steps(nsteps, scheme, dt, final_state, nullptr));
+    MP_CALL(enqueue_steps(w, nsteps, scheme, dt, final_state, nullptr));
+    return finish_step(w, nsteps, scheme, dt, final_state, report);
+}
+
+// Wait for the enqueued steps, commit the new state only if the error word is
+// clean, and fill the report from the per-step totals.  The totals are read
+// through a fixed host buffer, 64 steps at a time, so this allocates nothing.
+mp_status mp_transport::finish_step(const StepWork& w, int nsteps, int scheme, double dt,
+                                    int final_state, mp_step_report* report)
+{
+    (void)scheme;
     uint32_t bits = 0;
-    MP_HIP(hipMemcpyAsync(&bits, error.ptr, sizeof bits, hipMemcpyDeviceToHost, stream));
-    std::vector<double> tot;
-    if (report && nsteps > 0) {
-        tot.resize(size_t(nsteps) * 4);
-        MP_HIP(hipMemcpyAsync(tot.data(), totals.ptr, tot.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
-    }
+    MP_HIP(hipMemcpyAsync(&bits, w.error, sizeof bits, hipMemcpyDeviceToHost, stream));
     MP_HIP(hipStreamSynchronize(stream));
     if (bits) return fail(status_from_bits(bits), "%s (device error bits 0x%x); state unchanged",
                           describe_bits(bits), bits);
@@ -392,20 +416,86 @@ mp_status mp_transport::step(int nsteps, int scheme, double dt, mp_step_report* 
         report->courant = dt * max_rate;
         report->steps_done = nsteps;
         long double bh = 0, bc = 0;
-        for (int s = 0; s < nsteps; ++s) {
-            bh += (long double)dt * tot[size_t(s) * 4 + 2];
-            bc += (long double)dt * tot[size_t(s) * 4 + 3];
+        constexpr int kChunk = 64;
+        double tot[kChunk * 4];
+        for (int s0 = 0; s0 < nsteps; s0 += kChunk) {
+            const int n = nsteps - s0 < kChunk ? nsteps - s0 : kChunk;
+            MP_HIP(hipMemcpy(tot, w.totals + size_t(s0) * 4, size_t(n) * 4 * sizeof(double),
+                             hipMemcpyDeviceToHost));
+            for (int s = 0; s < n; ++s) {
+                bh += (long double)dt * tot[size_t(s) * 4 + 2];
+                bc += (long double)dt * tot[size_t(s) * 4 + 3];
+            }
+            if (s0 + n == nsteps) {
+                report->total_H = tot[size_t(n - 1) * 4 + 0];
+                report->total_C = tot[size_t(n - 1) * 4 + 1];
+            }
         }
         report->boundary_H = double(bh);
         report->boundary_C = double(bc);
-        if (nsteps > 0) {
-            report->total_H = tot[size_t(nsteps - 1) * 4 + 0];
-            report->total_C = tot[size_t(nsteps - 1) * 4 + 1];
-        } else {
-            MP_CALL(state_totals(report->total_H, report->total_C));
-        }
+        if (nsteps == 0) MP_CALL(state_totals(report->total_H, report->total_C));
     }
     return MP_OK;
+}
+
+// ------------------------------------------------- allocation-free stepping
+//
+// mp_transport_step_noalloc keeps a step's work memory (stage partials, the
+// step reduction's block results, per-step totals, the error word) in one
+// file-scope device array instead of buffers the handle allocates.  Each GPU
+// holds its own copy of the array at its own address until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device, never through the array's name.  Every call on one
+// GPU shares that GPU's array, so calls that overlap on one GPU race.
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+std::mutex scratch_mutex;
+void* scratch_addr[kMaxDevices] = {};
+
+// The scratch address on `device` (which must be current), looked up once.
+mp_status scratch_on(int device, void** out)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(MP_ERR_DEVICE, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MP_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        scratch_addr[device] = p;
+    }
+    *out = scratch_addr[device];
+    return MP_OK;
+}
+}  // namespace
+
+mp_status mp_transport::step_noalloc(int nsteps, int scheme, double dt, mp_step_report* report)
+{
+    MP_CALL(check_step_args(nsteps, scheme, dt));
+    const size_t need = work_bytes(nitems, nsteps);
+    if (need > kNoAllocScratchBytes)
+        return fail(MP_ERR_MEMORY,
+                    "mp_transport_step_noalloc: %d work items x %d steps need %zu bytes of scratch, "
+                    "more than its %zu; use mp_transport_step or fewer steps per call",
+                    nitems, nsteps, need, kNoAllocScratchBytes);
+    int dev = -1;
+    MP_HIP(hipGetDevice(&dev));
+    if (dev != device)
+        return fail(MP_ERR_STATE, "mp_transport_step_noalloc: the handle's device %d is not current (%d)",
+                    device, dev);
+    void* base = nullptr;
+    MP_CALL(scratch_on(dev, &base));
+    // error word first, then the totals, then the partials (each 256-byte aligned)
+    char* p = static_cast<char*>(base);
+    const size_t tot = (size_t(nsteps > 0 ? nsteps : 1) * 4 * sizeof(double) + 255) & ~size_t(255);
+    const StepWork w{reinterpret_cast<double*>(p + 256 + tot), reinterpret_cast<double*>(p + 256),
+                     reinterpret_cast<uint32_t*>(p)};
+    MP_HIP(launch_clear(w.error, 1, stream));
+    int final_state = cur;
+    MP_CALL(enqueue_steps(w, nsteps, scheme, dt, final_state, nullptr));
+    return finish_step(w, nsteps, scheme, dt, final_state, report);
 }
 
 mp_status mp_transport::state_totals(double& th, double& tc)
@@ -440,7 +530,7 @@ mp_status mp_transport::profile(int nsteps, int scheme, double dt, double* ms_to
     MP_HIP(hipMemsetAsync(error.ptr, 0, sizeof(uint32_t), stream));
     int final_state = cur;
     MP_HIP(hipEventRecord(t0, stream));
-    mp_status st = enqueue_steps(nsteps, scheme, dt, final_state, &ev);
+    mp_status st = enqueue_steps(own_work(), nsteps, scheme, dt, final_state, &ev);
     MP_HIP(hipEventRecord(t1, stream));
     MP_HIP(hipStreamSynchronize(stream));
     double plain = 0, fused = 0;
@@ -739,6 +829,13 @@ mp_status mp_transport_step(mp_transport* t, int32_t nsteps, int32_t scheme, dou
 {
     if (!t) return fail(MP_ERR_ARG, "null handle");
     MP_GUARD({ return t->step(nsteps, scheme, dt, report); })
+}
+
+mp_status mp_transport_step_noalloc(mp_transport* t, int32_t nsteps, int32_t scheme, double dt,
+                                    mp_step_report* report)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->step_noalloc(nsteps, scheme, dt, report); })
 }
 
 mp_status mp_transport_phase(mp_transport* t, double* T, double* phi, double* Cl, double* Cs,

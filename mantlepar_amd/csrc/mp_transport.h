@@ -14,6 +14,7 @@ struct mp_transport {
     mp::PhaseHost phase{};
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    int device = 0;               // HIP device current at create time
     mp::MeshDev md{};
     mp::DeviceBuffer d_md;
     size_t N = 0;
@@ -59,12 +60,27 @@ struct mp_transport {
     mp_status compute_rate();
     mp_status set_boundary(const uint8_t* const side[4]);
     mp_status set_inflow(const double* h, const double* c);
-    mp::StagePtrs stage_ptrs(int src, int kin, int out, int scheme, int st, double dt,
-                             bool phase_stage, int phase_set, int part_stage) const;
-    mp_status enqueue_steps(int nsteps, int scheme, double dt, int& final_state,
+    // Device memory a step works in beyond the states: stage partials of two
+    // consecutive steps plus the step reduction's block results (partials),
+    // per-step totals [step][4] and the error word.
+    struct StepWork {
+        double* partials;
+        double* totals;
+        uint32_t* error;
+    };
+    StepWork own_work() const;
+    static size_t work_bytes(int nitems, int nsteps);
+    mp::StagePtrs stage_ptrs(const StepWork& w, int src, int kin, int out, int scheme, int st,
+                             double dt, bool phase_stage, int phase_set, int part_stage) const;
+    mp_status enqueue_steps(const StepWork& w, int nsteps, int scheme, double dt, int& final_state,
                             std::vector<hipEvent_t>* events);
     mp_status check_step_args(int nsteps, int scheme, double dt);
     mp_status step(int nsteps, int scheme, double dt, mp_step_report* report);
+    // step() with its work memory in the library's static device scratch
+    // (no device allocation; see mp_transport_step_noalloc).
+    mp_status step_noalloc(int nsteps, int scheme, double dt, mp_step_report* report);
+    mp_status finish_step(const StepWork& w, int nsteps, int scheme, double dt, int final_state,
+                          mp_step_report* report);
     mp_status state_totals(double& th, double& tc);
     mp_status profile(int nsteps, int scheme, double dt, double* ms_total, double* ms_stage,
                       double* ms_phase_stage);

@@ -149,6 +149,12 @@ class Transport:
         check(lib().mp_transport_step(self._h, nsteps, scheme, dt, C.byref(rep)))
         return rep.as_dict()
 
+    def step_noalloc(self, nsteps, dt, scheme=SCHEME_SSPRK3):
+        """mp_transport_step with its work memory in the library's static scratch."""
+        rep = StepReport()
+        check(lib().mp_transport_step_noalloc(self._h, nsteps, scheme, dt, C.byref(rep)))
+        return rep.as_dict()
+
     def phase_state(self):
         out = {k: np.empty(self.N) for k in ("T", "phi", "Cl", "Cs")}
         reg = np.empty(self.N, dtype=np.uint8)
