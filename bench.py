@@ -3,7 +3,8 @@
 Workload (default): the 2048x4096 mesh on [0,0.5]x[0,1] with seeded H, C and
 divergence-free face velocities, advanced with SSP-RK3 at CFL 0.4, each step
 followed by the per-cell eutectic phase.  One "step" of this benchmark is one
-SSP-RK3 step over the whole mesh (3 stage launches + phase).  The metric is
+SSP-RK3 step over the whole mesh (3 stages + phase; on the mixed path the
+three stages run in one launch).  The metric is
 cell-steps per second in mixed precision; the plain FP64 path runs the same
 steps for the ratio and for the mixed-vs-FP64 error.
 
@@ -12,6 +13,7 @@ Prints one JSON line (rank 0).  Only one GPU is used: --gpus above 1 exits.
 import argparse
 import json
 import os
+import platform
 import sys
 import tempfile
 import time
@@ -114,24 +116,67 @@ def algorithmic_bytes(q, state_b=8, vel_b=8, phase_b=8):
     return {"step": s1 + s2 + s3, "stage1": s1, "stage2": s2, "stage3": s3}
 
 
+def launch_bytes(q, one_launch):
+    """Algorithmic bytes per cell of the mixed path's dominant launch (FP32
+    velocity samples, then FP64 samples) and its name.  With one launch per
+    step the kernel runs the three stages (the phase runs in its own kernel
+    beside the next step): the three stages' state and velocity bytes, without
+    the phase outputs.  Otherwise the stage-1/2 launch, mean of the two."""
+    b = algorithmic_bytes(q, vel_b=VEL_BYTES["mixed"])
+    b64 = algorithmic_bytes(q, vel_b=VEL_BYTES["fp64"])
+    if one_launch:
+        phase_out = 4 * 8 + 1
+        return (b["stage1"] + b["stage2"] + b["stage3"] - phase_out,
+                b64["stage1"] + b64["stage2"] + b64["stage3"] - phase_out,
+                "step32_kernel<false,false> (mixed SSP-RK3 step, its three stages in one launch)")
+    return (0.5 * (b["stage1"] + b["stage2"]), 0.5 * (b64["stage1"] + b64["stage2"]),
+            "stage32_kernel<false,false> (mixed sweep, stages 1-2)")
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(args):
+    """The oracle (FP64, the reference's algorithm) on the host, timed on
+    bounded samples of the headline workload: all the threads this process
+    may use (capped at the 16 threads that a one-GPU share of the host gets)
+    and one thread."""
     import oracle as O
     from mantlepar_amd import workloads as W
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    spec = W.MeshSpec(args.nx, args.ny // 4, y1=0.25, seed=2)
-    H = W.enthalpy(spec)
-    Cc = W.composition(spec)
-    vx, vy = W.face_velocity(spec)
-    o = O.Transport(W.vertex_coords(spec), spec.nx, spec.ny, vx, vy, q=spec.q, table_mode=1,
-                    threads=threads)
-    _, mr = o.rates()
-    t0 = time.perf_counter()
-    o.advance(H, Cc, 1, 0.4 / mr, 3)
-    el = time.perf_counter() - t0
-    return {"value": spec.cells / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
-            "sample": f"1 SSP-RK3 step (FP64, per-point polynomial rebuild, long-double "
-                      f"accumulation) on a {spec.nx}x{spec.ny} band (y in [0,0.25]) with the "
-                      f"same cell size as the {args.nx}x{args.ny} mesh; {el:.2f} s"}
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = args.cpu_threads or min(16, avail)
+    out = {"unit": "cell-steps/s", "kind": "port", "cpu_model": _cpu_model(),
+           "host_cpus": os.cpu_count(), "affinity_cpus": avail}
+    # the same cell size as the headline mesh: a band of its rows
+    for label, nthreads, rows in (("all", threads, args.ny // 4), ("one", 1, args.ny // 32)):
+        spec = W.MeshSpec(args.nx, rows, y1=rows / args.ny, seed=2)
+        H = W.enthalpy(spec)
+        Cc = W.composition(spec)
+        vx, vy = W.face_velocity(spec)
+        o = O.Transport(W.vertex_coords(spec), spec.nx, spec.ny, vx, vy, q=spec.q, table_mode=1,
+                        threads=nthreads)
+        _, mr = o.rates()
+        t0 = time.perf_counter()
+        o.advance(H, Cc, 1, 0.4 / mr, 3)
+        el = time.perf_counter() - t0
+        sample = (f"1 SSP-RK3 step (FP64, per-point polynomial rebuild, long-double "
+                  f"accumulation) on a {spec.nx}x{spec.ny} band with the cell size of the "
+                  f"{args.nx}x{args.ny} mesh, {nthreads} thread(s); {el:.2f} s")
+        if label == "all":
+            out.update({"value": spec.cells / el, "cores": nthreads, "sample": sample})
+        else:
+            out["one_thread"] = {"value": spec.cells / el, "cores": 1, "sample": sample}
+    out["note"] = ("cores = OpenMP threads used; the host is shared, and a one-GPU share is "
+                   "16 threads, so larger thread counts are not used")
+    return out
 
 
 def log(msg):
@@ -361,10 +406,12 @@ def bench_full_chip(args):
         Hf, Cf = t.download()
         ph = t.phase_state()
         prof = t.profile(4, dt)
+        one_launch = t.layout()["one_launch_steps"]
         t.close()
         res[name] = {"t_step": t_step, "t_refresh": t_refresh, "H": Hf, "C": Cf, "T": ph["T"],
                      "phi": ph["phi"], "tot0": (tot0["total_H"], tot0["total_C"]),
-                     "tot1": (rep["total_H"], rep["total_C"]), "b": (b_h, b_c), "profile": prof}
+                     "tot1": (rep["total_H"], rep["total_C"]), "b": (b_h, b_c), "profile": prof,
+                     "one_launch": one_launch}
         log(f"full-chip {name}: {spec.cells * K / t_step:.3e} cell-steps/s "
             f"(refresh {t_refresh:.1f} s)")
 
@@ -381,6 +428,9 @@ def bench_full_chip(args):
             "value": spec.cells * K / r["t_step"],
             "ms_per_step": r["t_step"] / K * 1e3,
             "refresh_s_total": r["t_refresh"],
+            # one launch per step: that launch and its phase kernel; otherwise
+            # a stage launch and the last stage (with the phase on FP64)
+            "one_launch_steps": r["one_launch"],
             "ms_stage": r["profile"]["ms_stage"],
             "ms_phase_stage": r["profile"]["ms_phase_stage"],
             # total(end) - total(start) + outward boundary flux: zero up to rounding
@@ -402,12 +452,11 @@ def bench_full_chip(args):
                              "cells_H_diff_above_1e-6":
                                  int(np.count_nonzero(np.abs(fpp["H"] - fp["H"]) > 1e-6))},
     }
-    b = algorithmic_bytes(spec.q, vel_b=VEL_BYTES["mixed"])
-    out["roofline_stage"] = {"ms_per_launch": mx["profile"]["ms_stage"],
-                             "bytes_per_cell_launch": 0.5 * (b["stage1"] + b["stage2"]),
-                             "achieved": spec.cells * 0.5 * (b["stage1"] + b["stage2"]) /
-                             (mx["profile"]["ms_stage"] * 1e-3) / 1e9,
-                             "unit": "GB/s"}
+    launch_b, _, kernel = launch_bytes(spec.q, mx["one_launch"])
+    out["roofline_launch"] = {"kernel": kernel, "ms_per_launch": mx["profile"]["ms_stage"],
+                              "bytes_per_cell_launch": launch_b,
+                              "achieved": spec.cells * launch_b / (mx["profile"]["ms_stage"] * 1e-3) / 1e9,
+                              "unit": "GB/s"}
     return out
 
 
@@ -462,6 +511,7 @@ def bench_diffusion(args, spec, H, Cc, vx, vy, phase):
         for k in (0.0, kappa):
             t.set_diffusion(k, 4, 0.9)
             t.upload(H, Cc)
+            # mixed: the one-launch step; FP64: one stage launch
             res["diffusion" if k else "advection_only"] = t.profile(10, dt)["ms_stage"]
         res["stage_overhead"] = res["diffusion"] / res["advection_only"]
         out[name] = res
@@ -571,10 +621,17 @@ def main():
         if args.warmup:
             t.step(args.warmup, dt)
         t.upload(H, Cc)
+        # exactly K steps, timed as up to five windows with a synchronisation
+        # between them; the windows give the spread of the per-step time
         t.synchronize()
-        t0 = time.perf_counter()
-        rep = t.step(args.steps, dt)
-        el = time.perf_counter() - t0
+        nwin = min(5, args.steps)
+        sizes = [args.steps // nwin + (1 if k < args.steps % nwin else 0) for k in range(nwin)]
+        windows = []
+        for n in sizes:
+            t0 = time.perf_counter()
+            rep = t.step(n, dt)
+            windows.append((time.perf_counter() - t0) / n)
+        el = sum(w * n for w, n in zip(windows, sizes))
         Hf, Cf = t.download()
         ph = t.phase_state()
         # kernel durations with HIP events on the handle's stream, over the same work
@@ -582,7 +639,8 @@ def main():
         prof = t.profile(min(args.steps, 20), dt)
         res[name] = {"seconds": el, "report": rep, "H": Hf, "C": Cf, "T": ph["T"],
                      "phi": ph["phi"], "region": ph["region"], "profile": prof,
-                     "rows": rows_used}
+                     "rows": rows_used, "windows_ms": sorted(w * 1e3 for w in windows),
+                     "layout": t.layout()}
         t.close()
         log(f"single {name}: {spec.cells * args.steps / el:.3e} cell-steps/s (rows {rows_used})")
 
@@ -599,9 +657,9 @@ def main():
     b = algorithmic_bytes(spec.q, vel_b=VEL_BYTES["mixed"])
     b64 = algorithmic_bytes(spec.q, vel_b=VEL_BYTES["fp64"])
     prof = mixed["profile"]
-    stage_bytes = N * 0.5 * (b["stage1"] + b["stage2"])
-    achieved = stage_bytes / (prof["ms_stage"] * 1e-3) / 1e9
-    achieved64 = N * 0.5 * (b64["stage1"] + b64["stage2"]) / (prof["ms_stage"] * 1e-3) / 1e9
+    launch_b, launch_b64, kernel = launch_bytes(spec.q, mixed["layout"]["one_launch_steps"])
+    achieved = N * launch_b / (prof["ms_stage"] * 1e-3) / 1e9
+    achieved64 = N * launch_b64 / (prof["ms_stage"] * 1e-3) / 1e9
     out = {
         "metric": "H/C cell-updates/sec, mixed vs FP64 at <=1e-6 rel err; serve p50/p99 ms",
         "value": value,
@@ -610,6 +668,9 @@ def main():
         "steps": K,
         "warmup": args.warmup,
         "ms_per_step": mixed["seconds"] / K * 1e3,
+        "ms_per_step_windows": {"min": mixed["windows_ms"][0],
+                                "median": mixed["windows_ms"][len(mixed["windows_ms"]) // 2],
+                                "max": mixed["windows_ms"][-1], "windows": len(mixed["windows_ms"])},
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -626,11 +687,12 @@ def main():
                           "region_cells_differing": region_diff, "steps": K},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "stage32_kernel<false> (mixed sweep without diffusion terms, stages 1-2)",
-                     "bytes_per_cell_launch": 0.5 * (b["stage1"] + b["stage2"]),
+                     "kernel": kernel,
+                     "bytes_per_cell_launch": launch_b,
                      "bytes_per_cell_step": b["step"],
-                     "ms_per_launch": prof["ms_stage"], "rows_per_item": mixed["rows"],
-                     "with_fp64_samples": {"bytes_per_cell_launch": 0.5 * (b64["stage1"] + b64["stage2"]),
+                     "ms_per_launch": prof["ms_stage"], "ms_phase_kernel": prof["ms_phase_stage"],
+                     "rows_per_item": mixed["rows"],
+                     "with_fp64_samples": {"bytes_per_cell_launch": launch_b64,
                                            "achieved": achieved64,
                                            "frac": achieved64 / HBM_PEAK_GBS}},
         "conservation": {"total_H": mixed["report"]["total_H"], "total_C": mixed["report"]["total_C"]},

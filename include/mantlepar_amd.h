@@ -169,11 +169,33 @@ mp_status mp_transport_device_state(mp_transport* t, double** H, double** C);
 mp_status mp_transport_synchronize(mp_transport* t);
 /* Time nsteps of the stage kernel with HIP events on the handle's stream:
  * ms_total = wall time of the steps; ms_stage = mean duration of one stage
- * launch; ms_phase_stage = mean duration of the phase-fused final stage. */
+ * launch; ms_phase_stage = mean duration of the phase-fused final stage.  With
+ * one launch per step (MP_LAUNCH_PER_STEP on a mixed handle), ms_stage is the
+ * mean duration of a step's launch (all its stages) and ms_phase_stage that of
+ * its phase kernel. */
 mp_status mp_transport_profile(mp_transport* t, int32_t nsteps, int32_t scheme, double dt,
                                double* ms_total, double* ms_stage, double* ms_phase_stage);
 /* Sweep tile shape: rows per work item (>= 4). */
 mp_status mp_transport_set_tile(mp_transport* t, int32_t rows_per_item);
+/* How a mixed-precision handle launches its SSPRK2/SSPRK3 steps:
+ * MP_LAUNCH_PER_STEP runs all stages of a step in one launch, each work item
+ * of a stage starting when the items whose output it reads are done;
+ * MP_LAUNCH_PER_STAGE runs one launch per stage; MP_LAUNCH_AUTO (the default)
+ * takes the per-step launch for meshes that give each wave slot of the GPU
+ * fewer than eight work items, where a launch's ramp and drain weigh most.
+ * All give bitwise the same results.  FP64 handles, forward Euler and the
+ * limiter always launch per stage.  MP_LAUNCH_STALL_CHECK is a diagnostic of
+ * the per-step launch's bounded waits: items wait for a completion that is
+ * never published and give up after 64 polls, so a step fails with
+ * MP_ERR_DEVICE and leaves the state as it was. */
+enum { MP_LAUNCH_PER_STAGE = 0, MP_LAUNCH_PER_STEP = 1, MP_LAUNCH_STALL_CHECK = 2, MP_LAUNCH_AUTO = 3 };
+mp_status mp_transport_set_launch(mp_transport* t, int32_t mode);
+/* The sweep's work layout (any output may be NULL): rows per work item, work
+ * items per stage, the first row of the 4-row tail items (ny: none; it
+ * depends on the GPU's CU count), and whether SSPRK3 steps run as one launch
+ * each (1) or one launch per stage (0). */
+mp_status mp_transport_layout(mp_transport* t, int32_t* rows_per_item, int32_t* items,
+                              int32_t* tail_split, int32_t* one_launch_steps);
 /* Bound limiter of every reconstruction the stage kernels build: replaces
  * StateReconstruction::Limit (src/couple/state_reconstruction.cpp:136-192;
  * the reference calls it per coupled RHS, src/couple/evolve.cpp:281), with

@@ -20,6 +20,7 @@ BATCH_EAGER, BATCH_GRAPH = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 BC_ZERO_FLUX, BC_OUTFLOW, BC_INFLOW_OUTFLOW = 0, 1, 2
 SCHEME_FE, SCHEME_SSPRK2, SCHEME_SSPRK3 = 1, 2, 3
+LAUNCH_PER_STAGE, LAUNCH_PER_STEP, LAUNCH_STALL_CHECK, LAUNCH_AUTO = 0, 1, 2, 3
 
 
 class MantleError(RuntimeError):
@@ -152,6 +153,9 @@ SIGNATURES = [
     ("mp_transport_synchronize", C.c_int32, [_vp]),
     ("mp_transport_profile", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double, _dp, _dp, _dp]),
     ("mp_transport_set_tile", C.c_int32, [_vp, C.c_int32]),
+    ("mp_transport_set_launch", C.c_int32, [_vp, C.c_int32]),
+    ("mp_transport_layout", C.c_int32, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("mp_transport_set_limiter", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_double]),
     ("mp_transport_limits", C.c_int32, [_vp, _dp, _dp]),
     ("mp_transport_set_diffusion", C.c_int32, [_vp, C.c_double, C.c_int32, C.c_double]),

@@ -27,6 +27,7 @@ enum : uint32_t {
     kErrInadmissible = 1u << 7,   // limiter: average outside H >= 0, 0 <= C <= Xe
     kErrCoupledRange = 1u << 8,   // coupled law: T < 0 or phi >= 1 at a point
     kErrLFSpeed = 1u << 9,        // coupled law: global LF speed below a flow speed
+    kErrStall = 1u << 10,         // one-launch step: an item's inputs were not done in time
 };
 
 // Binary eutectic parameters as the device uses them.  Tep/Tmp are derived

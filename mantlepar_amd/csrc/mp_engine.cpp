@@ -407,23 +407,66 @@ mp_status Engine::set_active(const std::vector<int>& list, const std::vector<int
         q_common = q_common < 0 ? s.desc.q : (q_common == s.desc.q ? q_common : 0);
     }
     if (q_common < 0) q_common = 0;
-    // One interleaved item list per distinct prefix of the ordered slots (a
-    // prefix is the set still stepping at some step).  Interleaving meshes
-    // band by band lets a persistent wave stride spread each mesh over the
-    // whole chip.
+    // Two item lists per distinct prefix of the ordered slots (a prefix is the
+    // set still stepping at some step), both over the same logical order:
+    // mesh after mesh, each mesh's items in item order.
+    //  * queue order: the logical order itself, for the persistent grid whose
+    //    waves take items from a counter;
+    //  * XCD slabs, for one wave per item: the dispatcher places workgroup b on
+    //    XCD b % 8, so the logical order is cut into eight runs of equal cost
+    //    and run x fills the positions b = x, x + 8, ...  A mesh's adjacent row
+    //    bands, which share halo rows, then sit in one XCD's L2, and the
+    //    waves of a CU mostly serve one mesh, whose tables stay in the scalar
+    //    cache.  Runs of unequal length are padded with (-1, 0) entries that
+    //    the kernels skip.  An item costs about its rows plus the two rows it
+    //    reads before its first output row.
     std::vector<int> prefixes;
     for (size_t i = 0; i < active.size(); ++i)
         if (i + 1 == active.size() || active_steps[i + 1] != active_steps[i]) prefixes.push_back(int(i + 1));
     std::vector<int2> items;
     lists.clear();
     for (int p : prefixes) {
-        const int off = int(items.size());
-        int maxn = 0;
-        for (int i = 0; i < p; ++i) maxn = std::max(maxn, slots[active[i]].nitems);
-        for (int it = 0; it < maxn; ++it)
-            for (int i = 0; i < p; ++i)
-                if (it < slots[active[i]].nitems) items.push_back(make_int2(active[i], it));
-        lists[p] = {off, int(items.size()) - off};
+        std::vector<int2> logical;
+        std::vector<double> cost;
+        double total = 0.0;
+        for (int i = 0; i < p; ++i) {
+            const Slot& sl = slots[active[i]];
+            for (int it = 0; it < sl.nitems; ++it) {
+                logical.push_back(make_int2(active[i], it));
+                const bool tail = it >= sl.layout.nlong;
+                const int band = tail ? (it - sl.layout.nlong) / sl.layout.strips : it / sl.layout.strips;
+                const int j0 = tail ? sl.layout.split + band * sl.layout.tail_rows : band * sl.layout.rows;
+                const int j1 = std::min(j0 + (tail ? sl.layout.tail_rows : sl.layout.rows),
+                                        tail ? sl.desc.ny : sl.layout.split);
+                cost.push_back(double(j1 - j0) + 2.0);
+                total += cost.back();
+            }
+        }
+        List L;
+        L.queue_off = int(items.size());
+        L.queue_count = int(logical.size());
+        items.insert(items.end(), logical.begin(), logical.end());
+        int start[kXcds + 1], count[kXcds];
+        {
+            size_t pos = 0;
+            double acc = 0.0;
+            for (int x = 0; x < kXcds; ++x) {
+                start[x] = int(pos);
+                const double goal = total * double(x + 1) / kXcds;
+                while (pos < logical.size() && (x == kXcds - 1 || acc + 0.5 * cost[pos] <= goal))
+                    acc += cost[pos++];
+                count[x] = int(pos) - start[x];
+            }
+            start[kXcds] = int(pos);
+        }
+        int per = 0;
+        for (int x = 0; x < kXcds; ++x) per = std::max(per, count[x]);
+        L.slab_off = int(items.size());
+        L.slab_count = per * kXcds;
+        for (int k = 0; k < per; ++k)
+            for (int x = 0; x < kXcds; ++x)
+                items.push_back(k < count[x] ? logical[size_t(start[x] + k)] : make_int2(-1, 0));
+        lists[p] = L;
     }
     nitems_total = int(items.size());
     ++generation;
@@ -512,7 +555,9 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
                 }
             continue;
         }
-        const auto& L = lists.at(p);
+        const List& Lst = lists.at(p);
+        const std::pair<int, int> L = blocks > 0 ? std::make_pair(Lst.queue_off, Lst.queue_count)
+                                                 : std::make_pair(Lst.slab_off, Lst.slab_count);
         for (int st = 0; st < scheme; ++st) {
             // the mixed sweep leaves the phase to its own streaming kernel, as
             // the single-mesh handle does

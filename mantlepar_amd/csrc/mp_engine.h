@@ -124,7 +124,13 @@ public:
     std::vector<int> active;         // slots, ordered by steps (descending)
     std::vector<int> active_steps;   // steps per entry of active (INT32_MAX: all)
     std::vector<int> slot_steps;     // per slot: steps in the current pattern (0: idle)
-    std::map<int, std::pair<int, int>> lists;   // prefix length -> (item offset, count)
+    // Work lists of a prefix of the ordered slots: offsets and counts in the
+    // item array, in queue order and in XCD slabs (padded with (-1, 0)).
+    struct List {
+        int queue_off = 0, queue_count = 0, slab_off = 0, slab_count = 0;
+    };
+    static constexpr int kXcds = 8;
+    std::map<int, List> lists;   // prefix length -> lists
     int capacity = 0;
     int blocks = 0;            // persistent grid in blocks of 4 waves (0: one wave per item)
     int nitems_total = 0;

@@ -457,12 +457,14 @@ __device__ __forceinline__ void step_out(const double (&sh)[6][kStepSlices], dou
     out[3] = sh[5][0];
 }
 
-// One wave per slice: kStepSlices / 4 blocks of 256 threads.
-__global__ __launch_bounds__(256) void reduce_step_kernel(
+// One wave per slice and per workgroup (kStepSlices blocks of 64 threads): on
+// the side stream this runs beside the next step's sweep, whose one-wave
+// workgroups leave few CUs with room for a larger workgroup.
+__global__ __launch_bounds__(64) void reduce_step_kernel(
     const double* __restrict__ partials, int nitems, int nstages, double w0, double w1,
     double w2, double* __restrict__ part)
 {
-    const int slice = int(blockIdx.x) * 4 + int(threadIdx.x >> 6);
+    const int slice = int(blockIdx.x);
     const int chunk = step_chunk(nitems);
     const int lo = min(nitems, slice * chunk), hi = min(nitems, lo + chunk);
     const double wst[3] = {w0, w1, w2};
@@ -514,12 +516,14 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
         const int2 w = items[it];
         const int slot = __builtin_amdgcn_readfirstlane(w.x);
         const int local = __builtin_amdgcn_readfirstlane(w.y);
-        const MeshDev& m = meshes[slot];
-        const StagePtrs& sp = stages[slot];
-        int strip, j0, j1;
-        item_region(sp, m.ny, local, strip, j0, j1);
-        if (j0 < m.ny)                 // always true with a current work list
-            sweep_item<MODE, PHASE, NQ, false, false>(m, sp, local, strip, j0, j1);
+        if (slot >= 0) {               // (-1, 0) pads an XCD slab
+            const MeshDev& m = meshes[slot];
+            const StagePtrs& sp = stages[slot];
+            int strip, j0, j1;
+            item_region(sp, m.ny, local, strip, j0, j1);
+            if (j0 < m.ny)             // always true with a current work list
+                sweep_item<MODE, PHASE, NQ, false, false>(m, sp, local, strip, j0, j1);
+        }
         if (!queue) break;
     }
 }
@@ -851,6 +855,7 @@ __global__ __launch_bounds__(64) void batch_phase_items_kernel(const MeshDev* __
     const int2 w = items[it];
     const int slot = __builtin_amdgcn_readfirstlane(w.x);
     const int local = __builtin_amdgcn_readfirstlane(w.y);
+    if (slot < 0) return;              // pads an XCD slab
     const MeshDev& m = meshes[slot];
     const StagePtrs& sp = stages[slot];
     int strip, j0, j1;
@@ -978,8 +983,8 @@ hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H
 hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, const double* w,
                               double area, double* out, double* scratch, hipStream_t stream)
 {
-    reduce_step_kernel<<<kStepSlices / 4, 256, 0, stream>>>(partials, nitems, nstages, w[0], w[1], w[2],
-                                                             scratch);
+    reduce_step_kernel<<<kStepSlices, kWave, 0, stream>>>(partials, nitems, nstages, w[0], w[1], w[2],
+                                                           scratch);
     reduce_step_final_kernel<<<1, kStepSlices, 0, stream>>>(scratch, area, out);
     return hipGetLastError();
 }

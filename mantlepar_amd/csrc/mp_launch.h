@@ -53,6 +53,16 @@ int batch_blocks_per_cu32();
 // records instead of their one-sided form.
 hipError_t launch_stage32(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
                           bool diffusion, bool global_lf);
+// All stages of one mixed-precision step in one launch: tab[0..nstages) are
+// the stage records, flags [nstages][nitems] completion flags that hold
+// earlier epochs only (never `epoch`) when the launch starts.  Items publish
+// `epoch` and wait for `wait_epoch` (equal to it except in the stall check),
+// each wait giving up after spin_limit polls.
+constexpr int kSpinLimit = 1 << 20;
+constexpr uint32_t kEpochNever = 0xffffffffu;   // no flag ever holds it
+hipError_t launch_step32(const MeshDev* d_mesh, const StagePtrs* tab, int nitems, int nstages,
+                         uint32_t* flags, uint32_t epoch, uint32_t wait_epoch, int spin_limit,
+                         hipStream_t stream, bool diffusion, bool global_lf);
 hipError_t launch_batch_stage32(const MeshDev* meshes, const StagePtrs* stages, const int2* items,
                                 int nitems, int blocks, int* queue, bool global_lf, hipStream_t stream);
 // Phase and state sums of a stage's output over the stage's work items

@@ -33,6 +33,8 @@ const char* last_error() { return g_error; }
 
 mp_status status_from_bits(uint32_t bits)
 {
+    // a stalled one-launch step left items unwritten: nothing else is reliable
+    if (bits & kErrStall) return MP_ERR_DEVICE;
     if (bits & kErrBackflow) return MP_ERR_BACKFLOW;
     // the limiter runs before any phase evaluation of the stage
     if (bits & kErrInadmissible) return MP_ERR_ARG;
@@ -46,6 +48,7 @@ mp_status status_from_bits(uint32_t bits)
 
 const char* describe_bits(uint32_t bits)
 {
+    if (bits & kErrStall) return "A work item's inputs were not completed in time; the step was abandoned";
     if (bits & kErrBackflow) return "Backflow encountered on an outflow-only boundary";
     if (bits & kErrInadmissible) return "Inadmissible cell average in coupled reconstruction";
     if (bits & kErrCoupledRange) return "Coupled Darcy-Stokes requires nonnegative temperature and porosity < 1";

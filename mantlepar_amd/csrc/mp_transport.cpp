@@ -83,6 +83,10 @@ mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
     }
     MP_CALL(error.alloc(sizeof(uint64_t) * 2));
     MP_CALL(d_md.alloc(sizeof(MeshDev)));
+    MP_CALL(d_tab.alloc(sizeof(StagePtrs) * 9));
+    MP_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_tab), sizeof(StagePtrs) * 9, hipHostMallocDefault));
+    MP_HIP(hipEventCreateWithFlags(&tab_copied, hipEventDisableTiming));
+    MP_HIP(hipEventRecord(tab_copied, stream));
     MP_HIP(hipMemsetAsync(H[0].ptr, 0, N * sizeof(double), stream));
     MP_HIP(hipMemsetAsync(C[0].ptr, 0, N * sizeof(double), stream));
     md.vnx = vnx.as<double>();
@@ -127,6 +131,8 @@ mp_transport::~mp_transport()
         if (side_done[k]) (void)hipEventDestroy(side_done[k]);
     }
     if (side_join) (void)hipEventDestroy(side_join);
+    if (tab_copied) (void)hipEventDestroy(tab_copied);
+    if (h_tab) (void)hipHostFree(h_tab);
     if (side) (void)hipStreamDestroy(side);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
@@ -165,12 +171,37 @@ mp_status mp_transport::ensure_scratch(int nsteps)
     if (partials.bytes < pbytes) MP_CALL(partials.alloc(pbytes));
     const size_t tbytes = size_t(nsteps > 0 ? nsteps : 1) * 4 * sizeof(double);
     if (totals.bytes < tbytes) MP_CALL(totals.alloc(tbytes));
+    const size_t fbytes = size_t(3) * nitems * sizeof(uint32_t);
+    if (flags.bytes < fbytes) {
+        // a new flag array holds no epoch yet
+        MP_CALL(flags.alloc(fbytes));
+        MP_HIP(hipMemsetAsync(flags.ptr, 0, fbytes, stream));
+    }
     return MP_OK;
 }
 
-mp_transport::StepWork mp_transport::own_work() const
+mp_transport::StepWork mp_transport::own_work()
 {
-    return StepWork{partials.as<double>(), totals.as<double>(), reinterpret_cast<uint32_t*>(error.ptr)};
+    return StepWork{partials.as<double>(), totals.as<double>(), reinterpret_cast<uint32_t*>(error.ptr),
+                    flags.as<uint32_t>(), flags.bytes / sizeof(uint32_t), &epoch};
+}
+
+// One launch per step on the mixed path.  MP_LAUNCH_AUTO takes it for meshes
+// that give each wave slot of the chip fewer than eight items: there a stage
+// launch's ramp and drain (about 90 us) are a large part of it, and the
+// per-step launch, which pays them once per step, measured 5% faster at
+// 2048 x 4096 (0.666 against 3 x 0.234 ms); at 8192 x 8192 the per-stage
+// launches were 3% faster (3 x 1.32 against 4.09 ms), the waits for
+// neighbouring items costing what the shared ramp saves.
+bool mp_transport::step_fused(int scheme) const
+{
+    if (precision != MP_PREC_MIXED || scheme < 2 || md.limit != 0) return false;
+    if (launch_mode == MP_LAUNCH_PER_STAGE) return false;
+    if (launch_mode != MP_LAUNCH_AUTO) return true;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    return (long long)nitems < 8LL * cus * 4 * 3;
 }
 
 mp_status mp_transport::upload(const double* h, const double* c)
@@ -331,12 +362,59 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
     } join{stream, side, side_join, overlap && nsteps > 0};
     const size_t set_doubles = size_t(nitems) * kPartials;
     double* scratch = w.partials + 6 * set_doubles;
+    const bool fused = step_fused(scheme);
+    if (fused) {
+        // Stage records of the call's three kinds of step (step 0, odd steps,
+        // even steps >= 2), uploaded once; the previous call's upload from the
+        // same pinned buffer has completed first.
+        MP_HIP(hipEventSynchronize(tab_copied));
+        const int src_of[3] = {cur, others[0], others[1]};
+        const int dst_of[3] = {others[0], others[1], others[0]};
+        const int par_of[3] = {0, overlap ? 1 : 0, 0};
+        for (int set = 0; set < 3; ++set) {
+            int kin = src_of[set];
+            for (int st = 0; st < scheme; ++st) {
+                const bool last = st == scheme - 1;
+                const int out = last ? dst_of[set] : 3 + st;
+                h_tab[set * 3 + st] = stage_ptrs(w, src_of[set], kin, out, scheme, st, dt, last, phase_set,
+                                                 par_of[set] * 3 + st);
+                kin = out;
+            }
+        }
+        MP_HIP(hipMemcpyAsync(d_tab.ptr, h_tab, sizeof(StagePtrs) * 9, hipMemcpyHostToDevice, stream));
+        MP_HIP(hipEventRecord(tab_copied, stream));
+    }
     for (int s = 0; s < nsteps; ++s) {
         const int dst = others[s % 2];
         const int par = overlap ? s % 2 : 0;
         if (overlap && s >= 2) MP_HIP(hipStreamWaitEvent(stream, side_done[par], 0));
+        if (fused) {
+            // one launch for the step's stages, then its phase as below
+            const int set = s == 0 ? 0 : (s % 2 ? 1 : 2);
+            if (*w.epoch >= 0xfffffff0u) {   // flags hold only epochs: restart them
+                MP_HIP(launch_clear(w.flags, int(w.flag_words), stream));
+                *w.epoch = 0;
+            }
+            const uint32_t ep = ++*w.epoch;
+            if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3)], stream));
+            const bool stall = launch_mode == MP_LAUNCH_STALL_CHECK;
+            MP_HIP(launch_step32(d_md.as<MeshDev>(), d_tab.as<StagePtrs>() + set * 3, nitems, scheme,
+                                 w.flags, ep, stall ? kEpochNever : ep, stall ? 64 : kSpinLimit, stream,
+                                 has_diffusion(), md.lf_mode != 0));
+            if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3) + 1], stream));
+            const StagePtrs sp = h_tab[set * 3 + scheme - 1];
+            if (overlap) {
+                MP_HIP(hipEventRecord(stage3[par], stream));
+                MP_HIP(hipStreamWaitEvent(side, stage3[par], 0));
+                MP_HIP(launch_phase_items(d_md.as<MeshDev>(), sp, nitems, side));
+            } else {
+                if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + 1)], stream));
+                MP_HIP(launch_phase_items(d_md.as<MeshDev>(), sp, nitems, stream));
+                if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + 1) + 1], stream));
+            }
+        }
         int kin = src;
-        for (int st = 0; st < scheme; ++st) {
+        for (int st = 0; st < scheme && !fused; ++st) {
             const bool last = st == scheme - 1;
             const int out = last ? dst : 3 + st;
             const StagePtrs sp = stage_ptrs(w, src, kin, out, scheme, st, dt, last, phase_set, par * 3 + st);
@@ -451,8 +529,11 @@ namespace {
 constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
 constexpr int kMaxDevices = 64;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+// completion flags of the one-launch step: 3 per work item, up to 262144 items
+constexpr size_t kScratchFlagWords = size_t(3) << 18;
 std::mutex scratch_mutex;
 void* scratch_addr[kMaxDevices] = {};
+uint32_t scratch_epoch[kMaxDevices] = {};   // epochs used on each device's flags
 
 // The scratch address on `device` (which must be current), looked up once.
 mp_status scratch_on(int device, void** out)
@@ -474,7 +555,7 @@ mp_status scratch_on(int device, void** out)
 mp_status mp_transport::step_noalloc(int nsteps, int scheme, double dt, mp_step_report* report)
 {
     MP_CALL(check_step_args(nsteps, scheme, dt));
-    const size_t need = work_bytes(nitems, nsteps);
+    const size_t need = work_bytes(nitems, nsteps) + kScratchFlagWords * sizeof(uint32_t);
     if (need > kNoAllocScratchBytes)
         return fail(MP_ERR_MEMORY,
                     "mp_transport_step_noalloc: %d work items x %d steps need %zu bytes of scratch, "
@@ -485,14 +566,23 @@ mp_status mp_transport::step_noalloc(int nsteps, int scheme, double dt, mp_step_
     if (dev != device)
         return fail(MP_ERR_STATE, "mp_transport_step_noalloc: the handle's device %d is not current (%d)",
                     device, dev);
+    if (size_t(3) * nitems > kScratchFlagWords)
+        return fail(MP_ERR_MEMORY, "mp_transport_step_noalloc: %d work items exceed the scratch's %zu",
+                    nitems, kScratchFlagWords / 3);
     void* base = nullptr;
     MP_CALL(scratch_on(dev, &base));
-    // error word first, then the totals, then the partials (each 256-byte aligned)
+    // The error word, the completion flags (a fixed region that never holds
+    // anything but epochs), the totals, then the partials (each 256-byte aligned).
     char* p = static_cast<char*>(base);
     const size_t tot = (size_t(nsteps > 0 ? nsteps : 1) * 4 * sizeof(double) + 255) & ~size_t(255);
-    const StepWork w{reinterpret_cast<double*>(p + 256 + tot), reinterpret_cast<double*>(p + 256),
-                     reinterpret_cast<uint32_t*>(p)};
+    char* q = p + 256 + kScratchFlagWords * sizeof(uint32_t);
+    const StepWork w{reinterpret_cast<double*>(q + tot), reinterpret_cast<double*>(q),
+                     reinterpret_cast<uint32_t*>(p), reinterpret_cast<uint32_t*>(p + 256),
+                     kScratchFlagWords, &scratch_epoch[dev]};
     MP_HIP(launch_clear(w.error, 1, stream));
+    // before the first epoch on this device the flags are cleared, whatever
+    // the array held at load time
+    if (scratch_epoch[dev] == 0) MP_HIP(launch_clear(w.flags, int(kScratchFlagWords), stream));
     int final_state = cur;
     MP_CALL(enqueue_steps(w, nsteps, scheme, dt, final_state, nullptr));
     return finish_step(w, nsteps, scheme, dt, final_state, report);
@@ -535,11 +625,14 @@ mp_status mp_transport::profile(int nsteps, int scheme, double dt, double* ms_to
     MP_HIP(hipStreamSynchronize(stream));
     double plain = 0, fused = 0;
     int np = 0, nf = 0;
+    const bool one_launch = step_fused(scheme);
     for (int s = 0; s < nsteps && st == MP_OK; ++s)
-        for (int k = 0; k < scheme; ++k) {
+        for (int k = 0; k < (one_launch ? 2 : scheme); ++k) {
+            // one launch per step: pair 0 times the step's launch, pair 1 its phase
             float ms = 0;
             MP_HIP(hipEventElapsedTime(&ms, ev[2 * (s * 3 + k)], ev[2 * (s * 3 + k) + 1]));
-            if (k == scheme - 1) { fused += ms; ++nf; } else { plain += ms; ++np; }
+            const bool second = one_launch ? k == 1 : k == scheme - 1;
+            if (second) { fused += ms; ++nf; } else { plain += ms; ++np; }
         }
     float total = 0;
     MP_HIP(hipEventElapsedTime(&total, t0, t1));
@@ -911,6 +1004,27 @@ mp_status mp_transport_limits(mp_transport* t, double* thetaH, double* thetaC)
 {
     if (!t) return fail(MP_ERR_ARG, "null handle");
     MP_GUARD({ return t->limits(thetaH, thetaC); })
+}
+
+mp_status mp_transport_set_launch(mp_transport* t, int32_t mode)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    if (mode != MP_LAUNCH_PER_STAGE && mode != MP_LAUNCH_PER_STEP && mode != MP_LAUNCH_STALL_CHECK &&
+        mode != MP_LAUNCH_AUTO)
+        return fail(MP_ERR_ARG, "launch mode must be MP_LAUNCH_PER_STAGE, _PER_STEP, _STALL_CHECK or _AUTO");
+    t->launch_mode = mode;
+    return MP_OK;
+}
+
+mp_status mp_transport_layout(mp_transport* t, int32_t* rows_per_item, int32_t* items,
+                              int32_t* tail_split, int32_t* one_launch_steps)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    if (rows_per_item) *rows_per_item = t->rows;
+    if (items) *items = t->nitems;
+    if (tail_split) *tail_split = t->layout.split;
+    if (one_launch_steps) *one_launch_steps = t->step_fused(MP_SCHEME_SSPRK3) ? 1 : 0;
+    return MP_OK;
 }
 
 mp_status mp_transport_set_tile(mp_transport* t, int32_t rows_per_item)

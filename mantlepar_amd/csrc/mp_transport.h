@@ -34,6 +34,16 @@ struct mp_transport {
     int phase_cur = 0;
     bool phase_valid = false;
     mp::DeviceBuffer partials, totals, error;
+    // One launch per step (mixed precision, see step_fused): completion
+    // flags of the handle's own work memory and their epoch counter; the
+    // stage records of a call's three step kinds (step 0, odd, even >= 2) in
+    // device memory, staged through pinned host memory.
+    mp::DeviceBuffer flags;
+    uint32_t epoch = 0;
+    int32_t launch_mode = MP_LAUNCH_AUTO;
+    mp::DeviceBuffer d_tab;
+    mp::StagePtrs* h_tab = nullptr;
+    hipEvent_t tab_copied = nullptr;
     // Mixed path: each step's phase kernel and totals run on `side`, beside the
     // next step's stages on `stream` (stage3: the step's last stage is done;
     // side_done: its phase and totals are done; by step parity).
@@ -63,12 +73,18 @@ struct mp_transport {
     // Device memory a step works in beyond the states: stage partials of two
     // consecutive steps plus the step reduction's block results (partials),
     // per-step totals [step][4] and the error word.
+    // The one-launch step also needs completion flags [3][nitems] (flag_words
+    // words in all, holding only epochs) and the host counter of the epochs
+    // already used on those flags.
     struct StepWork {
         double* partials;
         double* totals;
         uint32_t* error;
+        uint32_t* flags;
+        size_t flag_words;
+        uint32_t* epoch;
     };
-    StepWork own_work() const;
+    StepWork own_work();
     static size_t work_bytes(int nitems, int nsteps);
     mp::StagePtrs stage_ptrs(const StepWork& w, int src, int kin, int out, int scheme, int st,
                              double dt, bool phase_stage, int phase_set, int part_stage) const;
@@ -94,4 +110,6 @@ struct mp_transport {
     mp_status set_lithostatic(int enable, double surface_y);
     // thermal diffusion or thermal boundary data are set
     bool has_diffusion() const;
+    // this call's steps run as one launch each
+    bool step_fused(int scheme) const;
 };

@@ -11,7 +11,8 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (BC_INFLOW_OUTFLOW, BC_OUTFLOW, BC_ZERO_FLUX, PREC_FP64, PREC_MIXED,
-                   SCHEME_FE, SCHEME_SSPRK2, SCHEME_SSPRK3, MantleError, MeshDesc, PhaseParams,
+                   LAUNCH_AUTO, LAUNCH_PER_STAGE, LAUNCH_PER_STEP, SCHEME_FE, SCHEME_SSPRK2,
+                   SCHEME_SSPRK3, MantleError, MeshDesc, PhaseParams,
                    StepReport, WenoOptions, check, dptr, lib, u8ptr)
 
 __all__ = ["EutecticModel", "Transport", "MantleError", "MeshDesc", "PhaseParams",
@@ -177,6 +178,19 @@ class Transport:
         check(lib().mp_transport_reconstruct(self._h, field, dptr(sig), dptr(om), dptr(tr)))
         return (sig.reshape(self.ny, self.nx, 5), om.reshape(self.ny, self.nx, 5),
                 tr.reshape(self.ny, self.nx, 4, self.q))
+
+    def set_launch(self, per_step=True):
+        """One launch per step (True), one per stage (False) or chosen by mesh
+        size (None, the default); results are bitwise equal."""
+        mode = LAUNCH_AUTO if per_step is None else (LAUNCH_PER_STEP if per_step else LAUNCH_PER_STAGE)
+        check(lib().mp_transport_set_launch(self._h, mode))
+
+    def layout(self):
+        """rows per item, items per stage, first tail row, one launch per SSPRK3 step."""
+        v = [C.c_int32() for _ in range(4)]
+        check(lib().mp_transport_layout(self._h, *[C.byref(x) for x in v]))
+        return {"rows": v[0].value, "items": v[1].value, "tail_split": v[2].value,
+                "one_launch_steps": bool(v[3].value)}
 
     def set_tile(self, rows):
         check(lib().mp_transport_set_tile(self._h, rows))

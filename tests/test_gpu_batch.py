@@ -51,6 +51,9 @@ def single_run(spec, prec, nsteps, cfl=0.4, vel=None, bc=None, inflow=None, sche
                     phase=W.config_phase(), precision=prec)
     if rows is not None:
         t.set_tile(rows)
+    if spec.nx == 2048 and spec.ny == 3400 and rows == 12:
+        # the tail-item case: the reference layout really has tail items
+        assert t.layout()["tail_split"] < spec.ny
     t.upload(H, Cc)
     t.set_face_velocity(vx, vy)
     if bc is not None:
