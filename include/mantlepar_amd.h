@@ -117,6 +117,9 @@ void mp_default_weno_options(mp_weno_options* o);
 void mp_default_phase_params(mp_phase_params* p);
 const char* mp_last_error(void);
 const char* mp_version(void);
+/* Sets the thread's last error (as mp_last_error returns it) and returns
+ * code: for wrappers layered on this ABI that reject a call themselves. */
+mp_status mp_report_error(mp_status code, const char* message);
 
 /* --------------------------------------------------------- single mesh --- */
 /* stream: a hipStream_t, or NULL for a stream owned by the handle. */
@@ -164,6 +167,16 @@ mp_status mp_transport_phase(mp_transport* t, double* T, double* phi, double* Cl
 mp_status mp_transport_rhs(mp_transport* t, double* rhsH, double* rhsC, double* boundary);
 mp_status mp_transport_reconstruct(mp_transport* t, int32_t field, double* sigma,
                                    double* omega, double* traces);
+/* The reconstructions of the current state at n points, replacing
+ * StateReconstruction::Evaluate (src/couple/state_reconstruction.cpp:121-134)
+ * over Reconstruction::Evaluate (src/mlweno/reconstruction.cpp:474-502):
+ * point k = (x[k], y[k]) in physical coordinates is evaluated on the
+ * reconstruction of cell cells[k] (j*nx + i), limited by the cell's factor
+ * theta when the bound limiter is on: H[k] = mean + theta*(R(p) - mean), C[k]
+ * likewise.  Cells outside the mesh and non-finite points fail with
+ * MP_ERR_ARG.  H or C may be NULL. */
+mp_status mp_transport_evaluate(mp_transport* t, int64_t n, const int32_t* cells, const double* x,
+                                const double* y, double* H, double* C);
 /* Device pointers of the current state (valid until the next step/upload). */
 mp_status mp_transport_device_state(mp_transport* t, double** H, double** C);
 mp_status mp_transport_synchronize(mp_transport* t);
@@ -498,6 +511,24 @@ mp_status mp_phase_evaluate(const mp_phase_params* params, int64_t n, const doub
                             const double* C, const double* P, double* T, double* phi,
                             double* Cl, double* Cs, uint8_t* region, double* phi1,
                             double* phi2, double* dTdC, double* dTdH);
+/* DerivedParameters of src/phase/eutectic_rescaled.h:29-41, in its order. */
+typedef struct {
+    double dT, h0, gamma, LD, TDm0, TDe0, invk0, l0, u0, p0, t0;
+} mp_phase_derived;
+/* EutecticModel's derived scales (derive(), src/phase/eutectic_rescaled.cpp:
+ * 42-86), with its checks in its order: invalid parameters fail with
+ * MP_ERR_ARG (std::invalid_argument there), unrepresentable scales with
+ * MP_ERR_FP (std::overflow_error).  Host only. */
+mp_status mp_phase_derive(const mp_phase_params* params, mp_phase_derived* out);
+/* EutecticModel::staticPressure(zD) (use_length = 0: the length scale l0) and
+ * staticPressure(zD, length_scale) (src/phase/eutectic_rescaled.cpp:151-161).
+ * Host only. */
+mp_status mp_phase_static_pressure(const mp_phase_params* params, double zD, double length_scale,
+                                   int32_t use_length, double* P);
+/* EutecticModel::pressureCorrectedTemperature (src/phase/eutectic_rescaled.cpp:
+ * 143-149).  Host only. */
+mp_status mp_phase_pressure_corrected_temperature(const mp_phase_params* params, double TD, double P,
+                                                  double* out);
 mp_status mp_phase_boundaries(const mp_phase_params* params, double C, double P,
                               double* Tep, double* Tmp, double* H_eutectic_end,
                               double* H_liquidus);

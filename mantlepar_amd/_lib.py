@@ -133,6 +133,7 @@ SIGNATURES = [
     ("mp_default_phase_params", None, [C.POINTER(PhaseParams)]),
     ("mp_last_error", C.c_char_p, []),
     ("mp_version", C.c_char_p, []),
+    ("mp_report_error", C.c_int32, [C.c_int32, C.c_char_p]),
     ("mp_transport_create", C.c_int32, [C.POINTER(MeshDesc), C.POINTER(WenoOptions),
                                         C.POINTER(PhaseParams), C.c_int32, _vp, C.POINTER(_vp)]),
     ("mp_transport_destroy", C.c_int32, [_vp]),
@@ -149,6 +150,7 @@ SIGNATURES = [
     ("mp_transport_phase", C.c_int32, [_vp, _dp, _dp, _dp, _dp, _u8p]),
     ("mp_transport_rhs", C.c_int32, [_vp, _dp, _dp, _dp]),
     ("mp_transport_reconstruct", C.c_int32, [_vp, C.c_int32, _dp, _dp, _dp]),
+    ("mp_transport_evaluate", C.c_int32, [_vp, C.c_int64, C.c_void_p, _dp, _dp, _dp, _dp]),
     ("mp_transport_device_state", C.c_int32, [_vp, C.POINTER(_dp), C.POINTER(_dp)]),
     ("mp_transport_synchronize", C.c_int32, [_vp]),
     ("mp_transport_profile", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double, _dp, _dp, _dp]),
@@ -216,6 +218,11 @@ SIGNATURES = [
                                       _dp, _dp, _dp, _u8p, _dp, _dp, _dp, _dp]),
     ("mp_phase_boundaries", C.c_int32, [C.POINTER(PhaseParams), C.c_double, C.c_double, _dp,
                                         _dp, _dp, _dp]),
+    ("mp_phase_derive", C.c_int32, [C.POINTER(PhaseParams), _dp]),
+    ("mp_phase_static_pressure", C.c_int32, [C.POINTER(PhaseParams), C.c_double, C.c_double,
+                                             C.c_int32, _dp]),
+    ("mp_phase_pressure_corrected_temperature", C.c_int32, [C.POINTER(PhaseParams), C.c_double,
+                                                            C.c_double, _dp]),
 ]
 
 _lib = None

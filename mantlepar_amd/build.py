@@ -42,6 +42,24 @@ def _compile(src):
     return obj, err
 
 
+CPP_CHECK_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "facade_check.cpp")
+CPP_CHECK = os.path.join(os.path.dirname(HERE), "tests", "cpp", "facade_check")
+
+
+def build_cpp_check():
+    """The compiled C++ caller of include/mantlepar_amd.hpp (host code only:
+    g++ against the library, no device code)."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra",
+           "-I", os.path.join(os.path.dirname(HERE), "include"), CPP_CHECK_SRC,
+           "-L", HERE, "-lmantlepar_amd", "-Wl,-rpath,$ORIGIN/../../mantlepar_amd",
+           "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib", "-o", CPP_CHECK + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on {CPP_CHECK_SRC}:\n{r.stderr}")
+    os.replace(CPP_CHECK + ".tmp", CPP_CHECK)
+    return CPP_CHECK
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
@@ -58,6 +76,7 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cpp_check()
     return LIB
 
 

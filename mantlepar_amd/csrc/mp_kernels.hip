@@ -636,6 +636,46 @@ __global__ void limit_diag_kernel(const MeshDev* __restrict__ mesh, const double
     if (err) atomicOr(error, err);
 }
 
+// The reconstructions of H and C at points: query k is cell cell[k] and the
+// point's offset from the cell centre in units of h = sqrt(area), (X[k], Y[k]).
+// Value t + theta*P(X, Y), P the centred blend (Reconstruction::Evaluate,
+// src/mlweno/reconstruction.cpp:474-502) and theta the limiter factor when the
+// limiter is on (StateReconstruction::Evaluate, src/couple/state_reconstruction.cpp:121-134).
+template <int MODE>
+__global__ void point_eval_kernel(const MeshDev* __restrict__ mesh, const double* __restrict__ H,
+                                  const double* __restrict__ C, int64_t n,
+                                  const int32_t* __restrict__ cell, const double* __restrict__ X,
+                                  const double* __restrict__ Y, double* outH, double* outC,
+                                  uint32_t* error)
+{
+    using R = RealT<MODE>;
+    const MeshDev& m = *mesh;
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = cell[k];
+    const int i = c % m.nx, j = c / m.nx;
+    const unsigned mask = candidate_mask(i, j, m.nx, m.ny);
+    const unsigned sides = (j == 0 ? 1u : 0u) | (i == m.nx - 1 ? 2u : 0u) |
+                           (j == m.ny - 1 ? 4u : 0u) | (i == 0 ? 8u : 0u);
+    const double x = X[k], y = Y[k];
+    uint32_t err = 0;
+    for (int field = 0; field < 2; ++field) {
+        double w[3][3];
+        load_window<MODE>(m, field ? C : H, i, j, w);
+        R B[3][3];
+        blend<R>(m, w, mask, B, err);
+        double th = 1.0;
+        if constexpr (MODE == 0) {
+            if (m.limit) th = limit_factor<0>(m, w, B, sides, field ? m.lim_cupper : INFINITY, err);
+        }
+        double p = 0.0;
+        for (int b = 2; b >= 0; --b)
+            p = p * y + ((double(B[b][2]) * x + double(B[b][1])) * x + double(B[b][0]));
+        (field ? outC : outH)[k] = w[1][1] + th * p;
+    }
+    if (err) atomicOr(error, err);
+}
+
 // L(u) per cell by reconstructing the cell and its four neighbours (5x the
 // work of the sweep; diagnostics only).  bnd[c*2+f] = outward boundary flux.
 template <int MODE>
@@ -948,6 +988,17 @@ hipError_t launch_rhs_diag(int mode, const MeshDev* d_mesh, int ncells, const do
     const dim3 grid((ncells + 127) / 128), block(128);
     if (mode == 0) rhs_diag_kernel<0><<<grid, block, 0, stream>>>(d_mesh, H, C, rhsH, rhsC, bnd, error);
     else rhs_diag_kernel<1><<<grid, block, 0, stream>>>(d_mesh, H, C, rhsH, rhsC, bnd, error);
+    return hipGetLastError();
+}
+
+hipError_t launch_point_eval(int mode, const MeshDev* d_mesh, const double* H, const double* C,
+                             int64_t n, const int32_t* cell, const double* X, const double* Y,
+                             double* outH, double* outC, uint32_t* error, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const dim3 grid(unsigned((n + 255) / 256)), block(256);
+    if (mode == 0) point_eval_kernel<0><<<grid, block, 0, stream>>>(d_mesh, H, C, n, cell, X, Y, outH, outC, error);
+    else point_eval_kernel<1><<<grid, block, 0, stream>>>(d_mesh, H, C, n, cell, X, Y, outH, outC, error);
     return hipGetLastError();
 }
 

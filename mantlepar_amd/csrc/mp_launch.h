@@ -24,6 +24,11 @@ hipError_t launch_recon_diag(int mode, const MeshDev* d_mesh, int ncells, const 
 hipError_t launch_rhs_diag(int mode, const MeshDev* d_mesh, int ncells, const double* H,
                            const double* C, double* rhsH, double* rhsC, double* bnd,
                            uint32_t* error, hipStream_t stream);
+// Reconstructions of H and C at n points (cell index, offset from the cell
+// centre in units of sqrt(area)).
+hipError_t launch_point_eval(int mode, const MeshDev* d_mesh, const double* H, const double* C,
+                             int64_t n, const int32_t* cell, const double* X, const double* Y,
+                             double* outH, double* outC, uint32_t* error, hipStream_t stream);
 hipError_t launch_rate(const MeshDev* d_mesh, int ncells, unsigned long long* max_bits,
                        hipStream_t stream);
 hipError_t launch_phase(const PhaseDev& ph, double TDe0, double nu, double dT, int64_t n,

@@ -722,6 +722,49 @@ mp_status mp_transport::reconstruct(int field, double* sigma, double* omega, dou
     return MP_OK;
 }
 
+mp_status mp_transport::evaluate(int64_t n, const int32_t* cells, const double* x, const double* y,
+                                 double* outH, double* outC)
+{
+    if (n < 0 || (n > 0 && (!cells || !x || !y))) return fail(MP_ERR_ARG, "null argument");
+    if (n == 0) return MP_OK;
+    const int64_t ncells = int64_t(mesh.nx) * mesh.ny;
+    // cell centres and h as the reference forms them: the mean of the cell's
+    // (snapped) vertices, h = sqrt(area)
+    std::vector<double> vx(size_t(mesh.nx) + 1), vy(size_t(mesh.ny) + 1);
+    uniform_vertices(mesh.x0, mesh.x1, mesh.nx, vx.data());
+    uniform_vertices(mesh.y0, mesh.y1, mesh.ny, vy.data());
+    const double h = std::sqrt(md.area);
+    std::vector<double> X(static_cast<size_t>(n)), Y(static_cast<size_t>(n));
+    for (int64_t k = 0; k < n; ++k) {
+        if (cells[k] < 0 || cells[k] >= ncells) return fail(MP_ERR_ARG, "Cell outside reconstruction halo");
+        if (!std::isfinite(x[k]) || !std::isfinite(y[k])) return fail(MP_ERR_ARG, "Evaluation point must be finite");
+        const int i = cells[k] % mesh.nx, j = cells[k] / mesh.nx;
+        X[size_t(k)] = (x[k] - 0.5 * (vx[size_t(i)] + vx[size_t(i) + 1])) / h;
+        Y[size_t(k)] = (y[k] - 0.5 * (vy[size_t(j)] + vy[size_t(j) + 1])) / h;
+    }
+    DeviceBuffer buf;
+    const size_t nd = size_t(n) * sizeof(double);
+    MP_CALL(buf.alloc(4 * nd + size_t(n) * sizeof(int32_t)));
+    double* d = buf.as<double>();
+    int32_t* dc = reinterpret_cast<int32_t*>(d + 4 * n);
+    MP_HIP(hipMemcpyAsync(d, X.data(), nd, hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(d + n, Y.data(), nd, hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(dc, cells, size_t(n) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemsetAsync(error.ptr, 0, sizeof(uint32_t), stream));
+    MP_HIP(launch_point_eval(precision, d_md.as<MeshDev>(), H[cur].as<double>(), C[cur].as<double>(), n,
+                             dc, d, d + n, d + 2 * n, d + 3 * n, reinterpret_cast<uint32_t*>(error.ptr),
+                             stream));
+    uint32_t bits = 0;
+    std::vector<double> host(2 * size_t(n));
+    MP_HIP(hipMemcpyAsync(host.data(), d + 2 * n, 2 * nd, hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipMemcpyAsync(&bits, error.ptr, sizeof bits, hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    if (bits) return fail(status_from_bits(bits), "%s (device error bits 0x%x)", describe_bits(bits), bits);
+    if (outH) std::memcpy(outH, host.data(), nd);
+    if (outC) std::memcpy(outC, host.data() + n, nd);
+    return MP_OK;
+}
+
 mp_status mp_transport::set_limiter(int enable, int cell_points, int samples, double extent)
 {
     if (!enable) {
@@ -847,6 +890,10 @@ void mp_default_phase_params(mp_phase_params* p)
 }
 
 const char* mp_last_error(void) { return mp::last_error(); }
+mp_status mp_report_error(mp_status code, const char* message)
+{
+    return fail(code, "%s", message ? message : "");
+}
 const char* mp_version(void) { return "mantlepar_amd 0.1 (gfx950)"; }
 
 mp_status mp_transport_create(const mp_mesh_desc* mesh, const mp_weno_options* weno,
@@ -949,6 +996,13 @@ mp_status mp_transport_reconstruct(mp_transport* t, int32_t field, double* sigma
 {
     if (!t) return fail(MP_ERR_ARG, "null handle");
     MP_GUARD({ return t->reconstruct(field, sigma, omega, traces); })
+}
+
+mp_status mp_transport_evaluate(mp_transport* t, int64_t n, const int32_t* cells, const double* x,
+                                const double* y, double* H, double* C)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    MP_GUARD({ return t->evaluate(n, cells, x, y, H, C); })
 }
 
 mp_status mp_transport_device_state(mp_transport* t, double** H, double** C)
@@ -1076,6 +1130,102 @@ mp_status mp_phase_evaluate(const mp_phase_params* params, int64_t n, const doub
         MP_HIP(hipStreamSynchronize(s));
         return MP_OK;
     })
+}
+
+// EutecticModel's derive(), src/phase/eutectic_rescaled.cpp:42-86: the same
+// checks in the same order (invalid input -> MP_ERR_ARG, like
+// std::invalid_argument; an unrepresentable scale -> MP_ERR_FP, like
+// std::overflow_error), long-double arithmetic rounded once per scale.
+mp_status mp_phase_derive(const mp_phase_params* params, mp_phase_derived* out)
+{
+    if (!params || !out) return fail(MP_ERR_ARG, "null argument");
+    const mp_phase_params& p = *params;
+    auto finite = [](double v, const char* n) -> mp_status {
+        return std::isfinite(v) ? mp_status(MP_OK) : fail(MP_ERR_ARG, "%s must be finite", n);
+    };
+    auto positive = [&](double v, const char* n) -> mp_status {
+        MP_CALL(finite(v, n));
+        return v > 0.0 ? mp_status(MP_OK) : fail(MP_ERR_ARG, "%s must be positive", n);
+    };
+    MP_CALL(positive(p.Tm0, "Tm0"));
+    MP_CALL(positive(p.Te0, "Te0"));
+    if (p.Tm0 <= p.Te0) return fail(MP_ERR_ARG, "Tm0 must be greater than Te0");
+    MP_CALL(finite(p.nu, "nu"));
+    if (p.nu == 0.0) return fail(MP_ERR_ARG, "nu must be nonzero");
+    MP_CALL(positive(p.L, "L"));
+    MP_CALL(positive(p.cp, "cp"));
+    MP_CALL(finite(p.Xe, "Xe"));
+    if (p.Xe <= 0.0 || p.Xe >= 1.0) return fail(MP_ERR_ARG, "Xe must lie strictly between 0 and 1");
+    MP_CALL(positive(p.rho, "rho"));
+    MP_CALL(positive(p.rhor, "rhor"));
+    MP_CALL(positive(p.mus, "mus"));
+    MP_CALL(positive(p.mul, "mul"));
+    MP_CALL(positive(p.k0, "k0"));
+    MP_CALL(positive(p.g, "g"));
+    MP_CALL(finite(p.alpha0, "alpha0"));
+    const long double dmax = DBL_MAX;
+    auto checked = [&](long double v, const char* n, double& r) -> mp_status {
+        if (!std::isfinite(v) || fabsl(v) > dmax) return fail(MP_ERR_FP, "%s is not representable", n);
+        r = double(v);
+        return MP_OK;
+    };
+    auto scale = [&](long double v, const char* n, double& r) -> mp_status {
+        MP_CALL(checked(v, n, r));
+        return r > 0.0 ? mp_status(MP_OK) : fail(MP_ERR_FP, "%s is not a positive scale", n);
+    };
+    mp_phase_derived d;
+    const long double deltaT = (long double)p.Tm0 - p.Te0;
+    const long double h0 = (long double)p.cp * deltaT;
+    MP_CALL(scale(deltaT, "dT", d.dT));
+    MP_CALL(scale(h0, "h0", d.h0));
+    MP_CALL(checked(1.0L / p.nu, "gamma", d.gamma));
+    if (d.gamma == 0.0) return fail(MP_ERR_FP, "gamma underflows to zero");
+    MP_CALL(scale(p.L / h0, "LD", d.LD));
+    MP_CALL(scale(p.Tm0 / deltaT, "TDm0", d.TDm0));
+    MP_CALL(scale(p.Te0 / deltaT, "TDe0", d.TDe0));
+    MP_CALL(scale(1.0L / p.k0, "invk0", d.invk0));
+    const long double l0 = sqrtl((long double)p.mus * p.k0 / p.mul);
+    const long double u0 = (long double)p.k0 * p.rhor * p.g / p.mul;
+    MP_CALL(scale(l0, "l0", d.l0));
+    MP_CALL(scale(u0, "u0", d.u0));
+    MP_CALL(scale((long double)p.rhor * p.g * l0, "p0", d.p0));
+    MP_CALL(scale(l0 / u0, "t0", d.t0));
+    *out = d;
+    return MP_OK;
+}
+
+// EutecticModel::staticPressure, src/phase/eutectic_rescaled.cpp:151-161.
+mp_status mp_phase_static_pressure(const mp_phase_params* params, double zD, double length_scale,
+                                   int32_t use_length, double* P)
+{
+    if (!params || !P) return fail(MP_ERR_ARG, "null argument");
+    mp_phase_derived d;
+    MP_CALL(mp_phase_derive(params, &d));
+    const double len = use_length ? length_scale : d.l0;
+    if (!std::isfinite(zD)) return fail(MP_ERR_ARG, "zD must be finite");
+    if (!std::isfinite(len)) return fail(MP_ERR_ARG, "length_scale must be finite");
+    if (!(len > 0.0)) return fail(MP_ERR_ARG, "length_scale must be positive");
+    const long double v = (long double)params->rho * params->g * zD * len;
+    if (!std::isfinite(v) || fabsl(v) > (long double)DBL_MAX)
+        return fail(MP_ERR_FP, "static pressure is not representable");
+    *P = double(v);
+    return MP_OK;
+}
+
+// EutecticModel::pressureCorrectedTemperature, src/phase/eutectic_rescaled.cpp:143-149.
+mp_status mp_phase_pressure_corrected_temperature(const mp_phase_params* params, double TD, double P,
+                                                  double* out)
+{
+    if (!params || !out) return fail(MP_ERR_ARG, "null argument");
+    mp_phase_derived d;
+    MP_CALL(mp_phase_derive(params, &d));
+    if (!std::isfinite(TD)) return fail(MP_ERR_ARG, "TD must be finite");
+    if (!std::isfinite(P)) return fail(MP_ERR_ARG, "P must be finite");
+    const long double v = (long double)TD + (long double)P / params->nu / d.dT;
+    if (!std::isfinite(v) || fabsl(v) > (long double)DBL_MAX)
+        return fail(MP_ERR_FP, "pressure-corrected temperature is not representable");
+    *out = double(v);
+    return MP_OK;
 }
 
 mp_status mp_phase_boundaries(const mp_phase_params* params, double C, double P, double* Tep,

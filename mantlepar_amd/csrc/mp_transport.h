@@ -103,6 +103,8 @@ struct mp_transport {
     mp_status phase_out(double* T, double* phi, double* Cl, double* Cs, uint8_t* region);
     mp_status rhs(double* rh, double* rc, double* boundary);
     mp_status reconstruct(int field, double* sigma, double* omega, double* traces);
+    mp_status evaluate(int64_t n, const int32_t* cells, const double* x, const double* y,
+                       double* outH, double* outC);
     mp_status set_limiter(int enable, int cell_points, int samples, double extent);
     mp_status limits(double* thetaH, double* thetaC);
     mp_status set_diffusion(double kappa, int samples, double extent);

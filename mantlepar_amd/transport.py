@@ -171,6 +171,17 @@ class Transport:
         check(lib().mp_transport_rhs(self._h, dptr(rh), dptr(rc), dptr(b)))
         return rh.reshape(self.ny, self.nx), rc.reshape(self.ny, self.nx), b[0], b[1]
 
+    def evaluate(self, cells, x, y):
+        """Reconstructed (limited) H and C of cells[k] (flat j*nx+i) at points (x[k], y[k])."""
+        cells = np.ascontiguousarray(cells, dtype=np.int32).ravel()
+        x, y = _f64(x).ravel(), _f64(y).ravel()
+        n = cells.size
+        assert x.size == n and y.size == n
+        h, c = np.empty(n), np.empty(n)
+        check(lib().mp_transport_evaluate(self._h, n, cells.ctypes.data, dptr(x), dptr(y),
+                                          dptr(h), dptr(c)))
+        return h, c
+
     def reconstruct(self, field):
         sig = np.empty(self.N * 5)
         om = np.empty(self.N * 5)

@@ -73,3 +73,55 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
         assert "no CPU fallback" in str(e)
     else:
         raise AssertionError("expected ImportError")
+
+
+def _golden(name):
+    import numpy as np
+    return dict(np.load(os.path.join(ROOT, "tests", "golden", name), allow_pickle=False))
+
+
+def _params(row):
+    from mantlepar_amd.transport import default_phase
+    import oracle as O
+    names = [n for n, _ in O.Material._fields_]
+    return default_phase(**dict(zip(names, row)))
+
+
+def test_derived_scales_match_reference_golden():
+    """mp_phase_derive (host) is bitwise the reference's DerivedParameters."""
+    import numpy as np
+    g = _golden("phase_golden.npz")
+    for row, ref in zip(g["params"], g["derived"]):
+        out = np.zeros(11)
+        assert _lib.lib().mp_phase_derive(C.byref(_params(row)), _lib.dptr(out)) == 0
+        assert np.array_equal(out, ref)
+    bad = _params(g["params"][0])
+    bad.Xe = 1.0
+    out = np.zeros(11)
+    assert _lib.lib().mp_phase_derive(C.byref(bad), _lib.dptr(out)) == 1   # invalid_argument
+
+
+def test_static_pressure_matches_reference_golden():
+    """mp_phase_static_pressure (host) against the reference's staticPressure:
+    values bitwise, and its failures in the reference's categories
+    (1 invalid_argument -> MP_ERR_ARG, 2 overflow_error -> MP_ERR_FP)."""
+    g = _golden("static_pressure_golden.npz")
+    out = C.c_double()
+    for k in range(g["zD"].size):
+        p = _params(g["params"][int(g["set"][k])])
+        rc = _lib.lib().mp_phase_static_pressure(C.byref(p), float(g["zD"][k]), float(g["length"][k]),
+                                                 int(g["scaled"][k]), C.byref(out))
+        st = int(g["status"][k])
+        assert rc == {0: 0, 1: 1, 2: 2}[st], (k, rc, st)
+        if st == 0:
+            assert out.value == g["P"][k] or (out.value == 0 and g["P"][k] == 0), k
+
+
+def test_cpp_facade_program_links():
+    """The compiled C++ caller of include/mantlepar_amd.hpp loads the library
+    (usage message, no device call)."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cpp", "facade_check")
+    assert os.path.exists(exe), "build() compiles tests/cpp/facade_check"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
