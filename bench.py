@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--batch-steps", type=int, default=50)
     p.add_argument("--serve-requests", type=int, default=400)
     p.add_argument("--serve-rate", type=float, default=300.0, help="Poisson arrivals per second")
+    p.add_argument("--serve-rates", default="300,1000,2000,4000",
+                   help="arrival rates of the serving sweep (comma-separated; empty: none)")
     p.add_argument("--full-steps", type=int, default=1000,
                    help="steps of the 8192x8192 full-chip run (0: skip it)")
     p.add_argument("--limiter-steps", type=int, default=20,
@@ -286,45 +288,69 @@ def bench_batch(args, meshes):
     }
 
 
-def bench_serve(args, meshes):
-    """Poisson arrivals of transport runs (meshes of the batch distribution,
-    20-200 seeded SSP-RK3 steps), continuous batching under a 200 GB HBM
-    budget; p50/p99 submit-to-done latency and aggregate cell-steps/s."""
-    import mantlepar_amd as M
-    from mantlepar_amd import workloads as W
-    rng = np.random.default_rng(17)
-    n = args.serve_requests
+def _serve_once(M, W, meshes, prec, rate, n, seed=17):
+    """n Poisson arrivals at `rate` per second into a fresh server; its stats."""
+    rng = np.random.default_rng(seed)
     picks = rng.integers(0, len(meshes), n)
     steps = rng.integers(20, 201, n)
-    gaps = rng.exponential(1.0 / args.serve_rate, n)
+    gaps = rng.exponential(1.0 / rate, n)
+    srv = M.Server(max_slots=512, phase=W.config_phase(), precision=prec,
+                   hbm_budget_bytes=200 * 10**9, max_epoch_steps=8)
+    t_next = time.perf_counter()
+    ids = []
+    for i in range(n):
+        t_next += gaps[i]
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        s, H, Cc, (vx, vy) = meshes[picks[i]]
+        ids.append(srv.submit(s.nx, s.ny, H, Cc, vx, vy, int(steps[i]), 0.4,
+                              s.x0, s.x1, s.y0, s.y1, q=s.q, outputs=("H", "C")))
+    srv.drain()
+    st = srv.stats()
+    for rid in ids:
+        srv.wait(rid)
+    srv.close()
+    return {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "max_ms": st["max_ms"], "cell_steps_per_s": st["cell_steps"] / st["elapsed_s"],
+            "completed": st["completed"], "failed": st["failed"],
+            "max_resident": st["max_resident"], "epochs": st["epochs"],
+            "hbm_peak_bytes": st["hbm_peak"]}
+
+
+def bench_serve(args, meshes, batch=None):
+    """Poisson arrivals of transport runs (meshes of the batch distribution,
+    20-200 seeded SSP-RK3 steps), continuous batching under a 200 GB HBM
+    budget; p50/p99 submit-to-done latency and aggregate cell-steps/s, at the
+    default rate and over a sweep of rates towards saturation, where the
+    throughput is set beside the batch leg's (the same meshes advanced as one
+    batch)."""
+    import mantlepar_amd as M
+    from mantlepar_amd import workloads as W
+    n = args.serve_requests
     out = {"requests": n, "rate_per_s": args.serve_rate, "hbm_budget_bytes": 200 * 10**9,
            "workload": "Poisson arrivals; mesh drawn from the 256-mesh batch pool; "
                        "20-200 SSP-RK3 steps; CFL 0.4"}
     for prec, name in ((M.PREC_MIXED, "mixed"), (M.PREC_FP64, "fp64")):
-        srv = M.Server(max_slots=512, phase=W.config_phase(), precision=prec,
-                       hbm_budget_bytes=200 * 10**9, max_epoch_steps=8)
-        t_next = time.perf_counter()
-        ids = []
-        for i in range(n):
-            t_next += gaps[i]
-            d = t_next - time.perf_counter()
-            if d > 0:
-                time.sleep(d)
-            s, H, Cc, (vx, vy) = meshes[picks[i]]
-            ids.append(srv.submit(s.nx, s.ny, H, Cc, vx, vy, int(steps[i]), 0.4,
-                                  s.x0, s.x1, s.y0, s.y1, q=s.q, outputs=("H", "C")))
-        srv.drain()
-        st = srv.stats()
-        for rid in ids:
-            srv.wait(rid)
-        srv.close()
-        out[name] = {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
-                     "max_ms": st["max_ms"], "cell_steps_per_s": st["cell_steps"] / st["elapsed_s"],
-                     "completed": st["completed"], "failed": st["failed"],
-                     "max_resident": st["max_resident"], "epochs": st["epochs"],
-                     "hbm_peak_bytes": st["hbm_peak"]}
-        log(f"serve {name}: p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms, "
+        out[name] = _serve_once(M, W, meshes, prec, args.serve_rate, n)
+        log(f"serve {name}: p50 {out[name]['p50_ms']:.2f} ms p99 {out[name]['p99_ms']:.2f} ms, "
             f"{out[name]['cell_steps_per_s']:.3e} cell-steps/s")
+    rates = [float(r) for r in args.serve_rates.split(",") if r]
+    out["rate_sweep_requests"] = {f"{r:.0f}": max(n, int(0.5 * r)) for r in rates}
+    sweep = []
+    for rate in rates:
+        row = {"rate_per_s": rate}
+        for prec, name in ((M.PREC_MIXED, "mixed"), (M.PREC_FP64, "fp64")):
+            # enough requests for about half a second of arrivals, so the
+            # run reaches its steady state between the ramp and the drain
+            r = _serve_once(M, W, meshes, prec, rate, max(n, int(0.5 * rate)), seed=17 + int(rate))
+            if batch and name in batch:
+                r["of_batch_throughput"] = r["cell_steps_per_s"] / batch[name]["value"]
+            row[name] = r
+            log(f"serve {name} at {rate:.0f}/s: p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms, "
+                f"{r['cell_steps_per_s']:.3e} cell-steps/s, max resident {r['max_resident']}")
+        sweep.append(row)
+    out["rate_sweep"] = sweep
     return out
 
 
@@ -711,7 +737,7 @@ def main():
         if not args.no_batch:
             out["batch"] = bench_batch(args, meshes)
         if not args.no_serve:
-            out["serve"] = bench_serve(args, meshes)
+            out["serve"] = bench_serve(args, meshes, out.get("batch"))
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(args)

@@ -138,8 +138,10 @@ mp_status mp_transport_set_boundary(mp_transport* t, const uint8_t* bottom, cons
 mp_status mp_transport_set_inflow(mp_transport* t, const double* inflow_H, const double* inflow_C);
 /* dt = cfl / max_K rate_K for the current face velocities. */
 mp_status mp_transport_stable_dt(mp_transport* t, double cfl, double* dt);
-/* Advance nsteps at fixed dt; each step ends with the per-cell phase.  Launches
- * are asynchronous; the report (may be NULL) synchronises. */
+/* Advance nsteps at fixed dt; each step ends with the per-cell phase.  The
+ * call enqueues every step, then waits once for the device (it reads the
+ * error word to decide whether the new state is committed); report may be
+ * NULL. */
 mp_status mp_transport_step(mp_transport* t, int32_t nsteps, int32_t scheme, double dt,
                             mp_step_report* report);
 /* mp_transport_step without device allocation.  No counterpart in the
@@ -360,7 +362,7 @@ typedef struct {
     int64_t submitted, completed, failed, queued, resident;
     double cell_steps;            /* sum over completed runs of cells * steps */
     double elapsed_s;             /* first submit to last completion */
-    double p50_ms, p99_ms, mean_ms, max_ms;   /* submit -> done latency */
+    double p50_ms, p99_ms, mean_ms, max_ms;   /* submit -> done latency, last 65536 runs */
     uint64_t hbm_in_use, hbm_peak, hbm_budget;
     int64_t epochs, max_resident;
 } mp_server_stats;
@@ -370,12 +372,14 @@ mp_status mp_server_create(int32_t max_slots, const mp_weno_options* weno,
                            const mp_phase_params* phase, int32_t precision, int32_t scheme,
                            uint64_t hbm_budget_bytes, int32_t max_epoch_steps, mp_server** out);
 mp_status mp_server_submit(mp_server* s, const mp_request* req, int64_t* id);
-/* Wait up to timeout_s (< 0: forever) for request id; MP_ERR_PENDING on timeout. */
+/* Wait up to timeout_s (< 0: forever) for request id; MP_ERR_PENDING on
+ * timeout.  A finished request's result is handed back once: the server then
+ * releases it, and a later wait on the same id fails with MP_ERR_ARG. */
 mp_status mp_server_wait(mp_server* s, int64_t id, double timeout_s, mp_request_result* out);
 /* Wait until every submitted request has completed. */
 mp_status mp_server_drain(mp_server* s);
 mp_status mp_server_stats_get(mp_server* s, mp_server_stats* out);
-/* Rows per work item for runs admitted afterwards (>= 8; default 8).  A run's
+/* Rows per work item for runs admitted afterwards (>= 8; default 12).  A run's
  * reported boundary sums depend on it at rounding level; states and phase
  * outputs do not. */
 mp_status mp_server_set_tile(mp_server* s, int32_t rows_per_item);

@@ -159,6 +159,8 @@ __device__ __forceinline__ double boundary_flux(const MeshDev& m, const double* 
                                                 const double* inflow, uint32_t& err)
 {
     double F = 0.0;
+    // a zero-flux face carries +0 whatever its velocity: skip the loads
+    if (policy == 0) return F;
     for (int q = 0; q < m.q; ++q) {
         const double v = sign * gmem(vn)[q * vstride];
         double f = 0.0;

@@ -89,6 +89,13 @@ void Arena::release(void* p)
 Engine::~Engine()
 {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (side) (void)hipStreamSynchronize(side);
+    for (int k = 0; k < 2; ++k) {
+        if (stage3[k]) (void)hipEventDestroy(stage3[k]);
+        if (side_done[k]) (void)hipEventDestroy(side_done[k]);
+    }
+    if (side_join) (void)hipEventDestroy(side_join);
+    if (side) (void)hipStreamDestroy(side);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
     if (h_stages) (void)hipHostFree(h_stages);
 }
@@ -109,6 +116,14 @@ mp_status Engine::init(int cap, int32_t prec, const mp_weno_options& w, const mp
     } else {
         MP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         own_stream = true;
+    }
+    if (prec == MP_PREC_MIXED) {
+        MP_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        for (int k = 0; k < 2; ++k) {
+            MP_HIP(hipEventCreateWithFlags(&stage3[k], hipEventDisableTiming));
+            MP_HIP(hipEventCreateWithFlags(&side_done[k], hipEventDisableTiming));
+        }
+        MP_HIP(hipEventCreateWithFlags(&side_join, hipEventDisableTiming));
     }
     arena.configure(budget_bytes, size_t(1) << 30);
     slots.assign(size_t(cap), Slot());
@@ -150,7 +165,7 @@ size_t Engine::slot_bytes(const mp_mesh_desc& d, int32_t precision, int rows_min
         b += Arena::round(faces_x(d) * sizeof(FaceRec)) + Arena::round(faces_y(d) * sizeof(FaceRec));
     }
     b += 2 * Arena::round(N * (4 * sizeof(double) + 1));
-    b += Arena::round(size_t(3) * items_for(d, rows_min) * kPartials * sizeof(double));
+    b += Arena::round(size_t(6) * items_for(d, rows_min) * kPartials * sizeof(double));
     b += Arena::round(size_t(2) * (d.nx + d.ny));
     b += Arena::round(size_t(4) * (d.nx + d.ny) * d.q * sizeof(double));
     b += Arena::round(size_t(d.ny) * sizeof(TepPair));
@@ -186,7 +201,7 @@ mp_status Engine::add_mesh(const mp_mesh_desc& d, int* out)
     }
     sl.phase[0] = reinterpret_cast<double*>(take(N * (4 * sizeof(double) + 1)));
     sl.phase[1] = reinterpret_cast<double*>(take(N * (4 * sizeof(double) + 1)));
-    sl.partials = reinterpret_cast<double*>(take(size_t(3) * items_for(d, 8) * kPartials * sizeof(double)));
+    sl.partials = reinterpret_cast<double*>(take(size_t(6) * items_for(d, 8) * kPartials * sizeof(double)));
     sl.bc = reinterpret_cast<uint8_t*>(take(size_t(2) * (d.nx + d.ny)));
     sl.inflow = reinterpret_cast<double*>(take(size_t(4) * (d.nx + d.ny) * d.q * sizeof(double)));
     sl.tep = reinterpret_cast<TepPair*>(take(size_t(d.ny) * sizeof(TepPair)));
@@ -409,7 +424,7 @@ mp_status Engine::set_active(const std::vector<int>& list, const std::vector<int
     if (q_common < 0) q_common = 0;
     // Two item lists per distinct prefix of the ordered slots (a prefix is the
     // set still stepping at some step), both over the same logical order:
-    // mesh after mesh, each mesh's items in item order.
+    // items by decreasing cost, mesh after mesh among equal costs.
     //  * queue order: the logical order itself, for the persistent grid whose
     //    waves take items from a counter;
     //  * XCD slabs, for one wave per item: the dispatcher places workgroup b on
@@ -441,6 +456,22 @@ mp_status Engine::set_active(const std::vector<int>& list, const std::vector<int
                 cost.push_back(double(j1 - j0) + 2.0);
                 total += cost.back();
             }
+        }
+        // Longest items first (a stable sort keeps each mesh's items in item
+        // order among equals): the items a launch dispatches last, whose
+        // durations set its drain, are the shortest.
+        {
+            std::vector<size_t> idx(logical.size());
+            for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+            std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+            std::vector<int2> l2(logical.size());
+            std::vector<double> c2(cost.size());
+            for (size_t i = 0; i < idx.size(); ++i) {
+                l2[i] = logical[idx[i]];
+                c2[i] = cost[idx[i]];
+            }
+            logical.swap(l2);
+            cost.swap(c2);
         }
         List L;
         L.queue_off = int(items.size());
@@ -487,7 +518,9 @@ int Engine::steps_of(int k) const
 }
 
 // Three pointer sets per slot: step 0 (cur -> o1), odd steps (o1 -> o2),
-// even steps >= 2 (o2 -> o1); each with the scheme's stages.
+// even steps >= 2 (o2 -> o1); each with the scheme's stages.  Odd steps use
+// the second set of stage partials, so a step's totals can be reduced beside
+// the next step.
 mp_status Engine::write_tables(int scheme)
 {
     for (int k : active) {
@@ -516,7 +549,7 @@ mp_status Engine::write_tables(int scheme)
                     sp.T = base; sp.phi = base + s.N; sp.Cl = base + 2 * s.N; sp.Cs = base + 3 * s.N;
                     sp.region = reinterpret_cast<uint8_t*>(base + 4 * s.N);
                 }
-                sp.partials = s.partials + size_t(st) * s.nitems * kPartials;
+                sp.partials = s.partials + size_t((set == 1 ? 3 : 0) + st) * s.nitems * kPartials;
                 sp.error = d_errors.as<uint32_t>() + k;
                 apply_layout(s.layout, sp);
                 h_stages[size_t(set * 3 + st) * capacity + k] = sp;
@@ -537,9 +570,28 @@ mp_status Engine::reserve_steps(int nsteps)
     return MP_OK;
 }
 
+// The mixed path runs each step's phase kernel and per-mesh totals on the
+// side stream, beside the next step's stages: the phase reads the step's
+// output states, which the next step only reads, and writes the phase outputs
+// and the sum slots of the step's partials (the parity set of its step).
+// Step s + 2 writes step s's output states and partial set again, so it first
+// waits for step s's phase and totals; the main stream waits for the side
+// stream at the end, error returns included.  The fork and join are events,
+// so a captured graph holds them too.  Profiling (events) keeps everything on
+// the main stream, so each launch is timed alone.
 mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* events)
 {
     if (nsteps > steps_capacity) return fail(MP_ERR_STATE, "per-step totals not reserved");
+    const bool overlap = precision == MP_PREC_MIXED && side && !events;
+    struct Join {
+        hipStream_t main, side;
+        hipEvent_t ev;
+        bool on;
+        ~Join()
+        {
+            if (on && hipEventRecord(ev, side) == hipSuccess) (void)hipStreamWaitEvent(main, ev, 0);
+        }
+    } join{stream, side, side_join, overlap && nsteps > 0};
     MP_HIP(launch_clear(d_errors.as<uint32_t>(), capacity, stream));
     if (blocks > 0) MP_HIP(launch_clear(d_queues.as<uint32_t>(), 3 * nsteps, stream));
     const StagePtrs* base = d_stages.as<StagePtrs>();
@@ -558,6 +610,8 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
         const List& Lst = lists.at(p);
         const std::pair<int, int> L = blocks > 0 ? std::make_pair(Lst.queue_off, Lst.queue_count)
                                                  : std::make_pair(Lst.slab_off, Lst.slab_count);
+        const int par = set == 1 ? 1 : 0;    // the partial set of this step
+        if (overlap && s >= 2) MP_HIP(hipStreamWaitEvent(stream, side_done[par], 0));
         for (int st = 0; st < scheme; ++st) {
             // the mixed sweep leaves the phase to its own streaming kernel, as
             // the single-mesh handle does
@@ -567,15 +621,22 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
             MP_HIP(launch_batch_stage(precision, last && !split, q_common, d_meshes.as<MeshDev>(), sps,
                                       d_items.as<int2>() + L.first, L.second, blocks,
                                       d_queues.as<int>() + 3 * s + st, stream, weno.lf_mode != 0));
-            if (split)
+            if (split && overlap) {
+                MP_HIP(hipEventRecord(stage3[par], stream));
+                MP_HIP(hipStreamWaitEvent(side, stage3[par], 0));
+                MP_HIP(launch_batch_phase_items(d_meshes.as<MeshDev>(), sps, d_items.as<int2>() + L.first,
+                                                L.second, side));
+            } else if (split) {
                 MP_HIP(launch_batch_phase_items(d_meshes.as<MeshDev>(), sps, d_items.as<int2>() + L.first,
                                                 L.second, stream));
+            }
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
         }
         MP_HIP(launch_batch_reduce(base + size_t(set * 3) * capacity, d_nitems.as<int>(),
                                    d_meshes.as<MeshDev>(), d_active.as<int>(), p, scheme,
                                    kW[scheme - 1], d_totals.as<double>() + size_t(s) * 4 * capacity,
-                                   stream));
+                                   overlap ? side : stream));
+        if (overlap) MP_HIP(hipEventRecord(side_done[par], side));
     }
     return MP_OK;
 }

@@ -55,7 +55,7 @@ struct Slot {
     double* H[5] = {};
     double* C[5] = {};
     double* phase[2] = {};     // T, phi, Cl, Cs (N each) then region bytes
-    double* partials = nullptr;
+    double* partials = nullptr;   // two step-parity sets of three stages
     uint8_t* bc = nullptr;     // bottom(nx) right(ny) top(nx) left(ny) policies
     double* inflow = nullptr;  // H then C, each [face][q] in side order
     TepPair* tep = nullptr;    // lithostatic {Tep, Tmp} per row (md.tep_row when enabled)
@@ -119,6 +119,12 @@ public:
     PhaseHost phase{};
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // Mixed precision: each step's phase kernel and totals run on `side`,
+    // beside the next step's stages (stage3: the step's last stage is done;
+    // side_done: its phase and totals are done; by step parity).
+    hipStream_t side = nullptr;
+    hipEvent_t stage3[2] = {nullptr, nullptr}, side_done[2] = {nullptr, nullptr};
+    hipEvent_t side_join = nullptr;
     Arena arena;
     std::vector<Slot> slots;
     std::vector<int> active;         // slots, ordered by steps (descending)

@@ -1,10 +1,11 @@
 // Serving front end: continuous batching of transport runs on one device.
 //
 // Callers submit requests from any thread.  One server thread owns the
-// engine: at each epoch boundary it admits waiting requests (arrival order,
-// first fit) while slots and the HBM budget allow, advances every resident run
-// by the same number of steps (one persistent launch per stage over all of
-// them), and retires runs that reached their step count.  A run's results are
+// engine: while an epoch runs it admits waiting requests (arrival order,
+// first fit) while slots and the HBM budget allow, uploading them on a second
+// stream; then it waits for the epoch, retires runs that reached their step
+// count, and starts the next epoch, which advances every resident run by up
+// to E steps (one launch per stage over all of them).  A run's results are
 // independent of its co-residents: each mesh has its own tables, dt,
 // partials and error word, and the stage kernel's per-mesh arithmetic does
 // not depend on the work list.
@@ -46,14 +47,18 @@ struct mp_server {
     int scheme = MP_SCHEME_SSPRK3;
     int max_epoch = 8;
     int device = 0;
-    int rows = 8;                      // rows per work item of every admitted run
+    int rows = 12;                     // rows per work item of every admitted run (the batch autotuner's usual pick)
     std::vector<int> last_list, last_steps;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> pending;
-    std::map<int64_t, Request*> all;
+    std::map<int64_t, Request*> all;   // submitted and not yet collected by mp_server_wait
+    // Latencies (ms) of the most recent completed runs, a ring of kLatencies.
+    static constexpr size_t kLatencies = size_t(1) << 16;
+    std::vector<double> latencies;
+    size_t latency_next = 0;
     std::vector<Request*> resident;
     int64_t next_id = 1, submitted = 0, completed = 0, failed = 0;
     int64_t epochs = 0, max_resident = 0;
@@ -62,6 +67,19 @@ struct mp_server {
     bool stop = false;
     bool membership_changed = true;   // a slot was filled or freed since the last work list
     std::thread worker;
+    // Requests are admitted on `upload` while the previous epoch runs on the
+    // engine's stream; the engine's stream waits for `uploaded` before its
+    // next work.
+    hipStream_t upload = nullptr;
+    hipEvent_t uploaded = nullptr;
+    hipEvent_t epoch_done = nullptr;   // recorded after the epoch in flight
+    // The epoch in flight: its runs, step count and launch status.
+    struct Flight {
+        bool on = false;
+        int E = 0;
+        mp_status st = MP_OK;
+        std::vector<Request*> run;
+    } flight;
 
     ~mp_server()
     {
@@ -72,6 +90,9 @@ struct mp_server {
         cv_work.notify_all();
         if (worker.joinable()) worker.join();
         for (auto& kv : all) delete kv.second;
+        if (uploaded) (void)hipEventDestroy(uploaded);
+        if (epoch_done) (void)hipEventDestroy(epoch_done);
+        if (upload) (void)hipStreamDestroy(upload);
     }
 
     double now() const
@@ -80,8 +101,9 @@ struct mp_server {
     }
 
     void loop();
-    void admit(std::vector<Request*>& batch);
-    void epoch();
+    void admit(std::vector<Request*>& batch, int rows_now);
+    void start_epoch();
+    void finish_epoch();
     void finish(Request* q, mp_status st, const char* msg);
 };
 
@@ -104,17 +126,42 @@ void mp_server::finish(Request* q, mp_status st, const char* msg)
         ++failed;
     }
     last_done = q->res.t_done;
+    if (st == MP_OK) {
+        const double ms = 1e3 * (q->res.t_done - q->res.t_submit);
+        if (latencies.size() < kLatencies) latencies.push_back(ms);
+        else latencies[latency_next] = ms;
+        latency_next = (latency_next + 1) % kLatencies;
+    }
     hbm_in_use = eng.arena.in_use();
     q->r.H = q->r.C = q->r.vnx = q->r.vny = nullptr;
     cv_done.notify_all();
 }
 
 // Bring admitted requests onto the device: slot, boundary data, state,
-// velocities, then one read of all their advective rates.
-void mp_server::admit(std::vector<Request*>& batch)
+// velocities, then one read of all their advective rates.  An upload takes
+// about 0.2 ms of host time, so while an epoch runs, admission stops once that
+// epoch has completed (after at least kMinAdmit requests) and hands the rest
+// back to the front of the queue: the GPU is never left idle behind a long
+// admission.
+void mp_server::admit(std::vector<Request*>& batch, int rows_now)
 {
+    constexpr size_t kMinAdmit = 4;
     std::vector<Request*> ok;
-    for (Request* q : batch) {
+    for (size_t i = 0; i < batch.size(); ++i) {
+        if (i >= kMinAdmit && flight.on && hipEventQuery(epoch_done) == hipSuccess) {
+            // back into the queue in arrival (id) order
+            std::lock_guard<std::mutex> g(mu);
+            std::deque<Request*> merged;
+            auto a = batch.begin() + long(i);
+            auto b = pending.begin();
+            while (a != batch.end() || b != pending.end()) {
+                if (b == pending.end() || (a != batch.end() && (*a)->id < (*b)->id)) merged.push_back(*a++);
+                else merged.push_back(*b++);
+            }
+            pending.swap(merged);
+            break;
+        }
+        Request* q = batch[i];
         int k = -1;
         mp_status st = eng.add_mesh(q->r.mesh, &k);
         if (st == MP_OK) {
@@ -152,32 +199,30 @@ void mp_server::admit(std::vector<Request*>& batch)
         s.dt = q->r.cfl / s.max_rate;
         q->res.dt = s.dt;
         q->res.t_admit = t;
-        eng.set_rows(q->slot, rows);
+        eng.set_rows(q->slot, rows_now);
         resident.push_back(q);
         membership_changed = true;
     }
 }
 
-void mp_server::epoch()
+// Ragged epoch: every resident run takes min(remaining, E) steps, so a run
+// never waits for the shortest co-resident one.  The launches are enqueued
+// here and left running; finish_epoch collects them.
+void mp_server::start_epoch()
 {
-    // Ragged epoch: every resident run takes min(remaining, E) steps, so a run
-    // never waits for the shortest co-resident one; runs with zero steps
-    // retire at once.
-    std::vector<Request*> run;
+    Flight f;
+    f.on = true;
     for (Request* q : resident) {
-        if (q->remaining > 0) run.push_back(q);
+        if (q->remaining > 0) f.run.push_back(q);
     }
-    int E = 0;
-    for (Request* q : run) E = std::max(E, std::min(q->remaining, max_epoch));
-    std::vector<uint32_t> errors(size_t(eng.capacity), 0u);
-    std::vector<double> totals;
-    mp_status st = MP_OK;
-    if (!run.empty()) {
+    for (Request* q : f.run) f.E = std::max(f.E, std::min(q->remaining, max_epoch));
+    if (!f.run.empty()) {
         std::vector<int> list, steps;
-        for (Request* q : run) {
+        for (Request* q : f.run) {
             list.push_back(q->slot);
-            steps.push_back(std::min(q->remaining, E));
+            steps.push_back(std::min(q->remaining, f.E));
         }
+        mp_status st = MP_OK;
         // A freed slot may be refilled by a mesh of another size under the
         // same id, so the work list follows membership events, not ids.
         if (membership_changed || list != last_list || steps != last_steps) {
@@ -186,12 +231,32 @@ void mp_server::epoch()
             last_steps = steps;
         }
         if (!st) membership_changed = false;
-        if (!st) st = eng.reserve_steps(E);
+        if (!st) st = eng.reserve_steps(f.E);
         if (!st) st = eng.write_tables(scheme);
-        if (!st) st = eng.enqueue(E, scheme, nullptr);
+        if (!st) st = eng.enqueue(f.E, scheme, nullptr);
+        if (!st && hipEventRecord(epoch_done, eng.stream) != hipSuccess)
+            st = fail(MP_ERR_DEVICE, "epoch event");
+        f.st = st;
+        ++epochs;
+    }
+    flight = std::move(f);
+}
+
+// Wait for the epoch in flight, commit its runs, and retire those that are
+// done (runs admitted with zero steps retire here too, once their upload has
+// landed: the engine's stream waits for the upload stream).
+void mp_server::finish_epoch()
+{
+    if (!flight.on) return;
+    const std::vector<Request*> run = flight.run;
+    const int E = flight.E;
+    mp_status st = flight.st;
+    flight = Flight();
+    std::vector<uint32_t> errors(size_t(eng.capacity), 0u);
+    std::vector<double> totals;
+    if (!run.empty()) {
         if (!st) st = eng.fetch_status(E, errors, &totals);
         if (!st) eng.commit(E, errors);
-        ++epochs;
     }
     std::vector<Request*> keep, retire;
     for (Request* q : resident) {
@@ -245,15 +310,21 @@ void mp_server::epoch()
     }
 }
 
+// Pipelined: while epoch k runs on the engine's stream, the worker admits the
+// requests that arrived (their uploads, face moments and rates on the upload
+// stream), then waits for epoch k, retires its finished runs and starts epoch
+// k + 1 with the new runs in it.
 void mp_server::loop()
 {
     (void)hipSetDevice(device);
     for (;;) {
         std::vector<Request*> batch;
+        int rows_now = 8;
         {
             std::unique_lock<std::mutex> lk(mu);
+            rows_now = rows;
             cv_work.wait(lk, [&] { return stop || !pending.empty() || !resident.empty(); });
-            if (stop && resident.empty()) {
+            if (stop && resident.empty() && !flight.on) {
                 for (Request* q : pending) {
                     q->res.status = MP_ERR_STATE;
                     std::snprintf(q->message, sizeof q->message, "server shut down before admission");
@@ -280,14 +351,24 @@ void mp_server::loop()
                 }
             }
         }
-        if (!batch.empty()) admit(batch);
+        if (!batch.empty()) {
+            // the engine works on the upload stream for the admission, so its
+            // synchronisations wait for the uploads only, not for the epoch
+            const hipStream_t main = eng.stream;
+            eng.stream = upload;
+            admit(batch, rows_now);
+            eng.stream = main;
+            if (hipEventRecord(uploaded, upload) == hipSuccess)
+                (void)hipStreamWaitEvent(main, uploaded, 0);
+        }
+        finish_epoch();
         {
             std::lock_guard<std::mutex> g(mu);
             hbm_in_use = eng.arena.in_use();
             hbm_peak = std::max(hbm_peak, uint64_t(hbm_in_use));
             max_resident = std::max(max_resident, int64_t(resident.size()));
         }
-        if (!resident.empty()) epoch();
+        if (!resident.empty()) start_epoch();
     }
 }
 
@@ -316,6 +397,12 @@ mp_status mp_server_create(int32_t max_slots, const mp_weno_options* weno, const
         MP_HIP(hipGetDevice(&s->device));
         mp_status st = s->eng.init(max_slots, precision, w, p, nullptr, size_t(hbm_budget_bytes));
         if (st) { delete s; return st; }
+        if (hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s->uploaded, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s->epoch_done, hipEventDisableTiming) != hipSuccess) {
+            delete s;
+            return fail(MP_ERR_DEVICE, "server upload stream");
+        }
         s->scheme = scheme;
         s->max_epoch = max_epoch_steps;
         // One wave per work item: with few resident runs this measured faster
@@ -377,7 +464,13 @@ mp_status mp_server_wait(mp_server* s, int64_t id, double timeout_s, mp_request_
         return fail(MP_ERR_PENDING, "request %lld not finished", (long long)id);
     }
     if (out) *out = q->res;
-    if (q->res.status != MP_OK) return fail(mp_status(q->res.status), "request %lld: %s", (long long)id, q->message);
+    // the result is handed back once: the request is released
+    const mp_status st = mp_status(q->res.status);
+    char message[sizeof q->message];
+    std::memcpy(message, q->message, sizeof message);
+    s->all.erase(it);
+    delete q;
+    if (st != MP_OK) return fail(st, "request %lld: %s", (long long)id, message);
     return MP_OK;
 }
 
@@ -401,10 +494,7 @@ mp_status mp_server_stats_get(mp_server* s, mp_server_stats* o)
     o->resident = int64_t(s->resident.size());
     o->cell_steps = s->cell_steps;
     o->elapsed_s = s->first_submit >= 0 ? s->last_done - s->first_submit : 0.0;
-    std::vector<double> lat;
-    for (auto& kv : s->all)
-        if (kv.second->done && kv.second->res.status == MP_OK)
-            lat.push_back(1e3 * (kv.second->res.t_done - kv.second->res.t_submit));
+    std::vector<double> lat(s->latencies);
     if (!lat.empty()) {
         std::sort(lat.begin(), lat.end());
         auto rank = [&](double p) {

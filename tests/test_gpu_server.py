@@ -77,7 +77,7 @@ def test_alone_equals_mixed(prec):
         s = r["spec"]
         t = M.Transport(s.nx, s.ny, s.x0, s.x1, s.y0, s.y1, q=s.q, phase=W.config_phase(),
                         precision=prec)
-        t.set_tile(8)                  # the server's rows per work item
+        t.set_tile(12)                 # the server's rows per work item
         t.upload(r["H"], r["C"])
         t.set_face_velocity(r["vx"], r["vy"])
         rep = t.step(r["nsteps"], t.stable_dt(0.4))
@@ -111,7 +111,7 @@ def test_lithostatic_requests_equal_single_mesh(prec):
         s = r["spec"]
         t = M.Transport(s.nx, s.ny, s.x0, s.x1, s.y0, s.y1, q=s.q, phase=W.config_phase(),
                         precision=prec)
-        t.set_tile(8)
+        t.set_tile(12)
         t.upload(r["H"], r["C"])
         t.set_face_velocity(r["vx"], r["vy"])
         if "surface" in r:
