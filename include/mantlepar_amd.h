@@ -55,7 +55,8 @@ enum {
     MP_ERR_MEMORY = 6,     /* allocation or HBM budget exhausted */
     MP_ERR_BACKFLOW = 7,   /* backflow on an outflow-only boundary (initialization.cpp:478-481) */
     MP_ERR_COMPOSITION = 8, /* C outside the admissible interval (eutectic_rescaled.cpp:88-93) */
-    MP_ERR_PENDING = 9      /* request not finished within the wait timeout */
+    MP_ERR_PENDING = 9,     /* request not finished within the wait timeout */
+    MP_ERR_BALANCE = 10     /* a step's conservation balance failed (PETSC_ERR_PLIB) */
 };
 
 enum { MP_PREC_FP64 = 0, MP_PREC_MIXED = 1 };
@@ -151,9 +152,9 @@ mp_status mp_transport_step(mp_transport* t, int32_t nsteps, int32_t scheme, dou
  * per-step totals and the error word) lives in a static 32 MiB device array of
  * the library instead of buffers the handle allocates; the states, face data
  * and phase outputs are the handle's, allocated at create time.  Results are
- * bitwise those of mp_transport_step.  A call needing more than 32 MiB (384
- * bytes per work item plus 32 per step: an 8192 x 8192 mesh fits) fails with
- * MP_ERR_MEMORY.  The
+ * bitwise those of mp_transport_step.  Beside a fixed 3 MiB of completion
+ * flags, a call may use 29 MiB: 384 bytes per work item plus 32 per step (an
+ * 8192 x 8192 mesh fits); a larger one fails with MP_ERR_MEMORY.  The
  * handle's device must be current.  Each GPU holds one such array, shared by
  * every call on it: calls that overlap on one GPU (several handles on
  * different threads or streams) race; serialise them. */
@@ -190,6 +191,15 @@ mp_status mp_transport_synchronize(mp_transport* t);
  * its phase kernel. */
 mp_status mp_transport_profile(mp_transport* t, int32_t nsteps, int32_t scheme, double dt,
                                double* ms_total, double* ms_stage, double* ms_phase_stage);
+/* Every step of mp_transport_step is checked against the preheat's
+ * conservation balance (src/couple/preheat.cpp:268-273), for H and for C:
+ * |total(after) - total(before) + dt*outward boundary flux| must not exceed
+ * tol*(1 + |after| + |before| + |dt*flux|), the totals being the compensated
+ * sums of |K|*u.  A step that fails is rejected with MP_ERR_BALANCE
+ * ("Preheat heat balance failed; candidate rejected", PETSC_ERR_PLIB there)
+ * and the state is left unchanged.  tol is 1e-10, the reference's value,
+ * unless set here (finite, >= 0). */
+mp_status mp_transport_set_balance_tolerance(mp_transport* t, double tol);
 /* Sweep tile shape: rows per work item (>= 4). */
 mp_status mp_transport_set_tile(mp_transport* t, int32_t rows_per_item);
 /* How a mixed-precision handle launches its SSPRK2/SSPRK3 steps:

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libmantlepar_amd.so")
 MP_OK = 0
 STATUS_NAMES = {1: "MP_ERR_ARG", 2: "MP_ERR_FP", 3: "MP_ERR_STATE", 4: "MP_ERR_PHASE",
                 5: "MP_ERR_DEVICE", 6: "MP_ERR_MEMORY", 7: "MP_ERR_BACKFLOW",
-                8: "MP_ERR_COMPOSITION", 9: "MP_ERR_PENDING"}
+                8: "MP_ERR_COMPOSITION", 9: "MP_ERR_PENDING", 10: "MP_ERR_BALANCE"}
 MP_ERR_PENDING = 9
 BATCH_EAGER, BATCH_GRAPH = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
@@ -155,6 +155,7 @@ SIGNATURES = [
     ("mp_transport_synchronize", C.c_int32, [_vp]),
     ("mp_transport_profile", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double, _dp, _dp, _dp]),
     ("mp_transport_set_tile", C.c_int32, [_vp, C.c_int32]),
+    ("mp_transport_set_balance_tolerance", C.c_int32, [_vp, C.c_double]),
     ("mp_transport_set_launch", C.c_int32, [_vp, C.c_int32]),
     ("mp_transport_layout", C.c_int32, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),

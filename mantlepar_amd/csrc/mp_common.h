@@ -28,6 +28,7 @@ enum : uint32_t {
     kErrCoupledRange = 1u << 8,   // coupled law: T < 0 or phi >= 1 at a point
     kErrLFSpeed = 1u << 9,        // coupled law: global LF speed below a flow speed
     kErrStall = 1u << 10,         // one-launch step: an item's inputs were not done in time
+    kErrBalance = 1u << 11,       // a step's conservation balance failed (PETSC_ERR_PLIB)
 };
 
 // Binary eutectic parameters as the device uses them.  Tep/Tmp are derived

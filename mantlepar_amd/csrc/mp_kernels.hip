@@ -475,8 +475,14 @@ __global__ __launch_bounds__(64) void reduce_step_kernel(
         for (int k = 0; k < 6; ++k) part[size_t(slice) * 6 + k] = v[k];
 }
 
+// The step's totals; with prev (the totals before the step) also the
+// preheat's balance check of each field (src/couple/preheat.cpp:268-273):
+// |after - before + dt*outward flux| <= tol*(1 + |after| + |before| + |dt*flux|),
+// kErrBalance otherwise.
 __global__ __launch_bounds__(kStepSlices) void reduce_step_final_kernel(const double* __restrict__ part,
-                                                                       double area, double* out)
+                                                                       double area, double* out,
+                                                                       const double* prev, double dt,
+                                                                       double tol, uint32_t* error)
 {
     __shared__ double sh[6][kStepSlices];
     const int tid = threadIdx.x;
@@ -484,7 +490,47 @@ __global__ __launch_bounds__(kStepSlices) void reduce_step_final_kernel(const do
     for (int k = 0; k < 6; ++k) sh[k][tid] = part[size_t(tid) * 6 + k];
     __syncthreads();
     step_tree(sh, tid);
-    if (tid == 0) step_out(sh, area, out);
+    if (tid == 0) {
+        step_out(sh, area, out);
+        if (prev) {
+            bool ok = true;
+            for (int f = 0; f < 2; ++f) {
+                const double after = out[f], before = prev[f], flux = dt * out[2 + f];
+                const double bal = after - before + flux;
+                ok = ok && fabs(bal) <= tol * (1.0 + fabs(after) + fabs(before) + fabs(flux));
+            }
+            if (!ok) atomicOr(error, kErrBalance);
+        }
+    }
+}
+
+// Compensated sums of a state (H, C over n cells) as a step's slices:
+// part[slice][6] with the boundary slots zero.  One wave per slice.
+__global__ __launch_bounds__(64) void state_sums_kernel(const double* __restrict__ H,
+                                                        const double* __restrict__ C, int64_t n,
+                                                        double* __restrict__ part)
+{
+    const int slice = int(blockIdx.x), lane = int(threadIdx.x);
+    const int64_t chunk = (n + kStepSlices - 1) / kStepSlices;
+    const int64_t lo = min(n, int64_t(slice) * chunk), hi = min(n, lo + chunk);
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t k = lo + lane; k < hi; k += kWave) {
+        two_sum_acc(v[0], v[1], H[k]);
+        two_sum_acc(v[2], v[3], C[k]);
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        double o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = __shfl_xor(v[k], off);
+        double s0 = v[0], e0 = v[1] + o[1];
+        two_sum_acc(s0, e0, o[0]);
+        double s1 = v[2], e1 = v[3] + o[3];
+        two_sum_acc(s1, e1, o[2]);
+        v[0] = s0; v[1] = e0; v[2] = s1; v[3] = e1;
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) part[size_t(slice) * 6 + k] = v[k];
 }
 
 // ----------------------------------------------------------- batched sweep
@@ -1032,11 +1078,20 @@ hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H
 }
 
 hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, const double* w,
-                              double area, double* out, double* scratch, hipStream_t stream)
+                              double area, double* out, double* scratch, hipStream_t stream,
+                              const double* prev, double dt, double tol, uint32_t* error)
 {
     reduce_step_kernel<<<kStepSlices, kWave, 0, stream>>>(partials, nitems, nstages, w[0], w[1], w[2],
                                                            scratch);
-    reduce_step_final_kernel<<<1, kStepSlices, 0, stream>>>(scratch, area, out);
+    reduce_step_final_kernel<<<1, kStepSlices, 0, stream>>>(scratch, area, out, prev, dt, tol, error);
+    return hipGetLastError();
+}
+
+hipError_t launch_state_sums(const double* H, const double* C, int64_t n, double area, double* out,
+                             double* scratch, hipStream_t stream)
+{
+    state_sums_kernel<<<kStepSlices, kWave, 0, stream>>>(H, C, n, scratch);
+    reduce_step_final_kernel<<<1, kStepSlices, 0, stream>>>(scratch, area, out, nullptr, 0.0, 0.0, nullptr);
     return hipGetLastError();
 }
 

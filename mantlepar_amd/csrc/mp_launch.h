@@ -79,7 +79,14 @@ hipError_t launch_batch_phase_items(const MeshDev* meshes, const StagePtrs* stag
 // Step totals (two passes over kStepSlices wave-sized slices of the item
 // list); scratch: kStepSlices * 6 doubles.
 constexpr int kStepSlices = 128;
+// prev (may be null): the totals before the step; the step is then checked
+// against the conservation balance with relative tolerance tol (kErrBalance).
 hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, const double* w,
-                              double area, double* out, double* scratch, hipStream_t stream);
+                              double area, double* out, double* scratch, hipStream_t stream,
+                              const double* prev = nullptr, double dt = 0.0, double tol = 0.0,
+                              uint32_t* error = nullptr);
+// out[0..1]: compensated |K|-weighted sums of H and C (out[2..3] zero).
+hipError_t launch_state_sums(const double* H, const double* C, int64_t n, double area, double* out,
+                             double* scratch, hipStream_t stream);
 
 }  // namespace mp

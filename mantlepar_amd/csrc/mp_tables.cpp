@@ -43,6 +43,8 @@ mp_status status_from_bits(uint32_t bits)
     if (bits & kErrFraction) return MP_ERR_PHASE;
     if (bits & kErrInflow) return MP_ERR_ARG;
     if (bits & (kErrNonFinite | kErrWeights | kErrPhaseRange)) return MP_ERR_FP;
+    // last: a non-finite state fails the balance too, and is the cause
+    if (bits & kErrBalance) return MP_ERR_BALANCE;
     return MP_OK;
 }
 
@@ -59,6 +61,7 @@ const char* describe_bits(uint32_t bits)
     if (bits & kErrNonFinite) return "Non-finite state, trace or flux";
     if (bits & kErrWeights) return "No finite active log weight";
     if (bits & kErrPhaseRange) return "Phase temperature/enthalpy intervals are not resolvable";
+    if (bits & kErrBalance) return "Preheat heat balance failed; candidate rejected";
     return "ok";
 }
 

@@ -153,12 +153,13 @@ void mp_transport::set_tile(int r)
 }
 
 // Work memory of a step over nitems work items and nsteps steps: two sets of
-// three stage partials, the step reduction's block results, the per-step
-// totals and the error word (the last two 256-byte aligned).
+// three stage partials, the step reduction's block results, the totals (of the
+// starting state, then of each step) and the error word (the last two
+// 256-byte aligned).
 size_t mp_transport::work_bytes(int nitems, int nsteps)
 {
     const size_t part = (size_t(6) * nitems * kPartials + size_t(kStepSlices) * 6) * sizeof(double);
-    const size_t tot = size_t(nsteps > 0 ? nsteps : 1) * 4 * sizeof(double);
+    const size_t tot = size_t(nsteps + 1) * 4 * sizeof(double);
     auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
     return up(part) + up(tot) + 256;
 }
@@ -169,7 +170,7 @@ mp_status mp_transport::ensure_scratch(int nsteps)
     // block results
     const size_t pbytes = (size_t(6) * nitems * kPartials + size_t(kStepSlices) * 6) * sizeof(double);
     if (partials.bytes < pbytes) MP_CALL(partials.alloc(pbytes));
-    const size_t tbytes = size_t(nsteps > 0 ? nsteps : 1) * 4 * sizeof(double);
+    const size_t tbytes = size_t(nsteps + 1) * 4 * sizeof(double);
     if (totals.bytes < tbytes) MP_CALL(totals.alloc(tbytes));
     const size_t fbytes = size_t(3) * nitems * sizeof(uint32_t);
     if (flags.bytes < fbytes) {
@@ -362,6 +363,9 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
     } join{stream, side, side_join, overlap && nsteps > 0};
     const size_t set_doubles = size_t(nitems) * kPartials;
     double* scratch = w.partials + 6 * set_doubles;
+    // the starting state's totals, against which the first step is balanced
+    if (nsteps > 0) MP_HIP(launch_state_sums(H[cur].as<double>(), C[cur].as<double>(), int64_t(N), md.area,
+                                             w.totals, scratch, stream));
     const bool fused = step_fused(scheme);
     if (fused) {
         // Stage records of the call's three kinds of step (step 0, odd steps,
@@ -433,9 +437,12 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
             kin = out;
         }
+        // totals of step s in slot s + 1, checked against slot s (the
+        // preheat's balance, src/couple/preheat.cpp:268-273)
         MP_HIP(launch_reduce_step(w.partials + size_t(par) * 3 * set_doubles, nitems, scheme,
-                                  kW[scheme - 1], md.area, w.totals + size_t(s) * 4, scratch,
-                                  overlap ? side : stream));
+                                  kW[scheme - 1], md.area, w.totals + size_t(s + 1) * 4, scratch,
+                                  overlap ? side : stream, w.totals + size_t(s) * 4, dt,
+                                  balance_tol, w.error));
         if (overlap) MP_HIP(hipEventRecord(side_done[par], side));
         src = dst;
         final_state = dst;
@@ -498,7 +505,7 @@ mp_status mp_transport::finish_step(const StepWork& w, int nsteps, int scheme, d
         double tot[kChunk * 4];
         for (int s0 = 0; s0 < nsteps; s0 += kChunk) {
             const int n = nsteps - s0 < kChunk ? nsteps - s0 : kChunk;
-            MP_HIP(hipMemcpy(tot, w.totals + size_t(s0) * 4, size_t(n) * 4 * sizeof(double),
+            MP_HIP(hipMemcpy(tot, w.totals + size_t(s0 + 1) * 4, size_t(n) * 4 * sizeof(double),
                              hipMemcpyDeviceToHost));
             for (int s = 0; s < n; ++s) {
                 bh += (long double)dt * tot[size_t(s) * 4 + 2];
@@ -574,7 +581,7 @@ mp_status mp_transport::step_noalloc(int nsteps, int scheme, double dt, mp_step_
     // The error word, the completion flags (a fixed region that never holds
     // anything but epochs), the totals, then the partials (each 256-byte aligned).
     char* p = static_cast<char*>(base);
-    const size_t tot = (size_t(nsteps > 0 ? nsteps : 1) * 4 * sizeof(double) + 255) & ~size_t(255);
+    const size_t tot = (size_t(nsteps + 1) * 4 * sizeof(double) + 255) & ~size_t(255);
     char* q = p + 256 + kScratchFlagWords * sizeof(uint32_t);
     const StepWork w{reinterpret_cast<double*>(q + tot), reinterpret_cast<double*>(q),
                      reinterpret_cast<uint32_t*>(p), reinterpret_cast<uint32_t*>(p + 256),
@@ -1078,6 +1085,14 @@ mp_status mp_transport_layout(mp_transport* t, int32_t* rows_per_item, int32_t* 
     if (items) *items = t->nitems;
     if (tail_split) *tail_split = t->layout.split;
     if (one_launch_steps) *one_launch_steps = t->step_fused(MP_SCHEME_SSPRK3) ? 1 : 0;
+    return MP_OK;
+}
+
+mp_status mp_transport_set_balance_tolerance(mp_transport* t, double tol)
+{
+    if (!t) return fail(MP_ERR_ARG, "null handle");
+    if (!std::isfinite(tol) || tol < 0.0) return fail(MP_ERR_ARG, "balance tolerance must be finite and >= 0");
+    t->balance_tol = tol;
     return MP_OK;
 }
 

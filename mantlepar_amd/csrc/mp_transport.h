@@ -41,6 +41,7 @@ struct mp_transport {
     mp::DeviceBuffer flags;
     uint32_t epoch = 0;
     int32_t launch_mode = MP_LAUNCH_AUTO;
+    double balance_tol = 1e-10;   // relative tolerance of a step's balance check
     mp::DeviceBuffer d_tab;
     mp::StagePtrs* h_tab = nullptr;
     hipEvent_t tab_copied = nullptr;

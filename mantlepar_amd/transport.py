@@ -196,6 +196,10 @@ class Transport:
         mode = LAUNCH_AUTO if per_step is None else (LAUNCH_PER_STEP if per_step else LAUNCH_PER_STAGE)
         check(lib().mp_transport_set_launch(self._h, mode))
 
+    def set_balance_tolerance(self, tol=1e-10):
+        """Relative tolerance of every step's conservation balance check."""
+        check(lib().mp_transport_set_balance_tolerance(self._h, tol))
+
     def layout(self):
         """rows per item, items per stage, first tail row, one launch per SSPRK3 step."""
         v = [C.c_int32() for _ in range(4)]

@@ -261,3 +261,31 @@ def test_lithostatic_phase_per_row(prec):
     T0, _, _, _, reg0 = om.batch(Hs.ravel(), Cs.ravel(), 0.0, hyper=True)
     assert np.array_equal(t.phase_state()["region"].ravel(), reg0)
     t.close()
+
+
+@pytest.mark.parametrize("prec", [M.PREC_FP64, M.PREC_MIXED], ids=["fp64", "mixed"])
+def test_step_rejects_a_failed_heat_balance(prec):
+    """Every step is checked against the preheat's conservation balance
+    (src/couple/preheat.cpp:268-273).  Forced from the host: with tolerance 0
+    only an exact balance passes, and on a through-flow the rounding residue is
+    not zero, so the step is rejected with the reference's message, the state
+    unchanged; at the reference's 1e-10 the same steps pass (as they do in
+    every other test here)."""
+    spec = W.MeshSpec(96, 128, seed=41)
+    vel = W.throughflow_velocity(spec)
+    bc, inflow = W.throughflow_boundary(spec)
+    t, _, _, _ = setup(spec, prec, vel=vel, bc=bc, inflow=inflow)
+    dt = t.stable_dt(0.4)
+    H0, C0 = t.download()
+    t.set_balance_tolerance(0.0)
+    with pytest.raises(M.MantleError) as e:
+        t.step(3, dt)
+    assert e.value.code == 10 and "heat balance failed; candidate rejected" in str(e.value)
+    H1, C1 = t.download()
+    assert np.array_equal(H0, H1) and np.array_equal(C0, C1)
+    t.set_balance_tolerance()
+    rep = t.step(3, dt)
+    assert rep["steps_done"] == 3
+    with pytest.raises(M.MantleError):
+        t.set_balance_tolerance(-1.0)
+    t.close()
