@@ -120,6 +120,7 @@ struct MeshDev {
     int32_t limit, lim_ncp, lim_ni, lim_nb;
     double lim_cupper;          // upper bound of C: Xe (1 with the mirror)
     double lim_cx[kMaxQ], lim_cy[kMaxQ];
+    double lim_cw[kMaxQ];       // cell Gauss weights on [-1, 1]
     double lim_xi[kMaxLimS], lim_xb[kMaxLimS];
     double lim_yi[kMaxLimS], lim_yb[kMaxLimS];
     // Thermal diffusion of H in the legacy preheat form (samples are the cell

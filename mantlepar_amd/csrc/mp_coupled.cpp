@@ -151,9 +151,6 @@ mp_status mp_coupled::init(const mp_mesh_desc& m, const mp_phase_params& p,
     d.yqd = d.yvs + q * nyf;
     d.yphi = d.yqd + q * nyf;
     d.cmix = cmix.as<double>();
-    double xc[kMaxQ], wc[kMaxQ];
-    gauss_legendre(cp.cell_points_per_axis, xc, wc);
-    for (int k = 0; k < cp.cell_points_per_axis; ++k) d.cw[k] = wc[k];
     d.cooling = cp.cooling;
     d.kappa = cp.thermal_diffusivity;
     d.global_speed = cp.global_speed;

@@ -18,7 +18,6 @@ struct CoupledDev {
     // porosity; vertical faces [q][ny][nx+1], horizontal faces [q][ny+1][nx]
     const double *xvs, *xqd, *xphi, *yvs, *yqd, *yphi;
     const double* cmix;          // [cell point b*n+a][N]: mixture velocity, y component
-    double cw[kMaxQ];            // cell Gauss weights
     double cooling;              // alpha0*l0*g/cp
     double kappa;                // thermal diffusivity
     double global_speed;

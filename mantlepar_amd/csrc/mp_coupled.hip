@@ -107,23 +107,25 @@ __global__ __launch_bounds__(128) void coupled_cell_kernel(CoupledDev d, const d
     const unsigned sides = (j == 0 ? 1u : 0u) | (i == nx - 1 ? 2u : 0u) | (j == ny - 1 ? 4u : 0u) |
                            (i == 0 ? 8u : 0u);
     CellPoly p;
-    for (int field = 0; field < 2; ++field) {
+    // Each field writes its own members: a destination chosen at run time
+    // would put p in scratch memory.
+    auto limited = [&](const double* U, double cap, double& mean, double (&out)[9]) {
         double w[3][3], B[3][3];
 #pragma unroll
         for (int dj = -1; dj <= 1; ++dj)
 #pragma unroll
             for (int di = -1; di <= 1; ++di)
-                w[dj + 1][di + 1] = (field ? C : H)[size_t(clampi(j + dj, 0, ny - 1)) * nx +
-                                                   clampi(i + di, 0, nx - 1)];
+                w[dj + 1][di + 1] = U[size_t(clampi(j + dj, 0, ny - 1)) * nx + clampi(i + di, 0, nx - 1)];
         blend<double>(m, w, mask, B, err);
-        const double th = limit_factor<0>(m, w, B, sides, field ? m.lim_cupper : INFINITY, err);
-        double* out = field ? p.BC : p.BH;
+        const double th = limit_factor<0>(m, w, B, sides, cap, err);
 #pragma unroll
         for (int b = 0; b < 3; ++b)
 #pragma unroll
             for (int a = 0; a < 3; ++a) out[b * 3 + a] = th * B[b][a];
-        (field ? p.tC : p.tH) = w[1][1];
-    }
+        mean = w[1][1];
+    };
+    limited(H, INFINITY, p.tH, p.BH);
+    limited(C, m.lim_cupper, p.tC, p.BC);
     d.blend[c] = p.tH;
     d.blend[N + c] = p.tC;
 #pragma unroll
@@ -141,7 +143,7 @@ __global__ __launch_bounds__(128) void coupled_cell_kernel(CoupledDev d, const d
             double Hp, Cp;
             const PhaseOut o = point_phase<false>(m.ph, d.tep_cell, (size_t(j) * n + b) * n + a, p,
                                                   m.lim_cx[a], m.lim_cy[b], Hp, Cp, err);
-            const double wab = d.cw[a] * d.cw[b] * 0.25;      // weight / area
+            const double wab = m.lim_cw[a] * m.lim_cw[b] * 0.25;      // weight / area
             const double mixy = d.cmix[size_t(b * n + a) * N + c];
             const double source = -d.cooling * o.T * mixy;
             heat += wab * source;

@@ -215,24 +215,27 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
             const unsigned mask = col_ok ? candidate_mask(i, r, nx, ny) : 2u;
             uint32_t e2 = 0;
             if constexpr (LIM) {
-                double BH[3][3], BC[3][3];
-                blend<double>(m, wH, mask, BH, e2);
-                blend<double>(m, wC, mask, BC, e2);
                 const unsigned sides = (r == 0 ? 1u : 0u) | (i == nx - 1 ? 2u : 0u) |
                                        (r == ny - 1 ? 4u : 0u) | (i == 0 ? 8u : 0u);
-                const double thH = limit_factor<NQ>(m, wH, BH, sides, INFINITY, e2);
-                const double thC = limit_factor<NQ>(m, wC, BC, sides, m.lim_cupper, e2);
+                // one field at a time: its blend, limit factor and face lines
+                // (both blends live at once spilled to scratch)
+                auto limited = [&](const double (&w)[3][3], double cap, FaceLines<R>& L) {
+                    double B[3][3];
+                    blend<double>(m, w, mask, B, e2);
+                    const double th = limit_factor<NQ>(m, w, B, sides, cap, e2);
 #pragma unroll
-                for (int b = 0; b < 3; ++b)
+                    for (int b = 0; b < 3; ++b)
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        BH[b][a] *= thH;
-                        BC[b][a] *= thC;
-                    }
-                face_lines<double>(m, BH, LH);
-                face_lines<double>(m, BC, LC);
+                        for (int a = 0; a < 3; ++a) B[b][a] *= th;
+                    face_lines<double>(m, B, L);
+                };
+                limited(wH, INFINITY, LH);
+                limited(wC, m.lim_cupper, LC);
             } else {
                 reconstruct<R>(m, wH, mask, LH, e2);
+                // H's reconstruction is done before C's starts: interleaved,
+                // the two spill past 256 registers with the diffusion terms
+                __builtin_amdgcn_sched_barrier(0);
                 reconstruct<R>(m, wC, mask, LC, e2);
             }
             if (col_ok) err |= e2;
@@ -505,32 +508,64 @@ __global__ __launch_bounds__(kStepSlices) void reduce_step_final_kernel(const do
 }
 
 // Compensated sums of a state (H, C over n cells) as a step's slices:
-// part[slice][6] with the boundary slots zero.  One wave per slice.
-__global__ __launch_bounds__(64) void state_sums_kernel(const double* __restrict__ H,
-                                                        const double* __restrict__ C, int64_t n,
-                                                        double* __restrict__ part)
+// part[slice][6] with the boundary slots zero.
+__global__ __launch_bounds__(256) void state_sums_kernel(const double* __restrict__ H,
+                                                         const double* __restrict__ C, int64_t n,
+                                                         double* __restrict__ part)
 {
-    const int slice = int(blockIdx.x), lane = int(threadIdx.x);
+    // Four waves per slice, four independent compensated sums per lane so that
+    // eight loads are in flight per lane; merged in a fixed order (the result
+    // does not depend on timing).
+    constexpr int kThreads = 256, kUnroll = 4;
+    const int slice = int(blockIdx.x), t = int(threadIdx.x);
     const int64_t chunk = (n + kStepSlices - 1) / kStepSlices;
     const int64_t lo = min(n, int64_t(slice) * chunk), hi = min(n, lo + chunk);
-    double v[6] = {0, 0, 0, 0, 0, 0};
-    for (int64_t k = lo + lane; k < hi; k += kWave) {
-        two_sum_acc(v[0], v[1], H[k]);
-        two_sum_acc(v[2], v[3], C[k]);
+    double v[kUnroll][4] = {};
+    int64_t k = lo + t;
+    for (; k + (kUnroll - 1) * kThreads < hi; k += kUnroll * kThreads) {
+        double h[kUnroll], c[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            h[u] = H[k + u * kThreads];
+            c[u] = C[k + u * kThreads];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            two_sum_acc(v[u][0], v[u][1], h[u]);
+            two_sum_acc(v[u][2], v[u][3], c[u]);
+        }
     }
+    for (; k < hi; k += kThreads) {
+        two_sum_acc(v[0][0], v[0][1], H[k]);
+        two_sum_acc(v[0][2], v[0][3], C[k]);
+    }
+    auto merge = [](double* a, const double* b) {
+        double s0 = a[0], e0 = a[1] + b[1];
+        two_sum_acc(s0, e0, b[0]);
+        double s1 = a[2], e1 = a[3] + b[3];
+        two_sum_acc(s1, e1, b[2]);
+        a[0] = s0; a[1] = e0; a[2] = s1; a[3] = e1;
+    };
+#pragma unroll
+    for (int u = 1; u < kUnroll; ++u) merge(v[0], v[u]);
     for (int off = 32; off >= 1; off >>= 1) {
         double o[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) o[k] = __shfl_xor(v[k], off);
-        double s0 = v[0], e0 = v[1] + o[1];
-        two_sum_acc(s0, e0, o[0]);
-        double s1 = v[2], e1 = v[3] + o[3];
-        two_sum_acc(s1, e1, o[2]);
-        v[0] = s0; v[1] = e0; v[2] = s1; v[3] = e1;
+        for (int q = 0; q < 4; ++q) o[q] = __shfl_xor(v[0][q], off);
+        merge(v[0], o);
     }
-    if (lane == 0)
+    __shared__ double wave_sums[kThreads / kWave][4];
+    if ((t & (kWave - 1)) == 0)
 #pragma unroll
-        for (int k = 0; k < 6; ++k) part[size_t(slice) * 6 + k] = v[k];
+        for (int q = 0; q < 4; ++q) wave_sums[t / kWave][q] = v[0][q];
+    __syncthreads();
+    if (t == 0) {
+        for (int w = 1; w < kThreads / kWave; ++w) merge(v[0], wave_sums[w]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[size_t(slice) * 6 + q] = v[0][q];
+        part[size_t(slice) * 6 + 4] = 0.0;
+        part[size_t(slice) * 6 + 5] = 0.0;
+    }
 }
 
 // ----------------------------------------------------------- batched sweep
@@ -660,7 +695,7 @@ __global__ void recon_diag_kernel(const MeshDev* __restrict__ mesh, const double
 }
 
 // Limiter factors of both fields per cell (FP64 blend), for diagnostics.
-__global__ void limit_diag_kernel(const MeshDev* __restrict__ mesh, const double* __restrict__ H,
+__global__ __launch_bounds__(128) void limit_diag_kernel(const MeshDev* __restrict__ mesh, const double* __restrict__ H,
                                   const double* __restrict__ C, double* thetaH, double* thetaC,
                                   uint32_t* error)
 {
@@ -672,6 +707,7 @@ __global__ void limit_diag_kernel(const MeshDev* __restrict__ mesh, const double
     const unsigned sides = (j == 0 ? 1u : 0u) | (i == m.nx - 1 ? 2u : 0u) |
                            (j == m.ny - 1 ? 4u : 0u) | (i == 0 ? 8u : 0u);
     uint32_t err = 0;
+#pragma unroll
     for (int field = 0; field < 2; ++field) {
         double w[3][3], B[3][3];
         load_window<0>(m, field ? C : H, i, j, w);
@@ -688,7 +724,7 @@ __global__ void limit_diag_kernel(const MeshDev* __restrict__ mesh, const double
 // src/mlweno/reconstruction.cpp:474-502) and theta the limiter factor when the
 // limiter is on (StateReconstruction::Evaluate, src/couple/state_reconstruction.cpp:121-134).
 template <int MODE>
-__global__ void point_eval_kernel(const MeshDev* __restrict__ mesh, const double* __restrict__ H,
+__global__ __launch_bounds__(256) void point_eval_kernel(const MeshDev* __restrict__ mesh, const double* __restrict__ H,
                                   const double* __restrict__ C, int64_t n,
                                   const int32_t* __restrict__ cell, const double* __restrict__ X,
                                   const double* __restrict__ Y, double* outH, double* outC,
@@ -705,6 +741,7 @@ __global__ void point_eval_kernel(const MeshDev* __restrict__ mesh, const double
                            (j == m.ny - 1 ? 4u : 0u) | (i == 0 ? 8u : 0u);
     const double x = X[k], y = Y[k];
     uint32_t err = 0;
+#pragma unroll
     for (int field = 0; field < 2; ++field) {
         double w[3][3];
         load_window<MODE>(m, field ? C : H, i, j, w);
@@ -715,6 +752,7 @@ __global__ void point_eval_kernel(const MeshDev* __restrict__ mesh, const double
             if (m.limit) th = limit_factor<0>(m, w, B, sides, field ? m.lim_cupper : INFINITY, err);
         }
         double p = 0.0;
+#pragma unroll
         for (int b = 2; b >= 0; --b)
             p = p * y + ((double(B[b][2]) * x + double(B[b][1])) * x + double(B[b][0]));
         (field ? outC : outH)[k] = w[1][1] + th * p;
@@ -725,7 +763,7 @@ __global__ void point_eval_kernel(const MeshDev* __restrict__ mesh, const double
 // L(u) per cell by reconstructing the cell and its four neighbours (5x the
 // work of the sweep; diagnostics only).  bnd[c*2+f] = outward boundary flux.
 template <int MODE>
-__global__ void rhs_diag_kernel(const MeshDev* __restrict__ mesh, const double* __restrict__ H,
+__global__ __launch_bounds__(128) void rhs_diag_kernel(const MeshDev* __restrict__ mesh, const double* __restrict__ H,
                                 const double* __restrict__ C, double* rhsH, double* rhsC,
                                 double* bnd, uint32_t* error)
 {
@@ -737,6 +775,7 @@ __global__ void rhs_diag_kernel(const MeshDev* __restrict__ mesh, const double* 
     const int i = c % nx, j = c / nx;
     const size_t vxs = size_t(ny) * (nx + 1), vys = size_t(ny + 1) * nx;
     uint32_t err = 0;
+#pragma unroll
     for (int field = 0; field < 2; ++field) {
         const double* u = field ? C : H;
         double w[3][3];
@@ -747,6 +786,7 @@ __global__ void rhs_diag_kernel(const MeshDev* __restrict__ mesh, const double* 
         double Fin = 0.0, Fout_b = 0.0;   // net inflow, outward boundary flux
         // neighbours: 0 bottom (i,j-1), 1 right (i+1,j), 2 top (i,j+1), 3 left (i-1,j)
         const int ni[4] = {i, i + 1, i, i - 1}, nj[4] = {j - 1, j, j + 1, j};
+#pragma unroll
         for (int f = 0; f < 4; ++f) {
             const bool vert = f & 1;
             const double* vn = vert ? m.vnx + size_t(j) * (nx + 1) + (f == 1 ? i + 1 : i)
@@ -968,15 +1008,15 @@ hipError_t launch_batch_phase_items(const MeshDev* meshes, const StagePtrs* stag
     return hipGetLastError();
 }
 
-// The FP64 sweep's instances: (LIM, DIFF) in {(false, false), (false, true),
-// (true, true)} -- the limiter, which the coupled set-up pairs with diffusion,
-// keeps the diffusion terms.
+// The FP64 sweep's instances: the limiter and the diffusion terms each
+// compiled in only when the mesh uses them.
 template <bool PHASE, int NQ>
 static void launch_fp64(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
                         bool limit, bool diffusion)
 {
     const dim3 grid(nitems), block(kWave);
-    if (limit) stage_sweep_kernel<0, PHASE, NQ, true, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    if (limit && diffusion) stage_sweep_kernel<0, PHASE, NQ, true, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    else if (limit) stage_sweep_kernel<0, PHASE, NQ, true, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
     else if (diffusion) stage_sweep_kernel<0, PHASE, NQ, false, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
     else stage_sweep_kernel<0, PHASE, NQ, false, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
 }
@@ -1090,7 +1130,7 @@ hipError_t launch_reduce_step(const double* partials, int nitems, int nstages, c
 hipError_t launch_state_sums(const double* H, const double* C, int64_t n, double area, double* out,
                              double* scratch, hipStream_t stream)
 {
-    state_sums_kernel<<<kStepSlices, kWave, 0, stream>>>(H, C, n, scratch);
+    state_sums_kernel<<<kStepSlices, 256, 0, stream>>>(H, C, n, scratch);
     reduce_step_final_kernel<<<1, kStepSlices, 0, stream>>>(scratch, area, out, nullptr, 0.0, 0.0, nullptr);
     return hipGetLastError();
 }

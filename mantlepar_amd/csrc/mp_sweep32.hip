@@ -358,6 +358,9 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
             const unsigned mask = col_ok ? candidate_mask(i, r, nx, ny) : 2u;
             uint32_t e2 = 0;
             recon32<true>(k, H0, H1, H2, mask, LH, e2);
+            // with full face records and the diffusion terms, the two
+            // reconstructions interleaved spill past the register budget
+            if constexpr (FULL && DIFF) __builtin_amdgcn_sched_barrier(0);
             recon32<true>(k, C0, C1, C2, mask, LC, e2);
             if (col_ok) err |= e2;
         } else {

@@ -313,6 +313,7 @@ mp_status set_limiter_tables(MeshDev& m, int cell_points, int samples, double ex
     for (int k = 0; k < cell_points; ++k) {
         m.lim_cx[k] = m.hx * xc[k];
         m.lim_cy[k] = m.hy * xc[k];
+        m.lim_cw[k] = wc[k];
     }
     // A face's in-cell samples sit at depth d_k inward from the face along its
     // normal.  The available normal-ray distance of a uniform rectangle is its
