@@ -84,6 +84,46 @@ void Arena::release(void* p)
     c.free[off] = sz;
 }
 
+// ------------------------------------------------------------- PinnedRing
+
+PinnedRing::~PinnedRing()
+{
+    for (Buf& b : bufs_) {
+        if (b.pending) (void)hipEventSynchronize(b.read);
+        if (b.read) (void)hipEventDestroy(b.read);
+        if (b.p) (void)hipHostFree(b.p);
+    }
+}
+
+mp_status PinnedRing::next(size_t bytes, char** out)
+{
+    Buf& b = bufs_[cur_];
+    if (b.pending) {
+        MP_HIP(hipEventSynchronize(b.read));
+        b.pending = false;
+    }
+    if (!b.read) MP_HIP(hipEventCreateWithFlags(&b.read, hipEventDisableTiming));
+    if (b.cap < bytes) {
+        if (b.p) MP_HIP(hipHostFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+        const size_t cap = std::max(bytes + bytes / 2, size_t(4096));
+        MP_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.p), cap, hipHostMallocDefault));
+        b.cap = cap;
+    }
+    *out = b.p;
+    return MP_OK;
+}
+
+mp_status PinnedRing::issued(hipStream_t stream)
+{
+    Buf& b = bufs_[cur_];
+    MP_HIP(hipEventRecord(b.read, stream));
+    b.pending = true;
+    cur_ = (cur_ + 1) % kBufs;
+    return MP_OK;
+}
+
 // ----------------------------------------------------------------- Engine
 
 Engine::~Engine()
@@ -97,7 +137,10 @@ Engine::~Engine()
     if (side_join) (void)hipEventDestroy(side_join);
     if (side) (void)hipStreamDestroy(side);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
-    if (h_stages) (void)hipHostFree(h_stages);
+    for (int k = 0; k < 2; ++k) {
+        if (h_errors[k]) (void)hipHostFree(h_errors[k]);
+        if (h_totals[k]) (void)hipHostFree(h_totals[k]);
+    }
 }
 
 mp_status Engine::init(int cap, int32_t prec, const mp_weno_options& w, const mp_phase_params& p,
@@ -134,9 +177,6 @@ mp_status Engine::init(int cap, int32_t prec, const mp_weno_options& w, const mp
     MP_CALL(d_errors.alloc(sizeof(uint32_t) * cap));
     MP_CALL(d_maxbits.alloc(sizeof(unsigned long long) * cap));
     MP_CALL(reserve_steps(1));
-    MP_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_stages), sizeof(StagePtrs) * 9 * cap,
-                         hipHostMallocDefault));
-    std::memset(h_stages, 0, sizeof(StagePtrs) * 9 * cap);
     h_nitems.assign(size_t(cap), 0);
     return MP_OK;
 }
@@ -240,11 +280,12 @@ void Engine::set_rows(int k, int rows)
     s.nitems = s.layout.nitems;
 }
 
-mp_status Engine::upload_async(int k, const double* H, const double* C)
+mp_status Engine::upload_async(int k, const double* H, const double* C, bool checked)
 {
     Slot& s = slots[k];
     if (!H || !C) return fail(MP_ERR_ARG, "null state array");
-    if (!all_finite(H, s.N) || !all_finite(C, s.N)) return fail(MP_ERR_ARG, "Cell averages must be finite");
+    if (!checked && (!all_finite(H, s.N) || !all_finite(C, s.N)))
+        return fail(MP_ERR_ARG, "Cell averages must be finite");
     MP_HIP(hipMemcpyAsync(s.H[s.cur], H, s.N * sizeof(double), hipMemcpyHostToDevice, stream));
     MP_HIP(hipMemcpyAsync(s.C[s.cur], C, s.N * sizeof(double), hipMemcpyHostToDevice, stream));
     s.phase_valid = false;
@@ -273,13 +314,14 @@ mp_status Engine::download(int k, double* H, double* C)
     return MP_OK;
 }
 
-mp_status Engine::set_velocity(int k, const double* vx, const double* vy, bool wait)
+mp_status Engine::set_velocity(int k, const double* vx, const double* vy, bool wait, bool checked)
 {
     Slot& s = slots[k];
     const mp_mesh_desc& d = s.desc;
     const size_t nxv = size_t(d.q) * faces_x(d), nyv = size_t(d.q) * faces_y(d);
     if (!vx || !vy) return fail(MP_ERR_ARG, "null velocity array");
-    if (!all_finite(vx, nxv) || !all_finite(vy, nyv)) return fail(MP_ERR_ARG, "Edge velocity must be finite");
+    if (!checked && (!all_finite(vx, nxv) || !all_finite(vy, nyv)))
+        return fail(MP_ERR_ARG, "Edge velocity must be finite");
     if (weno.lf_mode == 1) {
         double vmax = 0;
         for (size_t i = 0; i < nxv; ++i) vmax = std::fmax(vmax, std::fabs(vx[i]));
@@ -435,43 +477,51 @@ mp_status Engine::set_active(const std::vector<int>& list, const std::vector<int
     //    cache.  Runs of unequal length are padded with (-1, 0) entries that
     //    the kernels skip.  An item costs about its rows plus the two rows it
     //    reads before its first output row.
+    // The order is built once for all active slots (a stable bucket sort on
+    // the integer cost); a prefix's order is the subsequence of its slots,
+    // which is what sorting the prefix alone would give.
     std::vector<int> prefixes;
     for (size_t i = 0; i < active.size(); ++i)
         if (i + 1 == active.size() || active_steps[i + 1] != active_steps[i]) prefixes.push_back(int(i + 1));
+    struct Entry {
+        int2 item;
+        int rank, cost;
+    };
+    std::vector<Entry> all;
+    int max_cost = 0;
+    for (size_t i = 0; i < active.size(); ++i) {
+        const Slot& sl = slots[active[i]];
+        for (int it = 0; it < sl.nitems; ++it) {
+            const bool tail = it >= sl.layout.nlong;
+            const int band = tail ? (it - sl.layout.nlong) / sl.layout.strips : it / sl.layout.strips;
+            const int j0 = tail ? sl.layout.split + band * sl.layout.tail_rows : band * sl.layout.rows;
+            const int j1 = std::min(j0 + (tail ? sl.layout.tail_rows : sl.layout.rows),
+                                    tail ? sl.desc.ny : sl.layout.split);
+            const int cost = (j1 - j0) + 2;
+            all.push_back({make_int2(active[i], it), int(i), cost});
+            max_cost = std::max(max_cost, cost);
+        }
+    }
+    std::vector<Entry> sorted(all.size());
+    {
+        // longest items first: the items a launch dispatches last, whose
+        // durations set its drain, are the shortest
+        std::vector<size_t> start(size_t(max_cost) + 2, 0);
+        for (const Entry& e : all) ++start[size_t(max_cost - e.cost) + 1];
+        for (size_t c = 1; c < start.size(); ++c) start[c] += start[c - 1];
+        for (const Entry& e : all) sorted[start[size_t(max_cost - e.cost)]++] = e;
+    }
     std::vector<int2> items;
     lists.clear();
     for (int p : prefixes) {
         std::vector<int2> logical;
         std::vector<double> cost;
         double total = 0.0;
-        for (int i = 0; i < p; ++i) {
-            const Slot& sl = slots[active[i]];
-            for (int it = 0; it < sl.nitems; ++it) {
-                logical.push_back(make_int2(active[i], it));
-                const bool tail = it >= sl.layout.nlong;
-                const int band = tail ? (it - sl.layout.nlong) / sl.layout.strips : it / sl.layout.strips;
-                const int j0 = tail ? sl.layout.split + band * sl.layout.tail_rows : band * sl.layout.rows;
-                const int j1 = std::min(j0 + (tail ? sl.layout.tail_rows : sl.layout.rows),
-                                        tail ? sl.desc.ny : sl.layout.split);
-                cost.push_back(double(j1 - j0) + 2.0);
-                total += cost.back();
-            }
-        }
-        // Longest items first (a stable sort keeps each mesh's items in item
-        // order among equals): the items a launch dispatches last, whose
-        // durations set its drain, are the shortest.
-        {
-            std::vector<size_t> idx(logical.size());
-            for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
-            std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-            std::vector<int2> l2(logical.size());
-            std::vector<double> c2(cost.size());
-            for (size_t i = 0; i < idx.size(); ++i) {
-                l2[i] = logical[idx[i]];
-                c2[i] = cost[idx[i]];
-            }
-            logical.swap(l2);
-            cost.swap(c2);
+        for (const Entry& e : sorted) {
+            if (e.rank >= p) continue;
+            logical.push_back(e.item);
+            cost.push_back(double(e.cost));
+            total += double(e.cost);
         }
         List L;
         L.queue_off = int(items.size());
@@ -501,14 +551,23 @@ mp_status Engine::set_active(const std::vector<int>& list, const std::vector<int
     }
     nitems_total = int(items.size());
     ++generation;
+    // Uploaded from pinned memory without waiting: the copies run after the
+    // work already queued on the stream, which still reads the old lists.
+    // The item array grows geometrically (a reallocation synchronises).
     if (d_items.bytes < items.size() * sizeof(int2))
-        MP_CALL(d_items.alloc(items.size() * sizeof(int2) + 1024));
-    if (!items.empty())
-        MP_HIP(hipMemcpyAsync(d_items.ptr, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
-    MP_HIP(hipMemcpyAsync(d_nitems.ptr, h_nitems.data(), sizeof(int) * capacity, hipMemcpyHostToDevice, stream));
+        MP_CALL(d_items.alloc(std::max(items.size() * sizeof(int2) + 1024, 2 * d_items.bytes)));
+    const size_t b_items = items.size() * sizeof(int2), b_slots = sizeof(int) * size_t(capacity);
+    char* h = nullptr;
+    MP_CALL(ring_lists.next(b_items + 2 * b_slots, &h));
+    if (b_items) std::memcpy(h, items.data(), b_items);
+    std::memcpy(h + b_items, h_nitems.data(), b_slots);
+    if (!active.empty()) std::memcpy(h + b_items + b_slots, active.data(), sizeof(int) * active.size());
+    if (b_items) MP_HIP(hipMemcpyAsync(d_items.ptr, h, b_items, hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(d_nitems.ptr, h + b_items, b_slots, hipMemcpyHostToDevice, stream));
     if (!active.empty())
-        MP_HIP(hipMemcpyAsync(d_active.ptr, active.data(), sizeof(int) * active.size(), hipMemcpyHostToDevice, stream));
-    MP_HIP(hipStreamSynchronize(stream));
+        MP_HIP(hipMemcpyAsync(d_active.ptr, h + b_items + b_slots, sizeof(int) * active.size(),
+                              hipMemcpyHostToDevice, stream));
+    MP_CALL(ring_lists.issued(stream));
     return MP_OK;
 }
 
@@ -523,6 +582,10 @@ int Engine::steps_of(int k) const
 // the next step.
 mp_status Engine::write_tables(int scheme)
 {
+    const size_t bytes = sizeof(StagePtrs) * 9 * size_t(capacity);
+    char* h = nullptr;
+    MP_CALL(ring_tables.next(bytes, &h));
+    StagePtrs* h_stages = reinterpret_cast<StagePtrs*>(h);
     for (int k : active) {
         Slot& s = slots[k];
         int o[2], n = 0;
@@ -555,8 +618,8 @@ mp_status Engine::write_tables(int scheme)
                 h_stages[size_t(set * 3 + st) * capacity + k] = sp;
             }
     }
-    MP_HIP(hipMemcpyAsync(d_stages.ptr, h_stages, sizeof(StagePtrs) * 9 * capacity,
-                          hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(d_stages.ptr, h_stages, bytes, hipMemcpyHostToDevice, stream));
+    MP_CALL(ring_tables.issued(stream));
     return MP_OK;
 }
 
@@ -651,6 +714,31 @@ mp_status Engine::fetch_status(int nsteps, std::vector<uint32_t>& errors, std::v
                               hipMemcpyDeviceToHost, stream));
     }
     MP_HIP(hipStreamSynchronize(stream));
+    return MP_OK;
+}
+
+mp_status Engine::copy_status(int set, int nsteps)
+{
+    if (set < 0 || set > 1) return fail(MP_ERR_ARG, "status set must be 0 or 1");
+    if (nsteps > steps_capacity) return fail(MP_ERR_STATE, "per-step totals not reserved");
+    if (!h_errors[set]) MP_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_errors[set]),
+                                             sizeof(uint32_t) * capacity, hipHostMallocDefault));
+    if (status_steps < steps_capacity) {
+        // both sets are reallocated together; a caller reserves its steps
+        // before it keeps copies in flight
+        for (int k = 0; k < 2; ++k) {
+            if (h_totals[k]) MP_HIP(hipHostFree(h_totals[k]));
+            h_totals[k] = nullptr;
+            MP_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_totals[k]),
+                                 sizeof(double) * 4 * size_t(capacity) * steps_capacity, hipHostMallocDefault));
+        }
+        status_steps = steps_capacity;
+    }
+    MP_HIP(hipMemcpyAsync(h_errors[set], d_errors.ptr, sizeof(uint32_t) * capacity, hipMemcpyDeviceToHost,
+                          stream));
+    if (nsteps > 0)
+        MP_HIP(hipMemcpyAsync(h_totals[set], d_totals.ptr, sizeof(double) * 4 * size_t(capacity) * nsteps,
+                              hipMemcpyDeviceToHost, stream));
     return MP_OK;
 }
 

@@ -46,6 +46,33 @@ private:
     size_t budget_ = 0, chunk_ = size_t(1) << 30, in_use_ = 0;
 };
 
+// Pinned host buffers for uploads, used in turn.  The host writes a buffer
+// only after the device has run the copies that read it the last time, so an
+// upload never waits for the work queued before it.
+class PinnedRing {
+public:
+    PinnedRing() = default;
+    PinnedRing(const PinnedRing&) = delete;
+    PinnedRing& operator=(const PinnedRing&) = delete;
+    ~PinnedRing();
+    // The next buffer, at least `bytes` long, free for the host to write.
+    mp_status next(size_t bytes, char** out);
+    // The buffer that next() returned is read by the copies enqueued on
+    // `stream` so far.
+    mp_status issued(hipStream_t stream);
+
+private:
+    static constexpr int kBufs = 3;
+    struct Buf {
+        char* p = nullptr;
+        size_t cap = 0;
+        hipEvent_t read = nullptr;
+        bool pending = false;
+    };
+    Buf bufs_[kBufs];
+    int cur_ = 0;
+};
+
 struct Slot {
     bool live = false;
     mp_mesh_desc desc{};
@@ -80,10 +107,13 @@ public:
     mp_status add_mesh(const mp_mesh_desc& d, int* slot);
     void remove_mesh(int slot);
     mp_status upload(int slot, const double* H, const double* C);
-    mp_status upload_async(int slot, const double* H, const double* C);
+    // checked: the caller has already rejected non-finite values (the
+    // server's stagers do it while they copy the inputs).
+    mp_status upload_async(int slot, const double* H, const double* C, bool checked = false);
     mp_status download(int slot, double* H, double* C);
     mp_status download_async(int slot, double* H, double* C);
-    mp_status set_velocity(int slot, const double* vnx, const double* vny, bool wait = true);
+    mp_status set_velocity(int slot, const double* vnx, const double* vny, bool wait = true,
+                           bool checked = false);
     mp_status set_boundary(int slot, const uint8_t* const side[4]);
     mp_status set_inflow(int slot, const double* H, const double* C);
     mp_status set_lithostatic(int slot, int enable, double surface_y);
@@ -106,6 +136,13 @@ public:
     // After the enqueued work completes: commit each slot whose error word is 0.
     void commit(int nsteps, const std::vector<uint32_t>& errors);
     mp_status fetch_status(int nsteps, std::vector<uint32_t>& errors, std::vector<double>* totals);
+    // Without waiting: the error words and the per-step totals of the steps
+    // enqueued so far, copied on the stream into pinned set `set` (0 or 1).
+    // They are valid once the stream has passed this point, and stay valid
+    // until the next copy into the same set.
+    mp_status copy_status(int set, int nsteps);
+    const uint32_t* copied_errors(int set) const { return h_errors[set]; }
+    const double* copied_totals(int set) const { return h_totals[set]; }
     // Per-slot report from the per-step totals (same arithmetic as the
     // single-mesh handle).
     void make_report(int slot, int nsteps, const std::vector<double>& totals, uint32_t bits,
@@ -144,8 +181,13 @@ public:
     int q_common = 0;          // q shared by all active slots, or 0
     DeviceBuffer d_meshes, d_stages, d_items, d_nitems, d_active, d_errors, d_totals, d_maxbits;
     DeviceBuffer d_queues;     // per step and stage: work counter of the persistent grid
-    StagePtrs* h_stages = nullptr;   // pinned mirror of d_stages [9][capacity]
     std::vector<int> h_nitems;
+    // Uploads of the work lists and of the stage tables (which hold only the
+    // active slots' entries; the kernels read no others).
+    PinnedRing ring_lists, ring_tables;
+    uint32_t* h_errors[2] = {nullptr, nullptr};
+    double* h_totals[2] = {nullptr, nullptr};
+    int status_steps = 0;      // per-step totals that h_totals holds
 };
 
 }  // namespace mp

@@ -1,11 +1,14 @@
 // Serving front end: continuous batching of transport runs on one device.
 //
-// Callers submit requests from any thread.  One server thread owns the
-// engine: while an epoch runs it admits waiting requests (arrival order,
-// first fit) while slots and the HBM budget allow, uploading them on a second
-// stream; then it waits for the epoch, retires runs that reached their step
-// count, and starts the next epoch, which advances every resident run by up
-// to E steps (one launch per stage over all of them).  A run's results are
+// Callers submit requests from any thread.  Stager threads copy each
+// request's inputs, in arrival order, into pinned host blocks and reject
+// non-finite values on the way.  One server thread owns the engine and keeps
+// two epochs queued on the device: while they run it admits staged requests
+// (arrival order, first fit, while slots and the HBM budget allow; uploads on
+// a second stream), then waits for the older epoch, reads its status (copied
+// on the stream into pinned memory), retires runs that reached their step
+// count and queues the next epoch, which advances every resident run by up to
+// E steps (one launch per stage over all of them).  A run's results are
 // independent of its co-residents: each mesh has its own tables, dt,
 // partials and error word, and the stage kernel's per-mesh arithmetic does
 // not depend on the work list.
@@ -33,11 +36,78 @@ struct Request {
     mp_request r{};
     size_t bytes = 0;
     int slot = -1;
-    int remaining = 0;
+    int remaining = 0;        // steps not yet queued
+    int inflight = 0;         // queued epochs that advance the run
     long double bH = 0, bC = 0;
     mp_request_result res{};
     char message[256] = {0};
     bool done = false;
+    mp_status fail_st = MP_OK;   // a failure found while later epochs were queued
+    // Inputs staged in pinned memory: H, C (N each), then the vertical and
+    // horizontal face velocities.
+    enum { kWaiting, kStaging, kStaged } stage = kWaiting;
+    char* staged = nullptr;
+    size_t staged_bytes = 0;
+    mp_status stage_st = MP_OK;
+    const char* stage_msg = nullptr;
+};
+
+size_t staged_size(const mp_mesh_desc& d)
+{
+    const size_t N = size_t(d.nx) * d.ny;
+    const size_t fx = size_t(d.ny) * (d.nx + 1), fy = size_t(d.ny + 1) * d.nx;
+    return sizeof(double) * (2 * N + size_t(d.q) * (fx + fy));
+}
+
+// Copies n values and reports whether all are finite (one pass over each).
+bool copy_finite(double* dst, const double* src, size_t n)
+{
+    uint64_t bad = 0;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t b;
+        std::memcpy(&b, src + i, sizeof b);
+        bad |= uint64_t((b & 0x7ff0000000000000ull) == 0x7ff0000000000000ull);
+        std::memcpy(dst + i, &b, sizeof b);
+    }
+    return bad == 0;
+}
+
+// Pinned host blocks for staged inputs, reused by size class (powers of
+// two): pinned allocations are slow, copies from them are not.
+class PinnedPool {
+public:
+    ~PinnedPool()
+    {
+        for (auto& kv : blocks_) (void)hipHostFree(kv.first);
+    }
+    char* get(size_t bytes, size_t* cap)
+    {
+        size_t c = 4096;
+        while (c < bytes) c <<= 1;
+        *cap = c;
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = free_.find(c);
+        if (it != free_.end()) {
+            char* p = it->second;
+            free_.erase(it);
+            return p;
+        }
+        char* p = nullptr;
+        if (hipHostMalloc(reinterpret_cast<void**>(&p), c, hipHostMallocDefault) != hipSuccess) return nullptr;
+        blocks_[p] = c;
+        return p;
+    }
+    void put(char* p, size_t cap)
+    {
+        if (!p) return;
+        std::lock_guard<std::mutex> g(mu_);
+        free_.emplace(cap, p);
+    }
+
+private:
+    std::mutex mu_;
+    std::multimap<size_t, char*> free_;
+    std::map<char*, size_t> blocks_;
 };
 
 }  // namespace
@@ -52,8 +122,9 @@ struct mp_server {
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::deque<Request*> pending;
+    std::condition_variable cv_work, cv_done, cv_stage;
+    std::deque<Request*> pending;      // submitted, not admitted (arrival order)
+    std::deque<Request*> to_stage;     // submitted, not yet taken by a stager
     std::map<int64_t, Request*> all;   // submitted and not yet collected by mp_server_wait
     // Latencies (ms) of the most recent completed runs, a ring of kLatencies.
     static constexpr size_t kLatencies = size_t(1) << 16;
@@ -67,19 +138,28 @@ struct mp_server {
     bool stop = false;
     bool membership_changed = true;   // a slot was filled or freed since the last work list
     std::thread worker;
-    // Requests are admitted on `upload` while the previous epoch runs on the
+    // Stagers copy at most kStageAhead bytes ahead of admission.
+    static constexpr int kStagers = 4;
+    static constexpr size_t kStageAhead = size_t(512) << 20;
+    size_t staged_ahead = 0;
+    int staging = 0;                   // requests a stager is copying now
+    std::vector<std::thread> stagers;
+    PinnedPool pool;
+    // Requests are admitted on `upload` while the queued epochs run on the
     // engine's stream; the engine's stream waits for `uploaded` before its
-    // next work.
-    hipStream_t upload = nullptr;
+    // next work.  Results of finished runs come back on `download`.
+    hipStream_t upload = nullptr, download = nullptr;
     hipEvent_t uploaded = nullptr;
-    hipEvent_t epoch_done = nullptr;   // recorded after the epoch in flight
-    // The epoch in flight: its runs, step count and launch status.
+    // Queued epochs (at most two), oldest first; epoch i uses status set i % 2.
     struct Flight {
-        bool on = false;
-        int E = 0;
+        int E = 0, set = 0;
         mp_status st = MP_OK;
         std::vector<Request*> run;
-    } flight;
+        std::vector<int> steps;        // steps of each run in this epoch
+    };
+    std::deque<Flight> flights;
+    hipEvent_t flight_done[2] = {nullptr, nullptr};
+    int next_set = 0;
 
     ~mp_server()
     {
@@ -88,11 +168,16 @@ struct mp_server {
             stop = true;
         }
         cv_work.notify_all();
+        cv_stage.notify_all();
         if (worker.joinable()) worker.join();
+        for (std::thread& t : stagers)
+            if (t.joinable()) t.join();
         for (auto& kv : all) delete kv.second;
         if (uploaded) (void)hipEventDestroy(uploaded);
-        if (epoch_done) (void)hipEventDestroy(epoch_done);
+        for (hipEvent_t e : flight_done)
+            if (e) (void)hipEventDestroy(e);
         if (upload) (void)hipStreamDestroy(upload);
+        if (download) (void)hipStreamDestroy(download);
     }
 
     double now() const
@@ -101,11 +186,26 @@ struct mp_server {
     }
 
     void loop();
+    void stage_loop();
     void admit(std::vector<Request*>& batch, int rows_now);
-    void start_epoch();
-    void finish_epoch();
+    bool start_epoch();
+    void finish_oldest();
+    void retire();
     void finish(Request* q, mp_status st, const char* msg);
+    void unstage(Request* q);
 };
+
+// Returns the request's pinned block to the pool (server thread, or the
+// destructor after the threads have ended).
+void mp_server::unstage(Request* q)
+{
+    if (!q->staged) return;
+    pool.put(q->staged, q->staged_bytes);
+    q->staged = nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    staged_ahead -= q->staged_bytes;
+    cv_stage.notify_all();
+}
 
 void mp_server::finish(Request* q, mp_status st, const char* msg)
 {
@@ -114,6 +214,7 @@ void mp_server::finish(Request* q, mp_status st, const char* msg)
         q->slot = -1;
         membership_changed = true;
     }
+    unstage(q);
     std::lock_guard<std::mutex> g(mu);
     q->res.status = st;
     if (msg) std::snprintf(q->message, sizeof q->message, "%s", msg);
@@ -137,18 +238,76 @@ void mp_server::finish(Request* q, mp_status st, const char* msg)
     cv_done.notify_all();
 }
 
-// Bring admitted requests onto the device: slot, boundary data, state,
-// velocities, then one read of all their advective rates.  An upload takes
-// about 0.2 ms of host time, so while an epoch runs, admission stops once that
-// epoch has completed (after at least kMinAdmit requests) and hands the rest
-// back to the front of the queue: the GPU is never left idle behind a long
-// admission.
+// Stager thread: takes requests in arrival order and copies their inputs
+// into a pinned block, checking them as the engine's upload does (the
+// reference rejects non-finite averages and velocities).  It stays at most
+// kStageAhead bytes ahead of admission.
+void mp_server::stage_loop()
+{
+    for (;;) {
+        Request* q = nullptr;
+        size_t need = 0;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_stage.wait(lk, [&] { return stop || (!to_stage.empty() && staged_ahead < kStageAhead); });
+            if (stop) return;
+            q = to_stage.front();
+            to_stage.pop_front();
+            q->stage = Request::kStaging;
+            ++staging;
+            need = staged_size(q->r.mesh);
+            staged_ahead += need;
+        }
+        size_t cap = 0;
+        char* p = pool.get(need, &cap);
+        mp_status st = MP_OK;
+        const char* msg = nullptr;
+        if (!p) {
+            st = MP_ERR_MEMORY;
+            msg = "pinned staging allocation failed";
+        } else {
+            const mp_mesh_desc& d = q->r.mesh;
+            const size_t N = size_t(d.nx) * d.ny;
+            const size_t nxv = size_t(d.q) * d.ny * (d.nx + 1), nyv = size_t(d.q) * (d.ny + 1) * d.nx;
+            double* h = reinterpret_cast<double*>(p);
+            const bool okH = copy_finite(h, q->r.H, N);
+            const bool okC = copy_finite(h + N, q->r.C, N);
+            const bool okx = copy_finite(h + 2 * N, q->r.vnx, nxv);
+            const bool oky = copy_finite(h + 2 * N + nxv, q->r.vny, nyv);
+            if (!okH || !okC) {
+                st = MP_ERR_ARG;
+                msg = "Cell averages must be finite";
+            } else if (!okx || !oky) {
+                st = MP_ERR_ARG;
+                msg = "Edge velocity must be finite";
+            }
+        }
+        {
+            std::lock_guard<std::mutex> g(mu);
+            staged_ahead += (p ? cap : 0) - need;
+            q->staged = p;
+            q->staged_bytes = p ? cap : 0;
+            q->stage_st = st;
+            q->stage_msg = msg;
+            q->stage = Request::kStaged;
+            --staging;
+        }
+        cv_work.notify_all();
+    }
+}
+
+// Bring admitted requests onto the device from their staged copies: slot,
+// boundary data, state, velocities, then one read of all their advective
+// rates.  While epochs are queued, admission stops once the oldest of them
+// has completed (after at least kMinAdmit requests) and hands the rest back
+// to the front of the queue, so that the next epoch is queued in time.
 void mp_server::admit(std::vector<Request*>& batch, int rows_now)
 {
     constexpr size_t kMinAdmit = 4;
     std::vector<Request*> ok;
     for (size_t i = 0; i < batch.size(); ++i) {
-        if (i >= kMinAdmit && flight.on && hipEventQuery(epoch_done) == hipSuccess) {
+        if (i >= kMinAdmit && !flights.empty() &&
+            hipEventQuery(flight_done[flights.front().set]) == hipSuccess) {
             // back into the queue in arrival (id) order
             std::lock_guard<std::mutex> g(mu);
             std::deque<Request*> merged;
@@ -162,6 +321,13 @@ void mp_server::admit(std::vector<Request*>& batch, int rows_now)
             break;
         }
         Request* q = batch[i];
+        if (q->stage_st != MP_OK) {
+            finish(q, q->stage_st, q->stage_msg);
+            continue;
+        }
+        const mp_mesh_desc& d = q->r.mesh;
+        const size_t N = size_t(d.nx) * d.ny, nxv = size_t(d.q) * d.ny * (d.nx + 1);
+        const double* h = reinterpret_cast<const double*>(q->staged);
         int k = -1;
         mp_status st = eng.add_mesh(q->r.mesh, &k);
         if (st == MP_OK) {
@@ -170,8 +336,8 @@ void mp_server::admit(std::vector<Request*>& batch, int rows_now)
             for (int f = 0; f < 4; ++f) any_bc = any_bc || q->r.bc[f];
             if (any_bc) st = eng.set_boundary(k, q->r.bc);
         }
-        if (st == MP_OK) st = eng.upload_async(k, q->r.H, q->r.C);
-        if (st == MP_OK) st = eng.set_velocity(k, q->r.vnx, q->r.vny, false);
+        if (st == MP_OK) st = eng.upload_async(k, h, h + N, true);
+        if (st == MP_OK) st = eng.set_velocity(k, h + 2 * N, h + 2 * N + nxv, false, true);
         if (st == MP_OK && q->r.lithostatic) st = eng.set_lithostatic(k, 1, q->r.surface_y);
         if (st != MP_OK) {
             finish(q, st, last_error());
@@ -186,6 +352,7 @@ void mp_server::admit(std::vector<Request*>& batch, int rows_now)
     if (e == hipSuccess) e = hipStreamSynchronize(eng.stream);
     const double t = now();
     for (Request* q : ok) {
+        unstage(q);    // its copies have run
         if (e != hipSuccess) {
             finish(q, MP_ERR_DEVICE, hipGetErrorString(e));
             continue;
@@ -205,103 +372,124 @@ void mp_server::admit(std::vector<Request*>& batch, int rows_now)
     }
 }
 
-// Ragged epoch: every resident run takes min(remaining, E) steps, so a run
-// never waits for the shortest co-resident one.  The launches are enqueued
-// here and left running; finish_epoch collects them.
-void mp_server::start_epoch()
+// Queue a ragged epoch behind the ones in flight: every resident run with
+// steps left takes min(remaining, E) steps, so a run never waits for the
+// shortest co-resident one.  Each run's state is committed as if the epoch
+// succeeds, so the next epoch can be queued before this one ends; a run that
+// the epoch fails is retired with its error once nothing queued advances it
+// (its later steps are discarded).  Returns false if no run has steps left.
+bool mp_server::start_epoch()
 {
     Flight f;
-    f.on = true;
-    for (Request* q : resident) {
-        if (q->remaining > 0) f.run.push_back(q);
-    }
+    for (Request* q : resident)
+        if (q->remaining > 0 && q->fail_st == MP_OK) f.run.push_back(q);
+    if (f.run.empty()) return false;
     for (Request* q : f.run) f.E = std::max(f.E, std::min(q->remaining, max_epoch));
-    if (!f.run.empty()) {
-        std::vector<int> list, steps;
-        for (Request* q : f.run) {
-            list.push_back(q->slot);
-            steps.push_back(std::min(q->remaining, f.E));
-        }
-        mp_status st = MP_OK;
-        // A freed slot may be refilled by a mesh of another size under the
-        // same id, so the work list follows membership events, not ids.
-        if (membership_changed || list != last_list || steps != last_steps) {
-            st = eng.set_active(list, steps);
-            last_list = list;
-            last_steps = steps;
-        }
-        if (!st) membership_changed = false;
-        if (!st) st = eng.reserve_steps(f.E);
-        if (!st) st = eng.write_tables(scheme);
-        if (!st) st = eng.enqueue(f.E, scheme, nullptr);
-        if (!st && hipEventRecord(epoch_done, eng.stream) != hipSuccess)
-            st = fail(MP_ERR_DEVICE, "epoch event");
-        f.st = st;
-        ++epochs;
+    std::vector<int> list;
+    for (Request* q : f.run) {
+        list.push_back(q->slot);
+        f.steps.push_back(std::min(q->remaining, f.E));
     }
-    flight = std::move(f);
+    f.set = next_set;
+    next_set ^= 1;
+    mp_status st = MP_OK;
+    // A freed slot may be refilled by a mesh of another size under the same
+    // id, so the work list follows membership events, not ids.
+    if (membership_changed || list != last_list || f.steps != last_steps) {
+        st = eng.set_active(list, f.steps);
+        last_list = list;
+        last_steps = f.steps;
+        if (!st) membership_changed = false;
+    }
+    if (!st) st = eng.write_tables(scheme);
+    if (!st) st = eng.enqueue(f.E, scheme, nullptr);
+    if (!st) st = eng.copy_status(f.set, f.E);
+    if (!st && hipEventRecord(flight_done[f.set], eng.stream) != hipSuccess)
+        st = fail(MP_ERR_DEVICE, "epoch event");
+    if (!st) eng.commit(f.E, std::vector<uint32_t>(size_t(eng.capacity), 0u));
+    f.st = st;
+    for (size_t i = 0; i < f.run.size(); ++i) {
+        f.run[i]->remaining -= f.steps[i];
+        ++f.run[i]->inflight;
+    }
+    ++epochs;
+    flights.push_back(std::move(f));
+    return true;
 }
 
-// Wait for the epoch in flight, commit its runs, and retire those that are
-// done (runs admitted with zero steps retire here too, once their upload has
-// landed: the engine's stream waits for the upload stream).
-void mp_server::finish_epoch()
+// Wait for the oldest queued epoch and account its runs: steps, totals and
+// boundary sums, or the failure that its status reports.
+void mp_server::finish_oldest()
 {
-    if (!flight.on) return;
-    const std::vector<Request*> run = flight.run;
-    const int E = flight.E;
-    mp_status st = flight.st;
-    flight = Flight();
-    std::vector<uint32_t> errors(size_t(eng.capacity), 0u);
-    std::vector<double> totals;
-    if (!run.empty()) {
-        if (!st) st = eng.fetch_status(E, errors, &totals);
-        if (!st) eng.commit(E, errors);
-    }
-    std::vector<Request*> keep, retire;
-    for (Request* q : resident) {
-        const bool stepped = std::find(run.begin(), run.end(), q) != run.end();
-        if (stepped) {
-            if (st != MP_OK) {
-                finish(q, st, last_error());
-                continue;
-            }
-            const uint32_t b = errors[q->slot];
-            if (b) {
-                char msg[200];
-                std::snprintf(msg, sizeof msg, "%s (device error bits 0x%x) after %d steps",
-                              describe_bits(b), b, q->res.steps_done);
-                q->res.error_bits = b;
-                finish(q, status_from_bits(b), msg);
-                continue;
-            }
-            const Slot& s = eng.slots[q->slot];
-            const int n = eng.steps_of(q->slot);
-            for (int i = 0; i < n; ++i) {
-                const double* t = totals.data() + (size_t(i) * eng.capacity + q->slot) * 4;
-                q->bH += (long double)s.dt * t[2];
-                q->bC += (long double)s.dt * t[3];
-            }
-            const double* t = totals.data() + (size_t(n - 1) * eng.capacity + q->slot) * 4;
-            q->res.total_H = t[0];
-            q->res.total_C = t[1];
-            q->remaining -= n;
-            q->res.steps_done += n;
+    Flight f = std::move(flights.front());
+    flights.pop_front();
+    mp_status st = f.st;
+    if (!st && hipEventSynchronize(flight_done[f.set]) != hipSuccess) st = fail(MP_ERR_DEVICE, "epoch wait");
+    const uint32_t* errors = eng.copied_errors(f.set);
+    const double* totals = eng.copied_totals(f.set);
+    for (size_t i = 0; i < f.run.size(); ++i) {
+        Request* q = f.run[i];
+        --q->inflight;
+        if (q->fail_st != MP_OK) continue;      // failed in an earlier epoch
+        if (st != MP_OK) {
+            q->fail_st = st;
+            std::snprintf(q->message, sizeof q->message, "%s", last_error());
+            continue;
         }
-        (q->remaining == 0 ? retire : keep).push_back(q);
+        const uint32_t b = errors[q->slot];
+        if (b) {
+            q->res.error_bits = b;
+            q->fail_st = status_from_bits(b);
+            std::snprintf(q->message, sizeof q->message, "%s (device error bits 0x%x) after %d steps",
+                          describe_bits(b), b, q->res.steps_done);
+            continue;
+        }
+        const Slot& s = eng.slots[q->slot];
+        const int n = f.steps[i];
+        for (int k = 0; k < n; ++k) {
+            const double* t = totals + (size_t(k) * eng.capacity + q->slot) * 4;
+            q->bH += (long double)s.dt * t[2];
+            q->bC += (long double)s.dt * t[3];
+        }
+        const double* t = totals + (size_t(n - 1) * eng.capacity + q->slot) * 4;
+        q->res.total_H = t[0];
+        q->res.total_C = t[1];
+        q->res.steps_done += n;
     }
+}
+
+// Retire the runs that no queued epoch advances and that are done or failed.
+// Outputs of finished runs come back on the download stream (their epochs
+// have completed; the queued ones do not touch their slots), with one
+// synchronisation; a run admitted with zero steps computes its phase on the
+// engine's stream, behind the queued epochs, whose first launch clears the
+// error words that the phase evaluation reports through.
+void mp_server::retire()
+{
+    std::vector<Request*> keep, out;
+    for (Request* q : resident)
+        ((q->inflight == 0 && (q->remaining == 0 || q->fail_st != MP_OK)) ? out : keep).push_back(q);
+    if (out.empty()) return;
     resident.swap(keep);
-    // Outputs of finished runs: queue every copy, then one synchronisation.
     std::vector<Request*> good;
-    for (Request* q : retire) {
+    for (Request* q : out) {
+        if (q->fail_st != MP_OK) {
+            finish(q, q->fail_st, nullptr);
+            continue;
+        }
+        const bool need_phase = q->r.T || q->r.phi || q->r.Cl || q->r.Cs || q->r.region;
+        const hipStream_t main = eng.stream;
+        if (!need_phase || eng.slots[q->slot].phase_valid) eng.stream = download;
         mp_status s2 = eng.download_async(q->slot, q->r.H_out, q->r.C_out);
-        if (!s2 && (q->r.T || q->r.phi || q->r.Cl || q->r.Cs || q->r.region))
-            s2 = eng.phase_out(q->slot, q->r.T, q->r.phi, q->r.Cl, q->r.Cs, q->r.region, true);
+        if (!s2 && need_phase) s2 = eng.phase_out(q->slot, q->r.T, q->r.phi, q->r.Cl, q->r.Cs, q->r.region, true);
+        if (!s2 && eng.stream == main && hipStreamSynchronize(main) != hipSuccess)
+            s2 = fail(MP_ERR_DEVICE, "phase of an unstepped run");
+        eng.stream = main;
         if (s2) finish(q, s2, last_error());
         else good.push_back(q);
     }
     if (!good.empty()) {
-        const hipError_t e = hipStreamSynchronize(eng.stream);
+        const hipError_t e = hipStreamSynchronize(download);
         for (Request* q : good) {
             q->res.boundary_H = double(q->bH);
             q->res.boundary_C = double(q->bC);
@@ -310,10 +498,10 @@ void mp_server::finish_epoch()
     }
 }
 
-// Pipelined: while epoch k runs on the engine's stream, the worker admits the
-// requests that arrived (their uploads, face moments and rates on the upload
-// stream), then waits for epoch k, retires its finished runs and starts epoch
-// k + 1 with the new runs in it.
+// Pipelined: with epochs k and k + 1 queued, the worker admits the requests
+// that are staged (uploads, face moments and rates on the upload stream),
+// queues epochs until two are in flight, then waits for the oldest, retires
+// its finished runs, and goes round again.
 void mp_server::loop()
 {
     (void)hipSetDevice(device);
@@ -323,9 +511,17 @@ void mp_server::loop()
         {
             std::unique_lock<std::mutex> lk(mu);
             rows_now = rows;
-            cv_work.wait(lk, [&] { return stop || !pending.empty() || !resident.empty(); });
-            if (stop && resident.empty() && !flight.on) {
+            auto staged_head = [&] { return !pending.empty() && pending.front()->stage == Request::kStaged; };
+            cv_work.wait(lk, [&] { return stop || staged_head() || !resident.empty() || !flights.empty(); });
+            if (stop && resident.empty() && flights.empty()) {
+                // no stager may still be writing into a request failed here
+                cv_work.wait(lk, [&] { return staging == 0; });
                 for (Request* q : pending) {
+                    if (q->staged) {
+                        pool.put(q->staged, q->staged_bytes);
+                        staged_ahead -= q->staged_bytes;
+                        q->staged = nullptr;
+                    }
                     q->res.status = MP_ERR_STATE;
                     std::snprintf(q->message, sizeof q->message, "server shut down before admission");
                     q->res.t_done = now();
@@ -333,14 +529,17 @@ void mp_server::loop()
                     ++failed;
                 }
                 pending.clear();
+                to_stage.clear();
                 cv_done.notify_all();
                 return;
             }
-            // Admission: arrival order, first fit, while slots and budget allow.
+            // Admission: arrival order over the staged requests, first fit,
+            // while slots and budget allow.
             int free_slots = eng.capacity - int(resident.size());
             size_t planned = 0;
             for (auto it = pending.begin(); it != pending.end() && free_slots > 0 && !stop;) {
                 Request* q = *it;
+                if (q->stage != Request::kStaged) break;
                 if (eng.arena.in_use() + planned + q->bytes <= eng.arena.budget()) {
                     planned += q->bytes;
                     --free_slots;
@@ -353,7 +552,7 @@ void mp_server::loop()
         }
         if (!batch.empty()) {
             // the engine works on the upload stream for the admission, so its
-            // synchronisations wait for the uploads only, not for the epoch
+            // synchronisations wait for the uploads only, not for the epochs
             const hipStream_t main = eng.stream;
             eng.stream = upload;
             admit(batch, rows_now);
@@ -361,14 +560,16 @@ void mp_server::loop()
             if (hipEventRecord(uploaded, upload) == hipSuccess)
                 (void)hipStreamWaitEvent(main, uploaded, 0);
         }
-        finish_epoch();
+        while (flights.size() < 2 && start_epoch()) {
+        }
+        if (!flights.empty()) finish_oldest();
+        retire();
         {
             std::lock_guard<std::mutex> g(mu);
             hbm_in_use = eng.arena.in_use();
             hbm_peak = std::max(hbm_peak, uint64_t(hbm_in_use));
             max_resident = std::max(max_resident, int64_t(resident.size()));
         }
-        if (!resident.empty()) start_epoch();
     }
 }
 
@@ -398,17 +599,26 @@ mp_status mp_server_create(int32_t max_slots, const mp_weno_options* weno, const
         mp_status st = s->eng.init(max_slots, precision, w, p, nullptr, size_t(hbm_budget_bytes));
         if (st) { delete s; return st; }
         if (hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&s->download, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&s->uploaded, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&s->epoch_done, hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&s->flight_done[0], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s->flight_done[1], hipEventDisableTiming) != hipSuccess) {
             delete s;
-            return fail(MP_ERR_DEVICE, "server upload stream");
+            return fail(MP_ERR_DEVICE, "server streams");
         }
         s->scheme = scheme;
         s->max_epoch = max_epoch_steps;
+        // the per-step totals of an epoch, reserved before epochs overlap
+        st = s->eng.reserve_steps(max_epoch_steps);
+        if (!st) st = s->eng.copy_status(0, 0);
+        if (!st) st = s->eng.copy_status(1, 0);
+        if (!st && hipStreamSynchronize(s->eng.stream) != hipSuccess) st = fail(MP_ERR_DEVICE, "server set-up");
+        if (st) { delete s; return st; }
         // One wave per work item: with few resident runs this measured faster
         // than a persistent grid.
         s->eng.blocks = 0;
         s->worker = std::thread([s] { s->loop(); });
+        for (int i = 0; i < mp_server::kStagers; ++i) s->stagers.emplace_back([s] { s->stage_loop(); });
         *out = s;
         return MP_OK;
     })
@@ -442,10 +652,11 @@ mp_status mp_server_submit(mp_server* s, const mp_request* req, int64_t* id)
             if (s->first_submit < 0) s->first_submit = q->res.t_submit;
             s->all[q->id] = q;
             s->pending.push_back(q);
+            s->to_stage.push_back(q);
             ++s->submitted;
             *id = q->id;
         }
-        s->cv_work.notify_one();
+        s->cv_stage.notify_one();
         return MP_OK;
     })
 }
