@@ -308,11 +308,20 @@ def _serve_once(M, W, meshes, prec, rate, n, seed=17):
                               s.x0, s.x1, s.y0, s.y1, q=s.q, outputs=("H", "C")))
     srv.drain()
     st = srv.stats()
+    t, w = srv.progress()
     for rid in ids:
         srv.wait(rid)
     srv.close()
+    # steady throughput: the slope of the device-work curve between 20% and
+    # 80% of the run's work, away from the arrival ramp and the final drain
+    steady = None
+    if len(t) > 4 and w[-1] > 0:
+        a, b = int(np.searchsorted(w, 0.2 * w[-1])), int(np.searchsorted(w, 0.8 * w[-1]))
+        if b > a and t[b] > t[a]:
+            steady = float((w[b] - w[a]) / (t[b] - t[a]))
     return {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
             "max_ms": st["max_ms"], "cell_steps_per_s": st["cell_steps"] / st["elapsed_s"],
+            "cell_steps_per_s_steady": steady,
             "completed": st["completed"], "failed": st["failed"],
             "max_resident": st["max_resident"], "epochs": st["epochs"],
             "hbm_peak_bytes": st["hbm_peak"]}
@@ -346,9 +355,12 @@ def bench_serve(args, meshes, batch=None):
             r = _serve_once(M, W, meshes, prec, rate, max(n, int(0.5 * rate)), seed=17 + int(rate))
             if batch and name in batch:
                 r["of_batch_throughput"] = r["cell_steps_per_s"] / batch[name]["value"]
+                if r["cell_steps_per_s_steady"]:
+                    r["of_batch_throughput_steady"] = r["cell_steps_per_s_steady"] / batch[name]["value"]
             row[name] = r
             log(f"serve {name} at {rate:.0f}/s: p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms, "
-                f"{r['cell_steps_per_s']:.3e} cell-steps/s, max resident {r['max_resident']}")
+                f"{r['cell_steps_per_s']:.3e} cell-steps/s (steady {r['cell_steps_per_s_steady'] or 0:.3e}), "
+                f"max resident {r['max_resident']}")
         sweep.append(row)
     out["rate_sweep"] = sweep
     return out

@@ -396,6 +396,12 @@ mp_status mp_server_set_tile(mp_server* s, int32_t rows_per_item);
 mp_status mp_server_destroy(mp_server* s);
 /* Seconds on the server clock (the clock of t_submit / t_done). */
 double mp_server_now(mp_server* s);
+/* Progress of the device work, one sample per completed epoch (the most
+ * recent 65536): t[i] on the server clock, and the cell-steps that all epochs
+ * completed by then advanced (cumulative, successful runs).  Up to cap
+ * samples are written, oldest first; *n receives the count written.  Steady
+ * throughput is the slope of this curve away from the ramp and the drain. */
+mp_status mp_server_progress(mp_server* s, double* t, double* cell_steps, int32_t cap, int32_t* n);
 
 /* ------------------------------------------------------------- formats --- */
 /* The H/C handoff between a preheat and an evolution: cellH1.dat and

@@ -210,6 +210,7 @@ SIGNATURES = [
     ("mp_server_set_tile", C.c_int32, [_vp, C.c_int32]),
     ("mp_server_destroy", C.c_int32, [_vp]),
     ("mp_server_now", C.c_double, [_vp]),
+    ("mp_server_progress", C.c_int32, [_vp, _dp, _dp, C.c_int32, C.POINTER(C.c_int32)]),
     ("mp_handoff_write", C.c_int32, [C.c_char_p, C.POINTER(HandoffMeta), _dp, _dp]),
     ("mp_handoff_read", C.c_int32, [C.c_char_p, C.POINTER(HandoffMeta), C.c_double, _dp, _dp,
                                     C.POINTER(HandoffMeta)]),

@@ -224,3 +224,10 @@ class Server:
         s = ServerStats()
         check(lib().mp_server_stats_get(self._h, C.byref(s)))
         return s.as_dict()
+
+    def progress(self, cap=65536):
+        """(t, cumulative cell-steps) per completed epoch, oldest first."""
+        t, w = np.empty(cap), np.empty(cap)
+        n = C.c_int32()
+        check(lib().mp_server_progress(self._h, dptr(t), dptr(w), cap, C.byref(n)))
+        return t[:n.value].copy(), w[:n.value].copy()

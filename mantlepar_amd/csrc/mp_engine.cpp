@@ -249,11 +249,12 @@ mp_status Engine::add_mesh(const mp_mesh_desc& d, int* out)
     slots[k] = sl;
     set_rows(k, 64);
     mp_status s = MP_OK;
-    // zero state until the first upload (as a fresh single-mesh handle)
+    // zero state until the first upload (as a fresh single-mesh handle); the
+    // record is copied from pageable memory, which the copy reads before it
+    // returns
     hipError_t e = hipMemsetAsync(sl.H[0], 0, 10 * N * sizeof(double), stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_meshes.as<MeshDev>() + k, &slots[k].md, sizeof(MeshDev),
                                   hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) s = fail(MP_ERR_DEVICE, "mesh upload: %s", hipGetErrorString(e));
     if (s) {
         remove_mesh(k);

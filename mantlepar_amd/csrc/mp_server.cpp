@@ -130,6 +130,10 @@ struct mp_server {
     static constexpr size_t kLatencies = size_t(1) << 16;
     std::vector<double> latencies;
     size_t latency_next = 0;
+    // (completion time, cumulative cell-steps) of the most recent epochs, a ring
+    std::vector<std::pair<double, double>> progress;
+    size_t progress_next = 0;
+    double work_done = 0;
     std::vector<Request*> resident;
     int64_t next_id = 1, submitted = 0, completed = 0, failed = 0;
     int64_t epochs = 0, max_resident = 0;
@@ -160,6 +164,20 @@ struct mp_server {
     std::deque<Flight> flights;
     hipEvent_t flight_done[2] = {nullptr, nullptr};
     int next_set = 0;
+    // Finished runs whose outputs are on their way into pinned blocks: the
+    // finisher thread waits for the copies, moves the outputs into the
+    // caller's arrays and reports the runs.
+    struct Retired {
+        hipEvent_t copied = nullptr;
+        std::vector<Request*> runs;
+        std::vector<char*> blocks;
+        std::vector<size_t> caps;
+    };
+    std::deque<Retired> to_finish;
+    std::vector<hipEvent_t> spare_events;
+    std::condition_variable cv_finish;
+    bool finisher_stop = false;
+    std::thread finisher;
 
     ~mp_server()
     {
@@ -170,8 +188,15 @@ struct mp_server {
         cv_work.notify_all();
         cv_stage.notify_all();
         if (worker.joinable()) worker.join();
+        {
+            std::lock_guard<std::mutex> g(mu);
+            finisher_stop = true;
+        }
+        cv_finish.notify_all();
+        if (finisher.joinable()) finisher.join();
         for (std::thread& t : stagers)
             if (t.joinable()) t.join();
+        for (hipEvent_t e : spare_events) (void)hipEventDestroy(e);
         for (auto& kv : all) delete kv.second;
         if (uploaded) (void)hipEventDestroy(uploaded);
         for (hipEvent_t e : flight_done)
@@ -187,11 +212,14 @@ struct mp_server {
 
     void loop();
     void stage_loop();
+    void finish_loop();
     void admit(std::vector<Request*>& batch, int rows_now);
     bool start_epoch();
     void finish_oldest();
     void retire();
-    void finish(Request* q, mp_status st, const char* msg);
+    void finish(Request* q, mp_status st, const char* msg) { release(q); report(q, st, msg); }
+    void release(Request* q);
+    void report(Request* q, mp_status st, const char* msg);
     void unstage(Request* q);
 };
 
@@ -207,13 +235,19 @@ void mp_server::unstage(Request* q)
     cv_stage.notify_all();
 }
 
-void mp_server::finish(Request* q, mp_status st, const char* msg)
+// Frees the run's slot (server thread).
+void mp_server::release(Request* q)
 {
     if (q->slot >= 0) {
         eng.remove_mesh(q->slot);
         q->slot = -1;
         membership_changed = true;
     }
+}
+
+// Publishes the run's result (server or finisher thread).
+void mp_server::report(Request* q, mp_status st, const char* msg)
+{
     unstage(q);
     std::lock_guard<std::mutex> g(mu);
     q->res.status = st;
@@ -233,7 +267,6 @@ void mp_server::finish(Request* q, mp_status st, const char* msg)
         else latencies[latency_next] = ms;
         latency_next = (latency_next + 1) % kLatencies;
     }
-    hbm_in_use = eng.arena.in_use();
     q->r.H = q->r.C = q->r.vnx = q->r.vny = nullptr;
     cv_done.notify_all();
 }
@@ -427,6 +460,7 @@ void mp_server::finish_oldest()
     if (!st && hipEventSynchronize(flight_done[f.set]) != hipSuccess) st = fail(MP_ERR_DEVICE, "epoch wait");
     const uint32_t* errors = eng.copied_errors(f.set);
     const double* totals = eng.copied_totals(f.set);
+    double work = 0;
     for (size_t i = 0; i < f.run.size(); ++i) {
         Request* q = f.run[i];
         --q->inflight;
@@ -455,15 +489,24 @@ void mp_server::finish_oldest()
         q->res.total_H = t[0];
         q->res.total_C = t[1];
         q->res.steps_done += n;
+        work += double(q->r.mesh.nx) * q->r.mesh.ny * n;
     }
+    std::lock_guard<std::mutex> g(mu);
+    work_done += work;
+    const std::pair<double, double> sample(now(), work_done);
+    if (progress.size() < kLatencies) progress.push_back(sample);
+    else progress[progress_next] = sample;
+    progress_next = (progress_next + 1) % kLatencies;
 }
 
 // Retire the runs that no queued epoch advances and that are done or failed.
-// Outputs of finished runs come back on the download stream (their epochs
-// have completed; the queued ones do not touch their slots), with one
-// synchronisation; a run admitted with zero steps computes its phase on the
-// engine's stream, behind the queued epochs, whose first launch clears the
-// error words that the phase evaluation reports through.
+// Outputs of finished runs are copied into pinned blocks on the download
+// stream (their epochs have completed; the queued ones do not touch their
+// slots); the upload stream waits for those copies before it reuses the
+// freed slots, and the finisher thread hands the outputs over.  A run
+// admitted with zero steps computes its phase on the engine's stream behind
+// the queued epochs, whose first launch clears the error words that the
+// phase evaluation reports through, and waits for it there.
 void mp_server::retire()
 {
     std::vector<Request*> keep, out;
@@ -471,30 +514,103 @@ void mp_server::retire()
         ((q->inflight == 0 && (q->remaining == 0 || q->fail_st != MP_OK)) ? out : keep).push_back(q);
     if (out.empty()) return;
     resident.swap(keep);
-    std::vector<Request*> good;
+    Retired r;
     for (Request* q : out) {
         if (q->fail_st != MP_OK) {
             finish(q, q->fail_st, nullptr);
             continue;
         }
+        const Slot& s = eng.slots[q->slot];
+        const size_t N = s.N;
         const bool need_phase = q->r.T || q->r.phi || q->r.Cl || q->r.Cs || q->r.region;
+        size_t cap = 0;
+        char* blk = pool.get(need_phase ? 6 * N * sizeof(double) + N : 2 * N * sizeof(double), &cap);
+        if (!blk) {
+            finish(q, MP_ERR_MEMORY, "pinned output allocation failed");
+            continue;
+        }
+        double* h = reinterpret_cast<double*>(blk);
         const hipStream_t main = eng.stream;
-        if (!need_phase || eng.slots[q->slot].phase_valid) eng.stream = download;
-        mp_status s2 = eng.download_async(q->slot, q->r.H_out, q->r.C_out);
-        if (!s2 && need_phase) s2 = eng.phase_out(q->slot, q->r.T, q->r.phi, q->r.Cl, q->r.Cs, q->r.region, true);
+        if (!need_phase || s.phase_valid) eng.stream = download;
+        mp_status s2 = eng.download_async(q->slot, q->r.H_out ? h : nullptr, q->r.C_out ? h + N : nullptr);
+        if (!s2 && need_phase)
+            s2 = eng.phase_out(q->slot, q->r.T ? h + 2 * N : nullptr, q->r.phi ? h + 3 * N : nullptr,
+                               q->r.Cl ? h + 4 * N : nullptr, q->r.Cs ? h + 5 * N : nullptr,
+                               q->r.region ? reinterpret_cast<uint8_t*>(h + 6 * N) : nullptr, true);
         if (!s2 && eng.stream == main && hipStreamSynchronize(main) != hipSuccess)
             s2 = fail(MP_ERR_DEVICE, "phase of an unstepped run");
         eng.stream = main;
-        if (s2) finish(q, s2, last_error());
-        else good.push_back(q);
-    }
-    if (!good.empty()) {
-        const hipError_t e = hipStreamSynchronize(download);
-        for (Request* q : good) {
-            q->res.boundary_H = double(q->bH);
-            q->res.boundary_C = double(q->bC);
-            finish(q, e == hipSuccess ? MP_OK : MP_ERR_DEVICE, e == hipSuccess ? nullptr : hipGetErrorString(e));
+        if (s2) {
+            (void)hipStreamSynchronize(download);   // copies may already read the block
+            pool.put(blk, cap);
+            finish(q, s2, last_error());
+            continue;
         }
+        q->res.boundary_H = double(q->bH);
+        q->res.boundary_C = double(q->bC);
+        r.runs.push_back(q);
+        r.blocks.push_back(blk);
+        r.caps.push_back(cap);
+    }
+    if (r.runs.empty()) return;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        if (!spare_events.empty()) {
+            r.copied = spare_events.back();
+            spare_events.pop_back();
+        }
+    }
+    hipError_t e = r.copied ? hipSuccess : hipEventCreateWithFlags(&r.copied, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(r.copied, download);
+    if (e == hipSuccess) e = hipStreamWaitEvent(upload, r.copied, 0);
+    if (e != hipSuccess) {
+        // report the runs here, after a full wait for the copies
+        const hipError_t e2 = hipStreamSynchronize(download);
+        for (size_t i = 0; i < r.runs.size(); ++i) {
+            pool.put(r.blocks[i], r.caps[i]);
+            finish(r.runs[i], MP_ERR_DEVICE, hipGetErrorString(e2 != hipSuccess ? e2 : e));
+        }
+        if (r.copied) (void)hipEventDestroy(r.copied);
+        return;
+    }
+    for (Request* q : r.runs) release(q);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        to_finish.push_back(std::move(r));
+    }
+    cv_finish.notify_one();
+}
+
+// Finisher thread: waits for a retired group's copies, moves the outputs into
+// the callers' arrays and reports the runs.
+void mp_server::finish_loop()
+{
+    (void)hipSetDevice(device);
+    for (;;) {
+        Retired r;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_finish.wait(lk, [&] { return finisher_stop || !to_finish.empty(); });
+            if (to_finish.empty()) return;
+            r = std::move(to_finish.front());
+            to_finish.pop_front();
+        }
+        const hipError_t e = hipEventSynchronize(r.copied);
+        for (size_t i = 0; i < r.runs.size(); ++i) {
+            Request* q = r.runs[i];
+            if (e == hipSuccess) {
+                const size_t N = size_t(q->r.mesh.nx) * q->r.mesh.ny;
+                const double* h = reinterpret_cast<const double*>(r.blocks[i]);
+                double* outs[6] = {q->r.H_out, q->r.C_out, q->r.T, q->r.phi, q->r.Cl, q->r.Cs};
+                for (int k = 0; k < 6; ++k)
+                    if (outs[k]) std::memcpy(outs[k], h + size_t(k) * N, N * sizeof(double));
+                if (q->r.region) std::memcpy(q->r.region, h + 6 * N, N);
+            }
+            pool.put(r.blocks[i], r.caps[i]);
+            report(q, e == hipSuccess ? MP_OK : MP_ERR_DEVICE, e == hipSuccess ? nullptr : hipGetErrorString(e));
+        }
+        std::lock_guard<std::mutex> g(mu);
+        spare_events.push_back(r.copied);
     }
 }
 
@@ -619,6 +735,7 @@ mp_status mp_server_create(int32_t max_slots, const mp_weno_options* weno, const
         s->eng.blocks = 0;
         s->worker = std::thread([s] { s->loop(); });
         for (int i = 0; i < mp_server::kStagers; ++i) s->stagers.emplace_back([s] { s->stage_loop(); });
+        s->finisher = std::thread([s] { s->finish_loop(); });
         *out = s;
         return MP_OK;
     })
@@ -724,6 +841,23 @@ mp_status mp_server_stats_get(mp_server* s, mp_server_stats* o)
     o->hbm_budget = s->eng.arena.budget();
     o->epochs = s->epochs;
     o->max_resident = s->max_resident;
+    return MP_OK;
+}
+
+mp_status mp_server_progress(mp_server* s, double* t, double* cell_steps, int32_t cap, int32_t* n)
+{
+    if (!s || !n || cap < 0 || (cap > 0 && (!t || !cell_steps))) return fail(MP_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    const size_t count = s->progress.size();
+    const size_t first = count < mp_server::kLatencies ? 0 : s->progress_next;
+    const size_t m = std::min(count, size_t(cap));
+    // the most recent m samples, oldest first
+    for (size_t i = 0; i < m; ++i) {
+        const auto& p = s->progress[(first + count - m + i) % count];
+        t[i] = p.first;
+        cell_steps[i] = p.second;
+    }
+    *n = int32_t(m);
     return MP_OK;
 }
 

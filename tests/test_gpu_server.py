@@ -203,3 +203,40 @@ def test_wait_timeout_reports_pending():
     res, _ = srv.wait(rid)
     assert res["steps_done"] == 400
     srv.close()
+
+
+def test_nonfinite_inputs_rejected_alone():
+    """The stagers check inputs as the single-mesh upload does: a NaN average
+    or velocity fails its own request with the reference's message."""
+    reqs = make_requests(5, seed=71)
+    bad_h, bad_v = dict(reqs[0]), dict(reqs[1])
+    bad_h["H"] = reqs[0]["H"].copy()
+    bad_h["H"][1, 2] = np.nan
+    bad_v["vy"] = np.array(reqs[1]["vy"], dtype=float, copy=True)
+    bad_v["vy"].flat[7] = np.inf
+    srv = M.Server(max_slots=4, phase=W.config_phase(), precision=M.PREC_MIXED)
+    ids = [submit(srv, bad_h), submit(srv, bad_v)] + [submit(srv, r) for r in reqs[2:]]
+    srv.drain()
+    for rid, msg in ((ids[0], "Cell averages must be finite"), (ids[1], "Edge velocity must be finite")):
+        with pytest.raises(M.MantleError) as ei:
+            srv.wait(rid)
+        assert ei.value.code == 1 and msg in str(ei.value)
+    for rid in ids[2:]:
+        res, _ = srv.wait(rid)
+        assert res["status"] == 0
+    srv.close()
+
+
+def test_progress_curve_accounts_every_step():
+    reqs = make_requests(16, seed=72, max_steps=30)
+    srv = M.Server(max_slots=6, phase=W.config_phase(), precision=M.PREC_MIXED, max_epoch_steps=4)
+    ids = [submit(srv, r) for r in reqs]
+    srv.drain()
+    st = srv.stats()
+    t, w = srv.progress()
+    for rid in ids:
+        srv.wait(rid)
+    srv.close()
+    assert len(t) == st["epochs"]
+    assert np.all(np.diff(t) >= 0) and np.all(np.diff(w) > 0)
+    assert w[-1] == st["cell_steps"] == sum(r["spec"].cells * r["nsteps"] for r in reqs)
