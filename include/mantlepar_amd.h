@@ -312,10 +312,24 @@ mp_status mp_batch_phase(mp_batch* b, int32_t id, double* T, double* phi, double
  * four waves (0: one wave per work item, the default with 12 rows). */
 mp_status mp_batch_set_schedule(mp_batch* b, int32_t rows_per_item, int32_t blocks);
 mp_status mp_batch_get_schedule(mp_batch* b, int32_t* rows_per_item, int32_t* blocks);
+/* Mixed precision with one wave per work item: MP_LAUNCH_PER_STEP runs each
+ * step's stages in one launch over the work list, each item waiting on the
+ * items of the previous stage whose rows it reads (bounded waits; a stall
+ * fails the step with MP_ERR_DEVICE); MP_LAUNCH_PER_STAGE launches per stage;
+ * MP_LAUNCH_AUTO (the default) takes the autotuner's measured choice and,
+ * before it has run, one launch per step when the work list gives each wave
+ * slot of the chip fewer than eight items (the single-mesh rule; the server
+ * applies it to every epoch step).  Results are bitwise the same in every
+ * mode.
+ * mp_batch_get_launch reports whether steps run in one launch now.  No
+ * reference counterpart (launch scheduling). */
+mp_status mp_batch_set_launch(mp_batch* b, int32_t mode);
+mp_status mp_batch_get_launch(mp_batch* b, int32_t* per_step);
 /* Time candidate schedules on the resident meshes (state is left unchanged),
  * apply the fastest and return it.  cache_path (may be NULL) is a JSON file
  * keyed by device, precision, scheme and batch shape: a hit skips the search,
- * a miss appends the result. */
+ * a miss appends the result.  A mixed batch also times per-stage against
+ * per-step launches (mp_batch_set_launch). */
 mp_status mp_batch_autotune(mp_batch* b, int32_t nsteps, int32_t scheme, const char* cache_path,
                             int32_t* rows_per_item, int32_t* blocks, double* ms_per_step);
 /* Time nsteps (state unchanged): ms_total, mean stage launch, mean

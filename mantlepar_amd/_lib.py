@@ -195,6 +195,8 @@ SIGNATURES = [
     ("mp_batch_phase", C.c_int32, [_vp, C.c_int32, _dp, _dp, _dp, _dp, _u8p]),
     ("mp_batch_set_schedule", C.c_int32, [_vp, C.c_int32, C.c_int32]),
     ("mp_batch_get_schedule", C.c_int32, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("mp_batch_set_launch", C.c_int32, [_vp, C.c_int32]),
+    ("mp_batch_get_launch", C.c_int32, [_vp, C.POINTER(C.c_int32)]),
     ("mp_batch_autotune", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_char_p,
                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32), _dp]),
     ("mp_batch_profile", C.c_int32, [_vp, C.c_int32, C.c_int32, _dp, _dp, _dp]),

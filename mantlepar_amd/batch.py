@@ -9,7 +9,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (BATCH_EAGER, BATCH_GRAPH, MP_ERR_PENDING, PREC_FP64, SCHEME_SSPRK3,
+from ._lib import (BATCH_EAGER, BATCH_GRAPH, LAUNCH_AUTO, LAUNCH_PER_STAGE, LAUNCH_PER_STEP,
+                   MP_ERR_PENDING, PREC_FP64, SCHEME_SSPRK3,
                    MantleError, MeshDesc, Request, RequestResult, ServerStats, StepReport,
                    check, dptr, lib, u8ptr)
 from .transport import default_phase, default_weno
@@ -129,6 +130,18 @@ class Batch:
         r, b = C.c_int32(), C.c_int32()
         check(lib().mp_batch_get_schedule(self._h, C.byref(r), C.byref(b)))
         return r.value, b.value
+
+    def set_launch(self, per_step=True):
+        """Mixed batches: one launch per step (True), one per stage (False), or
+        the autotuner's choice (None, the default); results are bitwise equal."""
+        mode = LAUNCH_AUTO if per_step is None else (LAUNCH_PER_STEP if per_step else LAUNCH_PER_STAGE)
+        check(lib().mp_batch_set_launch(self._h, mode))
+
+    def per_step_launch(self):
+        """Whether steps currently run in one launch."""
+        v = C.c_int32()
+        check(lib().mp_batch_get_launch(self._h, C.byref(v)))
+        return bool(v.value)
 
     def autotune(self, nsteps=2, scheme=SCHEME_SSPRK3, cache_path=None):
         r, b, ms = C.c_int32(), C.c_int32(), C.c_double()

@@ -23,6 +23,11 @@ struct mp_batch {
     hipGraphExec_t graph = nullptr;
     uint64_t graph_gen = 0;
     int graph_nsteps = 0, graph_scheme = 0, graph_blocks = -1;
+    bool graph_step = false;
+    // MP_LAUNCH_PER_STAGE, _PER_STEP or _AUTO (the autotuner's choice, per-step
+    // launches until it has measured)
+    int launch_mode = MP_LAUNCH_AUTO;
+    bool tuned = false, tuned_step = true;
 
     ~mp_batch() { drop_graph(); }
     void drop_graph()
@@ -47,6 +52,11 @@ struct mp_batch {
     mp_status run(int nsteps, int scheme, bool use_graph, std::vector<hipEvent_t>* events);
     mp_status step(int nsteps, int scheme, int flags, mp_step_report* reports);
     mp_status profile(int nsteps, int scheme, double* ms_total, double* ms_stage, double* ms_phase);
+    mp_status time_steps(int nsteps, int scheme, double* ms);
+    bool want_step_launch() const
+    {
+        return launch_mode == MP_LAUNCH_PER_STEP || (launch_mode == MP_LAUNCH_AUTO && tuned_step);
+    }
     mp_status autotune(int nsteps, int scheme, const char* cache, int* rows_out, int* blocks_out,
                        double* ms_out);
     std::string shape_key(int scheme) const;
@@ -74,15 +84,20 @@ mp_status mp_batch::prepare(int nsteps, int scheme)
         MP_CALL(eng.set_active(list));
         dirty = false;
     }
+    eng.step_launch = want_step_launch();
+    // untuned AUTO follows the item-count rule; a measured or forced choice holds
+    eng.step_launch_items = (launch_mode == MP_LAUNCH_AUTO && !tuned) ? Engine::auto_step_items() : 0;
     MP_CALL(eng.reserve_steps(nsteps));
     return eng.write_tables(scheme);
 }
 
 mp_status mp_batch::run(int nsteps, int scheme, bool use_graph, std::vector<hipEvent_t>* events)
 {
+    const bool one = eng.uses_step_launch(events != nullptr);
+    if (one) MP_CALL(eng.reserve_epochs(nsteps));
     if (!use_graph) return eng.enqueue(nsteps, scheme, events);
     if (!graph || graph_gen != eng.generation || graph_nsteps != nsteps || graph_scheme != scheme ||
-        graph_blocks != eng.blocks) {
+        graph_blocks != eng.blocks || graph_step != one) {
         drop_graph();
         hipGraph_t g = nullptr;
         MP_HIP(hipStreamBeginCapture(eng.stream, hipStreamCaptureModeThreadLocal));
@@ -103,6 +118,7 @@ mp_status mp_batch::run(int nsteps, int scheme, bool use_graph, std::vector<hipE
         graph_nsteps = nsteps;
         graph_scheme = scheme;
         graph_blocks = eng.blocks;
+        graph_step = one;
     }
     MP_HIP(hipGraphLaunch(graph, eng.stream));
     return MP_OK;
@@ -176,6 +192,28 @@ mp_status mp_batch::profile(int nsteps, int scheme, double* ms_total, double* ms
     return MP_OK;
 }
 
+// Eager wall time of nsteps steps on the device (events around the enqueue);
+// no commit, like profile.
+mp_status mp_batch::time_steps(int nsteps, int scheme, double* ms)
+{
+    MP_CALL(prepare(nsteps, scheme));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    mp_status st = MP_OK;
+    if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess)
+        st = fail(MP_ERR_DEVICE, "event create failed");
+    if (!st && hipEventRecord(t0, eng.stream) != hipSuccess) st = fail(MP_ERR_DEVICE, "event record failed");
+    if (!st) st = run(nsteps, scheme, false, nullptr);
+    if (!st && hipEventRecord(t1, eng.stream) != hipSuccess) st = fail(MP_ERR_DEVICE, "event record failed");
+    if (!st && hipStreamSynchronize(eng.stream) != hipSuccess) st = fail(MP_ERR_DEVICE, "synchronize failed");
+    float t = 0;
+    if (!st && hipEventElapsedTime(&t, t0, t1) != hipSuccess) st = fail(MP_ERR_DEVICE, "event timing failed");
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    MP_CALL(st);
+    *ms = t;
+    return MP_OK;
+}
+
 std::string mp_batch::shape_key(int scheme) const
 {
     int dev = 0;
@@ -196,8 +234,10 @@ std::string mp_batch::shape_key(int scheme) const
 }
 
 // Cache file: one JSON object whose keys are shape keys, values
-// {"rows": r, "blocks": b, "ms_per_step": t}.  Parsed by key lookup only.
-static bool cache_lookup(const char* path, const std::string& key, int& rows, int& blocks, double& ms)
+// {"rows": r, "blocks": b, "step": s, "ms_per_step": t} (s: one launch per
+// step, 0 or 1; absent: 0).  Parsed by key lookup only.
+static bool cache_lookup(const char* path, const std::string& key, int& rows, int& blocks, int& step,
+                         double& ms)
 {
     std::ifstream f(path);
     if (!f) return false;
@@ -214,11 +254,13 @@ static bool cache_lookup(const char* path, const std::string& key, int& rows, in
     if (r == std::string::npos || b == std::string::npos) return false;
     rows = std::atoi(body.c_str() + r + 7);
     blocks = std::atoi(body.c_str() + b + 9);
+    const size_t sp = body.find("\"step\":");
+    step = sp == std::string::npos ? 0 : std::atoi(body.c_str() + sp + 7);
     ms = m == std::string::npos ? 0.0 : std::atof(body.c_str() + m + 14);
     return rows >= 8 && blocks >= 0;
 }
 
-static void cache_store(const char* path, const std::string& key, int rows, int blocks, double ms)
+static void cache_store(const char* path, const std::string& key, int rows, int blocks, int step, double ms)
 {
     std::string text;
     {
@@ -230,8 +272,9 @@ static void cache_store(const char* path, const std::string& key, int rows, int 
         }
     }
     char entry[512];
-    std::snprintf(entry, sizeof entry, "  \"%s\": {\"rows\": %d, \"blocks\": %d, \"ms_per_step\": %.6f}",
-                  key.c_str(), rows, blocks, ms);
+    std::snprintf(entry, sizeof entry,
+                  "  \"%s\": {\"rows\": %d, \"blocks\": %d, \"step\": %d, \"ms_per_step\": %.6f}",
+                  key.c_str(), rows, blocks, step, ms);
     const size_t end = text.rfind('}');
     if (text.empty() || end == std::string::npos) {
         text = std::string("{\n") + entry + "\n}\n";
@@ -254,11 +297,13 @@ mp_status mp_batch::autotune(int nsteps, int scheme, const char* cache, int* row
     if (nsteps < 1) return fail(MP_ERR_ARG, "autotune needs at least one step");
     if (live().empty()) return fail(MP_ERR_STATE, "the batch holds no mesh");
     const std::string key = shape_key(scheme);
-    int br = rows, bb = eng.blocks;
+    int br = rows, bb = eng.blocks, bs = 0;
     double best = -1;
-    if (cache && cache_lookup(cache, key, br, bb, best)) {
+    if (cache && cache_lookup(cache, key, br, bb, bs, best)) {
         rows = br;
         eng.blocks = bb;
+        tuned_step = bs != 0;
+        tuned = true;
         dirty = true;
     } else {
         int dev = 0, cus = 256;
@@ -268,6 +313,8 @@ mp_status mp_batch::autotune(int nsteps, int scheme, const char* cache, int* row
         // 1..3 resident blocks per CU taking items from a queue
         const int row_set[] = {8, 12, 16, 24, 32, 64};
         const int grid_mult[] = {0, 1, 2, 3};
+        tuned = true;                       // measured choices from here on
+        tuned_step = false;                 // the per-stage launches are timed one by one
         MP_CALL(prepare(nsteps, scheme));   // warm the code objects and tables
         MP_CALL(eng.enqueue(1, scheme, nullptr));
         MP_HIP(hipStreamSynchronize(eng.stream));
@@ -291,7 +338,26 @@ mp_status mp_batch::autotune(int nsteps, int scheme, const char* cache, int* row
         rows = br;
         eng.blocks = bb;
         dirty = true;
-        if (cache) cache_store(cache, key, br, bb, best);
+        // mixed, one wave per item: per-stage against per-step launches, by
+        // the wall time of whole steps
+        if (eng.precision == MP_PREC_MIXED && bb == 0) {
+            double t[2] = {-1, -1};
+            for (int rep = 0; rep < 3; ++rep)
+                for (int one = 0; one < 2; ++one) {
+                    tuned_step = one != 0;
+                    const int save = launch_mode;
+                    launch_mode = MP_LAUNCH_AUTO;
+                    double ms = 0;
+                    mp_status st = time_steps(nsteps, scheme, &ms);
+                    launch_mode = save;
+                    MP_CALL(st);
+                    if (t[one] < 0 || ms < t[one]) t[one] = ms;
+                }
+            bs = t[1] < t[0] ? 1 : 0;
+            best = t[bs] / nsteps;
+        }
+        tuned_step = bs != 0;
+        if (cache) cache_store(cache, key, br, bb, bs, best);
     }
     drop_graph();
     if (rows_out) *rows_out = br;
@@ -454,6 +520,27 @@ mp_status mp_batch_get_schedule(mp_batch* b, int32_t* rows_per_item, int32_t* bl
     if (!b) return fail(MP_ERR_ARG, "null handle");
     if (rows_per_item) *rows_per_item = b->rows;
     if (blocks) *blocks = b->eng.blocks;
+    return MP_OK;
+}
+
+mp_status mp_batch_set_launch(mp_batch* b, int32_t mode)
+{
+    if (!b) return fail(MP_ERR_ARG, "null handle");
+    if (mode != MP_LAUNCH_PER_STAGE && mode != MP_LAUNCH_PER_STEP && mode != MP_LAUNCH_AUTO)
+        return fail(MP_ERR_ARG, "launch mode must be MP_LAUNCH_PER_STAGE, _PER_STEP or _AUTO");
+    b->launch_mode = mode;
+    return MP_OK;
+}
+
+mp_status mp_batch_get_launch(mp_batch* b, int32_t* per_step)
+{
+    if (!b || !per_step) return fail(MP_ERR_ARG, "null argument");
+    Engine& e = b->eng;
+    long long items = 0;
+    for (int k : b->live()) items += e.slots[k].nitems;
+    const bool rule = b->launch_mode == MP_LAUNCH_AUTO && !b->tuned;
+    *per_step = (b->want_step_launch() && e.precision == MP_PREC_MIXED && e.blocks == 0 &&
+                 (!rule || items < Engine::auto_step_items())) ? 1 : 0;
     return MP_OK;
 }
 

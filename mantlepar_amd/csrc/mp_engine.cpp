@@ -176,6 +176,9 @@ mp_status Engine::init(int cap, int32_t prec, const mp_weno_options& w, const mp
     MP_CALL(d_active.alloc(sizeof(int) * cap));
     MP_CALL(d_errors.alloc(sizeof(uint32_t) * cap));
     MP_CALL(d_maxbits.alloc(sizeof(unsigned long long) * cap));
+    MP_CALL(d_flag_off.alloc(sizeof(int) * cap));
+    MP_CALL(d_epoch.alloc(256));
+    MP_HIP(hipMemsetAsync(d_epoch.ptr, 0, 256, stream));
     MP_CALL(reserve_steps(1));
     h_nitems.assign(size_t(cap), 0);
     return MP_OK;
@@ -512,6 +515,13 @@ mp_status Engine::set_active(const std::vector<int>& list, const std::vector<int
         for (size_t c = 1; c < start.size(); ++c) start[c] += start[c - 1];
         for (const Entry& e : all) sorted[start[size_t(max_cost - e.cost)]++] = e;
     }
+    // completion flags of the one-launch step: per slot, a run of its items
+    std::vector<int> flag_off(size_t(capacity), 0);
+    nflags = 0;
+    for (int k : active) {
+        flag_off[size_t(k)] = nflags;
+        nflags += slots[k].nitems;
+    }
     std::vector<int2> items;
     lists.clear();
     for (int p : prefixes) {
@@ -557,18 +567,51 @@ mp_status Engine::set_active(const std::vector<int>& list, const std::vector<int
     // The item array grows geometrically (a reallocation synchronises).
     if (d_items.bytes < items.size() * sizeof(int2))
         MP_CALL(d_items.alloc(std::max(items.size() * sizeof(int2) + 1024, 2 * d_items.bytes)));
+    MP_CALL(ensure_flags());
     const size_t b_items = items.size() * sizeof(int2), b_slots = sizeof(int) * size_t(capacity);
     char* h = nullptr;
-    MP_CALL(ring_lists.next(b_items + 2 * b_slots, &h));
+    MP_CALL(ring_lists.next(b_items + 3 * b_slots, &h));
     if (b_items) std::memcpy(h, items.data(), b_items);
     std::memcpy(h + b_items, h_nitems.data(), b_slots);
     if (!active.empty()) std::memcpy(h + b_items + b_slots, active.data(), sizeof(int) * active.size());
+    std::memcpy(h + b_items + 2 * b_slots, flag_off.data(), b_slots);
     if (b_items) MP_HIP(hipMemcpyAsync(d_items.ptr, h, b_items, hipMemcpyHostToDevice, stream));
     MP_HIP(hipMemcpyAsync(d_nitems.ptr, h + b_items, b_slots, hipMemcpyHostToDevice, stream));
     if (!active.empty())
         MP_HIP(hipMemcpyAsync(d_active.ptr, h + b_items + b_slots, sizeof(int) * active.size(),
                               hipMemcpyHostToDevice, stream));
+    MP_HIP(hipMemcpyAsync(d_flag_off.ptr, h + b_items + 2 * b_slots, b_slots, hipMemcpyHostToDevice, stream));
     MP_CALL(ring_lists.issued(stream));
+    return MP_OK;
+}
+
+mp_status Engine::ensure_flags()
+{
+    const size_t need = sizeof(uint32_t) * 2 * size_t(std::max(nflags, 1));
+    if (d_flags.bytes >= need) return MP_OK;
+    MP_CALL(d_flags.alloc(std::max(need, 2 * d_flags.bytes)));
+    MP_HIP(hipMemsetAsync(d_flags.ptr, 0, d_flags.bytes, stream));
+    ++generation;
+    return MP_OK;
+}
+
+int Engine::auto_step_items()
+{
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    return 8 * cus * 4 * 3;
+}
+
+mp_status Engine::reserve_epochs(int nsteps)
+{
+    // flags hold counter values 1 .. 2^32 - 1; restart well before a wrap
+    if (epochs_used + uint64_t(std::max(nsteps, 0)) >= (uint64_t(1) << 31)) {
+        MP_HIP(hipMemsetAsync(d_flags.ptr, 0, d_flags.bytes, stream));
+        MP_HIP(hipMemsetAsync(d_epoch.ptr, 0, sizeof(uint32_t), stream));
+        epochs_used = 0;
+    }
+    epochs_used += uint64_t(std::max(nsteps, 0));
     return MP_OK;
 }
 
@@ -676,15 +719,22 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
                                                  : std::make_pair(Lst.slab_off, Lst.slab_count);
         const int par = set == 1 ? 1 : 0;    // the partial set of this step
         if (overlap && s >= 2) MP_HIP(hipStreamWaitEvent(stream, side_done[par], 0));
+        const bool one = one_launch_for(L.second, events != nullptr);
+        if (one)
+            MP_HIP(launch_batch_step32(d_meshes.as<MeshDev>(), base + size_t(set * 3) * capacity, capacity,
+                                       d_items.as<int2>() + L.first, L.second, scheme, d_flags.as<uint32_t>(),
+                                       d_flag_off.as<int>(), nflags, d_epoch.as<uint32_t>(), kSpinLimit,
+                                       weno.lf_mode != 0, stream));
         for (int st = 0; st < scheme; ++st) {
             // the mixed sweep leaves the phase to its own streaming kernel, as
             // the single-mesh handle does
             const bool last = st == scheme - 1, split = last && precision == MP_PREC_MIXED;
             const StagePtrs* sps = base + size_t(set * 3 + st) * capacity;
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
-            MP_HIP(launch_batch_stage(precision, last && !split, q_common, d_meshes.as<MeshDev>(), sps,
-                                      d_items.as<int2>() + L.first, L.second, blocks,
-                                      d_queues.as<int>() + 3 * s + st, stream, weno.lf_mode != 0));
+            if (!one)
+                MP_HIP(launch_batch_stage(precision, last && !split, q_common, d_meshes.as<MeshDev>(), sps,
+                                          d_items.as<int2>() + L.first, L.second, blocks,
+                                          d_queues.as<int>() + 3 * s + st, stream, weno.lf_mode != 0));
             if (split && overlap) {
                 MP_HIP(hipEventRecord(stage3[par], stream));
                 MP_HIP(hipStreamWaitEvent(side, stage3[par], 0));

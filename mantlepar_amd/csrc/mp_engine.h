@@ -141,6 +141,9 @@ public:
     // They are valid once the stream has passed this point, and stay valid
     // until the next copy into the same set.
     mp_status copy_status(int set, int nsteps);
+    // A [scheme-1][nflags] flag array for the current lists (zeroed when it
+    // grows; a reallocation synchronises).
+    mp_status ensure_flags();
     const uint32_t* copied_errors(int set) const { return h_errors[set]; }
     const double* copied_totals(int set) const { return h_totals[set]; }
     // Per-slot report from the per-step totals (same arithmetic as the
@@ -176,6 +179,31 @@ public:
     std::map<int, List> lists;   // prefix length -> lists
     int capacity = 0;
     int blocks = 0;            // persistent grid in blocks of 4 waves (0: one wave per item)
+    // Mixed precision with one wave per item: each step's stages run in one
+    // launch (batch_step32_kernel) that orders them with per-item completion
+    // flags, instead of one launch per stage.  Profiling keeps the per-stage
+    // launches, which it times one by one.
+    bool step_launch = false;
+    // > 0: only steps whose work list is shorter take the one launch (the
+    // single-mesh rule: fewer than eight items per wave slot of the chip;
+    // with more, the waits cost what the shared ramp saves)
+    int step_launch_items = 0;
+    bool uses_step_launch(bool profiling) const
+    {
+        return step_launch && precision == MP_PREC_MIXED && blocks == 0 && !profiling;
+    }
+    bool one_launch_for(int list_len, bool profiling) const
+    {
+        return uses_step_launch(profiling) && (step_launch_items <= 0 || list_len < step_launch_items);
+    }
+    // eight items per wave slot (three waves per SIMD, four SIMDs per CU)
+    static int auto_step_items();
+    // Before enqueueing or replaying nsteps steps: restarts the epoch counter
+    // (and clears the flags) when it would wrap.  Not capturable.
+    mp_status reserve_epochs(int nsteps);
+    DeviceBuffer d_flags, d_flag_off, d_epoch;
+    int nflags = 0;            // flags per stage of the current work lists
+    uint64_t epochs_used = 0;  // counter advances since it was last cleared
     int nitems_total = 0;
     int steps_capacity = 0;    // per-step totals reserved
     int q_common = 0;          // q shared by all active slots, or 0

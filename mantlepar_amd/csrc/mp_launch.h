@@ -70,6 +70,16 @@ hipError_t launch_step32(const MeshDev* d_mesh, const StagePtrs* tab, int nitems
                          hipStream_t stream, bool diffusion, bool global_lf);
 hipError_t launch_batch_stage32(const MeshDev* meshes, const StagePtrs* stages, const int2* items,
                                 int nitems, int blocks, int* queue, bool global_lf, hipStream_t stream);
+// All stages of one mixed-precision step over a batch work list (one wave per
+// entry, (-1, 0) pads skipped) in one launch.  Stage st of slot k reads
+// tab[st * capacity + k]; completion flags are flags[st * nflags +
+// flag_off[k] + item].  The launch first advances *epoch_ctr (a one-thread
+// kernel) and its items publish and wait for the new value, so a captured
+// graph uses a new epoch on every replay; flags must hold earlier values.
+hipError_t launch_batch_step32(const MeshDev* meshes, const StagePtrs* tab, int capacity,
+                               const int2* items, int nitems, int nstages, uint32_t* flags,
+                               const int* flag_off, int nflags, uint32_t* epoch_ctr, int spin_limit,
+                               bool global_lf, hipStream_t stream);
 // Phase and state sums of a stage's output over the stage's work items
 // (sp.Ho/sp.Co in, sp.T..region and partial slots 0-3 out).
 hipError_t launch_phase_items(const MeshDev* d_mesh, const StagePtrs& sp, int nitems,

@@ -435,6 +435,7 @@ bool mp_server::start_epoch()
         if (!st) membership_changed = false;
     }
     if (!st) st = eng.write_tables(scheme);
+    if (!st) st = eng.reserve_epochs(f.E);
     if (!st) st = eng.enqueue(f.E, scheme, nullptr);
     if (!st) st = eng.copy_status(f.set, f.E);
     if (!st && hipEventRecord(flight_done[f.set], eng.stream) != hipSuccess)
@@ -731,8 +732,10 @@ mp_status mp_server_create(int32_t max_slots, const mp_weno_options* weno, const
         if (!st && hipStreamSynchronize(s->eng.stream) != hipSuccess) st = fail(MP_ERR_DEVICE, "server set-up");
         if (st) { delete s; return st; }
         // One wave per work item: with few resident runs this measured faster
-        // than a persistent grid.
+        // than a persistent grid.  Mixed steps run their stages in one launch.
         s->eng.blocks = 0;
+        s->eng.step_launch = true;
+        s->eng.step_launch_items = Engine::auto_step_items();
         s->worker = std::thread([s] { s->loop(); });
         for (int i = 0; i < mp_server::kStagers; ++i) s->stagers.emplace_back([s] { s->stage_loop(); });
         s->finisher = std::thread([s] { s->finish_loop(); });

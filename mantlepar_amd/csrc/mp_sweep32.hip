@@ -551,6 +551,42 @@ __global__ __launch_bounds__(256, 3) void batch32_kernel(const MeshDev* __restri
     }
 }
 
+// One launch per step over a batch work list: nstages x nitems one-wave
+// workgroups, stage-major; workgroup b runs stage b / nitems on list entry
+// b % nitems, so each stage's segment keeps the list's XCD slabs.  The
+// dependencies are those of step32_kernel within each mesh.
+template <bool FULL>
+__global__ __launch_bounds__(64, 3) void batch_step32_kernel(
+    const MeshDev* __restrict__ meshes, const StagePtrs* __restrict__ tab, int capacity,
+    const int2* __restrict__ items, int nitems, int nstages, uint32_t* __restrict__ flags,
+    const int* __restrict__ flag_off, int nflags, const uint32_t* __restrict__ epoch_ctr, int spin_limit)
+{
+    const int b = int(blockIdx.x);
+    const int stage = b / nitems;
+    if (stage >= nstages) return;
+    const int2 w = items[b - stage * nitems];
+    const int slot = __builtin_amdgcn_readfirstlane(w.x);
+    const int local = __builtin_amdgcn_readfirstlane(w.y);
+    if (slot < 0) return;              // (-1, 0) pads an XCD slab
+    // written by the preceding one-thread kernel, so a plain load sees it
+    const uint32_t epoch = __builtin_amdgcn_readfirstlane(*epoch_ctr);
+    const MeshDev& m = meshes[slot];
+    const StagePtrs& sp = tab[size_t(stage) * capacity + slot];
+    int strip, j0, j1;
+    item_region(sp, m.ny, local, strip, j0, j1);
+    const int off = __builtin_amdgcn_readfirstlane(flag_off[slot]);
+    if (stage > 0 &&
+        !wait_inputs(sp, m.ny, strip, j0, j1, flags + size_t(stage - 1) * nflags + off, epoch, spin_limit))
+        return;
+    sweep32_item<false, FULL, true>(m, sp, local, strip, j0, j1);
+    if (stage + 1 < nstages) publish_item(flags + size_t(stage) * nflags + off + local, epoch);
+}
+
+__global__ void epoch_bump_kernel(uint32_t* ctr)
+{
+    if (threadIdx.x == 0) *ctr += 1u;
+}
+
 }  // namespace
 
 hipError_t launch_stage32(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
@@ -579,6 +615,23 @@ hipError_t launch_step32(const MeshDev* d_mesh, const StagePtrs* tab, int nitems
         if (diffusion) step32_kernel<true, false><<<grid, block, 0, stream>>>(d_mesh, tab, nitems, nstages, flags, epoch, wait_epoch, spin_limit);
         else step32_kernel<false, false><<<grid, block, 0, stream>>>(d_mesh, tab, nitems, nstages, flags, epoch, wait_epoch, spin_limit);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_step32(const MeshDev* meshes, const StagePtrs* tab, int capacity,
+                               const int2* items, int nitems, int nstages, uint32_t* flags,
+                               const int* flag_off, int nflags, uint32_t* epoch_ctr, int spin_limit,
+                               bool global_lf, hipStream_t stream)
+{
+    if (nitems <= 0) return hipSuccess;
+    epoch_bump_kernel<<<1, 64, 0, stream>>>(epoch_ctr);
+    const dim3 grid(unsigned(nitems) * unsigned(nstages)), block(kWave);
+    if (global_lf)
+        batch_step32_kernel<true><<<grid, block, 0, stream>>>(meshes, tab, capacity, items, nitems, nstages, flags,
+                                                              flag_off, nflags, epoch_ctr, spin_limit);
+    else
+        batch_step32_kernel<false><<<grid, block, 0, stream>>>(meshes, tab, capacity, items, nitems, nstages, flags,
+                                                               flag_off, nflags, epoch_ctr, spin_limit);
     return hipGetLastError();
 }
 

@@ -214,6 +214,50 @@ def test_graph_replays_match_eager(prec):
     other.close()
 
 
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_per_step_launch_matches_per_stage_bitwise(graph):
+    """Mixed batch: each step's stages in one launch, every work item waiting
+    on the previous stage's items whose rows it reads, against one launch per
+    stage.  States, phase outputs and step totals agree bit for bit, with open
+    boundaries, a mesh with tail items and three calls in a row."""
+    specs = SPECS + [W.MeshSpec(2048, 3400, seed=19)]
+    runs = {}
+    for per_step in (False, True):
+        b = M.Batch(16, phase=W.config_phase(), precision=M.PREC_MIXED)
+        ids = fill_batch(b, specs, open_ids=(2,))
+        b.set_schedule(12, 0)
+        b.set_launch(per_step)
+        assert b.per_step_launch() == per_step
+        reps = [b.step(2, graph=graph) for _ in range(3)]
+        runs[per_step] = ([b.download(k) for k in ids], [b.phase_state(k) for k in ids], reps, ids)
+        b.close()
+    (st_a, ph_a, rep_a, ids), (st_b, ph_b, rep_b, _) = runs[False], runs[True]
+    for (Ha, Ca), (Hb, Cb) in zip(st_a, st_b):
+        assert np.array_equal(Ha, Hb) and np.array_equal(Ca, Cb)
+    for pa, pb in zip(ph_a, ph_b):
+        for n in ("T", "phi", "Cl", "Cs", "region"):
+            assert np.array_equal(pa[n], pb[n]), n
+    for ra, rb in zip(rep_a, rep_b):
+        for k in ids:
+            for n in ("total_H", "total_C", "boundary_H", "boundary_C", "steps_done"):
+                assert ra[k][n] == rb[k][n], (k, n)
+
+
+def test_launch_mode_choices():
+    b = M.Batch(4, phase=W.config_phase(), precision=M.PREC_MIXED)
+    fill_batch(b, SPECS[:2])
+    assert b.per_step_launch()                 # the default before any tuning
+    b.set_launch(False)
+    assert not b.per_step_launch()
+    b.set_launch(None)
+    b.set_schedule(12, 256)                    # a persistent grid launches per stage
+    assert not b.per_step_launch()
+    with pytest.raises(M.MantleError) as ei:
+        M._lib.check(M._lib.lib().mp_batch_set_launch(b._h, M._lib.LAUNCH_STALL_CHECK))
+    assert ei.value.code == 1
+    b.close()
+
+
 def test_membership_and_schedule_changes():
     """Meshes join and leave between calls and the schedule changes; every
     resident mesh still reproduces its single-mesh run."""
