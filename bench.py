@@ -59,6 +59,10 @@ def parse():
     return p.parse_args()
 
 
+# step32_kernel at 2048x4096, q = 5: 847891 KiB read x 2 + 409033 KiB written
+MEASURED_STEP_LAUNCH_BYTES = (847891 * 2 + 409033) * 1024
+
+
 def copy_calibration():
     import torch
     n = 1 << 27                      # 2^27 doubles = 1 GiB
@@ -192,9 +196,10 @@ def batch_inputs(count=256, seed=3):
 
 
 def bench_batch(args, meshes):
-    """The 256-mesh batch: one persistent launch per stage over all meshes,
-    autotuned schedule, hipGraph replay; mixed and FP64.  The plain FP64 GPU
-    path runs the same meshes one handle at a time."""
+    """The 256-mesh batch: one launch per stage, or per step (mixed), over all
+    meshes, autotuned schedule, hipGraph replay; mixed and FP64.  The profile
+    times per-stage launches one by one.  The plain FP64 GPU path runs the
+    same meshes one handle at a time."""
     import mantlepar_amd as M
     from mantlepar_amd import workloads as W
     K = args.batch_steps
@@ -217,6 +222,7 @@ def bench_batch(args, meshes):
             ids.append(k)
         os.makedirs(os.path.dirname(args.tune_cache), exist_ok=True)
         tune = b.autotune(2, cache_path=args.tune_cache)
+        tune["per_step_launch"] = b.per_step_launch()
         b.step(2, graph=True)                        # warm-up and graph capture of 2 steps
         for (H, Cc), k in zip(inputs, ids):
             b.upload(k, H, Cc)
@@ -735,6 +741,14 @@ def main():
                                            "frac": achieved64 / HBM_PEAK_GBS}},
         "conservation": {"total_H": mixed["report"]["total_H"], "total_C": mixed["report"]["total_C"]},
     }
+    # HBM bytes per launch of this kernel on this workload, from two rocprofv3
+    # counter runs (FETCH_SIZE x 2, WRITE_SIZE x 1 by a plain 1 GiB copy's
+    # counts); the line's own run collects no counters
+    if (spec.nx, spec.ny, spec.q) == (2048, 4096, 5) and mixed["layout"]["one_launch_steps"]:
+        out["roofline"]["traffic"] = MEASURED_STEP_LAUNCH_BYTES
+        out["roofline"]["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of step32_kernel, "
+                                             "copy-calibrated, 24 rows per item "
+                                             "(profiles/step_launch_2048x4096.md)")
     out["roofline"]["practical_copy_GBs"] = copy_gbs
     out["roofline"]["frac_of_practical_copy"] = achieved / copy_gbs if copy_gbs else None
     if args.limiter_steps > 0:
