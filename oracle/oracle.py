@@ -6,7 +6,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle_mp.so")
+# MPO_ORACLE_LIB selects another build of the same source (the sanitizer build)
+LIB_PATH = os.environ.get("MPO_ORACLE_LIB") or os.path.join(HERE, "liboracle_mp.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref_phase.so")
 
 __all__ = [
