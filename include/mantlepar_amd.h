@@ -324,6 +324,12 @@ mp_status mp_batch_get_schedule(mp_batch* b, int32_t* rows_per_item, int32_t* bl
  * mp_batch_get_launch reports whether steps run in one launch now.  No
  * reference counterpart (launch scheduling). */
 mp_status mp_batch_set_launch(mp_batch* b, int32_t mode);
+/* Every step of every mesh is checked against the preheat's heat balance
+ * (src/couple/preheat.cpp:268-273), as mp_transport_step does: a mesh whose
+ * balance fails keeps its state and reports MP_ERR_BALANCE; the others
+ * commit.  Relative tolerance, default 1e-10 (server requests use the
+ * default). */
+mp_status mp_batch_set_balance_tolerance(mp_batch* b, double tol);
 mp_status mp_batch_get_launch(mp_batch* b, int32_t* per_step);
 /* Time candidate schedules on the resident meshes (state is left unchanged),
  * apply the fastest and return it.  cache_path (may be NULL) is a JSON file

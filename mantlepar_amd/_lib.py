@@ -196,6 +196,7 @@ SIGNATURES = [
     ("mp_batch_set_schedule", C.c_int32, [_vp, C.c_int32, C.c_int32]),
     ("mp_batch_get_schedule", C.c_int32, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("mp_batch_set_launch", C.c_int32, [_vp, C.c_int32]),
+    ("mp_batch_set_balance_tolerance", C.c_int32, [_vp, C.c_double]),
     ("mp_batch_get_launch", C.c_int32, [_vp, C.POINTER(C.c_int32)]),
     ("mp_batch_autotune", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_char_p,
                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32), _dp]),

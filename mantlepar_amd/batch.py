@@ -137,6 +137,10 @@ class Batch:
         mode = LAUNCH_AUTO if per_step is None else (LAUNCH_PER_STEP if per_step else LAUNCH_PER_STAGE)
         check(lib().mp_batch_set_launch(self._h, mode))
 
+    def set_balance_tolerance(self, tol=1e-10):
+        """Relative tolerance of every mesh's per-step heat-balance check."""
+        check(lib().mp_batch_set_balance_tolerance(self._h, tol))
+
     def per_step_launch(self):
         """Whether steps currently run in one launch."""
         v = C.c_int32()

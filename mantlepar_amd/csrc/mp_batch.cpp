@@ -523,6 +523,15 @@ mp_status mp_batch_get_schedule(mp_batch* b, int32_t* rows_per_item, int32_t* bl
     return MP_OK;
 }
 
+mp_status mp_batch_set_balance_tolerance(mp_batch* b, double tol)
+{
+    if (!b) return fail(MP_ERR_ARG, "null handle");
+    if (!std::isfinite(tol) || tol < 0) return fail(MP_ERR_ARG, "balance tolerance must be finite and >= 0");
+    b->eng.balance_tol = tol;
+    ++b->eng.generation;      // a captured graph holds the old value
+    return MP_OK;
+}
+
 mp_status mp_batch_set_launch(mp_batch* b, int32_t mode)
 {
     if (!b) return fail(MP_ERR_ARG, "null handle");

@@ -177,6 +177,8 @@ mp_status Engine::init(int cap, int32_t prec, const mp_weno_options& w, const mp
     MP_CALL(d_errors.alloc(sizeof(uint32_t) * cap));
     MP_CALL(d_maxbits.alloc(sizeof(unsigned long long) * cap));
     MP_CALL(d_flag_off.alloc(sizeof(int) * cap));
+    MP_CALL(d_init_totals.alloc(sizeof(double) * 4 * size_t(cap)));
+    MP_CALL(d_init_part.alloc(sizeof(double) * 4 * size_t(kStateSlicesMax) * size_t(cap)));
     MP_CALL(d_epoch.alloc(256));
     MP_HIP(hipMemsetAsync(d_epoch.ptr, 0, 256, stream));
     MP_CALL(reserve_steps(1));
@@ -702,6 +704,10 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
     MP_HIP(launch_clear(d_errors.as<uint32_t>(), capacity, stream));
     if (blocks > 0) MP_HIP(launch_clear(d_queues.as<uint32_t>(), 3 * nsteps, stream));
     const StagePtrs* base = d_stages.as<StagePtrs>();
+    // the first step's stage-0 records name each mesh's starting state
+    if (nsteps > 0 && !active.empty())
+        MP_HIP(launch_batch_state_sums(base, d_meshes.as<MeshDev>(), d_active.as<int>(), int(active.size()),
+                                       d_init_totals.as<double>(), d_init_part.as<double>(), stream));
     for (int s = 0; s < nsteps; ++s) {
         const int set = s == 0 ? 0 : (s % 2 ? 1 : 2);
         int p = 0;      // slots still stepping: a prefix of the ordered list
@@ -746,10 +752,12 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
             }
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
         }
+        const double* prev = s == 0 ? d_init_totals.as<double>()
+                                    : d_totals.as<double>() + size_t(s - 1) * 4 * capacity;
         MP_HIP(launch_batch_reduce(base + size_t(set * 3) * capacity, d_nitems.as<int>(),
                                    d_meshes.as<MeshDev>(), d_active.as<int>(), p, scheme,
                                    kW[scheme - 1], d_totals.as<double>() + size_t(s) * 4 * capacity,
-                                   overlap ? side : stream));
+                                   overlap ? side : stream, prev, balance_tol));
         if (overlap) MP_HIP(hipEventRecord(side_done[par], side));
     }
     return MP_OK;

@@ -126,7 +126,8 @@ public:
     // run, so runs of different lengths share an epoch without padding.
     mp_status set_active(const std::vector<int>& slots, const std::vector<int>& steps = {});
     int steps_of(int slot) const;
-    // Enqueue nsteps of `scheme` over the active slots, each with its own dt.
+    // Enqueue nsteps of `scheme` over the active slots, each with its own dt;
+    // a mesh whose step fails the heat balance gets kErrBalance.
     // write_tables uploads the per-slot stage pointers (not capturable);
     // enqueue then issues kernels only, reading device tables, so it may be
     // captured into a graph.  totals receive [step][slot][4] per-step sums.
@@ -209,6 +210,11 @@ public:
     int q_common = 0;          // q shared by all active slots, or 0
     DeviceBuffer d_meshes, d_stages, d_items, d_nitems, d_active, d_errors, d_totals, d_maxbits;
     DeviceBuffer d_queues;     // per step and stage: work counter of the persistent grid
+    // Every step's totals are checked against the totals before it (the
+    // preheat's heat balance, src/couple/preheat.cpp:268-273) at this relative
+    // tolerance; the totals before a call's first step are formed at its start.
+    double balance_tol = 1e-10;
+    DeviceBuffer d_init_totals, d_init_part;
     std::vector<int> h_nitems;
     // Uploads of the work lists and of the stage tables (which hold only the
     // active slots' entries; the kernels read no others).

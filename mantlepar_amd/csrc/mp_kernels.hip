@@ -478,10 +478,22 @@ __global__ __launch_bounds__(64) void reduce_step_kernel(
         for (int k = 0; k < 6; ++k) part[size_t(slice) * 6 + k] = v[k];
 }
 
+// The preheat's balance check of each field (src/couple/preheat.cpp:268-273):
+// |after - before + dt*outward flux| <= tol*(1 + |after| + |before| + |dt*flux|).
+// out and prev: sum H, sum C (times area), outward boundary flux H, C.
+__device__ __forceinline__ bool balance_holds(const double* out, const double* prev, double dt, double tol)
+{
+    bool ok = true;
+    for (int f = 0; f < 2; ++f) {
+        const double after = out[f], before = prev[f], flux = dt * out[2 + f];
+        const double bal = after - before + flux;
+        ok = ok && fabs(bal) <= tol * (1.0 + fabs(after) + fabs(before) + fabs(flux));
+    }
+    return ok;
+}
+
 // The step's totals; with prev (the totals before the step) also the
-// preheat's balance check of each field (src/couple/preheat.cpp:268-273):
-// |after - before + dt*outward flux| <= tol*(1 + |after| + |before| + |dt*flux|),
-// kErrBalance otherwise.
+// balance check, kErrBalance when it fails.
 __global__ __launch_bounds__(kStepSlices) void reduce_step_final_kernel(const double* __restrict__ part,
                                                                        double area, double* out,
                                                                        const double* prev, double dt,
@@ -495,15 +507,7 @@ __global__ __launch_bounds__(kStepSlices) void reduce_step_final_kernel(const do
     step_tree(sh, tid);
     if (tid == 0) {
         step_out(sh, area, out);
-        if (prev) {
-            bool ok = true;
-            for (int f = 0; f < 2; ++f) {
-                const double after = out[f], before = prev[f], flux = dt * out[2 + f];
-                const double bal = after - before + flux;
-                ok = ok && fabs(bal) <= tol * (1.0 + fabs(after) + fabs(before) + fabs(flux));
-            }
-            if (!ok) atomicOr(error, kErrBalance);
-        }
+        if (prev && !balance_holds(out, prev, dt, tol)) atomicOr(error, kErrBalance);
     }
 }
 
@@ -613,10 +617,12 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
 // of the single-mesh reduction in turn, then the same tree.  Partials of mesh
 // k start at stages[k].partials with nitems[k] items per stage; out[k*4 + 0..3]
 // as in reduce_step_final_kernel.
+// With prev (the totals before the step, prev[k*4 + 0..3]), each mesh's
+// balance is checked at tolerance tol with its own dt and error word.
 __global__ __launch_bounds__(256) void batch_reduce_kernel(
     const StagePtrs* __restrict__ stages, const int* __restrict__ nitems,
     const MeshDev* __restrict__ meshes, const int* __restrict__ slots, int nstages, double w0,
-    double w1, double w2, double* out)
+    double w1, double w2, double* out, const double* prev, double tol)
 {
     __shared__ double sh[6][kStepSlices];
     const int k = slots[blockIdx.x];
@@ -635,7 +641,99 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(
     }
     __syncthreads();
     step_tree(sh, tid);
-    if (tid == 0) step_out(sh, meshes[k].area, out + size_t(k) * 4);
+    if (tid == 0) {
+        double* o = out + size_t(k) * 4;
+        step_out(sh, meshes[k].area, o);
+        if (prev && !balance_holds(o, prev + size_t(k) * 4, stages[k].dt, tol))
+            atomicOr(stages[k].error, kErrBalance);
+    }
+}
+
+// Totals of each mesh's state before a call's first step (the state that the
+// first step's stage-0 record names as u^n): compensated sums of H and C times
+// the area in out[k*4 + 0..1], zero boundary slots.  Pass 1: kStateSlices
+// blocks per mesh (grid x: the mesh, y: the slice), each summing a contiguous slice with four
+// independent sums per lane, so that many loads are in flight; pass 2 merges
+// a mesh's slices in a fixed order.
+constexpr int kStateSlices = kStateSlicesMax;
+
+__device__ __forceinline__ void merge_sums(double* a, const double* b)
+{
+    double s0 = a[0], e0 = a[1] + b[1];
+    two_sum_acc(s0, e0, b[0]);
+    double s1 = a[2], e1 = a[3] + b[3];
+    two_sum_acc(s1, e1, b[2]);
+    a[0] = s0; a[1] = e0; a[2] = s1; a[3] = e1;
+}
+
+__global__ __launch_bounds__(256) void batch_state_sums_kernel(const StagePtrs* __restrict__ stages,
+                                                               const MeshDev* __restrict__ meshes,
+                                                               const int* __restrict__ slots,
+                                                               double* __restrict__ part)
+{
+    constexpr int kThreads = 256, kUnroll = 4;
+    __shared__ double sh[kThreads / kWave][4];
+    const int k = slots[blockIdx.x];
+    const int tid = threadIdx.x;
+    const MeshDev& m = meshes[k];
+    const double* H = stages[k].Hn;
+    const double* C = stages[k].Cn;
+    const int64_t n = int64_t(m.nx) * m.ny;
+    const int64_t chunk = (n + kStateSlices - 1) / kStateSlices;
+    const int64_t lo = min(n, int64_t(blockIdx.y) * chunk), hi = min(n, lo + chunk);
+    double v[kUnroll][4] = {};
+    int64_t c = lo + tid;
+    for (; c + (kUnroll - 1) * kThreads < hi; c += kUnroll * kThreads) {
+        double h[kUnroll], q[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            h[u] = H[c + u * kThreads];
+            q[u] = C[c + u * kThreads];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            two_sum_acc(v[u][0], v[u][1], h[u]);
+            two_sum_acc(v[u][2], v[u][3], q[u]);
+        }
+    }
+    for (; c < hi; c += kThreads) {
+        two_sum_acc(v[0][0], v[0][1], H[c]);
+        two_sum_acc(v[0][2], v[0][3], C[c]);
+    }
+#pragma unroll
+    for (int u = 1; u < kUnroll; ++u) merge_sums(v[0], v[u]);
+    for (int off = 32; off >= 1; off >>= 1) {
+        double o[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) o[q] = __shfl_xor(v[0][q], off);
+        merge_sums(v[0], o);
+    }
+    if ((tid & (kWave - 1)) == 0)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sh[tid / kWave][q] = v[0][q];
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < kThreads / kWave; ++w) merge_sums(v[0], sh[w]);
+        double* p = part + (size_t(blockIdx.x) * kStateSlices + blockIdx.y) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) p[q] = v[0][q];
+    }
+}
+
+__global__ void batch_state_sums_final_kernel(const MeshDev* __restrict__ meshes,
+                                              const int* __restrict__ slots,
+                                              const double* __restrict__ part, double* out)
+{
+    if (threadIdx.x != 0) return;
+    const int k = slots[blockIdx.x];
+    const double* p = part + size_t(blockIdx.x) * kStateSlices * 4;
+    double v[4] = {p[0], p[1], p[2], p[3]};
+    for (int s = 1; s < kStateSlices; ++s) merge_sums(v, p + size_t(s) * 4);
+    double* o = out + size_t(k) * 4;
+    o[0] = (v[0] + v[1]) * meshes[k].area;
+    o[1] = (v[2] + v[3]) * meshes[k].area;
+    o[2] = 0.0;
+    o[3] = 0.0;
 }
 
 // Zero a small device array with a kernel (graph-safe: no memset node).
@@ -1168,11 +1266,21 @@ int batch_blocks_per_cu(int mode)
 
 hipError_t launch_batch_reduce(const StagePtrs* stages, const int* nitems, const MeshDev* meshes,
                                const int* slots, int nslots, int nstages, const double* w,
-                               double* out, hipStream_t stream)
+                               double* out, hipStream_t stream, const double* prev, double tol)
 {
     if (nslots <= 0) return hipSuccess;
-    batch_reduce_kernel<<<nslots, 256, 0, stream>>>(stages, nitems, meshes, slots,
-                                                               nstages, w[0], w[1], w[2], out);
+    batch_reduce_kernel<<<nslots, 256, 0, stream>>>(stages, nitems, meshes, slots, nstages, w[0], w[1],
+                                                    w[2], out, prev, tol);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_state_sums(const StagePtrs* stages, const MeshDev* meshes, const int* slots,
+                                   int nslots, double* out, double* part, hipStream_t stream)
+{
+    if (nslots <= 0) return hipSuccess;
+    batch_state_sums_kernel<<<dim3(unsigned(nslots), kStateSlices), 256, 0, stream>>>(stages, meshes, slots,
+                                                                                      part);
+    batch_state_sums_final_kernel<<<nslots, 64, 0, stream>>>(meshes, slots, part, out);
     return hipGetLastError();
 }
 

@@ -44,9 +44,17 @@ hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H
 hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes,
                               const StagePtrs* stages, const int2* items, int nitems, int blocks,
                               int* queue, hipStream_t stream, bool global_lf = false);
+// prev (may be null): per-mesh totals before the step; with it each mesh's
+// heat balance is checked at tolerance tol (kErrBalance in its error word).
 hipError_t launch_batch_reduce(const StagePtrs* stages, const int* nitems, const MeshDev* meshes,
                                const int* slots, int nslots, int nstages, const double* w,
-                               double* out, hipStream_t stream);
+                               double* out, hipStream_t stream, const double* prev = nullptr,
+                               double tol = 0.0);
+// Per-mesh totals of the state that stages[k] names as u^n (out[k*4 + 0..3]);
+// part: scratch of nslots * kStateSlicesMax * 4 doubles.
+constexpr int kStateSlicesMax = 32;
+hipError_t launch_batch_state_sums(const StagePtrs* stages, const MeshDev* meshes, const int* slots,
+                                   int nslots, double* out, double* part, hipStream_t stream);
 hipError_t launch_clear(uint32_t* p, int n, hipStream_t stream);
 // Resident blocks per CU of the batched stage kernel (0 if the query fails).
 int batch_blocks_per_cu(int mode);

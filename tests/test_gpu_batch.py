@@ -243,6 +243,40 @@ def test_per_step_launch_matches_per_stage_bitwise(graph):
                 assert ra[k][n] == rb[k][n], (k, n)
 
 
+@pytest.mark.parametrize("prec", [M.PREC_FP64, M.PREC_MIXED], ids=["fp64", "mixed"])
+def test_failed_heat_balance_rejects_the_mesh(prec):
+    """Every mesh's steps are checked against the preheat's heat balance
+    (src/couple/preheat.cpp:268-273), as the single-mesh step does.  Forced from
+    the host: at tolerance 0 only an exact balance passes, and a through-flow's
+    rounding residue is not zero, so the call fails with MP_ERR_BALANCE and the
+    failing meshes keep their state; at the default 1e-10 the same steps pass."""
+    spec = W.MeshSpec(96, 128, seed=41)
+    vx, vy = W.throughflow_velocity(spec)
+    bc, (inH, inC) = W.throughflow_boundary(spec)
+    b = M.Batch(4, phase=W.config_phase(), precision=prec)
+    k = b.add_mesh(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q)
+    b.upload(k, W.enthalpy(spec), W.composition(spec))
+    b.set_face_velocity(k, vx, vy)
+    b.set_boundary(k, bc.get("bottom"), bc.get("right"), bc.get("top"), bc.get("left"))
+    b.set_inflow(k, inH, inC)
+    b.set_dt(k, b.stable_dt(k, 0.4))
+    others = fill_batch(b, SPECS[:2])
+    before = {i: b.download(i) for i in [k] + others}
+    b.set_balance_tolerance(0.0)
+    with pytest.raises(M.MantleError) as e:
+        b.step(3)
+    assert e.value.code == 10 and "heat balance failed" in str(e.value)
+    H1, C1 = b.download(k)
+    assert np.array_equal(H1, before[k][0]) and np.array_equal(C1, before[k][1])
+    b.set_balance_tolerance()
+    reps = b.step(3, graph=True)
+    for i in [k] + others:
+        assert reps[i]["steps_done"] == 3 and reps[i]["error_bits"] == 0
+    with pytest.raises(M.MantleError):
+        b.set_balance_tolerance(-1.0)
+    b.close()
+
+
 def test_launch_mode_choices():
     b = M.Batch(4, phase=W.config_phase(), precision=M.PREC_MIXED)
     fill_batch(b, SPECS[:2])
