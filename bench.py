@@ -504,15 +504,19 @@ def bench_full_chip(args):
     return out
 
 
-def bench_limiter(args, spec, H, Cc, vx, vy, phase):
-    """FP64 sweep with the bound limiter of the coupled reconstruction on every
-    stage, beside the same sweep without it, on the single-mesh workload."""
+def bench_limiter(args, spec, H, Cc, vx, vy, phase, prec=None):
+    """The sweep (FP64, or mixed with per-stage launches, as the limiter runs)
+    with the bound limiter of the coupled reconstruction on every stage,
+    beside the same sweep without it, on the single-mesh workload."""
     import mantlepar_amd as M
+    prec = M.PREC_FP64 if prec is None else prec
     t = M.Transport(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q,
-                    phase=phase, precision=M.PREC_FP64)
+                    phase=phase, precision=prec)
     t.set_face_velocity(vx, vy)
     dt = t.stable_dt(0.4)
     t.set_tile(16)
+    if prec == M.PREC_MIXED:
+        t.set_launch(False)
     out = {}
     for on in (False, True):
         t.set_limiter(on)
@@ -536,7 +540,9 @@ def bench_limiter(args, spec, H, Cc, vx, vy, phase):
     out["stage_overhead"] = out["limited"]["ms_stage"] / out["unlimited"]["ms_stage"]
     out["steps"] = args.limiter_steps
     out["points_per_cell_field"] = 86
-    log(f"limiter: {out['limited']['cell_steps_per_s']:.3e} cell-steps/s, stage x{out['stage_overhead']:.2f}")
+    name = "mixed" if prec == M.PREC_MIXED else "fp64"
+    log(f"limiter {name}: {out['limited']['cell_steps_per_s']:.3e} cell-steps/s, "
+        f"stage x{out['stage_overhead']:.2f}")
     return out
 
 
@@ -753,6 +759,7 @@ def main():
     out["roofline"]["frac_of_practical_copy"] = achieved / copy_gbs if copy_gbs else None
     if args.limiter_steps > 0:
         out["limiter_fp64"] = bench_limiter(args, spec, H, Cc, vx, vy, phase)
+        out["limiter_mixed"] = bench_limiter(args, spec, H, Cc, vx, vy, phase, M.PREC_MIXED)
         out["diffusion"] = bench_diffusion(args, spec, H, Cc, vx, vy, phase)
     if args.coupled_steps > 0:
         out["coupled_fp64"] = bench_coupled(args)

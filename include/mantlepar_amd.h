@@ -228,8 +228,10 @@ mp_status mp_transport_layout(mp_transport* t, int32_t* rows_per_item, int32_t* 
  * and extentFraction (src/mlweno/diffusiveflux.h:30-38; defaults 5, 4, 0.9)
  * as arguments.  Each cell's factor theta makes mean + theta*(R(p) - mean)
  * stay within its 3x3 neighbourhood's bounds, clamped to H >= 0 and
- * 0 <= C <= Xe (C <= 1 with the hypereutectic mirror).  FP64 precision mode
- * only; an average outside those ranges fails the step with MP_ERR_ARG
+ * 0 <= C <= Xe (C <= 1 with the hypereutectic mirror).  Both precision
+ * modes: the mixed sweep evaluates the blend's extremes at the same points in
+ * FP32 and takes the bounds relative to the mean from its FP64 differences.
+ * An average outside those ranges fails the step with MP_ERR_ARG
  * ("Inadmissible cell average in coupled reconstruction").  enable = 0
  * switches it off (the default). */
 mp_status mp_transport_set_limiter(mp_transport* t, int32_t enable, int32_t cell_points_per_axis,

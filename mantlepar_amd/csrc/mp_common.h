@@ -123,6 +123,10 @@ struct MeshDev {
     double lim_cw[kMaxQ];       // cell Gauss weights on [-1, 1]
     double lim_xi[kMaxLimS], lim_xb[kMaxLimS];
     double lim_yi[kMaxLimS], lim_yb[kMaxLimS];
+    // The same point offsets rounded to FP32 for the mixed sweep's limiter,
+    // with the edge point offsets sv, sh.
+    float lim32_cx[kMaxQ], lim32_cy[kMaxQ], lim32_sv[kMaxQ], lim32_sh[kMaxQ];
+    float lim32_xi[kMaxLimS], lim32_xb[kMaxLimS], lim32_yi[kMaxLimS], lim32_yb[kMaxLimS];
     // Thermal diffusion of H in the legacy preheat form (samples are the cell
     // averages of each side): interior face flux dif_x*(H_L - H_R) along +x on
     // vertical faces, dif_y likewise on horizontal ones (0: off); a boundary

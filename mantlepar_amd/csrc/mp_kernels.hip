@@ -774,14 +774,19 @@ __global__ void recon_diag_kernel(const MeshDev* __restrict__ mesh, const double
     R B[3][3];
     blend<R>(m, w, candidate_mask(i, j, m.nx, m.ny), B, err, sigma + size_t(c) * 5,
              omega + size_t(c) * 5);
-    if constexpr (MODE == 0) {
-        if (m.limit) {
-            const unsigned sides = (j == 0 ? 1u : 0u) | (i == m.nx - 1 ? 2u : 0u) |
-                                   (j == m.ny - 1 ? 4u : 0u) | (i == 0 ? 8u : 0u);
-            const double th = limit_factor<0>(m, w, B, sides, field ? m.lim_cupper : INFINITY, err);
-            for (int b = 0; b < 3; ++b)
-                for (int a = 0; a < 3; ++a) B[b][a] *= th;
-        }
+    if (m.limit) {
+        const unsigned sides = (j == 0 ? 1u : 0u) | (i == m.nx - 1 ? 2u : 0u) |
+                               (j == m.ny - 1 ? 4u : 0u) | (i == 0 ? 8u : 0u);
+        double Bd[3][3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) Bd[b][a] = double(B[b][a]);
+        const double th = limit_factor<0>(m, w, Bd, sides, field ? m.lim_cupper : INFINITY, err);
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) B[b][a] = R(double(B[b][a]) * th);
     }
     face_lines<R>(m, B, L);
     for (int f = 0; f < 4; ++f)
@@ -846,8 +851,13 @@ __global__ __launch_bounds__(256) void point_eval_kernel(const MeshDev* __restri
         R B[3][3];
         blend<R>(m, w, mask, B, err);
         double th = 1.0;
-        if constexpr (MODE == 0) {
-            if (m.limit) th = limit_factor<0>(m, w, B, sides, field ? m.lim_cupper : INFINITY, err);
+        if (m.limit) {
+            double Bd[3][3];
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) Bd[b][a] = double(B[b][a]);
+            th = limit_factor<0>(m, w, Bd, sides, field ? m.lim_cupper : INFINITY, err);
         }
         double p = 0.0;
 #pragma unroll
@@ -1120,18 +1130,19 @@ static void launch_fp64(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, 
 }
 
 hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
-                        int nitems, hipStream_t stream, bool limit, bool diffusion, bool global_lf)
+                        int nitems, hipStream_t stream, int limit, bool diffusion, bool global_lf)
 {
     if (mode != 0) {
-        if (phase || limit) return hipErrorInvalidValue;   // mixed: phase_items_kernel follows
-        return launch_stage32(d_mesh, sp, nitems, stream, diffusion, global_lf);
+        if (phase) return hipErrorInvalidValue;   // mixed: phase_items_kernel follows
+        return launch_stage32(d_mesh, sp, nitems, stream, diffusion, global_lf, limit);
     }
+    const bool lim = limit != 0;
     if (q == 5) {
-        if (phase) launch_fp64<true, 5>(d_mesh, sp, nitems, stream, limit, diffusion);
-        else launch_fp64<false, 5>(d_mesh, sp, nitems, stream, limit, diffusion);
+        if (phase) launch_fp64<true, 5>(d_mesh, sp, nitems, stream, lim, diffusion);
+        else launch_fp64<false, 5>(d_mesh, sp, nitems, stream, lim, diffusion);
     } else {
-        if (phase) launch_fp64<true, 0>(d_mesh, sp, nitems, stream, limit, diffusion);
-        else launch_fp64<false, 0>(d_mesh, sp, nitems, stream, limit, diffusion);
+        if (phase) launch_fp64<true, 0>(d_mesh, sp, nitems, stream, lim, diffusion);
+        else launch_fp64<false, 0>(d_mesh, sp, nitems, stream, lim, diffusion);
     }
     return hipGetLastError();
 }

@@ -11,8 +11,9 @@ namespace mp {
 // diffusion: the mesh has thermal diffusion or thermal boundary data (the
 // mixed sweep compiles the terms out otherwise).
 // global_lf: the mesh uses the global LF speed (mixed sweep's record form).
+// limit: 0 off, else the bound limiter (mixed: the mode of limit_mode32).
 hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
-                        int nitems, hipStream_t stream, bool limit = false, bool diffusion = true,
+                        int nitems, hipStream_t stream, int limit = 0, bool diffusion = true,
                         bool global_lf = false);
 hipError_t launch_limit_diag(const MeshDev* d_mesh, int ncells, const double* H, const double* C,
                              double* thetaH, double* thetaC, uint32_t* error, hipStream_t stream);
@@ -64,8 +65,17 @@ int batch_blocks_per_cu32();
 // diffusion: the mesh has thermal diffusion or thermal boundary data.
 // global_lf: the mesh uses the global LF speed, so the sweep fetches full face
 // records instead of their one-sided form.
+// limit: the bound limiter (the mesh's limiter tables set): 0 off, 1 any
+// tables, 2 the default tables (q = 5, 5 cell points per axis, 4 samples),
+// whose point loops are unrolled at compile time.
 hipError_t launch_stage32(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
-                          bool diffusion, bool global_lf);
+                          bool diffusion, bool global_lf, int limit = 0);
+// The limit argument of launch_stage32 for a mesh record.
+inline int limit_mode32(const MeshDev& m)
+{
+    if (!m.limit) return 0;
+    return (m.q == 5 && m.lim_ncp == 5 && m.lim_ni == 2 && m.lim_nb == 4) ? 2 : 1;
+}
 // All stages of one mixed-precision step in one launch: tab[0..nstages) are
 // the stage records, flags [nstages][nitems] completion flags that hold
 // earlier epochs only (never `epoch`) when the launch starts.  Items publish

@@ -22,6 +22,8 @@
 // differences, and their x and y slopes take two values each.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mp_common.h"
 #include "mp_device.h"
 #include "mp_launch.h"
@@ -99,12 +101,13 @@ __device__ __forceinline__ Row32 make_row(double u, double outer, double below, 
     return R;
 }
 
-// Reconstruction of the cell at row r from the rows below (R0), at (R1) and
-// above (R2) it.  EDGE = false is the interior form: every candidate lies in
-// the mesh and there is no constant candidate, so no weight needs masking.
+// Centred blend B[b][a] (coefficient of x^a y^b) of the cell at row r from
+// the rows below (R0), at (R1) and above (R2) it.  EDGE = false is the
+// interior form: every candidate lies in the mesh and there is no constant
+// candidate, so no weight needs masking.
 template <bool EDGE>
-__device__ __forceinline__ void recon32(const K32& k, const Row32& R0, const Row32& R1,
-                                        const Row32& R2, unsigned mask, FaceLines<float>& out,
+__device__ __forceinline__ void blend32(const K32& k, const Row32& R0, const Row32& R1,
+                                        const Row32& R2, unsigned mask, float (&B)[3][3],
                                         uint32_t& err)
 {
     const float elo = R1.e, ehi = R2.e;   // u(r) - u(r-1), u(r+1) - u(r)
@@ -191,7 +194,6 @@ __device__ __forceinline__ void recon32(const K32& k, const Row32& R0, const Row
     for (int s = 0; s < 5; ++s) om[s] *= inv;
     // Blend (Q1 candidates embed in the Q2 monomials; the constant candidate's
     // centred polynomial is zero).
-    float B[3][3];
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
@@ -199,6 +201,11 @@ __device__ __forceinline__ void recon32(const K32& k, const Row32& R0, const Row
     B[0][1] += (om[1] + om[2]) * q10p + (om[3] + om[4]) * q10m;
     B[1][0] += (om[2] + om[4]) * q01p + (om[1] + om[3]) * q01m;
     B[1][1] += om[1] * q11[0] + om[2] * q11[1] + om[3] * q11[2] + om[4] * q11[3];
+}
+
+// The blend's four face lines c0 + s*c1 + s^2*c2.
+__device__ __forceinline__ void lines32(const K32& k, const float (&B)[3][3], FaceLines<float>& out)
+{
     const float hx = k.hx, hy = k.hy;
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
@@ -210,6 +217,149 @@ __device__ __forceinline__ void recon32(const K32& k, const Row32& R0, const Row
         out.c[2][a] = B[0][a] + hy * (B[1][a] + hy * B[2][a]);   // top,    y = +hy
         out.c[0][a] = B[0][a] - hy * (B[1][a] - hy * B[2][a]);   // bottom, y = -hy
     }
+}
+
+template <bool EDGE>
+__device__ __forceinline__ void recon32(const K32& k, const Row32& R0, const Row32& R1,
+                                        const Row32& R2, unsigned mask, FaceLines<float>& out,
+                                        uint32_t& err)
+{
+    float B[3][3];
+    blend32<EDGE>(k, R0, R1, R2, mask, B, err);
+    lines32(k, B, out);
+}
+
+// Extremes of P along x = const over ys[0..n) (FP32; the offsets are
+// wave-uniform, read from the mesh record).  N > 0 fixes n at compile time.
+template <int N>
+__device__ __forceinline__ void col_ext32(const float (&B)[3][3], float x, const float* ys, int n,
+                                          float& lo, float& hi)
+{
+    const float q0 = B[0][0] + x * (B[0][1] + x * B[0][2]);
+    const float q1 = B[1][0] + x * (B[1][1] + x * B[1][2]);
+    const float q2 = B[2][0] + x * (B[2][1] + x * B[2][2]);
+#pragma unroll
+    for (int k = 0; k < (N > 0 ? N : kMaxQ); ++k) {
+        if (N == 0 && k >= n) break;
+        const float y = ys[k];
+        const float v = q0 + y * (q1 + y * q2);
+        lo = fminf(lo, v);
+        hi = fmaxf(hi, v);
+    }
+}
+
+// Extremes along y = const over xs[0..n).
+template <int N>
+__device__ __forceinline__ void row_ext32(const float (&B)[3][3], float y, const float* xs, int n,
+                                          float& lo, float& hi)
+{
+    const float r0 = B[0][0] + y * (B[1][0] + y * B[2][0]);
+    const float r1 = B[0][1] + y * (B[1][1] + y * B[2][1]);
+    const float r2 = B[0][2] + y * (B[1][2] + y * B[2][2]);
+#pragma unroll
+    for (int k = 0; k < (N > 0 ? N : kMaxQ); ++k) {
+        if (N == 0 && k >= n) break;
+        const float x = xs[k];
+        const float v = r0 + x * (r1 + x * r2);
+        lo = fminf(lo, v);
+        hi = fmaxf(hi, v);
+    }
+}
+
+// Limiter tables fixed at compile time (edge points NQ, cell points per
+// axis NCP, normal-line samples NS) or read from the mesh record (all 0).
+template <int NQ, int NCP, int NS> struct LimShape {
+    static constexpr int nq = NQ, ncp = NCP, ns = NS;
+};
+using LimAny = LimShape<0, 0, 0>;
+using LimDefault = LimShape<5, 5, 4>;     // q = 5, cellPointsPerAxis 5, 4 samples
+
+// The bound limiter (StateReconstruction::Limit, src/couple/state_reconstruction.cpp:
+// 136-192) on the mixed sweep's data: the limit_factor of mp_limit.h with the
+// extremes of the centred blend at the same 86 points evaluated in FP32.  The
+// neighbourhood bounds come relative to the cell mean from the rows' FP64
+// differences (rounded once), so no FP64 average is re-read: dhi = max(0, the
+// eight differences) capped at upper - t, dlo likewise floored at -t.  theta is
+// rounded inward by 2^-22 so the limited FP32 values stay inside the bounds.
+template <class S>
+__device__ __forceinline__ float limit32(const MeshDev& m, const K32& k, const Row32& R0, const Row32& R1,
+                                         const Row32& R2, const float (&B)[3][3], unsigned sides,
+                                         double upper, uint32_t& err)
+{
+    constexpr int NQ = S::nq, NCP = S::ncp, NS = S::ns;
+    const double t = R1.u;
+    if (!(t >= 0.0 && t <= upper)) err |= kErrInadmissible;
+    const float e0 = -R1.e, e2 = R2.e;        // centre rows below and above, minus t
+    const float c00 = e0 + R0.dl, c01 = e0 + R0.dr, c20 = e2 + R2.dl, c21 = e2 + R2.dr;
+    float dhi = fmaxf(fmaxf(fmaxf(0.0f, R1.dl), fmaxf(R1.dr, e0)), fmaxf(fmaxf(e2, c00), fmaxf(c01, fmaxf(c20, c21))));
+    float dlo = fminf(fminf(fminf(0.0f, R1.dl), fminf(R1.dr, e0)), fminf(fminf(e2, c00), fminf(c01, fminf(c20, c21))));
+    dhi = fminf(dhi, float(upper - t));
+    dlo = fmaxf(dlo, float(-t));
+    bool finite = true;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) finite &= isfinite(B[j][i]);
+    if (!finite) err |= kErrNonFinite;
+
+    const int nq = NQ > 0 ? NQ : m.q;
+    const int ncp = NCP > 0 ? NCP : m.lim_ncp;
+    const int ni = NS > 0 ? NS / 2 : m.lim_ni, nb = NS > 0 ? NS : m.lim_nb;
+    float pmin = B[0][0], pmax = B[0][0];                              // cell centre
+#pragma unroll
+    for (int a = 0; a < (NCP > 0 ? NCP : kMaxQ); ++a) {                // cell Gauss points
+        if (NCP == 0 && a >= ncp) break;
+        col_ext32<NCP>(B, m.lim32_cx[a], m.lim32_cy, ncp, pmin, pmax);
+    }
+    col_ext32<NQ>(B, k.hx, m.lim32_sv, nq, pmin, pmax);                 // right and left edge points
+    col_ext32<NQ>(B, -k.hx, m.lim32_sv, nq, pmin, pmax);
+    row_ext32<NQ>(B, k.hy, m.lim32_sh, nq, pmin, pmax);                 // top and bottom edge points
+    row_ext32<NQ>(B, -k.hy, m.lim32_sh, nq, pmin, pmax);
+    // normal-line samples: n/2 in-cell samples of an interior face, n of a
+    // boundary face; the left and bottom faces mirror the right and top ones
+    auto cols = [&](float sign, const float* xs, int n, auto nn) {
+        constexpr int N = decltype(nn)::value;
+#pragma unroll
+        for (int s = 0; s < (N > 0 ? N : kMaxLimS); ++s) {
+            if (N == 0 && s >= n) break;
+            col_ext32<NQ>(B, sign * xs[s], m.lim32_sv, nq, pmin, pmax);
+        }
+    };
+    auto rows = [&](float sign, const float* ys, int n, auto nn) {
+        constexpr int N = decltype(nn)::value;
+#pragma unroll
+        for (int s = 0; s < (N > 0 ? N : kMaxLimS); ++s) {
+            if (N == 0 && s >= n) break;
+            row_ext32<NQ>(B, sign * ys[s], m.lim32_sh, nq, pmin, pmax);
+        }
+    };
+    using Ni = std::integral_constant<int, NS / 2>;
+    using Nb = std::integral_constant<int, NS>;
+    if (sides & 2u) cols(1.0f, m.lim32_xb, nb, Nb{}); else cols(1.0f, m.lim32_xi, ni, Ni{});
+    if (sides & 8u) cols(-1.0f, m.lim32_xb, nb, Nb{}); else cols(-1.0f, m.lim32_xi, ni, Ni{});
+    if (sides & 4u) rows(1.0f, m.lim32_yb, nb, Nb{}); else rows(1.0f, m.lim32_yi, ni, Ni{});
+    if (sides & 1u) rows(-1.0f, m.lim32_yb, nb, Nb{}); else rows(-1.0f, m.lim32_yi, ni, Ni{});
+    float theta = 1.0f;
+    if (pmax > dhi) theta = fminf(theta, dhi / pmax);
+    if (pmin < dlo) theta = fminf(theta, dlo / pmin);
+    return theta < 1.0f ? fmaxf(0.0f, theta * (1.0f - 0x1p-22f)) : 1.0f;
+}
+
+// Limited reconstruction of one field: blend, limit factor, scaled face lines.
+template <bool EDGE, class S>
+__device__ __forceinline__ void recon32_limited(const MeshDev& m, const K32& k, const Row32& R0,
+                                                const Row32& R1, const Row32& R2, unsigned mask,
+                                                unsigned sides, double upper, FaceLines<float>& out,
+                                                uint32_t& err)
+{
+    float B[3][3];
+    blend32<EDGE>(k, R0, R1, R2, mask, B, err);
+    const float th = limit32<S>(m, k, R0, R1, R2, B, sides, upper, err);
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) B[b][a] *= th;
+    lines32(k, B, out);
 }
 
 // The face's full moments from its one-sided record: the side that v's sign
@@ -264,7 +414,9 @@ __device__ __forceinline__ void face_flux(const FaceMom& f, double D, double tLH
 // in the first stage's loads and the last stage's stores, which only meet
 // other launches, measured slower: 0.657 against 0.624 ms per 2048 x 4096
 // step launch.)
-template <bool DIFF, bool FULL, bool SHARED = false>
+// LIM: 0 no limiter, 1 the limiter with the mesh record's tables, 2 with the
+// default tables fixed at compile time (LimDefault).
+template <bool DIFF, bool FULL, bool SHARED = false, int LIM = 0>
 __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, int strip, int j0,
                                              int j1)
 {
@@ -351,7 +503,29 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
             load(sp.Ck, r + 2, nC, oC);
         }
         FaceLines<float> LH, LC;
-        if (have && inner_strip && r >= 1 && r <= ny - 2) {
+        if constexpr (LIM != 0) {
+            using S = std::conditional_t<LIM == 2, LimDefault, LimAny>;
+            if (have) {
+                // one field at a time: its blend, limit factor and face lines
+                const unsigned sides = (r == 0 ? 1u : 0u) | (i == nx - 1 ? 2u : 0u) |
+                                       (r == ny - 1 ? 4u : 0u) | (i == 0 ? 8u : 0u);
+                uint32_t e2 = 0;
+                if (inner_strip && r >= 1 && r <= ny - 2) {
+                    recon32_limited<false, S>(m, k, H0, H1, H2, 31u, sides, INFINITY, LH, e2);
+                    recon32_limited<false, S>(m, k, C0, C1, C2, 31u, sides, m.lim_cupper, LC, e2);
+                } else {
+                    const unsigned mask = col_ok ? candidate_mask(i, r, nx, ny) : 2u;
+                    recon32_limited<true, S>(m, k, H0, H1, H2, mask, sides, INFINITY, LH, e2);
+                    recon32_limited<true, S>(m, k, C0, C1, C2, mask, sides, m.lim_cupper, LC, e2);
+                }
+                if (col_ok) err |= e2;
+            } else {
+#pragma unroll
+                for (int f = 0; f < 4; ++f)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) { LH.c[f][c] = 0.f; LC.c[f][c] = 0.f; }
+            }
+        } else if (have && inner_strip && r >= 1 && r <= ny - 2) {
             recon32<false>(k, H0, H1, H2, 31u, LH, err);
             recon32<false>(k, C0, C1, C2, 31u, LC, err);
         } else if (have) {
@@ -474,15 +648,20 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
 // One wave per workgroup: a finished item frees its slot at once instead of
 // waiting for the other waves of a larger workgroup (7% faster at 2048x4096).
 // Register budget: three waves per SIMD, twelve workgroups per CU.
-template <bool DIFF, bool FULL>
-__global__ __launch_bounds__(64, 3) void stage32_kernel(const MeshDev* __restrict__ mesh, StagePtrs sp, int nitems)
+// LIM (the bound limiter): three waves per SIMD hold the limiter's point
+// loops without scratch on their own; the unrolled default tables, the
+// diffusion terms or the full face records would spill 12-20 B per lane
+// there, so those instances run two waves.
+template <bool DIFF, bool FULL, int LIM = 0>
+__global__ __launch_bounds__(64, LIM == 2 || (LIM != 0 && (DIFF || FULL)) ? 2 : 3) void stage32_kernel(const MeshDev* __restrict__ mesh, StagePtrs sp,
+                                                                  int nitems)
 {
     if (int(blockIdx.x) >= nitems) return;
     const int item = item_of_block(sp, int(blockIdx.x), nitems);
     const MeshDev& m = *mesh;
     int strip, j0, j1;
     item_region(sp, m.ny, item, strip, j0, j1);
-    sweep32_item<DIFF, FULL>(m, sp, item, strip, j0, j1);
+    sweep32_item<DIFF, FULL, false, LIM>(m, sp, item, strip, j0, j1);
 }
 
 // One launch per step: nstages * nitems one-wave workgroups.  Workgroup b runs
@@ -590,10 +769,26 @@ __global__ void epoch_bump_kernel(uint32_t* ctr)
 }  // namespace
 
 hipError_t launch_stage32(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
-                          bool diffusion, bool global_lf)
+                          bool diffusion, bool global_lf, int limit)
 {
     const dim3 grid(nitems), block(kWave);
-    if (global_lf) {
+    if (limit == 2) {
+        if (global_lf) {
+            if (diffusion) stage32_kernel<true, true, 2><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+            else stage32_kernel<false, true, 2><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        } else {
+            if (diffusion) stage32_kernel<true, false, 2><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+            else stage32_kernel<false, false, 2><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        }
+    } else if (limit) {
+        if (global_lf) {
+            if (diffusion) stage32_kernel<true, true, 1><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+            else stage32_kernel<false, true, 1><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        } else {
+            if (diffusion) stage32_kernel<true, false, 1><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+            else stage32_kernel<false, false, 1><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+        }
+    } else if (global_lf) {
         if (diffusion) stage32_kernel<true, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
         else stage32_kernel<false, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
     } else {

@@ -334,6 +334,18 @@ mp_status set_limiter_tables(MeshDev& m, int cell_points, int samples, double ex
         m.lim_xb[k] = (double)((long double)m.hx - depth * m.aw);
         m.lim_yb[k] = (double)((long double)m.hy - depth * m.bw);
     }
+    for (int k = 0; k < kMaxQ; ++k) {
+        m.lim32_cx[k] = float(m.lim_cx[k]);
+        m.lim32_cy[k] = float(m.lim_cy[k]);
+        m.lim32_sv[k] = float(m.sv[k]);
+        m.lim32_sh[k] = float(m.sh[k]);
+    }
+    for (int k = 0; k < kMaxLimS; ++k) {
+        m.lim32_xi[k] = float(m.lim_xi[k]);
+        m.lim32_xb[k] = float(m.lim_xb[k]);
+        m.lim32_yi[k] = float(m.lim_yi[k]);
+        m.lim32_yb[k] = float(m.lim_yb[k]);
+    }
     m.lim_cupper = c_upper;
     m.limit = 1;
     return MP_OK;

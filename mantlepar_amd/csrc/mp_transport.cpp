@@ -426,7 +426,7 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
             const bool split = last && precision == MP_PREC_MIXED;
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
             MP_HIP(launch_stage(precision, last && !split, mesh.q, d_md.as<MeshDev>(), sp, nitems,
-                                stream, md.limit != 0, has_diffusion(), md.lf_mode != 0));
+                                stream, limit_mode32(md), has_diffusion(), md.lf_mode != 0));
             if (split && overlap) {
                 MP_HIP(hipEventRecord(stage3[par], stream));
                 MP_HIP(hipStreamWaitEvent(side, stage3[par], 0));
@@ -778,8 +778,6 @@ mp_status mp_transport::set_limiter(int enable, int cell_points, int samples, do
         md.limit = 0;
         return sync_md();
     }
-    if (precision != MP_PREC_FP64)
-        return fail(MP_ERR_ARG, "the bound limiter runs in the FP64 precision mode");
     MP_CALL(set_limiter_tables(md, cell_points, samples, extent,
                                phase_params.hypereutectic ? 1.0 : phase_params.Xe));
     return sync_md();

@@ -1,4 +1,4 @@
-"""Bound limiter on the GPU (FP64 sweep) against the oracle, through the C ABI.
+"""Bound limiter on the GPU (FP64 and mixed sweeps) against the oracle, through the C ABI.
 
 Bars: limiter factors within 1e-9 of the oracle's (they are ratios of
 rounding-sized differences where the reconstruction barely leaves its bounds,
@@ -6,7 +6,7 @@ so their own last digits are not meaningful); limited traces within 1e-13 of
 the field maximum; states after 10 SSP-RK3 steps with the limiter on every
 stage within 1e-12 relative (max norm), every state inside H >= 0 and
 0 <= C <= Xe; switching the limiter off restores the unlimited results
-bitwise.
+bitwise.  The mixed sweep's limiter: states and traces within 1e-6.
 """
 import numpy as np
 import pytest
@@ -54,10 +54,10 @@ def interior_fields(spec, seed=7):
     return H, Cc
 
 
-def make(spec, H, Cc, limiter=True, points=5, samples=4, extent=0.9):
+def make(spec, H, Cc, limiter=True, points=5, samples=4, extent=0.9, precision=M.PREC_FP64):
     vx, vy = W.face_velocity(spec)
     t = M.Transport(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q,
-                    phase=phase(), precision=M.PREC_FP64)
+                    phase=phase(), precision=precision)
     t.upload(H, Cc)
     t.set_face_velocity(vx, vy)
     if limiter:
@@ -142,8 +142,48 @@ def test_errors():
         with pytest.raises(M.MantleError):
             t.set_limiter(*args)
     t.close()
-    m = M.Transport(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q,
-                    phase=phase(), precision=M.PREC_MIXED)
-    with pytest.raises(M.MantleError, match="FP64"):
-        m.set_limiter(True)
+    m, _ = make(spec, H, bad, precision=M.PREC_MIXED)
+    with pytest.raises(M.MantleError, match="Inadmissible cell average") as ei:
+        m.step(1, m.stable_dt(0.3))
+    assert ei.value.code == 1
     m.close()
+
+
+@pytest.mark.parametrize("q,points,samples,extent", [(5, 5, 4, 0.9), (3, 3, 6, 0.75)],
+                         ids=["q5-5pts-4samples", "q3-3pts-6samples"])
+def test_mixed_limited_steps_match_oracle(q, points, samples, extent):
+    """The mixed sweep's limiter (FP32 extremes at the same points, bounds from
+    the FP64 differences): ten limited SSP-RK3 steps within 1e-6 of the
+    oracle's FP64 run relative to the field maximum, states inside the bounds,
+    and its limiter factors within 1e-5 of the FP64 sweep's where they act."""
+    spec = W.MeshSpec(48, 40, seed=3, q=q)
+    H, Cc = interior_fields(spec)
+    t, o = make(spec, H, Cc, points=points, samples=samples, extent=extent, precision=M.PREC_MIXED)
+    dt = t.stable_dt(0.4)
+    for chunk in range(2):
+        t.step(5, dt)
+        Hg, Cg = t.download()
+        assert Hg.min() >= 0 and Cg.min() >= 0 and Cg.max() <= XE, chunk
+    Ho, Co, _ = o.advance(H, Cc, 10, dt, 3)
+    assert np.abs(Hg - Ho).max() <= 1e-6 * np.abs(Ho).max()
+    assert np.abs(Cg - Co).max() <= 1e-6 * np.abs(Co).max()
+    # the limiter acted: the unlimited mixed run is further from the oracle
+    u, _ = make(spec, H, Cc, limiter=False, precision=M.PREC_MIXED)
+    u.step(10, dt)
+    Hu, _ = u.download()
+    assert np.abs(Hu - Ho).max() > 1e2 * np.abs(Hg - Ho).max()
+    t.close()
+    u.close()
+
+
+def test_mixed_limited_traces_and_evaluation():
+    """Limited traces of the mixed handle (FP32 blend, limiter factor from it)
+    within 1e-6 of the oracle's, on states with cells on the bounds."""
+    spec = W.MeshSpec(48, 40, seed=3)
+    H, Cc = fields(spec)
+    t, o = make(spec, H, Cc, precision=M.PREC_MIXED)
+    for field, u in ((0, H), (1, Cc)):
+        _, _, tr = t.reconstruct(field)
+        want = o.traces(u, field)
+        assert np.abs(tr - want).max() <= 1e-6 * np.abs(u).max(), field
+    t.close()
