@@ -308,6 +308,12 @@ mp_status mp_batch_step(mp_batch* b, int32_t nsteps, int32_t scheme, int32_t fla
 /* Lithostatic phase pressure of mesh id, as mp_transport_set_lithostatic
  * (src/couple/initialization.cpp:155-162). */
 mp_status mp_batch_set_lithostatic(mp_batch* b, int32_t id, int32_t enable, double surface_y);
+/* Bound limiter of mesh id, as mp_transport_set_limiter
+ * (src/couple/state_reconstruction.cpp:136-192), in both precisions.  Meshes
+ * with and without it share the launches; while any mesh of the batch has it
+ * on, mixed steps launch per stage. */
+mp_status mp_batch_set_limiter(mp_batch* b, int32_t id, int32_t enable, int32_t cell_points_per_axis,
+                               int32_t diffusion_samples, double extent_fraction);
 mp_status mp_batch_phase(mp_batch* b, int32_t id, double* T, double* phi, double* Cl,
                          double* Cs, uint8_t* region);
 /* Schedule: rows per work item (>= 8) and persistent grid size in blocks of
@@ -378,6 +384,12 @@ typedef struct {
      * surface_y, as mp_transport_set_lithostatic */
     int32_t lithostatic;
     double surface_y;
+    /* bound limiter as mp_transport_set_limiter: limiter = 0 off (zero-filled
+     * requests), else with the given tables (defaults 5, 4, 0.9) */
+    int32_t limiter;
+    int32_t cell_points_per_axis;
+    int32_t diffusion_samples;
+    double extent_fraction;
 } mp_request;
 
 typedef struct {

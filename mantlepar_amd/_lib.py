@@ -64,7 +64,9 @@ class Request(C.Structure):
                 ("vny", C.c_void_p), ("bc", C.c_void_p * 4), ("cfl", C.c_double),
                 ("nsteps", C.c_int32), ("H_out", C.c_void_p), ("C_out", C.c_void_p),
                 ("T", C.c_void_p), ("phi", C.c_void_p), ("Cl", C.c_void_p), ("Cs", C.c_void_p),
-                ("region", C.c_void_p), ("lithostatic", C.c_int32), ("surface_y", C.c_double)]
+                ("region", C.c_void_p), ("lithostatic", C.c_int32), ("surface_y", C.c_double),
+                ("limiter", C.c_int32), ("cell_points_per_axis", C.c_int32),
+                ("diffusion_samples", C.c_int32), ("extent_fraction", C.c_double)]
 
 
 class RequestResult(C.Structure):
@@ -179,6 +181,7 @@ SIGNATURES = [
     ("mp_transport_set_lithostatic", C.c_int32, [_vp, C.c_int32, C.c_double]),
     ("mp_coupled_set_lithostatic", C.c_int32, [_vp, C.c_int32, C.c_double]),
     ("mp_batch_set_lithostatic", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double]),
+    ("mp_batch_set_limiter", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double]),
     ("mp_batch_create", C.c_int32, [C.c_int32, C.POINTER(WenoOptions), C.POINTER(PhaseParams),
                                     C.c_int32, C.c_uint64, _vp, C.POINTER(_vp)]),
     ("mp_batch_destroy", C.c_int32, [_vp]),

@@ -99,6 +99,12 @@ class Batch:
         check(lib().mp_batch_set_lithostatic(self._h, k, int(surface_y is not None),
                                              0.0 if surface_y is None else surface_y))
 
+    def set_limiter(self, k, enable=True, cell_points_per_axis=5, diffusion_samples=4,
+                    extent_fraction=0.9):
+        """Bound limiter of mesh k (StateReconstruction::Limit); enable=False: off."""
+        check(lib().mp_batch_set_limiter(self._h, k, int(enable), cell_points_per_axis,
+                                         diffusion_samples, extent_fraction))
+
     def stable_dt(self, k, cfl):
         dt = C.c_double()
         check(lib().mp_batch_stable_dt(self._h, k, cfl, C.byref(dt)))
@@ -194,9 +200,12 @@ class Server:
         return lib().mp_server_now(self._h)
 
     def submit(self, nx, ny, H, Cd, vnx, vny, nsteps, cfl, x0=0.0, x1=0.5, y0=0.0, y1=1.0, q=5,
-               bc=None, outputs=("H", "C", "T", "phi", "Cl", "Cs", "region"), surface_y=None):
+               bc=None, outputs=("H", "C", "T", "phi", "Cl", "Cs", "region"), surface_y=None,
+               limiter=None):
         """Queue one run; returns its id.  Inputs are kept referenced until done.
-        surface_y: lithostatic phase pressure below that surface (None: constant)."""
+        surface_y: lithostatic phase pressure below that surface (None: constant).
+        limiter: None (off) or (cell_points_per_axis, diffusion_samples,
+        extent_fraction) of the bound limiter."""
         ins = [_f64(H), _f64(Cd), _f64(vnx), _f64(vny)]
         n = nx * ny
         assert ins[0].size == n and ins[1].size == n
@@ -213,6 +222,9 @@ class Server:
         r.nsteps = nsteps
         r.lithostatic = int(surface_y is not None)
         r.surface_y = 0.0 if surface_y is None else surface_y
+        if limiter is not None:
+            r.limiter = 1
+            r.cell_points_per_axis, r.diffusion_samples, r.extent_fraction = limiter
         names = {"H": "H_out", "C": "C_out", "T": "T", "phi": "phi", "Cl": "Cl", "Cs": "Cs",
                  "region": "region"}
         for k, a in outs.items():

@@ -496,6 +496,16 @@ mp_status mp_batch_set_lithostatic(mp_batch* b, int32_t id, int32_t enable, doub
     MP_GUARD({ return b->eng.set_lithostatic(id, enable, surface_y); })
 }
 
+mp_status mp_batch_set_limiter(mp_batch* b, int32_t id, int32_t enable, int32_t cell_points_per_axis,
+                              int32_t diffusion_samples, double extent_fraction)
+{
+    if (!b) return fail(MP_ERR_ARG, "null handle");
+    MP_CALL(b->check_id(id));
+    MP_GUARD({
+        return b->eng.set_limiter(id, enable, cell_points_per_axis, diffusion_samples, extent_fraction);
+    })
+}
+
 mp_status mp_batch_phase(mp_batch* b, int32_t id, double* T, double* phi, double* Cl, double* Cs,
                          uint8_t* region)
 {
@@ -546,9 +556,13 @@ mp_status mp_batch_get_launch(mp_batch* b, int32_t* per_step)
     if (!b || !per_step) return fail(MP_ERR_ARG, "null argument");
     Engine& e = b->eng;
     long long items = 0;
-    for (int k : b->live()) items += e.slots[k].nitems;
+    bool limited = false;
+    for (int k : b->live()) {
+        items += e.slots[k].nitems;
+        limited = limited || e.slots[k].md.limit != 0;
+    }
     const bool rule = b->launch_mode == MP_LAUNCH_AUTO && !b->tuned;
-    *per_step = (b->want_step_launch() && e.precision == MP_PREC_MIXED && e.blocks == 0 &&
+    *per_step = (b->want_step_launch() && e.precision == MP_PREC_MIXED && e.blocks == 0 && !limited &&
                  (!rule || items < Engine::auto_step_items())) ? 1 : 0;
     return MP_OK;
 }

@@ -422,6 +422,34 @@ mp_status Engine::set_lithostatic(int k, int enable, double surface_y)
     return MP_OK;
 }
 
+mp_status Engine::set_limiter(int k, int enable, int cell_points, int samples, double extent)
+{
+    Slot& s = slots[k];
+    if (!enable) {
+        s.md.limit = 0;
+    } else {
+        MeshDev md = s.md;
+        MP_CALL(set_limiter_tables(md, cell_points, samples, extent,
+                                   phase_params.hypereutectic ? 1.0 : phase_params.Xe));
+        s.md = md;
+    }
+    MP_HIP(hipMemcpyAsync(d_meshes.as<MeshDev>() + k, &s.md, sizeof(MeshDev), hipMemcpyHostToDevice, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    ++generation;       // the launches' limiter instance may change
+    return MP_OK;
+}
+
+int Engine::limit_mode() const
+{
+    int mode = 0;
+    for (int k : active) {
+        const int m = limit_mode32(slots[k].md);
+        if (m == 1) return 1;
+        if (m == 2) mode = 2;
+    }
+    return mode;
+}
+
 mp_status Engine::phase_out(int k, double* T, double* phi, double* Cl, double* Cs, uint8_t* region,
                             bool async)
 {
@@ -726,6 +754,7 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
         const int par = set == 1 ? 1 : 0;    // the partial set of this step
         if (overlap && s >= 2) MP_HIP(hipStreamWaitEvent(stream, side_done[par], 0));
         const bool one = one_launch_for(L.second, events != nullptr);
+        const int lim = limit_mode();
         if (one)
             MP_HIP(launch_batch_step32(d_meshes.as<MeshDev>(), base + size_t(set * 3) * capacity, capacity,
                                        d_items.as<int2>() + L.first, L.second, scheme, d_flags.as<uint32_t>(),
@@ -740,7 +769,7 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
             if (!one)
                 MP_HIP(launch_batch_stage(precision, last && !split, q_common, d_meshes.as<MeshDev>(), sps,
                                           d_items.as<int2>() + L.first, L.second, blocks,
-                                          d_queues.as<int>() + 3 * s + st, stream, weno.lf_mode != 0));
+                                          d_queues.as<int>() + 3 * s + st, stream, weno.lf_mode != 0, lim));
             if (split && overlap) {
                 MP_HIP(hipEventRecord(stage3[par], stream));
                 MP_HIP(hipStreamWaitEvent(side, stage3[par], 0));

@@ -117,6 +117,11 @@ public:
     mp_status set_boundary(int slot, const uint8_t* const side[4]);
     mp_status set_inflow(int slot, const double* H, const double* C);
     mp_status set_lithostatic(int slot, int enable, double surface_y);
+    // Bound limiter of one mesh (StateReconstruction::Limit); enable = 0: off.
+    mp_status set_limiter(int slot, int enable, int cell_points, int samples, double extent);
+    // The limiter argument of the stage launches over the active slots: 0 none
+    // limited, 2 every limited mesh on the default tables, else 1.
+    int limit_mode() const;
     mp_status phase_out(int slot, double* T, double* phi, double* Cl, double* Cs, uint8_t* region,
                         bool async = false);
     void set_rows(int slot, int rows);
@@ -195,7 +200,8 @@ public:
     }
     bool one_launch_for(int list_len, bool profiling) const
     {
-        return uses_step_launch(profiling) && (step_launch_items <= 0 || list_len < step_launch_items);
+        return uses_step_launch(profiling) && (step_launch_items <= 0 || list_len < step_launch_items) &&
+               limit_mode() == 0;
     }
     // eight items per wave slot (three waves per SIMD, four SIMDs per CU)
     static int auto_step_items();

@@ -582,7 +582,9 @@ __global__ __launch_bounds__(256) void state_sums_kernel(const double* __restric
 //
 // Persistent form (queue != nullptr): every wave takes the next work item from
 // a counter until the list is exhausted; otherwise one wave per item.
-template <int MODE, bool PHASE, int NQ>
+// LIM: meshes whose limiter is on (m.limit, wave-uniform) take the limited
+// sweep, the others the plain one, in the same launch.
+template <int MODE, bool PHASE, int NQ, bool LIM = false>
 __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
     const MeshDev* __restrict__ meshes, const StagePtrs* __restrict__ stages,
     const int2* __restrict__ items, int nitems, int* queue)
@@ -606,8 +608,10 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
             const StagePtrs& sp = stages[slot];
             int strip, j0, j1;
             item_region(sp, m.ny, local, strip, j0, j1);
-            if (j0 < m.ny)             // always true with a current work list
-                sweep_item<MODE, PHASE, NQ, false, false>(m, sp, local, strip, j0, j1);
+            if (j0 < m.ny) {           // always true with a current work list
+                if (LIM && m.limit) sweep_item<MODE, PHASE, NQ, LIM, false>(m, sp, local, strip, j0, j1);
+                else sweep_item<MODE, PHASE, NQ, false, false>(m, sp, local, strip, j0, j1);
+            }
         }
         if (!queue) break;
     }
@@ -1246,23 +1250,26 @@ hipError_t launch_state_sums(const double* H, const double* C, int64_t n, double
 
 hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes,
                               const StagePtrs* stages, const int2* items, int nitems, int blocks,
-                              int* queue, hipStream_t stream, bool global_lf)
+                              int* queue, hipStream_t stream, bool global_lf, int limit)
 {
     if (nitems <= 0) return hipSuccess;
     // A persistent grid (blocks > 0) runs 4-wave workgroups that take items from
     // the queue; otherwise one wave per workgroup and per item.
     int* qp = blocks > 0 ? queue : nullptr;
     const dim3 grid(blocks > 0 ? blocks : nitems), block(blocks > 0 ? 256 : kWave);
-    if (mode == 0 && q == 5) {
-        if (phase) batch_sweep_kernel<0, true, 5><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp);
-        else batch_sweep_kernel<0, false, 5><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp);
-    } else if (mode == 0) {
-        if (phase) batch_sweep_kernel<0, true, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp);
-        else batch_sweep_kernel<0, false, 0><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp);
-    } else {
+    if (mode != 0) {
         if (phase) return hipErrorInvalidValue;   // mixed: batch_phase_items_kernel follows
-        return launch_batch_stage32(meshes, stages, items, nitems, blocks, queue, global_lf, stream);
+        return launch_batch_stage32(meshes, stages, items, nitems, blocks, queue, global_lf, stream, limit);
     }
+#define MP_BATCH64(P, N, L) batch_sweep_kernel<0, P, N, L><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp)
+    if (q == 5) {
+        if (limit) { if (phase) MP_BATCH64(true, 5, true); else MP_BATCH64(false, 5, true); }
+        else { if (phase) MP_BATCH64(true, 5, false); else MP_BATCH64(false, 5, false); }
+    } else {
+        if (limit) { if (phase) MP_BATCH64(true, 0, true); else MP_BATCH64(false, 0, true); }
+        else { if (phase) MP_BATCH64(true, 0, false); else MP_BATCH64(false, 0, false); }
+    }
+#undef MP_BATCH64
     return hipGetLastError();
 }
 

@@ -42,9 +42,12 @@ hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H
                               uint32_t* error, hipStream_t stream);
 // blocks > 0: persistent grid of that many blocks taking items from *queue
 // (an int that must be zero at launch); blocks == 0: one wave per item.
+// limit: 0 when no mesh of the list has its limiter on; else the limiter
+// instance, in which each mesh takes the limited sweep only if its limiter is
+// on (mixed: with the table instance of its single-mesh launch).
 hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes,
                               const StagePtrs* stages, const int2* items, int nitems, int blocks,
-                              int* queue, hipStream_t stream, bool global_lf = false);
+                              int* queue, hipStream_t stream, bool global_lf = false, int limit = 0);
 // prev (may be null): per-mesh totals before the step; with it each mesh's
 // heat balance is checked at tolerance tol (kErrBalance in its error word).
 hipError_t launch_batch_reduce(const StagePtrs* stages, const int* nitems, const MeshDev* meshes,
@@ -71,7 +74,7 @@ int batch_blocks_per_cu32();
 hipError_t launch_stage32(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
                           bool diffusion, bool global_lf, int limit = 0);
 // The limit argument of launch_stage32 for a mesh record.
-inline int limit_mode32(const MeshDev& m)
+__host__ __device__ inline int limit_mode32(const MeshDev& m)
 {
     if (!m.limit) return 0;
     return (m.q == 5 && m.lim_ncp == 5 && m.lim_ni == 2 && m.lim_nb == 4) ? 2 : 1;
@@ -87,7 +90,8 @@ hipError_t launch_step32(const MeshDev* d_mesh, const StagePtrs* tab, int nitems
                          uint32_t* flags, uint32_t epoch, uint32_t wait_epoch, int spin_limit,
                          hipStream_t stream, bool diffusion, bool global_lf);
 hipError_t launch_batch_stage32(const MeshDev* meshes, const StagePtrs* stages, const int2* items,
-                                int nitems, int blocks, int* queue, bool global_lf, hipStream_t stream);
+                                int nitems, int blocks, int* queue, bool global_lf, hipStream_t stream,
+                                int limit = 0);
 // All stages of one mixed-precision step over a batch work list (one wave per
 // entry, (-1, 0) pads skipped) in one launch.  Stage st of slot k reads
 // tab[st * capacity + k]; completion flags are flags[st * nflags +

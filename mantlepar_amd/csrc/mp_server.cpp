@@ -372,6 +372,8 @@ void mp_server::admit(std::vector<Request*>& batch, int rows_now)
         if (st == MP_OK) st = eng.upload_async(k, h, h + N, true);
         if (st == MP_OK) st = eng.set_velocity(k, h + 2 * N, h + 2 * N + nxv, false, true);
         if (st == MP_OK && q->r.lithostatic) st = eng.set_lithostatic(k, 1, q->r.surface_y);
+        if (st == MP_OK && q->r.limiter)
+            st = eng.set_limiter(k, 1, q->r.cell_points_per_axis, q->r.diffusion_samples, q->r.extent_fraction);
         if (st != MP_OK) {
             finish(q, st, last_error());
             continue;

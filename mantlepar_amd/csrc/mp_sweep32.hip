@@ -415,7 +415,9 @@ __device__ __forceinline__ void face_flux(const FaceMom& f, double D, double tLH
 // other launches, measured slower: 0.657 against 0.624 ms per 2048 x 4096
 // step launch.)
 // LIM: 0 no limiter, 1 the limiter with the mesh record's tables, 2 with the
-// default tables fixed at compile time (LimDefault).
+// default tables fixed at compile time (LimDefault), 3 either, chosen per mesh
+// as the single-mesh launch chooses (the batch's mixed lists), so that every
+// mesh runs the arithmetic of its own single-mesh sweep.
 template <bool DIFF, bool FULL, bool SHARED = false, int LIM = 0>
 __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, int strip, int j0,
                                              int j1)
@@ -504,20 +506,26 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
         }
         FaceLines<float> LH, LC;
         if constexpr (LIM != 0) {
-            using S = std::conditional_t<LIM == 2, LimDefault, LimAny>;
             if (have) {
                 // one field at a time: its blend, limit factor and face lines
                 const unsigned sides = (r == 0 ? 1u : 0u) | (i == nx - 1 ? 2u : 0u) |
                                        (r == ny - 1 ? 4u : 0u) | (i == 0 ? 8u : 0u);
                 uint32_t e2 = 0;
-                if (inner_strip && r >= 1 && r <= ny - 2) {
-                    recon32_limited<false, S>(m, k, H0, H1, H2, 31u, sides, INFINITY, LH, e2);
-                    recon32_limited<false, S>(m, k, C0, C1, C2, 31u, sides, m.lim_cupper, LC, e2);
-                } else {
-                    const unsigned mask = col_ok ? candidate_mask(i, r, nx, ny) : 2u;
-                    recon32_limited<true, S>(m, k, H0, H1, H2, mask, sides, INFINITY, LH, e2);
-                    recon32_limited<true, S>(m, k, C0, C1, C2, mask, sides, m.lim_cupper, LC, e2);
-                }
+                auto limited = [&](auto shape) {
+                    using S = decltype(shape);
+                    if (inner_strip && r >= 1 && r <= ny - 2) {
+                        recon32_limited<false, S>(m, k, H0, H1, H2, 31u, sides, INFINITY, LH, e2);
+                        recon32_limited<false, S>(m, k, C0, C1, C2, 31u, sides, m.lim_cupper, LC, e2);
+                    } else {
+                        const unsigned mask = col_ok ? candidate_mask(i, r, nx, ny) : 2u;
+                        recon32_limited<true, S>(m, k, H0, H1, H2, mask, sides, INFINITY, LH, e2);
+                        recon32_limited<true, S>(m, k, C0, C1, C2, mask, sides, m.lim_cupper, LC, e2);
+                    }
+                };
+                if constexpr (LIM == 1) limited(LimAny{});
+                else if constexpr (LIM == 2) limited(LimDefault{});
+                else if (limit_mode32(m) == 2) limited(LimDefault{});
+                else limited(LimAny{});
                 if (col_ok) err |= e2;
             } else {
 #pragma unroll
@@ -700,8 +708,11 @@ __global__ __launch_bounds__(64, 3) void step32_kernel(const MeshDev* __restrict
 // from a counter until the list is exhausted; otherwise one wave per item
 // (and per workgroup).
 // Batch meshes carry no thermal diffusion and share one LF mode (FULL: global).
-template <bool FULL>
-__global__ __launch_bounds__(256, 3) void batch32_kernel(const MeshDev* __restrict__ meshes,
+// LIM != 0: meshes whose limiter is on take the limited sweep (with the
+// table instance that their single-mesh launch takes), the others the plain
+// one, at the limited instance's occupancy.
+template <bool FULL, int LIM = 0>
+__global__ __launch_bounds__(256, LIM != 0 ? 2 : 3) void batch32_kernel(const MeshDev* __restrict__ meshes,
                                                          const StagePtrs* __restrict__ stages,
                                                          const int2* __restrict__ items, int nitems,
                                                          int* queue)
@@ -723,8 +734,10 @@ __global__ __launch_bounds__(256, 3) void batch32_kernel(const MeshDev* __restri
             const StagePtrs& sp = stages[slot];
             int strip, j0, j1;
             item_region(sp, m.ny, local, strip, j0, j1);
-            if (j0 < m.ny)             // always true with a current work list
-                sweep32_item<false, FULL>(m, sp, local, strip, j0, j1);
+            if (j0 < m.ny) {           // always true with a current work list
+                if (LIM != 0 && m.limit) sweep32_item<false, FULL, false, 3>(m, sp, local, strip, j0, j1);
+                else sweep32_item<false, FULL>(m, sp, local, strip, j0, j1);
+            }
         }
         if (!queue) break;
     }
@@ -838,13 +851,20 @@ int batch_blocks_per_cu32()
 }
 
 hipError_t launch_batch_stage32(const MeshDev* meshes, const StagePtrs* stages, const int2* items,
-                                int nitems, int blocks, int* queue, bool global_lf, hipStream_t stream)
+                                int nitems, int blocks, int* queue, bool global_lf, hipStream_t stream,
+                                int limit)
 {
     if (nitems <= 0) return hipSuccess;
     int* q = blocks > 0 ? queue : nullptr;
     const dim3 grid(blocks > 0 ? blocks : nitems), block(blocks > 0 ? 256 : kWave);
-    if (global_lf) batch32_kernel<true><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q);
-    else batch32_kernel<false><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q);
+#define MP_BATCH32(F, L) batch32_kernel<F, L><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q)
+    // a limited mesh takes the instance its single-mesh launch takes
+    if (global_lf) {
+        if (limit) MP_BATCH32(true, 1); else MP_BATCH32(true, 0);
+    } else {
+        if (limit) MP_BATCH32(false, 1); else MP_BATCH32(false, 0);
+    }
+#undef MP_BATCH32
     return hipGetLastError();
 }
 

@@ -213,7 +213,7 @@ class Transport:
     def set_limiter(self, enable=True, cell_points_per_axis=5, diffusion_samples=4,
                     extent_fraction=0.9):
         """StateReconstruction::Limit (src/couple/state_reconstruction.cpp:136-192) on
-        every reconstruction of the stage kernels (FP64 precision mode)."""
+        every reconstruction of the stage kernels (either precision)."""
         check(lib().mp_transport_set_limiter(self._h, int(enable), cell_points_per_axis,
                                              diffusion_samples, extent_fraction))
 

@@ -277,6 +277,74 @@ def test_failed_heat_balance_rejects_the_mesh(prec):
     b.close()
 
 
+LIM_SPECS = [W.MeshSpec(48, 40, seed=3), W.MeshSpec(130, 97, seed=13), W.MeshSpec(20, 20, seed=11)]
+
+
+def limiter_fields(spec, seed=7):
+    """A step in H the reconstruction overshoots and grid-scale noise in C
+    inside (0, Xe): the limiter acts on many cells."""
+    rng = np.random.default_rng(seed)
+    H = W.enthalpy(spec)
+    H[:, spec.nx // 3:] += 0.3
+    return H, 0.05 + 0.3 * rng.random((spec.ny, spec.nx))
+
+
+@pytest.mark.parametrize("prec", [M.PREC_FP64, M.PREC_MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("tables", ["default", "generic"])
+def test_limited_meshes_match_single_mesh_bitwise(prec, tables):
+    """Meshes with the bound limiter on share launches with meshes without it;
+    each reproduces a single-mesh handle with the same limiter setting bit for
+    bit (states, phase outputs, totals).  "generic" adds a q = 3 mesh with other
+    limiter tables, which puts the launches on the limiter's generic instance."""
+    ph = W.config_phase(hypereutectic=0)
+    specs = list(LIM_SPECS)
+    lim = {0: (5, 4, 0.9), 1: (5, 4, 0.9)}
+    if tables == "generic":
+        specs.append(W.MeshSpec(40, 30, seed=5, q=3))
+        lim = {0: (5, 4, 0.9), 3: (3, 6, 0.75)}
+    b = M.Batch(8, phase=ph, precision=prec)
+    ids = []
+    for i, spec in enumerate(specs):
+        H, Cc = limiter_fields(spec)
+        k = b.add_mesh(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q)
+        b.upload(k, H, Cc)
+        b.set_face_velocity(k, *W.face_velocity(spec))
+        if i in lim:
+            b.set_limiter(k, True, *lim[i])
+        b.set_dt(k, b.stable_dt(k, 0.4))
+        ids.append(k)
+    b.set_schedule(12, 0)
+    assert not b.per_step_launch()            # limited meshes launch per stage
+    reps = b.step(4)
+    limited0 = b.download(ids[0])[0]
+    for i, (k, spec) in enumerate(zip(ids, specs)):
+        H, Cc = limiter_fields(spec)
+        t = M.Transport(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q, phase=ph,
+                        precision=prec)
+        t.set_tile(12)
+        t.upload(H, Cc)
+        t.set_face_velocity(*W.face_velocity(spec))
+        if i in lim:
+            t.set_limiter(True, *lim[i])
+        rep = t.step(4, t.stable_dt(0.4))
+        Hs, Cs = t.download()
+        Hb, Cb = b.download(k)
+        assert np.array_equal(Hb, Hs) and np.array_equal(Cb, Cs), (i, tables)
+        ps, pb = t.phase_state(), b.phase_state(k)
+        for n in ("T", "phi", "Cl", "Cs", "region"):
+            assert np.array_equal(pb[n], ps[n]), n
+        for n in ("total_H", "total_C", "boundary_H", "boundary_C"):
+            assert reps[k][n] == rep[n], (i, n)
+        t.close()
+    # the limiter acted on mesh 0: without it the same steps end elsewhere
+    for k, spec in zip(ids, specs):
+        b.upload(k, *limiter_fields(spec))
+    b.set_limiter(ids[0], False)
+    b.step(4)
+    assert np.abs(b.download(ids[0])[0] - limited0).max() > 1e-6
+    b.close()
+
+
 def test_launch_mode_choices():
     b = M.Batch(4, phase=W.config_phase(), precision=M.PREC_MIXED)
     fill_batch(b, SPECS[:2])

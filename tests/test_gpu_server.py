@@ -34,7 +34,8 @@ def make_requests(count, seed=21, max_steps=24):
 def submit(srv, r, cfl=0.4):
     s = r["spec"]
     return srv.submit(s.nx, s.ny, r["H"], r["C"], r["vx"], r["vy"], r["nsteps"], cfl,
-                      s.x0, s.x1, s.y0, s.y1, q=s.q, bc=r.get("bc"), surface_y=r.get("surface"))
+                      s.x0, s.x1, s.y0, s.y1, q=s.q, bc=r.get("bc"), surface_y=r.get("surface"),
+                      limiter=r.get("limiter"))
 
 
 OUT = ("H", "C", "T", "phi", "Cl", "Cs", "region")
@@ -240,3 +241,43 @@ def test_progress_curve_accounts_every_step():
     assert len(t) == st["epochs"]
     assert np.all(np.diff(t) >= 0) and np.all(np.diff(w) > 0)
     assert w[-1] == st["cell_steps"] == sum(r["spec"].cells * r["nsteps"] for r in reqs)
+
+
+@pytest.mark.parametrize("prec", [M.PREC_FP64, M.PREC_MIXED], ids=["fp64", "mixed"])
+def test_limited_requests_equal_single_mesh(prec):
+    """Requests with the bound limiter (default and other tables) among requests
+    without it: each result is bitwise a single-mesh handle's with the same
+    limiter setting, alone and in the mix."""
+    ph = W.config_phase(hypereutectic=0)
+    reqs = make_requests(6, seed=81, max_steps=9)
+    rng = np.random.default_rng(3)
+    for k, r in enumerate(reqs):
+        spec = r["spec"]
+        r["H"] = r["H"].copy()
+        r["H"][:, spec.nx // 3:] += 0.3
+        r["C"] = 0.05 + 0.3 * rng.random((spec.ny, spec.nx))
+        if k % 3 == 0:
+            r["limiter"] = (5, 4, 0.9)
+        elif k % 3 == 1 and spec.q == 5:
+            r["limiter"] = (3, 6, 0.75)
+    srv = M.Server(max_slots=8, phase=ph, precision=prec, max_epoch_steps=4)
+    ids = [submit(srv, r) for r in reqs]
+    srv.drain()
+    outs = [srv.wait(i) for i in ids]
+    srv.close()
+    for r, (res, out) in zip(reqs, outs):
+        s = r["spec"]
+        t = M.Transport(s.nx, s.ny, s.x0, s.x1, s.y0, s.y1, q=s.q, phase=ph, precision=prec)
+        t.set_tile(12)
+        t.upload(r["H"], r["C"])
+        t.set_face_velocity(r["vx"], r["vy"])
+        if r.get("limiter"):
+            t.set_limiter(True, *r["limiter"])
+        rep = t.step(r["nsteps"], t.stable_dt(0.4))
+        H, Cc = t.download()
+        assert np.array_equal(H, out["H"]) and np.array_equal(Cc, out["C"]), r.get("limiter")
+        ph_s = t.phase_state()
+        for n in ("T", "phi", "region"):
+            assert np.array_equal(ph_s[n], out[n]), n
+        assert rep["total_H"] == res["total_H"] and rep["boundary_C"] == res["boundary_C"]
+        t.close()
