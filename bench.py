@@ -30,7 +30,9 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=100)
-    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=100,
+                   help="untimed steps right before the timed ones (the device's first "
+                        "steps after idling run slower)")
     p.add_argument("--nx", type=int, default=2048)
     p.add_argument("--ny", type=int, default=4096)
     p.add_argument("--rows", type=int, default=0, help="rows per work item (0: autotune)")
@@ -667,12 +669,15 @@ def main():
                     best = (ms, rows)
             t.set_tile(best[1])
             rows_used = best[1]
+        # W untimed steps from the initial state, then exactly K timed steps
+        # straight after them: an upload between the two left the device idle
+        # for the PCIe copy, and its first timed steps ran ~12% slower while it
+        # came back to speed.  Both precisions run the same W + K steps.
         t.upload(H, Cc)
         if args.warmup:
             t.step(args.warmup, dt)
-        t.upload(H, Cc)
-        # exactly K steps, timed as up to five windows with a synchronisation
-        # between them; the windows give the spread of the per-step time
+        # timed as up to five windows with a synchronisation between them; the
+        # windows give the spread of the per-step time
         t.synchronize()
         nwin = min(5, args.steps)
         sizes = [args.steps // nwin + (1 if k < args.steps % nwin else 0) for k in range(nwin)]
@@ -690,6 +695,7 @@ def main():
         res[name] = {"seconds": el, "report": rep, "H": Hf, "C": Cf, "T": ph["T"],
                      "phi": ph["phi"], "region": ph["region"], "profile": prof,
                      "rows": rows_used, "windows_ms": sorted(w * 1e3 for w in windows),
+                     "windows_ms_in_order": [w * 1e3 for w in windows],
                      "layout": t.layout()}
         t.close()
         log(f"single {name}: {spec.cells * args.steps / el:.3e} cell-steps/s (rows {rows_used})")
@@ -720,7 +726,8 @@ def main():
         "ms_per_step": mixed["seconds"] / K * 1e3,
         "ms_per_step_windows": {"min": mixed["windows_ms"][0],
                                 "median": mixed["windows_ms"][len(mixed["windows_ms"]) // 2],
-                                "max": mixed["windows_ms"][-1], "windows": len(mixed["windows_ms"])},
+                                "max": mixed["windows_ms"][-1], "windows": len(mixed["windows_ms"]),
+                                "in_order": mixed["windows_ms_in_order"]},
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -734,7 +741,7 @@ def main():
         "fp64": {"value": fp64_value, "ms_per_step": fp64["seconds"] / K * 1e3,
                  "mixed_over_fp64": value / fp64_value},
         "error_vs_fp64": {"H": errH, "C": errC, "T": errT, "phi_abs": errphi,
-                          "region_cells_differing": region_diff, "steps": K},
+                          "region_cells_differing": region_diff, "steps": args.warmup + K},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": kernel,

@@ -170,8 +170,11 @@ mp_status mp_transport::ensure_scratch(int nsteps)
     // block results
     const size_t pbytes = (size_t(6) * nitems * kPartials + size_t(kStepSlices) * 6) * sizeof(double);
     if (partials.bytes < pbytes) MP_CALL(partials.alloc(pbytes));
+    // per-step totals: at least 256 steps, grown geometrically, so that calls
+    // of varying length do not reallocate (a reallocation synchronises)
     const size_t tbytes = size_t(nsteps + 1) * 4 * sizeof(double);
-    if (totals.bytes < tbytes) MP_CALL(totals.alloc(tbytes));
+    if (totals.bytes < tbytes)
+        MP_CALL(totals.alloc(std::max({tbytes, 2 * totals.bytes, size_t(257) * 4 * sizeof(double)})));
     const size_t fbytes = size_t(3) * nitems * sizeof(uint32_t);
     if (flags.bytes < fbytes) {
         // a new flag array holds no epoch yet
