@@ -314,6 +314,15 @@ mp_status mp_batch_set_lithostatic(mp_batch* b, int32_t id, int32_t enable, doub
  * on, mixed steps launch per stage. */
 mp_status mp_batch_set_limiter(mp_batch* b, int32_t id, int32_t enable, int32_t cell_points_per_axis,
                                int32_t diffusion_samples, double extent_fraction);
+/* Thermal diffusion of H and thermal boundary data of mesh id, as
+ * mp_transport_set_diffusion / mp_transport_set_thermal_boundary (same array
+ * layouts, checks and messages); both refresh the mesh's stable step.  Meshes
+ * with and without them share the launches; while any mesh has them, mixed
+ * steps launch per stage. */
+mp_status mp_batch_set_diffusion(mp_batch* b, int32_t id, double kappa, int32_t diffusion_samples,
+                                 double extent_fraction);
+mp_status mp_batch_set_thermal_boundary(mp_batch* b, int32_t id, const uint8_t* types,
+                                        const double* values);
 mp_status mp_batch_phase(mp_batch* b, int32_t id, double* T, double* phi, double* Cl,
                          double* Cs, uint8_t* region);
 /* Schedule: rows per work item (>= 8) and persistent grid size in blocks of
@@ -390,6 +399,12 @@ typedef struct {
     int32_t cell_points_per_axis;
     int32_t diffusion_samples;
     double extent_fraction;
+    /* thermal diffusion of H as mp_transport_set_diffusion (0: off), with the
+     * sampling above, and thermal boundary data as
+     * mp_transport_set_thermal_boundary (NULL: zero flux) */
+    double thermal_diffusivity;
+    const uint8_t* thermal_types;
+    const double* thermal_values;
 } mp_request;
 
 typedef struct {

@@ -66,7 +66,9 @@ class Request(C.Structure):
                 ("T", C.c_void_p), ("phi", C.c_void_p), ("Cl", C.c_void_p), ("Cs", C.c_void_p),
                 ("region", C.c_void_p), ("lithostatic", C.c_int32), ("surface_y", C.c_double),
                 ("limiter", C.c_int32), ("cell_points_per_axis", C.c_int32),
-                ("diffusion_samples", C.c_int32), ("extent_fraction", C.c_double)]
+                ("diffusion_samples", C.c_int32), ("extent_fraction", C.c_double),
+                ("thermal_diffusivity", C.c_double), ("thermal_types", C.c_void_p),
+                ("thermal_values", C.c_void_p)]
 
 
 class RequestResult(C.Structure):
@@ -182,6 +184,8 @@ SIGNATURES = [
     ("mp_coupled_set_lithostatic", C.c_int32, [_vp, C.c_int32, C.c_double]),
     ("mp_batch_set_lithostatic", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_double]),
     ("mp_batch_set_limiter", C.c_int32, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double]),
+    ("mp_batch_set_diffusion", C.c_int32, [_vp, C.c_int32, C.c_double, C.c_int32, C.c_double]),
+    ("mp_batch_set_thermal_boundary", C.c_int32, [_vp, C.c_int32, _vp, _vp]),
     ("mp_batch_create", C.c_int32, [C.c_int32, C.POINTER(WenoOptions), C.POINTER(PhaseParams),
                                     C.c_int32, C.c_uint64, _vp, C.POINTER(_vp)]),
     ("mp_batch_destroy", C.c_int32, [_vp]),

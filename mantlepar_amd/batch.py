@@ -105,6 +105,16 @@ class Batch:
         check(lib().mp_batch_set_limiter(self._h, k, int(enable), cell_points_per_axis,
                                          diffusion_samples, extent_fraction))
 
+    def set_diffusion(self, k, kappa, diffusion_samples=4, extent_fraction=0.9):
+        """Legacy preheat thermal diffusion of H of mesh k (src/couple/preheat.cpp:176-190)."""
+        check(lib().mp_batch_set_diffusion(self._h, k, kappa, diffusion_samples, extent_fraction))
+
+    def set_thermal_boundary(self, k, types=None, values=None):
+        """As Transport.set_thermal_boundary, for mesh k."""
+        t = None if types is None else np.ascontiguousarray(types, dtype=np.uint8).ravel()
+        v = None if values is None else _f64(values).ravel()
+        check(lib().mp_batch_set_thermal_boundary(self._h, k, u8ptr(t), dptr(v)))
+
     def stable_dt(self, k, cfl):
         dt = C.c_double()
         check(lib().mp_batch_stable_dt(self._h, k, cfl, C.byref(dt)))
@@ -201,11 +211,12 @@ class Server:
 
     def submit(self, nx, ny, H, Cd, vnx, vny, nsteps, cfl, x0=0.0, x1=0.5, y0=0.0, y1=1.0, q=5,
                bc=None, outputs=("H", "C", "T", "phi", "Cl", "Cs", "region"), surface_y=None,
-               limiter=None):
+               limiter=None, diffusion=None):
         """Queue one run; returns its id.  Inputs are kept referenced until done.
         surface_y: lithostatic phase pressure below that surface (None: constant).
         limiter: None (off) or (cell_points_per_axis, diffusion_samples,
-        extent_fraction) of the bound limiter."""
+        extent_fraction) of the bound limiter.  diffusion: None (off) or
+        (kappa, thermal types or None, thermal values or None)."""
         ins = [_f64(H), _f64(Cd), _f64(vnx), _f64(vny)]
         n = nx * ny
         assert ins[0].size == n and ins[1].size == n
@@ -225,13 +236,22 @@ class Server:
         if limiter is not None:
             r.limiter = 1
             r.cell_points_per_axis, r.diffusion_samples, r.extent_fraction = limiter
+        thermal = []
+        if diffusion is not None:
+            kappa, types, values = diffusion
+            r.thermal_diffusivity = kappa
+            if types is not None:
+                thermal = [np.ascontiguousarray(types, dtype=np.uint8).ravel(),
+                           None if values is None else _f64(values).ravel()]
+                r.thermal_types = thermal[0].ctypes.data
+                r.thermal_values = None if thermal[1] is None else thermal[1].ctypes.data
         names = {"H": "H_out", "C": "C_out", "T": "T", "phi": "phi", "Cl": "Cl", "Cs": "Cs",
                  "region": "region"}
         for k, a in outs.items():
             setattr(r, names[k], a.ctypes.data)
         rid = C.c_int64()
         check(lib().mp_server_submit(self._h, C.byref(r), C.byref(rid)))
-        self._live[rid.value] = (ins, outs, bcs, (ny, nx))
+        self._live[rid.value] = (ins + thermal, outs, bcs, (ny, nx))
         return rid.value
 
     def wait(self, rid, timeout=-1.0):

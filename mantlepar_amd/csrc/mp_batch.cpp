@@ -506,6 +506,21 @@ mp_status mp_batch_set_limiter(mp_batch* b, int32_t id, int32_t enable, int32_t 
     })
 }
 
+mp_status mp_batch_set_diffusion(mp_batch* b, int32_t id, double kappa, int32_t diffusion_samples,
+                                double extent_fraction)
+{
+    if (!b) return fail(MP_ERR_ARG, "null handle");
+    MP_CALL(b->check_id(id));
+    MP_GUARD({ return b->eng.set_diffusion(id, kappa, diffusion_samples, extent_fraction); })
+}
+
+mp_status mp_batch_set_thermal_boundary(mp_batch* b, int32_t id, const uint8_t* types, const double* values)
+{
+    if (!b) return fail(MP_ERR_ARG, "null handle");
+    MP_CALL(b->check_id(id));
+    MP_GUARD({ return b->eng.set_thermal_boundary(id, types, values); })
+}
+
 mp_status mp_batch_phase(mp_batch* b, int32_t id, double* T, double* phi, double* Cl, double* Cs,
                          uint8_t* region)
 {

@@ -214,6 +214,8 @@ size_t Engine::slot_bytes(const mp_mesh_desc& d, int32_t precision, int rows_min
     b += Arena::round(size_t(2) * (d.nx + d.ny));
     b += Arena::round(size_t(4) * (d.nx + d.ny) * d.q * sizeof(double));
     b += Arena::round(size_t(d.ny) * sizeof(TepPair));
+    b += Arena::round(size_t(2) * (d.nx + d.ny) * d.q);                    // thermal policies
+    b += Arena::round(size_t(2) * (d.nx + d.ny) * d.q * sizeof(double));   // thermal data
     return b;
 }
 
@@ -250,6 +252,8 @@ mp_status Engine::add_mesh(const mp_mesh_desc& d, int* out)
     sl.bc = reinterpret_cast<uint8_t*>(take(size_t(2) * (d.nx + d.ny)));
     sl.inflow = reinterpret_cast<double*>(take(size_t(4) * (d.nx + d.ny) * d.q * sizeof(double)));
     sl.tep = reinterpret_cast<TepPair*>(take(size_t(d.ny) * sizeof(TepPair)));
+    sl.tbc = reinterpret_cast<uint8_t*>(take(size_t(2) * (d.nx + d.ny) * d.q));
+    sl.tval = reinterpret_cast<double*>(take(size_t(2) * (d.nx + d.ny) * d.q * sizeof(double)));
     sl.live = true;
     slots[k] = sl;
     set_rows(k, 64);
@@ -437,6 +441,73 @@ mp_status Engine::set_limiter(int k, int enable, int cell_points, int samples, d
     MP_HIP(hipStreamSynchronize(stream));
     ++generation;       // the launches' limiter instance may change
     return MP_OK;
+}
+
+mp_status Engine::refresh_rate(int k)
+{
+    Slot& s = slots[k];
+    if (!s.have_velocity) return MP_OK;
+    unsigned long long* bits = d_maxbits.as<unsigned long long>() + k;
+    MP_HIP(launch_clear(reinterpret_cast<uint32_t*>(bits), 2, stream));
+    MP_HIP(launch_rate(d_meshes.as<MeshDev>() + k, int(s.N), bits, stream));
+    unsigned long long host = 0;
+    MP_HIP(hipMemcpyAsync(&host, bits, sizeof host, hipMemcpyDeviceToHost, stream));
+    MP_HIP(hipStreamSynchronize(stream));
+    std::memcpy(&s.max_rate, &host, sizeof s.max_rate);
+    return MP_OK;
+}
+
+mp_status Engine::set_diffusion(int k, double kappa, int samples, double extent)
+{
+    Slot& s = slots[k];
+    MeshDev md = s.md;
+    MP_CALL(set_diffusion_tables(md, kappa, samples, extent));
+    s.md = md;
+    MP_HIP(hipMemcpyAsync(d_meshes.as<MeshDev>() + k, &s.md, sizeof(MeshDev), hipMemcpyHostToDevice, stream));
+    ++generation;
+    return refresh_rate(k);
+}
+
+// Same checks and messages as mp_transport_set_thermal_boundary.
+mp_status Engine::set_thermal_boundary(int k, const uint8_t* types, const double* values)
+{
+    Slot& s = slots[k];
+    const mp_mesh_desc& d = s.desc;
+    const size_t total = size_t(2) * (d.nx + d.ny) * d.q;
+    const size_t off[4] = {0, size_t(d.nx) * d.q, size_t(d.nx + d.ny) * d.q, size_t(2 * d.nx + d.ny) * d.q};
+    if (!types) {
+        for (int f = 0; f < 4; ++f) { s.md.tbc[f] = nullptr; s.md.tval[f] = nullptr; }
+    } else {
+        for (size_t i = 0; i < total; ++i)
+            if (types[i] > 2) return fail(MP_ERR_ARG, "Unknown thermal boundary policy");
+        std::vector<double> v(total, 0.0);
+        if (values) {
+            for (size_t i = 0; i < total; ++i)
+                if (types[i] != 0 && !std::isfinite(values[i]))
+                    return fail(MP_ERR_ARG, "Thermal boundary data must be finite");
+            std::memcpy(v.data(), values, total * sizeof(double));
+        } else {
+            for (size_t i = 0; i < total; ++i)
+                if (types[i] != 0) return fail(MP_ERR_ARG, "Missing thermal boundary profile.");
+        }
+        MP_HIP(hipMemcpyAsync(s.tbc, types, total, hipMemcpyHostToDevice, stream));
+        MP_HIP(hipMemcpyAsync(s.tval, v.data(), total * sizeof(double), hipMemcpyHostToDevice, stream));
+        MP_HIP(hipStreamSynchronize(stream));     // v is a local
+        for (int f = 0; f < 4; ++f) {
+            s.md.tbc[f] = s.tbc + off[f];
+            s.md.tval[f] = s.tval + off[f];
+        }
+    }
+    MP_HIP(hipMemcpyAsync(d_meshes.as<MeshDev>() + k, &s.md, sizeof(MeshDev), hipMemcpyHostToDevice, stream));
+    ++generation;
+    return refresh_rate(k);
+}
+
+bool Engine::any_diffusion() const
+{
+    for (int k : active)
+        if (has_diffusion(slots[k].md)) return true;
+    return false;
 }
 
 int Engine::limit_mode() const
@@ -755,6 +826,7 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
         if (overlap && s >= 2) MP_HIP(hipStreamWaitEvent(stream, side_done[par], 0));
         const bool one = one_launch_for(L.second, events != nullptr);
         const int lim = limit_mode();
+        const bool dif = any_diffusion();
         if (one)
             MP_HIP(launch_batch_step32(d_meshes.as<MeshDev>(), base + size_t(set * 3) * capacity, capacity,
                                        d_items.as<int2>() + L.first, L.second, scheme, d_flags.as<uint32_t>(),
@@ -769,7 +841,8 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
             if (!one)
                 MP_HIP(launch_batch_stage(precision, last && !split, q_common, d_meshes.as<MeshDev>(), sps,
                                           d_items.as<int2>() + L.first, L.second, blocks,
-                                          d_queues.as<int>() + 3 * s + st, stream, weno.lf_mode != 0, lim));
+                                          d_queues.as<int>() + 3 * s + st, stream, weno.lf_mode != 0, lim,
+                                          dif));
             if (split && overlap) {
                 MP_HIP(hipEventRecord(stage3[par], stream));
                 MP_HIP(hipStreamWaitEvent(side, stage3[par], 0));

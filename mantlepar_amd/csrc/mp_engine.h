@@ -86,6 +86,8 @@ struct Slot {
     uint8_t* bc = nullptr;     // bottom(nx) right(ny) top(nx) left(ny) policies
     double* inflow = nullptr;  // H then C, each [face][q] in side order
     TepPair* tep = nullptr;    // lithostatic {Tep, Tmp} per row (md.tep_row when enabled)
+    uint8_t* tbc = nullptr;    // thermal boundary policies [face][q] in side order
+    double* tval = nullptr;    // thermal boundary data [face][q]
     std::vector<TepPair> tep_host;   // its host source, kept until the slot is reused
     int cur = 0, phase_cur = 0;
     bool phase_valid = false, have_velocity = false;
@@ -119,6 +121,13 @@ public:
     mp_status set_lithostatic(int slot, int enable, double surface_y);
     // Bound limiter of one mesh (StateReconstruction::Limit); enable = 0: off.
     mp_status set_limiter(int slot, int enable, int cell_points, int samples, double extent);
+    // Thermal diffusion of H and thermal boundary data of one mesh, as the
+    // single-mesh handle's; both refresh the mesh's CFL rate (diffusion terms).
+    mp_status set_diffusion(int slot, double kappa, int samples, double extent);
+    mp_status set_thermal_boundary(int slot, const uint8_t* types, const double* values);
+    // Whether any active slot carries diffusion (the launches' diffusion instance).
+    bool any_diffusion() const;
+    mp_status refresh_rate(int slot);
     // The limiter argument of the stage launches over the active slots: 0 none
     // limited, 2 every limited mesh on the default tables, else 1.
     int limit_mode() const;
@@ -201,7 +210,7 @@ public:
     bool one_launch_for(int list_len, bool profiling) const
     {
         return uses_step_launch(profiling) && (step_launch_items <= 0 || list_len < step_launch_items) &&
-               limit_mode() == 0;
+               limit_mode() == 0 && !any_diffusion();
     }
     // eight items per wave slot (three waves per SIMD, four SIMDs per CU)
     static int auto_step_items();

@@ -583,8 +583,9 @@ __global__ __launch_bounds__(256) void state_sums_kernel(const double* __restric
 // Persistent form (queue != nullptr): every wave takes the next work item from
 // a counter until the list is exhausted; otherwise one wave per item.
 // LIM: meshes whose limiter is on (m.limit, wave-uniform) take the limited
-// sweep, the others the plain one, in the same launch.
-template <int MODE, bool PHASE, int NQ, bool LIM = false>
+// sweep, the others the plain one, in the same launch; DIFF likewise for
+// meshes with diffusion or thermal boundary data (has_diffusion).
+template <int MODE, bool PHASE, int NQ, bool LIM = false, bool DIFF = false>
 __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
     const MeshDev* __restrict__ meshes, const StagePtrs* __restrict__ stages,
     const int2* __restrict__ items, int nitems, int* queue)
@@ -609,7 +610,12 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
             int strip, j0, j1;
             item_region(sp, m.ny, local, strip, j0, j1);
             if (j0 < m.ny) {           // always true with a current work list
-                if (LIM && m.limit) sweep_item<MODE, PHASE, NQ, LIM, false>(m, sp, local, strip, j0, j1);
+                // the kernel's own flags as template arguments: a branch that
+                // cannot run instantiates nothing new
+                const bool lim = LIM && m.limit, dif = DIFF && has_diffusion(m);
+                if (lim && dif) sweep_item<MODE, PHASE, NQ, LIM, DIFF>(m, sp, local, strip, j0, j1);
+                else if (lim) sweep_item<MODE, PHASE, NQ, LIM, false>(m, sp, local, strip, j0, j1);
+                else if (dif) sweep_item<MODE, PHASE, NQ, false, DIFF>(m, sp, local, strip, j0, j1);
                 else sweep_item<MODE, PHASE, NQ, false, false>(m, sp, local, strip, j0, j1);
             }
         }
@@ -1250,7 +1256,7 @@ hipError_t launch_state_sums(const double* H, const double* C, int64_t n, double
 
 hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes,
                               const StagePtrs* stages, const int2* items, int nitems, int blocks,
-                              int* queue, hipStream_t stream, bool global_lf, int limit)
+                              int* queue, hipStream_t stream, bool global_lf, int limit, bool diffusion)
 {
     if (nitems <= 0) return hipSuccess;
     // A persistent grid (blocks > 0) runs 4-wave workgroups that take items from
@@ -1259,16 +1265,24 @@ hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes
     const dim3 grid(blocks > 0 ? blocks : nitems), block(blocks > 0 ? 256 : kWave);
     if (mode != 0) {
         if (phase) return hipErrorInvalidValue;   // mixed: batch_phase_items_kernel follows
-        return launch_batch_stage32(meshes, stages, items, nitems, blocks, queue, global_lf, stream, limit);
+        return launch_batch_stage32(meshes, stages, items, nitems, blocks, queue, global_lf, stream, limit,
+                                    diffusion);
     }
-#define MP_BATCH64(P, N, L) batch_sweep_kernel<0, P, N, L><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp)
+#define MP_BATCH64(P, N, L, D) \
+    batch_sweep_kernel<0, P, N, L, D><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp)
+#define MP_BATCH64_LD(P, N)                                                  \
+    do {                                                                     \
+        if (limit && diffusion) MP_BATCH64(P, N, true, true);                \
+        else if (limit) MP_BATCH64(P, N, true, false);                       \
+        else if (diffusion) MP_BATCH64(P, N, false, true);                   \
+        else MP_BATCH64(P, N, false, false);                                 \
+    } while (0)
     if (q == 5) {
-        if (limit) { if (phase) MP_BATCH64(true, 5, true); else MP_BATCH64(false, 5, true); }
-        else { if (phase) MP_BATCH64(true, 5, false); else MP_BATCH64(false, 5, false); }
+        if (phase) MP_BATCH64_LD(true, 5); else MP_BATCH64_LD(false, 5);
     } else {
-        if (limit) { if (phase) MP_BATCH64(true, 0, true); else MP_BATCH64(false, 0, true); }
-        else { if (phase) MP_BATCH64(true, 0, false); else MP_BATCH64(false, 0, false); }
+        if (phase) MP_BATCH64_LD(true, 0); else MP_BATCH64_LD(false, 0);
     }
+#undef MP_BATCH64_LD
 #undef MP_BATCH64
     return hipGetLastError();
 }

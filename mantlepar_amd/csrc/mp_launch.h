@@ -45,9 +45,12 @@ hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H
 // limit: 0 when no mesh of the list has its limiter on; else the limiter
 // instance, in which each mesh takes the limited sweep only if its limiter is
 // on (mixed: with the table instance of its single-mesh launch).
+// diffusion: some mesh of the list has diffusion or thermal boundary data;
+// each mesh then takes the diffusion terms only if it has them.
 hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes,
                               const StagePtrs* stages, const int2* items, int nitems, int blocks,
-                              int* queue, hipStream_t stream, bool global_lf = false, int limit = 0);
+                              int* queue, hipStream_t stream, bool global_lf = false, int limit = 0,
+                              bool diffusion = false);
 // prev (may be null): per-mesh totals before the step; with it each mesh's
 // heat balance is checked at tolerance tol (kErrBalance in its error word).
 hipError_t launch_batch_reduce(const StagePtrs* stages, const int* nitems, const MeshDev* meshes,
@@ -73,6 +76,15 @@ int batch_blocks_per_cu32();
 // whose point loops are unrolled at compile time.
 hipError_t launch_stage32(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
                           bool diffusion, bool global_lf, int limit = 0);
+// Whether a mesh record carries thermal diffusion or thermal boundary data
+// (the sweeps then compile in the diffusion terms).
+__host__ __device__ inline bool has_diffusion(const MeshDev& m)
+{
+    bool on = m.dif_x != 0.0 || m.dif_y != 0.0 || m.dif_bx != 0.0 || m.dif_by != 0.0;
+    for (int k = 0; k < 4; ++k) on = on || m.tbc[k] != nullptr;
+    return on;
+}
+
 // The limit argument of launch_stage32 for a mesh record.
 __host__ __device__ inline int limit_mode32(const MeshDev& m)
 {
@@ -91,7 +103,7 @@ hipError_t launch_step32(const MeshDev* d_mesh, const StagePtrs* tab, int nitems
                          hipStream_t stream, bool diffusion, bool global_lf);
 hipError_t launch_batch_stage32(const MeshDev* meshes, const StagePtrs* stages, const int2* items,
                                 int nitems, int blocks, int* queue, bool global_lf, hipStream_t stream,
-                                int limit = 0);
+                                int limit = 0, bool diffusion = false);
 // All stages of one mixed-precision step over a batch work list (one wave per
 // entry, (-1, 0) pads skipped) in one launch.  Stage st of slot k reads
 // tab[st * capacity + k]; completion flags are flags[st * nflags +

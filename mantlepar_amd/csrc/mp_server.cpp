@@ -374,6 +374,11 @@ void mp_server::admit(std::vector<Request*>& batch, int rows_now)
         if (st == MP_OK && q->r.lithostatic) st = eng.set_lithostatic(k, 1, q->r.surface_y);
         if (st == MP_OK && q->r.limiter)
             st = eng.set_limiter(k, 1, q->r.cell_points_per_axis, q->r.diffusion_samples, q->r.extent_fraction);
+        if (st == MP_OK && q->r.thermal_diffusivity != 0.0)
+            st = eng.set_diffusion(k, q->r.thermal_diffusivity, q->r.diffusion_samples ? q->r.diffusion_samples : 4,
+                                   q->r.extent_fraction != 0.0 ? q->r.extent_fraction : 0.9);
+        if (st == MP_OK && q->r.thermal_types)
+            st = eng.set_thermal_boundary(k, q->r.thermal_types, q->r.thermal_values);
         if (st != MP_OK) {
             finish(q, st, last_error());
             continue;

@@ -710,9 +710,10 @@ __global__ __launch_bounds__(64, 3) void step32_kernel(const MeshDev* __restrict
 // Batch meshes carry no thermal diffusion and share one LF mode (FULL: global).
 // LIM != 0: meshes whose limiter is on take the limited sweep (with the
 // table instance that their single-mesh launch takes), the others the plain
-// one, at the limited instance's occupancy.
-template <bool FULL, int LIM = 0>
-__global__ __launch_bounds__(256, LIM != 0 ? 2 : 3) void batch32_kernel(const MeshDev* __restrict__ meshes,
+// one, at the limited instance's occupancy; DIFF likewise for meshes with
+// diffusion or thermal boundary data.
+template <bool FULL, int LIM = 0, bool DIFF = false>
+__global__ __launch_bounds__(256, LIM != 0 || DIFF ? 2 : 3) void batch32_kernel(const MeshDev* __restrict__ meshes,
                                                          const StagePtrs* __restrict__ stages,
                                                          const int2* __restrict__ items, int nitems,
                                                          int* queue)
@@ -735,7 +736,13 @@ __global__ __launch_bounds__(256, LIM != 0 ? 2 : 3) void batch32_kernel(const Me
             int strip, j0, j1;
             item_region(sp, m.ny, local, strip, j0, j1);
             if (j0 < m.ny) {           // always true with a current work list
-                if (LIM != 0 && m.limit) sweep32_item<false, FULL, false, 3>(m, sp, local, strip, j0, j1);
+                // the kernel's own flags as template arguments: a branch that
+                // cannot run instantiates nothing new
+                constexpr int L = LIM != 0 ? 3 : 0;
+                const bool lim = LIM != 0 && m.limit, dif = DIFF && has_diffusion(m);
+                if (lim && dif) sweep32_item<DIFF, FULL, false, L>(m, sp, local, strip, j0, j1);
+                else if (lim) sweep32_item<false, FULL, false, L>(m, sp, local, strip, j0, j1);
+                else if (dif) sweep32_item<DIFF, FULL>(m, sp, local, strip, j0, j1);
                 else sweep32_item<false, FULL>(m, sp, local, strip, j0, j1);
             }
         }
@@ -852,18 +859,23 @@ int batch_blocks_per_cu32()
 
 hipError_t launch_batch_stage32(const MeshDev* meshes, const StagePtrs* stages, const int2* items,
                                 int nitems, int blocks, int* queue, bool global_lf, hipStream_t stream,
-                                int limit)
+                                int limit, bool diffusion)
 {
     if (nitems <= 0) return hipSuccess;
     int* q = blocks > 0 ? queue : nullptr;
     const dim3 grid(blocks > 0 ? blocks : nitems), block(blocks > 0 ? 256 : kWave);
-#define MP_BATCH32(F, L) batch32_kernel<F, L><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q)
-    // a limited mesh takes the instance its single-mesh launch takes
-    if (global_lf) {
-        if (limit) MP_BATCH32(true, 1); else MP_BATCH32(true, 0);
-    } else {
-        if (limit) MP_BATCH32(false, 1); else MP_BATCH32(false, 0);
-    }
+#define MP_BATCH32(F, L, D) batch32_kernel<F, L, D><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, q)
+#define MP_BATCH32_LD(F)                                   \
+    do {                                                   \
+        if (limit && diffusion) MP_BATCH32(F, 1, true);    \
+        else if (limit) MP_BATCH32(F, 1, false);           \
+        else if (diffusion) MP_BATCH32(F, 0, true);        \
+        else MP_BATCH32(F, 0, false);                      \
+    } while (0)
+    // a limited mesh takes the table instance its single-mesh launch takes
+    if (global_lf) MP_BATCH32_LD(true);
+    else MP_BATCH32_LD(false);
+#undef MP_BATCH32_LD
 #undef MP_BATCH32
     return hipGetLastError();
 }

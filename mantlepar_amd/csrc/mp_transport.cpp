@@ -455,9 +455,7 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
 
 bool mp_transport::has_diffusion() const
 {
-    bool on = md.dif_x != 0.0 || md.dif_y != 0.0 || md.dif_bx != 0.0 || md.dif_by != 0.0;
-    for (int k = 0; k < 4; ++k) on = on || md.tbc[k] != nullptr;
-    return on;
+    return mp::has_diffusion(md);
 }
 
 mp_status mp_transport::check_step_args(int nsteps, int scheme, double dt)

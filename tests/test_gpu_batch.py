@@ -345,6 +345,81 @@ def test_limited_meshes_match_single_mesh_bitwise(prec, tables):
     b.close()
 
 
+def thermal_profile(spec):
+    """Bottom: prescribed temperature 0.9; top: outward flux 0.01 on its left
+    half; left: prescribed temperature varying along the side; right: zero
+    flux (flat arrays, sides bottom, right, top, left, each [face][q])."""
+    nx, ny, q = spec.nx, spec.ny, spec.q
+    types = [np.ones((nx, q)), np.zeros((ny, q)),
+             np.repeat((np.arange(nx) < nx // 2)[:, None] * 2, q, axis=1), np.ones((ny, q))]
+    vals = [np.full((nx, q), 0.9), np.zeros((ny, q)), np.full((nx, q), 0.01),
+            0.7 + 0.2 * np.linspace(0, 1, ny * q).reshape(ny, q)]
+    return (np.concatenate([np.asarray(t, np.uint8).ravel() for t in types]),
+            np.concatenate([v.ravel() for v in vals]))
+
+
+@pytest.mark.parametrize("prec", [M.PREC_FP64, M.PREC_MIXED], ids=["fp64", "mixed"])
+def test_diffusion_meshes_match_single_mesh_bitwise(prec):
+    """Meshes with thermal diffusion (one also with thermal boundary data, one
+    also with the limiter) share launches with plain meshes; each reproduces a
+    single-mesh handle with the same settings bit for bit, stable step
+    included."""
+    ph = W.config_phase(hypereutectic=0)
+    specs = [W.MeshSpec(48, 40, seed=3), W.MeshSpec(130, 97, seed=13), W.MeshSpec(20, 20, seed=11),
+             W.MeshSpec(40, 30, seed=5, q=3)]
+    setup = {0: dict(kappa=2e-4, thermal=True), 1: dict(kappa=1e-4), 3: dict(kappa=3e-4, limiter=True)}
+
+    def configure(h, k, i):
+        cfg = setup.get(i)
+        if not cfg:
+            return
+        args = (k,) if k is not None else ()
+        h.set_diffusion(*args, cfg["kappa"], 4, 0.9)
+        if cfg.get("thermal"):
+            h.set_thermal_boundary(*args, *thermal_profile(specs[i]))
+        if cfg.get("limiter"):
+            h.set_limiter(*args, True, 5, 4, 0.9)
+
+    b = M.Batch(8, phase=ph, precision=prec)
+    ids = []
+    for i, spec in enumerate(specs):
+        H, Cc = limiter_fields(spec)
+        k = b.add_mesh(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q)
+        b.upload(k, H, Cc)
+        b.set_face_velocity(k, *W.face_velocity(spec))
+        configure(b, k, i)
+        b.set_dt(k, b.stable_dt(k, 0.4))
+        ids.append(k)
+    b.set_schedule(12, 0)
+    assert not b.per_step_launch()
+    reps = b.step(4, graph=True)
+    for i, (k, spec) in enumerate(zip(ids, specs)):
+        H, Cc = limiter_fields(spec)
+        t = M.Transport(spec.nx, spec.ny, spec.x0, spec.x1, spec.y0, spec.y1, q=spec.q, phase=ph,
+                        precision=prec)
+        t.set_tile(12)
+        t.upload(H, Cc)
+        t.set_face_velocity(*W.face_velocity(spec))
+        configure(t, None, i)
+        dt = t.stable_dt(0.4)
+        assert dt == b.stable_dt(k, 0.4), i
+        rep = t.step(4, dt)
+        Hs, Cs = t.download()
+        Hb, Cb = b.download(k)
+        assert np.array_equal(Hb, Hs) and np.array_equal(Cb, Cs), i
+        ps, pb = t.phase_state(), b.phase_state(k)
+        for n in ("T", "phi", "region"):
+            assert np.array_equal(pb[n], ps[n]), n
+        for n in ("total_H", "total_C", "boundary_H", "boundary_C"):
+            assert reps[k][n] == rep[n], (i, n)
+        t.close()
+    with pytest.raises(M.MantleError, match="Unknown thermal boundary policy"):
+        bad = thermal_profile(specs[0])[0].copy()
+        bad[0] = 3
+        b.set_thermal_boundary(ids[0], bad, thermal_profile(specs[0])[1])
+    b.close()
+
+
 def test_launch_mode_choices():
     b = M.Batch(4, phase=W.config_phase(), precision=M.PREC_MIXED)
     fill_batch(b, SPECS[:2])

@@ -281,3 +281,48 @@ def test_limited_requests_equal_single_mesh(prec):
             assert np.array_equal(ph_s[n], out[n]), n
         assert rep["total_H"] == res["total_H"] and rep["boundary_C"] == res["boundary_C"]
         t.close()
+
+
+@pytest.mark.parametrize("prec", [M.PREC_FP64, M.PREC_MIXED], ids=["fp64", "mixed"])
+def test_diffusion_requests_equal_single_mesh(prec):
+    """Requests with thermal diffusion (one with thermal boundary data) among
+    requests without it: each result is bitwise a single-mesh handle's."""
+    reqs = make_requests(5, seed=91, max_steps=9)
+    for k, r in enumerate(reqs):
+        s = r["spec"]
+        if k == 0:
+            q = s.q
+            types = np.concatenate([np.ones(s.nx * q), np.zeros(s.ny * q), np.zeros(s.nx * q),
+                                    np.ones(s.ny * q)]).astype(np.uint8)
+            vals = np.concatenate([np.full(s.nx * q, 0.9), np.zeros(s.ny * q), np.zeros(s.nx * q),
+                                   np.full(s.ny * q, 0.7)])
+            r["diffusion"] = (2e-4, types, vals)
+        elif k == 2:
+            r["diffusion"] = (1e-4, None, None)
+    srv = M.Server(max_slots=8, phase=W.config_phase(), precision=prec, max_epoch_steps=4)
+    ids = []
+    for r in reqs:
+        s = r["spec"]
+        ids.append(srv.submit(s.nx, s.ny, r["H"], r["C"], r["vx"], r["vy"], r["nsteps"], 0.4,
+                              s.x0, s.x1, s.y0, s.y1, q=s.q, diffusion=r.get("diffusion")))
+    srv.drain()
+    outs = [srv.wait(i) for i in ids]
+    srv.close()
+    for r, (res, out) in zip(reqs, outs):
+        s = r["spec"]
+        t = M.Transport(s.nx, s.ny, s.x0, s.x1, s.y0, s.y1, q=s.q, phase=W.config_phase(), precision=prec)
+        t.set_tile(12)
+        t.upload(r["H"], r["C"])
+        t.set_face_velocity(r["vx"], r["vy"])
+        if r.get("diffusion"):
+            kappa, types, vals = r["diffusion"]
+            t.set_diffusion(kappa, 4, 0.9)
+            if types is not None:
+                t.set_thermal_boundary(types, vals)
+        dt = t.stable_dt(0.4)
+        assert dt == res["dt"]
+        rep = t.step(r["nsteps"], dt)
+        H, Cc = t.download()
+        assert np.array_equal(H, out["H"]) and np.array_equal(Cc, out["C"])
+        assert rep["total_H"] == res["total_H"] and rep["boundary_H"] == res["boundary_H"]
+        t.close()
