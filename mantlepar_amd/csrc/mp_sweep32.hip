@@ -33,10 +33,14 @@ namespace mp {
 namespace {
 
 // Wavefront shifts (DPP wave_shr:1 and wave_shl:1).  Lane L receives lane
-// L-1 (shr) or L+1 (shl); the end lane keeps its own value.  All lanes must
-// be active, so callers use them only in wave-uniform control flow.
-__device__ __forceinline__ int shr1_b32(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ int shl1_b32(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+// L-1 (shr) or L+1 (shl); the end lane, which has no source, receives 0
+// (bound_ctrl), so one DPP move does the shift with no copy of the operand
+// first.  Every use discards the end lane's value: lane 0 (shr) and lane 63
+// (shl) are the halo lanes, which take their outer neighbour from memory and
+// own no cell.  All lanes must be active, so callers use them only in
+// wave-uniform control flow.
+__device__ __forceinline__ int shr1_b32(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int shl1_b32(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true); }
 
 __device__ __forceinline__ double shr1(double v)
 {
@@ -189,7 +193,9 @@ __device__ __forceinline__ void blend32(const K32& k, const Row32& R0, const Row
         }
         om[5] = 0.0f;
     }
-    const float inv = 1.0f / sum;
+    // sum lies in [1, 6] (the largest term is exp2(0) = 1): the hardware
+    // reciprocal (1 ulp) needs none of a division's range fix-ups
+    const float inv = __builtin_amdgcn_rcpf(sum);
 #pragma unroll
     for (int s = 0; s < 5; ++s) om[s] *= inv;
     // Blend (Q1 candidates embed in the Q2 monomials; the constant candidate's
@@ -362,22 +368,6 @@ __device__ __forceinline__ void recon32_limited(const MeshDev& m, const K32& k, 
     lines32(k, B, out);
 }
 
-// The face's full moments from its one-sided record: the side that v's sign
-// names, +0 on the other; a flagged face (b NaN) reads its full record.
-__device__ __forceinline__ FaceMom expand_rec(const FaceRec& h, const FaceMom* full, size_t face)
-{
-    if (__builtin_isnan(h.b)) return gload(full + face);
-    const bool pos = h.v >= 0.0;
-    FaceMom o;
-    o.vp = pos ? h.v : 0.0;
-    o.vm = pos ? 0.0 : h.v;
-    o.p1 = pos ? h.a : 0.f;
-    o.p2 = pos ? h.b : 0.f;
-    o.m1 = pos ? 0.f : h.a;
-    o.m2 = pos ? 0.f : h.b;
-    return o;
-}
-
 // The record a sweep fetches per face: the one-sided form under the local LF
 // speed (FULL = false), the full record under the global speed.
 template <bool FULL> struct FaceFetch { using type = FaceRec; };
@@ -399,6 +389,32 @@ __device__ __forceinline__ void face_flux(const FaceMom& f, double D, double tLH
     const double adv = fma(tLH, f.vp, tRH * f.vm) + double(ch);
     FH = DIFF ? fma(D, tLH - tRH, adv) : adv;
     FC = fma(tLC, f.vp, tRC * f.vm) + double(cc);
+}
+
+// The same flux from the face's one-sided record (the local LF speed): the
+// side that v's sign names is the only one with weight, so its centre and
+// line are selected and dotted with the record's three moments, instead of
+// expanding the record and multiplying the other side by zeros.  A flagged
+// face (b NaN: weight on both sides) takes the full record.
+template <bool DIFF>
+__device__ __forceinline__ void face_flux_rec(const FaceRec& h, const FaceMom* full, size_t face, double D,
+                                              double tLH, const float (&cLH)[3], double tRH,
+                                              const float (&cRH)[3], double tLC,
+                                              const float (&cLC)[3], double tRC,
+                                              const float (&cRC)[3], double& FH, double& FC)
+{
+    if (__builtin_isnan(h.b)) {
+        face_flux<DIFF>(gload(full + face), D, tLH, cLH, tRH, cRH, tLC, cLC, tRC, cRC, FH, FC);
+        return;
+    }
+    const bool pos = h.v >= 0.0;
+    const double tH = pos ? tLH : tRH, tC = pos ? tLC : tRC;
+    const float v = float(h.v);
+    const float ch = (pos ? cLH[0] : cRH[0]) * v + (pos ? cLH[1] : cRH[1]) * h.a + (pos ? cLH[2] : cRH[2]) * h.b;
+    const float cc = (pos ? cLC[0] : cRC[0]) * v + (pos ? cLC[1] : cRC[1]) * h.a + (pos ? cLC[2] : cRC[2]) * h.b;
+    const double adv = tH * h.v + double(ch);
+    FH = DIFF ? fma(D, tLH - tRH, adv) : adv;
+    FC = tC * h.v + double(cc);
 }
 
 // One work item of a stage.  The stage's phase and state sums come from
@@ -481,10 +497,6 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
         if constexpr (FULL) return gload(m.fx + fx_index(row));
         else return gload(m.rx + fx_index(row));
     };
-    auto moments = [&](const Rec& h, const FaceMom* full, size_t face) -> FaceMom {
-        if constexpr (FULL) return h;
-        else return expand_rec(h, full, face);
-    };
     const bool need_n = sp.a != 0.0;
     Rec fyc = load_fy(r), fxc = load_fx(r);
     double hn = 0.0, cn = 0.0;                     // u_n of the row being finished
@@ -495,7 +507,12 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
     double accH = 0, accC = 0;
     double bH = 0, bC = 0;
 
-    for (; r <= j1; ++r) {
+    // One row of the sweep.  H0/C0, H1/C1, H2/C2 are rows r-1, r, r+1; the
+    // row r+2 that the next call needs is built into H0/C0, whose row is no
+    // longer used, and the caller passes the three rows rotated.  Unrolled
+    // three times, the rotation is a renaming of registers, not moves.
+    auto row_step = [&](Row32& H0, Row32& H1, Row32& H2, Row32& C0, Row32& C1,
+                        Row32& C2) __attribute__((always_inline)) {
         const bool have = r < ny;
         const bool preroll = r == j0 - 1;
         const bool shift = r + 1 <= j1;
@@ -572,8 +589,11 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
                 FvC = sign * oCf;
                 if (owned) { bH += oHf; bC += oCf; }
             } else {
-                const FaceMom f = moments(fyc, m.fy, fy_index(r));
-                face_flux<DIFF>(f, m.dif_y, H0.u, topH, tH, LH.c[0], C0.u, topC, tC, LC.c[0], FvH, FvC);
+                if constexpr (FULL)
+                    face_flux<DIFF>(fyc, m.dif_y, H0.u, topH, tH, LH.c[0], C0.u, topC, tC, LC.c[0], FvH, FvC);
+                else
+                    face_flux_rec<DIFF>(fyc, m.fy, fy_index(r), m.dif_y, H0.u, topH, tH, LH.c[0], C0.u, topC,
+                                        tC, LC.c[0], FvH, FvC);
             }
         }
         if (shift) fyc = load_fy(r + 1);   // used next iteration
@@ -632,8 +652,11 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
                     FrC = sign * oCf;
                     if (left || owned) { bH += oHf; bC += oCf; }
                 } else {
-                    const FaceMom f = moments(fxc, m.fx, fx_index(r));
-                    face_flux<DIFF>(f, m.dif_x, tH, LH.c[1], nbtH, nbH, tC, LC.c[1], nbtC, nbC, FrH, FrC);
+                    if constexpr (FULL)
+                        face_flux<DIFF>(fxc, m.dif_x, tH, LH.c[1], nbtH, nbH, tC, LC.c[1], nbtC, nbC, FrH, FrC);
+                    else
+                        face_flux_rec<DIFF>(fxc, m.fx, fx_index(r), m.dif_x, tH, LH.c[1], nbtH, nbH, tC, LC.c[1],
+                                            nbtC, nbC, FrH, FrC);
                 }
             }
             accH = shr1(FrH) - FrH + FvH;
@@ -644,11 +667,20 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
         for (int c = 0; c < 3; ++c) { topH[c] = LH.c[2][c]; topC[c] = LC.c[2][c]; }
         if (shift) fxc = load_fx(r + 1);   // used next iteration
         if (shift) {
-            H0 = H1; H1 = H2;
-            C0 = C1; C1 = C2;
-            H2 = make_row(nH, oH, H1.u, first, last, k);
-            C2 = make_row(nC, oC, C1.u, first, last, k);
+            H0 = make_row(nH, oH, H2.u, first, last, k);
+            C0 = make_row(nC, oC, C2.u, first, last, k);
         }
+    };
+    for (;;) {
+        if (r > j1) break;
+        row_step(H0, H1, H2, C0, C1, C2);
+        ++r;
+        if (r > j1) break;
+        row_step(H1, H2, H0, C1, C2, C0);
+        ++r;
+        if (r > j1) break;
+        row_step(H2, H0, H1, C2, C0, C1);
+        ++r;
     }
     item_partials(sp, item, 0.0, 0.0, 0.0, 0.0, bH, bC, err);
 }
