@@ -97,6 +97,9 @@ struct MeshDev {
     // 2x2 bilinear candidates: slopes 1/aw, 1/bw, 1/(aw*bw) and the
     // target-cell mean of the xy-term (aw^2+bw^2)/12.
     double inv_aw, inv_bw, inv_awbw, q11w;
+    // the mixed sweep's indicator works on q11 * sqrt(q11w): its scale
+    // inv_awbw * sqrt(q11w) and the inverse 1 / sqrt(q11w)
+    double q11s, q11u;
     // ML-WENO weights (src/mlweno/reconstruction.cpp:212-265)
     double epsA;                // epsilon * area
     double pL, pS, pC;          // exponents s*r + eta(r): large, small, constant

@@ -59,7 +59,8 @@ __device__ __forceinline__ float shl1(float v) { return __builtin_bit_cast(float
 // Per-mesh constants in FP32, converted once per work item.
 struct K32 {
     float kx0, kx1, kx2, ky0, ky1, ky2;
-    float iaw, ibw, iawbw, q11w, hx, hy;
+    float iaw, ibw, hx, hy;
+    float q11s, q11u;   // scale of the mixed term's row differences in the indicator, and its inverse
     float epsA, pL, pS, ldL, ldS[4], laC;
     bool use_constant;
 };
@@ -69,8 +70,12 @@ __device__ __forceinline__ K32 make_k32(const MeshDev& m)
     K32 k;
     k.kx0 = float(m.kx0); k.kx1 = float(m.kx1); k.kx2 = float(m.kx2);
     k.ky0 = float(m.ky0); k.ky1 = float(m.ky1); k.ky2 = float(m.ky2);
-    k.iaw = float(m.inv_aw); k.ibw = float(m.inv_bw); k.iawbw = float(m.inv_awbw);
-    k.q11w = float(m.q11w); k.hx = float(m.hx); k.hy = float(m.hy);
+    k.iaw = float(m.inv_aw); k.ibw = float(m.inv_bw);
+    k.hx = float(m.hx); k.hy = float(m.hy);
+    // q11 * q11 * q11w == (q11 * sqrt(q11w))^2: the indicator squares the
+    // scaled difference, the blend scales it back
+    k.q11s = float(m.q11s);
+    k.q11u = float(m.q11u);
     const float sc = wscale<float>();
     k.epsA = float(m.epsA);
     k.pL = float(m.pL);
@@ -132,11 +137,12 @@ __device__ __forceinline__ void blend32(const K32& k, const Row32& R0, const Row
     {
         const float p10 = P[0][1], p01 = P[1][0], p20 = P[0][2], p02 = P[2][0], p11 = P[1][1];
         const float p21 = P[1][2], p12 = P[2][1], p22 = P[2][2];
+        // (13/12) (2 x)^2 = (13/3) x^2 for the second-derivative terms
         const float e1 = p10 + p12 * (1.0f / 12.0f);
         const float e2 = p01 + p21 * (1.0f / 12.0f);
-        const float x20 = 2.0f * (p20 + p22 * (1.0f / 12.0f));
-        const float y02 = 2.0f * (p02 + p22 * (1.0f / 12.0f));
-        sig[0] = e1 * e1 + e2 * e2 + (13.0f / 12.0f) * (x20 * x20 + y02 * y02) +
+        const float x20 = p20 + p22 * (1.0f / 12.0f);
+        const float y02 = p02 + p22 * (1.0f / 12.0f);
+        sig[0] = e1 * e1 + e2 * e2 + (13.0f / 3.0f) * (x20 * x20 + y02 * y02) +
                  (7.0f / 6.0f) * p11 * p11 + (7.0f / 10.0f) * (p12 * p12 + p21 * p21) +
                  (43.0f / 270.0f) * p22 * p22;
     }
@@ -145,17 +151,17 @@ __device__ __forceinline__ void blend32(const K32& k, const Row32& R0, const Row
     // mixed term is a difference of neighbour differences of two rows.
     const float q10p = R1.dr * k.iaw, q10m = -R1.dl * k.iaw;
     const float q01p = ehi * k.ibw, q01m = elo * k.ibw;
-    float q11[4];
-    q11[0] = (R1.dr - R0.dr) * k.iawbw;
-    q11[1] = (R2.dr - R1.dr) * k.iawbw;
-    q11[2] = (R0.dl - R1.dl) * k.iawbw;
-    q11[3] = (R1.dl - R2.dl) * k.iawbw;
+    float q11[4];   // mixed terms, scaled by sqrt(q11w)
+    q11[0] = (R1.dr - R0.dr) * k.q11s;
+    q11[1] = (R2.dr - R1.dr) * k.q11s;
+    q11[2] = (R0.dl - R1.dl) * k.q11s;
+    q11[3] = (R1.dl - R2.dl) * k.q11s;
     {
         const float xp = q10p * q10p, xm = q10m * q10m, yp = q01p * q01p, ym = q01m * q01m;
-        sig[1] = xp + ym + q11[0] * q11[0] * k.q11w;
-        sig[2] = xp + yp + q11[1] * q11[1] * k.q11w;
-        sig[3] = xm + ym + q11[2] * q11[2] * k.q11w;
-        sig[4] = xm + yp + q11[3] * q11[3] * k.q11w;
+        sig[1] = xp + ym + q11[0] * q11[0];
+        sig[2] = xp + yp + q11[1] * q11[1];
+        sig[3] = xm + ym + q11[2] * q11[2];
+        sig[4] = xm + yp + q11[3] * q11[3];
     }
     // Normalised weights in log space, base 2 (src/mlweno/reconstruction.cpp:212-265).
     float la[6], om[6], sum = 0.0f;
@@ -206,7 +212,7 @@ __device__ __forceinline__ void blend32(const K32& k, const Row32& R0, const Row
         for (int a = 0; a < 3; ++a) B[b][a] = om[0] * P[b][a];
     B[0][1] += (om[1] + om[2]) * q10p + (om[3] + om[4]) * q10m;
     B[1][0] += (om[2] + om[4]) * q01p + (om[1] + om[3]) * q01m;
-    B[1][1] += om[1] * q11[0] + om[2] * q11[1] + om[3] * q11[2] + om[4] * q11[3];
+    B[1][1] += (om[1] * q11[0] + om[2] * q11[1] + om[3] * q11[2] + om[4] * q11[3]) * k.q11u;
 }
 
 // The blend's four face lines c0 + s*c1 + s^2*c2.
@@ -383,9 +389,19 @@ __device__ __forceinline__ void face_flux(const FaceMom& f, double D, double tLH
                                           const float (&cLC)[3], double tRC,
                                           const float (&cRC)[3], double& FH, double& FC)
 {
+    // The sums are written as explicit fma chains: left to the compiler, a
+    // sum of products may be contracted differently in different kernel
+    // instances, and every instance must give the same bits.
     const float vp = float(f.vp), vm = float(f.vm);
-    const float ch = cLH[0] * vp + cLH[1] * f.p1 + cLH[2] * f.p2 + cRH[0] * vm + cRH[1] * f.m1 + cRH[2] * f.m2;
-    const float cc = cLC[0] * vp + cLC[1] * f.p1 + cLC[2] * f.p2 + cRC[0] * vm + cRC[1] * f.m1 + cRC[2] * f.m2;
+    auto dot6 = [&](const float (&l)[3], const float (&r)[3]) {
+        float c = l[0] * vp;
+        c = fmaf(l[1], f.p1, c);
+        c = fmaf(l[2], f.p2, c);
+        c = fmaf(r[0], vm, c);
+        c = fmaf(r[1], f.m1, c);
+        return fmaf(r[2], f.m2, c);
+    };
+    const float ch = dot6(cLH, cRH), cc = dot6(cLC, cRC);
     const double adv = fma(tLH, f.vp, tRH * f.vm) + double(ch);
     FH = DIFF ? fma(D, tLH - tRH, adv) : adv;
     FC = fma(tLC, f.vp, tRC * f.vm) + double(cc);
@@ -410,11 +426,11 @@ __device__ __forceinline__ void face_flux_rec(const FaceRec& h, const FaceMom* f
     const bool pos = h.v >= 0.0;
     const double tH = pos ? tLH : tRH, tC = pos ? tLC : tRC;
     const float v = float(h.v);
-    const float ch = (pos ? cLH[0] : cRH[0]) * v + (pos ? cLH[1] : cRH[1]) * h.a + (pos ? cLH[2] : cRH[2]) * h.b;
-    const float cc = (pos ? cLC[0] : cRC[0]) * v + (pos ? cLC[1] : cRC[1]) * h.a + (pos ? cLC[2] : cRC[2]) * h.b;
-    const double adv = tH * h.v + double(ch);
+    const float ch = fmaf(pos ? cLH[2] : cRH[2], h.b, fmaf(pos ? cLH[1] : cRH[1], h.a, (pos ? cLH[0] : cRH[0]) * v));
+    const float cc = fmaf(pos ? cLC[2] : cRC[2], h.b, fmaf(pos ? cLC[1] : cRC[1], h.a, (pos ? cLC[0] : cRC[0]) * v));
+    const double adv = fma(tH, h.v, double(ch));
     FH = DIFF ? fma(D, tLH - tRH, adv) : adv;
-    FC = tC * h.v + double(cc);
+    FC = fma(tC, h.v, double(cc));
 }
 
 // One work item of a stage.  The stage's phase and state sums come from
@@ -434,7 +450,10 @@ __device__ __forceinline__ void face_flux_rec(const FaceRec& h, const FaceMom* f
 // default tables fixed at compile time (LimDefault), 3 either, chosen per mesh
 // as the single-mesh launch chooses (the batch's mixed lists), so that every
 // mesh runs the arithmetic of its own single-mesh sweep.
-template <bool DIFF, bool FULL, bool SHARED = false, int LIM = 0>
+// UNROLL: the row loop unrolled three times (the one-launch steps and the
+// global-speed instances, which fit it in their register budget; the others
+// would spill).  The arithmetic is the same either way.
+template <bool DIFF, bool FULL, bool SHARED = false, int LIM = 0, bool UNROLL = SHARED || FULL>
 __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& sp, int item, int strip, int j0,
                                              int j1)
 {
@@ -507,10 +526,11 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
     double accH = 0, accC = 0;
     double bH = 0, bC = 0;
 
-    // One row of the sweep.  H0/C0, H1/C1, H2/C2 are rows r-1, r, r+1; the
-    // row r+2 that the next call needs is built into H0/C0, whose row is no
-    // longer used, and the caller passes the three rows rotated.  Unrolled
-    // three times, the rotation is a renaming of registers, not moves.
+    // One row of the sweep.  H0/C0, H1/C1, H2/C2 are rows r-1, r, r+1.
+    // Unrolled, the row r+2 that the next call needs is built into H0/C0,
+    // whose row is no longer used, and the caller passes the three rows
+    // rotated, so the rotation is a renaming of registers, not moves; in the
+    // plain loop the rows shift down and r+2 is built into H2/C2.
     auto row_step = [&](Row32& H0, Row32& H1, Row32& H2, Row32& C0, Row32& C1,
                         Row32& C2) __attribute__((always_inline)) {
         const bool have = r < ny;
@@ -557,9 +577,9 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
             const unsigned mask = col_ok ? candidate_mask(i, r, nx, ny) : 2u;
             uint32_t e2 = 0;
             recon32<true>(k, H0, H1, H2, mask, LH, e2);
-            // with full face records and the diffusion terms, the two
-            // reconstructions interleaved spill past the register budget
-            if constexpr (FULL && DIFF) __builtin_amdgcn_sched_barrier(0);
+            // with full face records, the two reconstructions interleaved
+            // spill past the register budget
+            if constexpr (FULL) __builtin_amdgcn_sched_barrier(0);
             recon32<true>(k, C0, C1, C2, mask, LC, e2);
             if (col_ok) err |= e2;
         } else {
@@ -666,21 +686,32 @@ __device__ __forceinline__ void sweep32_item(const MeshDev& m, const StagePtrs& 
 #pragma unroll
         for (int c = 0; c < 3; ++c) { topH[c] = LH.c[2][c]; topC[c] = LC.c[2][c]; }
         if (shift) fxc = load_fx(r + 1);   // used next iteration
-        if (shift) {
-            H0 = make_row(nH, oH, H2.u, first, last, k);
-            C0 = make_row(nC, oC, C2.u, first, last, k);
+        if constexpr (UNROLL) {
+            if (shift) {
+                H0 = make_row(nH, oH, H2.u, first, last, k);
+                C0 = make_row(nC, oC, C2.u, first, last, k);
+            }
+        } else if (shift) {
+            H0 = H1; H1 = H2;
+            C0 = C1; C1 = C2;
+            H2 = make_row(nH, oH, H1.u, first, last, k);
+            C2 = make_row(nC, oC, C1.u, first, last, k);
         }
     };
-    for (;;) {
-        if (r > j1) break;
-        row_step(H0, H1, H2, C0, C1, C2);
-        ++r;
-        if (r > j1) break;
-        row_step(H1, H2, H0, C1, C2, C0);
-        ++r;
-        if (r > j1) break;
-        row_step(H2, H0, H1, C2, C0, C1);
-        ++r;
+    if constexpr (UNROLL) {
+        for (;;) {
+            if (r > j1) break;
+            row_step(H0, H1, H2, C0, C1, C2);
+            ++r;
+            if (r > j1) break;
+            row_step(H1, H2, H0, C1, C2, C0);
+            ++r;
+            if (r > j1) break;
+            row_step(H2, H0, H1, C2, C0, C1);
+            ++r;
+        }
+    } else {
+        for (; r <= j1; ++r) row_step(H0, H1, H2, C0, C1, C2);
     }
     item_partials(sp, item, 0.0, 0.0, 0.0, 0.0, bH, bC, err);
 }

@@ -275,6 +275,11 @@ mp_status make_mesh_dev(const mp_mesh_desc& mesh, const mp_weno_options& weno,
     m.inv_bw = (double)(1.0L / bw);
     m.inv_awbw = (double)(1.0L / ((long double)aw * bw));
     m.q11w = (double)(((long double)aw * aw + (long double)bw * bw) / 12.0L);
+    {
+        const long double rw = sqrtl(((long double)aw * aw + (long double)bw * bw) / 12.0L);
+        m.q11s = (double)(rw / ((long double)aw * bw));
+        m.q11u = (double)(1.0L / rw);
+    }
     m.epsA = weno.epsilon * m.area;
     if (!(m.epsA > 0)) return fail(MP_ERR_FP, "epsilon*area underflows to zero");
     // exponents s*r + eta(r): r = order + 1 (src/mlweno/reconstruction.cpp:43-44, 225-226)
