@@ -572,6 +572,20 @@ mp_status mp_coupled_rhs(mp_coupled* c, double* rhsH, double* rhsC, double* rate
  * on an error the state and *time are those of the last accepted step. */
 mp_status mp_coupled_run(mp_coupled* c, const mp_coupled_time* time_controls, double* time,
                          int32_t nsteps, mp_coupled_step* steps, int32_t* accepted);
+/* Batched form of mp_coupled_run (no counterpart in the reference, which
+ * advances one mesh per rank): n handles advance together, each with its own
+ * time controls time_controls[k] and time times[k].  Every round enqueues one
+ * step attempt of each handle still running as one launch per kernel over all
+ * of them and reads their totals back with one wait; each handle takes
+ * mp_coupled_run's decisions, so it ends as its own mp_coupled_run call would
+ * (state, time, step records).  steps: n*nsteps records, handle k's from
+ * steps + k*nsteps (may be NULL); accepted[k], status[k] per handle (may be
+ * NULL).  A handle that fails rolls back alone; the others go on.  Returns
+ * the status of the first failed handle.  The handles must share a device
+ * and not be used elsewhere during the call; it runs on handles[0]'s stream. */
+mp_status mp_coupled_run_batch(mp_coupled* const* handles, int32_t n, const mp_coupled_time* time_controls,
+                               double* times, int32_t nsteps, mp_coupled_step* steps, int32_t* accepted,
+                               mp_status* status);
 
 /* ------------------------------------------------------------- phase only --- */
 /* EutecticModel::evaluate over n cells on the device (host arrays in/out).

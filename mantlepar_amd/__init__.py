@@ -6,4 +6,4 @@ from .transport import (BC_INFLOW_OUTFLOW, BC_OUTFLOW, BC_ZERO_FLUX, PREC_FP64, 
                         PREC_MIXED, SCHEME_FE, SCHEME_SSPRK2, SCHEME_SSPRK3, EutecticModel,
                         Transport, default_phase, default_weno)
 from .batch import BATCH_EAGER, BATCH_GRAPH, Batch, Server  # noqa: F401,E402
-from .coupled import Coupled, default_coupled  # noqa: F401,E402
+from .coupled import Coupled, default_coupled, run_batch as coupled_run_batch  # noqa: F401,E402

@@ -179,6 +179,9 @@ SIGNATURES = [
     ("mp_coupled_rhs", C.c_int32, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(CoupledRates)]),
     ("mp_coupled_run", C.c_int32, [_vp, C.POINTER(CoupledTime), _dp, C.c_int32,
                                    C.POINTER(CoupledStep), C.POINTER(C.c_int32)]),
+    ("mp_coupled_run_batch", C.c_int32, [C.POINTER(_vp), C.c_int32, C.POINTER(CoupledTime), _dp, C.c_int32,
+                                         C.POINTER(CoupledStep), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32)]),
     ("mp_transport_set_thermal_boundary", C.c_int32, [_vp, _u8p, _dp]),
     ("mp_transport_set_lithostatic", C.c_int32, [_vp, C.c_int32, C.c_double]),
     ("mp_coupled_set_lithostatic", C.c_int32, [_vp, C.c_int32, C.c_double]),

@@ -105,3 +105,36 @@ class Coupled:
         rec = [{n: getattr(s, n) for n, _ in CoupledStep._fields_ if n != "pad"}
                for s in steps[:acc.value]]
         return float(t[0]), rec
+
+
+def run_batch(handles, nsteps, controls, times=None):
+    """Advance several Coupled handles together (mp_coupled_run_batch): one
+    step attempt of every running handle per round, one launch per kernel over
+    all of them.  controls: one dict of Coupled.run's time keywords (cfl,
+    initial, maximum, minimum, end, fixed) per handle, or one dict for all.
+    Returns a list of (status, time, step records) per handle; a handle that
+    failed keeps its last accepted state.  Raises nothing for per-handle
+    failures: check each status (0 is success)."""
+    n = len(handles)
+    if isinstance(controls, dict):
+        controls = [controls] * n
+    keys = ("cfl", "initial", "maximum", "minimum", "end", "fixed")
+    dflt = {"cfl": 0.5, "initial": 1e-2, "maximum": 1e-2, "minimum": 1e-12, "end": 1e30, "fixed": False}
+    tcs = (CoupledTime * n)()
+    for k, c in enumerate(controls):
+        v = {**dflt, **c}
+        tcs[k] = CoupledTime(*(float(v[x]) for x in keys[:5]), int(v["fixed"]))
+    t = np.zeros(n) if times is None else np.array(times, dtype=np.float64)
+    hs = (C.c_void_p * n)(*(h._h for h in handles))
+    steps = (CoupledStep * max(n * nsteps, 1))()
+    acc = (C.c_int32 * n)()
+    st = (C.c_int32 * n)()
+    code = lib().mp_coupled_run_batch(hs, n, tcs, dptr(t), nsteps, steps, acc, st)
+    if code != 0 and all(s == 0 for s in st):
+        check(code)                     # the call itself failed
+    out = []
+    for k in range(n):
+        rec = [{f: getattr(s, f) for f, _ in CoupledStep._fields_ if f != "pad"}
+               for s in steps[k * nsteps:k * nsteps + acc[k]]]
+        out.append((int(st[k]), float(t[k]), rec))
+    return out

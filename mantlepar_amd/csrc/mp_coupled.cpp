@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "mp_coupled.h"
@@ -32,9 +33,10 @@ struct mp_coupled {
     // (offsets below) and the error words of the RHS evaluations (slots).
     double* host_red = nullptr;         // [kRedSize]
     uint32_t* host_err = nullptr;       // [kErrSlots]
-    static constexpr int kRedRhs0 = 0, kRedAdm0 = 8, kRedAdm1 = 16, kRedRhs1 = 24, kRedAdm2 = 32,
-                         kRedRhs2 = 40, kRedSize = 48;
-    static constexpr int kErrSlots = 3;
+    static constexpr int kRedRhs0 = kCoupledRedRhs0, kRedAdm0 = kCoupledRedAdm0, kRedAdm1 = kCoupledRedAdm1,
+                         kRedRhs1 = kCoupledRedRhs1, kRedAdm2 = kCoupledRedAdm2, kRedRhs2 = kCoupledRedRhs2,
+                         kRedSize = kCoupledRedSize;
+    static constexpr int kErrSlots = kCoupledErrSlots;
 
     ~mp_coupled()
     {
@@ -65,6 +67,7 @@ struct mp_coupled {
                   mp_coupled_rates* tot);
     mp_status run(const mp_coupled_time& tc, double* time, int nsteps, mp_coupled_step* steps,
                   int32_t* accepted);
+    CoupledBatchMesh batch_record() const;
 };
 
 mp_status mp_coupled::sync_md()
@@ -578,6 +581,329 @@ mp_status mp_coupled::run(const mp_coupled_time& tc, double* time, int nsteps,
     return st;
 }
 
+CoupledBatchMesh mp_coupled::batch_record() const
+{
+    CoupledBatchMesh b;
+    std::memset(&b, 0, sizeof b);
+    b.d = cd;
+    b.S = state.as<double>();
+    b.O = old.as<double>();
+    b.R = rbuf.as<double>();
+    b.F = first.as<double>();
+    b.bnd = bnd.as<double>();
+    b.red = red.as<double>();
+    b.part = part.as<double>();
+    b.kinds = kinds.as<int>();
+    b.N = int64_t(N);
+    b.nf = nxf + nyf;
+    b.nb = coupled_reduce_blocks(int64_t(N));
+    return b;
+}
+
+namespace {
+
+// Pinned host memory of a batched run, freed on every exit.
+struct PinnedBlock {
+    void* p = nullptr;
+    ~PinnedBlock() { if (p) (void)hipHostFree(p); }
+};
+
+// One handle's place in the step loop of mp_coupled::run (same variables,
+// same decisions), advanced one step attempt per round.
+struct BatchMeshState {
+    enum { kRunning, kDone, kFailed } phase = kRunning;
+    bool new_step = true;          // the next attempt starts step s
+    int s = 0, accepted = 0, retries = 0;
+    double start = 0, dt = 0, remaining = 0, courant = 0;
+    double oldH = 0, oldC = 0, newH = 0, newC = 0;
+    mp_coupled_rates current{}, second{}, first{};
+    mp_status st = MP_OK;
+    std::string msg;
+};
+
+}  // namespace
+
+// Several handles advanced together.  Each round enqueues, for every handle
+// still running, the copies its loop needs and one step attempt (stage 1, its
+// state check, the second RHS, stage 2, its state check, the RHS of the new
+// state) as one launch per kernel over all those handles, then reads every
+// handle's totals and error words back with one wait; each handle then takes
+// the decisions of mp_coupled::run in the same order.  The kernels run the
+// single-mesh bodies and reductions on each handle's own buffers, so every
+// handle ends as its own run would: states, times and step records.
+static mp_status coupled_run_batch(mp_coupled* const* hs, int n, const mp_coupled_time* tcs, double* times,
+                                   int nsteps, mp_coupled_step* steps, int32_t* accepted_out,
+                                   mp_status* status_out)
+{
+    if (n <= 0 || !hs || !tcs || !times) return fail(MP_ERR_ARG, "batched coupled run needs handles, time controls and times");
+    if (nsteps < 0) return fail(MP_ERR_ARG, "nsteps must be nonnegative");
+    for (int k = 0; k < n; ++k) {
+        if (!hs[k]) return fail(MP_ERR_ARG, "null coupled handle %d", k);
+        for (int j = 0; j < k; ++j)
+            if (hs[j] == hs[k]) return fail(MP_ERR_ARG, "coupled handle %d appears twice", k);
+    }
+    hipStream_t stream = hs[0]->stream;
+    for (int k = 0; k < n; ++k) MP_HIP(hipStreamSynchronize(hs[k]->stream));
+    std::vector<BatchMeshState> ms(static_cast<size_t>(n));
+    for (int k = 0; k < n; ++k) {
+        const mp_coupled_time& tc = tcs[k];
+        if (!(tc.cfl > 0 && tc.cfl <= 1)) ms[k].st = fail(MP_ERR_ARG, "Coupled CFL must be in (0,1]");
+        else if (!hs[k]->have_flow) ms[k].st = fail(MP_ERR_STATE, "set the flow samples first");
+        if (ms[k].st) {
+            ms[k].phase = BatchMeshState::kFailed;
+            ms[k].msg = last_error();
+        }
+    }
+
+    // Device records, the per-round tables and the read-back, sized for n.
+    // ints: act[n], 5 bases [n+1] (cell, face, gather, reduction, stage),
+    // copy segments [2][2n] and their base [2n+1]; doubles: dt[n].
+    const size_t n1 = size_t(n) + 1;
+    const size_t nints = size_t(n) + 5 * n1 + 4 * size_t(n) + 2 * size_t(n) + 1;
+    DeviceBuffer d_rec, d_int, d_dbl, d_rb_red, d_rb_err;
+    MP_CALL(d_rec.alloc(sizeof(CoupledBatchMesh) * size_t(n)));
+    MP_CALL(d_int.alloc(sizeof(int) * nints));
+    MP_CALL(d_dbl.alloc(sizeof(double) * size_t(n)));
+    MP_CALL(d_rb_red.alloc(sizeof(double) * kCoupledRedSize * size_t(n)));
+    MP_CALL(d_rb_err.alloc(sizeof(uint32_t) * kCoupledErrSlots * size_t(n)));
+    PinnedBlock h_rec, h_int, h_dbl, h_rb_red, h_rb_err;
+    MP_HIP(hipHostMalloc(&h_rec.p, sizeof(CoupledBatchMesh) * size_t(n), hipHostMallocDefault));
+    MP_HIP(hipHostMalloc(&h_int.p, sizeof(int) * nints, hipHostMallocDefault));
+    MP_HIP(hipHostMalloc(&h_dbl.p, sizeof(double) * size_t(n), hipHostMallocDefault));
+    MP_HIP(hipHostMalloc(&h_rb_red.p, sizeof(double) * kCoupledRedSize * size_t(n), hipHostMallocDefault));
+    MP_HIP(hipHostMalloc(&h_rb_err.p, sizeof(uint32_t) * kCoupledErrSlots * size_t(n), hipHostMallocDefault));
+    auto* rec = static_cast<CoupledBatchMesh*>(h_rec.p);
+    for (int k = 0; k < n; ++k) rec[k] = hs[k]->batch_record();
+    MP_HIP(hipMemcpyAsync(d_rec.ptr, rec, sizeof(CoupledBatchMesh) * size_t(n), hipMemcpyHostToDevice, stream));
+    const CoupledBatchMesh* M = d_rec.as<CoupledBatchMesh>();
+    int* hi = static_cast<int*>(h_int.p);
+    int* di = d_int.as<int>();
+    double* hd = static_cast<double*>(h_dbl.p);
+    int* h_act = hi;
+    int* h_base[5];
+    for (int t = 0; t < 5; ++t) h_base[t] = hi + n + t * n1;
+    int* h_seg = hi + n + 5 * n1;
+    int* h_segbase = h_seg + 4 * size_t(n);
+    auto dev = [&](const int* h) { return di + (h - hi); };
+    std::vector<int> seg_pending;  // (handle, kind) pairs owed before the next attempt
+
+    // One round: the owed copies, then (if act non-empty) one attempt, or
+    // in the first round the starting RHS and state check, then the read-back.
+    auto round = [&](const std::vector<int>& act, bool first_round) -> mp_status {
+        const int na = int(act.size());
+        int tot[5] = {0, 0, 0, 0, 0};
+        for (int a = 0; a < na; ++a) {
+            const CoupledBatchMesh& r = rec[act[a]];
+            h_act[a] = act[a];
+            const int blocks[5] = {coupled_cell_blocks(r.N), coupled_face_blocks(r.nf), coupled_gather_blocks(r.N),
+                                   r.nb, coupled_stage_blocks(r.N)};
+            for (int t = 0; t < 5; ++t) {
+                h_base[t][a] = tot[t];
+                tot[t] += blocks[t];
+            }
+            hd[a] = ms[act[a]].dt;
+        }
+        for (int t = 0; t < 5; ++t) h_base[t][na] = tot[t];
+        const int nseg = int(seg_pending.size() / 2);
+        int copy_blocks = 0;
+        for (int q = 0; q < nseg; ++q) {
+            h_seg[2 * q] = seg_pending[2 * q];
+            h_seg[2 * q + 1] = seg_pending[2 * q + 1];
+            h_segbase[q] = copy_blocks;
+            copy_blocks += coupled_copy_blocks(rec[seg_pending[2 * q]].N);
+        }
+        h_segbase[nseg] = copy_blocks;
+        seg_pending.clear();
+        MP_HIP(hipMemcpyAsync(di, hi, sizeof(int) * nints, hipMemcpyHostToDevice, stream));
+        MP_HIP(hipMemcpyAsync(d_dbl.ptr, hd, sizeof(double) * size_t(n), hipMemcpyHostToDevice, stream));
+        MP_HIP(launch_coupled_batch_copy(M, dev(h_seg), dev(h_segbase), nseg, copy_blocks, stream));
+        if (na == 0) {
+            MP_HIP(hipStreamSynchronize(stream));
+            return MP_OK;
+        }
+        const CoupledBatchSpan cell{dev(h_act), dev(h_base[0]), na}, face{dev(h_act), dev(h_base[1]), na},
+            gath{dev(h_act), dev(h_base[2]), na}, red{dev(h_act), dev(h_base[3]), na},
+            stg{dev(h_act), dev(h_base[4]), na};
+        const double* ddt = d_dbl.as<double>();
+        if (first_round) {
+            MP_HIP(launch_coupled_batch_clear(M, dev(h_act), na, 0, 1, stream));
+            MP_HIP(launch_coupled_batch_rhs(M, cell, tot[0], face, tot[1], gath, tot[2], red, tot[3], 0,
+                                            kCoupledRedRhs0, stream));
+            MP_HIP(launch_coupled_batch_admissible(M, red, tot[3], kCoupledRedAdm0, stream));
+        } else {
+            MP_HIP(launch_coupled_batch_clear(M, dev(h_act), na, 1, 2, stream));
+            MP_HIP(launch_coupled_batch_stage(M, stg, tot[4], 1, ddt, stream));
+            MP_HIP(launch_coupled_batch_admissible(M, red, tot[3], kCoupledRedAdm1, stream));
+            MP_HIP(launch_coupled_batch_rhs(M, cell, tot[0], face, tot[1], gath, tot[2], red, tot[3], 1,
+                                            kCoupledRedRhs1, stream));
+            MP_HIP(launch_coupled_batch_stage(M, stg, tot[4], 2, ddt, stream));
+            MP_HIP(launch_coupled_batch_admissible(M, red, tot[3], kCoupledRedAdm2, stream));
+            MP_HIP(launch_coupled_batch_rhs(M, cell, tot[0], face, tot[1], gath, tot[2], red, tot[3], 2,
+                                            kCoupledRedRhs2, stream));
+        }
+        MP_HIP(launch_coupled_batch_readback(M, dev(h_act), na, d_rb_red.as<double>(), d_rb_err.as<uint32_t>(),
+                                             stream));
+        MP_HIP(hipMemcpyAsync(h_rb_red.p, d_rb_red.ptr, sizeof(double) * kCoupledRedSize * size_t(na),
+                              hipMemcpyDeviceToHost, stream));
+        MP_HIP(hipMemcpyAsync(h_rb_err.p, d_rb_err.ptr, sizeof(uint32_t) * kCoupledErrSlots * size_t(na),
+                              hipMemcpyDeviceToHost, stream));
+        MP_HIP(hipStreamSynchronize(stream));
+        for (int a = 0; a < na; ++a) {
+            mp_coupled& h = *hs[act[a]];
+            std::memcpy(h.host_red, static_cast<double*>(h_rb_red.p) + size_t(a) * kCoupledRedSize,
+                        sizeof(double) * kCoupledRedSize);
+            std::memcpy(h.host_err, static_cast<uint32_t*>(h_rb_err.p) + size_t(a) * kCoupledErrSlots,
+                        sizeof(uint32_t) * kCoupledErrSlots);
+        }
+        return MP_OK;
+    };
+    auto failed = [&](int k, mp_status st) {
+        ms[k].st = st;
+        ms[k].msg = last_error();
+        ms[k].phase = BatchMeshState::kFailed;
+    };
+
+    // the starting RHS and state check (mp_coupled::run before its loop)
+    std::vector<int> act;
+    for (int k = 0; k < n; ++k)
+        if (ms[k].phase == BatchMeshState::kRunning) {
+            act.push_back(k);
+            seg_pending.push_back(k);
+            seg_pending.push_back(1);      // O <- S
+        }
+    MP_CALL(round(act, true));
+    for (int k : act) {
+        BatchMeshState& m = ms[k];
+        mp_status st = hs[k]->rhs_result(0, kCoupledRedRhs0, m.current);
+        if (st) {
+            failed(k, st);
+            continue;
+        }
+        bool ok;
+        hs[k]->admissible_result(kCoupledRedAdm0, ok, m.oldH, m.oldC);
+    }
+    for (;;) {
+        act.clear();
+        for (int k = 0; k < n; ++k) {
+            BatchMeshState& m = ms[k];
+            if (m.phase != BatchMeshState::kRunning) continue;
+            const mp_coupled_time& tc = tcs[k];
+            if (m.new_step) {
+                if (m.s >= nsteps) {
+                    m.phase = BatchMeshState::kDone;
+                    continue;
+                }
+                m.remaining = tc.end_time - times[k];
+                if (m.remaining <= 32 * 2.220446049250313e-16 * std::fmax(1.0, std::fabs(tc.end_time))) {
+                    times[k] = tc.end_time;
+                    m.phase = BatchMeshState::kDone;
+                    continue;
+                }
+                m.start = times[k];
+                m.dt = std::fmin(m.remaining, tc.fixed ? tc.initial_step
+                                                        : (m.accepted == 0 ? tc.initial_step : tc.maximum_step));
+                const double limit = m.current.maximum_rate > 0 ? tc.cfl / m.current.maximum_rate : tc.maximum_step;
+                if (tc.fixed && !(m.dt <= limit * (1 + 1e-12))) {
+                    failed(k, fail(MP_ERR_ARG, "Fixed time step exceeds the phase-coupled CFL bound; use control: cfl"));
+                    continue;
+                }
+                if (!tc.fixed) m.dt = std::fmin(m.dt, limit);
+                seg_pending.push_back(k);
+                seg_pending.push_back(0);  // F <- R
+                m.first = m.current;
+                m.retries = 0;
+                m.new_step = false;
+            }
+            if (!(m.dt >= std::fmin(m.remaining, tc.minimum_step) && m.retries < 40)) {
+                failed(k, fail(MP_ERR_PHASE, "Coupled step cannot satisfy admissibility/CFL above minimum step"));
+                times[k] = m.start;        // roll back to the latest accepted state
+                seg_pending.push_back(k);
+                seg_pending.push_back(2);  // S <- O
+                continue;
+            }
+            act.push_back(k);
+        }
+        if (act.empty()) break;
+        MP_CALL(round(act, false));
+        for (int k : act) {
+            BatchMeshState& m = ms[k];
+            const mp_coupled_time& tc = tcs[k];
+            mp_coupled& h = *hs[k];
+            times[k] = m.start + m.dt;
+            bool ok;
+            double ih, ic;
+            mp_status st = MP_OK;
+            h.admissible_result(kCoupledRedAdm1, ok, ih, ic);
+            if (ok) {
+                st = h.rhs_result(1, kCoupledRedRhs1, m.second);
+                if (!st) {
+                    m.courant = m.dt * std::fmax(m.current.maximum_rate, m.second.maximum_rate);
+                    ok = m.courant <= tc.cfl * (1 + 1e-12);
+                }
+            }
+            if (!st && ok) h.admissible_result(kCoupledRedAdm2, ok, m.newH, m.newC);
+            if (!st && !ok) {
+                if (!tc.fixed) {           // retry at half the step next round
+                    m.dt *= .5;
+                    ++m.retries;
+                    continue;
+                }
+                st = fail(MP_ERR_ARG, "Fixed coupled step failed stage admissibility/CFL");
+            }
+            double balH = 0, balC = 0;
+            if (!st) {
+                balH = m.newH - m.oldH + .5 * m.dt * (m.current.boundary_H + m.second.boundary_H -
+                                                      m.current.source_H - m.second.source_H);
+                balC = m.newC - m.oldC + .5 * m.dt * (m.current.boundary_C + m.second.boundary_C);
+                if (!(std::fabs(balH) < 1e-10 * std::fmax(1.0, std::fabs(m.oldH)) && std::fabs(balC) < 1e-10))
+                    st = fail(MP_ERR_PHASE, "Coupled finite-volume conservation balance failed");
+            }
+            if (!st) st = h.rhs_result(2, kCoupledRedRhs2, m.current);
+            if (st) {
+                failed(k, st);
+                times[k] = m.start;        // roll back to the latest accepted state
+                seg_pending.push_back(k);
+                seg_pending.push_back(2);  // S <- O
+                continue;
+            }
+            ++m.accepted;
+            m.oldH = m.newH;
+            m.oldC = m.newC;
+            seg_pending.push_back(k);
+            seg_pending.push_back(1);      // O <- S
+            if (steps) {
+                mp_coupled_step& r = steps[size_t(k) * nsteps + m.s];
+                r.dt = m.dt;
+                r.courant = m.courant;
+                r.balance_H = balH;
+                r.balance_C = balC;
+                r.boundary_H = .5 * (m.first.boundary_H + m.second.boundary_H);
+                r.boundary_C = .5 * (m.first.boundary_C + m.second.boundary_C);
+                r.source_H = .5 * (m.first.source_H + m.second.source_H);
+                r.maximum_rate = m.current.maximum_rate;
+                r.retries = m.retries;
+                r.pad = 0;
+            }
+            ++m.s;
+            m.new_step = true;
+        }
+    }
+    act.clear();
+    MP_CALL(round(act, false));            // the copies still owed (accepted states, roll-backs)
+    mp_status first_st = MP_OK;
+    int first_k = -1;
+    for (int k = 0; k < n; ++k) {
+        if (accepted_out) accepted_out[k] = ms[k].accepted;
+        if (status_out) status_out[k] = ms[k].st;
+        if (ms[k].st && first_k < 0) {
+            first_st = ms[k].st;
+            first_k = k;
+        }
+    }
+    if (first_k >= 0) return fail(first_st, "coupled handle %d: %s", first_k, ms[first_k].msg.c_str());
+    return MP_OK;
+}
+
 // ===================================================================== C ABI
 
 extern "C" {
@@ -690,6 +1016,13 @@ mp_status mp_coupled_run(mp_coupled* c, const mp_coupled_time* time_controls, do
     if (!c || !time_controls) return fail(MP_ERR_ARG, "null argument");
     if (nsteps < 0) return fail(MP_ERR_ARG, "nsteps must be >= 0");
     MP_GUARD({ return c->run(*time_controls, time, nsteps, steps, accepted); })
+}
+
+mp_status mp_coupled_run_batch(mp_coupled* const* handles, int32_t n, const mp_coupled_time* time_controls,
+                               double* times, int32_t nsteps, mp_coupled_step* steps, int32_t* accepted,
+                               mp_status* status)
+{
+    MP_GUARD({ return coupled_run_batch(handles, n, time_controls, times, nsteps, steps, accepted, status); })
 }
 
 }  // extern "C"

@@ -88,16 +88,13 @@ __device__ __forceinline__ size_t sample_index(const CoupledDev& d, bool vert, s
     return vert ? fq : fq * d.n + k;
 }
 
-}  // namespace
-
-// Launched with 128 threads per block; the bound lets the compiler use the
-// registers that a 1024-thread default would deny (202 VGPRs in the face
-// kernel, no scratch).
-__global__ __launch_bounds__(128) void coupled_cell_kernel(CoupledDev d, const double* __restrict__ H,
-                                    const double* __restrict__ C)
+// The per-thread bodies of the three RHS kernels, the reductions and the stage
+// update, shared by the single-mesh kernels and their batched forms, so a
+// mesh gives the same bits in both.
+__device__ __forceinline__ void cell_body(const CoupledDev& d, uint32_t* error, const double* __restrict__ H,
+                                          const double* __restrict__ C, int c)
 {
     const MeshDev& m = *d.m;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int nx = m.nx, ny = m.ny;
     if (c >= nx * ny) return;
     const size_t N = size_t(nx) * ny;
@@ -155,19 +152,18 @@ __global__ __launch_bounds__(128) void coupled_cell_kernel(CoupledDev d, const d
     d.cell_out[N + c] = T;
     d.cell_out[2 * N + c] = phi;
     d.cell_out[3 * N + c] = srate;
-    if (err) atomicOr(d.error, err);
+    if (err) atomicOr(error, err);
 }
 
 // One face.  Vertical faces f < ny*(nx+1) at x_i (i = f % (nx+1), j = f / (nx+1)),
 // horizontal faces after them at y_j.  Outputs in the canonical direction
 // (+x / +y): flux_H, flux_C, mixture flux, and the rate (same for both sides).
-__global__ __launch_bounds__(128) void coupled_face_kernel(CoupledDev d)
+__device__ __forceinline__ void face_body(const CoupledDev& d, uint32_t* error, int64_t f)
 {
     const MeshDev& m = *d.m;
     const int nx = m.nx, ny = m.ny, nq = m.q;
     const size_t N = size_t(nx) * ny;
     const int64_t nxf = int64_t(ny) * (nx + 1), nyf = int64_t(ny + 1) * nx;
-    int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (f >= nxf + nyf) return;
     const bool vert = f < nxf;
     const int64_t fl = vert ? f : f - nxf;
@@ -334,14 +330,13 @@ __global__ __launch_bounds__(128) void coupled_face_kernel(CoupledDev d)
     d.face[(nxf + nyf) + f] = o * fluxC;
     d.face[2 * (nxf + nyf) + f] = o * fluxMix;
     d.face[3 * (nxf + nyf) + f] = rate;
-    if (err) atomicOr(d.error, err);
+    if (err) atomicOr(error, err);
 }
 
-__global__ void coupled_gather_kernel(CoupledDev d, double* rhsH, double* rhsC, double* rate,
-                                      double* div, double* bnd)
+__device__ __forceinline__ void gather_body(const CoupledDev& d, double* rhsH, double* rhsC, double* rate,
+                                            double* div, double* bnd, int c)
 {
     const MeshDev& m = *d.m;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int nx = m.nx, ny = m.ny;
     if (c >= nx * ny) return;
     const size_t N = size_t(nx) * ny;
@@ -365,6 +360,28 @@ __global__ void coupled_gather_kernel(CoupledDev d, double* rhsH, double* rhsC, 
     bnd[c] = bh;
     bnd[N + c] = bc;
     bnd[2 * N + c] = d.cell_out[c] * m.area;
+}
+
+}  // namespace
+
+// Launched with 128 threads per block; the bound lets the compiler use the
+// registers that a 1024-thread default would deny (202 VGPRs in the face
+// kernel, no scratch).
+__global__ __launch_bounds__(128) void coupled_cell_kernel(CoupledDev d, const double* __restrict__ H,
+                                    const double* __restrict__ C)
+{
+    cell_body(d, d.error, H, C, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(128) void coupled_face_kernel(CoupledDev d)
+{
+    face_body(d, d.error, int64_t(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
+__global__ void coupled_gather_kernel(CoupledDev d, double* rhsH, double* rhsC, double* rate,
+                                      double* div, double* bnd)
+{
+    gather_body(d, rhsH, rhsC, rate, div, bnd, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Fixed-order reductions of k arrays of n values: sum (Neumaier), or max |x|,
@@ -409,13 +426,13 @@ __device__ void block_reduce(int mode, double& s, double& e, double* ss, double*
 // of all k arrays in flight at once (k <= kReduceMax).
 constexpr int kReduceMax = 6;
 
-__global__ __launch_bounds__(256) void reduce_partial_kernel(const double* __restrict__ x, int64_t n,
-                                                             int k, const int* kind, const int* src,
-                                                             double* part)
+// Block b of nb reduces chunk b of every array into part.
+__device__ __forceinline__ void reduce_partial_body(const double* __restrict__ x, int64_t n, int k,
+                                                    const int* kind, const int* src, double* part,
+                                                    int nb, int b, double* ss, double* se)
 {
-    __shared__ double ss[256], se[256];
-    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-    const int64_t b0 = int64_t(blockIdx.x) * chunk, b1 = min(n, b0 + chunk);
+    const int64_t chunk = (n + nb - 1) / nb;
+    const int64_t b0 = int64_t(b) * chunk, b1 = min(n, b0 + chunk);
     const double* v[kReduceMax];
     int mode[kReduceMax];
     double s[kReduceMax], e[kReduceMax];
@@ -444,16 +461,23 @@ __global__ __launch_bounds__(256) void reduce_partial_kernel(const double* __res
         double sa = s[a], ea = e[a];
         block_reduce(mode[a], sa, ea, ss, se);
         if (threadIdx.x == 0) {
-            part[(size_t(a) * gridDim.x + blockIdx.x) * 2] = sa;
-            part[(size_t(a) * gridDim.x + blockIdx.x) * 2 + 1] = ea;
+            part[(size_t(a) * nb + b) * 2] = sa;
+            part[(size_t(a) * nb + b) * 2 + 1] = ea;
         }
     }
 }
 
-__global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restrict__ part, int nb,
-                                                           int k, const int* kind, double* out)
+__global__ __launch_bounds__(256) void reduce_partial_kernel(const double* __restrict__ x, int64_t n,
+                                                             int k, const int* kind, const int* src,
+                                                             double* part)
 {
     __shared__ double ss[256], se[256];
+    reduce_partial_body(x, n, k, kind, src, part, int(gridDim.x), int(blockIdx.x), ss, se);
+}
+
+__device__ __forceinline__ void reduce_final_body(const double* __restrict__ part, int nb, int k,
+                                                  const int* kind, double* out, double* ss, double* se)
+{
     int mode[kReduceMax];
     double s[kReduceMax], e[kReduceMax];
 #pragma unroll
@@ -478,13 +502,19 @@ __global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restr
     }
 }
 
+__global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restrict__ part, int nb,
+                                                           int k, const int* kind, double* out)
+{
+    __shared__ double ss[256], se[256];
+    reduce_final_body(part, nb, k, kind, out, ss, se);
+}
+
 // SSPRK2 stage combinations of src/couple/evolve.cpp:469-479:
 // stage 1: u = u_old + dt*r1;  stage 2: u = 0.5*u_old + 0.5*(u + dt*r2).
-__global__ void coupled_stage_kernel(int64_t n, int stage, double dt, const double* __restrict__ oldH,
-                                     const double* __restrict__ oldC, const double* __restrict__ rH,
-                                     const double* __restrict__ rC, double* H, double* C)
+__device__ __forceinline__ void stage_body(int64_t n, int stage, double dt, const double* __restrict__ oldH,
+                                           const double* __restrict__ oldC, const double* __restrict__ rH,
+                                           const double* __restrict__ rC, double* H, double* C, int64_t k)
 {
-    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     if (stage == 1) {
         H[k] = dt * rH[k] + oldH[k];
@@ -494,6 +524,13 @@ __global__ void coupled_stage_kernel(int64_t n, int stage, double dt, const doub
         H[k] = .5 * oldH[k] + .5 * h;
         C[k] = .5 * oldC[k] + .5 * c;
     }
+}
+
+__global__ void coupled_stage_kernel(int64_t n, int stage, double dt, const double* __restrict__ oldH,
+                                     const double* __restrict__ oldC, const double* __restrict__ rH,
+                                     const double* __restrict__ rC, double* H, double* C)
+{
+    stage_body(n, stage, dt, oldH, oldC, rH, rC, H, C, int64_t(blockIdx.x) * blockDim.x + threadIdx.x);
 }
 
 hipError_t launch_coupled_rhs(const CoupledDev& d, int nx, int ny, const double* H, const double* C,
@@ -508,11 +545,16 @@ hipError_t launch_coupled_rhs(const CoupledDev& d, int nx, int ny, const double*
     return hipGetLastError();
 }
 
+int coupled_reduce_blocks(int64_t n)
+{
+    return int(std::min<int64_t>(kReducePartials, std::max<int64_t>(1, (n + 2047) / 2048)));
+}
+
 hipError_t launch_reduce(const double* x, int64_t n, int k, const int* kind, const int* src,
                          double* out, double* part, hipStream_t stream)
 {
     if (k < 1 || k > kReduceMax) return hipErrorInvalidValue;
-    const int nb = int(std::min<int64_t>(kReducePartials, std::max<int64_t>(1, (n + 2047) / 2048)));
+    const int nb = coupled_reduce_blocks(n);
     reduce_partial_kernel<<<nb, 256, 0, stream>>>(x, n, k, kind, src, part);
     reduce_final_kernel<<<1, 256, 0, stream>>>(part, nb, k, kind, out);
     return hipGetLastError();
@@ -524,6 +566,171 @@ hipError_t launch_coupled_stage(int64_t n, int stage, double dt, const double* o
 {
     coupled_stage_kernel<<<unsigned((n + 255) / 256), 256, 0, stream>>>(n, stage, dt, oldH, oldC,
                                                                         rH, rC, H, C);
+    return hipGetLastError();
+}
+
+// ============================================================ batched form
+// Each kernel covers the workgroups of every handle of its span; a workgroup
+// finds its handle by bisecting the span's base table (wave-uniform scalar
+// loads) and runs the single-mesh body on its local index.
+
+namespace {
+
+// The k < n with base[k] <= b < base[k+1].
+__device__ __forceinline__ int span_of(const int* __restrict__ base, int n, int b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+}  // namespace
+
+__global__ void coupled_batch_clear_kernel(const CoupledBatchMesh* __restrict__ M, const int* __restrict__ act,
+                                           int nact, int first, int count)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nact * count) return;
+    M[act[t / count]].d.error[first + t % count] = 0u;
+}
+
+__global__ void coupled_batch_copy_kernel(const CoupledBatchMesh* __restrict__ M, const int* __restrict__ seg,
+                                          const int* __restrict__ base, int nseg)
+{
+    const int s = span_of(base, nseg, int(blockIdx.x));
+    const CoupledBatchMesh& B = M[seg[2 * s]];
+    const int kind = seg[2 * s + 1];
+    const double* src = kind == 0 ? B.R : (kind == 1 ? B.S : B.O);
+    double* dst = kind == 0 ? B.F : (kind == 1 ? B.O : B.S);
+    const int64_t k = int64_t(int(blockIdx.x) - base[s]) * blockDim.x + threadIdx.x;
+    if (k < 2 * B.N) dst[k] = src[k];
+}
+
+__global__ void coupled_batch_stage_kernel(const CoupledBatchMesh* __restrict__ M, CoupledBatchSpan sp, int stage,
+                                           const double* __restrict__ dt)
+{
+    const int k = span_of(sp.base, sp.nact, int(blockIdx.x));
+    const CoupledBatchMesh& B = M[sp.act[k]];
+    const double* r = stage == 1 ? B.F : B.R;
+    stage_body(B.N, stage, dt[k], B.O, B.O + B.N, r, r + B.N, B.S, B.S + B.N,
+               int64_t(int(blockIdx.x) - sp.base[k]) * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(128) void coupled_batch_cell_kernel(const CoupledBatchMesh* __restrict__ M,
+                                                                 CoupledBatchSpan sp, int slot)
+{
+    const int k = span_of(sp.base, sp.nact, int(blockIdx.x));
+    const CoupledBatchMesh& B = M[sp.act[k]];
+    cell_body(B.d, B.d.error + slot, B.S, B.S + B.N, (int(blockIdx.x) - sp.base[k]) * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(128) void coupled_batch_face_kernel(const CoupledBatchMesh* __restrict__ M,
+                                                                 CoupledBatchSpan sp, int slot)
+{
+    const int k = span_of(sp.base, sp.nact, int(blockIdx.x));
+    const CoupledBatchMesh& B = M[sp.act[k]];
+    face_body(B.d, B.d.error + slot, int64_t(int(blockIdx.x) - sp.base[k]) * blockDim.x + threadIdx.x);
+}
+
+__global__ void coupled_batch_gather_kernel(const CoupledBatchMesh* __restrict__ M, CoupledBatchSpan sp)
+{
+    const int k = span_of(sp.base, sp.nact, int(blockIdx.x));
+    const CoupledBatchMesh& B = M[sp.act[k]];
+    double* b = B.bnd;
+    const int64_t N = B.N;
+    gather_body(B.d, B.R, B.R + N, b + 3 * N, b + 4 * N, b,
+                (int(blockIdx.x) - sp.base[k]) * blockDim.x + threadIdx.x);
+}
+
+// which 0: the RHS totals of bnd (5 arrays), 1: the state checks of S (6)
+__global__ __launch_bounds__(256) void coupled_batch_reduce_partial_kernel(const CoupledBatchMesh* __restrict__ M,
+                                                                           CoupledBatchSpan sp, int which)
+{
+    __shared__ double ss[256], se[256];
+    const int k = span_of(sp.base, sp.nact, int(blockIdx.x));
+    const CoupledBatchMesh& B = M[sp.act[k]];
+    if (which == 0)
+        reduce_partial_body(B.bnd, B.N, 5, B.kinds, nullptr, B.part, B.nb, int(blockIdx.x) - sp.base[k], ss, se);
+    else
+        reduce_partial_body(B.S, B.N, 6, B.kinds + 5, B.kinds + 11, B.part, B.nb, int(blockIdx.x) - sp.base[k],
+                            ss, se);
+}
+
+__global__ __launch_bounds__(256) void coupled_batch_reduce_final_kernel(const CoupledBatchMesh* __restrict__ M,
+                                                                         const int* __restrict__ act, int which,
+                                                                         int off)
+{
+    __shared__ double ss[256], se[256];
+    const CoupledBatchMesh& B = M[act[blockIdx.x]];
+    reduce_final_body(B.part, B.nb, which == 0 ? 5 : 6, which == 0 ? B.kinds : B.kinds + 5, B.red + off, ss, se);
+}
+
+__global__ void coupled_batch_readback_kernel(const CoupledBatchMesh* __restrict__ M, const int* __restrict__ act,
+                                              int nact, double* __restrict__ out_red, uint32_t* __restrict__ out_err)
+{
+    const int k = blockIdx.x;
+    if (k >= nact) return;
+    const CoupledBatchMesh& B = M[act[k]];
+    for (int t = threadIdx.x; t < kCoupledRedSize; t += blockDim.x) out_red[size_t(k) * kCoupledRedSize + t] = B.red[t];
+    if (int(threadIdx.x) < kCoupledErrSlots) out_err[size_t(k) * kCoupledErrSlots + threadIdx.x] = B.d.error[threadIdx.x];
+}
+
+hipError_t launch_coupled_batch_clear(const CoupledBatchMesh* M, const int* act, int nact, int first, int count,
+                                      hipStream_t stream)
+{
+    if (nact <= 0) return hipSuccess;
+    coupled_batch_clear_kernel<<<(nact * count + 255) / 256, 256, 0, stream>>>(M, act, nact, first, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_coupled_batch_copy(const CoupledBatchMesh* M, const int* seg, const int* base, int nseg,
+                                     int blocks, hipStream_t stream)
+{
+    if (nseg <= 0) return hipSuccess;
+    coupled_batch_copy_kernel<<<blocks, 256, 0, stream>>>(M, seg, base, nseg);
+    return hipGetLastError();
+}
+
+hipError_t launch_coupled_batch_stage(const CoupledBatchMesh* M, CoupledBatchSpan span, int blocks, int stage,
+                                      const double* dt, hipStream_t stream)
+{
+    if (span.nact <= 0) return hipSuccess;
+    coupled_batch_stage_kernel<<<blocks, 256, 0, stream>>>(M, span, stage, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_coupled_batch_rhs(const CoupledBatchMesh* M, CoupledBatchSpan cell, int cell_blocks,
+                                    CoupledBatchSpan face, int face_blocks, CoupledBatchSpan gather,
+                                    int gather_blocks, CoupledBatchSpan red, int red_blocks, int slot,
+                                    int off, hipStream_t stream)
+{
+    if (cell.nact <= 0) return hipSuccess;
+    coupled_batch_cell_kernel<<<cell_blocks, 128, 0, stream>>>(M, cell, slot);
+    coupled_batch_face_kernel<<<face_blocks, 128, 0, stream>>>(M, face, slot);
+    coupled_batch_gather_kernel<<<gather_blocks, 256, 0, stream>>>(M, gather);
+    coupled_batch_reduce_partial_kernel<<<red_blocks, 256, 0, stream>>>(M, red, 0);
+    coupled_batch_reduce_final_kernel<<<red.nact, 256, 0, stream>>>(M, red.act, 0, off);
+    return hipGetLastError();
+}
+
+hipError_t launch_coupled_batch_admissible(const CoupledBatchMesh* M, CoupledBatchSpan red, int red_blocks,
+                                           int off, hipStream_t stream)
+{
+    if (red.nact <= 0) return hipSuccess;
+    coupled_batch_reduce_partial_kernel<<<red_blocks, 256, 0, stream>>>(M, red, 1);
+    coupled_batch_reduce_final_kernel<<<red.nact, 256, 0, stream>>>(M, red.act, 1, off);
+    return hipGetLastError();
+}
+
+hipError_t launch_coupled_batch_readback(const CoupledBatchMesh* M, const int* act, int nact, double* out_red,
+                                         uint32_t* out_err, hipStream_t stream)
+{
+    if (nact <= 0) return hipSuccess;
+    coupled_batch_readback_kernel<<<nact, 64, 0, stream>>>(M, act, nact, out_red, out_err);
     return hipGetLastError();
 }
 
