@@ -49,6 +49,8 @@ def parse():
                    help="steps of the FP64 bound-limiter leg (0: skip)")
     p.add_argument("--coupled-steps", type=int, default=3,
                    help="SSPRK2 steps of the coupled-law leg (0: skip)")
+    p.add_argument("--coupled-batch", type=int, default=64,
+                   help="meshes of the batched coupled-law leg (0: skip)")
     p.add_argument("--coupled-nx", type=int, default=1024)
     p.add_argument("--coupled-ny", type=int, default=2048)
     p.add_argument("--no-batch", action="store_true")
@@ -609,6 +611,8 @@ def bench_coupled(args):
            "workload": f"{spec.nx}x{spec.ny}, flow samples fixed, limiter on, kappa 2e-4, "
                        "cooling 0.05, Dirichlet bottom temperature, bottom inflow, top outflow"}
     log(f"coupled: {out['cell_steps_per_s']:.3e} cell-steps/s ({len(steps)} SSPRK2 steps)")
+    if args.coupled_batch > 0:
+        out["batch"] = bench_coupled_batch(args)
     if not args.no_cpu:
         try:
             import oracle as O
@@ -630,6 +634,68 @@ def bench_coupled(args):
                                "sample": f"64x64, {len(st)} steps", "seconds": ce}
         except Exception as e:   # reported, never required
             out["cpu_port"] = {"error": str(e)}
+    return out
+
+
+def bench_coupled_batch(args):
+    """The coupled law on many small meshes (the batch size law): every handle
+    advanced by mp_coupled_run_batch, one launch per kernel over all of them
+    and one read-back per step attempt, against the same handles run one after
+    another (mp_coupled_run each), from the same states."""
+    import mantlepar_amd as M
+    from mantlepar_amd import workloads as W
+    specs = W.batch_specs(args.coupled_batch, seed=11)
+    hs, states = [], []
+    for sp in specs:
+        bc, inflow, tt, tv = W.coupled_boundary(sp)
+        g = M.Coupled(sp.nx, sp.ny, sp.x0, sp.x1, sp.y0, sp.y1, q=sp.q,
+                      phase=W.config_phase(hypereutectic=0),
+                      params=M.default_coupled(theta=1.0, cooling=0.05, thermal_diffusivity=2e-4))
+        g.set_flow(W.coupled_flow(sp))
+        g.set_boundary(0, bc)
+        g.set_boundary(1, bc)
+        g.set_inflow(*inflow)
+        g.set_thermal_boundary(tt, tv)
+        H, Cc = W.coupled_state(sp)
+        g.upload(H, Cc)
+        hs.append(g)
+        states.append((H, Cc))
+    tc = dict(cfl=0.5, initial=1e-3, maximum=1.0)
+    nsteps = args.coupled_steps
+    cells = sum(sp.cells for sp in specs)
+    M.coupled_run_batch(hs, 1, tc)                          # warm-up
+    for g, (H, Cc) in zip(hs, states):
+        g.upload(H, Cc)
+    t0 = time.perf_counter()
+    res = M.coupled_run_batch(hs, nsteps, tc)
+    el_b = time.perf_counter() - t0
+    done_b = sum(len(r[2]) * sp.cells for r, sp in zip(res, specs))
+    ends = [g.download() for g in hs]
+    for g, (H, Cc) in zip(hs, states):
+        g.upload(H, Cc)
+    t0 = time.perf_counter()
+    done_s, same = 0, True
+    for k, (g, sp) in enumerate(zip(hs, specs)):
+        try:
+            _, rec = g.run(nsteps, **tc)
+            done_s += len(rec) * sp.cells
+        except M.MantleError:
+            pass
+    el_s = time.perf_counter() - t0
+    for g, (Hb, Cb) in zip(hs, ends):
+        Hs, Cs = g.download()
+        same &= bool(np.array_equal(Hs, Hb) and np.array_equal(Cs, Cb))
+    for g in hs:
+        g.close()
+    out = {"meshes": len(specs), "cells": cells, "steps": nsteps,
+           "failed": sum(1 for r in res if r[0] != 0),
+           "batched": {"seconds": el_b, "cell_steps_per_s": done_b / el_b},
+           "one_by_one": {"seconds": el_s, "cell_steps_per_s": done_s / el_s},
+           "speedup": el_s / el_b, "states_bitwise_equal": same,
+           "workload": f"{len(specs)} meshes of the batch size law (seed 11), the coupled workload "
+                       "of the single-mesh leg on each"}
+    log(f"coupled batch: {out['batched']['cell_steps_per_s']:.3e} cell-steps/s batched, "
+        f"{out['one_by_one']['cell_steps_per_s']:.3e} one by one (x{out['speedup']:.2f}), equal {same}")
     return out
 
 
