@@ -23,6 +23,7 @@
 #include "mp_launch.h"
 #include "mp_limit.h"
 #include "mp_phase.h"
+#include "mp_phase_items.h"
 #include "mp_recon.h"
 
 namespace mp {
@@ -1043,36 +1044,6 @@ __global__ void state_phase_kernel(const MeshDev* __restrict__ mesh, const doubl
     const PhaseOut o = phase_eval(phase_at(m.ph, m.tep_row, c / m.nx), H[c], C[c], err);
     T[c] = o.T; phi[c] = o.phil; Cl[c] = o.cl; Cs[c] = o.cs; region[c] = uint8_t(o.region);
     if (err) atomicOr(error, err);
-}
-
-// Phase of a stage's output and its state sums, per work item of the stage
-// sweep: lane = column, rows in order, the same butterfly -- the values the
-// phase-fused sweep produces, bit for bit.  The mixed path runs its last
-// stage without the phase epilogue and this kernel after it: the sweep keeps
-// its fast interior form, and the phase work streams at full occupancy.
-__device__ __forceinline__ void phase_item(const MeshDev& m, const StagePtrs& sp, int item,
-                                           int strip, int j0, int j1)
-{
-    const int lane = threadIdx.x & (kWave - 1);
-    const int i = strip * kStripCells - 1 + lane;
-    const bool owned = i >= 0 && i < m.nx && lane >= 1 && lane <= kStripCells;
-    double sH = 0, eH = 0, sC = 0, eC = 0;
-    uint32_t err = 0;
-    if (owned) {
-        for (int r = j0; r < j1; ++r) {
-            const size_t k = size_t(r) * m.nx + i;
-            const double h = sp.Ho[k], c = sp.Co[k];
-            const PhaseOut po = phase_eval<false>(phase_at(m.ph, m.tep_row, r), h, c, err);
-            sp.T[k] = po.T;
-            sp.phi[k] = po.phil;
-            sp.Cl[k] = po.cl;
-            sp.Cs[k] = po.cs;
-            sp.region[k] = uint8_t(po.region);
-            two_sum_acc(sH, eH, h);
-            two_sum_acc(sC, eC, c);
-        }
-    }
-    item_partials<true>(sp, item, sH, eH, sC, eC, 0.0, 0.0, err);
 }
 
 // One wave per workgroup and per item, in the sweep's dispatch order (10%

@@ -98,9 +98,13 @@ __host__ __device__ inline int limit_mode32(const MeshDev& m)
 // each wait giving up after spin_limit polls.
 constexpr int kSpinLimit = 1 << 20;
 constexpr uint32_t kEpochNever = 0xffffffffu;   // no flag ever holds it
+// ptab (device memory, may be null) and nphase > 0: the launch also runs the
+// nphase phase items of the stage record *ptab (the previous step's last
+// stage) after its sweep items.
 hipError_t launch_step32(const MeshDev* d_mesh, const StagePtrs* tab, int nitems, int nstages,
                          uint32_t* flags, uint32_t epoch, uint32_t wait_epoch, int spin_limit,
-                         hipStream_t stream, bool diffusion, bool global_lf);
+                         hipStream_t stream, bool diffusion, bool global_lf,
+                         const StagePtrs* ptab = nullptr, int nphase = 0);
 hipError_t launch_batch_stage32(const MeshDev* meshes, const StagePtrs* stages, const int2* items,
                                 int nitems, int blocks, int* queue, bool global_lf, hipStream_t stream,
                                 int limit = 0, bool diffusion = false);

@@ -27,6 +27,7 @@
 #include "mp_common.h"
 #include "mp_device.h"
 #include "mp_launch.h"
+#include "mp_phase_items.h"
 #include "mp_recon.h"
 
 namespace mp {
@@ -746,15 +747,30 @@ __global__ __launch_bounds__(64, LIM == 2 || (LIM != 0 && (DIFF || FULL)) ? 2 : 
 // `wait_epoch`, the same value except in the stall check, where the host
 // passes a value no flag ever holds.  The ramp and drain of a launch are
 // paid once per step instead of once per stage.
+// nphase > 0: nphase more workgroups after the sweep items run the phase items
+// of the previous step's output (*ptab, its last stage record), which this
+// step only reads.  They wait on nothing and are dispatched last, so they run
+// in the slots that the launch's drain leaves idle.
 template <bool DIFF, bool FULL>
 __global__ __launch_bounds__(64, 3) void step32_kernel(const MeshDev* __restrict__ mesh,
                                                        const StagePtrs* __restrict__ tab, int nitems,
                                                        int nstages, uint32_t* __restrict__ flags,
-                                                       uint32_t epoch, uint32_t wait_epoch, int spin_limit)
+                                                       uint32_t epoch, uint32_t wait_epoch, int spin_limit,
+                                                       const StagePtrs* __restrict__ ptab, int nphase)
 {
     const int b = int(blockIdx.x);
     const int stage = b / nitems;
-    if (stage >= nstages) return;
+    if (stage >= nstages) {
+        const int pb = b - nstages * nitems;
+        if (pb >= nphase) return;
+        const StagePtrs& sp = *ptab;
+        const MeshDev& m = *mesh;
+        const int item = item_of_block(sp, pb, nphase);
+        int strip, j0, j1;
+        item_region(sp, m.ny, item, strip, j0, j1);
+        phase_item(m, sp, item, strip, j0, j1);
+        return;
+    }
     const StagePtrs& sp = tab[stage];
     const int item = item_of_block(sp, b - stage * nitems, nitems);
     const MeshDev& m = *mesh;
@@ -883,16 +899,22 @@ hipError_t launch_stage32(const MeshDev* d_mesh, const StagePtrs& sp, int nitems
 
 hipError_t launch_step32(const MeshDev* d_mesh, const StagePtrs* tab, int nitems, int nstages,
                          uint32_t* flags, uint32_t epoch, uint32_t wait_epoch, int spin_limit,
-                         hipStream_t stream, bool diffusion, bool global_lf)
+                         hipStream_t stream, bool diffusion, bool global_lf, const StagePtrs* ptab,
+                         int nphase)
 {
-    const dim3 grid(unsigned(nitems) * unsigned(nstages)), block(kWave);
+    if (!ptab) nphase = 0;
+    const dim3 grid(unsigned(nitems) * unsigned(nstages) + unsigned(nphase)), block(kWave);
+#define MP_STEP32(D, F)                                                                                   \
+    step32_kernel<D, F><<<grid, block, 0, stream>>>(d_mesh, tab, nitems, nstages, flags, epoch, wait_epoch, \
+                                                    spin_limit, ptab, nphase)
     if (global_lf) {
-        if (diffusion) step32_kernel<true, true><<<grid, block, 0, stream>>>(d_mesh, tab, nitems, nstages, flags, epoch, wait_epoch, spin_limit);
-        else step32_kernel<false, true><<<grid, block, 0, stream>>>(d_mesh, tab, nitems, nstages, flags, epoch, wait_epoch, spin_limit);
+        if (diffusion) MP_STEP32(true, true);
+        else MP_STEP32(false, true);
     } else {
-        if (diffusion) step32_kernel<true, false><<<grid, block, 0, stream>>>(d_mesh, tab, nitems, nstages, flags, epoch, wait_epoch, spin_limit);
-        else step32_kernel<false, false><<<grid, block, 0, stream>>>(d_mesh, tab, nitems, nstages, flags, epoch, wait_epoch, spin_limit);
+        if (diffusion) MP_STEP32(true, false);
+        else MP_STEP32(false, false);
     }
+#undef MP_STEP32
     return hipGetLastError();
 }
 

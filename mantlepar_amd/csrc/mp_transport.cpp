@@ -352,7 +352,15 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
     int src = cur;
     final_state = cur;
     const int phase_set = 1 - phase_cur;
-    const bool overlap = precision == MP_PREC_MIXED && side && !events;
+    const bool fused = step_fused(scheme);
+    // One launch per mixed step: each launch also runs the previous step's
+    // phase items after its own sweep items, in the launch's drain, and the
+    // previous step's totals follow it; the last step's phase and totals
+    // close the call.  Per-stage mixed launches run each step's phase and
+    // totals on the side stream, beside the next step.  Profiling (events)
+    // keeps every kernel in its own launch on the main stream.
+    const bool fold = fused && precision == MP_PREC_MIXED && !events;
+    const bool overlap = precision == MP_PREC_MIXED && side && !events && !fused;
     // However the call ends, error returns included, the main stream waits for
     // everything enqueued on the side stream.
     struct Join {
@@ -369,7 +377,6 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
     // the starting state's totals, against which the first step is balanced
     if (nsteps > 0) MP_HIP(launch_state_sums(H[cur].as<double>(), C[cur].as<double>(), int64_t(N), md.area,
                                              w.totals, scratch, stream));
-    const bool fused = step_fused(scheme);
     if (fused) {
         // Stage records of the call's three kinds of step (step 0, odd steps,
         // even steps >= 2), uploaded once; the previous call's upload from the
@@ -377,7 +384,7 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
         MP_HIP(hipEventSynchronize(tab_copied));
         const int src_of[3] = {cur, others[0], others[1]};
         const int dst_of[3] = {others[0], others[1], others[0]};
-        const int par_of[3] = {0, overlap ? 1 : 0, 0};
+        const int par_of[3] = {0, overlap || fold ? 1 : 0, 0};
         for (int set = 0; set < 3; ++set) {
             int kin = src_of[set];
             for (int st = 0; st < scheme; ++st) {
@@ -391,9 +398,17 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
         MP_HIP(hipMemcpyAsync(d_tab.ptr, h_tab, sizeof(StagePtrs) * 9, hipMemcpyHostToDevice, stream));
         MP_HIP(hipEventRecord(tab_copied, stream));
     }
+    const size_t set_doubles_ = set_doubles;
+    auto reduce_on = [&](int step, int par, hipStream_t st) {
+        // totals of step `step` in slot step + 1, checked against slot step
+        // (the preheat's balance, src/couple/preheat.cpp:268-273)
+        return launch_reduce_step(w.partials + size_t(par) * 3 * set_doubles_, nitems, scheme, kW[scheme - 1],
+                                  md.area, w.totals + size_t(step + 1) * 4, scratch, st,
+                                  w.totals + size_t(step) * 4, dt, balance_tol, w.error);
+    };
     for (int s = 0; s < nsteps; ++s) {
         const int dst = others[s % 2];
-        const int par = overlap ? s % 2 : 0;
+        const int par = overlap || fold ? s % 2 : 0;
         if (overlap && s >= 2) MP_HIP(hipStreamWaitEvent(stream, side_done[par], 0));
         if (fused) {
             // one launch for the step's stages, then its phase as below
@@ -405,15 +420,23 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
             const uint32_t ep = ++*w.epoch;
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3)], stream));
             const bool stall = launch_mode == MP_LAUNCH_STALL_CHECK;
+            // the previous step's last stage record: its phase items ride along
+            const int prev = s == 1 ? 0 : (s % 2 ? 2 : 1);
+            const StagePtrs* ptab = fold && s > 0 ? d_tab.as<StagePtrs>() + prev * 3 + scheme - 1 : nullptr;
             MP_HIP(launch_step32(d_md.as<MeshDev>(), d_tab.as<StagePtrs>() + set * 3, nitems, scheme,
                                  w.flags, ep, stall ? kEpochNever : ep, stall ? 64 : kSpinLimit, stream,
-                                 has_diffusion(), md.lf_mode != 0));
+                                 has_diffusion(), md.lf_mode != 0, ptab, ptab ? nitems : 0));
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3) + 1], stream));
             const StagePtrs sp = h_tab[set * 3 + scheme - 1];
-            if (overlap) {
-                MP_HIP(hipEventRecord(stage3[par], stream));
-                MP_HIP(hipStreamWaitEvent(side, stage3[par], 0));
-                MP_HIP(launch_phase_items(d_md.as<MeshDev>(), sp, nitems, side));
+            if (fold) {
+                if (s > 0) MP_HIP(reduce_on(s - 1, 1 - par, stream));
+                if (s == nsteps - 1) {
+                    MP_HIP(launch_phase_items(d_md.as<MeshDev>(), sp, nitems, stream));
+                    MP_HIP(reduce_on(s, par, stream));
+                }
+                src = dst;
+                final_state = dst;
+                continue;
             } else {
                 if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + 1)], stream));
                 MP_HIP(launch_phase_items(d_md.as<MeshDev>(), sp, nitems, stream));
@@ -440,12 +463,7 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
             kin = out;
         }
-        // totals of step s in slot s + 1, checked against slot s (the
-        // preheat's balance, src/couple/preheat.cpp:268-273)
-        MP_HIP(launch_reduce_step(w.partials + size_t(par) * 3 * set_doubles, nitems, scheme,
-                                  kW[scheme - 1], md.area, w.totals + size_t(s + 1) * 4, scratch,
-                                  overlap ? side : stream, w.totals + size_t(s) * 4, dt,
-                                  balance_tol, w.error));
+        MP_HIP(reduce_on(s, par, overlap ? side : stream));
         if (overlap) MP_HIP(hipEventRecord(side_done[par], side));
         src = dst;
         final_state = dst;
