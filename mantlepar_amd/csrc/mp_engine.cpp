@@ -778,17 +778,15 @@ mp_status Engine::reserve_steps(int nsteps)
     return MP_OK;
 }
 
-// The mixed path with per-stage launches runs each step's phase kernel and
-// per-mesh totals on the side stream, beside the next step's stages: the
-// phase reads the step's output states, which the next step only reads, and
-// writes the phase outputs and the sum slots of the step's partials (the
-// parity set of its step).  Step s + 2 writes step s's output states and
-// partial set again, so it first waits for step s's phase and totals; the
-// main stream waits for the side stream at the end, error returns included.
-// The fork and join are events, so a captured graph holds them too.  A step
-// launched whole instead hands its phase items to the next step's launch (see
-// below).  Profiling (events) keeps everything on the main stream, so each
-// launch is timed alone.
+// The mixed path runs each step's phase kernel and per-mesh totals on the
+// side stream, beside the next step's stages: the phase reads the step's
+// output states, which the next step only reads, and writes the phase outputs
+// and the sum slots of the step's partials (the parity set of its step).
+// Step s + 2 writes step s's output states and partial set again, so it first
+// waits for step s's phase and totals; the main stream waits for the side
+// stream at the end, error returns included.  The fork and join are events,
+// so a captured graph holds them too.  Profiling (events) keeps everything on
+// the main stream, so each launch is timed alone.
 mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* events)
 {
     if (nsteps > steps_capacity) return fail(MP_ERR_STATE, "per-step totals not reserved");
@@ -809,75 +807,31 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
     if (nsteps > 0 && !active.empty())
         MP_HIP(launch_batch_state_sums(base, d_meshes.as<MeshDev>(), d_active.as<int>(), int(active.size()),
                                        d_init_totals.as<double>(), d_init_part.as<double>(), stream));
-    auto steppers = [&](int s) {   // slots still stepping at step s: a prefix of the ordered list
-        int p = 0;
-        while (p < int(active.size()) && active_steps[p] > s) ++p;
-        return p;
-    };
-    auto list_of = [&](int p) {
-        const List& Lst = lists.at(p);
-        return blocks > 0 ? std::make_pair(Lst.queue_off, Lst.queue_count)
-                          : std::make_pair(Lst.slab_off, Lst.slab_count);
-    };
-    auto reduce = [&](int s, int set, int p, hipStream_t st) -> hipError_t {
-        const double* prev = s == 0 ? d_init_totals.as<double>()
-                                    : d_totals.as<double>() + size_t(s - 1) * 4 * capacity;
-        return launch_batch_reduce(base + size_t(set * 3) * capacity, d_nitems.as<int>(), d_meshes.as<MeshDev>(),
-                                   d_active.as<int>(), p, scheme, kW[scheme - 1],
-                                   d_totals.as<double>() + size_t(s) * 4 * capacity, st, prev, balance_tol);
-    };
-    // A mixed step launched whole carries the previous step's phase items
-    // after its sweep items (in the launch's drain) when that step was
-    // launched whole too; the previous step's totals follow the launch.  The
-    // phase and totals owed when no such launch follows run on their own.
-    int dfr_s = -1, dfr_set = 0, dfr_p = 0;
-    std::pair<int, int> dfr_L{0, 0};
-    auto flush = [&]() -> mp_status {
-        if (dfr_s < 0) return MP_OK;
-        MP_HIP(launch_batch_phase_items(d_meshes.as<MeshDev>(), base + size_t(dfr_set * 3 + scheme - 1) * capacity,
-                                        d_items.as<int2>() + dfr_L.first, dfr_L.second, stream));
-        MP_HIP(reduce(dfr_s, dfr_set, dfr_p, stream));
-        dfr_s = -1;
-        return MP_OK;
-    };
     for (int s = 0; s < nsteps; ++s) {
         const int set = s == 0 ? 0 : (s % 2 ? 1 : 2);
-        const int p = steppers(s);
+        int p = 0;      // slots still stepping: a prefix of the ordered list
+        while (p < int(active.size()) && active_steps[p] > s) ++p;
         if (p == 0) {
             if (events)
                 for (int st = 0; st < scheme; ++st) {
                     MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
                     MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
                 }
-            MP_CALL(flush());
             continue;
         }
-        const std::pair<int, int> L = list_of(p);
+        const List& Lst = lists.at(p);
+        const std::pair<int, int> L = blocks > 0 ? std::make_pair(Lst.queue_off, Lst.queue_count)
+                                                 : std::make_pair(Lst.slab_off, Lst.slab_count);
         const int par = set == 1 ? 1 : 0;    // the partial set of this step
         if (overlap && s >= 2) MP_HIP(hipStreamWaitEvent(stream, side_done[par], 0));
         const bool one = one_launch_for(L.second, events != nullptr);
         const int lim = limit_mode();
         const bool dif = any_diffusion();
-        if (one) {
-            const bool carry = dfr_s >= 0;
-            MP_HIP(launch_batch_step32(
-                d_meshes.as<MeshDev>(), base + size_t(set * 3) * capacity, capacity, d_items.as<int2>() + L.first,
-                L.second, scheme, d_flags.as<uint32_t>(), d_flag_off.as<int>(), nflags, d_epoch.as<uint32_t>(),
-                kSpinLimit, weno.lf_mode != 0, stream,
-                carry ? base + size_t(dfr_set * 3 + scheme - 1) * capacity : nullptr,
-                carry ? d_items.as<int2>() + dfr_L.first : nullptr, carry ? dfr_L.second : 0));
-            if (carry) {
-                MP_HIP(reduce(dfr_s, dfr_set, dfr_p, stream));
-                dfr_s = -1;
-            }
-        } else {
-            MP_CALL(flush());
-        }
-        bool fold_next = false;
-        if (one && precision == MP_PREC_MIXED && !events && s + 1 < nsteps) {
-            const int pn = steppers(s + 1);
-            fold_next = pn > 0 && one_launch_for(list_of(pn).second, false);
-        }
+        if (one)
+            MP_HIP(launch_batch_step32(d_meshes.as<MeshDev>(), base + size_t(set * 3) * capacity, capacity,
+                                       d_items.as<int2>() + L.first, L.second, scheme, d_flags.as<uint32_t>(),
+                                       d_flag_off.as<int>(), nflags, d_epoch.as<uint32_t>(), kSpinLimit,
+                                       weno.lf_mode != 0, stream));
         for (int st = 0; st < scheme; ++st) {
             // the mixed sweep leaves the phase to its own streaming kernel, as
             // the single-mesh handle does
@@ -889,9 +843,7 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
                                           d_items.as<int2>() + L.first, L.second, blocks,
                                           d_queues.as<int>() + 3 * s + st, stream, weno.lf_mode != 0, lim,
                                           dif));
-            if (split && fold_next) {
-                // owed to the next step's launch
-            } else if (split && overlap) {
+            if (split && overlap) {
                 MP_HIP(hipEventRecord(stage3[par], stream));
                 MP_HIP(hipStreamWaitEvent(side, stage3[par], 0));
                 MP_HIP(launch_batch_phase_items(d_meshes.as<MeshDev>(), sps, d_items.as<int2>() + L.first,
@@ -902,17 +854,14 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
             }
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
         }
-        if (fold_next) {
-            dfr_s = s;
-            dfr_set = set;
-            dfr_p = p;
-            dfr_L = L;
-        } else {
-            MP_HIP(reduce(s, set, p, overlap ? side : stream));
-            if (overlap) MP_HIP(hipEventRecord(side_done[par], side));
-        }
+        const double* prev = s == 0 ? d_init_totals.as<double>()
+                                    : d_totals.as<double>() + size_t(s - 1) * 4 * capacity;
+        MP_HIP(launch_batch_reduce(base + size_t(set * 3) * capacity, d_nitems.as<int>(),
+                                   d_meshes.as<MeshDev>(), d_active.as<int>(), p, scheme,
+                                   kW[scheme - 1], d_totals.as<double>() + size_t(s) * 4 * capacity,
+                                   overlap ? side : stream, prev, balance_tol));
+        if (overlap) MP_HIP(hipEventRecord(side_done[par], side));
     }
-    MP_CALL(flush());
     return MP_OK;
 }
 

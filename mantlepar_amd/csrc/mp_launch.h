@@ -114,14 +114,10 @@ hipError_t launch_batch_stage32(const MeshDev* meshes, const StagePtrs* stages, 
 // flag_off[k] + item].  The launch first advances *epoch_ctr (a one-thread
 // kernel) and its items publish and wait for the new value, so a captured
 // graph uses a new epoch on every replay; flags must hold earlier values.
-// ptab, pitems, npitems > 0: the launch also runs the phase items of the list
-// pitems with the stage records ptab[slot] (the previous step's last stage)
-// after its sweep items.
 hipError_t launch_batch_step32(const MeshDev* meshes, const StagePtrs* tab, int capacity,
                                const int2* items, int nitems, int nstages, uint32_t* flags,
                                const int* flag_off, int nflags, uint32_t* epoch_ctr, int spin_limit,
-                               bool global_lf, hipStream_t stream, const StagePtrs* ptab = nullptr,
-                               const int2* pitems = nullptr, int npitems = 0);
+                               bool global_lf, hipStream_t stream);
 // Phase and state sums of a stage's output over the stage's work items
 // (sp.Ho/sp.Co in, sp.T..region and partial slots 0-3 out).
 hipError_t launch_phase_items(const MeshDev* d_mesh, const StagePtrs& sp, int nitems,
