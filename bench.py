@@ -63,9 +63,10 @@ def parse():
     return p.parse_args()
 
 
-# step32_kernel at 2048x4096, q = 5, 24 rows per item: 810040 KiB read x 2 +
-# 407796 KiB written
-MEASURED_STEP_LAUNCH_BYTES = (810040 * 2 + 407796) * 1024
+# step32_kernel at 2048x4096, q = 5, by rows per item: (FETCH_SIZE, WRITE_SIZE)
+# in KiB per launch; HBM bytes = (FETCH x 2 + WRITE) x 1024 by the copy calibration
+MEASURED_STEP_LAUNCH_KIB = {8: (922321, 411511), 12: (862611, 410324), 16: (842528, 408546),
+                            24: (810040, 407796), 32: (805875, 407424)}
 
 
 def copy_calibration():
@@ -829,10 +830,11 @@ def main():
     # HBM bytes per launch of this kernel on this workload, from two rocprofv3
     # counter runs (FETCH_SIZE x 2, WRITE_SIZE x 1 by a plain 1 GiB copy's
     # counts); the line's own run collects no counters
-    if (spec.nx, spec.ny, spec.q) == (2048, 4096, 5) and mixed["layout"]["one_launch_steps"]:
-        out["roofline"]["traffic"] = MEASURED_STEP_LAUNCH_BYTES
+    kib = MEASURED_STEP_LAUNCH_KIB.get(mixed["rows"])
+    if (spec.nx, spec.ny, spec.q) == (2048, 4096, 5) and mixed["layout"]["one_launch_steps"] and kib:
+        out["roofline"]["traffic"] = (kib[0] * 2 + kib[1]) * 1024
         out["roofline"]["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of step32_kernel, "
-                                             "copy-calibrated, 24 rows per item "
+                                             f"copy-calibrated, {mixed['rows']} rows per item "
                                              "(profiles/step_launch_2048x4096.md)")
     out["roofline"]["practical_copy_GBs"] = copy_gbs
     out["roofline"]["frac_of_practical_copy"] = achieved / copy_gbs if copy_gbs else None
