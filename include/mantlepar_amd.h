@@ -206,9 +206,8 @@ mp_status mp_transport_set_tile(mp_transport* t, int32_t rows_per_item);
  * MP_LAUNCH_PER_STEP runs all stages of a step in one launch, each work item
  * of a stage starting when the items whose output it reads are done;
  * MP_LAUNCH_PER_STAGE runs one launch per stage; MP_LAUNCH_AUTO (the default)
- * takes the per-step launch for meshes that give each wave slot of the GPU
- * fewer than eight work items, where a launch's ramp and drain weigh most.
- * All give bitwise the same results.  FP64 handles, forward Euler and the
+ * takes the per-step launch, the faster one at every mesh size measured
+ * (2048x4096 to 8192^2).  All give bitwise the same results.  FP64 handles, forward Euler and the
  * limiter always launch per stage.  MP_LAUNCH_STALL_CHECK is a diagnostic of
  * the per-step launch's bounded waits: items wait for a completion that is
  * never published and give up after 64 polls, so a step fails with
