@@ -5,7 +5,8 @@
 // non-finite values on the way.  One server thread owns the engine and keeps
 // two epochs queued on the device: while they run it admits staged requests
 // (arrival order, first fit, while slots and the HBM budget allow; uploads on
-// a second stream), then waits for the older epoch, reads its status (copied
+// a second stream, not waited for: a run joins the epochs once its uploads
+// are seen complete), then waits for the older epoch, reads its status (copied
 // on the stream into pinned memory), retires runs that reached their step
 // count and queues the next epoch, which advances every resident run by up to
 // E steps (one launch per stage over all of them).  A run's results are
@@ -36,6 +37,7 @@ struct Request {
     mp_request r{};
     size_t bytes = 0;
     int slot = -1;
+    int rows_now = 12;        // rows per work item, fixed at admission
     int remaining = 0;        // steps not yet queued
     int inflight = 0;         // queued epochs that advance the run
     long double bH = 0, bC = 0;
@@ -150,10 +152,22 @@ struct mp_server {
     std::vector<std::thread> stagers;
     PinnedPool pool;
     // Requests are admitted on `upload` while the queued epochs run on the
-    // engine's stream; the engine's stream waits for `uploaded` before its
-    // next work.  Results of finished runs come back on `download`.
+    // engine's stream; an admitted run joins an epoch only after its uploads
+    // are seen complete.  Results of finished runs come back on `download`.
     hipStream_t upload = nullptr, download = nullptr;
-    hipEvent_t uploaded = nullptr;
+    // Admissions whose uploads are still on their way, oldest first: their
+    // runs hold slots and memory, and become resident once `done` has
+    // completed (their advective rates are then in the pinned `bits`).  The
+    // worker does not wait for the uploads, so it keeps the epochs queued.
+    struct Admission {
+        hipEvent_t done = nullptr;
+        unsigned long long* bits = nullptr;   // [capacity], pinned
+        std::vector<Request*> runs;
+    };
+    std::deque<Admission> admitting;
+    std::vector<unsigned long long*> spare_bits;
+    std::vector<hipEvent_t> spare_done;   // the worker's own; spare_events is shared with the finisher
+    int admitting_runs = 0;
     // Queued epochs (at most two), oldest first; epoch i uses status set i % 2.
     struct Flight {
         int E = 0, set = 0;
@@ -197,8 +211,9 @@ struct mp_server {
         for (std::thread& t : stagers)
             if (t.joinable()) t.join();
         for (hipEvent_t e : spare_events) (void)hipEventDestroy(e);
+        for (unsigned long long* b : spare_bits) (void)hipHostFree(b);
+        for (hipEvent_t e : spare_done) (void)hipEventDestroy(e);
         for (auto& kv : all) delete kv.second;
-        if (uploaded) (void)hipEventDestroy(uploaded);
         for (hipEvent_t e : flight_done)
             if (e) (void)hipEventDestroy(e);
         if (upload) (void)hipStreamDestroy(upload);
@@ -214,6 +229,7 @@ struct mp_server {
     void stage_loop();
     void finish_loop();
     void admit(std::vector<Request*>& batch, int rows_now);
+    void promote(bool wait);
     bool start_epoch();
     void finish_oldest();
     void retire();
@@ -386,29 +402,78 @@ void mp_server::admit(std::vector<Request*>& batch, int rows_now)
         ok.push_back(q);
     }
     if (ok.empty()) return;
-    std::vector<unsigned long long> bits(size_t(eng.capacity), 0ull);
-    hipError_t e = hipMemcpyAsync(bits.data(), eng.d_maxbits.ptr, sizeof(unsigned long long) * eng.capacity,
-                                  hipMemcpyDeviceToHost, eng.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(eng.stream);
-    const double t = now();
-    for (Request* q : ok) {
-        unstage(q);    // its copies have run
-        if (e != hipSuccess) {
+    Admission a;
+    a.runs = std::move(ok);
+    hipError_t e = hipSuccess;
+    if (!spare_bits.empty()) {
+        a.bits = spare_bits.back();
+        spare_bits.pop_back();
+    } else {
+        e = hipHostMalloc(reinterpret_cast<void**>(&a.bits), sizeof(unsigned long long) * eng.capacity,
+                          hipHostMallocDefault);
+    }
+    if (e == hipSuccess) {
+        if (!spare_done.empty()) {
+            a.done = spare_done.back();
+            spare_done.pop_back();
+        } else {
+            e = hipEventCreateWithFlags(&a.done, hipEventDisableTiming);
+        }
+    }
+    // the rates follow the admission's uploads and rate kernels on `upload`
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(a.bits, eng.d_maxbits.ptr, sizeof(unsigned long long) * eng.capacity,
+                           hipMemcpyDeviceToHost, eng.stream);
+    if (e == hipSuccess) e = hipEventRecord(a.done, eng.stream);
+    if (e != hipSuccess) {
+        // nothing to wait for: settle the uploads, then fail the runs
+        (void)hipStreamSynchronize(eng.stream);
+        for (Request* q : a.runs) {
+            unstage(q);
             finish(q, MP_ERR_DEVICE, hipGetErrorString(e));
-            continue;
         }
-        Slot& s = eng.slots[q->slot];
-        std::memcpy(&s.max_rate, &bits[q->slot], sizeof s.max_rate);
-        if (!(s.max_rate > 0)) {
-            finish(q, MP_ERR_STATE, "zero advective rate: the step is unbounded");
-            continue;
+        if (a.bits) spare_bits.push_back(a.bits);
+        if (a.done) spare_done.push_back(a.done);
+        return;
+    }
+    for (Request* q : a.runs) q->rows_now = rows_now;
+    admitting_runs += int(a.runs.size());
+    admitting.push_back(std::move(a));
+}
+
+// Make resident the runs of the admissions whose uploads have completed, in
+// admission order (wait: block for the oldest one first).
+void mp_server::promote(bool wait)
+{
+    while (!admitting.empty()) {
+        Admission& a = admitting.front();
+        hipError_t e = wait ? hipEventSynchronize(a.done) : hipEventQuery(a.done);
+        if (e == hipErrorNotReady) return;
+        wait = false;
+        const double t = now();
+        for (Request* q : a.runs) {
+            unstage(q);    // its copies have run
+            if (e != hipSuccess) {
+                finish(q, MP_ERR_DEVICE, hipGetErrorString(e));
+                continue;
+            }
+            Slot& s = eng.slots[q->slot];
+            std::memcpy(&s.max_rate, &a.bits[q->slot], sizeof s.max_rate);
+            if (!(s.max_rate > 0)) {
+                finish(q, MP_ERR_STATE, "zero advective rate: the step is unbounded");
+                continue;
+            }
+            s.dt = q->r.cfl / s.max_rate;
+            q->res.dt = s.dt;
+            q->res.t_admit = t;
+            eng.set_rows(q->slot, q->rows_now);
+            resident.push_back(q);
+            membership_changed = true;
         }
-        s.dt = q->r.cfl / s.max_rate;
-        q->res.dt = s.dt;
-        q->res.t_admit = t;
-        eng.set_rows(q->slot, rows_now);
-        resident.push_back(q);
-        membership_changed = true;
+        admitting_runs -= int(a.runs.size());
+        spare_bits.push_back(a.bits);
+        spare_done.push_back(a.done);
+        admitting.pop_front();
     }
 }
 
@@ -636,8 +701,10 @@ void mp_server::loop()
             std::unique_lock<std::mutex> lk(mu);
             rows_now = rows;
             auto staged_head = [&] { return !pending.empty() && pending.front()->stage == Request::kStaged; };
-            cv_work.wait(lk, [&] { return stop || staged_head() || !resident.empty() || !flights.empty(); });
-            if (stop && resident.empty() && flights.empty()) {
+            cv_work.wait(lk, [&] {
+                return stop || staged_head() || !resident.empty() || !flights.empty() || !admitting.empty();
+            });
+            if (stop && resident.empty() && flights.empty() && admitting.empty()) {
                 // no stager may still be writing into a request failed here
                 cv_work.wait(lk, [&] { return staging == 0; });
                 for (Request* q : pending) {
@@ -659,7 +726,7 @@ void mp_server::loop()
             }
             // Admission: arrival order over the staged requests, first fit,
             // while slots and budget allow.
-            int free_slots = eng.capacity - int(resident.size());
+            int free_slots = eng.capacity - int(resident.size()) - admitting_runs;
             size_t planned = 0;
             for (auto it = pending.begin(); it != pending.end() && free_slots > 0 && !stop;) {
                 Request* q = *it;
@@ -676,14 +743,18 @@ void mp_server::loop()
         }
         if (!batch.empty()) {
             // the engine works on the upload stream for the admission, so its
-            // synchronisations wait for the uploads only, not for the epochs
+            // synchronisations wait for the uploads only, not for the epochs;
+            // the runs join the epochs once promote() has seen their uploads done
             const hipStream_t main = eng.stream;
             eng.stream = upload;
             admit(batch, rows_now);
             eng.stream = main;
-            if (hipEventRecord(uploaded, upload) == hipSuccess)
-                (void)hipStreamWaitEvent(main, uploaded, 0);
         }
+        // with nothing else to run, wait for the oldest admission
+        bool idle = flights.empty();
+        for (Request* q : resident)
+            if (q->remaining > 0 && q->fail_st == MP_OK) idle = false;
+        promote(idle && !admitting.empty());
         while (flights.size() < 2 && start_epoch()) {
         }
         if (!flights.empty()) finish_oldest();
@@ -724,7 +795,6 @@ mp_status mp_server_create(int32_t max_slots, const mp_weno_options* weno, const
         if (st) { delete s; return st; }
         if (hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&s->download, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&s->uploaded, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&s->flight_done[0], hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&s->flight_done[1], hipEventDisableTiming) != hipSuccess) {
             delete s;
