@@ -105,6 +105,16 @@ struct MeshDev {
     double pL, pS, pC;          // exponents s*r + eta(r): large, small, constant
     double ldL, ldS[4], ldC;    // log of linear weights (-inf when zero)
     int32_t use_constant, pad0;
+    // The same weights in direct form for the FP64 reconstruction: the
+    // exponents are integers, so alpha_k = d_k * (1/(sigma_k + epsA))^p_k by a
+    // reciprocal and multiplications.  A target takes it when every active
+    // denominator lies in [wlo, whi], which keeps each alpha and their sum
+    // normal and finite; wdirect = 0 when the exponents or linear weights are
+    // out of that range for every target.
+    double dL, dS[4];           // linear weights (0 when off)
+    double aC;                  // alpha of the constant candidate (0 when off)
+    double wlo, whi;
+    int32_t ipL, ipS, wdirect, pad1;
     double global_speed;
     // edge quadrature
     double sv[kMaxQ];           // vertical faces: point offsets hy*xi_q

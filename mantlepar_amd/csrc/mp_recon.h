@@ -22,6 +22,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mp_common.h"
 
 namespace mp {
@@ -42,6 +44,28 @@ __device__ __forceinline__ unsigned candidate_mask(int i, int j, int nx, int ny)
     if (il && jb) m |= 8u;    // (-1,-1)
     if (il && jt) m |= 16u;   // (-1,0)
     return m;
+}
+
+// 1/s to within an ulp for a normal s: the hardware reciprocal refined by two
+// Newton steps, each of which doubles the number of correct bits.
+__device__ __forceinline__ double rcp64(double s)
+{
+    double r = __builtin_amdgcn_rcp(s);
+    double e = fma(-s, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-s, r, 1.0);
+    return fma(r, e, r);
+}
+
+// u^p (p >= 1) by repeated squaring; p is the same for every lane.
+__device__ __forceinline__ double ipow(double u, int p)
+{
+    double r = (p & 1) ? u : 1.0;
+    for (p >>= 1; p; p >>= 1) {
+        u *= u;
+        if (p & 1) r *= u;
+    }
+    return r;
 }
 
 template <typename R> __device__ __forceinline__ R wlog(R x);
@@ -123,32 +147,62 @@ __device__ __forceinline__ void blend(const MeshDev& m, const double (&w)[3][3],
         q11[k] = (sx * sy > 0 ? d : -d) * iawbw;
         sig[k + 1] = q10[k] * q10[k] + q01[k] * q01[k] + q11[k] * q11[k] * q11w;
     }
-    // Normalised weights in log space (src/mlweno/reconstruction.cpp:212-265).
-    const R sc = wscale<R>();
-    const R epsA = R(m.epsA);
-    R la[6];
-    la[0] = (mask & 1u) ? R(m.ldL) * sc - R(m.pL) * wlog<R>(sig[0] + epsA) : R(-INFINITY);
+    R om[6];
+    bool direct = false;
+    if constexpr (std::is_same<R, double>::value) {
+        // alpha_k = d_k / (sigma_k + epsA)^p_k with the reference's integer
+        // exponents, normalised: the same weights as the log form below to
+        // rounding, without FP64 log and exp.  Targets with a denominator out
+        // of [wlo, whi] (or not finite) take the log form.
+        if (m.wdirect) {
+            bool ok = true;
+            double sum = 0.0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-        la[k + 1] = (mask & (2u << k)) ? R(m.ldS[k]) * sc - R(m.pS) * wlog<R>(sig[k + 1] + epsA)
-                                        : R(-INFINITY);
-    la[5] = m.use_constant ? R(m.ldC) * sc - R(m.pC) * wlog<R>(epsA) : R(-INFINITY);
-    R mx = la[0];
+            for (int k = 0; k < 5; ++k) {
+                const bool on = k == 0 ? (mask & 1u) : (mask & (2u << (k - 1)));
+                const double s = sig[k] + m.epsA;
+                ok = ok && (!on || (s >= m.wlo && s <= m.whi));
+                om[k] = on ? (k == 0 ? m.dL : m.dS[k - 1]) * ipow(rcp64(s), k == 0 ? m.ipL : m.ipS) : 0.0;
+                sum += om[k];
+            }
+            om[5] = m.aC;
+            sum += om[5];
+            if (ok && sum > 0.0) {
+                const double inv = 1.0 / sum;
 #pragma unroll
-    for (int k = 1; k < 6; ++k) mx = la[k] > mx ? la[k] : mx;
-    if (!(mx > R(-INFINITY)) || !isfinite(mx)) {
-        err |= kErrWeights;
-        mx = R(0);
+                for (int k = 0; k < 6; ++k) om[k] *= inv;
+                direct = true;
+            }
+        }
     }
-    R om[6], sum = R(0);
+    if (!direct) {
+        // Normalised weights in log space (src/mlweno/reconstruction.cpp:212-265).
+        const R sc = wscale<R>();
+        const R epsA = R(m.epsA);
+        R la[6];
+        la[0] = (mask & 1u) ? R(m.ldL) * sc - R(m.pL) * wlog<R>(sig[0] + epsA) : R(-INFINITY);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        om[k] = la[k] > R(-INFINITY) ? wexp<R>(la[k] - mx) : R(0);
-        sum += om[k];
+        for (int k = 0; k < 4; ++k)
+            la[k + 1] = (mask & (2u << k)) ? R(m.ldS[k]) * sc - R(m.pS) * wlog<R>(sig[k + 1] + epsA)
+                                            : R(-INFINITY);
+        la[5] = m.use_constant ? R(m.ldC) * sc - R(m.pC) * wlog<R>(epsA) : R(-INFINITY);
+        R mx = la[0];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) mx = la[k] > mx ? la[k] : mx;
+        if (!(mx > R(-INFINITY)) || !isfinite(mx)) {
+            err |= kErrWeights;
+            mx = R(0);
+        }
+        R sum = R(0);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            om[k] = la[k] > R(-INFINITY) ? wexp<R>(la[k] - mx) : R(0);
+            sum += om[k];
+        }
+        const R inv = R(1) / sum;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) om[k] *= inv;
     }
-    const R inv = R(1) / sum;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) om[k] *= inv;
     if (sigma_out) {
 #pragma unroll
         for (int k = 0; k < 5; ++k) {

@@ -290,6 +290,30 @@ mp_status make_mesh_dev(const mp_mesh_desc& mesh, const mp_weno_options& weno,
     for (int k = 0; k < 4; ++k) m.ldS[k] = log_weight(weno.small_weight[k]);
     m.use_constant = (weno.use_constant && weno.constant_linear_weight > 0) ? 1 : 0;
     m.ldC = log_weight(weno.constant_linear_weight);
+    {
+        // Direct form: with linear weights in [2^-64, 2^64] and denominators in
+        // [2^-B, 2^B], every alpha lies in [2^-(pL*B + 64), 2^(pL*B + 64)]
+        // (pL is the largest exponent), so B = (1000 - 64) / pL keeps the
+        // alphas and their sum of at most six normal and finite.
+        m.dL = weno.large_weight;
+        for (int k = 0; k < 4; ++k) m.dS[k] = weno.small_weight[k];
+        m.ipL = weno.s <= 20 ? weno.s * 3 + eta(3) : 0;
+        m.ipS = weno.s <= 20 ? weno.s * 2 + eta(2) : 0;
+        const int B = m.ipL > 0 ? (1000 - 64) / m.ipL : 0;
+        m.wlo = std::ldexp(1.0, -B);
+        m.whi = std::ldexp(1.0, B);
+        auto weight_ok = [](double d) { return d == 0 || (d >= 0x1p-64 && d <= 0x1p64); };
+        bool ok = m.ipL > 0 && weight_ok(m.dL);
+        for (int k = 0; k < 4; ++k) ok = ok && weight_ok(m.dS[k]);
+        m.aC = 0;
+        if (ok && m.use_constant) {
+            const long double a = (long double)weno.constant_linear_weight *
+                                  powl((long double)m.epsA, -(long double)(weno.s + eta(1)));
+            ok = a >= 0x1p-1000L && a <= 0x1p1000L;
+            m.aC = ok ? (double)a : 0;
+        }
+        m.wdirect = ok ? 1 : 0;
+    }
     double xq[kMaxQ], wq[kMaxQ];
     gauss_legendre(mesh.q, xq, wq);
     for (int q = 0; q < mesh.q; ++q) {
