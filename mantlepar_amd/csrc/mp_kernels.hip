@@ -79,6 +79,30 @@ __device__ __forceinline__ void llf_pair(const MeshDev& m, const double* vn, siz
     FC = fc;
 }
 
+// llf_pair on the face's samples already in registers: the sweep issues a
+// row's interior samples before its reconstruction, so their load latency is
+// covered by the reconstruction instead of being waited for at the flux.
+template <int NQ>
+__device__ __forceinline__ void llf_pair_at(const MeshDev& m, const double (&vq)[NQ], const double* s,
+                                            const double* W, double tLH, const double (&cLH)[3], double tRH,
+                                            const double (&cRH)[3], double tLC, const double (&cLC)[3],
+                                            double tRC, const double (&cRC)[3], double& FH, double& FC)
+{
+    double fh = 0.0, fc = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double v = vq[q];
+        const double alpha = m.lf_mode ? m.global_speed : fabs(v);
+        const double sq = s[q], w = W[q];
+        const double uLH = tLH + line_at<double>(cLH, sq), uRH = tRH + line_at<double>(cRH, sq);
+        const double uLC = tLC + line_at<double>(cLC, sq), uRC = tRC + line_at<double>(cRC, sq);
+        fh += w * (0.5 * (v * uLH + v * uRH - alpha * (uRH - uLH)));
+        fc += w * (0.5 * (v * uLC + v * uRC - alpha * (uRC - uLC)));
+    }
+    FH = fh;
+    FC = fc;
+}
+
 // Both fields through one interior face from the precomputed flux moments:
 // the first-order part t*V in FP64, the high-order correction in FP32.
 __device__ __forceinline__ void moment_pair(double Vp, double Vm, float Mp1, float Mp2, float Mm1,
@@ -166,6 +190,18 @@ __device__ __forceinline__ void interior_pair(const MeshDev& m, bool vertical, s
     if constexpr (DIFF) FH = fma(vertical ? m.dif_x : m.dif_y, tLH - tRH, FH);
 }
 
+// interior_pair of the FP64 sweep on samples already in registers.
+template <int NQ, bool DIFF>
+__device__ __forceinline__ void interior_pair_at(const MeshDev& m, bool vertical, const double (&vq)[NQ],
+                                                 double tLH, const double (&cLH)[3], double tRH,
+                                                 const double (&cRH)[3], double tLC, const double (&cLC)[3],
+                                                 double tRC, const double (&cRC)[3], double& FH, double& FC)
+{
+    llf_pair_at<NQ>(m, vq, vertical ? m.sv : m.sh, vertical ? m.wv : m.wh, tLH, cLH, tRH, cRH, tLC, cLC,
+                    tRC, cRC, FH, FC);
+    if constexpr (DIFF) FH = fma(vertical ? m.dif_x : m.dif_y, tLH - tRH, FH);
+}
+
 // LIM: apply the bound limiter to both fields' reconstructions (FP64 only).
 // DIFF: the mesh has thermal diffusion or thermal boundary data; without it
 // the diffusion terms are compiled out, which keeps the sweep within its
@@ -247,6 +283,22 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
                 for (int k = 0; k < 3; ++k) { LH.c[f][k] = R(0); LC.c[f][k] = R(0); }
         }
         const double tH = wH[1][1], tC = wC[1][1];
+        // With the point count fixed, this row's interior face samples (bottom
+        // face y_r and right face x_{i+1}) are issued together here, from
+        // clamped in-bounds addresses, and waited for once; boundary faces and
+        // lanes without an interior face read them but do not use them.  (Held
+        // across the reconstruction instead, they spill.)
+        constexpr int NV = (MODE == 0 && NQ > 0) ? NQ : 1;
+        double vbot[NV], vrgt[NV];
+        if constexpr (MODE == 0 && NQ > 0) {
+            const auto* by = gmem(m.vny) + size_t(clampi(r, 0, ny)) * nx + ic;
+            const auto* bx = gmem(m.vnx) + size_t(clampi(r, 0, ny - 1)) * (nx + 1) + clampi(i + 1, 0, nx);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                vbot[q] = by[q * vys];
+                vrgt[q] = bx[q * vxs];
+            }
+        }
 
         // Horizontal face y_r: top of row r-1, bottom of row r.
         double FvH = 0.0, FvC = 0.0;
@@ -270,6 +322,9 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
                 const double oC = boundary_flux<R>(m, vn, vys, 1.0, tTopC, topC, m.sh, m.wh, pol, inC, err);
                 FvH = oH; FvC = oC;
                 if (owned) { bH += oH; bC += oC; }
+            } else if constexpr (MODE == 0 && NQ > 0) {
+                interior_pair_at<NQ, DIFF>(m, false, vbot, tTopH, topH, tH, LH.c[0], tTopC, topC, tC, LC.c[0],
+                                           FvH, FvC);
             } else {
                 interior_pair<MODE, NQ, DIFF>(m, false, size_t(r) * nx + ic, tTopH, topH, tH, LH.c[0],
                                         tTopC, topC, tC, LC.c[0], FvH, FvC);
@@ -337,6 +392,9 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
                     const double oC = boundary_flux<R>(m, vn, vxs, 1.0, tC, LC.c[1], m.sv, m.wv, pol, inC, err);
                     FrH = oH; FrC = oC;
                     if (owned) { bH += oH; bC += oC; }
+                } else if constexpr (MODE == 0 && NQ > 0) {
+                    interior_pair_at<NQ, DIFF>(m, true, vrgt, tH, LH.c[1], nbtH, nbH, tC, LC.c[1], nbtC, nbC,
+                                               FrH, FrC);
                 } else {
                     interior_pair<MODE, NQ, DIFF>(m, true, face, tH, LH.c[1], nbtH, nbH, tC, LC.c[1],
                                             nbtC, nbC, FrH, FrC);
