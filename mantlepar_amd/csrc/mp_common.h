@@ -83,6 +83,16 @@ struct __attribute__((aligned(16))) FaceRec {
     float a, b;
 };
 
+// The FP64 sweep's one-sided record of a face: under the local LF speed a
+// face whose points all have v >= 0 (or all v < 0) has flux
+// (t + c0)*v + c1*a + c2*b from its upwind side's centre t and face line c,
+// with v = sum_q W_q v_q, a = sum_q W_q v_q s_q, b = sum_q W_q v_q s_q^2 in
+// FP64.  A face whose points change sign, and every face under the global LF
+// speed, has b = NaN and is integrated point by point from its samples.
+struct __attribute__((aligned(16))) FaceRec64 {
+    double v, a, b, pad;
+};
+
 // One uniform rectangular mesh.  All reconstruction tables are the closed
 // forms of the reference's moment / smoothness construction on rectangles,
 // in coordinates normalised by h = sqrt(dx*dy) about the target centre.
@@ -161,6 +171,10 @@ struct MeshDev {
     const FaceMom* fy;
     const FaceRec* rx;
     const FaceRec* ry;
+    // FP64 one-sided records ([ny][nx+1] and [ny+1][nx]); null: every face of
+    // the FP64 sweep is integrated point by point.
+    const FaceRec64* rx64;
+    const FaceRec64* ry64;
 };
 
 // Per launch, per mesh: the stage u_out = a*u_n + b*(u_k + dt*L(u_k)).

@@ -208,6 +208,8 @@ size_t Engine::slot_bytes(const mp_mesh_desc& d, int32_t precision, int rows_min
     if (precision == MP_PREC_MIXED) {   // face moment records, full and one-sided
         b += Arena::round(faces_x(d) * sizeof(FaceMom)) + Arena::round(faces_y(d) * sizeof(FaceMom));
         b += Arena::round(faces_x(d) * sizeof(FaceRec)) + Arena::round(faces_y(d) * sizeof(FaceRec));
+    } else {                            // one-sided FP64 records
+        b += Arena::round(faces_x(d) * sizeof(FaceRec64)) + Arena::round(faces_y(d) * sizeof(FaceRec64));
     }
     b += 2 * Arena::round(N * (4 * sizeof(double) + 1));
     b += Arena::round(size_t(6) * items_for(d, rows_min) * kPartials * sizeof(double));
@@ -245,6 +247,9 @@ mp_status Engine::add_mesh(const mp_mesh_desc& d, int* out)
         sl.md.fy = reinterpret_cast<FaceMom*>(take(faces_y(d) * sizeof(FaceMom)));
         sl.md.rx = reinterpret_cast<FaceRec*>(take(faces_x(d) * sizeof(FaceRec)));
         sl.md.ry = reinterpret_cast<FaceRec*>(take(faces_y(d) * sizeof(FaceRec)));
+    } else {
+        sl.md.rx64 = reinterpret_cast<FaceRec64*>(take(faces_x(d) * sizeof(FaceRec64)));
+        sl.md.ry64 = reinterpret_cast<FaceRec64*>(take(faces_y(d) * sizeof(FaceRec64)));
     }
     sl.phase[0] = reinterpret_cast<double*>(take(N * (4 * sizeof(double) + 1)));
     sl.phase[1] = reinterpret_cast<double*>(take(N * (4 * sizeof(double) + 1)));
@@ -345,6 +350,9 @@ mp_status Engine::set_velocity(int k, const double* vx, const double* vy, bool w
     if (precision == MP_PREC_MIXED)
         MP_HIP(launch_face_prep(dm, d.nx, d.ny, const_cast<FaceMom*>(s.md.fx), const_cast<FaceMom*>(s.md.fy),
                                 const_cast<FaceRec*>(s.md.rx), const_cast<FaceRec*>(s.md.ry), stream));
+    else
+        MP_HIP(launch_face_rec64(dm, d.nx, d.ny, const_cast<FaceRec64*>(s.md.rx64),
+                                 const_cast<FaceRec64*>(s.md.ry64), stream));
     unsigned long long* bits = d_maxbits.as<unsigned long long>() + k;
     MP_CALL(launch_clear(reinterpret_cast<uint32_t*>(bits), 2, stream) == hipSuccess
                 ? MP_OK : fail(MP_ERR_DEVICE, "clear failed"));

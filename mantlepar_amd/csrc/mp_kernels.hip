@@ -52,6 +52,25 @@ __device__ __forceinline__ double interior_flux(const MeshDev& m, const double* 
     return F;
 }
 
+// One Gauss point of both fields' LLF fluxes, accumulated into fh, fc.  The
+// compiler may fuse v*uL + v*uR either way, and may choose differently in
+// different kernel instances of the same source; contraction is therefore off
+// here and the face lines are explicit fmas, so every instance (fixed or
+// runtime point count, single mesh or batch) gives the same bits.
+__device__ __forceinline__ void llf_point(const MeshDev& m, double v, double sq, double w, double tLH,
+                                          const double (&cLH)[3], double tRH, const double (&cRH)[3],
+                                          double tLC, const double (&cLC)[3], double tRC,
+                                          const double (&cRC)[3], double& fh, double& fc)
+{
+#pragma clang fp contract(off)
+    auto line = [](const double (&c)[3], double x) { return fma(x, fma(x, c[2], c[1]), c[0]); };
+    const double alpha = m.lf_mode ? m.global_speed : fabs(v);
+    const double uLH = tLH + line(cLH, sq), uRH = tRH + line(cRH, sq);
+    const double uLC = tLC + line(cLC, sq), uRC = tRC + line(cRC, sq);
+    fh += w * (0.5 * (v * uLH + v * uRH - alpha * (uRH - uLH)));
+    fc += w * (0.5 * (v * uLC + v * uRC - alpha * (uRC - uLC)));
+}
+
 // Both fields' LLF fluxes through one interior face, per Gauss point, in FP64
 // (the plain drop-in form).  NQ > 0 fixes the point count at compile time.
 template <int NQ>
@@ -67,13 +86,7 @@ __device__ __forceinline__ void llf_pair(const MeshDev& m, const double* vn, siz
 #pragma unroll
     for (int q = 0; q < (NQ > 0 ? NQ : kMaxQ); ++q) {
         if (NQ == 0 && q >= nq) break;
-        const double v = vn[q * vstride];
-        const double alpha = m.lf_mode ? m.global_speed : fabs(v);
-        const double sq = s[q], w = W[q];
-        const double uLH = tLH + line_at<double>(cLH, sq), uRH = tRH + line_at<double>(cRH, sq);
-        const double uLC = tLC + line_at<double>(cLC, sq), uRC = tRC + line_at<double>(cRC, sq);
-        fh += w * (0.5 * (v * uLH + v * uRH - alpha * (uRH - uLH)));
-        fc += w * (0.5 * (v * uLC + v * uRC - alpha * (uRC - uLC)));
+        llf_point(m, vn[q * vstride], s[q], W[q], tLH, cLH, tRH, cRH, tLC, cLC, tRC, cRC, fh, fc);
     }
     FH = fh;
     FC = fc;
@@ -90,15 +103,8 @@ __device__ __forceinline__ void llf_pair_at(const MeshDev& m, const double (&vq)
 {
     double fh = 0.0, fc = 0.0;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const double v = vq[q];
-        const double alpha = m.lf_mode ? m.global_speed : fabs(v);
-        const double sq = s[q], w = W[q];
-        const double uLH = tLH + line_at<double>(cLH, sq), uRH = tRH + line_at<double>(cRH, sq);
-        const double uLC = tLC + line_at<double>(cLC, sq), uRC = tRC + line_at<double>(cRC, sq);
-        fh += w * (0.5 * (v * uLH + v * uRH - alpha * (uRH - uLH)));
-        fc += w * (0.5 * (v * uLC + v * uRC - alpha * (uRC - uLC)));
-    }
+    for (int q = 0; q < NQ; ++q)
+        llf_point(m, vq[q], s[q], W[q], tLH, cLH, tRH, cRH, tLC, cLC, tRC, cRC, fh, fc);
     FH = fh;
     FC = fc;
 }
@@ -167,6 +173,55 @@ __global__ void face_prep_kernel(const MeshDev* __restrict__ mesh, FaceMom* fx, 
     (vert ? rx : ry)[f] = c;
 }
 
+// The FP64 sweep's one-sided face records (one thread per face), once per
+// velocity refresh: the moments v, a, b of the face's samples, flagged with
+// b = NaN where the points change sign or the LF speed is global.
+__global__ void face_rec64_kernel(const MeshDev* __restrict__ mesh, FaceRec64* rx, FaceRec64* ry)
+{
+    const MeshDev& m = *mesh;
+    const int64_t nxf = int64_t(m.ny) * (m.nx + 1), nyf = int64_t(m.ny + 1) * m.nx;
+    int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (f >= nxf + nyf) return;
+    const bool vert = f < nxf;
+    if (!vert) f -= nxf;
+    const int64_t nf = vert ? nxf : nyf;
+    const double* vn = (vert ? m.vnx : m.vny) + f;
+    const double* s = vert ? m.sv : m.sh;
+    const double* W = vert ? m.wv : m.wh;
+    double v = 0, a = 0, b = 0;
+    bool pos = false, neg = false;
+    for (int q = 0; q < m.q; ++q) {
+        const double x = vn[q * nf];
+        pos = pos || x > 0.0;
+        neg = neg || x < 0.0;
+        const double w = W[q] * x;
+        v += w;
+        a += w * s[q];
+        b += w * s[q] * s[q];
+    }
+    FaceRec64 r;
+    r.v = v;
+    r.a = a;
+    r.b = (m.lf_mode || (pos && neg)) ? __builtin_nan("") : b;
+    r.pad = 0.0;
+    (vert ? rx : ry)[f] = r;
+}
+
+// Both fields through one interior face from its FP64 one-sided record: the
+// upwind side's centre and face line (left/bottom where v >= 0) dotted with
+// the record, as explicit fma chains so every kernel instance rounds alike.
+__device__ __forceinline__ void rec64_pair(const FaceRec64& r, double tLH, const double (&cLH)[3], double tRH,
+                                           const double (&cRH)[3], double tLC, const double (&cLC)[3],
+                                           double tRC, const double (&cRC)[3], double& FH, double& FC)
+{
+#pragma clang fp contract(off)
+    const bool up = r.v >= 0.0;
+    const double uH = up ? tLH + cLH[0] : tRH + cRH[0];
+    const double uC = up ? tLC + cLC[0] : tRC + cRC[0];
+    FH = fma(up ? cLH[2] : cRH[2], r.b, fma(up ? cLH[1] : cRH[1], r.a, uH * r.v));
+    FC = fma(up ? cLC[2] : cRC[2], r.b, fma(up ? cLC[1] : cRC[1], r.a, uC * r.v));
+}
+
 // Flux of both fields through one interior face; DIFF adds the thermal
 // diffusion of H.
 template <int MODE, int NQ, bool DIFF>
@@ -190,16 +245,41 @@ __device__ __forceinline__ void interior_pair(const MeshDev& m, bool vertical, s
     if constexpr (DIFF) FH = fma(vertical ? m.dif_x : m.dif_y, tLH - tRH, FH);
 }
 
-// interior_pair of the FP64 sweep on samples already in registers.
+// interior_pair of the FP64 sweep point by point, its NQ samples issued
+// together and waited for once.
 template <int NQ, bool DIFF>
-__device__ __forceinline__ void interior_pair_at(const MeshDev& m, bool vertical, const double (&vq)[NQ],
-                                                 double tLH, const double (&cLH)[3], double tRH,
-                                                 const double (&cRH)[3], double tLC, const double (&cLC)[3],
-                                                 double tRC, const double (&cRC)[3], double& FH, double& FC)
+__device__ __forceinline__ void interior_pair_pts(const MeshDev& m, bool vertical, size_t face, double tLH,
+                                                  const double (&cLH)[3], double tRH, const double (&cRH)[3],
+                                                  double tLC, const double (&cLC)[3], double tRC,
+                                                  const double (&cRC)[3], double& FH, double& FC)
 {
+    const size_t vs = vertical ? size_t(m.ny) * (m.nx + 1) : size_t(m.ny + 1) * m.nx;
+    const auto* p = gmem(vertical ? m.vnx : m.vny) + face;
+    double vq[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) vq[q] = p[q * vs];
     llf_pair_at<NQ>(m, vq, vertical ? m.sv : m.sh, vertical ? m.wv : m.wh, tLH, cLH, tRH, cRH, tLC, cLC,
                     tRC, cRC, FH, FC);
     if constexpr (DIFF) FH = fma(vertical ? m.dif_x : m.dif_y, tLH - tRH, FH);
+}
+
+// interior_pair of the FP64 sweep from the face's one-sided record, or point
+// by point where the record is flagged.
+template <int NQ, bool DIFF>
+__device__ __forceinline__ void interior_pair_rec(const MeshDev& m, bool vertical, size_t face,
+                                                  const FaceRec64& rec, double tLH, const double (&cLH)[3],
+                                                  double tRH, const double (&cRH)[3], double tLC,
+                                                  const double (&cLC)[3], double tRC, const double (&cRC)[3],
+                                                  double& FH, double& FC)
+{
+    if (!isnan(rec.b)) {
+        rec64_pair(rec, tLH, cLH, tRH, cRH, tLC, cLC, tRC, cRC, FH, FC);
+        if constexpr (DIFF) FH = fma(vertical ? m.dif_x : m.dif_y, tLH - tRH, FH);
+    } else if constexpr (NQ > 0) {
+        interior_pair_pts<NQ, DIFF>(m, vertical, face, tLH, cLH, tRH, cRH, tLC, cLC, tRC, cRC, FH, FC);
+    } else {
+        interior_pair<0, 0, DIFF>(m, vertical, face, tLH, cLH, tRH, cRH, tLC, cLC, tRC, cRC, FH, FC);
+    }
 }
 
 // LIM: apply the bound limiter to both fields' reconstructions (FP64 only).
@@ -283,20 +363,19 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
                 for (int k = 0; k < 3; ++k) { LH.c[f][k] = R(0); LC.c[f][k] = R(0); }
         }
         const double tH = wH[1][1], tC = wC[1][1];
-        // With the point count fixed, this row's interior face samples (bottom
-        // face y_r and right face x_{i+1}) are issued together here, from
-        // clamped in-bounds addresses, and waited for once; boundary faces and
-        // lanes without an interior face read them but do not use them.  (Held
-        // across the reconstruction instead, they spill.)
-        constexpr int NV = (MODE == 0 && NQ > 0) ? NQ : 1;
-        double vbot[NV], vrgt[NV];
-        if constexpr (MODE == 0 && NQ > 0) {
-            const auto* by = gmem(m.vny) + size_t(clampi(r, 0, ny)) * nx + ic;
-            const auto* bx = gmem(m.vnx) + size_t(clampi(r, 0, ny - 1)) * (nx + 1) + clampi(i + 1, 0, nx);
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                vbot[q] = by[q * vys];
-                vrgt[q] = bx[q * vxs];
+        // In the FP64 sweep, the one-sided records of this row's
+        // bottom face y_r and right face x_{i+1} are loaded here, from clamped
+        // in-bounds addresses, both issued before either is needed; boundary
+        // faces and lanes without an interior face read them but do not use
+        // them.  (Held across the reconstruction instead, they spill.)
+        constexpr bool RECS = MODE == 0;
+        FaceRec64 rbot{}, rrgt{};
+        if constexpr (RECS) {
+            if (m.ry64) {
+                rbot = gload(m.ry64 + size_t(clampi(r, 0, ny)) * nx + ic);
+                rrgt = gload(m.rx64 + size_t(clampi(r, 0, ny - 1)) * (nx + 1) + clampi(i + 1, 0, nx));
+            } else {
+                rbot.b = rrgt.b = __builtin_nan("");
             }
         }
 
@@ -322,9 +401,9 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
                 const double oC = boundary_flux<R>(m, vn, vys, 1.0, tTopC, topC, m.sh, m.wh, pol, inC, err);
                 FvH = oH; FvC = oC;
                 if (owned) { bH += oH; bC += oC; }
-            } else if constexpr (MODE == 0 && NQ > 0) {
-                interior_pair_at<NQ, DIFF>(m, false, vbot, tTopH, topH, tH, LH.c[0], tTopC, topC, tC, LC.c[0],
-                                           FvH, FvC);
+            } else if constexpr (RECS) {
+                interior_pair_rec<NQ, DIFF>(m, false, size_t(r) * nx + ic, rbot, tTopH, topH, tH, LH.c[0],
+                                            tTopC, topC, tC, LC.c[0], FvH, FvC);
             } else {
                 interior_pair<MODE, NQ, DIFF>(m, false, size_t(r) * nx + ic, tTopH, topH, tH, LH.c[0],
                                         tTopC, topC, tC, LC.c[0], FvH, FvC);
@@ -392,9 +471,9 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
                     const double oC = boundary_flux<R>(m, vn, vxs, 1.0, tC, LC.c[1], m.sv, m.wv, pol, inC, err);
                     FrH = oH; FrC = oC;
                     if (owned) { bH += oH; bC += oC; }
-                } else if constexpr (MODE == 0 && NQ > 0) {
-                    interior_pair_at<NQ, DIFF>(m, true, vrgt, tH, LH.c[1], nbtH, nbH, tC, LC.c[1], nbtC, nbC,
-                                               FrH, FrC);
+                } else if constexpr (RECS) {
+                    interior_pair_rec<NQ, DIFF>(m, true, face, rrgt, tH, LH.c[1], nbtH, nbH, tC, LC.c[1],
+                                                nbtC, nbC, FrH, FrC);
                 } else {
                     interior_pair<MODE, NQ, DIFF>(m, true, face, tH, LH.c[1], nbtH, nbH, tC, LC.c[1],
                                             nbtC, nbC, FrH, FrC);
@@ -1202,6 +1281,15 @@ hipError_t launch_face_prep(const MeshDev* d_mesh, int nx, int ny, FaceMom* fx, 
     const int64_t n = int64_t(ny) * (nx + 1) + int64_t(ny + 1) * nx;
     const dim3 grid(unsigned((n + 255) / 256)), block(256);
     face_prep_kernel<<<grid, block, 0, stream>>>(d_mesh, fx, fy, rx, ry);
+    return hipGetLastError();
+}
+
+hipError_t launch_face_rec64(const MeshDev* d_mesh, int nx, int ny, FaceRec64* rx, FaceRec64* ry,
+                             hipStream_t stream)
+{
+    const int64_t n = int64_t(ny) * (nx + 1) + int64_t(ny + 1) * nx;
+    const dim3 grid(unsigned((n + 255) / 256)), block(256);
+    face_rec64_kernel<<<grid, block, 0, stream>>>(d_mesh, rx, ry);
     return hipGetLastError();
 }
 

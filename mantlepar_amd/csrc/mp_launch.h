@@ -19,6 +19,8 @@ hipError_t launch_limit_diag(const MeshDev* d_mesh, int ncells, const double* H,
                              double* thetaH, double* thetaC, uint32_t* error, hipStream_t stream);
 hipError_t launch_face_prep(const MeshDev* d_mesh, int nx, int ny, FaceMom* fx, FaceMom* fy,
                             FaceRec* rx, FaceRec* ry, hipStream_t stream);
+hipError_t launch_face_rec64(const MeshDev* d_mesh, int nx, int ny, FaceRec64* rx, FaceRec64* ry,
+                             hipStream_t stream);
 hipError_t launch_recon_diag(int mode, const MeshDev* d_mesh, int ncells, const double* u,
                              int field, double* sigma, double* omega, double* traces,
                              uint32_t* error, hipStream_t stream);

@@ -80,6 +80,12 @@ mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
         md.fy = fy.as<FaceMom>();
         md.rx = rx.as<FaceRec>();
         md.ry = ry.as<FaceRec>();
+    } else {                       // one-sided FP64 records: the FP64 sweep's flux input
+        const size_t nxf = size_t(m.ny) * (m.nx + 1), nyf = size_t(m.ny + 1) * m.nx;
+        MP_CALL(rx64.alloc(nxf * sizeof(FaceRec64)));
+        MP_CALL(ry64.alloc(nyf * sizeof(FaceRec64)));
+        md.rx64 = rx64.as<FaceRec64>();
+        md.ry64 = ry64.as<FaceRec64>();
     }
     MP_CALL(error.alloc(sizeof(uint64_t) * 2));
     MP_CALL(d_md.alloc(sizeof(MeshDev)));
@@ -243,6 +249,9 @@ mp_status mp_transport::set_velocity(const double* x, const double* y)
     if (precision == MP_PREC_MIXED)
         MP_HIP(launch_face_prep(d_md.as<MeshDev>(), mesh.nx, mesh.ny, fx.as<FaceMom>(), fy.as<FaceMom>(),
                                 rx.as<FaceRec>(), ry.as<FaceRec>(), stream));
+    else
+        MP_HIP(launch_face_rec64(d_md.as<MeshDev>(), mesh.nx, mesh.ny, rx64.as<FaceRec64>(),
+                                 ry64.as<FaceRec64>(), stream));
     MP_CALL(compute_rate());
     have_velocity = true;
     return MP_OK;

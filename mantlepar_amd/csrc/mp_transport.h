@@ -24,6 +24,7 @@ struct mp_transport {
     mp::DeviceBuffer vnx, vny;
     mp::DeviceBuffer fx, fy;   // per-face flux moments
     mp::DeviceBuffer rx, ry;   // and their one-sided 16-byte form
+    mp::DeviceBuffer rx64, ry64;   // FP64 handles: one-sided FP64 records
     bool have_velocity = false;
     double max_rate = 0;
     mp::DeviceBuffer bc[4], inflowH, inflowC;
