@@ -245,6 +245,21 @@ __device__ __forceinline__ void interior_pair(const MeshDev& m, bool vertical, s
     if constexpr (DIFF) FH = fma(vertical ? m.dif_x : m.dif_y, tLH - tRH, FH);
 }
 
+// The FP64 records of row r's bottom face y_r and right face x_{i+1}, from
+// clamped in-bounds addresses (boundary faces and lanes without an interior
+// face read them but do not use them); without records, b = NaN sends both
+// faces point by point.
+__device__ __forceinline__ void fetch_rec64(const MeshDev& m, int r, int ic, int i, FaceRec64& rb,
+                                            FaceRec64& rr)
+{
+    if (m.ry64) {
+        rb = gload(m.ry64 + size_t(clampi(r, 0, m.ny)) * m.nx + ic);
+        rr = gload(m.rx64 + size_t(clampi(r, 0, m.ny - 1)) * (m.nx + 1) + clampi(i + 1, 0, m.nx));
+    } else {
+        rb.b = rr.b = __builtin_nan("");
+    }
+}
+
 // interior_pair of the FP64 sweep point by point, its NQ samples issued
 // together and waited for once.
 template <int NQ, bool DIFF>
@@ -327,6 +342,13 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
             load_row(sp.Hk, r + 2, nH);
             load_row(sp.Ck, r + 2, nC);
         }
+        // In the FP64 sweep, the one-sided records of this row's bottom face
+        // y_r and right face x_{i+1}, both issued before either is needed:
+        // before the reconstruction, which covers their latency, except in
+        // the phase-fused sweep, where held that long they spill, and after it.
+        constexpr bool RECS = MODE == 0, EARLY = RECS && !PHASE;
+        FaceRec64 rbot{}, rrgt{};
+        if constexpr (EARLY) fetch_rec64(m, r, ic, i, rbot, rrgt);
         FaceLines<R> LH, LC;
         if (have) {
             const unsigned mask = col_ok ? candidate_mask(i, r, nx, ny) : 2u;
@@ -363,21 +385,7 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
                 for (int k = 0; k < 3; ++k) { LH.c[f][k] = R(0); LC.c[f][k] = R(0); }
         }
         const double tH = wH[1][1], tC = wC[1][1];
-        // In the FP64 sweep, the one-sided records of this row's
-        // bottom face y_r and right face x_{i+1} are loaded here, from clamped
-        // in-bounds addresses, both issued before either is needed; boundary
-        // faces and lanes without an interior face read them but do not use
-        // them.  (Held across the reconstruction instead, they spill.)
-        constexpr bool RECS = MODE == 0;
-        FaceRec64 rbot{}, rrgt{};
-        if constexpr (RECS) {
-            if (m.ry64) {
-                rbot = gload(m.ry64 + size_t(clampi(r, 0, ny)) * nx + ic);
-                rrgt = gload(m.rx64 + size_t(clampi(r, 0, ny - 1)) * (nx + 1) + clampi(i + 1, 0, nx));
-            } else {
-                rbot.b = rrgt.b = __builtin_nan("");
-            }
-        }
+        if constexpr (RECS && !EARLY) fetch_rec64(m, r, ic, i, rbot, rrgt);
 
         // Horizontal face y_r: top of row r-1, bottom of row r.
         double FvH = 0.0, FvC = 0.0;
