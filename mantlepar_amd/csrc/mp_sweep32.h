@@ -350,10 +350,39 @@ __device__ __forceinline__ float limit32(const MeshDev& m, const K32& k, const R
     };
     using Ni = std::integral_constant<int, NS / 2>;
     using Nb = std::integral_constant<int, NS>;
-    if (sides & 2u) cols(1.0f, m.lim32_xb, nb, Nb{}); else cols(1.0f, m.lim32_xi, ni, Ni{});
-    if (sides & 8u) cols(-1.0f, m.lim32_xb, nb, Nb{}); else cols(-1.0f, m.lim32_xi, ni, Ni{});
-    if (sides & 4u) rows(1.0f, m.lim32_yb, nb, Nb{}); else rows(1.0f, m.lim32_yi, ni, Ni{});
-    if (sides & 1u) rows(-1.0f, m.lim32_yb, nb, Nb{}); else rows(-1.0f, m.lim32_yi, ni, Ni{});
+    if constexpr (NS > 0) {
+        // Which table a face reads depends on the lane (a face on the mesh
+        // boundary takes the boundary samples); read through a per-lane choice
+        // of pointer, every offset would be a vector load waited for in turn.
+        // With the table sizes fixed, each sample offset is a per-lane choice
+        // between two uniform values, and only boundary faces take the samples
+        // beyond the interior count.  The samples are visited in the same order.
+        auto cols_fixed = [&](float sign, bool bnd) {
+#pragma unroll
+            for (int q = 0; q < NS; ++q) {
+                if (q >= NS / 2 && !bnd) break;
+                const float x = (q < NS / 2 && !bnd) ? m.lim32_xi[q] : m.lim32_xb[q];
+                col_ext32<NQ>(B, sign * x, m.lim32_sv, nq, pmin, pmax);
+            }
+        };
+        auto rows_fixed = [&](float sign, bool bnd) {
+#pragma unroll
+            for (int q = 0; q < NS; ++q) {
+                if (q >= NS / 2 && !bnd) break;
+                const float y = (q < NS / 2 && !bnd) ? m.lim32_yi[q] : m.lim32_yb[q];
+                row_ext32<NQ>(B, sign * y, m.lim32_sh, nq, pmin, pmax);
+            }
+        };
+        cols_fixed(1.0f, sides & 2u);
+        cols_fixed(-1.0f, sides & 8u);
+        rows_fixed(1.0f, sides & 4u);
+        rows_fixed(-1.0f, sides & 1u);
+    } else {
+        if (sides & 2u) cols(1.0f, m.lim32_xb, nb, Nb{}); else cols(1.0f, m.lim32_xi, ni, Ni{});
+        if (sides & 8u) cols(-1.0f, m.lim32_xb, nb, Nb{}); else cols(-1.0f, m.lim32_xi, ni, Ni{});
+        if (sides & 4u) rows(1.0f, m.lim32_yb, nb, Nb{}); else rows(1.0f, m.lim32_yi, ni, Ni{});
+        if (sides & 1u) rows(-1.0f, m.lim32_yb, nb, Nb{}); else rows(-1.0f, m.lim32_yi, ni, Ni{});
+    }
     float theta = 1.0f;
     if (pmax > dhi) theta = fminf(theta, dhi / pmax);
     if (pmin < dlo) theta = fminf(theta, dlo / pmin);
