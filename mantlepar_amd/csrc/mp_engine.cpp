@@ -160,7 +160,7 @@ mp_status Engine::init(int cap, int32_t prec, const mp_weno_options& w, const mp
         MP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         own_stream = true;
     }
-    if (prec == MP_PREC_MIXED) {
+    {   // side stream: each step's phase and totals beside the next step
         MP_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         for (int k = 0; k < 2; ++k) {
             MP_HIP(hipEventCreateWithFlags(&stage3[k], hipEventDisableTiming));
@@ -786,8 +786,8 @@ mp_status Engine::reserve_steps(int nsteps)
     return MP_OK;
 }
 
-// The mixed path runs each step's phase kernel and per-mesh totals on the
-// side stream, beside the next step's stages: the phase reads the step's
+// Each step's phase kernel and per-mesh totals run on the side stream,
+// beside the next step's stages: the phase reads the step's
 // output states, which the next step only reads, and writes the phase outputs
 // and the sum slots of the step's partials (the parity set of its step).
 // Step s + 2 writes step s's output states and partial set again, so it first
@@ -798,7 +798,7 @@ mp_status Engine::reserve_steps(int nsteps)
 mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* events)
 {
     if (nsteps > steps_capacity) return fail(MP_ERR_STATE, "per-step totals not reserved");
-    const bool overlap = precision == MP_PREC_MIXED && side && !events;
+    const bool overlap = side && !events;
     struct Join {
         hipStream_t main, side;
         hipEvent_t ev;
@@ -841,9 +841,9 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
                                        d_flag_off.as<int>(), nflags, d_epoch.as<uint32_t>(), kSpinLimit,
                                        weno.lf_mode != 0, stream));
         for (int st = 0; st < scheme; ++st) {
-            // the mixed sweep leaves the phase to its own streaming kernel, as
-            // the single-mesh handle does
-            const bool last = st == scheme - 1, split = last && precision == MP_PREC_MIXED;
+            // both sweeps leave the phase to its own streaming kernel, as the
+            // single-mesh handle does
+            const bool last = st == scheme - 1, split = last;
             const StagePtrs* sps = base + size_t(set * 3 + st) * capacity;
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
             if (!one)

@@ -53,7 +53,7 @@ mp_status mp_transport::init(const mp_mesh_desc& m, const mp_weno_options& w,
         MP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         own_stream = true;
     }
-    if (prec == MP_PREC_MIXED) {
+    {   // side stream: each step's phase and totals beside the next step
         MP_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         for (int k = 0; k < 2; ++k) {
             MP_HIP(hipEventCreateWithFlags(&stage3[k], hipEventDisableTiming));
@@ -342,8 +342,8 @@ StagePtrs mp_transport::stage_ptrs(const StepWork& w, int src, int kin, int out,
 
 // Enqueue nsteps; returns the index of the final state buffer.
 //
-// The mixed path runs each step's phase kernel and step totals on the side
-// stream, beside the next step's stages: the phase reads the step's output
+// Each step's phase kernel and step totals run on the side stream, beside
+// the next step's stages: the phase reads the step's output
 // state, which the next step only reads, and writes the phase outputs and the
 // sum slots of the step's partials.  Partials alternate between two sets by
 // step parity, and step s + 2, which writes step s's output buffer and
@@ -363,11 +363,11 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
     // One launch per mixed step: each launch also runs the previous step's
     // phase items after its own sweep items, in the launch's drain, and the
     // previous step's totals follow it; the last step's phase and totals
-    // close the call.  Per-stage mixed launches run each step's phase and
-    // totals on the side stream, beside the next step.  Profiling (events)
+    // close the call.  Per-stage launches (both precisions) run each step's
+    // phase and totals on the side stream, beside the next step.  Profiling (events)
     // keeps every kernel in its own launch on the main stream.
     const bool fold = fused && precision == MP_PREC_MIXED && !events;
-    const bool overlap = precision == MP_PREC_MIXED && side && !events && !fused;
+    const bool overlap = side && !events && !fused;
     // However the call ends, error returns included, the main stream waits for
     // everything enqueued on the side stream.
     struct Join {
@@ -455,8 +455,11 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
             const bool last = st == scheme - 1;
             const int out = last ? dst : 3 + st;
             const StagePtrs sp = stage_ptrs(w, src, kin, out, scheme, st, dt, last, phase_set, par * 3 + st);
-            // the mixed sweep leaves the phase to its own streaming kernel
-            const bool split = last && precision == MP_PREC_MIXED;
+            // both sweeps leave the phase to its own streaming kernel: fused
+            // into the FP64 sweep's last stage it cost that stage more than
+            // the streaming kernel takes, and it could not run beside the
+            // next step
+            const bool split = last;
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
             MP_HIP(launch_stage(precision, last && !split, mesh.q, d_md.as<MeshDev>(), sp, nitems,
                                 stream, limit_mode32(md), has_diffusion(), md.lf_mode != 0));
