@@ -843,20 +843,20 @@ mp_status Engine::enqueue(int nsteps, int scheme, std::vector<hipEvent_t>* event
         for (int st = 0; st < scheme; ++st) {
             // both sweeps leave the phase to its own streaming kernel, as the
             // single-mesh handle does
-            const bool last = st == scheme - 1, split = last;
+            const bool last = st == scheme - 1;
             const StagePtrs* sps = base + size_t(set * 3 + st) * capacity;
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
             if (!one)
-                MP_HIP(launch_batch_stage(precision, last && !split, q_common, d_meshes.as<MeshDev>(), sps,
+                MP_HIP(launch_batch_stage(precision, q_common, d_meshes.as<MeshDev>(), sps,
                                           d_items.as<int2>() + L.first, L.second, blocks,
                                           d_queues.as<int>() + 3 * s + st, stream, weno.lf_mode != 0, lim,
                                           dif));
-            if (split && overlap) {
+            if (last && overlap) {
                 MP_HIP(hipEventRecord(stage3[par], stream));
                 MP_HIP(hipStreamWaitEvent(side, stage3[par], 0));
                 MP_HIP(launch_batch_phase_items(d_meshes.as<MeshDev>(), sps, d_items.as<int2>() + L.first,
                                                 L.second, side));
-            } else if (split) {
+            } else if (last) {
                 MP_HIP(launch_batch_phase_items(d_meshes.as<MeshDev>(), sps, d_items.as<int2>() + L.first,
                                                 L.second, stream));
             }

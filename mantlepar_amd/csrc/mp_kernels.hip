@@ -301,7 +301,7 @@ __device__ __forceinline__ void interior_pair_rec(const MeshDev& m, bool vertica
 // DIFF: the mesh has thermal diffusion or thermal boundary data; without it
 // the diffusion terms are compiled out, which keeps the sweep within its
 // register budget (with them it spills 64 B more per lane).
-template <int MODE, bool PHASE, int NQ, bool LIM = false, bool DIFF = true>
+template <int MODE, int NQ, bool LIM = false, bool DIFF = true>
 __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp, int item, int strip, int j0,
                            int j1)
 {
@@ -331,7 +331,7 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
     R topH[3] = {R(0), R(0), R(0)}, topC[3] = {R(0), R(0), R(0)};
     double tTopH = 0.0, tTopC = 0.0;
     double pFlH = 0, pFrH = 0, pFbH = 0, pFlC = 0, pFrC = 0, pFbC = 0;
-    double sH = 0, eH = 0, sC = 0, eC = 0, bH = 0, bC = 0;
+    double bH = 0, bC = 0;
 
     for (; r <= j1; ++r) {
         const bool have = r < ny;
@@ -343,12 +343,11 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
             load_row(sp.Ck, r + 2, nC);
         }
         // In the FP64 sweep, the one-sided records of this row's bottom face
-        // y_r and right face x_{i+1}, both issued before either is needed:
-        // before the reconstruction, which covers their latency, except in
-        // the phase-fused sweep, where held that long they spill, and after it.
-        constexpr bool RECS = MODE == 0, EARLY = RECS && !PHASE;
+        // y_r and right face x_{i+1}, issued before the reconstruction, which
+        // covers their latency.
+        constexpr bool RECS = MODE == 0;
         FaceRec64 rbot{}, rrgt{};
-        if constexpr (EARLY) fetch_rec64(m, r, ic, i, rbot, rrgt);
+        if constexpr (RECS) fetch_rec64(m, r, ic, i, rbot, rrgt);
         FaceLines<R> LH, LC;
         if (have) {
             const unsigned mask = col_ok ? candidate_mask(i, r, nx, ny) : 2u;
@@ -385,7 +384,6 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
                 for (int k = 0; k < 3; ++k) { LH.c[f][k] = R(0); LC.c[f][k] = R(0); }
         }
         const double tH = wH[1][1], tC = wC[1][1];
-        if constexpr (RECS && !EARLY) fetch_rec64(m, r, ic, i, rbot, rrgt);
 
         // Horizontal face y_r: top of row r-1, bottom of row r.
         double FvH = 0.0, FvC = 0.0;
@@ -434,16 +432,6 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
                 if (!isfinite(oH) || !isfinite(oC)) err |= kErrNonFinite;
                 sp.Ho[k] = oH;
                 sp.Co[k] = oC;
-                if constexpr (PHASE) {
-                    const PhaseOut po = phase_eval<false>(phase_at(m.ph, m.tep_row, r - 1), oH, oC, err);
-                    sp.T[k] = po.T;
-                    sp.phi[k] = po.phil;
-                    sp.Cl[k] = po.cl;
-                    sp.Cs[k] = po.cs;
-                    sp.region[k] = uint8_t(po.region);
-                    two_sum_acc(sH, eH, oH);
-                    two_sum_acc(sC, eC, oC);
-                }
             }
         }
 
@@ -505,13 +493,13 @@ __device__ __forceinline__ void sweep_item(const MeshDev& m, const StagePtrs& sp
         }
     }
 
-    item_partials(sp, item, sH, eH, sC, eC, bH, bC, err);
+    item_partials(sp, item, 0.0, 0.0, 0.0, 0.0, bH, bC, err);
 }
 
 // Register budget: two waves per SIMD (256 registers), one wave per
 // workgroup, so a finished item frees its slot at once.  The mixed path has
 // its own sweep (mp_sweep32.hip).
-template <int MODE, bool PHASE, int NQ, bool LIM, bool DIFF>
+template <int MODE, int NQ, bool LIM, bool DIFF>
 __global__ __launch_bounds__(64, MODE == 0 ? 2 : 3) void stage_sweep_kernel(const MeshDev* __restrict__ mesh,
                                                           StagePtrs sp, int nitems)
 {
@@ -520,7 +508,7 @@ __global__ __launch_bounds__(64, MODE == 0 ? 2 : 3) void stage_sweep_kernel(cons
     const MeshDev& m = *mesh;
     int strip, j0, j1;
     item_region(sp, m.ny, item, strip, j0, j1);
-    sweep_item<MODE, PHASE, NQ, LIM, DIFF>(m, sp, item, strip, j0, j1);
+    sweep_item<MODE, NQ, LIM, DIFF>(m, sp, item, strip, j0, j1);
 }
 
 
@@ -731,7 +719,7 @@ __global__ __launch_bounds__(256) void state_sums_kernel(const double* __restric
 // LIM: meshes whose limiter is on (m.limit, wave-uniform) take the limited
 // sweep, the others the plain one, in the same launch; DIFF likewise for
 // meshes with diffusion or thermal boundary data (has_diffusion).
-template <int MODE, bool PHASE, int NQ, bool LIM = false, bool DIFF = false>
+template <int MODE, int NQ, bool LIM = false, bool DIFF = false>
 __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
     const MeshDev* __restrict__ meshes, const StagePtrs* __restrict__ stages,
     const int2* __restrict__ items, int nitems, int* queue)
@@ -759,10 +747,10 @@ __global__ __launch_bounds__(256, MODE == 0 ? 2 : 3) void batch_sweep_kernel(
                 // the kernel's own flags as template arguments: a branch that
                 // cannot run instantiates nothing new
                 const bool lim = LIM && m.limit, dif = DIFF && has_diffusion(m);
-                if (lim && dif) sweep_item<MODE, PHASE, NQ, LIM, DIFF>(m, sp, local, strip, j0, j1);
-                else if (lim) sweep_item<MODE, PHASE, NQ, LIM, false>(m, sp, local, strip, j0, j1);
-                else if (dif) sweep_item<MODE, PHASE, NQ, false, DIFF>(m, sp, local, strip, j0, j1);
-                else sweep_item<MODE, PHASE, NQ, false, false>(m, sp, local, strip, j0, j1);
+                if (lim && dif) sweep_item<MODE, NQ, LIM, DIFF>(m, sp, local, strip, j0, j1);
+                else if (lim) sweep_item<MODE, NQ, LIM, false>(m, sp, local, strip, j0, j1);
+                else if (dif) sweep_item<MODE, NQ, false, DIFF>(m, sp, local, strip, j0, j1);
+                else sweep_item<MODE, NQ, false, false>(m, sp, local, strip, j0, j1);
             }
         }
         if (!queue) break;
@@ -1244,32 +1232,24 @@ hipError_t launch_batch_phase_items(const MeshDev* meshes, const StagePtrs* stag
 
 // The FP64 sweep's instances: the limiter and the diffusion terms each
 // compiled in only when the mesh uses them.
-template <bool PHASE, int NQ>
+template <int NQ>
 static void launch_fp64(const MeshDev* d_mesh, const StagePtrs& sp, int nitems, hipStream_t stream,
                         bool limit, bool diffusion)
 {
     const dim3 grid(nitems), block(kWave);
-    if (limit && diffusion) stage_sweep_kernel<0, PHASE, NQ, true, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-    else if (limit) stage_sweep_kernel<0, PHASE, NQ, true, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-    else if (diffusion) stage_sweep_kernel<0, PHASE, NQ, false, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
-    else stage_sweep_kernel<0, PHASE, NQ, false, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    if (limit && diffusion) stage_sweep_kernel<0, NQ, true, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    else if (limit) stage_sweep_kernel<0, NQ, true, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    else if (diffusion) stage_sweep_kernel<0, NQ, false, true><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
+    else stage_sweep_kernel<0, NQ, false, false><<<grid, block, 0, stream>>>(d_mesh, sp, nitems);
 }
 
-hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
+hipError_t launch_stage(int mode, int q, const MeshDev* d_mesh, const StagePtrs& sp,
                         int nitems, hipStream_t stream, int limit, bool diffusion, bool global_lf)
 {
-    if (mode != 0) {
-        if (phase) return hipErrorInvalidValue;   // mixed: phase_items_kernel follows
-        return launch_stage32(d_mesh, sp, nitems, stream, diffusion, global_lf, limit);
-    }
+    if (mode != 0) return launch_stage32(d_mesh, sp, nitems, stream, diffusion, global_lf, limit);
     const bool lim = limit != 0;
-    if (q == 5) {
-        if (phase) launch_fp64<true, 5>(d_mesh, sp, nitems, stream, lim, diffusion);
-        else launch_fp64<false, 5>(d_mesh, sp, nitems, stream, lim, diffusion);
-    } else {
-        if (phase) launch_fp64<true, 0>(d_mesh, sp, nitems, stream, lim, diffusion);
-        else launch_fp64<false, 0>(d_mesh, sp, nitems, stream, lim, diffusion);
-    }
+    if (q == 5) launch_fp64<5>(d_mesh, sp, nitems, stream, lim, diffusion);
+    else launch_fp64<0>(d_mesh, sp, nitems, stream, lim, diffusion);
     return hipGetLastError();
 }
 
@@ -1379,7 +1359,7 @@ hipError_t launch_state_sums(const double* H, const double* C, int64_t n, double
     return hipGetLastError();
 }
 
-hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes,
+hipError_t launch_batch_stage(int mode, int q, const MeshDev* meshes,
                               const StagePtrs* stages, const int2* items, int nitems, int blocks,
                               int* queue, hipStream_t stream, bool global_lf, int limit, bool diffusion)
 {
@@ -1388,25 +1368,20 @@ hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes
     // the queue; otherwise one wave per workgroup and per item.
     int* qp = blocks > 0 ? queue : nullptr;
     const dim3 grid(blocks > 0 ? blocks : nitems), block(blocks > 0 ? 256 : kWave);
-    if (mode != 0) {
-        if (phase) return hipErrorInvalidValue;   // mixed: batch_phase_items_kernel follows
+    if (mode != 0)
         return launch_batch_stage32(meshes, stages, items, nitems, blocks, queue, global_lf, stream, limit,
                                     diffusion);
-    }
-#define MP_BATCH64(P, N, L, D) \
-    batch_sweep_kernel<0, P, N, L, D><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp)
-#define MP_BATCH64_LD(P, N)                                                  \
+#define MP_BATCH64(N, L, D) \
+    batch_sweep_kernel<0, N, L, D><<<grid, block, 0, stream>>>(meshes, stages, items, nitems, qp)
+#define MP_BATCH64_LD(N)                                                     \
     do {                                                                     \
-        if (limit && diffusion) MP_BATCH64(P, N, true, true);                \
-        else if (limit) MP_BATCH64(P, N, true, false);                       \
-        else if (diffusion) MP_BATCH64(P, N, false, true);                   \
-        else MP_BATCH64(P, N, false, false);                                 \
+        if (limit && diffusion) MP_BATCH64(N, true, true);                   \
+        else if (limit) MP_BATCH64(N, true, false);                          \
+        else if (diffusion) MP_BATCH64(N, false, true);                      \
+        else MP_BATCH64(N, false, false);                                    \
     } while (0)
-    if (q == 5) {
-        if (phase) MP_BATCH64_LD(true, 5); else MP_BATCH64_LD(false, 5);
-    } else {
-        if (phase) MP_BATCH64_LD(true, 0); else MP_BATCH64_LD(false, 0);
-    }
+    if (q == 5) MP_BATCH64_LD(5);
+    else MP_BATCH64_LD(0);
 #undef MP_BATCH64_LD
 #undef MP_BATCH64
     return hipGetLastError();
@@ -1416,7 +1391,7 @@ int batch_blocks_per_cu(int mode)
 {
     if (mode != 0) return batch_blocks_per_cu32();
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_sweep_kernel<0, false, 0>, 256, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_sweep_kernel<0, 0>, 256, 0) != hipSuccess)
         return 0;
     return n;
 }

@@ -7,12 +7,14 @@
 
 namespace mp {
 
+// One stage of either sweep; the phase of a step's last stage runs in its own
+// kernel after it (launch_phase_items).
 // limit: apply the bound limiter (FP64 sweep only).
 // diffusion: the mesh has thermal diffusion or thermal boundary data (the
 // mixed sweep compiles the terms out otherwise).
 // global_lf: the mesh uses the global LF speed (mixed sweep's record form).
 // limit: 0 off, else the bound limiter (mixed: the mode of limit_mode32).
-hipError_t launch_stage(int mode, bool phase, int q, const MeshDev* d_mesh, const StagePtrs& sp,
+hipError_t launch_stage(int mode, int q, const MeshDev* d_mesh, const StagePtrs& sp,
                         int nitems, hipStream_t stream, int limit = 0, bool diffusion = true,
                         bool global_lf = false);
 hipError_t launch_limit_diag(const MeshDev* d_mesh, int ncells, const double* H, const double* C,
@@ -49,7 +51,7 @@ hipError_t launch_state_phase(const MeshDev* d_mesh, int ncells, const double* H
 // on (mixed: with the table instance of its single-mesh launch).
 // diffusion: some mesh of the list has diffusion or thermal boundary data;
 // each mesh then takes the diffusion terms only if it has them.
-hipError_t launch_batch_stage(int mode, bool phase, int q, const MeshDev* meshes,
+hipError_t launch_batch_stage(int mode, int q, const MeshDev* meshes,
                               const StagePtrs* stages, const int2* items, int nitems, int blocks,
                               int* queue, hipStream_t stream, bool global_lf = false, int limit = 0,
                               bool diffusion = false);

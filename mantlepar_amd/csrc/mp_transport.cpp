@@ -459,15 +459,14 @@ mp_status mp_transport::enqueue_steps(const StepWork& w, int nsteps, int scheme,
             // into the FP64 sweep's last stage it cost that stage more than
             // the streaming kernel takes, and it could not run beside the
             // next step
-            const bool split = last;
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st)], stream));
-            MP_HIP(launch_stage(precision, last && !split, mesh.q, d_md.as<MeshDev>(), sp, nitems,
-                                stream, limit_mode32(md), has_diffusion(), md.lf_mode != 0));
-            if (split && overlap) {
+            MP_HIP(launch_stage(precision, mesh.q, d_md.as<MeshDev>(), sp, nitems, stream, limit_mode32(md),
+                                has_diffusion(), md.lf_mode != 0));
+            if (last && overlap) {
                 MP_HIP(hipEventRecord(stage3[par], stream));
                 MP_HIP(hipStreamWaitEvent(side, stage3[par], 0));
                 MP_HIP(launch_phase_items(d_md.as<MeshDev>(), sp, nitems, side));
-            } else if (split) {
+            } else if (last) {
                 MP_HIP(launch_phase_items(d_md.as<MeshDev>(), sp, nitems, stream));
             }
             if (events) MP_HIP(hipEventRecord((*events)[2 * (s * 3 + st) + 1], stream));
