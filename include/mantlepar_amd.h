@@ -185,7 +185,8 @@ mp_status mp_transport_device_state(mp_transport* t, double** H, double** C);
 mp_status mp_transport_synchronize(mp_transport* t);
 /* Time nsteps of the stage kernel with HIP events on the handle's stream:
  * ms_total = wall time of the steps; ms_stage = mean duration of one stage
- * launch; ms_phase_stage = mean duration of the phase-fused final stage.  With
+ * launch; ms_phase_stage = mean duration of the final stage together with the
+ * phase kernel that follows it (each launch timed alone, on one stream).  With
  * one launch per step (MP_LAUNCH_PER_STEP on a mixed handle), ms_stage is the
  * mean duration of a step's launch (all its stages) and ms_phase_stage that of
  * its phase kernel. */
@@ -355,7 +356,7 @@ mp_status mp_batch_get_launch(mp_batch* b, int32_t* per_step);
 mp_status mp_batch_autotune(mp_batch* b, int32_t nsteps, int32_t scheme, const char* cache_path,
                             int32_t* rows_per_item, int32_t* blocks, double* ms_per_step);
 /* Time nsteps (state unchanged): ms_total, mean stage launch, mean
- * phase-fused stage launch (HIP events; eager launches). */
+ * final stage launch with its phase kernel (HIP events; eager launches). */
 mp_status mp_batch_profile(mp_batch* b, int32_t nsteps, int32_t scheme, double* ms_total,
                            double* ms_stage, double* ms_phase_stage);
 /* Resident meshes, their cells, work items and arena bytes in use. */
