@@ -186,7 +186,7 @@ struct StagePtrs {
     double* Ho;
     double* Co;
     double a, b, dt;
-    double* T;          // phase outputs; non-null only on a phase-fused stage
+    double* T;          // phase outputs of a step's last stage (written by the phase kernel)
     double* phi;
     double* Cl;
     double* Cs;

@@ -71,7 +71,7 @@ hipError_t launch_clear(uint32_t* p, int n, hipStream_t stream);
 int batch_blocks_per_cu(int mode);
 // Mixed-precision sweeps (mp_sweep32.hip).
 int batch_blocks_per_cu32();
-// The mixed sweep has no phase-fused form: phase_items_kernel follows its last stage.
+// Neither sweep evaluates the phase: phase_items_kernel follows a step's last stage.
 // diffusion: the mesh has thermal diffusion or thermal boundary data.
 // global_lf: the mesh uses the global LF speed, so the sweep fetches full face
 // records instead of their one-sided form.

@@ -12,10 +12,10 @@
 namespace mp {
 
 // Phase of a stage's output and its state sums, per work item of the stage
-// sweep: lane = column, rows in order, the same butterfly -- the values the
-// phase-fused sweep produces, bit for bit.  The mixed path runs its last
-// stage without the phase epilogue and this kernel after it: the sweep keeps
-// its fast interior form, and the phase work streams at full occupancy.
+// sweep: lane = column, rows in order, the same butterfly as the sweep's own
+// partials.  Both precisions run their last stage without a phase epilogue
+// and this kernel after it (beside the next step when stepping): the sweep
+// keeps its register budget, and the phase work streams at full occupancy.
 __device__ __forceinline__ void phase_item(const MeshDev& m, const StagePtrs& sp, int item,
                                            int strip, int j0, int j1)
 {
